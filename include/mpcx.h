@@ -1,0 +1,164 @@
+/*
+ * mpcx -- MI355X (gfx950) engine for the MD + multiparticle-collision step of
+ * the sphere-dimer motor simulators (collective_code and selkov_code).
+ *
+ * C ABI: plain pointers and sizes, no C++ or torch types.  Every function
+ * returns 0 on success or a negative MPCX_E_* code; mpcx_last_error() gives
+ * the message of the last failure on the calling thread.
+ *
+ * Two ways in:
+ *   1. Reference-shaped entry points (mpcx_mpc_rotation, mpcx_distribute, ...)
+ *      take the host arrays and argument lists of the Fortran module
+ *      procedures they replace, so a Fortran caller binds them one-to-one
+ *      through ISO_C_BINDING.  Each call stages its arrays through the device.
+ *   2. A device-resident engine (mpcx_co_* / mpcx_sk_*) keeps the solvent and
+ *      dimer state in HBM across many collision intervals and replaces the
+ *      whole time loop of main_pattern.f95:202-225 / Main_SelkovDimer.f90:125-149.
+ *
+ * Index conventions follow the reference: particle and cell numbers stored in
+ * integer arrays are 1-based; 2-D/3-D arrays are column-major with the
+ * reference's shapes.
+ */
+#ifndef MPCX_H
+#define MPCX_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPCX_OK 0
+#define MPCX_E_ARG -1        /* invalid argument or size                        */
+#define MPCX_E_HIP -2        /* HIP runtime failure                             */
+#define MPCX_E_DEVICE -3     /* a kernel reported an out-of-range state         */
+#define MPCX_E_UNSUPPORTED -4/* configuration outside what this engine handles  */
+
+const char* mpcx_last_error(void);
+/* number of visible HIP devices (>0 when a GPU is usable) */
+int mpcx_device_count(void);
+
+/* Host Mersenne Twister streams (parity mode draws).
+ * mpcx_mtmod_*: sgrnd/grnd of collective_code/mtmod.f90:72-144 (state mt[624], mti).
+ * mpcx_mt95_*:  init_by_scalar/genrand_real1 of selkov_code/mt95.f90:378-404,828-841
+ *               (state val[624], 1-based cnt). */
+int mpcx_mtmod_seed(int32_t seed, uint32_t* mt, int32_t* mti);
+int mpcx_mtmod_fill(uint32_t* mt, int32_t* mti, double* out, int64_t n);
+int mpcx_mt95_seed(int32_t seed, uint32_t* val, int32_t* cnt);
+int mpcx_mt95_fill(uint32_t* val, int32_t* cnt, double* out, int64_t n);
+
+/* ------------------------------------------------------------------------
+ * Reference-shaped entry points (collective_code)
+ * ---------------------------------------------------------------------- */
+
+/* Replaces mpc_rotation (collective_code/mpc_rotation_mod.f95:6-36).
+ * draws: the 3 + 2*numcell uniforms the reference takes from grnd() at one
+ * OpenMP thread, in call order (shift x,y,z; then two per cell). */
+int mpcx_mpc_rotation(int32_t NT, double rotang, int64_t numcell, double a0, const double* box,
+                      const double* Xsol, const double* Ysol, const double* Zsol,
+                      double* VXsol, double* VYsol, double* VZsol,
+                      double* VXCM, double* VYCM, double* VZCM, double* tCellShift, int64_t* NewCellN,
+                      double* UVX, double* UVY, double* UVZ, const double* draws);
+
+/* Replaces reaction_simple (collective_code/reaction_mod.f95:8-27).
+ * vol is the leading dimension of sBCellN(vol,100); draws holds numcell uniforms. */
+int mpcx_reaction_simple(double MPCt, int64_t numcell, double k1, int32_t NT, int32_t* flag,
+                         const int64_t* tBCellN, const int64_t* sBCellN, int64_t vol, const double* draws);
+
+/* Replaces neighbour_list (collective_code/neighbour_list_mod.f95:6-26).  The
+ * list is written with the given element stride (2*numdims for a section
+ * neighbour(k,i,:) of the reference array); *neighbour_num is incremented as
+ * in the reference. */
+int mpcx_neighbour_list(int32_t NT, const double* box, double cutoff, double skinrad,
+                        const double* Xsol, const double* Ysol, const double* Zsol,
+                        double Xdim, double Ydim, double Zdim, int64_t* neighbour_num,
+                        int64_t* neighbour, int64_t stride, int64_t max_neighbour);
+
+/* Replaces inout_particle_list (collective_code/neighbour_list_mod.f95:63-87). */
+int mpcx_inout_particle_list(int32_t NT, const double* box, double cutoffC, double cutoffN, double skinrad,
+                             const double* xsol, const double* ysol, const double* zsol,
+                             const double* xdim, const double* ydim, const double* zdim,
+                             int64_t* outside_particle_num, int64_t* outside_particle,
+                             int64_t* inside_particle_num, int64_t* inside_particle,
+                             int64_t inside_capacity, int32_t numdims);
+
+/* Replaces distribute (collective_code/distribute_mod.f95:6-40); the member
+ * lists s?CellN are (vol, maxs) column-major. */
+int mpcx_distribute(int32_t NT, int32_t numdims, const int32_t* flag,
+                    const double* Xsol, const double* Ysol, const double* Zsol,
+                    const double* Xdim, const double* Ydim, const double* Zdim, const double* box,
+                    int64_t vol, int64_t* tACellN, int64_t* tBCellN, int64_t* tSCellN,
+                    int64_t* sACellN, int64_t* sBCellN, int64_t* sSCellN, int64_t maxs,
+                    double cutoffC2, double cutoffN2,
+                    int64_t outside_particle_num, const int64_t* outside_particle,
+                    int64_t inside_particle_num, const int64_t* inside_particle);
+
+/* Replaces sort_hilbert (collective_code/sort_hilbert_mod.f95:6-24).  tour holds
+ * the vol 1-based cell numbers the reference builds from hilbert.txt (:57-62). */
+int mpcx_sort_hilbert(int64_t vol, int32_t NT, const double* box, const int64_t* tour,
+                      double* xsol, double* ysol, double* zsol, int32_t* flag,
+                      double* vxsol, double* vysol, double* vzsol);
+
+/* ------------------------------------------------------------------------
+ * Device-resident collective engine
+ * ---------------------------------------------------------------------- */
+typedef struct mpcx_co mpcx_co;
+
+/* Parameters of one collective_code system: the input.txt quantities
+ * (main_pattern.f95:19-64) and the derived ones (:66-76, :117-127). */
+typedef struct {
+  int64_t NT, numdims, vol, mpcnumcell, rxnnumcell, max_neighbour;
+  int32_t MPCtstep, sort_step, react_dir, rng_mode; /* rng_mode 0: host draws, 1: device Philox */
+  uint64_t seed;                                    /* Philox key in rng_mode 1              */
+  double box[3];
+  double MDt, Ma, Mb, Ms, Mc, Mn, Dcn, cutoff, Rc, Rn;
+  double EaC, EbC, EsC, EaN, EbN, EsN, Rw, Ew, Rm_delta, Em, RPab, RPba;
+  double skinrad, rotang, a0, MPCt, k1;
+  double sinrot, cosrot;   /* dsin/dcos(rotang) evaluated on the host (mpc_rotation_mod.f95:83-84) */
+  double wall_w6;          /* 3.d0**(1./6.) as the reference evaluates it (evolve_mod.f95:218)    */
+  double reaction_p[101];  /* 1 - dexp(-k1*n*MPCt) for n = 0..100 (reaction_mod.f95:45-54)        */
+  double sphere_guard;     /* bound on a sphere's displacement per interval (near/far split)      */
+} mpcx_co_config;
+
+int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg);
+int mpcx_co_destroy(mpcx_co* e);
+
+/* Upload solvent and dimer state; dimer arrays are (numdims,2) column-major.
+ * react_box is (Lx*Ly) from reactive_grid_setup; tour as in mpcx_sort_hilbert. */
+int mpcx_co_set_state(mpcx_co* e, int32_t cstep, const double* x, const double* y, const double* z,
+                      const double* vx, const double* vy, const double* vz, const int32_t* flag,
+                      const double* xd, const double* yd, const double* zd,
+                      const double* vxd, const double* vyd, const double* vzd,
+                      const double* fxd, const double* fyd, const double* fzd,
+                      const int32_t* react_box, const int64_t* tour);
+
+/* Set-up of main_pattern.f95:170-188: optional sort, neighbour and in/out
+ * lists, time_penetrate; prepares the near/far split of the first interval. */
+int mpcx_co_setup(mpcx_co* e, int32_t do_sort);
+
+/* Advance ncoll collision intervals (MPCtstep MD steps plus one collision
+ * each).  In rng_mode 0, draws holds ncoll blocks of
+ * 3 + 2*mpcnumcell + rxnnumcell uniforms (grnd() order of one collision). */
+int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws);
+
+/* Download state.  Arrays may be NULL to skip them. */
+int mpcx_co_get_state(mpcx_co* e, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                      int32_t* flag, double* dimer12 /* (numdims*2, 12): x y z vx vy vz fx fy fz qx qy qz */,
+                      int32_t* scalars /* cstep, time_count, time_penetrate, inside_num */);
+/* Inside list (ascending 0-based particle indices), the neighbour-list
+ * membership bit mask of each entry (bit 2k: C sphere of dimer k, 2k+1: N),
+ * and the force on each entry from each sphere it belongs to (up to 4,
+ * in sphere order: fp[m*3*cap + c*cap + q]).  Returns the count in *n. */
+int mpcx_co_get_inside(mpcx_co* e, int32_t* list, uint32_t* memb, double* fp, double* pot, int64_t cap,
+                       int64_t* n);
+/* Collision-cell data of the last collision: (mpcnumcell) arrays, 0-based
+ * cell of each particle, and unshifted census counts (3, vol). */
+int mpcx_co_get_cells(mpcx_co* e, double* vcm_uv /* (mpcnumcell, 8) */, int32_t* cell_of,
+                      int32_t* census /* (3, vol) */, double* tpotential);
+/* Wall time of the HIP events bracketing the last mpcx_co_run (ms), and of
+ * the bulk kernels alone (ms, summed over the run). */
+int mpcx_co_timing(mpcx_co* e, double* run_ms, double* bulk_ms, double* near_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,188 @@
+"""Host side of the collective_code path on the MI355X engine.
+
+CollectiveEngine mirrors the reference's time loop (main_pattern.f95:170-240):
+``setup()`` is the set-up of :170-188 and ``run(n)`` performs n collision
+intervals of :202-225 (MPCtstep calls of evolve, then distribute,
+mpc_rotation and reaction_simple).  The state arrays it takes and returns
+have the reference's names, kinds and layouts.  The per-call drop-ins of the
+individual module procedures live in ``main_amd.dropin``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from ._lib import MtmodStream, check, lib, ptr, require_gpu
+
+MAXM = 4            # force slots stored per inside particle by the engine
+MAX_INSIDE = 4096   # register slots of the near engine
+
+
+class CoConfig(C.Structure):
+    _fields_ = [("NT", C.c_int64), ("numdims", C.c_int64), ("vol", C.c_int64), ("mpcnumcell", C.c_int64),
+                ("rxnnumcell", C.c_int64), ("max_neighbour", C.c_int64),
+                ("MPCtstep", C.c_int32), ("sort_step", C.c_int32), ("react_dir", C.c_int32),
+                ("rng_mode", C.c_int32), ("seed", C.c_uint64), ("box", C.c_double * 3)] + \
+               [(k, C.c_double) for k in ("MDt", "Ma", "Mb", "Ms", "Mc", "Mn", "Dcn", "cutoff", "Rc", "Rn",
+                                          "EaC", "EbC", "EsC", "EaN", "EbN", "EsN", "Rw", "Ew", "Rm_delta",
+                                          "Em", "RPab", "RPba", "skinrad", "rotang", "a0", "MPCt", "k1",
+                                          "sinrot", "cosrot", "wall_w6")] + \
+               [("reaction_p", C.c_double * 101), ("sphere_guard", C.c_double)]
+
+
+def reaction_table(k1: float, MPCt: float) -> np.ndarray:
+    """prob1(n) = 1 - dexp(-a1*MPCt), a1 = k1*n (reaction_mod.f95:45-54), host libm."""
+    t = np.zeros(101)
+    for n in range(101):
+        a1 = 0.0 if n == 0 else k1 * float(n)
+        t[n] = 1.0 - math.exp(-(a1 * MPCt))
+    return t
+
+
+def make_config(p: dict, rng_mode: int = 0, seed: int = 4357787, sphere_guard: float = 1.0) -> CoConfig:
+    c = CoConfig()
+    for k in ("NT", "numdims", "vol", "mpcnumcell", "rxnnumcell"):
+        setattr(c, k, int(p[k]))
+    c.max_neighbour = int(p.get("max_nb", 30000))
+    c.MPCtstep, c.sort_step, c.react_dir = int(p["MPCtstep"]), int(p["sort_step"]), int(p["react_dir"])
+    c.rng_mode, c.seed = int(rng_mode), int(seed)
+    for i in range(3):
+        c.box[i] = float(p["box"][i])
+    for k in ("MDt", "Ma", "Mb", "Ms", "Mc", "Mn", "Dcn", "cutoff", "Rc", "Rn", "EaC", "EbC", "EsC", "EaN",
+              "EbN", "EsN", "Rw", "Ew", "Rm_delta", "Em", "RPab", "RPba", "skinrad", "rotang", "a0", "MPCt", "k1"):
+        setattr(c, k, float(p[k]))
+    c.sinrot = math.sin(p["rotang"])
+    c.cosrot = math.cos(p["rotang"])
+    c.wall_w6 = 3.0 ** float(np.float32(1.0) / np.float32(6.0))
+    tab = reaction_table(p["k1"], p["MPCt"])
+    for i in range(101):
+        c.reaction_p[i] = tab[i]
+    c.sphere_guard = sphere_guard
+    return c
+
+
+def draws_per_collision(p: dict) -> int:
+    """grnd() calls of one collision at one thread: 3 shift + 2 per MPC cell +
+    1 per reaction cell (mpc_rotation_mod.f95:102-107,187-189; reaction_mod.f95:57)."""
+    return 3 + 2 * int(p["mpcnumcell"]) + int(p["rxnnumcell"])
+
+
+class CollectiveEngine:
+    """Device-resident collective_code system (one GPU)."""
+
+    def __init__(self, p: dict, rng_mode: int = 0, seed: int = 4357787, sphere_guard: float = 1.0):
+        require_gpu()
+        self.p = p
+        self.cfg = make_config(p, rng_mode, seed, sphere_guard)
+        self.h = C.c_void_p()
+        check(lib().mpcx_co_create(C.byref(self.h), C.byref(self.cfg)), "mpcx_co_create")
+        self.rng_mode = rng_mode
+
+    def close(self):
+        if self.h:
+            lib().mpcx_co_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, s: dict, tour=None):
+        f = lambda k: np.ascontiguousarray(s[k], dtype=np.float64)
+        dims = {k: np.asfortranarray(s[k], dtype=np.float64) for k in
+                ("xd", "yd", "zd", "vxd", "vyd", "vzd", "fxd", "fyd", "fzd")}
+        self._keep = [f(k) for k in ("x", "y", "z", "vx", "vy", "vz")]
+        flag = np.ascontiguousarray(s["flag"], dtype=np.int32)
+        rb = np.ascontiguousarray(s["react_box"], dtype=np.int32)
+        t = None if tour is None else np.ascontiguousarray(tour, dtype=np.int64)
+        check(lib().mpcx_co_set_state(self.h, C.c_int32(int(s.get("cstep", 0))), *[ptr(a) for a in self._keep],
+                                      ptr(flag), *[ptr(dims[k]) for k in ("xd", "yd", "zd", "vxd", "vyd", "vzd",
+                                                                           "fxd", "fyd", "fzd")],
+                                      ptr(rb), ptr(t)), "mpcx_co_set_state")
+
+    def setup(self, do_sort: bool = True):
+        check(lib().mpcx_co_setup(self.h, C.c_int32(1 if do_sort else 0)), "mpcx_co_setup")
+
+    def run(self, ncoll: int, draws: np.ndarray | None = None):
+        d = None if draws is None else np.ascontiguousarray(draws, dtype=np.float64)
+        if self.rng_mode == 0:
+            need = ncoll * draws_per_collision(self.p)
+            if d is None or d.size < need:
+                raise ValueError(f"rng_mode 0 needs {need} host draws")
+        check(lib().mpcx_co_run(self.h, C.c_int64(ncoll), ptr(d)), "mpcx_co_run")
+
+    def get_state(self) -> dict:
+        NT, nd = self.p["NT"], self.p["numdims"]
+        out = {k: np.empty(NT) for k in ("x", "y", "z", "vx", "vy", "vz")}
+        out["flag"] = np.empty(NT, np.int32)
+        dim = np.empty(12 * 2 * nd)
+        sc = np.zeros(4, np.int32)
+        check(lib().mpcx_co_get_state(self.h, *[ptr(out[k]) for k in ("x", "y", "z", "vx", "vy", "vz")],
+                                      ptr(out["flag"]), ptr(dim), ptr(sc)), "mpcx_co_get_state")
+        dim = dim.reshape(12, 2 * nd)
+        for f, k in enumerate(("xd", "yd", "zd", "vxd", "vyd", "vzd", "fxd", "fyd", "fzd", "qx", "qy", "qz")):
+            out[k] = np.asfortranarray(dim[f].reshape(2, nd).T)   # s = k + i*nd -> (k, i)
+        out["cstep"], out["time_count"], out["time_penetrate"], out["in_num"] = (int(v) for v in sc)
+        return out
+
+    def get_lists(self) -> dict:
+        """Inside list, neighbour lists (reference layout, 1-based) and Fpair."""
+        nd, NT, mnb = self.p["numdims"], self.p["NT"], int(self.p.get("max_nb", 30000))
+        cap = MAX_INSIDE
+        lst = np.zeros(cap, np.int32)
+        memb = np.zeros(cap, np.uint32)
+        fp = np.zeros(MAXM * 3 * cap)
+        pot = np.zeros(MAXM * cap)
+        n = C.c_int64(0)
+        check(lib().mpcx_co_get_inside(self.h, ptr(lst), ptr(memb), ptr(fp), ptr(pot), C.c_int64(cap),
+                                       C.byref(n)), "mpcx_co_get_inside")
+        n = n.value
+        lst, memb = lst[:n], memb[:n]
+        fp = fp.reshape(MAXM, 3, cap)[:, :, :n]
+        pot = pot.reshape(MAXM, cap)[:, :n]
+        nb = np.zeros((nd, 2, mnb), np.int64, order="F")
+        nbnum = np.zeros((nd, 2), np.int64, order="F")
+        fpx = np.zeros((nd, 2, mnb), order="F")
+        fpy, fpz, potn = fpx.copy(order="F"), fpx.copy(order="F"), fpx.copy(order="F")
+        mcount = np.zeros(n, np.int64)
+        for b in range(2 * nd):
+            k, i = b >> 1, b & 1
+            sel = np.nonzero((memb >> b) & 1)[0]
+            m = mcount[sel]
+            cnt = sel.size
+            nb[k, i, :cnt] = lst[sel] + 1
+            nbnum[k, i] = cnt
+            fpx[k, i, :cnt] = fp[m, 0, sel]
+            fpy[k, i, :cnt] = fp[m, 1, sel]
+            fpz[k, i, :cnt] = fp[m, 2, sel]
+            potn[k, i, :cnt] = pot[m, sel]
+            mcount[sel] += 1
+        inside = lst.astype(np.int64) + 1
+        outside = np.setdiff1d(np.arange(1, NT + 1, dtype=np.int64), inside, assume_unique=True)
+        return dict(in_list=inside, out_list=outside, nb=nb, nbnum=nbnum, fpx=fpx, fpy=fpy, fpz=fpz, pot=potn)
+
+    def get_cells(self) -> dict:
+        mc, NT, vol = self.p["mpcnumcell"], self.p["NT"], self.p["vol"]
+        rec = np.empty((mc, 8))
+        cell = np.empty(NT, np.int32)
+        census = np.empty((3, vol), np.int32)
+        tp = np.zeros(1)
+        check(lib().mpcx_co_get_cells(self.h, ptr(rec), ptr(cell), ptr(census), ptr(tp)), "mpcx_co_get_cells")
+        return dict(vxcm=rec[:, 0].copy(), vycm=rec[:, 1].copy(), vzcm=rec[:, 2].copy(), uvx=rec[:, 3].copy(),
+                    uvy=rec[:, 4].copy(), uvz=rec[:, 5].copy(), tcs=rec[:, 6].copy(),
+                    ncn=cell.astype(np.int64) + 1, tA=census[0].astype(np.int64), tB=census[1].astype(np.int64),
+                    tS=census[2].astype(np.int64), tpotential=float(tp[0]))
+
+    def timing(self):
+        r, b, n = C.c_double(), C.c_double(), C.c_double()
+        check(lib().mpcx_co_timing(self.h, C.byref(r), C.byref(b), C.byref(n)), "mpcx_co_timing")
+        return r.value, b.value, n.value
+
+
+def host_draws(stream: MtmodStream, p: dict, ncoll: int) -> np.ndarray:
+    """The grnd() draws of ncoll collisions, in reference order."""
+    return stream.draws(ncoll * draws_per_collision(p))

@@ -1,0 +1,1529 @@
+// Collective engine: the MD + MPC time loop of collective_code on one MI355X.
+//
+// Work split per collision interval (MPCtstep MD steps + one collision):
+//  * near engine (one persistent 1024-thread workgroup per system): the
+//    particles that can enter a neighbour list during the interval ("near"
+//    set) are advanced exactly like evolve_mod.f95:6-732 -- inside particles
+//    live in registers across all MD steps, the dimer lives in LDS and is
+//    integrated by thread 0 (RATTLE), neighbour lists are rebuilt by ordered
+//    block-wide compaction of the near set.
+//  * bulk kernels over all particles: the "far" particles are only ever
+//    touched by the lazy streaming of evolve_mod.f95:467-529; the rebuild
+//    schedule recorded by the near engine is replayed in registers in one
+//    pass, fused with the shifted-cell binning of the collision.  The
+//    collision (mpc_rotation_mod.f95:83-232) sums each cell's velocities in
+//    ascending particle order (the reference's order at one thread), rotates,
+//    and the same pass selects the near set of the next interval.
+// The near/far split is exact: a far particle is one whose cell lies further
+// from every sphere than cutoff+skin+|v|*tau+guard, and the near engine checks
+// at every rebuild that no sphere moved more than the guard.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+#include <string>
+#include <vector>
+#include <algorithm>
+#include "mpcx_device.h"
+#include "mpcx_host.h"
+#include "../../include/mpcx.h"
+
+#pragma clang fp contract(off)
+
+namespace mpcx {
+
+constexpr int NE_T = 1024;   // near-engine threads
+constexpr int NE_K = 4;      // inside slots per thread (capacity 4096)
+constexpr int MAXD = 2;      // dimers the near engine handles
+constexpr int MAXS = 2 * MAXD;
+constexpr int MAXM = 4;      // neighbour lists one particle may belong to
+constexpr int MAXEV = 64;    // rebuilds recorded per interval
+constexpr int BMAX = 64;     // particles per collision cell in the ordered sum
+constexpr int CENSUS_MAX = 100;  // MaxA/MaxB/MaxS, global_mod.f95:114-116
+constexpr int CT = 256;      // threads of the per-cell kernel
+// The reference caps its RATTLE iterations at 2e8 (evolve_mod.f95:104,115); on
+// the GPU a non-converging constraint stops at this cap and is reported.
+constexpr long long RATTLE_CAP = 100000;
+
+enum : uint8_t { ST_INSIDE = 1, ST_EXCL = 2 };
+enum : int { M_SETUP = 1, M_STOP = 2, M_RESUME = 4 };
+enum : int { DX = 0, DY, DZ, DVX, DVY, DVZ, DFX, DFY, DFZ, DQX, DQY, DQZ };
+
+struct CoScalars {
+  int cstep, time_count, time_penetrate, in_num;
+  int near_num, n_ev, err, pad;
+  int events[MAXEV];
+  unsigned long long far_maxv2;    // bits of a non-negative double
+  double near_vmax2;
+  double tpotential;
+  double x0[MAXS], y0[MAXS], z0[MAXS];
+};
+
+struct CoP {
+  int NT, nd, MPCtstep, sort_step, react_dir, Lx, Ly, Lz, nrb, rng_mode;
+  long long vol, mpcnumcell, rxnnumcell;
+  unsigned long long seed;
+  double box[3];
+  double MDt, hA, hB, hS, hc, hn, IIMcIMn, Mc, Mn, Dcn2;
+  double cut2[2], full2[2], full[2];
+  double R6[2], R12[2], r6R6[2], r12R12[2];
+  double Ea[2], Eb[2], Es[2];
+  double Em, mm_cut2[3], mm_R6[3], mm_R12[3];
+  double Ew, wR3[2], wR9[2], wA[2], wB[2], wcut2[2], pref;
+  double hs2, a0, cosrot, sinrot, omc, twopi, tau, guard, skinrad;
+};
+
+struct CoD {
+  double *x, *y, *z, *vx, *vy, *vz;
+  int* flag;
+  double* dim;            // [12][MAXS], sphere s = k + i*nd
+  CoScalars* sc;
+  const int* react_box;
+  int* near_list;
+  int* near_memb;         // scratch of the classification
+  unsigned* near_bits;
+  unsigned char* status;
+  int* in_list;
+  unsigned* in_memb;
+  double* in_fp;          // [(m*3+c)*in_cap + q]
+  double* in_pot;         // [m*in_cap + q]
+  double* in_msd;         // max_solvent_dist of inside entries
+  int in_cap, near_cap;
+  int* cell_of;
+  unsigned short* rank_of;
+  int* cnt;
+  int* off;
+  int* bucket;
+  int* partial;
+  double* cellrec;        // [mpcnumcell][8]: vcm x y z, uv x y z, n, 0
+  int* tcount;            // [3][vol]
+  unsigned* firstB;       // [vol]
+  const double* U;        // host draws of this collision (rng_mode 0)
+  const double* ptab;     // reaction probabilities, 101 entries
+  unsigned long long coll;  // collision counter for Philox
+};
+
+// ------------------------------------------------------------- streaming
+// One streaming update of evolve_mod.f95:259-312 (tc = 1: inside particles,
+// whose "x + v*MDt" equals "(v*MDt)*1"), or of the lazy update :474-527.
+__device__ __forceinline__ bool stream_particle(double& x, double& y, double& z, double& vx, double& vy,
+                                                double& vz, int& flag, double tc, const CoP& P,
+                                                const int* react_box, int* err) {
+  const double L0 = P.box[0], L1 = P.box[1], L2 = P.box[2], MDt = P.MDt;
+  x = x + (vx * MDt) * tc;
+  if (x >= L0) x = x - L0; else if (x < 0.0) x = x + L0;
+  y = y + (vy * MDt) * tc;
+  if (y >= L1) y = y - L1; else if (y < 0.0) y = y + L1;
+  z = z + (vz * MDt) * tc;
+  if (z < 0.0) {
+    double t = z / vz;
+    // reactive_grid_mod.f95:128-149
+    double ax = x - vx * t, ay = y - vy * t;
+    if (ax >= L0) ax = ax - L0; else if (ax < 0.0) ax = ax + L0;
+    if (ay >= L1) ay = ay - L1; else if (ay < 0.0) ay = ay + L1;
+    int hit = (int)(ffloor_i(ax) + L0 * ffloor_i(ay) + 1.0);
+    if (hit < 1 || hit > P.nrb) raise_error(err, ERR_CELL_RANGE);
+    else if (react_box[hit - 1] == 1) {
+      if (P.react_dir == 0 && flag == 0) flag = 1;
+      else if (P.react_dir == 1 && flag == 1) flag = 0;
+    }
+    vx = -vx; vy = -vy; vz = -vz;
+    x = x + (2.0 * vx) * t; y = y + (2.0 * vy) * t; z = z + (2.0 * vz) * t;
+    if (x >= L0) x = x - L0; else if (x < 0.0) x = x + L0;
+    if (y >= L1) y = y - L1; else if (y < 0.0) y = y + L1;
+    return true;
+  } else if (z > L2) {
+    double t = (z - L2) / vz;
+    vx = -vx; vy = -vy; vz = -vz;
+    x = x + (2.0 * vx) * t; y = y + (2.0 * vy) * t; z = z + (2.0 * vz) * t;
+    if (x >= L0) x = x - L0; else if (x < 0.0) x = x + L0;
+    if (y >= L1) y = y - L1; else if (y < 0.0) y = y + L1;
+    return true;
+  }
+  return false;
+}
+
+__device__ __forceinline__ double kick_factor(const CoP& P, int flag, bool& ok) {
+  ok = true;
+  if (flag == 0) return P.hA;
+  if (flag == 1) return P.hB;
+  if (flag == 2) return P.hS;
+  ok = false;
+  return 0.0;
+}
+
+// neighbour-list membership of a position (bit 2k+i: sphere i of dimer k)
+template <int ND>
+__device__ __forceinline__ unsigned memberships(double x, double y, double z, const CoP& P,
+                                                const double (*sd)[MAXS]) {
+  unsigned m = 0;
+#pragma unroll
+  for (int k = 0; k < ND; k++)
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      int s = k + i * ND;
+      double d1 = pbc_half(x - sd[DX][s], P.box[0]);
+      double d2 = pbc_half(y - sd[DY][s], P.box[1]);
+      double d3 = z - sd[DZ][s];
+      if (d1 * d1 + d2 * d2 + d3 * d3 <= P.full2[i]) m |= 1u << (2 * k + i);
+    }
+  return m;
+}
+
+// ------------------------------------------------------------- dimer (thread 0)
+// evolve_mod.f95:325-415: Q = V + h F, advance, RATTLE on the bond length.
+template <int ND>
+__device__ __attribute__((noinline)) void rattle_positions(const CoP& P, double (*sd)[MAXS], int* err) {
+  const double MDt = P.MDt, L0 = P.box[0], L1 = P.box[1];
+  for (int k = 0; k < ND; k++) {
+    const int s0 = k, s1 = k + ND;
+    double prX[2] = {sd[DX][s0], sd[DX][s1]}, prY[2] = {sd[DY][s0], sd[DY][s1]},
+           prZ[2] = {sd[DZ][s0], sd[DZ][s1]};
+    double dP0 = pbc_half(sd[DX][s0] - sd[DX][s1], L0);
+    double dP1 = pbc_half(sd[DY][s0] - sd[DY][s1], L1);
+    double dP2 = sd[DZ][s0] - sd[DZ][s1];
+    sd[DQX][s0] = sd[DVX][s0] + P.hc * sd[DFX][s0];
+    sd[DQY][s0] = sd[DVY][s0] + P.hc * sd[DFY][s0];
+    sd[DQZ][s0] = sd[DVZ][s0] + P.hc * sd[DFZ][s0];
+    sd[DQX][s1] = sd[DVX][s1] + P.hn * sd[DFX][s1];
+    sd[DQY][s1] = sd[DVY][s1] + P.hn * sd[DFY][s1];
+    sd[DQZ][s1] = sd[DVZ][s1] + P.hn * sd[DFZ][s1];
+    for (int i = 0; i < 2; i++) {
+      int s = k + i * ND;
+      double x = sd[DX][s] + MDt * sd[DQX][s];
+      double y = sd[DY][s] + MDt * sd[DQY][s];
+      sd[DZ][s] = sd[DZ][s] + MDt * sd[DQZ][s];
+      sd[DX][s] = x - L0 * floor(x / L0);
+      sd[DY][s] = y - L1 * floor(y / L1);
+    }
+    for (long long acc = 0;; acc++) {
+      if (acc == RATTLE_CAP) { raise_error(err, ERR_RATTLE); break; }
+      double d0 = pbc_half(sd[DX][s0] - sd[DX][s1], L0);
+      double d1 = pbc_half(sd[DY][s0] - sd[DY][s1], L1);
+      double d2 = sd[DZ][s0] - sd[DZ][s1];
+      double c1 = (d0 * d0 + d1 * d1 + d2 * d2) - P.Dcn2;
+      if (fabs(c1) < 1e-6) break;
+      double G = -(((0.25 * c1) * P.IIMcIMn) / (d0 * dP0 + d1 * dP1 + d2 * dP2));
+      double g2 = 2.0 * G;
+      double x0 = (prX[0] + sd[DQX][s0] * MDt) + (g2 * dP0) / P.Mc;
+      double y0 = (prY[0] + sd[DQY][s0] * MDt) + (g2 * dP1) / P.Mc;
+      double z0 = (prZ[0] + sd[DQZ][s0] * MDt) + (g2 * dP2) / P.Mc;
+      double x1 = (prX[1] + sd[DQX][s1] * MDt) - (g2 * dP0) / P.Mn;
+      double y1 = (prY[1] + sd[DQY][s1] * MDt) - (g2 * dP1) / P.Mn;
+      double z1 = (prZ[1] + sd[DQZ][s1] * MDt) - (g2 * dP2) / P.Mn;
+      sd[DX][s0] = x0 - L0 * floor(x0 / L0); sd[DY][s0] = y0 - L1 * floor(y0 / L1); sd[DZ][s0] = z0;
+      sd[DX][s1] = x1 - L0 * floor(x1 / L0); sd[DY][s1] = y1 - L1 * floor(y1 / L1); sd[DZ][s1] = z1;
+      sd[DQX][s0] = sd[DQX][s0] + (g2 * dP0) / (P.Mc * MDt);
+      sd[DQY][s0] = sd[DQY][s0] + (g2 * dP1) / (P.Mc * MDt);
+      sd[DQZ][s0] = sd[DQZ][s0] + (g2 * dP2) / (P.Mc * MDt);
+      sd[DQX][s1] = sd[DQX][s1] - (g2 * dP0) / (P.Mn * MDt);
+      sd[DQY][s1] = sd[DQY][s1] - (g2 * dP1) / (P.Mn * MDt);
+      sd[DQZ][s1] = sd[DQZ][s1] - (g2 * dP2) / (P.Mn * MDt);
+    }
+  }
+}
+
+// forces_mod.f95:183-208
+__device__ __forceinline__ void lj_mot_mot(const CoP& P, double x1, double y1, double z1, double x2, double y2,
+                                           double z2, int kind, double& fx, double& fy, double& fz, double& fp) {
+  double d1 = pbc_half(x1 - x2, P.box[0]);
+  double d2 = pbc_half(y1 - y2, P.box[1]);
+  double d3 = z1 - z2;
+  double dRsq = d1 * d1 + d2 * d2 + d3 * d3;
+  fx = fy = fz = fp = 0.0;
+  if (dRsq <= P.mm_cut2[kind]) {
+    double F, U;
+    lj_terms(dRsq, P.mm_R6[kind], P.mm_R12[kind], 12.0 * P.mm_R12[kind], 6.0 * P.mm_R6[kind], F, U);
+    double EF = P.Em * F;
+    fx = EF * d1; fy = EF * d2; fz = EF * d3;
+    fp = P.Em * U;
+  }
+}
+
+// forces_mod.f95:303-333 with the dummy order of the call at evolve_mod.f95:597-600
+__device__ __forceinline__ void lj_mot_wall(const CoP& P, double zd, int i, double& fw, double& pot) {
+  const double R3 = P.wR3[i], R9 = P.wR9[i], r9R9 = P.wA[i], r3R3 = P.wB[i];
+  const double pe = P.pref * P.Ew;
+  fw = 0.0;
+  pot = 0.0;
+  double dR = P.box[2] - zd;
+  if (dR * dR < P.wcut2[i]) {
+    double IdRsq = 1.0 / (dR * dR);
+    double IdR3 = pow(IdRsq, 1.5);
+    double IdR9 = (IdR3 * IdR3) * IdR3;
+    double IdR5 = IdR3 * IdRsq;
+    double IdR11 = IdR9 * IdRsq;
+    fw = (pe * (r9R9 * IdR11 - r3R3 * IdR5)) * dR;
+    pot = pe * (R9 * IdR9 - R3 * IdR3) + 1.0;
+  }
+  dR = 0.0 - zd;
+  if (dR * dR < P.wcut2[i]) {
+    double IdRsq = 1.0 / (dR * dR);
+    double IdR3 = pow(IdRsq, 1.5);
+    double IdR9 = (IdR3 * IdR3) * IdR3;
+    double IdR5 = IdR3 * IdRsq;
+    double IdR11 = IdR9 * IdRsq;
+    fw = fw + (pe * (r9R9 * IdR11 - r3R3 * IdR5)) * dR;
+    pot = (pot + pe * (R9 * IdR9 - R3 * IdR3)) + 1.0;
+  }
+}
+
+// evolve_mod.f95:605-636 (solvent sums arrive as tree-reduced totals) and
+// :679-724 (velocity RATTLE).  sumF[s][0..2] = sum of Fpair, sumF[s][3] = sum of Potential.
+template <int ND>
+__device__ __attribute__((noinline)) void dimer_forces_and_rattle_v(const CoP& P, double (*sd)[MAXS],
+                                                                   const double (*sumF)[4], double& tpot, int* err) {
+  double tpp[MAXS];
+  for (int k = 0; k < ND; k++)
+    for (int i = 0; i < 2; i++) {
+      int s = k + i * ND;
+      double fw, wp;
+      lj_mot_wall(P, sd[DZ][s], i, fw, wp);
+      double fx = 0.0 - sumF[s][0], fy = 0.0 - sumF[s][1], fz = 0.0 - sumF[s][2];
+      double tp = 0.0 + sumF[s][3];
+      for (int j = 0; j < ND; j++)
+        for (int l = 0; l < 2; l++) {
+          if (j == k) continue;   // Fmot(k,i,k,l) stays zero (evolve_mod.f95:580)
+          int b = j + l * ND;
+          int kind = (i == 0 && l == 0) ? 0 : (i == 1 && l == 1) ? 2 : 1;
+          double mx, my, mz, mp;
+          lj_mot_mot(P, sd[DX][s], sd[DY][s], sd[DZ][s], sd[DX][b], sd[DY][b], sd[DZ][b], kind, mx, my, mz, mp);
+          fx = fx + mx; fy = fy + my; fz = fz + mz;
+          if (j < k) tp = tp + mp;
+        }
+      fz = fz + fw;
+      tp = tp + wp;
+      sd[DFX][s] = fx; sd[DFY][s] = fy; sd[DFZ][s] = fz;
+      tpp[s] = tp;
+    }
+  tpot = 0.0;
+  for (int k = 0; k < ND; k++)
+    for (int i = 0; i < 2; i++) tpot = tpot + tpp[k + i * ND];
+  for (int k = 0; k < ND; k++) {
+    const int s0 = k, s1 = k + ND;
+    sd[DVX][s0] = sd[DQX][s0] + P.hc * sd[DFX][s0];
+    sd[DVY][s0] = sd[DQY][s0] + P.hc * sd[DFY][s0];
+    sd[DVZ][s0] = sd[DQZ][s0] + P.hc * sd[DFZ][s0];
+    sd[DVX][s1] = sd[DQX][s1] + P.hn * sd[DFX][s1];
+    sd[DVY][s1] = sd[DQY][s1] + P.hn * sd[DFY][s1];
+    sd[DVZ][s1] = sd[DQZ][s1] + P.hn * sd[DFZ][s1];
+    for (long long acc = 0;; acc++) {
+      if (acc == RATTLE_CAP) { raise_error(err, ERR_RATTLE); break; }
+      double dvx = sd[DVX][s0] - sd[DVX][s1], dvy = sd[DVY][s0] - sd[DVY][s1], dvz = sd[DVZ][s0] - sd[DVZ][s1];
+      double d0 = pbc_half(sd[DX][s0] - sd[DX][s1], P.box[0]);
+      double d1 = pbc_half(sd[DY][s0] - sd[DY][s1], P.box[1]);
+      double d2 = sd[DZ][s0] - sd[DZ][s1];
+      double c2 = 2.0 * (dvx * d0 + dvy * d1 + dvz * d2);
+      if (fabs(c2) < 1e-8) break;
+      double O = -((0.25 * (P.IIMcIMn * c2)) / P.Dcn2);
+      double o2 = 2.0 * O;
+      sd[DVX][s0] = sd[DVX][s0] + (o2 * d0) / P.Mc;
+      sd[DVY][s0] = sd[DVY][s0] + (o2 * d1) / P.Mc;
+      sd[DVZ][s0] = sd[DVZ][s0] + (o2 * d2) / P.Mc;
+      sd[DVX][s1] = sd[DVX][s1] - (o2 * d0) / P.Mn;
+      sd[DVY][s1] = sd[DVY][s1] - (o2 * d1) / P.Mn;
+      sd[DVZ][s1] = sd[DVZ][s1] - (o2 * d2) / P.Mn;
+    }
+  }
+}
+
+// ------------------------------------------------------------- near engine
+struct NearShared {
+  double msd[NE_T * NE_K];     // max_solvent_dist of the slots
+  int flag[NE_T * NE_K];
+  unsigned memb[NE_T * NE_K];
+  double sd[12][MAXS];
+  double red[NE_T / 64][4 * MAXS + 1];
+  double sumF[MAXS][4];
+  unsigned long long scan[NE_T / 64 + 1];
+  double vmax2;
+  int cstep, tc, tp, in_num, nev, near_num, rebuild, stop;
+};
+
+// Ordered rebuild of the inside list from the near list (neighbour_list_mod.f95:63-177
+// restricted to the near set; every other particle is outside by construction).
+template <int ND>
+__device__ __attribute__((noinline)) void ne_classify(const CoP& P, const CoD& D, NearShared& S) {
+  const int tid = threadIdx.x;
+  const int nn = S.near_num;
+  const int E = (nn + NE_T - 1) / NE_T;
+  const int q0 = min(tid * E, nn), q1 = min(q0 + E, nn);
+  unsigned long long cnt = 0;
+  for (int q = q0; q < q1; q++) {
+    int p = D.near_list[q];
+    unsigned m = memberships<ND>(D.x[p], D.y[p], D.z[p], P, S.sd);
+    D.near_memb[q] = (int)m;
+    cnt += m != 0;
+  }
+  unsigned long long total;
+  unsigned long long off = block_excl_scan64(cnt, S.scan, total);
+  if (total > (unsigned long long)D.in_cap) {
+    if (tid == 0) raise_error(&D.sc->err, ERR_INSIDE_CAP);
+    total = D.in_cap;
+  }
+  int w = (int)off;
+  for (int q = q0; q < q1; q++) {
+    int p = D.near_list[q];
+    unsigned m = (unsigned)D.near_memb[q];
+    if (m) {
+      if (__popc(m) > MAXM) raise_error(&D.sc->err, ERR_MEMBERSHIPS);
+      if (w < D.in_cap) {
+        D.in_list[w] = p;
+        D.in_memb[w] = m;
+      }
+      w++;
+      D.status[p] = ST_INSIDE;
+    } else {
+      D.status[p] = 0;
+    }
+  }
+  if (tid == 0) {
+    S.in_num = (int)total;
+    // time_penetrate from the maximum of v^2 over all particles (evolve_mod.f95:555-563)
+    double fm = __longlong_as_double((long long)D.sc->far_maxv2);
+    double maxt = fmax(fm, S.vmax2);
+    S.tp = (int)floor(((0.5 * P.skinrad) / sqrt(maxt)) / P.MDt);
+    S.tc = 0;
+  }
+  __syncthreads();
+}
+
+// write inside slots back to HBM
+#define NE_SLOT_LOOP for (int k = 0; k < NE_K; k++)
+
+template <int ND>
+__global__ __launch_bounds__(NE_T) void co_near_engine(CoP P, CoD D, int nsteps, int mode) {
+  __shared__ NearShared S;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  constexpr int NSPH = 2 * ND;
+  constexpr int NV = 4 * NSPH + 1;
+  CoScalars* sc = D.sc;
+
+  for (int j = tid; j < 12 * MAXS; j += NE_T) S.sd[j / MAXS][j % MAXS] = D.dim[j];
+  if (tid == 0) {
+    S.cstep = sc->cstep; S.tc = sc->time_count; S.tp = sc->time_penetrate; S.in_num = sc->in_num;
+    S.nev = (mode & M_RESUME) ? sc->n_ev : 0;
+    S.near_num = sc->near_num;
+    S.vmax2 = sc->near_vmax2;
+    S.stop = 0;
+    S.rebuild = 0;
+  }
+  __syncthreads();
+
+  if (mode & M_SETUP) {
+    // main_pattern.f95:173-188 over the near set; near_vmax2 was set by the classifier kernel
+    ne_classify<ND>(P, D, S);
+    if (tid == 0) {
+      sc->in_num = S.in_num; sc->time_count = S.tc; sc->time_penetrate = S.tp; sc->n_ev = 0;
+      for (int s = 0; s < NSPH; s++) { sc->x0[s] = S.sd[DX][s]; sc->y0[s] = S.sd[DY][s]; sc->z0[s] = S.sd[DZ][s]; }
+    }
+    for (int q = tid; q < S.in_num; q += NE_T) {
+      for (int j = 0; j < MAXM * 3; j++) D.in_fp[j * D.in_cap + q] = 0.0;
+      for (int j = 0; j < MAXM; j++) D.in_pot[j * D.in_cap + q] = 0.0;
+      D.in_msd[q] = 0.0;
+    }
+    return;
+  }
+
+  constexpr int MM = 2 * ND;    // lists one particle can belong to
+  int qi[NE_K];
+  double px[NE_K], py[NE_K], pz[NE_K], pvx[NE_K], pvy[NE_K], pvz[NE_K];
+  double* msd = S.msd + tid;     // msd[k * NE_T] is slot k of this thread
+  int* pf = S.flag + tid;
+  unsigned* pm = S.memb + tid;
+#define SL(k) ((k) * NE_T)
+
+  auto load_slots = [&]() {
+    NE_SLOT_LOOP {
+      int q = tid + k * NE_T;
+      qi[k] = q < S.in_num ? D.in_list[q] : -1;
+      if (qi[k] >= 0) {
+        int p = qi[k];
+        px[k] = D.x[p]; py[k] = D.y[p]; pz[k] = D.z[p];
+        pvx[k] = D.vx[p]; pvy[k] = D.vy[p]; pvz[k] = D.vz[p];
+        pf[SL(k)] = D.flag[p]; pm[SL(k)] = D.in_memb[q];
+      } else {
+        px[k] = py[k] = pz[k] = pvx[k] = pvy[k] = pvz[k] = 0.0; pf[SL(k)] = -1; pm[SL(k)] = 0;
+      }
+      msd[SL(k)] = 0.0;
+    }
+  };
+  auto store_slots = [&]() {
+    NE_SLOT_LOOP {
+      if (qi[k] >= 0) {
+        int p = qi[k];
+        D.x[p] = px[k]; D.y[p] = py[k]; D.z[p] = pz[k];
+        D.vx[p] = pvx[k]; D.vy[p] = pvy[k]; D.vz[p] = pvz[k];
+        D.flag[p] = pf[SL(k)];
+      }
+    }
+  };
+  // kick (evolve_mod.f95:225-250) followed by inside streaming (:252-316)
+  auto kick_stream = [&](int k, const double* fx, const double* fy, const double* fz, int nm) {
+    bool ok;
+    double h = kick_factor(P, pf[SL(k)], ok);
+    if (ok)
+      for (int m = 0; m < nm; m++) {
+        pvx[k] = pvx[k] + h * fx[m];
+        pvy[k] = pvy[k] + h * fy[m];
+        pvz[k] = pvz[k] + h * fz[m];
+      }
+    stream_particle(px[k], py[k], pz[k], pvx[k], pvy[k], pvz[k], pf[SL(k)], 1.0, P, D.react_box, &sc->err);
+    double v2 = (pvx[k] * pvx[k] + pvy[k] * pvy[k]) + pvz[k] * pvz[k];
+    msd[SL(k)] = msd[SL(k)] + v2 * (P.MDt * P.MDt);
+  };
+
+  bool resume = (mode & M_RESUME) != 0;
+  if (!resume) {
+    // status of this interval's near set: only the inside list is inside
+    for (int q = tid; q < S.near_num; q += NE_T) D.status[D.near_list[q]] = 0;
+    __syncthreads();
+    for (int q = tid; q < S.in_num; q += NE_T) D.status[D.in_list[q]] = ST_INSIDE;
+    load_slots();
+    // first step: kick with the forces stored at the end of the previous interval
+    NE_SLOT_LOOP {
+      if (qi[k] < 0) continue;
+      int q = tid + k * NE_T;
+      double fx[MM], fy[MM], fz[MM];
+      int nm = __popc(pm[SL(k)]);
+      for (int m = 0; m < nm && m < MM; m++) {
+        fx[m] = D.in_fp[(m * 3 + 0) * D.in_cap + q];
+        fy[m] = D.in_fp[(m * 3 + 1) * D.in_cap + q];
+        fz[m] = D.in_fp[(m * 3 + 2) * D.in_cap + q];
+      }
+      msd[SL(k)] = D.in_msd[q];
+      kick_stream(k, fx, fy, fz, nm);
+    }
+    // reduce max_solvent_dist for the first rebuild test
+    double mx = 0.0;
+    NE_SLOT_LOOP mx = fmax(mx, msd[SL(k)]);
+    mx = wave_max(mx);
+    if (lane == 0) S.red[wv][0] = mx;
+    __syncthreads();
+    if (tid == 0) {
+      double m = 0.0;
+      for (int w = 0; w < NE_T / 64; w++) m = fmax(m, S.red[w][0]);
+      S.red[0][NV - 1] = m;
+    }
+    __syncthreads();
+  }
+
+  for (int t = 1; t <= nsteps; t++) {
+    // ---- thread 0: RATTLE positions, bookkeeping, rebuild test (evolve_mod.f95:318-465)
+    if (!(resume && t == nsteps)) {
+      if (tid == 0) {
+        double maxv = S.red[0][NV - 1];
+        S.cstep += 1;
+        S.tc += 1;
+        rattle_positions<ND>(P, S.sd, &sc->err);
+        S.rebuild = (maxv > P.hs2 || S.tc == S.tp || S.cstep % P.MPCtstep == 0);
+      }
+      __syncthreads();
+      if (S.rebuild) {
+        store_slots();
+        NE_SLOT_LOOP msd[SL(k)] = 0.0;
+        if (tid == 0) {
+          if (S.nev < MAXEV) sc->events[S.nev] = S.tc; else raise_error(&sc->err, ERR_EVENTS);
+          S.nev += 1;
+          for (int s = 0; s < NSPH; s++) {   // the near/far split needs every sphere inside its guard
+            double a = pbc_half(S.sd[DX][s] - sc->x0[s], P.box[0]);
+            double b = pbc_half(S.sd[DY][s] - sc->y0[s], P.box[1]);
+            double c = S.sd[DZ][s] - sc->z0[s];
+            if (a * a + b * b + c * c > P.guard * P.guard) raise_error(&sc->err, ERR_NEAR_GUARD);
+          }
+        }
+        __syncthreads();
+        // lazy streaming of the near particles outside the lists (evolve_mod.f95:467-529)
+        double vm = 0.0, tc = (double)S.tc;
+        for (int q = tid; q < S.near_num; q += NE_T) {
+          int p = D.near_list[q];
+          double vx = D.vx[p], vy = D.vy[p], vz = D.vz[p];
+          if (!(D.status[p] & ST_INSIDE)) {
+            double x = D.x[p], y = D.y[p], z = D.z[p];
+            int f = D.flag[p];
+            stream_particle(x, y, z, vx, vy, vz, f, tc, P, D.react_box, &sc->err);
+            D.x[p] = x; D.y[p] = y; D.z[p] = z; D.vx[p] = vx; D.vy[p] = vy; D.vz[p] = vz; D.flag[p] = f;
+          }
+          vm = fmax(vm, (vx * vx + vy * vy) + vz * vz);
+        }
+        vm = wave_max(vm);
+        if (lane == 0) S.red[wv][0] = vm;
+        __syncthreads();
+        if (tid == 0) {
+          double m = 0.0;
+          for (int w = 0; w < NE_T / 64; w++) m = fmax(m, S.red[w][0]);
+          S.vmax2 = m;
+          if ((mode & M_STOP) && t == nsteps) S.stop = 1;
+        }
+        __syncthreads();
+        if (S.stop) {   // the host sorts, then resumes with M_RESUME
+          if (tid == 0) {
+            sc->cstep = S.cstep; sc->time_count = S.tc; sc->time_penetrate = S.tp; sc->n_ev = S.nev;
+            sc->near_vmax2 = S.vmax2;
+          }
+          for (int j = tid; j < 12 * MAXS; j += NE_T) D.dim[j] = S.sd[j / MAXS][j % MAXS];
+          return;
+        }
+      }
+    }
+    if (S.rebuild || (resume && t == nsteps)) {
+      ne_classify<ND>(P, D, S);
+      load_slots();
+    }
+
+    // ---- particle phase: forces (forces_mod.f95:62-136), kick, and the next step's kick + stream
+    double acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) acc[v] = 0.0;
+    const bool last = (t == nsteps);
+    NE_SLOT_LOOP {
+      if (qi[k] < 0) continue;
+      double fx[MM], fy[MM], fz[MM], pot[MM];
+      int nm = 0;
+      unsigned bits = pm[SL(k)];
+      while (bits && nm < MM) {
+        int b = __ffs(bits) - 1;
+        bits &= bits - 1;
+        int kk = b >> 1, ii = b & 1, s = kk + ii * ND;
+        double d1 = pbc_half(px[k] - S.sd[DX][s], P.box[0]);
+        double d2 = pbc_half(py[k] - S.sd[DY][s], P.box[1]);
+        double d3 = pz[k] - S.sd[DZ][s];
+        double dRsq = d1 * d1 + d2 * d2 + d3 * d3;
+        double F = 0.0, E = 0.0, Ub = 0.0;
+        bool on = false;
+        if (dRsq <= P.cut2[ii]) {
+          if (pf[SL(k)] == 0) {
+            if (ii == 0) { pf[SL(k)] = 1; E = P.Eb[0]; } else E = P.Ea[1];
+            on = true;
+          } else if (pf[SL(k)] == 1) { E = P.Eb[ii]; on = true; }
+          else if (pf[SL(k)] == 2) { E = P.Es[ii]; on = true; }
+          if (on) lj_terms(dRsq, P.R6[ii], P.R12[ii], P.r12R12[ii], P.r6R6[ii], F, Ub);
+        }
+        double EF = E * F;
+        fx[nm] = on ? EF * d1 : 0.0;
+        fy[nm] = on ? EF * d2 : 0.0;
+        fz[nm] = on ? EF * d3 : 0.0;
+        pot[nm] = on ? E * Ub : 0.0;
+#pragma unroll
+        for (int ss = 0; ss < NSPH; ss++)
+          if (ss == s) {
+            acc[4 * ss + 0] += fx[nm]; acc[4 * ss + 1] += fy[nm];
+            acc[4 * ss + 2] += fz[nm]; acc[4 * ss + 3] += pot[nm];
+          }
+        nm++;
+      }
+      // kick of evolve_mod.f95:645-670 with the flags after every flip of this step
+      bool ok;
+      double h = kick_factor(P, pf[SL(k)], ok);
+      if (ok)
+        for (int m = 0; m < nm; m++) {
+          pvx[k] = pvx[k] + h * fx[m];
+          pvy[k] = pvy[k] + h * fy[m];
+          pvz[k] = pvz[k] + h * fz[m];
+        }
+      if (!last) {
+        kick_stream(k, fx, fy, fz, nm);
+        acc[NV - 1] = fmax(acc[NV - 1], msd[SL(k)]);
+      } else {
+        int q = tid + k * NE_T;
+        for (int m = 0; m < nm; m++) {
+          D.in_fp[(m * 3 + 0) * D.in_cap + q] = fx[m];
+          D.in_fp[(m * 3 + 1) * D.in_cap + q] = fy[m];
+          D.in_fp[(m * 3 + 2) * D.in_cap + q] = fz[m];
+          D.in_pot[m * D.in_cap + q] = pot[m];
+        }
+      }
+    }
+    // block reduction (fixed order): sphere sums and max_solvent_dist
+#pragma unroll
+    for (int v = 0; v < NV - 1; v++) {
+      double r = wave_sum(acc[v]);
+      if (lane == 0) S.red[wv][v] = r;
+    }
+    {
+      double r = wave_max(acc[NV - 1]);
+      if (lane == 0) S.red[wv][NV - 1] = r;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      for (int v = 0; v < NV; v++) {
+        double a = S.red[0][v];
+        for (int w = 1; w < NE_T / 64; w++) a = (v == NV - 1) ? fmax(a, S.red[w][v]) : a + S.red[w][v];
+        S.red[0][v] = a;
+      }
+      for (int s = 0; s < NSPH; s++)
+        for (int c = 0; c < 4; c++) S.sumF[s][c] = S.red[0][4 * s + c];
+      double tpot;
+      dimer_forces_and_rattle_v<ND>(P, S.sd, S.sumF, tpot, &sc->err);
+      sc->tpotential = tpot;
+    }
+    __syncthreads();
+  }
+
+  // ---- end of the interval: census exclusion (distribute_mod.f95:149-158), write back
+  NE_SLOT_LOOP {
+    if (qi[k] < 0) continue;
+    bool excl = false;
+    for (int kk = 0; kk < ND; kk++)
+      for (int ii = 0; ii < 2; ii++) {
+        int s = kk + ii * ND;
+        double d1 = pbc_half(px[k] - S.sd[DX][s], P.box[0]);
+        double d2 = pbc_half(py[k] - S.sd[DY][s], P.box[1]);
+        double d3 = pz[k] - S.sd[DZ][s];
+        if (d1 * d1 + d2 * d2 + d3 * d3 <= P.cut2[ii]) excl = true;
+      }
+    D.status[qi[k]] = ST_INSIDE | (excl ? ST_EXCL : 0);
+    D.in_msd[tid + k * NE_T] = msd[SL(k)];
+  }
+  store_slots();
+  if (tid == 0) {
+    sc->cstep = S.cstep; sc->time_count = S.tc; sc->time_penetrate = S.tp; sc->in_num = S.in_num;
+    sc->n_ev = S.nev;
+    for (int s = 0; s < NSPH; s++) { sc->x0[s] = S.sd[DX][s]; sc->y0[s] = S.sd[DY][s]; sc->z0[s] = S.sd[DZ][s]; }
+  }
+  for (int j = tid; j < 12 * MAXS; j += NE_T) D.dim[j] = S.sd[j / MAXS][j % MAXS];
+}
+
+#undef SL
+
+// ------------------------------------------------------------- bulk kernels
+__device__ __forceinline__ void collision_shift(const CoP& P, const CoD& D, double& xs, double& ys, double& zs) {
+  double u0, u1, u2;
+  if (P.rng_mode == 0) {
+    u0 = D.U[0]; u1 = D.U[1]; u2 = D.U[2];
+  } else {
+    double t;
+    philox_uniform2(P.seed, 0, D.coll, 0, u0, u1);
+    philox_uniform2(P.seed, 0, D.coll, 1, u2, t);
+  }
+  xs = P.a0 * (u0 - 0.5);
+  ys = P.a0 * (u1 - 0.5);
+  zs = P.a0 * (u2 - 0.5);
+}
+
+// mpc_rotation_mod.f95:116-145: cell of the shifted grid, 0-based
+__device__ __forceinline__ long long shifted_cell(const CoP& P, double x, double y, double z, double xs, double ys,
+                                                  double zs) {
+  double X = x + xs;
+  X = X - P.box[0] * floor(X / P.box[0]);
+  double Y = y + ys;
+  Y = Y - P.box[1] * floor(Y / P.box[1]);
+  double Z = z + zs;
+  double fz = zs < 0.0 ? floor(Z + 1.0) : floor(Z);
+  double c = ffloor_i(X) + P.box[0] * ffloor_i(Y);
+  c = c + (P.box[0] * P.box[1]) * (double)(int)fz;
+  c = c + 1.0;
+  return (long long)c - 1;
+}
+
+// Far particles: replay the interval's lazy-streaming events; then bin every
+// particle into the shifted grid (count + rank) and the census.
+__global__ __launch_bounds__(256) void co_bulk(CoP P, CoD D, int do_stream, int do_bin) {
+  const int NT = P.NT;
+  const int nev = D.sc->n_ev;
+  __shared__ int ev[MAXEV];
+  for (int j = threadIdx.x; j < nev && j < MAXEV; j += blockDim.x) ev[j] = D.sc->events[j];
+  __syncthreads();
+  double xs = 0, ys = 0, zs = 0;
+  if (do_bin) collision_shift(P, D, xs, ys, zs);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < NT; i += gridDim.x * blockDim.x) {
+    const bool near = (D.near_bits[i >> 5] >> (i & 31)) & 1u;
+    double x = D.x[i], y = D.y[i], z = D.z[i];
+    int f = D.flag[i];
+    if (do_stream && !near && nev > 0) {
+      double vx = D.vx[i], vy = D.vy[i], vz = D.vz[i];
+      const int f0 = f;
+      bool bounced = false;
+      for (int e = 0; e < nev && e < MAXEV; e++)
+        bounced |= stream_particle(x, y, z, vx, vy, vz, f, (double)ev[e], P, D.react_box, &D.sc->err);
+      D.x[i] = x; D.y[i] = y; D.z[i] = z;
+      if (bounced) { D.vx[i] = vx; D.vy[i] = vy; D.vz[i] = vz; }
+      if (f != f0) D.flag[i] = f;
+    }
+    if (!do_bin) continue;
+    long long c = shifted_cell(P, x, y, z, xs, ys, zs);
+    if (c < 0 || c >= P.mpcnumcell) {
+      raise_error(&D.sc->err, ERR_CELL_RANGE);
+      D.cell_of[i] = 0;
+      D.rank_of[i] = 0xffff;   // skipped by the scatter
+      continue;
+    }
+    int r = atomicAdd(&D.cnt[c], 1);
+    if (r >= BMAX) raise_error(&D.sc->err, ERR_BUCKET);
+    D.cell_of[i] = (int)c;
+    D.rank_of[i] = (unsigned short)min(r, 65535);
+    // census of distribute_mod.f95:95-183: outside particles keyed by index,
+    // inside ones after all outside ones, inside ones within a cutoff skipped
+    unsigned key = (unsigned)i;
+    bool skip = false;
+    if (near) {
+      unsigned char st = D.status[i];
+      if (st & ST_INSIDE) { key = (unsigned)NT + (unsigned)i; skip = (st & ST_EXCL) != 0; }
+    }
+    int sp = f == 0 ? 0 : f == 1 ? 1 : f == 2 ? 2 : -1;
+    if (skip || sp < 0) continue;
+    long long u = cell_unshifted0(x, y, z, P.box[0], P.box[1]);
+    if (u < 0 || u >= P.vol) { raise_error(&D.sc->err, ERR_CELL_RANGE); continue; }
+    int n = atomicAdd(&D.tcount[sp * P.vol + u], 1);
+    if (n >= CENSUS_MAX) raise_error(&D.sc->err, ERR_MEMBERS);
+    if (sp == 1) atomicMin(&D.firstB[u], key);
+  }
+}
+
+// exclusive scan of cnt[0..n) into off[0..n] (off[n] = total), three passes
+__global__ __launch_bounds__(1024) void scan_partial(const int* cnt, long long n, int* partial) {
+  __shared__ unsigned long long s[17];
+  long long base = (long long)blockIdx.x * 4096;
+  unsigned long long v = 0;
+  for (int j = 0; j < 4; j++) {
+    long long i = base + threadIdx.x * 4 + j;
+    if (i < n) v += cnt[i];
+  }
+  unsigned long long tot;
+  block_excl_scan64(v, s, tot);
+  if (threadIdx.x == 0) partial[blockIdx.x] = (int)tot;
+}
+__global__ __launch_bounds__(1024) void scan_top(int* partial, int nb) {
+  __shared__ unsigned long long s[17];
+  __shared__ long long run;
+  if (threadIdx.x == 0) run = 0;
+  __syncthreads();
+  for (int b0 = 0; b0 < nb; b0 += 1024) {
+    int i = b0 + threadIdx.x;
+    unsigned long long v = i < nb ? (unsigned)partial[i] : 0;
+    unsigned long long tot;
+    unsigned long long e = block_excl_scan64(v, s, tot);
+    if (i < nb) partial[i] = (int)(run + (long long)e);
+    __syncthreads();
+    if (threadIdx.x == 0) run += (long long)tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[nb] = (int)run;
+}
+__global__ __launch_bounds__(1024) void scan_final(const int* cnt, long long n, const int* partial, int* off) {
+  __shared__ unsigned long long s[17];
+  long long base = (long long)blockIdx.x * 4096;
+  int v[4];
+  unsigned long long sum = 0;
+  for (int j = 0; j < 4; j++) {
+    long long i = base + threadIdx.x * 4 + j;
+    v[j] = i < n ? cnt[i] : 0;
+    sum += v[j];
+  }
+  unsigned long long tot;
+  unsigned long long e = block_excl_scan64(sum, s, tot);
+  long long run = partial[blockIdx.x] + (long long)e;
+  for (int j = 0; j < 4; j++) {
+    long long i = base + threadIdx.x * 4 + j;
+    if (i < n) off[i] = (int)run;
+    run += v[j];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) off[n] = partial[gridDim.x];
+}
+
+__global__ __launch_bounds__(256) void co_scatter(CoP P, CoD D) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.NT; i += gridDim.x * blockDim.x) {
+    int r = D.rank_of[i];
+    if (r < BMAX) D.bucket[D.off[D.cell_of[i]] + r] = i;
+  }
+}
+
+// Per collision cell: members in ascending particle order, centre-of-mass
+// velocity (mpc_rotation_mod.f95:153-182), random axis (:184-193); per
+// unshifted cell the B->A reaction (reaction_mod.f95:41-62).
+__global__ __launch_bounds__(CT) void co_collide(CoP P, CoD D) {
+  __shared__ int sidx[BMAX * CT];
+  const long long c = (long long)blockIdx.x * CT + threadIdx.x;
+  if (c >= P.mpcnumcell) return;
+  int n = D.cnt[c];
+  if (n > BMAX) n = BMAX;   // ERR_BUCKET already raised
+  int o = D.off[c];
+  for (int j = 0; j < n; j++) {
+    int v = D.bucket[o + j];
+    int k = j;
+    while (k > 0 && sidx[(k - 1) * CT + threadIdx.x] > v) {
+      sidx[k * CT + threadIdx.x] = sidx[(k - 1) * CT + threadIdx.x];
+      k--;
+    }
+    sidx[k * CT + threadIdx.x] = v;
+  }
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  for (int j = 0; j < n; j++) {
+    int i = sidx[j * CT + threadIdx.x];
+    sx = sx + D.vx[i];
+    sy = sy + D.vy[i];
+    sz = sz + D.vz[i];
+  }
+  if (n != 0) {
+    double tn = (double)n;
+    sx = sx / tn; sy = sy / tn; sz = sz / tn;
+  }
+  double u1, u2;
+  if (P.rng_mode == 0) {
+    u1 = D.U[3 + 2 * c];
+    u2 = D.U[4 + 2 * c];
+  } else {
+    philox_uniform2(P.seed, 1, D.coll, (unsigned long long)c, u1, u2);
+  }
+  double phi = P.twopi * u1;
+  double th = 2.0 * u2 - 1.0;
+  double st = sqrt(1.0 - th * th);
+  double* rec = D.cellrec + c * 8;
+  rec[0] = sx; rec[1] = sy; rec[2] = sz;
+  rec[3] = st * cos(phi); rec[4] = st * sin(phi); rec[5] = th;
+  rec[6] = (double)n; rec[7] = 0.0;
+  if (c < P.rxnnumcell) {
+    int nb = D.tcount[1 * P.vol + c];
+    double prob = D.ptab[min(nb, CENSUS_MAX)];
+    double r;
+    if (P.rng_mode == 0) r = D.U[3 + 2 * P.mpcnumcell + c];
+    else { double t; philox_uniform2(P.seed, 2, D.coll, (unsigned long long)c, r, t); }
+    if (0.0 < r && r <= prob) {
+      unsigned key = D.firstB[c];
+      int i = key >= (unsigned)P.NT ? (int)(key - (unsigned)P.NT) : (int)key;
+      D.flag[i] = 0;
+    }
+  }
+}
+
+// Rotation of mpc_rotation_mod.f95:205-230 and selection of the next
+// interval's near set.  The particle lies in the unit box of its shifted
+// cell, so the distance from that box to each sphere bounds its distance.
+__global__ __launch_bounds__(256) void co_rotate(CoP P, CoD D, int classify) {
+  double xs, ys, zs;
+  collision_shift(P, D, xs, ys, zs);
+  const int nd = P.nd;
+  double sx[MAXS], sy[MAXS], sz[MAXS];
+  for (int s = 0; s < 2 * nd; s++) { sx[s] = D.sc->x0[s]; sy[s] = D.sc->y0[s]; sz[s] = D.sc->z0[s]; }
+  const int Lx = P.Lx, Ly = P.Ly;
+  const double zoff = zs < 0.0 ? 1.0 : 0.0;
+  double fmax2 = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.NT; i += gridDim.x * blockDim.x) {
+    int c = D.cell_of[i];
+    const double* rec = D.cellrec + (long long)c * 8;
+    double cx = rec[0], cy = rec[1], cz = rec[2], ux = rec[3], uy = rec[4], uz = rec[5];
+    double vx = D.vx[i], vy = D.vy[i], vz = D.vz[i];
+    double dx = vx - cx, dy = vy - cy, dz = vz - cz;
+    const double cr = P.cosrot, sr = P.sinrot, omc = P.omc;
+    double rx = (dx * (cr + (ux * ux) * omc) + dy * ((ux * uy) * omc - uz * sr)) + dz * ((ux * uz) * omc + uy * sr);
+    double ry = (dx * ((ux * uy) * omc + uz * sr) + dy * (cr + (uy * uy) * omc)) + dz * ((uy * uz) * omc - ux * sr);
+    double rz = (dx * ((ux * uz) * omc - uy * sr) + dy * ((uy * uz) * omc + ux * sr)) + dz * (cr + (uz * uz) * omc);
+    vx = cx + rx; vy = cy + ry; vz = cz + rz;
+    D.vx[i] = vx; D.vy[i] = vy; D.vz[i] = vz;
+    if (!classify) continue;
+    double v2 = (vx * vx + vy * vy) + vz * vz;
+    double reach = sqrt(v2) * P.tau * (1.0 + 1e-9) + P.guard + 1e-6;
+    int ix = c % Lx, iy = (c / Lx) % Ly, iz = c / (Lx * Ly);
+    // particle box: x in [ix - xs, ix + 1 - xs) (mod Lx), z in [iz - zs - zoff, iz + 1 - zs - zoff)
+    double mx = (double)ix - xs + 0.5, my = (double)iy - ys + 0.5, mz = (double)iz - zs - zoff + 0.5;
+    bool near = false;
+    for (int s = 0; s < 2 * nd; s++) {
+      int ii = s >= nd;
+      double ax = fmax(fabs(pbc_half(mx - sx[s], P.box[0])) - 0.5 - 1e-9, 0.0);
+      double ay = fmax(fabs(pbc_half(my - sy[s], P.box[1])) - 0.5 - 1e-9, 0.0);
+      double az = fmax(fabs(mz - sz[s]) - 0.5 - 1e-9, 0.0);
+      double r = P.full[ii] + reach;
+      if (ax * ax + ay * ay + az * az <= r * r) near = true;
+    }
+    if (near) atomicOr(&D.near_bits[i >> 5], 1u << (i & 31));
+    else fmax2 = fmax(fmax2, v2);
+  }
+  if (!classify) return;
+  fmax2 = wave_max(fmax2);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = fmax2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    atomicMax(&D.sc->far_maxv2, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// Near set from particle positions (set-up, before any collision).
+__global__ __launch_bounds__(256) void co_classify_all(CoP P, CoD D) {
+  const int nd = P.nd;
+  double fmax2 = 0.0, nmax2 = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.NT; i += gridDim.x * blockDim.x) {
+    double x = D.x[i], y = D.y[i], z = D.z[i], vx = D.vx[i], vy = D.vy[i], vz = D.vz[i];
+    double v2 = (vx * vx + vy * vy) + vz * vz;
+    double reach = sqrt(v2) * P.tau * (1.0 + 1e-9) + P.guard + 1e-6;
+    bool near = false;
+    for (int s = 0; s < 2 * nd; s++) {
+      int ii = s >= nd;
+      double a = pbc_half(x - D.sc->x0[s], P.box[0]);
+      double b = pbc_half(y - D.sc->y0[s], P.box[1]);
+      double c = z - D.sc->z0[s];
+      double r = P.full[ii] + reach;
+      if (a * a + b * b + c * c <= r * r) near = true;
+    }
+    if (near) { atomicOr(&D.near_bits[i >> 5], 1u << (i & 31)); nmax2 = fmax(nmax2, v2); }
+    else fmax2 = fmax(fmax2, v2);
+  }
+  fmax2 = wave_max(fmax2);
+  nmax2 = wave_max(nmax2);
+  __shared__ double red[8];
+  if ((threadIdx.x & 63) == 0) { red[threadIdx.x >> 6] = fmax2; red[4 + (threadIdx.x >> 6)] = nmax2; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    double n = fmax(fmax(red[4], red[5]), fmax(red[6], red[7]));
+    atomicMax(&D.sc->far_maxv2, (unsigned long long)__double_as_longlong(m));
+    atomicMax((unsigned long long*)&D.sc->near_vmax2, (unsigned long long)__double_as_longlong(n));
+  }
+}
+
+// ordered list of the set bits: counts per 1024-word block, then positions
+__global__ __launch_bounds__(1024) void bits_count(const unsigned* bits, long long nwords, int* partial) {
+  __shared__ unsigned long long s[17];
+  long long w = (long long)blockIdx.x * 1024 + threadIdx.x;
+  unsigned long long v = w < nwords ? __popc(bits[w]) : 0;
+  unsigned long long tot;
+  block_excl_scan64(v, s, tot);
+  if (threadIdx.x == 0) partial[blockIdx.x] = (int)tot;
+}
+__global__ __launch_bounds__(1024) void bits_emit(const unsigned* bits, long long nwords, const int* partial,
+                                                  int* list, int cap, int NT, CoScalars* sc, int nblocks) {
+  __shared__ unsigned long long s[17];
+  long long w = (long long)blockIdx.x * 1024 + threadIdx.x;
+  unsigned word = w < nwords ? bits[w] : 0u;
+  unsigned long long tot;
+  unsigned long long e = block_excl_scan64(__popc(word), s, tot);
+  int pos = partial[blockIdx.x] + (int)e;
+  while (word) {
+    int b = __ffs(word) - 1;
+    word &= word - 1;
+    int idx = (int)(w * 32 + b);
+    if (idx < NT) {
+      if (pos < cap) list[pos] = idx;
+      pos++;
+    }
+  }
+  if (blockIdx.x == nblocks - 1 && threadIdx.x == 0) {
+    int n = partial[nblocks];
+    sc->near_num = n < cap ? n : cap;
+    if (n > cap) raise_error(&sc->err, ERR_NEAR_CAP);
+  }
+}
+
+template <typename T>
+__global__ void fill_kernel(T* p, long long n, T v) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void reset_far(CoScalars* sc) { sc->far_maxv2 = 0ull; }
+
+// --------------------------------------------------------------- Hilbert sort
+// sort_hilbert_mod.f95:50-92: stable bin by tour position of the unshifted cell
+__global__ __launch_bounds__(256) void sort_bin(CoP P, CoD D, const int* tourpos, int* scnt, int* key,
+                                                unsigned short* rk) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.NT; i += gridDim.x * blockDim.x) {
+    long long u = cell_unshifted0(D.x[i], D.y[i], D.z[i], P.box[0], P.box[1]);
+    if (u < 0 || u >= P.vol || tourpos[u] < 0) { raise_error(&D.sc->err, ERR_CELL_RANGE); key[i] = 0; rk[i] = 0; continue; }
+    int t = tourpos[u];
+    int r = atomicAdd(&scnt[t], 1);
+    if (r >= BMAX) raise_error(&D.sc->err, ERR_BUCKET);
+    key[i] = t;
+    rk[i] = (unsigned short)min(r, 65535);
+  }
+}
+__global__ __launch_bounds__(256) void sort_scatter(int NT, const int* key, const unsigned short* rk, const int* soff,
+                                                    int* sb) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < NT; i += gridDim.x * blockDim.x)
+    if (rk[i] < BMAX) sb[soff[key[i]] + rk[i]] = i;
+}
+__global__ __launch_bounds__(256) void sort_order(long long ncell, const int* scnt, const int* soff, int* sb) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncell) return;
+  int n = min(scnt[t], BMAX), o = soff[t];
+  for (int j = 1; j < n; j++) {   // insertion sort: ascending original index
+    int v = sb[o + j], k = j;
+    while (k > 0 && sb[o + k - 1] > v) { sb[o + k] = sb[o + k - 1]; k--; }
+    sb[o + k] = v;
+  }
+}
+__global__ __launch_bounds__(256) void sort_gather(CoP P, CoD D, const int* sb, double* nx, double* ny, double* nz,
+                                                   double* nvx, double* nvy, double* nvz, int* nf,
+                                                   unsigned char* nst, unsigned* nbits) {
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < P.NT; j += gridDim.x * blockDim.x) {
+    int i = sb[j];
+    nx[j] = D.x[i]; ny[j] = D.y[i]; nz[j] = D.z[i];
+    nvx[j] = D.vx[i]; nvy[j] = D.vy[i]; nvz[j] = D.vz[i];
+    nf[j] = D.flag[i];
+    nst[j] = D.status[i];
+    if ((D.near_bits[i >> 5] >> (i & 31)) & 1u) atomicOr(&nbits[j >> 5], 1u << (j & 31));
+  }
+}
+
+}  // namespace mpcx
+
+// =====================================================================================
+// Host side
+// =====================================================================================
+using namespace mpcx;
+
+struct mpcx_co {
+  mpcx_co_config cfg;
+  CoP P;
+  CoD D;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr, b0 = nullptr, b1 = nullptr, n0 = nullptr, n1 = nullptr;
+  double run_ms = 0, bulk_ms = 0, near_ms = 0;
+  long long ndraw = 0;
+  double* d_U = nullptr;
+  int* d_tourpos = nullptr;
+  bool has_tour = false;
+  // sort buffers
+  int *d_skey = nullptr, *d_scnt = nullptr, *d_soff = nullptr, *d_sb = nullptr, *d_spart = nullptr;
+  unsigned short* d_srk = nullptr;
+  double *alt[6] = {nullptr};
+  int* alt_flag = nullptr;
+  unsigned char* alt_status = nullptr;
+  unsigned* alt_bits = nullptr;
+  std::vector<void*> allocs;
+  long long nwords = 0;
+  int nd = 1;
+};
+
+template <typename T>
+static T* dalloc(mpcx_co* e, size_t n) {
+  void* p = nullptr;
+  if (hipMalloc(&p, n * sizeof(T) + 64) != hipSuccess) return nullptr;
+  e->allocs.push_back(p);
+  return (T*)p;
+}
+
+static int grid_for(long long n, int threads, int cap = 4096) {
+  long long g = (n + threads - 1) / threads;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (int)g;
+}
+
+static void launch_scan(hipStream_t st, const int* cnt, long long n, int* partial, int* off) {
+  int nb = (int)((n + 4095) / 4096);
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(scan_partial, dim3(nb), dim3(1024), 0, st, cnt, n, partial);
+  hipLaunchKernelGGL(scan_top, dim3(1), dim3(1024), 0, st, partial, nb);
+  hipLaunchKernelGGL(scan_final, dim3(nb), dim3(1024), 0, st, cnt, n, (const int*)partial, off);
+}
+
+static int check_device_error(mpcx_co* e) {
+  int err = 0;
+  if (hipMemcpy(&err, &e->D.sc->err, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+    return set_error(MPCX_E_HIP, "hipMemcpy of the error flag failed");
+  if (err) {
+    static const char* msg[] = {"none",
+                                "a particle maps outside the cell arrays",
+                                "a cell holds more members than the reference arrays (100)",
+                                "inside set larger than the near engine's 4096 register slots",
+                                "a sphere moved further than the near/far guard within one interval",
+                                "a particle belongs to more than 4 neighbour lists",
+                                "a collision cell holds more than 64 particles",
+                                "near set larger than its buffer",
+                                "a neighbour list is longer than max_neighbour",
+                                "more than 64 list rebuilds in one interval",
+                                "the dimer bond constraint did not converge"};
+    std::string m = std::string("device check failed: ") + (err > 0 && err < 11 ? msg[err] : "unknown");
+    int zero = 0;
+    (void)hipMemcpy(&e->D.sc->err, &zero, sizeof(int), hipMemcpyHostToDevice);
+    return set_error(MPCX_E_DEVICE, m.c_str());
+  }
+  return 0;
+}
+
+#define HIPCHK(x)                                                        \
+  do {                                                                   \
+    hipError_t _r = (x);                                                 \
+    if (_r != hipSuccess) return set_error(MPCX_E_HIP, hipGetErrorString(_r)); \
+  } while (0)
+
+extern "C" int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg) {
+  if (!out || !cfg) return set_error(MPCX_E_ARG, "null argument");
+  *out = nullptr;
+  if (cfg->numdims < 1 || cfg->numdims > MAXD)
+    return set_error(MPCX_E_UNSUPPORTED, "the device-resident collective engine handles 1 or 2 dimers");
+  if (cfg->NT < 1 || cfg->NT > 0x7fffff00LL || cfg->vol < 1) return set_error(MPCX_E_ARG, "bad sizes");
+  if (cfg->sort_step % cfg->MPCtstep != 0)
+    return set_error(MPCX_E_UNSUPPORTED, "sort_step must be a multiple of MPCtstep for the device-resident engine");
+  if (cfg->a0 != 1.0) return set_error(MPCX_E_UNSUPPORTED, "the reference's cell index assumes a0 = 1");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_error(MPCX_E_HIP, "no HIP device");
+  mpcx_co* e = new mpcx_co();
+  e->cfg = *cfg;
+  e->nd = (int)cfg->numdims;
+  CoP& P = e->P;
+  memset(&P, 0, sizeof P);
+  P.NT = (int)cfg->NT; P.nd = (int)cfg->numdims; P.MPCtstep = cfg->MPCtstep; P.sort_step = cfg->sort_step;
+  P.react_dir = cfg->react_dir; P.rng_mode = cfg->rng_mode; P.seed = cfg->seed;
+  P.Lx = (int)cfg->box[0]; P.Ly = (int)cfg->box[1]; P.Lz = (int)cfg->box[2];
+  P.nrb = P.Lx * P.Ly;
+  P.vol = cfg->vol; P.mpcnumcell = cfg->mpcnumcell; P.rxnnumcell = cfg->rxnnumcell;
+  for (int i = 0; i < 3; i++) P.box[i] = cfg->box[i];
+  const double MDt = cfg->MDt;
+  P.MDt = MDt;
+  P.hA = (0.5 * MDt) / cfg->Ma; P.hB = (0.5 * MDt) / cfg->Mb; P.hS = (0.5 * MDt) / cfg->Ms;
+  P.hc = (0.5 * MDt) / cfg->Mc; P.hn = (0.5 * MDt) / cfg->Mn;
+  P.IIMcIMn = 1.0 / ((1.0 / cfg->Mc) + (1.0 / cfg->Mn));
+  P.Mc = cfg->Mc; P.Mn = cfg->Mn; P.Dcn2 = cfg->Dcn * cfg->Dcn;
+  const double R[2] = {cfg->Rc, cfg->Rn};
+  for (int i = 0; i < 2; i++) {
+    double cut = R[i] * cfg->cutoff;
+    P.cut2[i] = cut * cut;
+    P.full[i] = cut + cfg->skinrad;
+    P.full2[i] = P.full[i] * P.full[i];
+    P.R6[i] = ipow(R[i], 6); P.R12[i] = ipow(R[i], 12);
+    P.r6R6[i] = 6.0 * P.R6[i]; P.r12R12[i] = 12.0 * P.R12[i];
+  }
+  P.Ea[0] = cfg->EaC; P.Eb[0] = cfg->EbC; P.Es[0] = cfg->EsC;
+  P.Ea[1] = cfg->EaN; P.Eb[1] = cfg->EbN; P.Es[1] = cfg->EsN;
+  P.Em = cfg->Em;
+  const double dd[3] = {cfg->Rc + cfg->Rc + cfg->Rm_delta, cfg->Rc + cfg->Rn + cfg->Rm_delta,
+                        cfg->Rn + cfg->Rn + cfg->Rm_delta};
+  for (int j = 0; j < 3; j++) {
+    P.mm_R6[j] = ipow(dd[j], 6); P.mm_R12[j] = ipow(dd[j], 12);
+    double c = dd[j] * cfg->cutoff;
+    P.mm_cut2[j] = c * c;
+  }
+  P.Ew = cfg->Ew;
+  for (int i = 0; i < 2; i++) {
+    double w = cfg->Rw + R[i];
+    P.wR3[i] = ipow(w, 3); P.wR9[i] = ipow(w, 9);
+    P.wA[i] = 3.0 * P.wR3[i];   // received as r9R9 (evolve_mod.f95:597-600)
+    P.wB[i] = 9.0 * P.wR9[i];   // received as r3R3
+    double c = w * cfg->wall_w6;
+    P.wcut2[i] = c * c;
+  }
+  P.pref = 1.5 * sqrt(3.0);
+  P.hs2 = (0.5 * cfg->skinrad) * (0.5 * cfg->skinrad);
+  P.a0 = cfg->a0; P.cosrot = cfg->cosrot; P.sinrot = cfg->sinrot; P.omc = 1.0 - cfg->cosrot;
+  P.twopi = 2.0 * (4.0 * atan(1.0));
+  P.tau = (double)cfg->MPCtstep * MDt;
+  P.guard = cfg->sphere_guard > 0 ? cfg->sphere_guard : 1.0;
+  P.skinrad = cfg->skinrad;
+
+  const long long NT = cfg->NT, vol = cfg->vol, mc = cfg->mpcnumcell;
+  CoD& D = e->D;
+  memset(&D, 0, sizeof D);
+  D.x = dalloc<double>(e, NT); D.y = dalloc<double>(e, NT); D.z = dalloc<double>(e, NT);
+  D.vx = dalloc<double>(e, NT); D.vy = dalloc<double>(e, NT); D.vz = dalloc<double>(e, NT);
+  D.flag = dalloc<int>(e, NT);
+  D.dim = dalloc<double>(e, 12 * MAXS);
+  D.sc = dalloc<CoScalars>(e, 1);
+  D.react_box = dalloc<int>(e, std::max(1, P.nrb));
+  D.in_cap = NE_T * NE_K;
+  D.near_cap = (int)std::min<long long>(NT, 1 << 22);
+  D.near_list = dalloc<int>(e, D.near_cap);
+  D.near_memb = dalloc<int>(e, D.near_cap);
+  e->nwords = (NT + 31) / 32;
+  D.near_bits = dalloc<unsigned>(e, e->nwords + 1);
+  D.status = dalloc<unsigned char>(e, NT);
+  D.in_list = dalloc<int>(e, D.in_cap);
+  D.in_memb = dalloc<unsigned>(e, D.in_cap);
+  D.in_fp = dalloc<double>(e, (size_t)MAXM * 3 * D.in_cap);
+  D.in_pot = dalloc<double>(e, (size_t)MAXM * D.in_cap);
+  D.in_msd = dalloc<double>(e, D.in_cap);
+  D.cell_of = dalloc<int>(e, NT);
+  D.rank_of = dalloc<unsigned short>(e, NT);
+  D.cnt = dalloc<int>(e, mc);
+  D.off = dalloc<int>(e, mc + 1);
+  D.bucket = dalloc<int>(e, NT);
+  long long npart = std::max((mc + 4095) / 4096, std::max((vol + 4095) / 4096, (e->nwords + 1023) / 1024)) + 2;
+  D.partial = dalloc<int>(e, npart);
+  D.cellrec = dalloc<double>(e, mc * 8);
+  D.tcount = dalloc<int>(e, 3 * vol);
+  D.firstB = dalloc<unsigned>(e, vol);
+  double* ptab = dalloc<double>(e, 101);
+  D.ptab = ptab;
+  e->ndraw = 3 + 2 * mc + cfg->rxnnumcell;
+  e->d_U = dalloc<double>(e, cfg->rng_mode == 0 ? e->ndraw : 4);
+  D.U = e->d_U;
+  e->d_tourpos = dalloc<int>(e, vol);
+  e->d_skey = dalloc<int>(e, NT); e->d_srk = dalloc<unsigned short>(e, NT);
+  e->d_scnt = dalloc<int>(e, vol); e->d_soff = dalloc<int>(e, vol + 1); e->d_sb = dalloc<int>(e, NT);
+  for (int j = 0; j < 6; j++) e->alt[j] = dalloc<double>(e, NT);
+  e->alt_flag = dalloc<int>(e, NT);
+  e->alt_status = dalloc<unsigned char>(e, NT);
+  e->alt_bits = dalloc<unsigned>(e, e->nwords + 1);
+  for (void* p : e->allocs)
+    if (!p) { mpcx_co_destroy(e); return set_error(MPCX_E_HIP, "device allocation failed"); }
+  if (std::find(e->allocs.begin(), e->allocs.end(), nullptr) != e->allocs.end() || !D.x || !e->alt_bits) {
+    mpcx_co_destroy(e);
+    return set_error(MPCX_E_HIP, "device allocation failed");
+  }
+  HIPCHK(hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking));
+  HIPCHK(hipEventCreate(&e->e0)); HIPCHK(hipEventCreate(&e->e1));
+  HIPCHK(hipEventCreate(&e->b0)); HIPCHK(hipEventCreate(&e->b1));
+  HIPCHK(hipEventCreate(&e->n0)); HIPCHK(hipEventCreate(&e->n1));
+  HIPCHK(hipMemcpy(ptab, cfg->reaction_p, 101 * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(D.sc, 0, sizeof(CoScalars)));
+  HIPCHK(hipMemset(D.status, 0, NT));
+  HIPCHK(hipMemset(D.near_bits, 0, (e->nwords + 1) * 4));
+  HIPCHK(hipMemset(D.in_fp, 0, (size_t)MAXM * 3 * D.in_cap * 8));
+  HIPCHK(hipMemset(D.in_pot, 0, (size_t)MAXM * D.in_cap * 8));
+  HIPCHK(hipMemset(D.in_msd, 0, (size_t)D.in_cap * 8));
+  *out = e;
+  return 0;
+}
+
+extern "C" int mpcx_co_destroy(mpcx_co* e) {
+  if (!e) return 0;
+  if (e->st) (void)hipStreamSynchronize(e->st);
+  for (void* p : e->allocs)
+    if (p) (void)hipFree(p);
+  if (e->st) (void)hipStreamDestroy(e->st);
+  for (hipEvent_t ev : {e->e0, e->e1, e->b0, e->b1, e->n0, e->n1})
+    if (ev) (void)hipEventDestroy(ev);
+  delete e;
+  return 0;
+}
+
+extern "C" int mpcx_co_set_state(mpcx_co* e, int32_t cstep, const double* x, const double* y, const double* z,
+                                 const double* vx, const double* vy, const double* vz, const int32_t* flag,
+                                 const double* xd, const double* yd, const double* zd, const double* vxd,
+                                 const double* vyd, const double* vzd, const double* fxd, const double* fyd,
+                                 const double* fzd, const int32_t* react_box, const int64_t* tour) {
+  if (!e || !x || !y || !z || !vx || !vy || !vz || !flag || !xd || !yd || !zd)
+    return set_error(MPCX_E_ARG, "null argument");
+  const size_t NT = (size_t)e->P.NT, nd = (size_t)e->nd;
+  CoD& D = e->D;
+  HIPCHK(hipMemcpy(D.x, x, NT * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(D.y, y, NT * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(D.z, z, NT * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(D.vx, vx, NT * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(D.vy, vy, NT * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(D.vz, vz, NT * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(D.flag, flag, NT * 4, hipMemcpyHostToDevice));
+  std::vector<double> dim(12 * MAXS, 0.0);
+  const double* src[9] = {xd, yd, zd, vxd, vyd, vzd, fxd, fyd, fzd};
+  for (int f = 0; f < 9; f++)
+    for (size_t s = 0; s < 2 * nd; s++) dim[f * MAXS + s] = src[f] ? src[f][s] : 0.0;
+  HIPCHK(hipMemcpy(D.dim, dim.data(), dim.size() * 8, hipMemcpyHostToDevice));
+  std::vector<int> rb((size_t)std::max(1, e->P.nrb), 0);
+  if (react_box)
+    for (int i = 0; i < e->P.nrb; i++) rb[i] = react_box[i];
+  HIPCHK(hipMemcpy((void*)D.react_box, rb.data(), rb.size() * 4, hipMemcpyHostToDevice));
+  e->has_tour = false;
+  if (tour) {
+    std::vector<int> tp((size_t)e->P.vol, -1);
+    for (long long t = 0; t < e->P.vol; t++) {
+      long long c = tour[t] - 1;
+      if (c < 0 || c >= e->P.vol || tp[c] >= 0)
+        return set_error(MPCX_E_ARG, "the tour must visit every cell exactly once");
+      tp[c] = (int)t;
+    }
+    HIPCHK(hipMemcpy(e->d_tourpos, tp.data(), tp.size() * 4, hipMemcpyHostToDevice));
+    e->has_tour = true;
+  }
+  CoScalars sc;
+  memset(&sc, 0, sizeof sc);
+  sc.cstep = cstep;
+  HIPCHK(hipMemcpy(D.sc, &sc, sizeof sc, hipMemcpyHostToDevice));
+  return 0;
+}
+
+static int co_sort(mpcx_co* e) {
+  CoD& D = e->D;
+  CoP& P = e->P;
+  const long long NT = P.NT, vol = P.vol;
+  if (!e->has_tour) return set_error(MPCX_E_ARG, "a sort step needs the cell tour (pass it to mpcx_co_set_state)");
+  hipStream_t st = e->st;
+  hipLaunchKernelGGL(fill_kernel<int>, dim3(grid_for(vol, 256)), dim3(256), 0, st, e->d_scnt, vol, 0);
+  hipLaunchKernelGGL(sort_bin, dim3(grid_for(NT, 256)), dim3(256), 0, st, P, D, (const int*)e->d_tourpos, e->d_scnt,
+                     e->d_skey, e->d_srk);
+  launch_scan(st, e->d_scnt, vol, D.partial, e->d_soff);
+  hipLaunchKernelGGL(fill_kernel<int>, dim3(grid_for(NT, 256)), dim3(256), 0, st, e->d_sb, NT, 0);
+  hipLaunchKernelGGL(sort_scatter, dim3(grid_for(NT, 256)), dim3(256), 0, st, (int)NT, (const int*)e->d_skey,
+                     (const unsigned short*)e->d_srk, (const int*)e->d_soff, e->d_sb);
+  hipLaunchKernelGGL(sort_order, dim3((unsigned)((vol + 255) / 256)), dim3(256), 0, st, vol, (const int*)e->d_scnt,
+                     (const int*)e->d_soff, e->d_sb);
+  hipLaunchKernelGGL(fill_kernel<unsigned>, dim3(grid_for(e->nwords + 1, 256)), dim3(256), 0, st, e->alt_bits,
+                     e->nwords + 1, 0u);
+  hipLaunchKernelGGL(sort_gather, dim3(grid_for(NT, 256)), dim3(256), 0, st, P, D, (const int*)e->d_sb, e->alt[0],
+                     e->alt[1], e->alt[2], e->alt[3], e->alt[4], e->alt[5], e->alt_flag, e->alt_status, e->alt_bits);
+  std::swap(D.x, e->alt[0]); std::swap(D.y, e->alt[1]); std::swap(D.z, e->alt[2]);
+  std::swap(D.vx, e->alt[3]); std::swap(D.vy, e->alt[4]); std::swap(D.vz, e->alt[5]);
+  std::swap(D.flag, e->alt_flag); std::swap(D.status, e->alt_status); std::swap(D.near_bits, e->alt_bits);
+  return 0;
+}
+
+static void co_near_list(mpcx_co* e) {
+  const long long nw = e->nwords;
+  int nb = (int)((nw + 1023) / 1024);
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(bits_count, dim3(nb), dim3(1024), 0, e->st, (const unsigned*)e->D.near_bits, nw, e->D.partial);
+  hipLaunchKernelGGL(scan_top, dim3(1), dim3(1024), 0, e->st, e->D.partial, nb);
+  hipLaunchKernelGGL(bits_emit, dim3(nb), dim3(1024), 0, e->st, (const unsigned*)e->D.near_bits, nw,
+                     (const int*)e->D.partial, e->D.near_list, e->D.near_cap, e->P.NT, e->D.sc, nb);
+}
+
+static void launch_near(mpcx_co* e, int nsteps, int mode) {
+  if (e->nd == 1)
+    hipLaunchKernelGGL(co_near_engine<1>, dim3(1), dim3(NE_T), 0, e->st, e->P, e->D, nsteps, mode);
+  else
+    hipLaunchKernelGGL(co_near_engine<2>, dim3(1), dim3(NE_T), 0, e->st, e->P, e->D, nsteps, mode);
+}
+
+extern "C" int mpcx_co_setup(mpcx_co* e, int32_t do_sort) {
+  if (!e) return set_error(MPCX_E_ARG, "null engine");
+  CoD& D = e->D;
+  const long long NT = e->P.NT;
+  hipStream_t st = e->st;
+  // sphere positions of this interval start
+  std::vector<double> dim(12 * MAXS);
+  HIPCHK(hipMemcpy(dim.data(), D.dim, dim.size() * 8, hipMemcpyDeviceToHost));
+  CoScalars sc;
+  HIPCHK(hipMemcpy(&sc, D.sc, sizeof sc, hipMemcpyDeviceToHost));
+  for (int s = 0; s < 2 * e->nd; s++) { sc.x0[s] = dim[DX * MAXS + s]; sc.y0[s] = dim[DY * MAXS + s]; sc.z0[s] = dim[DZ * MAXS + s]; }
+  sc.far_maxv2 = 0; sc.near_vmax2 = 0.0; sc.n_ev = 0; sc.err = 0;
+  HIPCHK(hipMemcpy(D.sc, &sc, sizeof sc, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(fill_kernel<unsigned>, dim3(grid_for(e->nwords + 1, 256)), dim3(256), 0, st, D.near_bits,
+                     e->nwords + 1, 0u);
+  if (do_sort) {
+    int r = co_sort(e);
+    if (r) return r;
+  }
+  hipLaunchKernelGGL(fill_kernel<unsigned>, dim3(grid_for(e->nwords + 1, 256)), dim3(256), 0, st, D.near_bits,
+                     e->nwords + 1, 0u);
+  hipLaunchKernelGGL(co_classify_all, dim3(grid_for(NT, 256, 2048)), dim3(256), 0, st, e->P, D);
+  co_near_list(e);
+  hipLaunchKernelGGL(fill_kernel<unsigned char>, dim3(grid_for(NT, 256)), dim3(256), 0, st, D.status, NT,
+                     (unsigned char)0);
+  launch_near(e, 0, M_SETUP);
+  HIPCHK(hipStreamSynchronize(st));
+  return check_device_error(e);
+}
+
+extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
+  if (!e || ncoll < 0) return set_error(MPCX_E_ARG, "bad arguments");
+  {
+    int cs = 0;
+    HIPCHK(hipMemcpy(&cs, &e->D.sc->cstep, sizeof(int), hipMemcpyDeviceToHost));
+    if (cs % e->P.MPCtstep != 0) return set_error(MPCX_E_ARG, "mpcx_co_run starts at a collision step");
+  }
+  if (e->P.rng_mode == 0 && !draws && ncoll > 0) return set_error(MPCX_E_ARG, "rng_mode 0 needs host draws");
+  CoD& D = e->D;
+  CoP& P = e->P;
+  const long long NT = P.NT, mc = P.mpcnumcell, vol = P.vol;
+  hipStream_t st = e->st;
+  int cstep = 0;
+  HIPCHK(hipMemcpy(&cstep, &D.sc->cstep, sizeof(int), hipMemcpyDeviceToHost));
+  e->bulk_ms = 0; e->near_ms = 0;
+  HIPCHK(hipEventRecord(e->e0, st));
+  const int gN = grid_for(NT, 256, 2048);
+  for (int64_t c = 0; c < ncoll; c++) {
+    const int cnext = cstep + P.MPCtstep;
+    const bool sort_due = (cnext % P.sort_step) == 0;
+    D.coll = (unsigned long long)(cnext / P.MPCtstep);
+    if (P.rng_mode == 0)
+      HIPCHK(hipMemcpyAsync(e->d_U, draws + c * e->ndraw, e->ndraw * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipEventRecord(e->n0, st));
+    launch_near(e, P.MPCtstep, sort_due ? M_STOP : 0);
+    HIPCHK(hipEventRecord(e->n1, st));
+    hipLaunchKernelGGL(fill_kernel<int>, dim3(grid_for(mc, 256)), dim3(256), 0, st, D.cnt, mc, 0);
+    hipLaunchKernelGGL(fill_kernel<int>, dim3(grid_for(3 * vol, 256)), dim3(256), 0, st, D.tcount, 3 * vol, 0);
+    hipLaunchKernelGGL(fill_kernel<unsigned>, dim3(grid_for(vol, 256)), dim3(256), 0, st, D.firstB, vol, 0xffffffffu);
+    if (sort_due) {
+      hipLaunchKernelGGL(co_bulk, dim3(gN), dim3(256), 0, st, P, D, 1, 0);
+      int r = co_sort(e);
+      if (r) return r;
+      co_near_list(e);
+      launch_near(e, 1, M_RESUME);
+      HIPCHK(hipEventRecord(e->b0, st));
+      hipLaunchKernelGGL(co_bulk, dim3(gN), dim3(256), 0, st, P, D, 0, 1);
+    } else {
+      HIPCHK(hipEventRecord(e->b0, st));
+      hipLaunchKernelGGL(co_bulk, dim3(gN), dim3(256), 0, st, P, D, 1, 1);
+    }
+    launch_scan(st, D.cnt, mc, D.partial, D.off);
+    hipLaunchKernelGGL(co_scatter, dim3(gN), dim3(256), 0, st, P, D);
+    hipLaunchKernelGGL(co_collide, dim3((unsigned)((mc + CT - 1) / CT)), dim3(CT), 0, st, P, D);
+    hipLaunchKernelGGL(fill_kernel<unsigned>, dim3(grid_for(e->nwords + 1, 256)), dim3(256), 0, st, D.near_bits,
+                       e->nwords + 1, 0u);
+    hipLaunchKernelGGL(reset_far, dim3(1), dim3(1), 0, st, D.sc);
+    hipLaunchKernelGGL(co_rotate, dim3(gN), dim3(256), 0, st, P, D, 1);
+    HIPCHK(hipEventRecord(e->b1, st));
+    co_near_list(e);
+    cstep = cnext;
+  }
+  {   // kernel-group times of the last interval
+    HIPCHK(hipEventSynchronize(e->b1));
+    float nm = 0, bm = 0;
+    (void)hipEventElapsedTime(&nm, e->n0, e->n1);
+    (void)hipEventElapsedTime(&bm, e->b0, e->b1);
+    e->near_ms = nm;
+    e->bulk_ms = bm;
+  }
+  HIPCHK(hipEventRecord(e->e1, st));
+  HIPCHK(hipEventSynchronize(e->e1));
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e->e0, e->e1);
+  e->run_ms = ms;
+  return check_device_error(e);
+}
+
+extern "C" int mpcx_co_get_state(mpcx_co* e, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                                 int32_t* flag, double* dimer12, int32_t* scalars) {
+  if (!e) return set_error(MPCX_E_ARG, "null engine");
+  HIPCHK(hipStreamSynchronize(e->st));
+  const size_t NT = (size_t)e->P.NT;
+  CoD& D = e->D;
+  if (x) HIPCHK(hipMemcpy(x, D.x, NT * 8, hipMemcpyDeviceToHost));
+  if (y) HIPCHK(hipMemcpy(y, D.y, NT * 8, hipMemcpyDeviceToHost));
+  if (z) HIPCHK(hipMemcpy(z, D.z, NT * 8, hipMemcpyDeviceToHost));
+  if (vx) HIPCHK(hipMemcpy(vx, D.vx, NT * 8, hipMemcpyDeviceToHost));
+  if (vy) HIPCHK(hipMemcpy(vy, D.vy, NT * 8, hipMemcpyDeviceToHost));
+  if (vz) HIPCHK(hipMemcpy(vz, D.vz, NT * 8, hipMemcpyDeviceToHost));
+  if (flag) HIPCHK(hipMemcpy(flag, D.flag, NT * 4, hipMemcpyDeviceToHost));
+  if (dimer12) {
+    std::vector<double> dim(12 * MAXS);
+    HIPCHK(hipMemcpy(dim.data(), D.dim, dim.size() * 8, hipMemcpyDeviceToHost));
+    const int ns = 2 * e->nd;
+    for (int f = 0; f < 12; f++)
+      for (int s = 0; s < ns; s++) dimer12[f * ns + s] = dim[f * MAXS + s];
+  }
+  if (scalars) {
+    CoScalars sc;
+    HIPCHK(hipMemcpy(&sc, D.sc, sizeof sc, hipMemcpyDeviceToHost));
+    scalars[0] = sc.cstep; scalars[1] = sc.time_count; scalars[2] = sc.time_penetrate; scalars[3] = sc.in_num;
+  }
+  return 0;
+}
+
+extern "C" int mpcx_co_get_inside(mpcx_co* e, int32_t* list, uint32_t* memb, double* fp, double* pot, int64_t cap,
+                                  int64_t* n) {
+  if (!e || !n) return set_error(MPCX_E_ARG, "null argument");
+  HIPCHK(hipStreamSynchronize(e->st));
+  CoScalars sc;
+  HIPCHK(hipMemcpy(&sc, e->D.sc, sizeof sc, hipMemcpyDeviceToHost));
+  *n = sc.in_num;
+  if (cap < sc.in_num) return set_error(MPCX_E_ARG, "buffer smaller than the inside list");
+  const size_t m = (size_t)sc.in_num, ic = (size_t)e->D.in_cap;
+  if (list) HIPCHK(hipMemcpy(list, e->D.in_list, m * 4, hipMemcpyDeviceToHost));
+  if (memb) HIPCHK(hipMemcpy(memb, e->D.in_memb, m * 4, hipMemcpyDeviceToHost));
+  if (fp)
+    for (int j = 0; j < MAXM * 3; j++)
+      HIPCHK(hipMemcpy(fp + j * cap, e->D.in_fp + j * ic, m * 8, hipMemcpyDeviceToHost));
+  if (pot)
+    for (int j = 0; j < MAXM; j++)
+      HIPCHK(hipMemcpy(pot + j * cap, e->D.in_pot + j * ic, m * 8, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int mpcx_co_get_cells(mpcx_co* e, double* vcm_uv, int32_t* cell_of, int32_t* census, double* tpotential) {
+  if (!e) return set_error(MPCX_E_ARG, "null engine");
+  HIPCHK(hipStreamSynchronize(e->st));
+  CoD& D = e->D;
+  if (vcm_uv) HIPCHK(hipMemcpy(vcm_uv, D.cellrec, (size_t)e->P.mpcnumcell * 64, hipMemcpyDeviceToHost));
+  if (cell_of) HIPCHK(hipMemcpy(cell_of, D.cell_of, (size_t)e->P.NT * 4, hipMemcpyDeviceToHost));
+  if (census) HIPCHK(hipMemcpy(census, D.tcount, (size_t)e->P.vol * 12, hipMemcpyDeviceToHost));
+  if (tpotential) HIPCHK(hipMemcpy(tpotential, &D.sc->tpotential, 8, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int mpcx_co_timing(mpcx_co* e, double* run_ms, double* bulk_ms, double* near_ms) {
+  if (!e) return set_error(MPCX_E_ARG, "null engine");
+  if (run_ms) *run_ms = e->run_ms;
+  if (bulk_ms) *bulk_ms = e->bulk_ms;
+  if (near_ms) *near_ms = e->near_ms;
+  return 0;
+}

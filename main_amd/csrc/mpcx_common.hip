@@ -1,0 +1,94 @@
+// Error reporting and device query of the C ABI.
+#include <hip/hip_runtime.h>
+#include <string>
+#include <stdint.h>
+#include "mpcx_host.h"
+#include "../../include/mpcx.h"
+
+namespace mpcx {
+static thread_local std::string g_last_error;
+int set_error(int code, const char* msg) {
+  g_last_error = msg ? msg : "";
+  return code;
+}
+}  // namespace mpcx
+
+extern "C" const char* mpcx_last_error(void) { return mpcx::g_last_error.c_str(); }
+
+extern "C" int mpcx_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+// ---------------------------------------------------------------------------
+// Host Mersenne Twister streams in the two flavours the simulators use.  They
+// produce the uniforms the parity mode uploads, in the reference's call order.
+// ---------------------------------------------------------------------------
+namespace {
+void mt_twist(uint32_t* mt) {
+  const uint32_t UM = 0x80000000u, LM = 0x7fffffffu, A = 0x9908b0dfu;
+  int kk;
+  uint32_t y;
+  for (kk = 0; kk < 624 - 397; kk++) {
+    y = (mt[kk] & UM) | (mt[kk + 1] & LM);
+    mt[kk] = mt[kk + 397] ^ (y >> 1) ^ ((y & 1u) ? A : 0u);
+  }
+  for (; kk < 623; kk++) {
+    y = (mt[kk] & UM) | (mt[kk + 1] & LM);
+    mt[kk] = mt[kk + (397 - 624)] ^ (y >> 1) ^ ((y & 1u) ? A : 0u);
+  }
+  y = (mt[623] & UM) | (mt[0] & LM);
+  mt[623] = mt[396] ^ (y >> 1) ^ ((y & 1u) ? A : 0u);
+}
+inline uint32_t mt_temper(uint32_t y) {
+  y ^= y >> 11;
+  y ^= (y << 7) & 0x9d2c5680u;
+  y ^= (y << 15) & 0xefc60000u;
+  y ^= y >> 18;
+  return y;
+}
+}  // namespace
+
+// sgrnd (collective_code/mtmod.f90:72-85)
+extern "C" int mpcx_mtmod_seed(int32_t seed, uint32_t* mt, int32_t* mti) {
+  if (!mt || !mti) return mpcx::set_error(MPCX_E_ARG, "null argument");
+  mt[0] = (uint32_t)seed;
+  for (int i = 1; i < 624; i++) mt[i] = 69069u * mt[i - 1];
+  *mti = 624;
+  return 0;
+}
+// n successive grnd() (mtmod.f90:88-144): u / (2^32 - 1)
+extern "C" int mpcx_mtmod_fill(uint32_t* mt, int32_t* mti, double* out, int64_t n) {
+  if (!mt || !mti || (!out && n > 0)) return mpcx::set_error(MPCX_E_ARG, "null argument");
+  int32_t i = *mti;
+  for (int64_t k = 0; k < n; k++) {
+    if (i >= 624) { mt_twist(mt); i = 0; }
+    out[k] = (double)mt_temper(mt[i++]) / 4294967295.0;
+  }
+  *mti = i;
+  return 0;
+}
+// init_by_scalar (selkov_code/mt95.f90:378-404); cnt is 1-based as in mt95
+extern "C" int mpcx_mt95_seed(int32_t seed, uint32_t* val, int32_t* cnt) {
+  if (!val || !cnt) return mpcx::set_error(MPCX_E_ARG, "null argument");
+  val[0] = (uint32_t)seed;
+  for (int i = 1; i < 624; i++) val[i] = 1812433253u * (val[i - 1] ^ (val[i - 1] >> 30)) + (uint32_t)i;
+  *cnt = 625;
+  return 0;
+}
+// n successive genrand_real1 (mt95.f90:828-841): real(a)*(1/(2^32-1)) + 2^31/(2^32-1)
+extern "C" int mpcx_mt95_fill(uint32_t* val, int32_t* cnt, double* out, int64_t n) {
+  if (!val || !cnt || (!out && n > 0)) return mpcx::set_error(MPCX_E_ARG, "null argument");
+  const double pi232_1 = 1.0 / 4294967295.0, p231d232_1 = 2147483648.0 / 4294967295.0;
+  int32_t c = *cnt;
+  for (int64_t k = 0; k < n; k++) {
+    if (c > 624) { mt_twist(val); c = 1; }
+    int32_t a = (int32_t)mt_temper(val[c - 1]);
+    c++;
+    volatile double m = (double)a * pi232_1;   // two roundings, as the reference
+    out[k] = m + p231d232_1;
+  }
+  *cnt = c;
+  return 0;
+}
