@@ -36,6 +36,8 @@ extern "C" {
 const char* mpcx_last_error(void);
 /* number of visible HIP devices (>0 when a GPU is usable) */
 int mpcx_device_count(void);
+/* hipDeviceSynchronize on the current device */
+int mpcx_device_synchronize(void);
 
 /* Host Mersenne Twister streams (parity mode draws).
  * mpcx_mtmod_*: sgrnd/grnd of collective_code/mtmod.f90:72-144 (state mt[624], mti).
@@ -157,6 +159,11 @@ int mpcx_co_get_cells(mpcx_co* e, double* vcm_uv /* (mpcnumcell, 8) */, int32_t*
 /* Wall time of the HIP events bracketing the last mpcx_co_run (ms), and of
  * the bulk kernels alone (ms, summed over the run). */
 int mpcx_co_timing(mpcx_co* e, double* run_ms, double* bulk_ms, double* near_ms);
+/* Near-engine counters accumulated since creation: real-time ticks (100 MHz)
+ * of [0] whole launches, [1] list rebuilds, [2] particle phases, [3] dimer
+ * (serial) sections; [4] rebuild count; [5] last near-set size; [6] last
+ * inside-list size; [7] launches. */
+int mpcx_co_near_profile(mpcx_co* e, uint64_t* prof8);
 
 #ifdef __cplusplus
 }

@@ -53,6 +53,10 @@ def device_count() -> int:
     return int(lib().mpcx_device_count())
 
 
+def device_synchronize():
+    check(lib().mpcx_device_synchronize(), "mpcx_device_synchronize")
+
+
 def require_gpu():
     if device_count() < 1:
         raise MpcxError("no HIP device visible: the mpcx engine has no CPU path")

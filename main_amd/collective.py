@@ -17,7 +17,7 @@ import numpy as np
 from ._lib import MtmodStream, check, lib, ptr, require_gpu
 
 MAXM = 4            # force slots stored per inside particle by the engine
-MAX_INSIDE = 4096   # register slots of the near engine
+MAX_INSIDE = 16384  # inside-list capacity of the near engine
 
 
 class CoConfig(C.Structure):
@@ -148,19 +148,16 @@ class CollectiveEngine:
         nbnum = np.zeros((nd, 2), np.int64, order="F")
         fpx = np.zeros((nd, 2, mnb), order="F")
         fpy, fpz, potn = fpx.copy(order="F"), fpx.copy(order="F"), fpx.copy(order="F")
-        mcount = np.zeros(n, np.int64)
-        for b in range(2 * nd):
+        for b in range(2 * nd):   # forces are stored by list b = 2k + i
             k, i = b >> 1, b & 1
             sel = np.nonzero((memb >> b) & 1)[0]
-            m = mcount[sel]
             cnt = sel.size
             nb[k, i, :cnt] = lst[sel] + 1
             nbnum[k, i] = cnt
-            fpx[k, i, :cnt] = fp[m, 0, sel]
-            fpy[k, i, :cnt] = fp[m, 1, sel]
-            fpz[k, i, :cnt] = fp[m, 2, sel]
-            potn[k, i, :cnt] = pot[m, sel]
-            mcount[sel] += 1
+            fpx[k, i, :cnt] = fp[b, 0, sel]
+            fpy[k, i, :cnt] = fp[b, 1, sel]
+            fpz[k, i, :cnt] = fp[b, 2, sel]
+            potn[k, i, :cnt] = pot[b, sel]
         inside = lst.astype(np.int64) + 1
         outside = np.setdiff1d(np.arange(1, NT + 1, dtype=np.int64), inside, assume_unique=True)
         return dict(in_list=inside, out_list=outside, nb=nb, nbnum=nbnum, fpx=fpx, fpy=fpy, fpz=fpz, pot=potn)
@@ -176,6 +173,15 @@ class CollectiveEngine:
                     uvy=rec[:, 4].copy(), uvz=rec[:, 5].copy(), tcs=rec[:, 6].copy(),
                     ncn=cell.astype(np.int64) + 1, tA=census[0].astype(np.int64), tB=census[1].astype(np.int64),
                     tS=census[2].astype(np.int64), tpotential=float(tp[0]))
+
+    def near_profile(self) -> dict:
+        a = np.zeros(8, np.uint64)
+        check(lib().mpcx_co_near_profile(self.h, ptr(a)), "mpcx_co_near_profile")
+        us = lambda v: float(v) / 100.0   # 100 MHz ticks -> microseconds
+        n = max(int(a[7]), 1)
+        return dict(launches=int(a[7]), total_us=us(a[0]) / n, rebuild_us=us(a[1]) / n,
+                    particle_us=us(a[2]) / n, serial_us=us(a[3]) / n, rebuilds=int(a[4]) / n,
+                    near_num=int(a[5]), inside_num=int(a[6]))
 
     def timing(self):
         r, b, n = C.c_double(), C.c_double(), C.c_double()

@@ -32,8 +32,8 @@
 
 namespace mpcx {
 
-constexpr int NE_T = 1024;   // near-engine threads
-constexpr int NE_K = 4;      // inside slots per thread (capacity 4096)
+constexpr int NE_T = 512;    // near-engine threads (one workgroup, 256 VGPRs per lane)
+constexpr int NE_CAP = 16384;  // inside-list capacity of the near engine
 constexpr int MAXD = 2;      // dimers the near engine handles
 constexpr int MAXS = 2 * MAXD;
 constexpr int MAXM = 4;      // neighbour lists one particle may belong to
@@ -57,6 +57,7 @@ struct CoScalars {
   double near_vmax2;
   double tpotential;
   double x0[MAXS], y0[MAXS], z0[MAXS];
+  unsigned long long prof[8];   // near engine: total, rebuild, particle phase, serial (100 MHz ticks), rebuilds
 };
 
 struct CoP {
@@ -87,7 +88,9 @@ struct CoD {
   unsigned* in_memb;
   double* in_fp;          // [(m*3+c)*in_cap + q]
   double* in_pot;         // [m*in_cap + q]
-  double* in_msd;         // max_solvent_dist of inside entries
+  double* in_msd;         // max_solvent_dist of inside entries (slot state below, by entry)
+  double *sl_x, *sl_y, *sl_z, *sl_vx, *sl_vy, *sl_vz;
+  int* sl_flag;
   int in_cap, near_cap;
   int* cell_of;
   unsigned short* rank_of;
@@ -95,7 +98,10 @@ struct CoD {
   int* off;
   int* bucket;
   int* partial;
-  double* cellrec;        // [mpcnumcell][8]: vcm x y z, uv x y z, n, 0
+  double* cellrec;        // [mpcnumcell][8] by tour position: vcm x y z, uv x y z, n, 0
+  const int* mpc_pos;     // tour position of each collision cell
+  const int* mpc_cell;    // collision cell at each tour position
+  unsigned char* dq;      // distance to the nearest neighbour-list sphere, 1/32 units, capped
   int* tcount;            // [3][vol]
   unsigned* firstB;       // [vol]
   const double* U;        // host draws of this collision (rng_mode 0)
@@ -328,33 +334,85 @@ __device__ __attribute__((noinline)) void dimer_forces_and_rattle_v(const CoP& P
 }
 
 // ------------------------------------------------------------- near engine
+constexpr int LCAP = 2304;   // inside-list entries staged in LDS (the rest stay in HBM)
 struct NearShared {
-  double msd[NE_T * NE_K];     // max_solvent_dist of the slots
-  int flag[NE_T * NE_K];
-  unsigned memb[NE_T * NE_K];
+  double lx[LCAP], ly[LCAP], lz[LCAP], lvx[LCAP], lvy[LCAP], lvz[LCAP], lmsd[LCAP];
+  int lflag[LCAP];
+  unsigned lmemb[LCAP];
   double sd[12][MAXS];
   double red[NE_T / 64][4 * MAXS + 1];
-  double sumF[MAXS][4];
+  double fin[4 * MAXS + 1];
   unsigned long long scan[NE_T / 64 + 1];
   double vmax2;
   int cstep, tc, tp, in_num, nev, near_num, rebuild, stop;
 };
 
-// Ordered rebuild of the inside list from the near list (neighbour_list_mod.f95:63-177
-// restricted to the near set; every other particle is outside by construction).
+// Membership pass over the near list (strided, coalesced): the lazy streaming
+// of the near particles outside the lists (evolve_mod.f95:467-529) when
+// do_stream, then the neighbour-list membership of every near particle at its
+// new position (neighbour_list_mod.f95:38-57, 129-167).  Also max v^2 over the
+// near set (evolve_mod.f95:555-562).
 template <int ND>
-__device__ __attribute__((noinline)) void ne_classify(const CoP& P, const CoD& D, NearShared& S) {
+__device__ __attribute__((noinline)) void ne_memb_pass(const CoP& P, const CoD& D, NearShared& S, bool do_stream,
+                                                       bool do_memb) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int nn = S.near_num;
+  const double tc = (double)S.tc;
+  double vm = 0.0;
+  for (int q0 = tid; q0 < nn; q0 += 2 * NE_T) {
+    int qq[2] = {q0, q0 + NE_T};
+    int pp[2];
+    double x[2], y[2], z[2], vx[2], vy[2], vz[2];
+    int f[2];
+    unsigned char st[2];
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      pp[u] = qq[u] < nn ? D.near_list[qq[u]] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      if (pp[u] < 0) continue;
+      int p = pp[u];
+      x[u] = D.x[p]; y[u] = D.y[p]; z[u] = D.z[p];
+      vx[u] = D.vx[p]; vy[u] = D.vy[p]; vz[u] = D.vz[p];
+      f[u] = D.flag[p];
+      st[u] = D.status[p];
+    }
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      if (pp[u] < 0) continue;
+      int p = pp[u];
+      if (do_stream && !(st[u] & ST_INSIDE)) {
+        stream_particle(x[u], y[u], z[u], vx[u], vy[u], vz[u], f[u], tc, P, D.react_box, &D.sc->err);
+        D.x[p] = x[u]; D.y[p] = y[u]; D.z[p] = z[u];
+        D.vx[p] = vx[u]; D.vy[p] = vy[u]; D.vz[p] = vz[u];
+        D.flag[p] = f[u];
+      }
+      vm = fmax(vm, (vx[u] * vx[u] + vy[u] * vy[u]) + vz[u] * vz[u]);
+      if (do_memb) D.near_memb[qq[u]] = (int)memberships<ND>(x[u], y[u], z[u], P, S.sd);
+    }
+  }
+  vm = wave_max(vm);
+  if (lane == 0) S.red[wv][0] = vm;
+  __syncthreads();
+  if (tid == 0) {
+    double m = 0.0;
+    for (int w = 0; w < NE_T / 64; w++) m = fmax(m, S.red[w][0]);
+    if (do_stream) S.vmax2 = m;
+  }
+  __syncthreads();
+}
+
+// Ordered compaction of the memberships into the inside list (ascending
+// particle order, as inout_particle_list builds it at one thread), status
+// bits, and time_penetrate (evolve_mod.f95:563).
+__device__ __attribute__((noinline)) void ne_compact(const CoP& P, const CoD& D, NearShared& S) {
   const int tid = threadIdx.x;
   const int nn = S.near_num;
   const int E = (nn + NE_T - 1) / NE_T;
   const int q0 = min(tid * E, nn), q1 = min(q0 + E, nn);
   unsigned long long cnt = 0;
-  for (int q = q0; q < q1; q++) {
-    int p = D.near_list[q];
-    unsigned m = memberships<ND>(D.x[p], D.y[p], D.z[p], P, S.sd);
-    D.near_memb[q] = (int)m;
-    cnt += m != 0;
-  }
+  for (int q = q0; q < q1; q++) cnt += D.near_memb[q] != 0;
   unsigned long long total;
   unsigned long long off = block_excl_scan64(cnt, S.scan, total);
   if (total > (unsigned long long)D.in_cap) {
@@ -379,7 +437,6 @@ __device__ __attribute__((noinline)) void ne_classify(const CoP& P, const CoD& D
   }
   if (tid == 0) {
     S.in_num = (int)total;
-    // time_penetrate from the maximum of v^2 over all particles (evolve_mod.f95:555-563)
     double fm = __longlong_as_double((long long)D.sc->far_maxv2);
     double maxt = fmax(fm, S.vmax2);
     S.tp = (int)floor(((0.5 * P.skinrad) / sqrt(maxt)) / P.MDt);
@@ -388,8 +445,83 @@ __device__ __attribute__((noinline)) void ne_classify(const CoP& P, const CoD& D
   __syncthreads();
 }
 
-// write inside slots back to HBM
-#define NE_SLOT_LOOP for (int k = 0; k < NE_K; k++)
+// thread 0: bookkeeping of the next MD step, RATTLE positions and the
+// rebuild test of evolve_mod.f95:457-465
+template <int ND>
+__device__ __attribute__((noinline)) void ne_begin_step(const CoP& P, const CoD& D, NearShared& S, double maxv) {
+  S.cstep += 1;
+  S.tc += 1;
+  rattle_positions<ND>(P, S.sd, &D.sc->err);
+  S.rebuild = (maxv > P.hs2 || S.tc == S.tp || S.cstep % P.MPCtstep == 0);
+  if (S.rebuild) {
+    if (S.nev < MAXEV) D.sc->events[S.nev] = S.tc; else raise_error(&D.sc->err, ERR_EVENTS);
+    S.nev += 1;
+    for (int s = 0; s < 2 * ND; s++) {   // the near/far split needs every sphere inside its guard
+      double a = pbc_half(S.sd[DX][s] - D.sc->x0[s], P.box[0]);
+      double b = pbc_half(S.sd[DY][s] - D.sc->y0[s], P.box[1]);
+      double c = S.sd[DZ][s] - D.sc->z0[s];
+      if (a * a + b * b + c * c > P.guard * P.guard) raise_error(&D.sc->err, ERR_NEAR_GUARD);
+    }
+  }
+}
+
+// thread 0, end of step t: dimer forces, velocity RATTLE and, unless t is the
+// last step of the launch, the start of step t+1
+template <int ND>
+__device__ __attribute__((noinline)) void ne_end_step(const CoP& P, const CoD& D, NearShared& S, bool last) {
+  double tpot;
+  dimer_forces_and_rattle_v<ND>(P, S.sd, (const double(*)[4])S.fin, tpot, &D.sc->err);
+  D.sc->tpotential = tpot;
+  if (!last) ne_begin_step<ND>(P, D, S, S.fin[4 * 2 * ND]);
+}
+
+// Inside-list entry q lives in LDS when q < LCAP, else in the HBM slot arrays;
+// Slot holds generic pointers to its fields.
+struct Slot {
+  double *x, *y, *z, *vx, *vy, *vz, *msd;
+  int* flag;
+  unsigned* memb;
+};
+__device__ __forceinline__ Slot slot_at(const CoD& D, NearShared& S, int q) {
+  Slot r;
+  if (q < LCAP) {
+    r.x = &S.lx[q]; r.y = &S.ly[q]; r.z = &S.lz[q]; r.vx = &S.lvx[q]; r.vy = &S.lvy[q]; r.vz = &S.lvz[q];
+    r.msd = &S.lmsd[q]; r.flag = &S.lflag[q]; r.memb = &S.lmemb[q];
+  } else {
+    r.x = &D.sl_x[q]; r.y = &D.sl_y[q]; r.z = &D.sl_z[q]; r.vx = &D.sl_vx[q]; r.vy = &D.sl_vy[q];
+    r.vz = &D.sl_vz[q]; r.msd = &D.in_msd[q]; r.flag = &D.sl_flag[q]; r.memb = &D.in_memb[q];
+  }
+  return r;
+}
+// gather the inside entries from the particle arrays (msd is reset by the caller)
+__device__ __forceinline__ void ne_load_slots(const CoD& D, NearShared& S, int n) {
+  for (int q = threadIdx.x; q < n; q += NE_T) {
+    int p = D.in_list[q];
+    Slot r = slot_at(D, S, q);
+    *r.x = D.x[p]; *r.y = D.y[p]; *r.z = D.z[p];
+    *r.vx = D.vx[p]; *r.vy = D.vy[p]; *r.vz = D.vz[p];
+    *r.flag = D.flag[p];
+    if (q < LCAP) *r.memb = D.in_memb[q];
+  }
+}
+__device__ __forceinline__ void ne_store_slots(const CoD& D, NearShared& S, int n) {
+  for (int q = threadIdx.x; q < n; q += NE_T) {
+    int p = D.in_list[q];
+    Slot r = slot_at(D, S, q);
+    D.x[p] = *r.x; D.y[p] = *r.y; D.z[p] = *r.z;
+    D.vx[p] = *r.vx; D.vy[p] = *r.vy; D.vz[p] = *r.vz;
+    D.flag[p] = *r.flag;
+  }
+}
+// max_solvent_dist of the entries: to and from the persistent per-entry array
+__device__ __forceinline__ void ne_msd_io(const CoD& D, NearShared& S, int n, bool load) {
+  for (int q = threadIdx.x; q < n && q < LCAP; q += NE_T) {
+    if (load) S.lmsd[q] = D.in_msd[q]; else D.in_msd[q] = S.lmsd[q];
+  }
+}
+__device__ __forceinline__ void ne_msd_zero(const CoD& D, NearShared& S, int n) {
+  for (int q = threadIdx.x; q < n; q += NE_T) *slot_at(D, S, q).msd = 0.0;
+}
 
 template <int ND>
 __global__ __launch_bounds__(NE_T) void co_near_engine(CoP P, CoD D, int nsteps, int mode) {
@@ -398,6 +530,7 @@ __global__ __launch_bounds__(NE_T) void co_near_engine(CoP P, CoD D, int nsteps,
   constexpr int NSPH = 2 * ND;
   constexpr int NV = 4 * NSPH + 1;
   CoScalars* sc = D.sc;
+  const int cap = D.in_cap;
 
   for (int j = tid; j < 12 * MAXS; j += NE_T) S.sd[j / MAXS][j % MAXS] = D.dim[j];
   if (tid == 0) {
@@ -412,151 +545,97 @@ __global__ __launch_bounds__(NE_T) void co_near_engine(CoP P, CoD D, int nsteps,
 
   if (mode & M_SETUP) {
     // main_pattern.f95:173-188 over the near set; near_vmax2 was set by the classifier kernel
-    ne_classify<ND>(P, D, S);
+    ne_memb_pass<ND>(P, D, S, false, true);
+    ne_compact(P, D, S);
     if (tid == 0) {
       sc->in_num = S.in_num; sc->time_count = S.tc; sc->time_penetrate = S.tp; sc->n_ev = 0;
       for (int s = 0; s < NSPH; s++) { sc->x0[s] = S.sd[DX][s]; sc->y0[s] = S.sd[DY][s]; sc->z0[s] = S.sd[DZ][s]; }
     }
     for (int q = tid; q < S.in_num; q += NE_T) {
-      for (int j = 0; j < MAXM * 3; j++) D.in_fp[j * D.in_cap + q] = 0.0;
-      for (int j = 0; j < MAXM; j++) D.in_pot[j * D.in_cap + q] = 0.0;
+      for (int j = 0; j < MAXM * 3; j++) D.in_fp[j * cap + q] = 0.0;
+      for (int j = 0; j < MAXM; j++) D.in_pot[j * cap + q] = 0.0;
       D.in_msd[q] = 0.0;
     }
     return;
   }
 
-  constexpr int MM = 2 * ND;    // lists one particle can belong to
-  int qi[NE_K];
-  double px[NE_K], py[NE_K], pz[NE_K], pvx[NE_K], pvy[NE_K], pvz[NE_K];
-  double* msd = S.msd + tid;     // msd[k * NE_T] is slot k of this thread
-  int* pf = S.flag + tid;
-  unsigned* pm = S.memb + tid;
-#define SL(k) ((k) * NE_T)
+  // kick (evolve_mod.f95:225-250 and :645-670) in list order, then inside
+  // streaming (:252-316) of one entry
+#define NE_KICK(vx, vy, vz, fl, mb, fx, fy, fz)                  \
+  {                                                              \
+    bool ok_;                                                    \
+    double h_ = kick_factor(P, fl, ok_);                         \
+    if (ok_) {                                                   \
+      _Pragma("unroll") for (int b_ = 0; b_ < NSPH; b_++)        \
+        if (((mb) >> b_) & 1u) {                                 \
+          vx = vx + h_ * fx[b_];                                 \
+          vy = vy + h_ * fy[b_];                                 \
+          vz = vz + h_ * fz[b_];                                 \
+        }                                                        \
+    }                                                            \
+  }
+#define NE_STREAM(x, y, z, vx, vy, vz, fl, ms)                                              \
+  {                                                                                        \
+    stream_particle(x, y, z, vx, vy, vz, fl, 1.0, P, D.react_box, &sc->err);               \
+    double v2_ = (vx * vx + vy * vy) + vz * vz;                                            \
+    ms = ms + v2_ * (P.MDt * P.MDt);                                                       \
+  }
 
-  auto load_slots = [&]() {
-    NE_SLOT_LOOP {
-      int q = tid + k * NE_T;
-      qi[k] = q < S.in_num ? D.in_list[q] : -1;
-      if (qi[k] >= 0) {
-        int p = qi[k];
-        px[k] = D.x[p]; py[k] = D.y[p]; pz[k] = D.z[p];
-        pvx[k] = D.vx[p]; pvy[k] = D.vy[p]; pvz[k] = D.vz[p];
-        pf[SL(k)] = D.flag[p]; pm[SL(k)] = D.in_memb[q];
-      } else {
-        px[k] = py[k] = pz[k] = pvx[k] = pvy[k] = pvz[k] = 0.0; pf[SL(k)] = -1; pm[SL(k)] = 0;
-      }
-      msd[SL(k)] = 0.0;
-    }
-  };
-  auto store_slots = [&]() {
-    NE_SLOT_LOOP {
-      if (qi[k] >= 0) {
-        int p = qi[k];
-        D.x[p] = px[k]; D.y[p] = py[k]; D.z[p] = pz[k];
-        D.vx[p] = pvx[k]; D.vy[p] = pvy[k]; D.vz[p] = pvz[k];
-        D.flag[p] = pf[SL(k)];
-      }
-    }
-  };
-  // kick (evolve_mod.f95:225-250) followed by inside streaming (:252-316)
-  auto kick_stream = [&](int k, const double* fx, const double* fy, const double* fz, int nm) {
-    bool ok;
-    double h = kick_factor(P, pf[SL(k)], ok);
-    if (ok)
-      for (int m = 0; m < nm; m++) {
-        pvx[k] = pvx[k] + h * fx[m];
-        pvy[k] = pvy[k] + h * fy[m];
-        pvz[k] = pvz[k] + h * fz[m];
-      }
-    stream_particle(px[k], py[k], pz[k], pvx[k], pvy[k], pvz[k], pf[SL(k)], 1.0, P, D.react_box, &sc->err);
-    double v2 = (pvx[k] * pvx[k] + pvy[k] * pvy[k]) + pvz[k] * pvz[k];
-    msd[SL(k)] = msd[SL(k)] + v2 * (P.MDt * P.MDt);
-  };
-
-  bool resume = (mode & M_RESUME) != 0;
+  const bool resume = (mode & M_RESUME) != 0;
+  unsigned long long T0 = __builtin_amdgcn_s_memrealtime(), Tr = 0, Tp = 0, Ts = 0, Nr = 0, ta, tb;
   if (!resume) {
     // status of this interval's near set: only the inside list is inside
     for (int q = tid; q < S.near_num; q += NE_T) D.status[D.near_list[q]] = 0;
     __syncthreads();
     for (int q = tid; q < S.in_num; q += NE_T) D.status[D.in_list[q]] = ST_INSIDE;
-    load_slots();
-    // first step: kick with the forces stored at the end of the previous interval
-    NE_SLOT_LOOP {
-      if (qi[k] < 0) continue;
-      int q = tid + k * NE_T;
-      double fx[MM], fy[MM], fz[MM];
-      int nm = __popc(pm[SL(k)]);
-      for (int m = 0; m < nm && m < MM; m++) {
-        fx[m] = D.in_fp[(m * 3 + 0) * D.in_cap + q];
-        fy[m] = D.in_fp[(m * 3 + 1) * D.in_cap + q];
-        fz[m] = D.in_fp[(m * 3 + 2) * D.in_cap + q];
-      }
-      msd[SL(k)] = D.in_msd[q];
-      kick_stream(k, fx, fy, fz, nm);
-    }
-    // reduce max_solvent_dist for the first rebuild test
+    ne_load_slots(D, S, S.in_num);
+    ne_msd_io(D, S, S.in_num, true);
+    // first step: kick with the forces stored at the end of the previous interval, stream
     double mx = 0.0;
-    NE_SLOT_LOOP mx = fmax(mx, msd[SL(k)]);
+    for (int q = tid; q < S.in_num; q += NE_T) {
+      Slot r = slot_at(D, S, q);
+      double x = *r.x, y = *r.y, z = *r.z, vx = *r.vx, vy = *r.vy, vz = *r.vz;
+      int fl = *r.flag;
+      double ms = *r.msd;
+      double fx[NSPH], fy[NSPH], fz[NSPH];
+      const unsigned mb = D.in_memb[q];
+#pragma unroll
+      for (int b = 0; b < NSPH; b++) {
+        fx[b] = D.in_fp[(b * 3 + 0) * cap + q];
+        fy[b] = D.in_fp[(b * 3 + 1) * cap + q];
+        fz[b] = D.in_fp[(b * 3 + 2) * cap + q];
+      }
+      NE_KICK(vx, vy, vz, fl, mb, fx, fy, fz);
+      NE_STREAM(x, y, z, vx, vy, vz, fl, ms);
+      *r.x = x; *r.y = y; *r.z = z; *r.vx = vx; *r.vy = vy; *r.vz = vz;
+      *r.flag = fl; *r.msd = ms;
+      mx = fmax(mx, ms);
+    }
     mx = wave_max(mx);
     if (lane == 0) S.red[wv][0] = mx;
     __syncthreads();
     if (tid == 0) {
       double m = 0.0;
       for (int w = 0; w < NE_T / 64; w++) m = fmax(m, S.red[w][0]);
-      S.red[0][NV - 1] = m;
+      ne_begin_step<ND>(P, D, S, m);
     }
     __syncthreads();
+  } else {
+    S.rebuild = 1;   // resuming inside the rebuild of the final step
   }
 
   for (int t = 1; t <= nsteps; t++) {
-    // ---- thread 0: RATTLE positions, bookkeeping, rebuild test (evolve_mod.f95:318-465)
-    if (!(resume && t == nsteps)) {
-      if (tid == 0) {
-        double maxv = S.red[0][NV - 1];
-        S.cstep += 1;
-        S.tc += 1;
-        rattle_positions<ND>(P, S.sd, &sc->err);
-        S.rebuild = (maxv > P.hs2 || S.tc == S.tp || S.cstep % P.MPCtstep == 0);
-      }
-      __syncthreads();
-      if (S.rebuild) {
-        store_slots();
-        NE_SLOT_LOOP msd[SL(k)] = 0.0;
-        if (tid == 0) {
-          if (S.nev < MAXEV) sc->events[S.nev] = S.tc; else raise_error(&sc->err, ERR_EVENTS);
-          S.nev += 1;
-          for (int s = 0; s < NSPH; s++) {   // the near/far split needs every sphere inside its guard
-            double a = pbc_half(S.sd[DX][s] - sc->x0[s], P.box[0]);
-            double b = pbc_half(S.sd[DY][s] - sc->y0[s], P.box[1]);
-            double c = S.sd[DZ][s] - sc->z0[s];
-            if (a * a + b * b + c * c > P.guard * P.guard) raise_error(&sc->err, ERR_NEAR_GUARD);
-          }
-        }
+    const bool last = (t == nsteps);
+    // ---- neighbour-list rebuild of step t (evolve_mod.f95:465-564) over the near set
+    ta = __builtin_amdgcn_s_memrealtime();
+    if (S.rebuild) {
+      Nr++;
+      if (!(resume && last)) {
+        ne_store_slots(D, S, S.in_num);
         __syncthreads();
-        // lazy streaming of the near particles outside the lists (evolve_mod.f95:467-529)
-        double vm = 0.0, tc = (double)S.tc;
-        for (int q = tid; q < S.near_num; q += NE_T) {
-          int p = D.near_list[q];
-          double vx = D.vx[p], vy = D.vy[p], vz = D.vz[p];
-          if (!(D.status[p] & ST_INSIDE)) {
-            double x = D.x[p], y = D.y[p], z = D.z[p];
-            int f = D.flag[p];
-            stream_particle(x, y, z, vx, vy, vz, f, tc, P, D.react_box, &sc->err);
-            D.x[p] = x; D.y[p] = y; D.z[p] = z; D.vx[p] = vx; D.vy[p] = vy; D.vz[p] = vz; D.flag[p] = f;
-          }
-          vm = fmax(vm, (vx * vx + vy * vy) + vz * vz);
-        }
-        vm = wave_max(vm);
-        if (lane == 0) S.red[wv][0] = vm;
-        __syncthreads();
-        if (tid == 0) {
-          double m = 0.0;
-          for (int w = 0; w < NE_T / 64; w++) m = fmax(m, S.red[w][0]);
-          S.vmax2 = m;
-          if ((mode & M_STOP) && t == nsteps) S.stop = 1;
-        }
-        __syncthreads();
-        if (S.stop) {   // the host sorts, then resumes with M_RESUME
+        const bool stop = (mode & M_STOP) && last;
+        ne_memb_pass<ND>(P, D, S, true, !stop);
+        if (stop) {   // the host sorts, then resumes with M_RESUME
           if (tid == 0) {
             sc->cstep = S.cstep; sc->time_count = S.tc; sc->time_penetrate = S.tp; sc->n_ev = S.nev;
             sc->near_vmax2 = S.vmax2;
@@ -564,77 +643,77 @@ __global__ __launch_bounds__(NE_T) void co_near_engine(CoP P, CoD D, int nsteps,
           for (int j = tid; j < 12 * MAXS; j += NE_T) D.dim[j] = S.sd[j / MAXS][j % MAXS];
           return;
         }
+      } else {
+        ne_memb_pass<ND>(P, D, S, false, true);
       }
+      ne_compact(P, D, S);
+      ne_load_slots(D, S, S.in_num);
+      ne_msd_zero(D, S, S.in_num);
+      __syncthreads();
     }
-    if (S.rebuild || (resume && t == nsteps)) {
-      ne_classify<ND>(P, D, S);
-      load_slots();
-    }
+    tb = __builtin_amdgcn_s_memrealtime();
+    Tr += tb - ta;
 
-    // ---- particle phase: forces (forces_mod.f95:62-136), kick, and the next step's kick + stream
+    // ---- particle phase: forces (forces_mod.f95:62-136), kick, next step's kick + stream
     double acc[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) acc[v] = 0.0;
-    const bool last = (t == nsteps);
-    NE_SLOT_LOOP {
-      if (qi[k] < 0) continue;
-      double fx[MM], fy[MM], fz[MM], pot[MM];
-      int nm = 0;
-      unsigned bits = pm[SL(k)];
-      while (bits && nm < MM) {
-        int b = __ffs(bits) - 1;
-        bits &= bits - 1;
-        int kk = b >> 1, ii = b & 1, s = kk + ii * ND;
-        double d1 = pbc_half(px[k] - S.sd[DX][s], P.box[0]);
-        double d2 = pbc_half(py[k] - S.sd[DY][s], P.box[1]);
-        double d3 = pz[k] - S.sd[DZ][s];
-        double dRsq = d1 * d1 + d2 * d2 + d3 * d3;
-        double F = 0.0, E = 0.0, Ub = 0.0;
-        bool on = false;
-        if (dRsq <= P.cut2[ii]) {
-          if (pf[SL(k)] == 0) {
-            if (ii == 0) { pf[SL(k)] = 1; E = P.Eb[0]; } else E = P.Ea[1];
-            on = true;
-          } else if (pf[SL(k)] == 1) { E = P.Eb[ii]; on = true; }
-          else if (pf[SL(k)] == 2) { E = P.Es[ii]; on = true; }
-          if (on) lj_terms(dRsq, P.R6[ii], P.R12[ii], P.r12R12[ii], P.r6R6[ii], F, Ub);
-        }
-        double EF = E * F;
-        fx[nm] = on ? EF * d1 : 0.0;
-        fy[nm] = on ? EF * d2 : 0.0;
-        fz[nm] = on ? EF * d3 : 0.0;
-        pot[nm] = on ? E * Ub : 0.0;
+    for (int q = tid; q < S.in_num; q += NE_T) {
+      Slot r = slot_at(D, S, q);
+      double x = *r.x, y = *r.y, z = *r.z, vx = *r.vx, vy = *r.vy, vz = *r.vz;
+      int fl = *r.flag;
+      const unsigned mb = *r.memb;
+      double ms = *r.msd;
+      double fx[NSPH], fy[NSPH], fz[NSPH], pot[NSPH];
+      // lists in the reference's order: (dimer k, sphere i) -> b = 2k + i
 #pragma unroll
-        for (int ss = 0; ss < NSPH; ss++)
-          if (ss == s) {
-            acc[4 * ss + 0] += fx[nm]; acc[4 * ss + 1] += fy[nm];
-            acc[4 * ss + 2] += fz[nm]; acc[4 * ss + 3] += pot[nm];
+      for (int b = 0; b < NSPH; b++) {
+        const int kk = b >> 1, ii = b & 1, s = kk + ii * ND;
+        fx[b] = fy[b] = fz[b] = pot[b] = 0.0;
+        if (!((mb >> b) & 1u)) continue;
+        double d1 = pbc_half(x - S.sd[DX][s], P.box[0]);
+        double d2 = pbc_half(y - S.sd[DY][s], P.box[1]);
+        double d3 = z - S.sd[DZ][s];
+        double dRsq = d1 * d1 + d2 * d2 + d3 * d3;
+        if (dRsq <= P.cut2[ii]) {
+          double E = 0.0;
+          bool on = false;
+          if (fl == 0) {
+            if (ii == 0) { fl = 1; E = P.Eb[0]; } else E = P.Ea[1];
+            on = true;
+          } else if (fl == 1) { E = P.Eb[ii]; on = true; }
+          else if (fl == 2) { E = P.Es[ii]; on = true; }
+          if (on) {
+            double F, Ub;
+            lj_terms(dRsq, P.R6[ii], P.R12[ii], P.r12R12[ii], P.r6R6[ii], F, Ub);
+            double EF = E * F;
+            fx[b] = EF * d1; fy[b] = EF * d2; fz[b] = EF * d3; pot[b] = E * Ub;
           }
-        nm++;
-      }
-      // kick of evolve_mod.f95:645-670 with the flags after every flip of this step
-      bool ok;
-      double h = kick_factor(P, pf[SL(k)], ok);
-      if (ok)
-        for (int m = 0; m < nm; m++) {
-          pvx[k] = pvx[k] + h * fx[m];
-          pvy[k] = pvy[k] + h * fy[m];
-          pvz[k] = pvz[k] + h * fz[m];
         }
+        acc[4 * s + 0] += fx[b]; acc[4 * s + 1] += fy[b];
+        acc[4 * s + 2] += fz[b]; acc[4 * s + 3] += pot[b];
+      }
+      // kick of evolve_mod.f95:645-670 with the flags after every flip of this
+      // step; unless this is the launch's last step, the next step's kick and
+      // streaming follow at once (they do not depend on the dimer)
+      NE_KICK(vx, vy, vz, fl, mb, fx, fy, fz);
       if (!last) {
-        kick_stream(k, fx, fy, fz, nm);
-        acc[NV - 1] = fmax(acc[NV - 1], msd[SL(k)]);
+        NE_KICK(vx, vy, vz, fl, mb, fx, fy, fz);
+        NE_STREAM(x, y, z, vx, vy, vz, fl, ms);
+        acc[NV - 1] = fmax(acc[NV - 1], ms);
       } else {
-        int q = tid + k * NE_T;
-        for (int m = 0; m < nm; m++) {
-          D.in_fp[(m * 3 + 0) * D.in_cap + q] = fx[m];
-          D.in_fp[(m * 3 + 1) * D.in_cap + q] = fy[m];
-          D.in_fp[(m * 3 + 2) * D.in_cap + q] = fz[m];
-          D.in_pot[m * D.in_cap + q] = pot[m];
+#pragma unroll
+        for (int b = 0; b < NSPH; b++) {
+          D.in_fp[(b * 3 + 0) * cap + q] = fx[b];
+          D.in_fp[(b * 3 + 1) * cap + q] = fy[b];
+          D.in_fp[(b * 3 + 2) * cap + q] = fz[b];
+          D.in_pot[b * cap + q] = pot[b];
         }
       }
+      *r.x = x; *r.y = y; *r.z = z; *r.vx = vx; *r.vy = vy; *r.vz = vz;
+      *r.flag = fl; *r.msd = ms;
     }
-    // block reduction (fixed order): sphere sums and max_solvent_dist
+    // ---- block reduction in a fixed order: sphere sums and max_solvent_dist
 #pragma unroll
     for (int v = 0; v < NV - 1; v++) {
       double r = wave_sum(acc[v]);
@@ -645,46 +724,48 @@ __global__ __launch_bounds__(NE_T) void co_near_engine(CoP P, CoD D, int nsteps,
       if (lane == 0) S.red[wv][NV - 1] = r;
     }
     __syncthreads();
-    if (tid == 0) {
-      for (int v = 0; v < NV; v++) {
-        double a = S.red[0][v];
-        for (int w = 1; w < NE_T / 64; w++) a = (v == NV - 1) ? fmax(a, S.red[w][v]) : a + S.red[w][v];
-        S.red[0][v] = a;
-      }
-      for (int s = 0; s < NSPH; s++)
-        for (int c = 0; c < 4; c++) S.sumF[s][c] = S.red[0][4 * s + c];
-      double tpot;
-      dimer_forces_and_rattle_v<ND>(P, S.sd, S.sumF, tpot, &sc->err);
-      sc->tpotential = tpot;
+    if (tid < NV) {
+      double a = S.red[0][tid];
+      for (int w = 1; w < NE_T / 64; w++) a = (tid == NV - 1) ? fmax(a, S.red[w][tid]) : a + S.red[w][tid];
+      S.fin[tid] = a;
     }
     __syncthreads();
+    ta = __builtin_amdgcn_s_memrealtime();
+    Tp += ta - tb;
+    if (tid == 0) ne_end_step<ND>(P, D, S, last);
+    __syncthreads();
+    Ts += __builtin_amdgcn_s_memrealtime() - ta;
   }
 
   // ---- end of the interval: census exclusion (distribute_mod.f95:149-158), write back
-  NE_SLOT_LOOP {
-    if (qi[k] < 0) continue;
+  for (int q = tid; q < S.in_num; q += NE_T) {
+    Slot r = slot_at(D, S, q);
+    double x = *r.x, y = *r.y, z = *r.z;
     bool excl = false;
     for (int kk = 0; kk < ND; kk++)
       for (int ii = 0; ii < 2; ii++) {
         int s = kk + ii * ND;
-        double d1 = pbc_half(px[k] - S.sd[DX][s], P.box[0]);
-        double d2 = pbc_half(py[k] - S.sd[DY][s], P.box[1]);
-        double d3 = pz[k] - S.sd[DZ][s];
+        double d1 = pbc_half(x - S.sd[DX][s], P.box[0]);
+        double d2 = pbc_half(y - S.sd[DY][s], P.box[1]);
+        double d3 = z - S.sd[DZ][s];
         if (d1 * d1 + d2 * d2 + d3 * d3 <= P.cut2[ii]) excl = true;
       }
-    D.status[qi[k]] = ST_INSIDE | (excl ? ST_EXCL : 0);
-    D.in_msd[tid + k * NE_T] = msd[SL(k)];
+    D.status[D.in_list[q]] = ST_INSIDE | (excl ? ST_EXCL : 0);
   }
-  store_slots();
+  ne_store_slots(D, S, S.in_num);
+  ne_msd_io(D, S, S.in_num, false);
   if (tid == 0) {
+    sc->prof[0] += __builtin_amdgcn_s_memrealtime() - T0;
+    sc->prof[1] += Tr; sc->prof[2] += Tp; sc->prof[3] += Ts; sc->prof[4] += Nr;
+    sc->prof[5] = S.near_num; sc->prof[6] = S.in_num; sc->prof[7] += 1;
     sc->cstep = S.cstep; sc->time_count = S.tc; sc->time_penetrate = S.tp; sc->in_num = S.in_num;
     sc->n_ev = S.nev;
     for (int s = 0; s < NSPH; s++) { sc->x0[s] = S.sd[DX][s]; sc->y0[s] = S.sd[DY][s]; sc->z0[s] = S.sd[DZ][s]; }
   }
   for (int j = tid; j < 12 * MAXS; j += NE_T) D.dim[j] = S.sd[j / MAXS][j % MAXS];
 }
-
-#undef SL
+#undef NE_KICK
+#undef NE_STREAM
 
 // ------------------------------------------------------------- bulk kernels
 __device__ __forceinline__ void collision_shift(const CoP& P, const CoD& D, double& xs, double& ys, double& zs) {
@@ -716,57 +797,136 @@ __device__ __forceinline__ long long shifted_cell(const CoP& P, double x, double
   return (long long)c - 1;
 }
 
+// Adds the number of active lanes holding `key` to counter[key], once per
+// distinct key of the wave: a ballot loop finds the groups (no memory access),
+// then every group leader issues its atomic in one instruction.  Returns the
+// lane's rank among equal keys plus the counter's old value.
+__device__ __forceinline__ int wave_atomic_rank(int* counter, int key, bool active) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  unsigned long long todo = __ballot(active);
+  int leader_of = 0, rank = 0, gsize = 0;
+  while (todo) {
+    int leader = __ffsll((unsigned long long)todo) - 1;
+    int k = __shfl(key, leader);
+    bool mine = active && key == k;
+    unsigned long long grp = __ballot(mine);
+    if (mine) { leader_of = leader; rank = __popcll(grp & lt); }
+    if (lane == leader) gsize = __popcll(grp);
+    todo &= ~grp;
+  }
+  int base = 0;
+  if (active && lane == leader_of) base = atomicAdd(&counter[key], gsize);
+  base = __shfl(base, leader_of);
+  return base + rank;
+}
+
 // Far particles: replay the interval's lazy-streaming events; then bin every
-// particle into the shifted grid (count + rank) and the census.
+// particle into the shifted grid (count + rank, by the cell's tour position)
+// and into the census; record its distance to the nearest sphere for the
+// selection of the next interval's near set.
 __global__ __launch_bounds__(256) void co_bulk(CoP P, CoD D, int do_stream, int do_bin) {
   const int NT = P.NT;
   const int nev = D.sc->n_ev;
   __shared__ int ev[MAXEV];
+  __shared__ double sph[3][MAXS];
   for (int j = threadIdx.x; j < nev && j < MAXEV; j += blockDim.x) ev[j] = D.sc->events[j];
+  if (threadIdx.x < 2 * P.nd) {
+    sph[0][threadIdx.x] = D.sc->x0[threadIdx.x];
+    sph[1][threadIdx.x] = D.sc->y0[threadIdx.x];
+    sph[2][threadIdx.x] = D.sc->z0[threadIdx.x];
+  }
   __syncthreads();
   double xs = 0, ys = 0, zs = 0;
   if (do_bin) collision_shift(P, D, xs, ys, zs);
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < NT; i += gridDim.x * blockDim.x) {
-    const bool near = (D.near_bits[i >> 5] >> (i & 31)) & 1u;
-    double x = D.x[i], y = D.y[i], z = D.z[i];
-    int f = D.flag[i];
-    if (do_stream && !near && nev > 0) {
-      double vx = D.vx[i], vy = D.vy[i], vz = D.vz[i];
-      const int f0 = f;
-      bool bounced = false;
-      for (int e = 0; e < nev && e < MAXEV; e++)
-        bounced |= stream_particle(x, y, z, vx, vy, vz, f, (double)ev[e], P, D.react_box, &D.sc->err);
-      D.x[i] = x; D.y[i] = y; D.z[i] = z;
-      if (bounced) { D.vx[i] = vx; D.vy[i] = vy; D.vz[i] = vz; }
-      if (f != f0) D.flag[i] = f;
+  const int lane = threadIdx.x & 63;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long base = (long long)blockIdx.x * blockDim.x; base < NT; base += stride) {
+    const int i = (int)(base + threadIdx.x);
+    const bool valid = i < NT;
+    bool near = false;
+    double x = 0, y = 0, z = 0;
+    int f = -1;
+    if (valid) {
+      near = (D.near_bits[i >> 5] >> (i & 31)) & 1u;
+      x = D.x[i]; y = D.y[i]; z = D.z[i];
+      f = D.flag[i];
+      if (do_stream && !near && nev > 0) {
+        double vx = D.vx[i], vy = D.vy[i], vz = D.vz[i];
+        const int f0 = f;
+        bool bounced = false;
+        for (int e = 0; e < nev && e < MAXEV; e++)
+          bounced |= stream_particle(x, y, z, vx, vy, vz, f, (double)ev[e], P, D.react_box, &D.sc->err);
+        D.x[i] = x; D.y[i] = y; D.z[i] = z;
+        if (bounced) { D.vx[i] = vx; D.vy[i] = vy; D.vz[i] = vz; }
+        if (f != f0) D.flag[i] = f;
+      }
     }
     if (!do_bin) continue;
-    long long c = shifted_cell(P, x, y, z, xs, ys, zs);
-    if (c < 0 || c >= P.mpcnumcell) {
-      raise_error(&D.sc->err, ERR_CELL_RANGE);
-      D.cell_of[i] = 0;
-      D.rank_of[i] = 0xffff;   // skipped by the scatter
-      continue;
+    // shifted cell, counted by tour position
+    bool ok = valid;
+    int t = 0;
+    if (valid) {
+      long long c = shifted_cell(P, x, y, z, xs, ys, zs);
+      if (c < 0 || c >= P.mpcnumcell) { raise_error(&D.sc->err, ERR_CELL_RANGE); ok = false; }
+      else t = D.mpc_pos[c];
     }
-    int r = atomicAdd(&D.cnt[c], 1);
-    if (r >= BMAX) raise_error(&D.sc->err, ERR_BUCKET);
-    D.cell_of[i] = (int)c;
-    D.rank_of[i] = (unsigned short)min(r, 65535);
+    int r = wave_atomic_rank(D.cnt, t, ok);
+    if (valid) {
+      if (ok && r >= BMAX) raise_error(&D.sc->err, ERR_BUCKET);
+      D.cell_of[i] = t;
+      D.rank_of[i] = ok ? (unsigned short)min(r, 65535) : (unsigned short)0xffff;
+      // distance to the nearest sphere beyond its list radius (next near set)
+      double dmin = 1e30;
+      for (int s = 0; s < 2 * P.nd; s++) {
+        double a = pbc_half(x - sph[0][s], P.box[0]);
+        double b = pbc_half(y - sph[1][s], P.box[1]);
+        double c = z - sph[2][s];
+        dmin = fmin(dmin, sqrt(a * a + b * b + c * c) - P.full[s >= P.nd]);
+      }
+      double qd = floor(fmax(dmin, 0.0) * 32.0 * (1.0 - 1e-12));
+      D.dq[i] = (unsigned char)fmin(qd, 255.0);
+    }
     // census of distribute_mod.f95:95-183: outside particles keyed by index,
     // inside ones after all outside ones, inside ones within a cutoff skipped
     unsigned key = (unsigned)i;
-    bool skip = false;
-    if (near) {
-      unsigned char st = D.status[i];
-      if (st & ST_INSIDE) { key = (unsigned)NT + (unsigned)i; skip = (st & ST_EXCL) != 0; }
+    bool cen = false;
+    int ck = 0;
+    if (valid && ok) {
+      bool skip = false;
+      if (near) {
+        unsigned char st = D.status[i];
+        if (st & ST_INSIDE) { key = (unsigned)NT + (unsigned)i; skip = (st & ST_EXCL) != 0; }
+      }
+      int sp = f == 0 ? 0 : f == 1 ? 1 : f == 2 ? 2 : -1;
+      if (!skip && sp >= 0) {
+        long long u = cell_unshifted0(x, y, z, P.box[0], P.box[1]);
+        if (u < 0 || u >= P.vol) raise_error(&D.sc->err, ERR_CELL_RANGE);
+        else { cen = true; ck = (int)(sp * P.vol + u); }
+      }
     }
-    int sp = f == 0 ? 0 : f == 1 ? 1 : f == 2 ? 2 : -1;
-    if (skip || sp < 0) continue;
-    long long u = cell_unshifted0(x, y, z, P.box[0], P.box[1]);
-    if (u < 0 || u >= P.vol) { raise_error(&D.sc->err, ERR_CELL_RANGE); continue; }
-    int n = atomicAdd(&D.tcount[sp * P.vol + u], 1);
-    if (n >= CENSUS_MAX) raise_error(&D.sc->err, ERR_MEMBERS);
-    if (sp == 1) atomicMin(&D.firstB[u], key);
+    {   // census groups: one count and one minimum key per (species, cell) of the wave
+      unsigned long long todo = __ballot(cen);
+      const unsigned long long outside = __ballot(cen && key < (unsigned)NT);
+      int gsize = 0;
+      unsigned gkey = 0;
+      bool leader = false;
+      while (todo) {
+        int ld = __ffsll((unsigned long long)todo) - 1;
+        int k = __shfl(ck, ld);
+        unsigned long long grp = __ballot(cen && ck == k);
+        unsigned long long go = grp & outside;
+        int src = __ffsll((unsigned long long)(go ? go : grp)) - 1;
+        unsigned mk = (unsigned)__shfl((int)key, src);
+        if (lane == ld) { leader = true; gsize = __popcll(grp); gkey = mk; }
+        todo &= ~grp;
+      }
+      if (leader) {
+        int old = atomicAdd(&D.tcount[ck], gsize);
+        if (old + gsize > CENSUS_MAX) raise_error(&D.sc->err, ERR_MEMBERS);
+        if (ck >= P.vol && ck < 2 * P.vol) atomicMin(&D.firstB[ck - P.vol], gkey);
+      }
+    }
   }
 }
 
@@ -828,58 +988,84 @@ __global__ __launch_bounds__(256) void co_scatter(CoP P, CoD D) {
   }
 }
 
-// Per collision cell: members in ascending particle order, centre-of-mass
-// velocity (mpc_rotation_mod.f95:153-182), random axis (:184-193); per
-// unshifted cell the B->A reaction (reaction_mod.f95:41-62).
+// Per collision cell, in tour order (thread t <-> t-th cell of the tour, so a
+// wave touches one compact patch of the Hilbert-sorted particle arrays):
+// members in ascending particle order, centre-of-mass velocity
+// (mpc_rotation_mod.f95:153-182), random axis (:184-193).  Thread t also does
+// the B->A reaction of unshifted cell t (reaction_mod.f95:41-62).
 __global__ __launch_bounds__(CT) void co_collide(CoP P, CoD D) {
-  __shared__ int sidx[BMAX * CT];
-  const long long c = (long long)blockIdx.x * CT + threadIdx.x;
-  if (c >= P.mpcnumcell) return;
-  int n = D.cnt[c];
-  if (n > BMAX) n = BMAX;   // ERR_BUCKET already raised
-  int o = D.off[c];
-  for (int j = 0; j < n; j++) {
-    int v = D.bucket[o + j];
-    int k = j;
-    while (k > 0 && sidx[(k - 1) * CT + threadIdx.x] > v) {
-      sidx[k * CT + threadIdx.x] = sidx[(k - 1) * CT + threadIdx.x];
-      k--;
+  constexpr int LB = 32;   // members sorted in LDS; larger cells sort in place in the bucket
+  __shared__ int sidx[LB * CT];
+  const long long t = (long long)blockIdx.x * CT + threadIdx.x;
+  if (t < P.mpcnumcell) {
+    const int c = D.mpc_cell[t];
+    int n = D.cnt[t];
+    if (n > BMAX) n = BMAX;   // ERR_BUCKET already raised
+    const int* b = D.bucket + D.off[t];
+    int* idx;
+    int istride;
+    if (n <= LB) {
+      idx = sidx + threadIdx.x;
+      istride = CT;
+      for (int j = 0; j < n; j++) {   // insertion: ascending particle index
+        int v = b[j], k = j;
+        while (k > 0 && idx[(k - 1) * CT] > v) { idx[k * CT] = idx[(k - 1) * CT]; k--; }
+        idx[k * CT] = v;
+      }
+    } else {
+      idx = (int*)b;
+      istride = 1;
+      for (int j = 1; j < n; j++) {
+        int v = idx[j], k = j;
+        while (k > 0 && idx[k - 1] > v) { idx[k] = idx[k - 1]; k--; }
+        idx[k] = v;
+      }
     }
-    sidx[k * CT + threadIdx.x] = v;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    int j = 0;
+    for (; j + 4 <= n; j += 4) {   // loads of four members in flight, sums in order
+      int i0 = idx[j * istride], i1 = idx[(j + 1) * istride], i2 = idx[(j + 2) * istride], i3 = idx[(j + 3) * istride];
+      double ax = D.vx[i0], bx = D.vx[i1], cx = D.vx[i2], dx = D.vx[i3];
+      double ay = D.vy[i0], by = D.vy[i1], cy = D.vy[i2], dy = D.vy[i3];
+      double az = D.vz[i0], bz = D.vz[i1], cz = D.vz[i2], dz = D.vz[i3];
+      sx = sx + ax; sx = sx + bx; sx = sx + cx; sx = sx + dx;
+      sy = sy + ay; sy = sy + by; sy = sy + cy; sy = sy + dy;
+      sz = sz + az; sz = sz + bz; sz = sz + cz; sz = sz + dz;
+    }
+    for (; j < n; j++) {
+      int i = idx[j * istride];
+      sx = sx + D.vx[i];
+      sy = sy + D.vy[i];
+      sz = sz + D.vz[i];
+    }
+    if (n != 0) {
+      double tn = (double)n;
+      sx = sx / tn; sy = sy / tn; sz = sz / tn;
+    }
+    double u1, u2;
+    if (P.rng_mode == 0) {
+      u1 = D.U[3 + 2 * (long long)c];
+      u2 = D.U[4 + 2 * (long long)c];
+    } else {
+      philox_uniform2(P.seed, 1, D.coll, (unsigned long long)c, u1, u2);
+    }
+    double phi = P.twopi * u1;
+    double th = 2.0 * u2 - 1.0;
+    double st = sqrt(1.0 - th * th);
+    double2* rec = (double2*)(D.cellrec + t * 8);
+    rec[0] = make_double2(sx, sy);
+    rec[1] = make_double2(sz, st * cos(phi));
+    rec[2] = make_double2(st * sin(phi), th);
+    rec[3] = make_double2((double)n, 0.0);
   }
-  double sx = 0.0, sy = 0.0, sz = 0.0;
-  for (int j = 0; j < n; j++) {
-    int i = sidx[j * CT + threadIdx.x];
-    sx = sx + D.vx[i];
-    sy = sy + D.vy[i];
-    sz = sz + D.vz[i];
-  }
-  if (n != 0) {
-    double tn = (double)n;
-    sx = sx / tn; sy = sy / tn; sz = sz / tn;
-  }
-  double u1, u2;
-  if (P.rng_mode == 0) {
-    u1 = D.U[3 + 2 * c];
-    u2 = D.U[4 + 2 * c];
-  } else {
-    philox_uniform2(P.seed, 1, D.coll, (unsigned long long)c, u1, u2);
-  }
-  double phi = P.twopi * u1;
-  double th = 2.0 * u2 - 1.0;
-  double st = sqrt(1.0 - th * th);
-  double* rec = D.cellrec + c * 8;
-  rec[0] = sx; rec[1] = sy; rec[2] = sz;
-  rec[3] = st * cos(phi); rec[4] = st * sin(phi); rec[5] = th;
-  rec[6] = (double)n; rec[7] = 0.0;
-  if (c < P.rxnnumcell) {
-    int nb = D.tcount[1 * P.vol + c];
+  if (t < P.rxnnumcell) {
+    int nb = D.tcount[1 * P.vol + t];
     double prob = D.ptab[min(nb, CENSUS_MAX)];
     double r;
-    if (P.rng_mode == 0) r = D.U[3 + 2 * P.mpcnumcell + c];
-    else { double t; philox_uniform2(P.seed, 2, D.coll, (unsigned long long)c, r, t); }
+    if (P.rng_mode == 0) r = D.U[3 + 2 * P.mpcnumcell + t];
+    else { double q; philox_uniform2(P.seed, 2, D.coll, (unsigned long long)t, r, q); }
     if (0.0 < r && r <= prob) {
-      unsigned key = D.firstB[c];
+      unsigned key = D.firstB[t];
       int i = key >= (unsigned)P.NT ? (int)(key - (unsigned)P.NT) : (int)key;
       D.flag[i] = 0;
     }
@@ -887,21 +1073,16 @@ __global__ __launch_bounds__(CT) void co_collide(CoP P, CoD D) {
 }
 
 // Rotation of mpc_rotation_mod.f95:205-230 and selection of the next
-// interval's near set.  The particle lies in the unit box of its shifted
-// cell, so the distance from that box to each sphere bounds its distance.
+// interval's near set: a particle is near when its distance beyond a list
+// radius (recorded by co_bulk, rounded down) is within |v| tau + guard.
 __global__ __launch_bounds__(256) void co_rotate(CoP P, CoD D, int classify) {
-  double xs, ys, zs;
-  collision_shift(P, D, xs, ys, zs);
-  const int nd = P.nd;
-  double sx[MAXS], sy[MAXS], sz[MAXS];
-  for (int s = 0; s < 2 * nd; s++) { sx[s] = D.sc->x0[s]; sy[s] = D.sc->y0[s]; sz[s] = D.sc->z0[s]; }
-  const int Lx = P.Lx, Ly = P.Ly;
-  const double zoff = zs < 0.0 ? 1.0 : 0.0;
   double fmax2 = 0.0;
+  const double tau = P.tau * (1.0 + 1e-9);
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.NT; i += gridDim.x * blockDim.x) {
-    int c = D.cell_of[i];
-    const double* rec = D.cellrec + (long long)c * 8;
-    double cx = rec[0], cy = rec[1], cz = rec[2], ux = rec[3], uy = rec[4], uz = rec[5];
+    int t = D.cell_of[i];
+    const double2* rec = (const double2*)(D.cellrec + (long long)t * 8);
+    double2 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+    double cx = r0.x, cy = r0.y, cz = r1.x, ux = r1.y, uy = r2.x, uz = r2.y;
     double vx = D.vx[i], vy = D.vy[i], vz = D.vz[i];
     double dx = vx - cx, dy = vy - cy, dz = vz - cz;
     const double cr = P.cosrot, sr = P.sinrot, omc = P.omc;
@@ -912,20 +1093,8 @@ __global__ __launch_bounds__(256) void co_rotate(CoP P, CoD D, int classify) {
     D.vx[i] = vx; D.vy[i] = vy; D.vz[i] = vz;
     if (!classify) continue;
     double v2 = (vx * vx + vy * vy) + vz * vz;
-    double reach = sqrt(v2) * P.tau * (1.0 + 1e-9) + P.guard + 1e-6;
-    int ix = c % Lx, iy = (c / Lx) % Ly, iz = c / (Lx * Ly);
-    // particle box: x in [ix - xs, ix + 1 - xs) (mod Lx), z in [iz - zs - zoff, iz + 1 - zs - zoff)
-    double mx = (double)ix - xs + 0.5, my = (double)iy - ys + 0.5, mz = (double)iz - zs - zoff + 0.5;
-    bool near = false;
-    for (int s = 0; s < 2 * nd; s++) {
-      int ii = s >= nd;
-      double ax = fmax(fabs(pbc_half(mx - sx[s], P.box[0])) - 0.5 - 1e-9, 0.0);
-      double ay = fmax(fabs(pbc_half(my - sy[s], P.box[1])) - 0.5 - 1e-9, 0.0);
-      double az = fmax(fabs(mz - sz[s]) - 0.5 - 1e-9, 0.0);
-      double r = P.full[ii] + reach;
-      if (ax * ax + ay * ay + az * az <= r * r) near = true;
-    }
-    if (near) atomicOr(&D.near_bits[i >> 5], 1u << (i & 31));
+    double reach = sqrt(v2) * tau + P.guard + 1e-6;
+    if ((double)D.dq[i] * (1.0 / 32.0) <= reach) atomicOr(&D.near_bits[i >> 5], 1u << (i & 31));
     else fmax2 = fmax(fmax2, v2);
   }
   if (!classify) return;
@@ -1044,13 +1213,23 @@ __global__ __launch_bounds__(256) void sort_order(long long ncell, const int* sc
 __global__ __launch_bounds__(256) void sort_gather(CoP P, CoD D, const int* sb, double* nx, double* ny, double* nz,
                                                    double* nvx, double* nvy, double* nvz, int* nf,
                                                    unsigned char* nst, unsigned* nbits) {
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < P.NT; j += gridDim.x * blockDim.x) {
-    int i = sb[j];
-    nx[j] = D.x[i]; ny[j] = D.y[i]; nz[j] = D.z[i];
-    nvx[j] = D.vx[i]; nvy[j] = D.vy[i]; nvz[j] = D.vz[i];
-    nf[j] = D.flag[i];
-    nst[j] = D.status[i];
-    if ((D.near_bits[i >> 5] >> (i & 31)) & 1u) atomicOr(&nbits[j >> 5], 1u << (j & 31));
+  const int lane = threadIdx.x & 63;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long base = (long long)blockIdx.x * blockDim.x; base < P.NT; base += stride) {
+    const long long j = base + threadIdx.x;
+    bool bit = false;
+    if (j < P.NT) {
+      int i = sb[j];
+      nx[j] = D.x[i]; ny[j] = D.y[i]; nz[j] = D.z[i];
+      nvx[j] = D.vx[i]; nvy[j] = D.vy[i]; nvz[j] = D.vz[i];
+      nf[j] = D.flag[i];
+      nst[j] = D.status[i];
+      bit = (D.near_bits[i >> 5] >> (i & 31)) & 1u;
+    }
+    unsigned long long m = __ballot(bit);   // the wave covers 64 consecutive j: two words
+    const long long w0 = (base + (threadIdx.x & ~63)) >> 5;
+    if (lane == 0) nbits[w0] = (unsigned)m;
+    if (lane == 32) nbits[w0 + 1] = (unsigned)(m >> 32);
   }
 }
 
@@ -1082,6 +1261,7 @@ struct mpcx_co {
   std::vector<void*> allocs;
   long long nwords = 0;
   int nd = 1;
+  std::vector<int> mpc_pos, mpc_cell;   // host copies of the collision-cell tour
 };
 
 template <typename T>
@@ -1115,7 +1295,7 @@ static int check_device_error(mpcx_co* e) {
     static const char* msg[] = {"none",
                                 "a particle maps outside the cell arrays",
                                 "a cell holds more members than the reference arrays (100)",
-                                "inside set larger than the near engine's 4096 register slots",
+                                "inside set larger than the near engine capacity (16384)",
                                 "a sphere moved further than the near/far guard within one interval",
                                 "a particle belongs to more than 4 neighbour lists",
                                 "a collision cell holds more than 64 particles",
@@ -1210,7 +1390,7 @@ extern "C" int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg) {
   D.dim = dalloc<double>(e, 12 * MAXS);
   D.sc = dalloc<CoScalars>(e, 1);
   D.react_box = dalloc<int>(e, std::max(1, P.nrb));
-  D.in_cap = NE_T * NE_K;
+  D.in_cap = NE_CAP;
   D.near_cap = (int)std::min<long long>(NT, 1 << 22);
   D.near_list = dalloc<int>(e, D.near_cap);
   D.near_memb = dalloc<int>(e, D.near_cap);
@@ -1222,6 +1402,9 @@ extern "C" int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg) {
   D.in_fp = dalloc<double>(e, (size_t)MAXM * 3 * D.in_cap);
   D.in_pot = dalloc<double>(e, (size_t)MAXM * D.in_cap);
   D.in_msd = dalloc<double>(e, D.in_cap);
+  D.sl_x = dalloc<double>(e, D.in_cap); D.sl_y = dalloc<double>(e, D.in_cap); D.sl_z = dalloc<double>(e, D.in_cap);
+  D.sl_vx = dalloc<double>(e, D.in_cap); D.sl_vy = dalloc<double>(e, D.in_cap); D.sl_vz = dalloc<double>(e, D.in_cap);
+  D.sl_flag = dalloc<int>(e, D.in_cap);
   D.cell_of = dalloc<int>(e, NT);
   D.rank_of = dalloc<unsigned short>(e, NT);
   D.cnt = dalloc<int>(e, mc);
@@ -1230,6 +1413,11 @@ extern "C" int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg) {
   long long npart = std::max((mc + 4095) / 4096, std::max((vol + 4095) / 4096, (e->nwords + 1023) / 1024)) + 2;
   D.partial = dalloc<int>(e, npart);
   D.cellrec = dalloc<double>(e, mc * 8);
+  D.dq = dalloc<unsigned char>(e, NT);
+  int* mpos = dalloc<int>(e, mc);
+  int* mcell = dalloc<int>(e, mc);
+  D.mpc_pos = mpos;
+  D.mpc_cell = mcell;
   D.tcount = dalloc<int>(e, 3 * vol);
   D.firstB = dalloc<unsigned>(e, vol);
   double* ptab = dalloc<double>(e, 101);
@@ -1303,6 +1491,23 @@ extern "C" int mpcx_co_set_state(mpcx_co* e, int32_t cstep, const double* x, con
     for (int i = 0; i < e->P.nrb; i++) rb[i] = react_box[i];
   HIPCHK(hipMemcpy((void*)D.react_box, rb.data(), rb.size() * 4, hipMemcpyHostToDevice));
   e->has_tour = false;
+  // collision cells are visited in the tour order of the unshifted cells they
+  // share an index with (x + Lx*y + Lx*Ly*z, z < Lz), then the extra z layer
+  {
+    const long long mc = e->P.mpcnumcell, vol = e->P.vol;
+    e->mpc_cell.assign((size_t)mc, 0);
+    e->mpc_pos.assign((size_t)mc, -1);
+    long long t = 0;
+    if (tour)
+      for (long long j = 0; j < vol; j++) {
+        long long c = tour[j] - 1;
+        if (c >= 0 && c < vol && c < mc && e->mpc_pos[c] < 0) { e->mpc_pos[c] = (int)t; e->mpc_cell[t] = (int)c; t++; }
+      }
+    for (long long c = 0; c < mc; c++)
+      if (e->mpc_pos[c] < 0) { e->mpc_pos[c] = (int)t; e->mpc_cell[t] = (int)c; t++; }
+    HIPCHK(hipMemcpy((void*)D.mpc_pos, e->mpc_pos.data(), mc * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy((void*)D.mpc_cell, e->mpc_cell.data(), mc * 4, hipMemcpyHostToDevice));
+  }
   if (tour) {
     std::vector<int> tp((size_t)e->P.vol, -1);
     for (long long t = 0; t < e->P.vol; t++) {
@@ -1513,10 +1718,25 @@ extern "C" int mpcx_co_get_cells(mpcx_co* e, double* vcm_uv, int32_t* cell_of, i
   if (!e) return set_error(MPCX_E_ARG, "null engine");
   HIPCHK(hipStreamSynchronize(e->st));
   CoD& D = e->D;
-  if (vcm_uv) HIPCHK(hipMemcpy(vcm_uv, D.cellrec, (size_t)e->P.mpcnumcell * 64, hipMemcpyDeviceToHost));
-  if (cell_of) HIPCHK(hipMemcpy(cell_of, D.cell_of, (size_t)e->P.NT * 4, hipMemcpyDeviceToHost));
+  const size_t mc = (size_t)e->P.mpcnumcell;
+  if (vcm_uv) {   // stored by tour position; returned by cell
+    std::vector<double> rec(mc * 8);
+    HIPCHK(hipMemcpy(rec.data(), D.cellrec, mc * 64, hipMemcpyDeviceToHost));
+    for (size_t t = 0; t < mc; t++) memcpy(vcm_uv + (size_t)e->mpc_cell[t] * 8, &rec[t * 8], 64);
+  }
+  if (cell_of) {
+    HIPCHK(hipMemcpy(cell_of, D.cell_of, (size_t)e->P.NT * 4, hipMemcpyDeviceToHost));
+    for (long long i = 0; i < e->P.NT; i++) cell_of[i] = e->mpc_cell[cell_of[i]];
+  }
   if (census) HIPCHK(hipMemcpy(census, D.tcount, (size_t)e->P.vol * 12, hipMemcpyDeviceToHost));
   if (tpotential) HIPCHK(hipMemcpy(tpotential, &D.sc->tpotential, 8, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int mpcx_co_near_profile(mpcx_co* e, uint64_t* prof8) {
+  if (!e || !prof8) return set_error(MPCX_E_ARG, "null argument");
+  HIPCHK(hipStreamSynchronize(e->st));
+  HIPCHK(hipMemcpy(prof8, e->D.sc->prof, 64, hipMemcpyDeviceToHost));
   return 0;
 }
 

@@ -15,6 +15,12 @@ int set_error(int code, const char* msg) {
 
 extern "C" const char* mpcx_last_error(void) { return mpcx::g_last_error.c_str(); }
 
+extern "C" int mpcx_device_synchronize(void) {
+  hipError_t r = hipDeviceSynchronize();
+  if (r != hipSuccess) return mpcx::set_error(MPCX_E_HIP, hipGetErrorString(r));
+  return 0;
+}
+
 extern "C" int mpcx_device_count(void) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;
