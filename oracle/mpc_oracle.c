@@ -1278,3 +1278,25 @@ int or_sk_run(int64_t *ip, double *rp, int32_t seed, int64_t cstep0, int64_t nst
   free(s.uvx); free(s.uvy); free(s.uvz); free(s.tcs); free(s.ncn);
   return rc;
 }
+
+int or_co_distribute_flat(int32_t NT, int32_t numdims, const int32_t *flag,
+                          const double *x, const double *y, const double *z,
+                          const double *xd, const double *yd, const double *zd, const double *box,
+                          int64_t vol, int64_t *tA, int64_t *tB, int64_t *tS,
+                          int64_t *sA, int64_t *sB, int64_t *sS, int64_t maxs,
+                          double cutC2, double cutN2,
+                          int64_t out_num, const int64_t *out_list, int64_t in_num, const int64_t *in_list) {
+  /* or_co_distribute with the flat argument list of distribute_mod.f95:6-40 */
+  or_co_params p;
+  or_co_state s;
+  memset(&p, 0, sizeof p);
+  memset(&s, 0, sizeof s);
+  p.NT = NT; p.numdims = numdims; p.vol = vol;
+  for (int d = 0; d < 3; d++) p.box[d] = box[d];
+  s.flag = (int32_t *)flag;
+  s.x = (double *)x; s.y = (double *)y; s.z = (double *)z;
+  s.xd = (double *)xd; s.yd = (double *)yd; s.zd = (double *)zd;
+  s.out_list = (int64_t *)out_list; s.in_list = (int64_t *)in_list;
+  s.out_num = out_num; s.in_num = in_num;
+  return or_co_distribute(&p, &s, tA, tB, tS, sA, sB, sS, maxs, cutC2, cutN2);
+}

@@ -89,6 +89,13 @@ int  or_co_distribute(const or_co_params *p, const or_co_state *s,
                       int64_t *tA, int64_t *tB, int64_t *tS,
                       int64_t *sA, int64_t *sB, int64_t *sS, int64_t maxs,
                       double cutC2, double cutN2);                          /* distribute_mod.f95:6-205 */
+int  or_co_distribute_flat(int32_t NT, int32_t numdims, const int32_t *flag,
+                           const double *x, const double *y, const double *z,
+                           const double *xd, const double *yd, const double *zd, const double *box,
+                           int64_t vol, int64_t *tA, int64_t *tB, int64_t *tS,
+                           int64_t *sA, int64_t *sB, int64_t *sS, int64_t maxs,
+                           double cutC2, double cutN2,
+                           int64_t out_num, const int64_t *out_list, int64_t in_num, const int64_t *in_list);
 int  or_co_mpc_rotation(int32_t NT, double rotang, int64_t numcell, double a0, const double *box,
                         const double *x, const double *y, const double *z,
                         double *vx, double *vy, double *vz,

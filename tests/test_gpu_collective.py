@@ -1,0 +1,169 @@
+"""GPU parity and properties of the device-resident collective engine.
+
+Parity (host Mersenne-Twister draws, the reference's draw order): the engine
+and the oracle advance the same seeded system through whole collision
+intervals; integer state (flags, cell indices, census and collision-cell
+counts, inside list, neighbour counts, cstep/time_count/time_penetrate) must
+be identical and positions/velocities within 1e-12 relative
+(tests/parity_cases.py states the scales).  The golden replays compare the
+engine's integer arrays with the SHA-256 the reference's own Fortran modules
+produced (tests/golden/reference_runs.json).
+
+Properties at the full 100^3 size (device Philox draws, where no oracle run is
+affordable): particle number, confinement, temperature, and bit-identical
+reruns.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import hashlib
+import sys
+
+import numpy as np
+import pytest
+
+from main_amd import systems as S
+from main_amd._lib import MtmodStream, device_synchronize
+from main_amd.collective import CollectiveEngine, host_draws
+from main_amd.tour import cell_tour
+from oracle import oracle as O
+from tests.parity_cases import collective_parity_case, small_collective_params
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def _reactive_grid(p, kind):
+    rb = np.zeros(int(p["box"][0]) * int(p["box"][1]), np.int32)
+    O.oracle_lib().or_co_reactive_grid_setup(O._ptr(p["box"]), O._ptr(rb), kind)
+    return rb
+
+
+CASES = {
+    "basic_3coll": dict(ncoll=3, seed=3),
+    "sort_and_species": dict(ncoll=4, seed=4, flags_mix=True, p=("small", dict(sort_step=20)), tour="hilbert"),
+    "two_dimers_sort": dict(ncoll=3, seed=7, flags_mix=True,
+                            p=("small", dict(numdims=2, Lx=20.0, Ly=20.0, Lz=24.0, sort_step=20, EbN=0.1)),
+                            tour="hilbert"),
+    "reactive_wall": dict(ncoll=3, seed=8, p=("small", dict(k1=0.2, Lz=12.0)), react="wall2"),
+    "dense_skin05": dict(ncoll=2, seed=9, flags_mix=True, p=("small", dict(rhoa=8.0, skinrad=0.5, Rn=2.5))),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_engine_matches_oracle(name):
+    kw = dict(CASES[name])
+    _say(f"[parity] {name}")
+    if "p" in kw:
+        kw["p"] = small_collective_params(**kw["p"][1])
+    p = kw.get("p") or small_collective_params()
+    if kw.pop("tour", None) == "hilbert":
+        kw["tour"] = cell_tour(int(p["box"][0]), int(p["box"][1]), int(p["box"][2]))
+    if kw.pop("react", None) == "wall2":
+        kw["react_box"] = _reactive_grid(p, 2)
+    r = collective_parity_case(**kw)
+    _say(f"[parity] {name}: {r['summary']}")
+    assert r["ok"], r["summary"]
+
+
+def test_engine_matches_oracle_30cube_headline_physics():
+    """The reference's CPU-runnable size: 30^3 cells, 10 per cell (266,984
+    solvent), Rc 2 / Rn 4 / bond 6.5, tau 0.5 = 50 x 0.01, Hilbert sort."""
+    from bench import headline_params
+    p = headline_params(30)
+    p["sort_step"] = 100     # a re-sort inside the compared window
+    _say("[parity] 30^3 headline physics, 3 collisions")
+    r = collective_parity_case(ncoll=3, seed=21, p=p, tour=cell_tour(30, 30, 30))
+    _say(f"[parity] 30^3: {r['summary']}")
+    assert r["ok"], r["summary"]
+
+
+def _h(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("case", ["co_one_dimer", "co_two_dimers_sort_wall"])
+def test_engine_integer_state_equals_reference_fixture(golden, case):
+    """Flags and cell arrays hashed from the reference's Fortran run."""
+    from tests.golden.make_golden import collective_inputs
+    rec = golden["cases"][case]
+    cs = rec["case"]
+    p, s = collective_inputs(cs)
+    L = [int(v) for v in p["box"]]
+    tour = cell_tour(*L) if cs["sort"] else np.arange(1, p["vol"] + 1, dtype=np.int64)
+    ncoll = cs["steps"] // p["MPCtstep"]
+    assert ncoll * p["MPCtstep"] == cs["steps"]
+    draws = host_draws(MtmodStream(mt=s["mt"], mti=s["mti"]), p, ncoll)
+    eng = CollectiveEngine(p, rng_mode=0)
+    try:
+        eng.set_state(s, tour=tour)
+        eng.setup(do_sort=cs["sort"])
+        eng.run(ncoll, draws)
+        g = eng.get_state()
+        cells = eng.get_cells()
+    finally:
+        eng.close()
+    got = {"flag": _h(g["flag"].astype(np.int32)), "ncn": _h(cells["ncn"].astype(np.int64)),
+           "tcs": _h(cells["tcs"]), "tA": _h(cells["tA"][: p["vol"]]), "tB": _h(cells["tB"][: p["vol"]]),
+           "tS": _h(cells["tS"][: p["vol"]])}
+    bad = [k for k, v in got.items() if v != rec["arrays"][k]]
+    assert not bad, f"{case}: differs from the reference run in {bad}"
+    for k in ("cstep", "time_count", "time_penetrate"):
+        assert g[k] == rec["scalars"][k], k
+
+
+def _full_size_engine(L, seed):
+    from bench import headline_params
+    p = headline_params(L)
+    s = S.collective_initial_state(p, seed=seed)
+    eng = CollectiveEngine(p, rng_mode=1, seed=seed)
+    eng.set_state(s, tour=cell_tour(L, L, L))
+    eng.setup(do_sort=True)
+    return p, s, eng
+
+
+def test_full_size_conservation_and_confinement():
+    """100^3 cells (9,996,984 solvent), 12 collisions with a re-sort, Philox draws.
+    (The z walls bounce particles back, so no momentum component is conserved.)"""
+    L = 100
+    _say("[props] building 100^3")
+    p, s, eng = _full_size_engine(L, 4357787)
+    try:
+        m = p["Ma"]
+        eng.run(12)
+        device_synchronize()
+        g = eng.get_state()
+        cells = eng.get_cells()
+    finally:
+        eng.close()
+    _say(f"[props] cstep {g['cstep']}")
+    assert g["cstep"] == 12 * p["MPCtstep"]
+    assert set(np.unique(g["flag"])) <= {0, 1}
+    assert np.all((g["x"] >= 0) & (g["x"] < L) & (g["y"] >= 0) & (g["y"] < L))
+    assert np.all((g["z"] >= 0) & (g["z"] <= L))
+    # every particle lies in exactly one collision cell
+    assert int(cells["tcs"].sum()) == p["NT"]
+    assert np.all((cells["ncn"] >= 1) & (cells["ncn"] <= p["mpcnumcell"]))
+    # the census counts only particles outside the sphere cutoffs
+    assert 0 < int(cells["tA"].sum() + cells["tB"].sum()) <= p["NT"]
+    # equipartition (2/3) KE / N = kT, within the 3/sqrt(N)-level noise
+    ke = 0.5 * m * (g["vx"] ** 2 + g["vy"] ** 2 + g["vz"] ** 2).sum()
+    T = (2.0 / 3.0) * ke / p["NT"]
+    assert abs(T / p["T"] - 1.0) < 2e-3, T
+
+
+def test_reruns_are_bit_identical():
+    """Same seed, same state -> identical arrays (ordered sums everywhere)."""
+    outs = []
+    for rep in range(2):
+        p, s, eng = _full_size_engine(40, 99)
+        try:
+            eng.run(5)
+            outs.append(eng.get_state())
+        finally:
+            eng.close()
+    for k in ("x", "y", "z", "vx", "vy", "vz", "flag", "xd", "vxd"):
+        assert np.array_equal(outs[0][k], outs[1][k]), k
