@@ -9,7 +9,8 @@ interval: 50 MD steps plus the collision.  Synthetic initial state; the
 collision draws come from the device Philox stream (the throughput mode; the
 parity mode with host Mersenne-Twister draws is measured in the tests).
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--probe-copy]
+    python bench.py --summarize-pmc FETCH_DIR WRITE_DIR    (rocprofv3 counter CSVs -> traffic)
 
 Prints one JSON line.  value = particle-updates/s (NT * K / wall time of the K
 timed intervals, state resident in HBM).
@@ -17,10 +18,11 @@ timed intervals, state resident in HBM).
 from __future__ import annotations
 
 import argparse
+import csv
+import glob
 import json
-import math
 import os
-import subprocess
+import re
 import sys
 import time
 
@@ -30,7 +32,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_UPDATE_FP64 = 100.0   # pos+vel read (48), pos+vel written (48), flag read (4)
+PROBE_BYTES = 256 << 20          # copy probe: reads and writes this many bytes per launch
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_100cube.txt")
+
+# Algorithmic HBM bytes per particle of each interval kernel (FP64 state):
+#  co_bulk   reads x,y,z,vx,vy,vz (48), flag (4), status (1); writes x,y,z (24),
+#            collision-cell position (4), rank (2), distance byte (1)        = 84
+#  co_rotate reads cell position (4), vx,vy,vz (24), distance byte (1); writes vx,vy,vz (24)
+#            (+ the cell records, 64 B per cell, about 6.4 B per particle)  = 59.4
+#  co_collide per particle: bucket entry (4) + vx,vy,vz gathered (24); per cell 64 B written
+#  co_scatter reads cell (4) + rank (2) + offset (4), writes bucket entry (4)     = 14
+ALGO_BYTES = {"co_bulk": 84.0, "co_rotate": 59.4, "co_collide": 34.4, "co_scatter": 14.0}
 
 
 def headline_params(L=100):
@@ -40,43 +52,108 @@ def headline_params(L=100):
                                skinrad=0.3, MDt=0.01, MPCt=0.5, sort_step=500, k1=1e-3, react_box_type=7)
 
 
-def cpu_baseline(p, s):
-    """The oracle (C restatement of the reference, one core) on a bounded
-    sample of the same workload: one collision interval of a 40^3-cell box
-    with the same physics; reported in the bench metric's unit."""
+def cpu_baseline(L=100, budget_s=15.0):
+    """The reference's own Fortran modules (oracle/_ref, flang, one thread) on a
+    bounded sample of the same workload: whole collision intervals of the same
+    100^3 system, as many as fit the time budget.  Falls back to the oracle's
+    C restatement when the reference build is absent."""
     from oracle import oracle as O
     from main_amd import systems as S
-    import copy
-    ps = S.collective_params(Lx=40.0, Ly=40.0, Lz=40.0, numdims=1, rhoa=10.0, Rc=2.0, Rn=4.0, Dcn=6.5,
-                             EbN=0.1, T=1.0 / 6.0, skinrad=0.3, MDt=0.01, MPCt=0.5, sort_step=1000000, k1=1e-3)
+    use_ref = O.ref_lib("collective") is not None
+    ps = headline_params(L)
+    ps["sort_step"] = 1000000   # no re-sort inside the sample
     ss = S.collective_initial_state(ps, seed=11)
     ss["mt"], ss["mti"] = O.mtmod_seed(4357787)
-    O.co_run(ss, 0, do_setup=1)
-    ncoll = 3
-    t0 = time.perf_counter()
-    O.co_run(ss, ncoll * ps["MPCtstep"])
-    dt = time.perf_counter() - t0
-    return {"value": ps["NT"] * ncoll / dt, "unit": "particle-updates/s", "cores": 1, "kind": "port",
-            "sample": f"oracle, {ncoll} collision intervals of a 40^3-cell box ({ps['NT']} particles), "
+    O.co_run(ss, 0, do_setup=1, use_ref=use_ref)
+    n, dt = 0, 0.0
+    while dt < budget_s and n < 20:
+        t0 = time.perf_counter()
+        O.co_run(ss, ps["MPCtstep"], use_ref=use_ref)
+        dt += time.perf_counter() - t0
+        n += 1
+    kind = "reference" if use_ref else "port"
+    what = "flang-built reference modules" if use_ref else "C restatement of the reference"
+    return {"value": ps["NT"] * n / dt, "unit": "particle-updates/s", "cores": 1, "kind": kind,
+            "sample": f"{what}, {n} collision intervals of the {L}^3 system ({ps['NT']} particles), "
                       f"{dt:.1f} s on one host core"}
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            m = re.match(r"traffic_bytes_per_launch\s+(\S+)\s+(\S+)", line)
+            if m and m.group(1) == kernel:
+                return float(m.group(2))
+    return None
+
+
+def summarize_pmc(fetch_dir, write_dir):
+    """Mean FETCH_SIZE / WRITE_SIZE (KiB) per dispatch of each kernel, scaled by
+    the readings of the copy probe (PROBE_BYTES read and written per launch)."""
+    def load(d, counter):
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if not files:
+            raise SystemExit(f"no counter_collection.csv under {d}")
+        per = {}
+        for fn in files:
+            with open(fn) as f:
+                for row in csv.DictReader(f):
+                    if row.get("Counter_Name") != counter:
+                        continue
+                    name = row["Kernel_Name"].split("(")[0].strip()
+                    name = re.sub(r"^void\s+", "", name)
+                    per.setdefault(name, []).append(float(row["Counter_Value"]))
+        return {k: float(np.mean(v)) for k, v in per.items()}, {k: len(v) for k, v in per.items()}
+    fetch, nf = load(fetch_dir, "FETCH_SIZE")
+    write, nw = load(write_dir, "WRITE_SIZE")
+    probe = [k for k in fetch if "probe_copy_kernel" in k]
+    if not probe:
+        raise SystemExit("the copy probe kernel is missing from the counter runs (run bench.py --probe-copy)")
+    pk = probe[0]
+    kib = PROBE_BYTES / 1024.0
+    f_scale = kib / fetch[pk]
+    w_scale = kib / write[pk]
+    print(f"# copy probe {PROBE_BYTES} B: FETCH_SIZE {fetch[pk]:.0f} KiB, WRITE_SIZE {write.get(pk, float('nan')):.0f} KiB")
+    print(f"# calibration: fetch factor {f_scale:.4f}, write factor {w_scale:.4f}")
+    print(f"# {'kernel':40s} {'dispatches':>10s} {'FETCH KiB':>12s} {'WRITE KiB':>12s} {'HBM bytes/launch':>16s}")
+    for k in sorted(fetch, key=lambda k: -fetch[k] - write.get(k, 0.0)):
+        fb = fetch[k] * f_scale * 1024.0
+        wb = write.get(k, 0.0) * w_scale * 1024.0
+        short = re.sub(r"^mpcx::", "", k)
+        print(f"# {short[:40]:40s} {nf[k]:10d} {fetch[k]:12.0f} {write.get(k, 0.0):12.0f} {fb + wb:16.4g}")
+        print(f"traffic_bytes_per_launch {short.split('<')[0]} {fb + wb:.6g}")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--size", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--probe-copy", action="store_true", help="also run the copy probe (counter calibration)")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     args = ap.parse_args()
+    if args.summarize_pmc:
+        summarize_pmc(*args.summarize_pmc)
+        return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
-                                   "and not launched", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1", "n_gpus": args.gpus}))
         sys.exit(2)
 
+    import ctypes as C
     from main_amd import systems as S
+    from main_amd._lib import check, device_synchronize, lib
     from main_amd.collective import CollectiveEngine
     from main_amd.tour import cell_tour
+
+    if args.probe_copy:
+        ms = C.c_double(0)
+        check(lib().mpcx_probe_copy(C.c_int64(PROBE_BYTES), C.c_int32(3), C.byref(ms)), "mpcx_probe_copy")
+        print(f"# copy probe: {2 * PROBE_BYTES / (ms.value * 1e-3) / 1e9:.0f} GB/s", file=sys.stderr, flush=True)
 
     p = headline_params(args.size)
     t0 = time.time()
@@ -87,21 +164,27 @@ def main():
     eng.set_state(s, tour=tour)
     eng.setup(do_sort=True)
     eng.run(args.warmup)
-    print(f"# warmup {args.warmup} intervals done", file=sys.stderr, flush=True)
-    from main_amd._lib import device_synchronize
     device_synchronize()
+    print(f"# warmup {args.warmup} intervals done", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     eng.run(args.steps)
     device_synchronize()
     dt = time.perf_counter() - t0
-    run_ms, bulk_ms, near_ms = eng.timing()
-    print("# near engine per launch:", eng.near_profile(), file=sys.stderr, flush=True)
+    kms = eng.kernel_ms()
+    prof = eng.near_profile()
+    print("# kernel ms per launch:", kms, file=sys.stderr, flush=True)
+    print("# near engine per launch:", prof, file=sys.stderr, flush=True)
     st = eng.get_state()
+    eng.close()
     NT = p["NT"]
     value = NT * args.steps / dt
     ms_step = dt * 1000.0 / args.steps
-    # roofline of the collision pipeline over all particles (bulk kernels of one interval)
-    achieved = BYTES_PER_UPDATE_FP64 * NT / (bulk_ms * 1e-3) / 1e9
+    # roofline of the dominant HBM-bound kernel of the interval
+    hbm = {k: v for k, v in kms.items() if k in ALGO_BYTES}
+    dom = max(hbm, key=hbm.get)
+    launch_s = hbm[dom] * 1e-3
+    achieved = ALGO_BYTES[dom] * NT / launch_s / 1e9
+    traffic = read_traffic(dom)
     out = {
         "metric": "particle-updates/s (MD+MPC)",
         "value": value,
@@ -115,20 +198,19 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "collective_code plain FP64, 100^3 cells with z walls, 10/cell "
+        "config": {"workload": f"collective_code plain FP64, {args.size}^3 cells with z walls, 10/cell "
                                f"({NT} solvent), 1 dimer, skin 0.3, tau 0.5 = 50 x 0.01, sort every 10 collisions",
                    "rng": "device Philox4x32-10", "md_steps_per_collision": p["MPCtstep"],
                    "particle_md_updates_per_s": value * p["MPCtstep"], "cstep": st["cstep"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "collision pipeline over all particles (co_bulk..co_rotate), 100 B/particle",
-                     "bulk_ms": bulk_ms, "near_engine_ms": near_ms},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": dom, "algorithmic_bytes_per_particle": ALGO_BYTES[dom],
+                     "launch_ms": hbm[dom], "kernel_ms": kms},
     }
-    eng.close()
     if not args.no_cpu_baseline:
         try:
-            out["cpu_baseline"] = cpu_baseline(p, s)
-        except Exception as ex:  # the oracle library is absent
+            out["cpu_baseline"] = cpu_baseline(args.size)
+        except Exception as ex:  # neither CPU build is present
             out["cpu_baseline"] = {"value": None, "error": str(ex)}
     print(json.dumps(out), flush=True)
 

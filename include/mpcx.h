@@ -38,6 +38,11 @@ const char* mpcx_last_error(void);
 int mpcx_device_count(void);
 /* hipDeviceSynchronize on the current device */
 int mpcx_device_synchronize(void);
+/* Measurement aid with no counterpart in the reference: copies nbytes between
+ * two device buffers reps times (after one warm-up copy) and returns the mean
+ * time of one copy.  Its hardware fetch/write counter readings calibrate the
+ * HBM traffic figures of the engine's kernels. */
+int mpcx_probe_copy(int64_t nbytes, int32_t reps, double* ms_per_copy);
 
 /* Host Mersenne Twister streams (parity mode draws).
  * mpcx_mtmod_*: sgrnd/grnd of collective_code/mtmod.f90:72-144 (state mt[624], mti).
@@ -164,6 +169,12 @@ int mpcx_co_timing(mpcx_co* e, double* run_ms, double* bulk_ms, double* near_ms)
  * (serial) sections; [4] rebuild count; [5] last near-set size; [6] last
  * inside-list size; [7] launches. */
 int mpcx_co_near_profile(mpcx_co* e, uint64_t* prof8);
+/* Mean duration (ms) of each kernel of the collision interval over the last
+ * mpcx_co_run, from HIP events on the engine's stream (at most 256 intervals
+ * are timed): [0] near engine, [1] co_bulk (streaming replay + binning; at a
+ * re-sort these run as two launches around the sort, which are not timed),
+ * [2] co_collide, [3] co_rotate, [4] co_scatter; count = timed intervals. */
+int mpcx_co_kernel_ms(mpcx_co* e, double* ms5, int64_t* count);
 
 #ifdef __cplusplus
 }

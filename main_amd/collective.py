@@ -183,6 +183,15 @@ class CollectiveEngine:
                     particle_us=us(a[2]) / n, serial_us=us(a[3]) / n, rebuilds=int(a[4]) / n,
                     near_num=int(a[5]), inside_num=int(a[6]))
 
+    KERNELS = ("co_near_engine", "co_bulk", "co_collide", "co_rotate", "co_scatter")
+
+    def kernel_ms(self) -> dict:
+        """Mean duration (ms) of each interval kernel over the last run (HIP events)."""
+        a = np.zeros(5)
+        n = C.c_int64(0)
+        check(lib().mpcx_co_kernel_ms(self.h, ptr(a), C.byref(n)), "mpcx_co_kernel_ms")
+        return dict(zip(self.KERNELS, a.tolist()), intervals=int(n.value))
+
     def timing(self):
         r, b, n = C.c_double(), C.c_double(), C.c_double()
         check(lib().mpcx_co_timing(self.h, C.byref(r), C.byref(b), C.byref(n)), "mpcx_co_timing")

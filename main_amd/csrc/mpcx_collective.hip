@@ -80,11 +80,14 @@ struct CoD {
   double* dim;            // [12][MAXS], sphere s = k + i*nd
   CoScalars* sc;
   const int* react_box;
-  int* near_list;
-  int* near_memb;         // scratch of the classification
+  int* near_list;         // near set of the interval, ascending particle index
+  double *ns_x, *ns_y, *ns_z, *ns_vx, *ns_vy, *ns_vz;   // near slots, by near index
+  int* ns_flag;
+  unsigned char* ns_stat;
   unsigned* near_bits;
   unsigned char* status;
-  int* in_list;
+  int* in_list;           // inside list as near indices
+  int* in_plist;          // the same entries as particle indices (written at launch end)
   unsigned* in_memb;
   double* in_fp;          // [(m*3+c)*in_cap + q]
   double* in_pot;         // [m*in_cap + q]
@@ -177,54 +180,78 @@ __device__ __forceinline__ unsigned memberships(double x, double y, double z, co
 }
 
 // ------------------------------------------------------------- dimer (thread 0)
+// The dimer state of one system, held in registers while thread 0 integrates
+// it (all indices are compile-time after unrolling).  Sphere s = k + i*ND.
+template <int ND>
+struct Dimer {
+  double v[12][2 * ND];
+};
+
+template <int ND>
+__device__ __forceinline__ void dimer_load(Dimer<ND>& d, const double (*sd)[MAXS]) {
+#pragma unroll
+  for (int f = 0; f < 12; f++)
+#pragma unroll
+    for (int s = 0; s < 2 * ND; s++) d.v[f][s] = sd[f][s];
+}
+template <int ND>
+__device__ __forceinline__ void dimer_store(const Dimer<ND>& d, double (*sd)[MAXS]) {
+#pragma unroll
+  for (int f = 0; f < 12; f++)
+#pragma unroll
+    for (int s = 0; s < 2 * ND; s++) sd[f][s] = d.v[f][s];
+}
+
 // evolve_mod.f95:325-415: Q = V + h F, advance, RATTLE on the bond length.
 template <int ND>
-__device__ __attribute__((noinline)) void rattle_positions(const CoP& P, double (*sd)[MAXS], int* err) {
+__device__ __forceinline__ void rattle_positions(const CoP& P, Dimer<ND>& d, int* err) {
   const double MDt = P.MDt, L0 = P.box[0], L1 = P.box[1];
+#pragma unroll
   for (int k = 0; k < ND; k++) {
     const int s0 = k, s1 = k + ND;
-    double prX[2] = {sd[DX][s0], sd[DX][s1]}, prY[2] = {sd[DY][s0], sd[DY][s1]},
-           prZ[2] = {sd[DZ][s0], sd[DZ][s1]};
-    double dP0 = pbc_half(sd[DX][s0] - sd[DX][s1], L0);
-    double dP1 = pbc_half(sd[DY][s0] - sd[DY][s1], L1);
-    double dP2 = sd[DZ][s0] - sd[DZ][s1];
-    sd[DQX][s0] = sd[DVX][s0] + P.hc * sd[DFX][s0];
-    sd[DQY][s0] = sd[DVY][s0] + P.hc * sd[DFY][s0];
-    sd[DQZ][s0] = sd[DVZ][s0] + P.hc * sd[DFZ][s0];
-    sd[DQX][s1] = sd[DVX][s1] + P.hn * sd[DFX][s1];
-    sd[DQY][s1] = sd[DVY][s1] + P.hn * sd[DFY][s1];
-    sd[DQZ][s1] = sd[DVZ][s1] + P.hn * sd[DFZ][s1];
+    double prX[2] = {d.v[DX][s0], d.v[DX][s1]}, prY[2] = {d.v[DY][s0], d.v[DY][s1]},
+           prZ[2] = {d.v[DZ][s0], d.v[DZ][s1]};
+    double dP0 = pbc_half(d.v[DX][s0] - d.v[DX][s1], L0);
+    double dP1 = pbc_half(d.v[DY][s0] - d.v[DY][s1], L1);
+    double dP2 = d.v[DZ][s0] - d.v[DZ][s1];
+    d.v[DQX][s0] = d.v[DVX][s0] + P.hc * d.v[DFX][s0];
+    d.v[DQY][s0] = d.v[DVY][s0] + P.hc * d.v[DFY][s0];
+    d.v[DQZ][s0] = d.v[DVZ][s0] + P.hc * d.v[DFZ][s0];
+    d.v[DQX][s1] = d.v[DVX][s1] + P.hn * d.v[DFX][s1];
+    d.v[DQY][s1] = d.v[DVY][s1] + P.hn * d.v[DFY][s1];
+    d.v[DQZ][s1] = d.v[DVZ][s1] + P.hn * d.v[DFZ][s1];
+#pragma unroll
     for (int i = 0; i < 2; i++) {
-      int s = k + i * ND;
-      double x = sd[DX][s] + MDt * sd[DQX][s];
-      double y = sd[DY][s] + MDt * sd[DQY][s];
-      sd[DZ][s] = sd[DZ][s] + MDt * sd[DQZ][s];
-      sd[DX][s] = x - L0 * floor(x / L0);
-      sd[DY][s] = y - L1 * floor(y / L1);
+      const int s = k + i * ND;
+      double x = d.v[DX][s] + MDt * d.v[DQX][s];
+      double y = d.v[DY][s] + MDt * d.v[DQY][s];
+      d.v[DZ][s] = d.v[DZ][s] + MDt * d.v[DQZ][s];
+      d.v[DX][s] = x - L0 * floor(x / L0);
+      d.v[DY][s] = y - L1 * floor(y / L1);
     }
     for (long long acc = 0;; acc++) {
       if (acc == RATTLE_CAP) { raise_error(err, ERR_RATTLE); break; }
-      double d0 = pbc_half(sd[DX][s0] - sd[DX][s1], L0);
-      double d1 = pbc_half(sd[DY][s0] - sd[DY][s1], L1);
-      double d2 = sd[DZ][s0] - sd[DZ][s1];
+      double d0 = pbc_half(d.v[DX][s0] - d.v[DX][s1], L0);
+      double d1 = pbc_half(d.v[DY][s0] - d.v[DY][s1], L1);
+      double d2 = d.v[DZ][s0] - d.v[DZ][s1];
       double c1 = (d0 * d0 + d1 * d1 + d2 * d2) - P.Dcn2;
       if (fabs(c1) < 1e-6) break;
       double G = -(((0.25 * c1) * P.IIMcIMn) / (d0 * dP0 + d1 * dP1 + d2 * dP2));
       double g2 = 2.0 * G;
-      double x0 = (prX[0] + sd[DQX][s0] * MDt) + (g2 * dP0) / P.Mc;
-      double y0 = (prY[0] + sd[DQY][s0] * MDt) + (g2 * dP1) / P.Mc;
-      double z0 = (prZ[0] + sd[DQZ][s0] * MDt) + (g2 * dP2) / P.Mc;
-      double x1 = (prX[1] + sd[DQX][s1] * MDt) - (g2 * dP0) / P.Mn;
-      double y1 = (prY[1] + sd[DQY][s1] * MDt) - (g2 * dP1) / P.Mn;
-      double z1 = (prZ[1] + sd[DQZ][s1] * MDt) - (g2 * dP2) / P.Mn;
-      sd[DX][s0] = x0 - L0 * floor(x0 / L0); sd[DY][s0] = y0 - L1 * floor(y0 / L1); sd[DZ][s0] = z0;
-      sd[DX][s1] = x1 - L0 * floor(x1 / L0); sd[DY][s1] = y1 - L1 * floor(y1 / L1); sd[DZ][s1] = z1;
-      sd[DQX][s0] = sd[DQX][s0] + (g2 * dP0) / (P.Mc * MDt);
-      sd[DQY][s0] = sd[DQY][s0] + (g2 * dP1) / (P.Mc * MDt);
-      sd[DQZ][s0] = sd[DQZ][s0] + (g2 * dP2) / (P.Mc * MDt);
-      sd[DQX][s1] = sd[DQX][s1] - (g2 * dP0) / (P.Mn * MDt);
-      sd[DQY][s1] = sd[DQY][s1] - (g2 * dP1) / (P.Mn * MDt);
-      sd[DQZ][s1] = sd[DQZ][s1] - (g2 * dP2) / (P.Mn * MDt);
+      double x0 = (prX[0] + d.v[DQX][s0] * MDt) + (g2 * dP0) / P.Mc;
+      double y0 = (prY[0] + d.v[DQY][s0] * MDt) + (g2 * dP1) / P.Mc;
+      double z0 = (prZ[0] + d.v[DQZ][s0] * MDt) + (g2 * dP2) / P.Mc;
+      double x1 = (prX[1] + d.v[DQX][s1] * MDt) - (g2 * dP0) / P.Mn;
+      double y1 = (prY[1] + d.v[DQY][s1] * MDt) - (g2 * dP1) / P.Mn;
+      double z1 = (prZ[1] + d.v[DQZ][s1] * MDt) - (g2 * dP2) / P.Mn;
+      d.v[DX][s0] = x0 - L0 * floor(x0 / L0); d.v[DY][s0] = y0 - L1 * floor(y0 / L1); d.v[DZ][s0] = z0;
+      d.v[DX][s1] = x1 - L0 * floor(x1 / L0); d.v[DY][s1] = y1 - L1 * floor(y1 / L1); d.v[DZ][s1] = z1;
+      d.v[DQX][s0] = d.v[DQX][s0] + (g2 * dP0) / (P.Mc * MDt);
+      d.v[DQY][s0] = d.v[DQY][s0] + (g2 * dP1) / (P.Mc * MDt);
+      d.v[DQZ][s0] = d.v[DQZ][s0] + (g2 * dP2) / (P.Mc * MDt);
+      d.v[DQX][s1] = d.v[DQX][s1] - (g2 * dP0) / (P.Mn * MDt);
+      d.v[DQY][s1] = d.v[DQY][s1] - (g2 * dP1) / (P.Mn * MDt);
+      d.v[DQZ][s1] = d.v[DQZ][s1] - (g2 * dP2) / (P.Mn * MDt);
     }
   }
 }
@@ -275,172 +302,366 @@ __device__ __forceinline__ void lj_mot_wall(const CoP& P, double zd, int i, doub
 }
 
 // evolve_mod.f95:605-636 (solvent sums arrive as tree-reduced totals) and
-// :679-724 (velocity RATTLE).  sumF[s][0..2] = sum of Fpair, sumF[s][3] = sum of Potential.
+// :679-724 (velocity RATTLE).  sumF[4*s + 0..2] = sum of Fpair of sphere s,
+// sumF[4*s + 3] = sum of its Potential.
 template <int ND>
-__device__ __attribute__((noinline)) void dimer_forces_and_rattle_v(const CoP& P, double (*sd)[MAXS],
-                                                                   const double (*sumF)[4], double& tpot, int* err) {
-  double tpp[MAXS];
+__device__ __forceinline__ void dimer_forces_and_rattle_v(const CoP& P, Dimer<ND>& d, const double* sumF,
+                                                          double& tpot, int* err) {
+  double tpp[2 * ND];
+#pragma unroll
   for (int k = 0; k < ND; k++)
+#pragma unroll
     for (int i = 0; i < 2; i++) {
-      int s = k + i * ND;
+      const int s = k + i * ND;
       double fw, wp;
-      lj_mot_wall(P, sd[DZ][s], i, fw, wp);
-      double fx = 0.0 - sumF[s][0], fy = 0.0 - sumF[s][1], fz = 0.0 - sumF[s][2];
-      double tp = 0.0 + sumF[s][3];
+      lj_mot_wall(P, d.v[DZ][s], i, fw, wp);
+      double fx = 0.0 - sumF[4 * s + 0], fy = 0.0 - sumF[4 * s + 1], fz = 0.0 - sumF[4 * s + 2];
+      double tp = 0.0 + sumF[4 * s + 3];
+#pragma unroll
       for (int j = 0; j < ND; j++)
+#pragma unroll
         for (int l = 0; l < 2; l++) {
           if (j == k) continue;   // Fmot(k,i,k,l) stays zero (evolve_mod.f95:580)
-          int b = j + l * ND;
-          int kind = (i == 0 && l == 0) ? 0 : (i == 1 && l == 1) ? 2 : 1;
+          const int b = j + l * ND;
+          const int kind = (i == 0 && l == 0) ? 0 : (i == 1 && l == 1) ? 2 : 1;
           double mx, my, mz, mp;
-          lj_mot_mot(P, sd[DX][s], sd[DY][s], sd[DZ][s], sd[DX][b], sd[DY][b], sd[DZ][b], kind, mx, my, mz, mp);
+          lj_mot_mot(P, d.v[DX][s], d.v[DY][s], d.v[DZ][s], d.v[DX][b], d.v[DY][b], d.v[DZ][b], kind, mx, my, mz,
+                     mp);
           fx = fx + mx; fy = fy + my; fz = fz + mz;
           if (j < k) tp = tp + mp;
         }
       fz = fz + fw;
       tp = tp + wp;
-      sd[DFX][s] = fx; sd[DFY][s] = fy; sd[DFZ][s] = fz;
+      d.v[DFX][s] = fx; d.v[DFY][s] = fy; d.v[DFZ][s] = fz;
       tpp[s] = tp;
     }
   tpot = 0.0;
+#pragma unroll
   for (int k = 0; k < ND; k++)
+#pragma unroll
     for (int i = 0; i < 2; i++) tpot = tpot + tpp[k + i * ND];
+#pragma unroll
   for (int k = 0; k < ND; k++) {
     const int s0 = k, s1 = k + ND;
-    sd[DVX][s0] = sd[DQX][s0] + P.hc * sd[DFX][s0];
-    sd[DVY][s0] = sd[DQY][s0] + P.hc * sd[DFY][s0];
-    sd[DVZ][s0] = sd[DQZ][s0] + P.hc * sd[DFZ][s0];
-    sd[DVX][s1] = sd[DQX][s1] + P.hn * sd[DFX][s1];
-    sd[DVY][s1] = sd[DQY][s1] + P.hn * sd[DFY][s1];
-    sd[DVZ][s1] = sd[DQZ][s1] + P.hn * sd[DFZ][s1];
+    d.v[DVX][s0] = d.v[DQX][s0] + P.hc * d.v[DFX][s0];
+    d.v[DVY][s0] = d.v[DQY][s0] + P.hc * d.v[DFY][s0];
+    d.v[DVZ][s0] = d.v[DQZ][s0] + P.hc * d.v[DFZ][s0];
+    d.v[DVX][s1] = d.v[DQX][s1] + P.hn * d.v[DFX][s1];
+    d.v[DVY][s1] = d.v[DQY][s1] + P.hn * d.v[DFY][s1];
+    d.v[DVZ][s1] = d.v[DQZ][s1] + P.hn * d.v[DFZ][s1];
     for (long long acc = 0;; acc++) {
       if (acc == RATTLE_CAP) { raise_error(err, ERR_RATTLE); break; }
-      double dvx = sd[DVX][s0] - sd[DVX][s1], dvy = sd[DVY][s0] - sd[DVY][s1], dvz = sd[DVZ][s0] - sd[DVZ][s1];
-      double d0 = pbc_half(sd[DX][s0] - sd[DX][s1], P.box[0]);
-      double d1 = pbc_half(sd[DY][s0] - sd[DY][s1], P.box[1]);
-      double d2 = sd[DZ][s0] - sd[DZ][s1];
+      double dvx = d.v[DVX][s0] - d.v[DVX][s1], dvy = d.v[DVY][s0] - d.v[DVY][s1], dvz = d.v[DVZ][s0] - d.v[DVZ][s1];
+      double d0 = pbc_half(d.v[DX][s0] - d.v[DX][s1], P.box[0]);
+      double d1 = pbc_half(d.v[DY][s0] - d.v[DY][s1], P.box[1]);
+      double d2 = d.v[DZ][s0] - d.v[DZ][s1];
       double c2 = 2.0 * (dvx * d0 + dvy * d1 + dvz * d2);
       if (fabs(c2) < 1e-8) break;
       double O = -((0.25 * (P.IIMcIMn * c2)) / P.Dcn2);
       double o2 = 2.0 * O;
-      sd[DVX][s0] = sd[DVX][s0] + (o2 * d0) / P.Mc;
-      sd[DVY][s0] = sd[DVY][s0] + (o2 * d1) / P.Mc;
-      sd[DVZ][s0] = sd[DVZ][s0] + (o2 * d2) / P.Mc;
-      sd[DVX][s1] = sd[DVX][s1] - (o2 * d0) / P.Mn;
-      sd[DVY][s1] = sd[DVY][s1] - (o2 * d1) / P.Mn;
-      sd[DVZ][s1] = sd[DVZ][s1] - (o2 * d2) / P.Mn;
+      d.v[DVX][s0] = d.v[DVX][s0] + (o2 * d0) / P.Mc;
+      d.v[DVY][s0] = d.v[DVY][s0] + (o2 * d1) / P.Mc;
+      d.v[DVZ][s0] = d.v[DVZ][s0] + (o2 * d2) / P.Mc;
+      d.v[DVX][s1] = d.v[DVX][s1] - (o2 * d0) / P.Mn;
+      d.v[DVY][s1] = d.v[DVY][s1] - (o2 * d1) / P.Mn;
+      d.v[DVZ][s1] = d.v[DVZ][s1] - (o2 * d2) / P.Mn;
     }
   }
 }
 
 // ------------------------------------------------------------- near engine
-constexpr int LCAP = 2304;   // inside-list entries staged in LDS (the rest stay in HBM)
+// One workgroup advances the near set of one system through a whole collision
+// interval.  The near particles are gathered once per interval into
+// contiguous slot arrays (ns_*, by near index q, ascending particle index;
+// they stay in L2), so list rebuilds read coalesced data.  The inside list
+// holds near indices; the state of its entries lives in LDS (entry k < LCAP)
+// or in the HBM slot arrays sl_* (k >= LCAP) for all MD steps between two
+// rebuilds.  Rebuild partition: wave w owns near indices
+// [w*64*E, (w+1)*64*E), lane-interleaved (q = w*64*E + j*64 + lane), so the
+// order of a compaction follows ballots within a wave and one scan over the
+// waves.
+constexpr int LCAP = 2304;     // inside entries staged in LDS (the rest stay in HBM)
+constexpr int NE_EMAX = 64;    // near entries per lane: near set <= NE_T * NE_EMAX
+constexpr int NE_NEAR = NE_T * NE_EMAX;
+constexpr int NE_W = NE_T / 64;
 struct NearShared {
   double lx[LCAP], ly[LCAP], lz[LCAP], lvx[LCAP], lvy[LCAP], lvz[LCAP], lmsd[LCAP];
   int lflag[LCAP];
   unsigned lmemb[LCAP];
   double sd[12][MAXS];
-  double red[NE_T / 64][4 * MAXS + 1];
+  double red[NE_W][4 * MAXS + 1];
   double fin[4 * MAXS + 1];
-  unsigned long long scan[NE_T / 64 + 1];
+  int wsum[NE_W + 1];
   double vmax2;
-  int cstep, tc, tp, in_num, nev, near_num, rebuild, stop;
+  int cstep, tc, tp, in_num, nev, near_num, rebuild;
+  CoP P;   // launch parameters, read from LDS by the helpers below
+  CoD D;
+};
+__shared__ NearShared g_ne;
+
+// 4-bit memberships of a lane's near entries (entry j in nibble j)
+struct Memb {
+  unsigned long long w[NE_EMAX / 16];
+  __device__ __forceinline__ void clear() {
+#pragma unroll
+    for (int i = 0; i < NE_EMAX / 16; i++) w[i] = 0ull;
+  }
+  __device__ __forceinline__ void set(int j, unsigned m) {
+#pragma unroll
+    for (int i = 0; i < NE_EMAX / 16; i++)
+      if ((j >> 4) == i) w[i] |= (unsigned long long)(m & 15u) << (4 * (j & 15));
+  }
+  __device__ __forceinline__ unsigned get(int j) const {
+    unsigned r = 0;
+#pragma unroll
+    for (int i = 0; i < NE_EMAX / 16; i++)
+      if ((j >> 4) == i) r = (unsigned)(w[i] >> (4 * (j & 15))) & 15u;
+    return r;
+  }
 };
 
-// Membership pass over the near list (strided, coalesced): the lazy streaming
-// of the near particles outside the lists (evolve_mod.f95:467-529) when
-// do_stream, then the neighbour-list membership of every near particle at its
-// new position (neighbour_list_mod.f95:38-57, 129-167).  Also max v^2 over the
-// near set (evolve_mod.f95:555-562).
+__device__ __forceinline__ unsigned long long lanes_below() {
+  const int lane = threadIdx.x & 63;
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// exclusive scan of the per-wave counts in g_ne.wsum (thread 0), total in wsum[NE_W]
+__device__ __forceinline__ void ne_wave_scan() {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int w = 0; w < NE_W; w++) { int t = g_ne.wsum[w]; g_ne.wsum[w] = run; run += t; }
+    g_ne.wsum[NE_W] = run;
+  }
+  __syncthreads();
+}
+
+// particle arrays -> near slots (interval start) and back (interval end)
+__device__ __forceinline__ void ne_gather(const CoD& D, int nn) {
+  for (int q0 = threadIdx.x; q0 < nn; q0 += 4 * NE_T) {
+    int p[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) { int q = q0 + u * NE_T; p[u] = q < nn ? D.near_list[q] : -1; }
+    double x[4], y[4], z[4], vx[4], vy[4], vz[4];
+    int f[4];
+    unsigned char st[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (p[u] >= 0) {
+        x[u] = D.x[p[u]]; y[u] = D.y[p[u]]; z[u] = D.z[p[u]];
+        vx[u] = D.vx[p[u]]; vy[u] = D.vy[p[u]]; vz[u] = D.vz[p[u]];
+        f[u] = D.flag[p[u]]; st[u] = D.status[p[u]];
+      }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (p[u] >= 0) {
+        int q = q0 + u * NE_T;
+        D.ns_x[q] = x[u]; D.ns_y[q] = y[u]; D.ns_z[q] = z[u];
+        D.ns_vx[q] = vx[u]; D.ns_vy[q] = vy[u]; D.ns_vz[q] = vz[u];
+        D.ns_flag[q] = f[u]; D.ns_stat[q] = st[u];
+      }
+  }
+}
+__device__ __forceinline__ void ne_scatter(const CoD& D, int nn) {
+  for (int q0 = threadIdx.x; q0 < nn; q0 += 4 * NE_T) {
+    int p[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) { int q = q0 + u * NE_T; p[u] = q < nn ? D.near_list[q] : -1; }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (p[u] >= 0) {
+        int q = q0 + u * NE_T;
+        D.x[p[u]] = D.ns_x[q]; D.y[p[u]] = D.ns_y[q]; D.z[p[u]] = D.ns_z[q];
+        D.vx[p[u]] = D.ns_vx[q]; D.vy[p[u]] = D.ns_vy[q]; D.vz[p[u]] = D.ns_vz[q];
+        D.flag[p[u]] = D.ns_flag[q]; D.status[p[u]] = D.ns_stat[q];
+      }
+  }
+}
+
+// inside entry k <-> slot storage (LDS below LCAP, HBM above)
+__device__ __forceinline__ void slot_from_ns(const CoD& D, int k, int q) {
+  if (k < LCAP) {
+    g_ne.lx[k] = D.ns_x[q]; g_ne.ly[k] = D.ns_y[q]; g_ne.lz[k] = D.ns_z[q];
+    g_ne.lvx[k] = D.ns_vx[q]; g_ne.lvy[k] = D.ns_vy[q]; g_ne.lvz[k] = D.ns_vz[q];
+    g_ne.lflag[k] = D.ns_flag[q];
+  } else {
+    D.sl_x[k] = D.ns_x[q]; D.sl_y[k] = D.ns_y[q]; D.sl_z[k] = D.ns_z[q];
+    D.sl_vx[k] = D.ns_vx[q]; D.sl_vy[k] = D.ns_vy[q]; D.sl_vz[k] = D.ns_vz[q];
+    D.sl_flag[k] = D.ns_flag[q];
+  }
+}
+__device__ __forceinline__ void ne_store_slots(const CoD& D, int n) {
+  for (int k = threadIdx.x; k < n; k += NE_T) {
+    const int q = D.in_list[k];
+    if (k < LCAP) {
+      D.ns_x[q] = g_ne.lx[k]; D.ns_y[q] = g_ne.ly[k]; D.ns_z[q] = g_ne.lz[k];
+      D.ns_vx[q] = g_ne.lvx[k]; D.ns_vy[q] = g_ne.lvy[k]; D.ns_vz[q] = g_ne.lvz[k];
+      D.ns_flag[q] = g_ne.lflag[k];
+    } else {
+      D.ns_x[q] = D.sl_x[k]; D.ns_y[q] = D.sl_y[k]; D.ns_z[q] = D.sl_z[k];
+      D.ns_vx[q] = D.sl_vx[k]; D.ns_vy[q] = D.sl_vy[k]; D.ns_vz[q] = D.sl_vz[k];
+      D.ns_flag[q] = D.sl_flag[k];
+    }
+  }
+}
+
+// Rebuild, first half: the lazy streaming of the near particles outside the
+// lists (evolve_mod.f95:467-529) when do_stream, the neighbour-list
+// memberships of every near particle at its position (neighbour_list_mod.f95:
+// 38-57, 129-167) when do_memb, and max v^2 over the near set
+// (evolve_mod.f95:555-562).  Returns the wave's count of list members.
 template <int ND>
-__device__ __attribute__((noinline)) void ne_memb_pass(const CoP& P, const CoD& D, NearShared& S, bool do_stream,
-                                                       bool do_memb) {
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int nn = S.near_num;
-  const double tc = (double)S.tc;
+__device__ __forceinline__ int ne_memb_pass(const CoP& P, const CoD& D, bool do_stream, bool do_memb, Memb& mk) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int nn = g_ne.near_num;
+  const int E = (nn + NE_T - 1) / NE_T;
+  const int qb = wv * 64 * E + lane;
+  const double tc = (double)g_ne.tc;
   double vm = 0.0;
-  for (int q0 = tid; q0 < nn; q0 += 2 * NE_T) {
-    int qq[2] = {q0, q0 + NE_T};
-    int pp[2];
+  int cnt = 0;
+  mk.clear();
+  for (int j0 = 0; j0 < E; j0 += 2) {
+    int qq[2];
     double x[2], y[2], z[2], vx[2], vy[2], vz[2];
     int f[2];
     unsigned char st[2];
 #pragma unroll
     for (int u = 0; u < 2; u++) {
-      pp[u] = qq[u] < nn ? D.near_list[qq[u]] : -1;
-    }
-#pragma unroll
-    for (int u = 0; u < 2; u++) {
-      if (pp[u] < 0) continue;
-      int p = pp[u];
-      x[u] = D.x[p]; y[u] = D.y[p]; z[u] = D.z[p];
-      vx[u] = D.vx[p]; vy[u] = D.vy[p]; vz[u] = D.vz[p];
-      f[u] = D.flag[p];
-      st[u] = D.status[p];
-    }
-#pragma unroll
-    for (int u = 0; u < 2; u++) {
-      if (pp[u] < 0) continue;
-      int p = pp[u];
-      if (do_stream && !(st[u] & ST_INSIDE)) {
-        stream_particle(x[u], y[u], z[u], vx[u], vy[u], vz[u], f[u], tc, P, D.react_box, &D.sc->err);
-        D.x[p] = x[u]; D.y[p] = y[u]; D.z[p] = z[u];
-        D.vx[p] = vx[u]; D.vy[p] = vy[u]; D.vz[p] = vz[u];
-        D.flag[p] = f[u];
+      qq[u] = (j0 + u < E) ? qb + (j0 + u) * 64 : nn;
+      if (qq[u] < nn) {
+        const int q = qq[u];
+        x[u] = D.ns_x[q]; y[u] = D.ns_y[q]; z[u] = D.ns_z[q];
+        vx[u] = D.ns_vx[q]; vy[u] = D.ns_vy[q]; vz[u] = D.ns_vz[q];
+        f[u] = D.ns_flag[q]; st[u] = D.ns_stat[q];
       }
-      vm = fmax(vm, (vx[u] * vx[u] + vy[u] * vy[u]) + vz[u] * vz[u]);
-      if (do_memb) D.near_memb[qq[u]] = (int)memberships<ND>(x[u], y[u], z[u], P, S.sd);
+    }
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const bool valid = qq[u] < nn;
+      unsigned m = 0;
+      if (valid) {
+        const int q = qq[u];
+        if (do_stream && !(st[u] & ST_INSIDE)) {
+          const int f0 = f[u];
+          bool b = stream_particle(x[u], y[u], z[u], vx[u], vy[u], vz[u], f[u], tc, P, D.react_box, &D.sc->err);
+          D.ns_x[q] = x[u]; D.ns_y[q] = y[u]; D.ns_z[q] = z[u];
+          if (b) { D.ns_vx[q] = vx[u]; D.ns_vy[q] = vy[u]; D.ns_vz[q] = vz[u]; }
+          if (f[u] != f0) D.ns_flag[q] = f[u];
+        }
+        vm = fmax(vm, (vx[u] * vx[u] + vy[u] * vy[u]) + vz[u] * vz[u]);
+        if (do_memb) m = memberships<ND>(x[u], y[u], z[u], P, g_ne.sd);
+      }
+      if (j0 + u < E) {
+        mk.set(j0 + u, m);
+        cnt += __popcll(__ballot(m != 0u));
+      }
     }
   }
   vm = wave_max(vm);
-  if (lane == 0) S.red[wv][0] = vm;
+  if (lane == 0) g_ne.red[wv][0] = vm;
   __syncthreads();
-  if (tid == 0) {
+  if (threadIdx.x == 0 && do_stream) {
     double m = 0.0;
-    for (int w = 0; w < NE_T / 64; w++) m = fmax(m, S.red[w][0]);
-    if (do_stream) S.vmax2 = m;
+    for (int w = 0; w < NE_W; w++) m = fmax(m, g_ne.red[w][0]);
+    g_ne.vmax2 = m;
+  }
+  return cnt;
+}
+
+// Rebuild, second half: the inside list in ascending particle order (as
+// inout_particle_list builds it at one thread), its memberships, its slot
+// state, and time_penetrate (evolve_mod.f95:563).
+template <int ND>
+__device__ __forceinline__ void ne_compact(const CoP& P, const CoD& D, const Memb& mk, int wcount) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int nn = g_ne.near_num;
+  const int E = (nn + NE_T - 1) / NE_T;
+  const int qb = wv * 64 * E + lane;
+  if (lane == 0) g_ne.wsum[wv] = wcount;
+  ne_wave_scan();
+  int total = g_ne.wsum[NE_W];
+  if (total > D.in_cap) {
+    if (threadIdx.x == 0) raise_error(&D.sc->err, ERR_INSIDE_CAP);
+    total = D.in_cap;
+  }
+  int base = g_ne.wsum[wv];
+  const unsigned long long lt = lanes_below();
+  for (int j = 0; j < E; j++) {
+    const int q = qb + j * 64;
+    const unsigned m = mk.get(j);
+    const unsigned long long b = __ballot(m != 0u);
+    const int k = base + __popcll(b & lt);
+    base += __popcll(b);
+    if (q < nn) {
+      if (m != 0u && k < D.in_cap) {
+        D.in_list[k] = q;
+        D.in_memb[k] = m;
+        if (k < LCAP) { g_ne.lmemb[k] = m; g_ne.lmsd[k] = 0.0; } else D.in_msd[k] = 0.0;
+        slot_from_ns(D, k, q);
+      }
+      D.ns_stat[q] = m != 0u ? ST_INSIDE : 0;
+    }
+  }
+  if (threadIdx.x == 0) {
+    g_ne.in_num = total;
+    double fm = __longlong_as_double((long long)D.sc->far_maxv2);
+    double maxt = fmax(fm, g_ne.vmax2);
+    g_ne.tp = (int)floor(((0.5 * P.skinrad) / sqrt(maxt)) / P.MDt);
+    g_ne.tc = 0;
   }
   __syncthreads();
 }
 
-// Ordered compaction of the memberships into the inside list (ascending
-// particle order, as inout_particle_list builds it at one thread), status
-// bits, and time_penetrate (evolve_mod.f95:563).
-__device__ __attribute__((noinline)) void ne_compact(const CoP& P, const CoD& D, NearShared& S) {
-  const int tid = threadIdx.x;
-  const int nn = S.near_num;
+// One list rebuild over the near set: optionally the inside entries written
+// back to their slots first, then the membership pass and, unless the launch
+// stops here for a sort, the compaction.
+template <int ND>
+__device__ __attribute__((noinline)) void ne_rebuild(bool store_first, bool do_stream, bool compact) {
+  const CoP& P = g_ne.P;
+  const CoD& D = g_ne.D;
+  if (store_first) {
+    ne_store_slots(D, g_ne.in_num);
+    __threadfence_block();
+    __syncthreads();
+  }
+  Memb mk;
+  int c = ne_memb_pass<ND>(P, D, do_stream, compact, mk);
+  if (compact) ne_compact<ND>(P, D, mk, c);
+}
+
+// Interval start: the inside list (unchanged since the last rebuild, same
+// ascending order, so entry k keeps its memberships, stored forces and
+// max_solvent_dist) re-expressed as near indices of this interval's set.
+__device__ __attribute__((noinline)) void ne_map_inside() {
+  const CoD& D = g_ne.D;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int nn = g_ne.near_num;
   const int E = (nn + NE_T - 1) / NE_T;
-  const int q0 = min(tid * E, nn), q1 = min(q0 + E, nn);
-  unsigned long long cnt = 0;
-  for (int q = q0; q < q1; q++) cnt += D.near_memb[q] != 0;
-  unsigned long long total;
-  unsigned long long off = block_excl_scan64(cnt, S.scan, total);
-  if (total > (unsigned long long)D.in_cap) {
-    if (tid == 0) raise_error(&D.sc->err, ERR_INSIDE_CAP);
-    total = D.in_cap;
+  const int qb = wv * 64 * E + lane;
+  int cnt = 0;
+  for (int j = 0; j < E; j++) {
+    const int q = qb + j * 64;
+    cnt += __popcll(__ballot(q < nn && (D.ns_stat[q] & ST_INSIDE)));
   }
-  int w = (int)off;
-  for (int q = q0; q < q1; q++) {
-    int p = D.near_list[q];
-    unsigned m = (unsigned)D.near_memb[q];
-    if (m) {
-      if (__popc(m) > MAXM) raise_error(&D.sc->err, ERR_MEMBERSHIPS);
-      if (w < D.in_cap) {
-        D.in_list[w] = p;
-        D.in_memb[w] = m;
-      }
-      w++;
-      D.status[p] = ST_INSIDE;
-    } else {
-      D.status[p] = 0;
+  if (lane == 0) g_ne.wsum[wv] = cnt;
+  ne_wave_scan();
+  if (g_ne.wsum[NE_W] != g_ne.in_num) {   // every list member must be in the near set
+    if (threadIdx.x == 0) raise_error(&D.sc->err, ERR_INSIDE_LOST);
+  }
+  const int n = min(g_ne.in_num, D.in_cap);
+  int base = g_ne.wsum[wv];
+  const unsigned long long lt = lanes_below();
+  for (int j = 0; j < E; j++) {
+    const int q = qb + j * 64;
+    const bool in = q < nn && (D.ns_stat[q] & ST_INSIDE);
+    const unsigned long long b = __ballot(in);
+    const int k = base + __popcll(b & lt);
+    base += __popcll(b);
+    if (in && k < n) {
+      D.in_list[k] = q;
+      slot_from_ns(D, k, q);
+      if (k < LCAP) { g_ne.lmemb[k] = D.in_memb[k]; g_ne.lmsd[k] = D.in_msd[k]; }
     }
-  }
-  if (tid == 0) {
-    S.in_num = (int)total;
-    double fm = __longlong_as_double((long long)D.sc->far_maxv2);
-    double maxt = fmax(fm, S.vmax2);
-    S.tp = (int)floor(((0.5 * P.skinrad) / sqrt(maxt)) / P.MDt);
-    S.tc = 0;
   }
   __syncthreads();
 }
@@ -448,125 +669,59 @@ __device__ __attribute__((noinline)) void ne_compact(const CoP& P, const CoD& D,
 // thread 0: bookkeeping of the next MD step, RATTLE positions and the
 // rebuild test of evolve_mod.f95:457-465
 template <int ND>
-__device__ __attribute__((noinline)) void ne_begin_step(const CoP& P, const CoD& D, NearShared& S, double maxv) {
-  S.cstep += 1;
-  S.tc += 1;
-  rattle_positions<ND>(P, S.sd, &D.sc->err);
-  S.rebuild = (maxv > P.hs2 || S.tc == S.tp || S.cstep % P.MPCtstep == 0);
-  if (S.rebuild) {
-    if (S.nev < MAXEV) D.sc->events[S.nev] = S.tc; else raise_error(&D.sc->err, ERR_EVENTS);
-    S.nev += 1;
+__device__ __forceinline__ void ne_begin_step_regs(const CoP& P, const CoD& D, Dimer<ND>& d, double maxv) {
+  g_ne.cstep += 1;
+  g_ne.tc += 1;
+  rattle_positions<ND>(P, d, &D.sc->err);
+  const int rb = (maxv > P.hs2 || g_ne.tc == g_ne.tp || g_ne.cstep % P.MPCtstep == 0);
+  g_ne.rebuild = rb;
+  if (rb) {
+    if (g_ne.nev < MAXEV) D.sc->events[g_ne.nev] = g_ne.tc; else raise_error(&D.sc->err, ERR_EVENTS);
+    g_ne.nev += 1;
+#pragma unroll
     for (int s = 0; s < 2 * ND; s++) {   // the near/far split needs every sphere inside its guard
-      double a = pbc_half(S.sd[DX][s] - D.sc->x0[s], P.box[0]);
-      double b = pbc_half(S.sd[DY][s] - D.sc->y0[s], P.box[1]);
-      double c = S.sd[DZ][s] - D.sc->z0[s];
+      double a = pbc_half(d.v[DX][s] - D.sc->x0[s], P.box[0]);
+      double b = pbc_half(d.v[DY][s] - D.sc->y0[s], P.box[1]);
+      double c = d.v[DZ][s] - D.sc->z0[s];
       if (a * a + b * b + c * c > P.guard * P.guard) raise_error(&D.sc->err, ERR_NEAR_GUARD);
     }
   }
 }
-
+template <int ND>
+__device__ __attribute__((noinline)) void ne_begin_step(double maxv) {
+  const CoP& P = g_ne.P;
+  const CoD& D = g_ne.D;
+  Dimer<ND> d;
+  dimer_load<ND>(d, g_ne.sd);
+  ne_begin_step_regs<ND>(P, D, d, maxv);
+  dimer_store<ND>(d, g_ne.sd);
+}
 // thread 0, end of step t: dimer forces, velocity RATTLE and, unless t is the
 // last step of the launch, the start of step t+1
 template <int ND>
-__device__ __attribute__((noinline)) void ne_end_step(const CoP& P, const CoD& D, NearShared& S, bool last) {
+__device__ __attribute__((noinline)) void ne_end_step(bool last) {
+  const CoP& P = g_ne.P;
+  const CoD& D = g_ne.D;
+  Dimer<ND> d;
+  dimer_load<ND>(d, g_ne.sd);
   double tpot;
-  dimer_forces_and_rattle_v<ND>(P, S.sd, (const double(*)[4])S.fin, tpot, &D.sc->err);
+  double fin[4 * 2 * ND + 1];
+#pragma unroll
+  for (int v = 0; v < 4 * 2 * ND + 1; v++) fin[v] = g_ne.fin[v];
+  dimer_forces_and_rattle_v<ND>(P, d, fin, tpot, &D.sc->err);
   D.sc->tpotential = tpot;
-  if (!last) ne_begin_step<ND>(P, D, S, S.fin[4 * 2 * ND]);
+  if (!last) ne_begin_step_regs<ND>(P, D, d, fin[4 * 2 * ND]);
+  dimer_store<ND>(d, g_ne.sd);
 }
 
-// Inside-list entry q lives in LDS when q < LCAP, else in the HBM slot arrays;
-// Slot holds generic pointers to its fields.
-struct Slot {
-  double *x, *y, *z, *vx, *vy, *vz, *msd;
-  int* flag;
-  unsigned* memb;
-};
-__device__ __forceinline__ Slot slot_at(const CoD& D, NearShared& S, int q) {
-  Slot r;
-  if (q < LCAP) {
-    r.x = &S.lx[q]; r.y = &S.ly[q]; r.z = &S.lz[q]; r.vx = &S.lvx[q]; r.vy = &S.lvy[q]; r.vz = &S.lvz[q];
-    r.msd = &S.lmsd[q]; r.flag = &S.lflag[q]; r.memb = &S.lmemb[q];
-  } else {
-    r.x = &D.sl_x[q]; r.y = &D.sl_y[q]; r.z = &D.sl_z[q]; r.vx = &D.sl_vx[q]; r.vy = &D.sl_vy[q];
-    r.vz = &D.sl_vz[q]; r.msd = &D.in_msd[q]; r.flag = &D.sl_flag[q]; r.memb = &D.in_memb[q];
-  }
-  return r;
-}
-// gather the inside entries from the particle arrays (msd is reset by the caller)
-__device__ __forceinline__ void ne_load_slots(const CoD& D, NearShared& S, int n) {
-  for (int q = threadIdx.x; q < n; q += NE_T) {
-    int p = D.in_list[q];
-    Slot r = slot_at(D, S, q);
-    *r.x = D.x[p]; *r.y = D.y[p]; *r.z = D.z[p];
-    *r.vx = D.vx[p]; *r.vy = D.vy[p]; *r.vz = D.vz[p];
-    *r.flag = D.flag[p];
-    if (q < LCAP) *r.memb = D.in_memb[q];
-  }
-}
-__device__ __forceinline__ void ne_store_slots(const CoD& D, NearShared& S, int n) {
-  for (int q = threadIdx.x; q < n; q += NE_T) {
-    int p = D.in_list[q];
-    Slot r = slot_at(D, S, q);
-    D.x[p] = *r.x; D.y[p] = *r.y; D.z[p] = *r.z;
-    D.vx[p] = *r.vx; D.vy[p] = *r.vy; D.vz[p] = *r.vz;
-    D.flag[p] = *r.flag;
-  }
-}
-// max_solvent_dist of the entries: to and from the persistent per-entry array
-__device__ __forceinline__ void ne_msd_io(const CoD& D, NearShared& S, int n, bool load) {
-  for (int q = threadIdx.x; q < n && q < LCAP; q += NE_T) {
-    if (load) S.lmsd[q] = D.in_msd[q]; else D.in_msd[q] = S.lmsd[q];
-  }
-}
-__device__ __forceinline__ void ne_msd_zero(const CoD& D, NearShared& S, int n) {
-  for (int q = threadIdx.x; q < n; q += NE_T) *slot_at(D, S, q).msd = 0.0;
-}
-
-template <int ND>
-__global__ __launch_bounds__(NE_T) void co_near_engine(CoP P, CoD D, int nsteps, int mode) {
-  __shared__ NearShared S;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  constexpr int NSPH = 2 * ND;
-  constexpr int NV = 4 * NSPH + 1;
-  CoScalars* sc = D.sc;
-  const int cap = D.in_cap;
-
-  for (int j = tid; j < 12 * MAXS; j += NE_T) S.sd[j / MAXS][j % MAXS] = D.dim[j];
-  if (tid == 0) {
-    S.cstep = sc->cstep; S.tc = sc->time_count; S.tp = sc->time_penetrate; S.in_num = sc->in_num;
-    S.nev = (mode & M_RESUME) ? sc->n_ev : 0;
-    S.near_num = sc->near_num;
-    S.vmax2 = sc->near_vmax2;
-    S.stop = 0;
-    S.rebuild = 0;
-  }
-  __syncthreads();
-
-  if (mode & M_SETUP) {
-    // main_pattern.f95:173-188 over the near set; near_vmax2 was set by the classifier kernel
-    ne_memb_pass<ND>(P, D, S, false, true);
-    ne_compact(P, D, S);
-    if (tid == 0) {
-      sc->in_num = S.in_num; sc->time_count = S.tc; sc->time_penetrate = S.tp; sc->n_ev = 0;
-      for (int s = 0; s < NSPH; s++) { sc->x0[s] = S.sd[DX][s]; sc->y0[s] = S.sd[DY][s]; sc->z0[s] = S.sd[DZ][s]; }
-    }
-    for (int q = tid; q < S.in_num; q += NE_T) {
-      for (int j = 0; j < MAXM * 3; j++) D.in_fp[j * cap + q] = 0.0;
-      for (int j = 0; j < MAXM; j++) D.in_pot[j * cap + q] = 0.0;
-      D.in_msd[q] = 0.0;
-    }
-    return;
-  }
-
-  // kick (evolve_mod.f95:225-250 and :645-670) in list order, then inside
-  // streaming (:252-316) of one entry
+// kick (evolve_mod.f95:225-250 and :645-670) in list order, then inside
+// streaming (:252-316) of one entry
 #define NE_KICK(vx, vy, vz, fl, mb, fx, fy, fz)                  \
   {                                                              \
     bool ok_;                                                    \
     double h_ = kick_factor(P, fl, ok_);                         \
     if (ok_) {                                                   \
-      _Pragma("unroll") for (int b_ = 0; b_ < NSPH; b_++)        \
+      _Pragma("unroll") for (int b_ = 0; b_ < 2 * ND; b_++)      \
         if (((mb) >> b_) & 1u) {                                 \
           vx = vx + h_ * fx[b_];                                 \
           vy = vy + h_ * fy[b_];                                 \
@@ -576,196 +731,279 @@ __global__ __launch_bounds__(NE_T) void co_near_engine(CoP P, CoD D, int nsteps,
   }
 #define NE_STREAM(x, y, z, vx, vy, vz, fl, ms)                                              \
   {                                                                                        \
-    stream_particle(x, y, z, vx, vy, vz, fl, 1.0, P, D.react_box, &sc->err);               \
+    stream_particle(x, y, z, vx, vy, vz, fl, 1.0, P, D.react_box, &D.sc->err);             \
     double v2_ = (vx * vx + vy * vy) + vz * vz;                                            \
     ms = ms + v2_ * (P.MDt * P.MDt);                                                       \
+  }
+
+// slot field access: LDS (L = true) or the HBM slot arrays
+#define SLOT(L, f, k) (*((L) ? &g_ne.l##f[k] : &D.sl_##f[k]))
+
+// Particle phase of one inside entry: forces (forces_mod.f95:62-136), kick,
+// and, unless this is the launch's last step, the next step's kick and
+// streaming (they do not depend on the dimer); at the last step the pair
+// forces are stored for the first kick of the next interval.
+template <int ND, bool L>
+__device__ __forceinline__ void ne_particle(const CoP& P, const CoD& D, int k, bool last, double (&acc)[8 * ND + 1]) {
+  const int cap = D.in_cap;
+  double x, y, z, vx, vy, vz, ms;
+  int fl;
+  unsigned mb;
+  if constexpr (L) {
+    x = g_ne.lx[k]; y = g_ne.ly[k]; z = g_ne.lz[k]; vx = g_ne.lvx[k]; vy = g_ne.lvy[k]; vz = g_ne.lvz[k];
+    fl = g_ne.lflag[k]; mb = g_ne.lmemb[k]; ms = g_ne.lmsd[k];
+  } else {
+    x = D.sl_x[k]; y = D.sl_y[k]; z = D.sl_z[k]; vx = D.sl_vx[k]; vy = D.sl_vy[k]; vz = D.sl_vz[k];
+    fl = D.sl_flag[k]; mb = D.in_memb[k]; ms = D.in_msd[k];
+  }
+  double fx[2 * ND], fy[2 * ND], fz[2 * ND], pot[2 * ND];
+  // lists in the reference's order: (dimer kk, sphere ii) -> b = 2kk + ii
+#pragma unroll
+  for (int b = 0; b < 2 * ND; b++) {
+    const int kk = b >> 1, ii = b & 1, s = kk + ii * ND;
+    fx[b] = fy[b] = fz[b] = pot[b] = 0.0;
+    if (!((mb >> b) & 1u)) continue;
+    double d1 = pbc_half(x - g_ne.sd[DX][s], P.box[0]);
+    double d2 = pbc_half(y - g_ne.sd[DY][s], P.box[1]);
+    double d3 = z - g_ne.sd[DZ][s];
+    double dRsq = d1 * d1 + d2 * d2 + d3 * d3;
+    if (dRsq <= P.cut2[ii]) {
+      double E = 0.0;
+      bool on = false;
+      if (fl == 0) {
+        if (ii == 0) { fl = 1; E = P.Eb[0]; } else E = P.Ea[1];
+        on = true;
+      } else if (fl == 1) { E = P.Eb[ii]; on = true; }
+      else if (fl == 2) { E = P.Es[ii]; on = true; }
+      if (on) {
+        double F, Ub;
+        lj_terms(dRsq, P.R6[ii], P.R12[ii], P.r12R12[ii], P.r6R6[ii], F, Ub);
+        double EF = E * F;
+        fx[b] = EF * d1; fy[b] = EF * d2; fz[b] = EF * d3; pot[b] = E * Ub;
+      }
+    }
+    acc[4 * s + 0] += fx[b]; acc[4 * s + 1] += fy[b];
+    acc[4 * s + 2] += fz[b]; acc[4 * s + 3] += pot[b];
+  }
+  NE_KICK(vx, vy, vz, fl, mb, fx, fy, fz);
+  if (!last) {
+    NE_KICK(vx, vy, vz, fl, mb, fx, fy, fz);
+    NE_STREAM(x, y, z, vx, vy, vz, fl, ms);
+    acc[8 * ND] = fmax(acc[8 * ND], ms);
+  } else {
+#pragma unroll
+    for (int b = 0; b < 2 * ND; b++) {
+      D.in_fp[(b * 3 + 0) * cap + k] = fx[b];
+      D.in_fp[(b * 3 + 1) * cap + k] = fy[b];
+      D.in_fp[(b * 3 + 2) * cap + k] = fz[b];
+      D.in_pot[b * cap + k] = pot[b];
+    }
+  }
+  SLOT(L, x, k) = x; SLOT(L, y, k) = y; SLOT(L, z, k) = z;
+  SLOT(L, vx, k) = vx; SLOT(L, vy, k) = vy; SLOT(L, vz, k) = vz;
+  SLOT(L, flag, k) = fl;
+  if constexpr (L) g_ne.lmsd[k] = ms; else D.in_msd[k] = ms;
+}
+
+// first step of an interval: kick with the forces stored at the end of the
+// previous interval, then stream
+template <int ND, bool L>
+__device__ __forceinline__ double ne_first_kick(const CoP& P, const CoD& D, int k) {
+  const int cap = D.in_cap;
+  double x = SLOT(L, x, k), y = SLOT(L, y, k), z = SLOT(L, z, k);
+  double vx = SLOT(L, vx, k), vy = SLOT(L, vy, k), vz = SLOT(L, vz, k);
+  int fl = SLOT(L, flag, k);
+  double ms = L ? g_ne.lmsd[k] : D.in_msd[k];
+  const unsigned mb = D.in_memb[k];
+  double fx[2 * ND], fy[2 * ND], fz[2 * ND];
+#pragma unroll
+  for (int b = 0; b < 2 * ND; b++) {
+    fx[b] = D.in_fp[(b * 3 + 0) * cap + k];
+    fy[b] = D.in_fp[(b * 3 + 1) * cap + k];
+    fz[b] = D.in_fp[(b * 3 + 2) * cap + k];
+  }
+  NE_KICK(vx, vy, vz, fl, mb, fx, fy, fz);
+  NE_STREAM(x, y, z, vx, vy, vz, fl, ms);
+  SLOT(L, x, k) = x; SLOT(L, y, k) = y; SLOT(L, z, k) = z;
+  SLOT(L, vx, k) = vx; SLOT(L, vy, k) = vy; SLOT(L, vz, k) = vz;
+  SLOT(L, flag, k) = fl;
+  if constexpr (L) g_ne.lmsd[k] = ms; else D.in_msd[k] = ms;
+  return ms;
+}
+
+// census exclusion of an inside entry (distribute_mod.f95:149-158)
+template <int ND, bool L>
+__device__ __forceinline__ bool ne_excluded(const CoP& P, const CoD& D, int k) {
+  const double x = SLOT(L, x, k), y = SLOT(L, y, k), z = SLOT(L, z, k);
+  bool excl = false;
+#pragma unroll
+  for (int kk = 0; kk < ND; kk++)
+#pragma unroll
+    for (int ii = 0; ii < 2; ii++) {
+      const int s = kk + ii * ND;
+      double d1 = pbc_half(x - g_ne.sd[DX][s], P.box[0]);
+      double d2 = pbc_half(y - g_ne.sd[DY][s], P.box[1]);
+      double d3 = z - g_ne.sd[DZ][s];
+      if (d1 * d1 + d2 * d2 + d3 * d3 <= P.cut2[ii]) excl = true;
+    }
+  return excl;
+}
+
+__device__ __forceinline__ void ne_save_scalars(CoScalars* sc) {
+  sc->cstep = g_ne.cstep; sc->time_count = g_ne.tc; sc->time_penetrate = g_ne.tp; sc->in_num = g_ne.in_num;
+  sc->n_ev = g_ne.nev; sc->near_vmax2 = g_ne.vmax2;
+}
+
+template <int ND>
+__global__ __launch_bounds__(NE_T) void co_near_engine(CoP P, CoD D, int nsteps, int mode) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  constexpr int NSPH = 2 * ND;
+  constexpr int NV = 4 * NSPH + 1;
+  CoScalars* sc = D.sc;
+
+  for (int j = tid; j < 12 * MAXS; j += NE_T) g_ne.sd[j / MAXS][j % MAXS] = D.dim[j];
+  if (tid == 0) {
+    g_ne.P = P;
+    g_ne.D = D;
+    g_ne.cstep = sc->cstep; g_ne.tc = sc->time_count; g_ne.tp = sc->time_penetrate; g_ne.in_num = sc->in_num;
+    g_ne.nev = (mode & M_RESUME) ? sc->n_ev : 0;
+    g_ne.near_num = sc->near_num;
+    g_ne.vmax2 = sc->near_vmax2;
+    g_ne.rebuild = 0;
+  }
+  __syncthreads();
+  if (g_ne.near_num > NE_NEAR) {   // the set does not fit the lanes' membership registers
+    if (tid == 0) raise_error(&sc->err, ERR_NEAR_CAP);
+    return;
+  }
+  ne_gather(D, g_ne.near_num);
+  __threadfence_block();
+  __syncthreads();
+
+  if (mode & M_SETUP) {
+    // main_pattern.f95:173-188 over the near set; near_vmax2 was set by the classifier kernel
+    ne_rebuild<ND>(false, false, true);
+    const int cap = D.in_cap;
+    for (int k = tid; k < g_ne.in_num; k += NE_T) {
+      D.in_plist[k] = D.near_list[D.in_list[k]];
+      for (int j = 0; j < MAXM * 3; j++) D.in_fp[j * cap + k] = 0.0;
+      for (int j = 0; j < MAXM; j++) D.in_pot[j * cap + k] = 0.0;
+      D.in_msd[k] = 0.0;
+    }
+    __threadfence_block();
+    __syncthreads();
+    ne_scatter(D, g_ne.near_num);
+    if (tid == 0) {
+      ne_save_scalars(sc);
+      sc->n_ev = 0;
+      for (int s = 0; s < NSPH; s++) { sc->x0[s] = g_ne.sd[DX][s]; sc->y0[s] = g_ne.sd[DY][s]; sc->z0[s] = g_ne.sd[DZ][s]; }
+    }
+    return;
   }
 
   const bool resume = (mode & M_RESUME) != 0;
   unsigned long long T0 = __builtin_amdgcn_s_memrealtime(), Tr = 0, Tp = 0, Ts = 0, Nr = 0, ta, tb;
   if (!resume) {
-    // status of this interval's near set: only the inside list is inside
-    for (int q = tid; q < S.near_num; q += NE_T) D.status[D.near_list[q]] = 0;
-    __syncthreads();
-    for (int q = tid; q < S.in_num; q += NE_T) D.status[D.in_list[q]] = ST_INSIDE;
-    ne_load_slots(D, S, S.in_num);
-    ne_msd_io(D, S, S.in_num, true);
-    // first step: kick with the forces stored at the end of the previous interval, stream
+    ne_map_inside();
     double mx = 0.0;
-    for (int q = tid; q < S.in_num; q += NE_T) {
-      Slot r = slot_at(D, S, q);
-      double x = *r.x, y = *r.y, z = *r.z, vx = *r.vx, vy = *r.vy, vz = *r.vz;
-      int fl = *r.flag;
-      double ms = *r.msd;
-      double fx[NSPH], fy[NSPH], fz[NSPH];
-      const unsigned mb = D.in_memb[q];
-#pragma unroll
-      for (int b = 0; b < NSPH; b++) {
-        fx[b] = D.in_fp[(b * 3 + 0) * cap + q];
-        fy[b] = D.in_fp[(b * 3 + 1) * cap + q];
-        fz[b] = D.in_fp[(b * 3 + 2) * cap + q];
-      }
-      NE_KICK(vx, vy, vz, fl, mb, fx, fy, fz);
-      NE_STREAM(x, y, z, vx, vy, vz, fl, ms);
-      *r.x = x; *r.y = y; *r.z = z; *r.vx = vx; *r.vy = vy; *r.vz = vz;
-      *r.flag = fl; *r.msd = ms;
-      mx = fmax(mx, ms);
-    }
+    for (int k = tid; k < g_ne.in_num; k += NE_T)
+      mx = fmax(mx, k < LCAP ? ne_first_kick<ND, true>(P, D, k) : ne_first_kick<ND, false>(P, D, k));
     mx = wave_max(mx);
-    if (lane == 0) S.red[wv][0] = mx;
+    if (lane == 0) g_ne.red[wv][0] = mx;
     __syncthreads();
     if (tid == 0) {
       double m = 0.0;
-      for (int w = 0; w < NE_T / 64; w++) m = fmax(m, S.red[w][0]);
-      ne_begin_step<ND>(P, D, S, m);
+      for (int w = 0; w < NE_W; w++) m = fmax(m, g_ne.red[w][0]);
+      ne_begin_step<ND>(m);
     }
     __syncthreads();
   } else {
-    S.rebuild = 1;   // resuming inside the rebuild of the final step
+    g_ne.rebuild = 1;   // resuming inside the rebuild of the final step
+    __syncthreads();
   }
 
   for (int t = 1; t <= nsteps; t++) {
     const bool last = (t == nsteps);
     // ---- neighbour-list rebuild of step t (evolve_mod.f95:465-564) over the near set
     ta = __builtin_amdgcn_s_memrealtime();
-    if (S.rebuild) {
+    if (g_ne.rebuild) {
       Nr++;
       if (!(resume && last)) {
-        ne_store_slots(D, S, S.in_num);
-        __syncthreads();
         const bool stop = (mode & M_STOP) && last;
-        ne_memb_pass<ND>(P, D, S, true, !stop);
+        ne_rebuild<ND>(true, true, !stop);
         if (stop) {   // the host sorts, then resumes with M_RESUME
-          if (tid == 0) {
-            sc->cstep = S.cstep; sc->time_count = S.tc; sc->time_penetrate = S.tp; sc->n_ev = S.nev;
-            sc->near_vmax2 = S.vmax2;
-          }
-          for (int j = tid; j < 12 * MAXS; j += NE_T) D.dim[j] = S.sd[j / MAXS][j % MAXS];
+          __threadfence_block();
+          __syncthreads();
+          ne_scatter(D, g_ne.near_num);
+          if (tid == 0) ne_save_scalars(sc);
+          for (int j = tid; j < 12 * MAXS; j += NE_T) D.dim[j] = g_ne.sd[j / MAXS][j % MAXS];
           return;
         }
       } else {
-        ne_memb_pass<ND>(P, D, S, false, true);
+        ne_rebuild<ND>(false, false, true);
       }
-      ne_compact(P, D, S);
-      ne_load_slots(D, S, S.in_num);
-      ne_msd_zero(D, S, S.in_num);
-      __syncthreads();
     }
     tb = __builtin_amdgcn_s_memrealtime();
     Tr += tb - ta;
 
-    // ---- particle phase: forces (forces_mod.f95:62-136), kick, next step's kick + stream
+    // ---- particle phase
     double acc[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) acc[v] = 0.0;
-    for (int q = tid; q < S.in_num; q += NE_T) {
-      Slot r = slot_at(D, S, q);
-      double x = *r.x, y = *r.y, z = *r.z, vx = *r.vx, vy = *r.vy, vz = *r.vz;
-      int fl = *r.flag;
-      const unsigned mb = *r.memb;
-      double ms = *r.msd;
-      double fx[NSPH], fy[NSPH], fz[NSPH], pot[NSPH];
-      // lists in the reference's order: (dimer k, sphere i) -> b = 2k + i
-#pragma unroll
-      for (int b = 0; b < NSPH; b++) {
-        const int kk = b >> 1, ii = b & 1, s = kk + ii * ND;
-        fx[b] = fy[b] = fz[b] = pot[b] = 0.0;
-        if (!((mb >> b) & 1u)) continue;
-        double d1 = pbc_half(x - S.sd[DX][s], P.box[0]);
-        double d2 = pbc_half(y - S.sd[DY][s], P.box[1]);
-        double d3 = z - S.sd[DZ][s];
-        double dRsq = d1 * d1 + d2 * d2 + d3 * d3;
-        if (dRsq <= P.cut2[ii]) {
-          double E = 0.0;
-          bool on = false;
-          if (fl == 0) {
-            if (ii == 0) { fl = 1; E = P.Eb[0]; } else E = P.Ea[1];
-            on = true;
-          } else if (fl == 1) { E = P.Eb[ii]; on = true; }
-          else if (fl == 2) { E = P.Es[ii]; on = true; }
-          if (on) {
-            double F, Ub;
-            lj_terms(dRsq, P.R6[ii], P.R12[ii], P.r12R12[ii], P.r6R6[ii], F, Ub);
-            double EF = E * F;
-            fx[b] = EF * d1; fy[b] = EF * d2; fz[b] = EF * d3; pot[b] = E * Ub;
-          }
-        }
-        acc[4 * s + 0] += fx[b]; acc[4 * s + 1] += fy[b];
-        acc[4 * s + 2] += fz[b]; acc[4 * s + 3] += pot[b];
-      }
-      // kick of evolve_mod.f95:645-670 with the flags after every flip of this
-      // step; unless this is the launch's last step, the next step's kick and
-      // streaming follow at once (they do not depend on the dimer)
-      NE_KICK(vx, vy, vz, fl, mb, fx, fy, fz);
-      if (!last) {
-        NE_KICK(vx, vy, vz, fl, mb, fx, fy, fz);
-        NE_STREAM(x, y, z, vx, vy, vz, fl, ms);
-        acc[NV - 1] = fmax(acc[NV - 1], ms);
-      } else {
-#pragma unroll
-        for (int b = 0; b < NSPH; b++) {
-          D.in_fp[(b * 3 + 0) * cap + q] = fx[b];
-          D.in_fp[(b * 3 + 1) * cap + q] = fy[b];
-          D.in_fp[(b * 3 + 2) * cap + q] = fz[b];
-          D.in_pot[b * cap + q] = pot[b];
-        }
-      }
-      *r.x = x; *r.y = y; *r.z = z; *r.vx = vx; *r.vy = vy; *r.vz = vz;
-      *r.flag = fl; *r.msd = ms;
+    const int n = g_ne.in_num;
+    for (int k = tid; k < n; k += NE_T) {
+      if (k < LCAP) ne_particle<ND, true>(P, D, k, last, acc);
+      else ne_particle<ND, false>(P, D, k, last, acc);
     }
     // ---- block reduction in a fixed order: sphere sums and max_solvent_dist
 #pragma unroll
     for (int v = 0; v < NV - 1; v++) {
       double r = wave_sum(acc[v]);
-      if (lane == 0) S.red[wv][v] = r;
+      if (lane == 0) g_ne.red[wv][v] = r;
     }
     {
       double r = wave_max(acc[NV - 1]);
-      if (lane == 0) S.red[wv][NV - 1] = r;
+      if (lane == 0) g_ne.red[wv][NV - 1] = r;
     }
     __syncthreads();
     if (tid < NV) {
-      double a = S.red[0][tid];
-      for (int w = 1; w < NE_T / 64; w++) a = (tid == NV - 1) ? fmax(a, S.red[w][tid]) : a + S.red[w][tid];
-      S.fin[tid] = a;
+      double a = g_ne.red[0][tid];
+      for (int w = 1; w < NE_W; w++) a = (tid == NV - 1) ? fmax(a, g_ne.red[w][tid]) : a + g_ne.red[w][tid];
+      g_ne.fin[tid] = a;
     }
     __syncthreads();
     ta = __builtin_amdgcn_s_memrealtime();
     Tp += ta - tb;
-    if (tid == 0) ne_end_step<ND>(P, D, S, last);
+    if (tid == 0) ne_end_step<ND>(last);
     __syncthreads();
     Ts += __builtin_amdgcn_s_memrealtime() - ta;
   }
 
-  // ---- end of the interval: census exclusion (distribute_mod.f95:149-158), write back
-  for (int q = tid; q < S.in_num; q += NE_T) {
-    Slot r = slot_at(D, S, q);
-    double x = *r.x, y = *r.y, z = *r.z;
-    bool excl = false;
-    for (int kk = 0; kk < ND; kk++)
-      for (int ii = 0; ii < 2; ii++) {
-        int s = kk + ii * ND;
-        double d1 = pbc_half(x - S.sd[DX][s], P.box[0]);
-        double d2 = pbc_half(y - S.sd[DY][s], P.box[1]);
-        double d3 = z - S.sd[DZ][s];
-        if (d1 * d1 + d2 * d2 + d3 * d3 <= P.cut2[ii]) excl = true;
-      }
-    D.status[D.in_list[q]] = ST_INSIDE | (excl ? ST_EXCL : 0);
+  // ---- end of the interval: census exclusion, write back through the near slots
+  const int n = g_ne.in_num;
+  for (int k = tid; k < n; k += NE_T) {
+    const bool ex = k < LCAP ? ne_excluded<ND, true>(P, D, k) : ne_excluded<ND, false>(P, D, k);
+    const int q = D.in_list[k];
+    D.ns_stat[q] = ST_INSIDE | (ex ? ST_EXCL : 0);
+    D.in_plist[k] = D.near_list[q];
+    if (k < LCAP) D.in_msd[k] = g_ne.lmsd[k];
   }
-  ne_store_slots(D, S, S.in_num);
-  ne_msd_io(D, S, S.in_num, false);
+  ne_store_slots(D, n);
+  __threadfence_block();
+  __syncthreads();
+  ne_scatter(D, g_ne.near_num);
   if (tid == 0) {
     sc->prof[0] += __builtin_amdgcn_s_memrealtime() - T0;
     sc->prof[1] += Tr; sc->prof[2] += Tp; sc->prof[3] += Ts; sc->prof[4] += Nr;
-    sc->prof[5] = S.near_num; sc->prof[6] = S.in_num; sc->prof[7] += 1;
-    sc->cstep = S.cstep; sc->time_count = S.tc; sc->time_penetrate = S.tp; sc->in_num = S.in_num;
-    sc->n_ev = S.nev;
-    for (int s = 0; s < NSPH; s++) { sc->x0[s] = S.sd[DX][s]; sc->y0[s] = S.sd[DY][s]; sc->z0[s] = S.sd[DZ][s]; }
+    sc->prof[5] = g_ne.near_num; sc->prof[6] = g_ne.in_num; sc->prof[7] += 1;
+    ne_save_scalars(sc);
+    for (int s = 0; s < NSPH; s++) { sc->x0[s] = g_ne.sd[DX][s]; sc->y0[s] = g_ne.sd[DY][s]; sc->z0[s] = g_ne.sd[DZ][s]; }
   }
-  for (int j = tid; j < 12 * MAXS; j += NE_T) D.dim[j] = S.sd[j / MAXS][j % MAXS];
+  for (int j = tid; j < 12 * MAXS; j += NE_T) D.dim[j] = g_ne.sd[j / MAXS][j % MAXS];
 }
 #undef NE_KICK
 #undef NE_STREAM
+#undef SLOT
 
 // ------------------------------------------------------------- bulk kernels
 __device__ __forceinline__ void collision_shift(const CoP& P, const CoD& D, double& xs, double& ys, double& zs) {
@@ -847,8 +1085,11 @@ __global__ __launch_bounds__(256) void co_bulk(CoP P, CoD D, int do_stream, int 
     bool near = false;
     double x = 0, y = 0, z = 0;
     int f = -1;
+    unsigned char st = 0;
     if (valid) {
       near = (D.near_bits[i >> 5] >> (i & 31)) & 1u;
+      st = D.status[i];
+      if (!near && st != 0) { D.status[i] = 0; st = 0; }   // left the near set earlier
       x = D.x[i]; y = D.y[i]; z = D.z[i];
       f = D.flag[i];
       if (do_stream && !near && nev > 0) {
@@ -885,6 +1126,7 @@ __global__ __launch_bounds__(256) void co_bulk(CoP P, CoD D, int do_stream, int 
         dmin = fmin(dmin, sqrt(a * a + b * b + c * c) - P.full[s >= P.nd]);
       }
       double qd = floor(fmax(dmin, 0.0) * 32.0 * (1.0 - 1e-12));
+      if (st & ST_INSIDE) qd = 0.0;   // list members stay near until the next rebuild
       D.dq[i] = (unsigned char)fmin(qd, 255.0);
     }
     // census of distribute_mod.f95:95-183: outside particles keyed by index,
@@ -894,10 +1136,7 @@ __global__ __launch_bounds__(256) void co_bulk(CoP P, CoD D, int do_stream, int 
     int ck = 0;
     if (valid && ok) {
       bool skip = false;
-      if (near) {
-        unsigned char st = D.status[i];
-        if (st & ST_INSIDE) { key = (unsigned)NT + (unsigned)i; skip = (st & ST_EXCL) != 0; }
-      }
+      if (near && (st & ST_INSIDE)) { key = (unsigned)NT + (unsigned)i; skip = (st & ST_EXCL) != 0; }
       int sp = f == 0 ? 0 : f == 1 ? 1 : f == 2 ? 2 : -1;
       if (!skip && sp >= 0) {
         long long u = cell_unshifted0(x, y, z, P.box[0], P.box[1]);
@@ -1247,6 +1486,11 @@ struct mpcx_co {
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr, b0 = nullptr, b1 = nullptr, n0 = nullptr, n1 = nullptr;
   double run_ms = 0, bulk_ms = 0, near_ms = 0;
+  static constexpr int NKT = 5, TMAX = 256;   // timed kernels, timed intervals
+  std::vector<hipEvent_t> tev;                // [TMAX][NKT][2]
+  std::vector<char> tev_on;                   // [TMAX][NKT]: pair recorded
+  double kt_ms[NKT] = {0};
+  long long kt_n = 0;
   long long ndraw = 0;
   double* d_U = nullptr;
   int* d_tourpos = nullptr;
@@ -1302,8 +1546,9 @@ static int check_device_error(mpcx_co* e) {
                                 "near set larger than its buffer",
                                 "a neighbour list is longer than max_neighbour",
                                 "more than 64 list rebuilds in one interval",
-                                "the dimer bond constraint did not converge"};
-    std::string m = std::string("device check failed: ") + (err > 0 && err < 11 ? msg[err] : "unknown");
+                                "the dimer bond constraint did not converge",
+                                "a neighbour-list member is missing from the near set"};
+    std::string m = std::string("device check failed: ") + (err > 0 && err < 12 ? msg[err] : "unknown");
     int zero = 0;
     (void)hipMemcpy(&e->D.sc->err, &zero, sizeof(int), hipMemcpyHostToDevice);
     return set_error(MPCX_E_DEVICE, m.c_str());
@@ -1391,13 +1636,18 @@ extern "C" int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg) {
   D.sc = dalloc<CoScalars>(e, 1);
   D.react_box = dalloc<int>(e, std::max(1, P.nrb));
   D.in_cap = NE_CAP;
-  D.near_cap = (int)std::min<long long>(NT, 1 << 22);
+  D.near_cap = (int)std::min<long long>(NT, NE_NEAR);
   D.near_list = dalloc<int>(e, D.near_cap);
-  D.near_memb = dalloc<int>(e, D.near_cap);
+  D.ns_x = dalloc<double>(e, D.near_cap); D.ns_y = dalloc<double>(e, D.near_cap); D.ns_z = dalloc<double>(e, D.near_cap);
+  D.ns_vx = dalloc<double>(e, D.near_cap); D.ns_vy = dalloc<double>(e, D.near_cap);
+  D.ns_vz = dalloc<double>(e, D.near_cap);
+  D.ns_flag = dalloc<int>(e, D.near_cap);
+  D.ns_stat = dalloc<unsigned char>(e, D.near_cap);
   e->nwords = (NT + 31) / 32;
   D.near_bits = dalloc<unsigned>(e, e->nwords + 1);
   D.status = dalloc<unsigned char>(e, NT);
   D.in_list = dalloc<int>(e, D.in_cap);
+  D.in_plist = dalloc<int>(e, D.in_cap);
   D.in_memb = dalloc<unsigned>(e, D.in_cap);
   D.in_fp = dalloc<double>(e, (size_t)MAXM * 3 * D.in_cap);
   D.in_pot = dalloc<double>(e, (size_t)MAXM * D.in_cap);
@@ -1442,6 +1692,9 @@ extern "C" int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg) {
   HIPCHK(hipEventCreate(&e->e0)); HIPCHK(hipEventCreate(&e->e1));
   HIPCHK(hipEventCreate(&e->b0)); HIPCHK(hipEventCreate(&e->b1));
   HIPCHK(hipEventCreate(&e->n0)); HIPCHK(hipEventCreate(&e->n1));
+  e->tev.assign((size_t)mpcx_co::TMAX * mpcx_co::NKT * 2, nullptr);
+  e->tev_on.assign((size_t)mpcx_co::TMAX * mpcx_co::NKT, 0);
+  for (auto& ev : e->tev) HIPCHK(hipEventCreate(&ev));
   HIPCHK(hipMemcpy(ptab, cfg->reaction_p, 101 * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemset(D.sc, 0, sizeof(CoScalars)));
   HIPCHK(hipMemset(D.status, 0, NT));
@@ -1460,6 +1713,8 @@ extern "C" int mpcx_co_destroy(mpcx_co* e) {
     if (p) (void)hipFree(p);
   if (e->st) (void)hipStreamDestroy(e->st);
   for (hipEvent_t ev : {e->e0, e->e1, e->b0, e->b1, e->n0, e->n1})
+    if (ev) (void)hipEventDestroy(ev);
+  for (hipEvent_t ev : e->tev)
     if (ev) (void)hipEventDestroy(ev);
   delete e;
   return 0;
@@ -1621,8 +1876,18 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
     D.coll = (unsigned long long)(cnext / P.MPCtstep);
     if (P.rng_mode == 0)
       HIPCHK(hipMemcpyAsync(e->d_U, draws + c * e->ndraw, e->ndraw * 8, hipMemcpyHostToDevice, st));
+    const bool timed = c < mpcx_co::TMAX;
+    auto tmark = [&](int k, int edge) -> hipError_t {
+      if (!timed) return hipSuccess;
+      e->tev_on[(size_t)c * mpcx_co::NKT + k] = 1;
+      return hipEventRecord(e->tev[((size_t)c * mpcx_co::NKT + k) * 2 + edge], st);
+    };
+    if (timed)
+      for (int k = 0; k < mpcx_co::NKT; k++) e->tev_on[(size_t)c * mpcx_co::NKT + k] = 0;
     HIPCHK(hipEventRecord(e->n0, st));
+    HIPCHK(tmark(0, 0));
     launch_near(e, P.MPCtstep, sort_due ? M_STOP : 0);
+    HIPCHK(tmark(0, 1));
     HIPCHK(hipEventRecord(e->n1, st));
     hipLaunchKernelGGL(fill_kernel<int>, dim3(grid_for(mc, 256)), dim3(256), 0, st, D.cnt, mc, 0);
     hipLaunchKernelGGL(fill_kernel<int>, dim3(grid_for(3 * vol, 256)), dim3(256), 0, st, D.tcount, 3 * vol, 0);
@@ -1637,15 +1902,23 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
       hipLaunchKernelGGL(co_bulk, dim3(gN), dim3(256), 0, st, P, D, 0, 1);
     } else {
       HIPCHK(hipEventRecord(e->b0, st));
+      HIPCHK(tmark(1, 0));
       hipLaunchKernelGGL(co_bulk, dim3(gN), dim3(256), 0, st, P, D, 1, 1);
+      HIPCHK(tmark(1, 1));
     }
     launch_scan(st, D.cnt, mc, D.partial, D.off);
+    HIPCHK(tmark(4, 0));
     hipLaunchKernelGGL(co_scatter, dim3(gN), dim3(256), 0, st, P, D);
+    HIPCHK(tmark(4, 1));
+    HIPCHK(tmark(2, 0));
     hipLaunchKernelGGL(co_collide, dim3((unsigned)((mc + CT - 1) / CT)), dim3(CT), 0, st, P, D);
+    HIPCHK(tmark(2, 1));
     hipLaunchKernelGGL(fill_kernel<unsigned>, dim3(grid_for(e->nwords + 1, 256)), dim3(256), 0, st, D.near_bits,
                        e->nwords + 1, 0u);
     hipLaunchKernelGGL(reset_far, dim3(1), dim3(1), 0, st, D.sc);
+    HIPCHK(tmark(3, 0));
     hipLaunchKernelGGL(co_rotate, dim3(gN), dim3(256), 0, st, P, D, 1);
+    HIPCHK(tmark(3, 1));
     HIPCHK(hipEventRecord(e->b1, st));
     co_near_list(e);
     cstep = cnext;
@@ -1663,6 +1936,19 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
   float ms = 0;
   (void)hipEventElapsedTime(&ms, e->e0, e->e1);
   e->run_ms = ms;
+  e->kt_n = std::min<long long>(ncoll, mpcx_co::TMAX);
+  for (int k = 0; k < mpcx_co::NKT; k++) {
+    double sum = 0;
+    long long n = 0;
+    for (long long c = 0; c < e->kt_n; c++) {
+      if (!e->tev_on[(size_t)c * mpcx_co::NKT + k]) continue;
+      float t = 0;
+      (void)hipEventElapsedTime(&t, e->tev[((size_t)c * mpcx_co::NKT + k) * 2], e->tev[((size_t)c * mpcx_co::NKT + k) * 2 + 1]);
+      sum += t;
+      n++;
+    }
+    e->kt_ms[k] = n ? sum / (double)n : 0.0;
+  }
   return check_device_error(e);
 }
 
@@ -1703,7 +1989,7 @@ extern "C" int mpcx_co_get_inside(mpcx_co* e, int32_t* list, uint32_t* memb, dou
   *n = sc.in_num;
   if (cap < sc.in_num) return set_error(MPCX_E_ARG, "buffer smaller than the inside list");
   const size_t m = (size_t)sc.in_num, ic = (size_t)e->D.in_cap;
-  if (list) HIPCHK(hipMemcpy(list, e->D.in_list, m * 4, hipMemcpyDeviceToHost));
+  if (list) HIPCHK(hipMemcpy(list, e->D.in_plist, m * 4, hipMemcpyDeviceToHost));
   if (memb) HIPCHK(hipMemcpy(memb, e->D.in_memb, m * 4, hipMemcpyDeviceToHost));
   if (fp)
     for (int j = 0; j < MAXM * 3; j++)
@@ -1737,6 +2023,13 @@ extern "C" int mpcx_co_near_profile(mpcx_co* e, uint64_t* prof8) {
   if (!e || !prof8) return set_error(MPCX_E_ARG, "null argument");
   HIPCHK(hipStreamSynchronize(e->st));
   HIPCHK(hipMemcpy(prof8, e->D.sc->prof, 64, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int mpcx_co_kernel_ms(mpcx_co* e, double* ms5, int64_t* count) {
+  if (!e || !ms5) return set_error(MPCX_E_ARG, "null argument");
+  for (int k = 0; k < mpcx_co::NKT; k++) ms5[k] = e->kt_ms[k];
+  if (count) *count = e->kt_n;
   return 0;
 }
 

@@ -27,6 +27,45 @@ extern "C" int mpcx_device_count(void) {
   return n;
 }
 
+// Plain device-to-device copy of a known size: the reference point that turns
+// the hardware's fetch/write counters into bytes.
+__global__ __launch_bounds__(256) void probe_copy_kernel(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                         long long n) {
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    dst[i] = src[i];
+}
+
+extern "C" int mpcx_probe_copy(int64_t nbytes, int32_t reps, double* ms_per_copy) {
+  if (nbytes < 16 || reps < 1 || !ms_per_copy) return mpcx::set_error(MPCX_E_ARG, "bad arguments");
+  const long long n = nbytes / 16;
+  void *a = nullptr, *b = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = 0;
+  if (hipMalloc(&a, n * 16) != hipSuccess || hipMalloc(&b, n * 16) != hipSuccess) {
+    rc = mpcx::set_error(MPCX_E_HIP, "hipMalloc failed");
+  } else {
+    (void)hipMemset(a, 0, n * 16);
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const int grid = 8192;
+    hipLaunchKernelGGL(probe_copy_kernel, dim3(grid), dim3(256), 0, 0, (const double2*)a, (double2*)b, n);
+    (void)hipEventRecord(e0, 0);
+    for (int r = 0; r < reps; r++)
+      hipLaunchKernelGGL(probe_copy_kernel, dim3(grid), dim3(256), 0, 0, (const double2*)a, (double2*)b, n);
+    (void)hipEventRecord(e1, 0);
+    hipError_t r = hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    *ms_per_copy = ms / reps;
+    if (r != hipSuccess) rc = mpcx::set_error(MPCX_E_HIP, hipGetErrorString(r));
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return rc;
+}
+
 // ---------------------------------------------------------------------------
 // Host Mersenne Twister streams in the two flavours the simulators use.  They
 // produce the uniforms the parity mode uploads, in the reference's call order.

@@ -24,6 +24,7 @@ enum DevError : int {
   ERR_LIST_CAP = 8,        // a neighbour list longer than max_neighbour
   ERR_EVENTS = 9,          // more rebuild events in one interval than recorded
   ERR_RATTLE = 10,         // the bond constraint did not converge
+  ERR_INSIDE_LOST = 11,    // a list member missing from the next interval's near set
 };
 
 __device__ __forceinline__ void raise_error(int* err, int code) { atomicCAS(err, 0, code); }
