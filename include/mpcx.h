@@ -162,7 +162,7 @@ int mpcx_co_get_inside(mpcx_co* e, int32_t* list, uint32_t* memb, double* fp, do
 int mpcx_co_get_cells(mpcx_co* e, double* vcm_uv /* (mpcnumcell, 8) */, int32_t* cell_of,
                       int32_t* census /* (3, vol) */, double* tpotential);
 /* Wall time of the HIP events bracketing the last mpcx_co_run (ms), and of
- * the bulk kernels alone (ms, summed over the run). */
+ * the last interval's bulk kernels (co_bulk .. co_rotate) and near engine (ms). */
 int mpcx_co_timing(mpcx_co* e, double* run_ms, double* bulk_ms, double* near_ms);
 /* Near-engine counters accumulated since creation: real-time ticks (100 MHz)
  * of [0] whole launches, [1] list rebuilds, [2] particle phases, [3] dimer

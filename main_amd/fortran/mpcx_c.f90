@@ -1,0 +1,146 @@
+! ISO_C_BINDING interfaces of libmpcx (include/mpcx.h) for Fortran callers.
+! Integers map to c_int32_t/c_int64_t, reals to c_double; arrays are passed
+! by address (explicit-size dummies), scalars by value as in the C prototypes.
+! The uint32 Mersenne-Twister words travel as default (32-bit) integers, the
+! type the collective_code driver keeps them in.
+module mpcx_c
+  use iso_c_binding
+  implicit none
+  private :: c_strlen
+
+  interface
+    function mpcx_last_error() bind(C, name="mpcx_last_error") result(p)
+      import :: c_ptr
+      type(c_ptr) :: p
+    end function
+
+    function c_strlen(p) bind(C, name="strlen") result(n)
+      import :: c_ptr, c_size_t
+      type(c_ptr), value :: p
+      integer(c_size_t) :: n
+    end function
+
+    function mpcx_device_count() bind(C, name="mpcx_device_count") result(n)
+      import :: c_int
+      integer(c_int) :: n
+    end function
+
+    function mpcx_mtmod_fill(mt, mti, out, n) bind(C, name="mpcx_mtmod_fill") result(rc)
+      import :: c_int, c_int32_t, c_int64_t, c_double
+      integer(c_int32_t) :: mt(*)
+      integer(c_int32_t) :: mti
+      real(c_double) :: out(*)
+      integer(c_int64_t), value :: n
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_mpc_rotation(NT, rotang, numcell, a0, box, x, y, z, vx, vy, vz, vxcm, vycm, vzcm, tcs, ncn, &
+                               uvx, uvy, uvz, draws) bind(C, name="mpcx_mpc_rotation") result(rc)
+      import :: c_int, c_int32_t, c_int64_t, c_double
+      integer(c_int32_t), value :: NT
+      real(c_double), value :: rotang
+      integer(c_int64_t), value :: numcell
+      real(c_double), value :: a0
+      real(c_double) :: box(*), x(*), y(*), z(*), vx(*), vy(*), vz(*)
+      real(c_double) :: vxcm(*), vycm(*), vzcm(*), tcs(*), uvx(*), uvy(*), uvz(*), draws(*)
+      integer(c_int64_t) :: ncn(*)
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_reaction_simple(MPCt, numcell, k1, NT, flag, tB, sB, vol, draws) &
+        bind(C, name="mpcx_reaction_simple") result(rc)
+      import :: c_int, c_int32_t, c_int64_t, c_double
+      real(c_double), value :: MPCt
+      integer(c_int64_t), value :: numcell
+      real(c_double), value :: k1
+      integer(c_int32_t), value :: NT
+      integer(c_int32_t) :: flag(*)
+      integer(c_int64_t) :: tB(*), sB(*)
+      integer(c_int64_t), value :: vol
+      real(c_double) :: draws(*)
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_neighbour_list(NT, box, cutoff, skinrad, x, y, z, xd, yd, zd, num, nb, stride, max_nb) &
+        bind(C, name="mpcx_neighbour_list") result(rc)
+      import :: c_int, c_int32_t, c_int64_t, c_double
+      integer(c_int32_t), value :: NT
+      real(c_double) :: box(*)
+      real(c_double), value :: cutoff, skinrad
+      real(c_double) :: x(*), y(*), z(*)
+      real(c_double), value :: xd, yd, zd
+      integer(c_int64_t) :: num
+      integer(c_int64_t) :: nb(*)
+      integer(c_int64_t), value :: stride, max_nb
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_inout_particle_list(NT, box, cutC, cutN, skinrad, x, y, z, xd, yd, zd, out_num, out_list, &
+                                      in_num, in_list, in_cap, numdims) &
+        bind(C, name="mpcx_inout_particle_list") result(rc)
+      import :: c_int, c_int32_t, c_int64_t, c_double
+      integer(c_int32_t), value :: NT
+      real(c_double) :: box(*)
+      real(c_double), value :: cutC, cutN, skinrad
+      real(c_double) :: x(*), y(*), z(*), xd(*), yd(*), zd(*)
+      integer(c_int64_t) :: out_num, in_num
+      integer(c_int64_t) :: out_list(*), in_list(*)
+      integer(c_int64_t), value :: in_cap
+      integer(c_int32_t), value :: numdims
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_distribute(NT, numdims, flag, x, y, z, xd, yd, zd, box, vol, tA, tB, tS, sA, sB, sS, maxs, &
+                             cutC2, cutN2, out_num, out_list, in_num, in_list) &
+        bind(C, name="mpcx_distribute") result(rc)
+      import :: c_int, c_int32_t, c_int64_t, c_double
+      integer(c_int32_t), value :: NT, numdims
+      integer(c_int32_t) :: flag(*)
+      real(c_double) :: x(*), y(*), z(*), xd(*), yd(*), zd(*), box(*)
+      integer(c_int64_t), value :: vol
+      integer(c_int64_t) :: tA(*), tB(*), tS(*), sA(*), sB(*), sS(*)
+      integer(c_int64_t), value :: maxs
+      real(c_double), value :: cutC2, cutN2
+      integer(c_int64_t), value :: out_num
+      integer(c_int64_t) :: out_list(*)
+      integer(c_int64_t), value :: in_num
+      integer(c_int64_t) :: in_list(*)
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_sort_hilbert(vol, NT, box, tour, x, y, z, flag, vx, vy, vz) &
+        bind(C, name="mpcx_sort_hilbert") result(rc)
+      import :: c_int, c_int32_t, c_int64_t, c_double
+      integer(c_int64_t), value :: vol
+      integer(c_int32_t), value :: NT
+      real(c_double) :: box(*)
+      integer(c_int64_t) :: tour(*)
+      real(c_double) :: x(*), y(*), z(*), vx(*), vy(*), vz(*)
+      integer(c_int32_t) :: flag(*)
+      integer(c_int) :: rc
+    end function
+  end interface
+
+contains
+
+  ! Stops the program with libmpcx's message when a call failed (the C ABI
+  ! returns 0 on success and a negative code otherwise).
+  subroutine mpcx_check(rc, what)
+    integer(c_int), intent(in) :: rc
+    character(*), intent(in) :: what
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: s(:)
+    character(len=:), allocatable :: msg
+    integer :: i, n
+    if (rc == 0) return
+    p = mpcx_last_error()
+    n = int(c_strlen(p))
+    call c_f_pointer(p, s, [n])
+    allocate(character(len=n) :: msg)
+    do i = 1, n
+      msg(i:i) = s(i)
+    end do
+    write(0, '(a)') what // ': ' // msg
+    error stop 1
+  end subroutine
+end module mpcx_c
