@@ -176,6 +176,53 @@ int mpcx_co_near_profile(mpcx_co* e, uint64_t* prof8);
  * [2] co_collide, [3] co_rotate, [4] co_scatter; count = timed intervals. */
 int mpcx_co_kernel_ms(mpcx_co* e, double* ms5, int64_t* count);
 
+/* ------------------------------------------------------------------------
+ * Device-resident Selkov engine (selkov_code)
+ * ---------------------------------------------------------------------- */
+typedef struct mpcx_sk mpcx_sk;
+
+/* Parameters of one selkov_code system: Input_SelkovDimer.txt quantities and
+ * the derived ones of Main_SelkovDimer.f90:60-104 (T already divided by 5 is
+ * not needed here: velocities come with the state). */
+typedef struct {
+  int64_t NT, space;                 /* active particles, allocated slots (FLAG 2 beyond NT)        */
+  int32_t MPCtstep, sort_step;       /* MD steps per collision; re-sort period (reference: 10000) */
+  int32_t rng_mode, pad;             /* 0: host mt95 draws, 1: device Philox                      */
+  uint64_t seed;                     /* Philox key in rng_mode 1                                  */
+  double box[3];
+  double MDt, MPCt, Mf, Mg, Mi, Mc, Mn, Dcn, Rc, Rn;
+  double EfC, EgC, EiC, EfN, EgN, EiN, RPfg, RPgf, A0;
+  double k[6];
+  double sinrot, cosrot;             /* sin/cos of pi/2 as the reference evaluates them          */
+  double sphere_guard;               /* bound on a sphere's displacement per interval             */
+} mpcx_sk_config;
+
+int mpcx_sk_create(mpcx_sk** out, const mpcx_sk_config* cfg);
+int mpcx_sk_destroy(mpcx_sk* e);
+/* Upload the state at a collision step (cstep a multiple of MPCtstep); arrays
+ * of `space` entries as the module arrays of Global_SelkovDimer.f90 (FLAG
+ * int64; FXsol.. the forces of the last step); dm is the (2,12) column-major
+ * dimer block x y z vx vy vz fx fy fz (+3 unused) of sphere C (row 1) and N. */
+int mpcx_sk_set_state(mpcx_sk* e, int64_t cstep, int64_t NT, const double* x, const double* y, const double* z,
+                      const double* vx, const double* vy, const double* vz, const double* fx, const double* fy,
+                      const double* fz, const int64_t* flag, const double* dm);
+/* Advance ncoll collision intervals of Main_SelkovDimer.f90:125-149 (Motion,
+ * ForcesMD every step; Sort, Distribute, Affected_Cell, RMPC at collision
+ * steps).  In rng_mode 0 `draws` holds ndraws successive genrand_real1()
+ * values of the mt95 stream in the reference's call order; *used returns how
+ * many the run took (the caller advances its stream by that many). */
+int mpcx_sk_run(mpcx_sk* e, int64_t ncoll, const double* draws, int64_t ndraws, int64_t* used);
+/* Download state (NULL skips an array); scalars: cstep, NT, draws used, near-set size. */
+int mpcx_sk_get_state(mpcx_sk* e, double* x, double* y, double* z, double* vx, double* vy, double* vz, double* fx,
+                      double* fy, double* fz, int64_t* flag, double* dm, int64_t* scalars, double* tpotential);
+/* Cells of the last collision: (vol, 8) records vcm xyz, axis xyz, count; the
+ * collision cell of each particle (-1: inactive); census counts (3, vol) F, G,
+ * I; unaffected mask (vol). */
+int mpcx_sk_get_cells(mpcx_sk* e, double* rec8, int32_t* cell_of, int32_t* census, uint8_t* unaffected);
+/* Mean kernel durations (ms) over the last run: near engine, sk_bulk,
+ * sk_collide, sk_rotate; and the run's wall time. */
+int mpcx_sk_kernel_ms(mpcx_sk* e, double* ms4, double* run_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -26,6 +26,7 @@
 #include <algorithm>
 #include "mpcx_device.h"
 #include "mpcx_host.h"
+#include "mpcx_kernels.h"
 #include "../../include/mpcx.h"
 
 #pragma clang fp contract(off)
@@ -1035,30 +1036,6 @@ __device__ __forceinline__ long long shifted_cell(const CoP& P, double x, double
   return (long long)c - 1;
 }
 
-// Adds the number of active lanes holding `key` to counter[key], once per
-// distinct key of the wave: a ballot loop finds the groups (no memory access),
-// then every group leader issues its atomic in one instruction.  Returns the
-// lane's rank among equal keys plus the counter's old value.
-__device__ __forceinline__ int wave_atomic_rank(int* counter, int key, bool active) {
-  const int lane = threadIdx.x & 63;
-  const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  unsigned long long todo = __ballot(active);
-  int leader_of = 0, rank = 0, gsize = 0;
-  while (todo) {
-    int leader = __ffsll((unsigned long long)todo) - 1;
-    int k = __shfl(key, leader);
-    bool mine = active && key == k;
-    unsigned long long grp = __ballot(mine);
-    if (mine) { leader_of = leader; rank = __popcll(grp & lt); }
-    if (lane == leader) gsize = __popcll(grp);
-    todo &= ~grp;
-  }
-  int base = 0;
-  if (active && lane == leader_of) base = atomicAdd(&counter[key], gsize);
-  base = __shfl(base, leader_of);
-  return base + rank;
-}
-
 // Far particles: replay the interval's lazy-streaming events; then bin every
 // particle into the shifted grid (count + rank, by the cell's tour position)
 // and into the census; record its distance to the nearest sphere for the
@@ -1167,57 +1144,6 @@ __global__ __launch_bounds__(256) void co_bulk(CoP P, CoD D, int do_stream, int 
       }
     }
   }
-}
-
-// exclusive scan of cnt[0..n) into off[0..n] (off[n] = total), three passes
-__global__ __launch_bounds__(1024) void scan_partial(const int* cnt, long long n, int* partial) {
-  __shared__ unsigned long long s[17];
-  long long base = (long long)blockIdx.x * 4096;
-  unsigned long long v = 0;
-  for (int j = 0; j < 4; j++) {
-    long long i = base + threadIdx.x * 4 + j;
-    if (i < n) v += cnt[i];
-  }
-  unsigned long long tot;
-  block_excl_scan64(v, s, tot);
-  if (threadIdx.x == 0) partial[blockIdx.x] = (int)tot;
-}
-__global__ __launch_bounds__(1024) void scan_top(int* partial, int nb) {
-  __shared__ unsigned long long s[17];
-  __shared__ long long run;
-  if (threadIdx.x == 0) run = 0;
-  __syncthreads();
-  for (int b0 = 0; b0 < nb; b0 += 1024) {
-    int i = b0 + threadIdx.x;
-    unsigned long long v = i < nb ? (unsigned)partial[i] : 0;
-    unsigned long long tot;
-    unsigned long long e = block_excl_scan64(v, s, tot);
-    if (i < nb) partial[i] = (int)(run + (long long)e);
-    __syncthreads();
-    if (threadIdx.x == 0) run += (long long)tot;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) partial[nb] = (int)run;
-}
-__global__ __launch_bounds__(1024) void scan_final(const int* cnt, long long n, const int* partial, int* off) {
-  __shared__ unsigned long long s[17];
-  long long base = (long long)blockIdx.x * 4096;
-  int v[4];
-  unsigned long long sum = 0;
-  for (int j = 0; j < 4; j++) {
-    long long i = base + threadIdx.x * 4 + j;
-    v[j] = i < n ? cnt[i] : 0;
-    sum += v[j];
-  }
-  unsigned long long tot;
-  unsigned long long e = block_excl_scan64(sum, s, tot);
-  long long run = partial[blockIdx.x] + (long long)e;
-  for (int j = 0; j < 4; j++) {
-    long long i = base + threadIdx.x * 4 + j;
-    if (i < n) off[i] = (int)run;
-    run += v[j];
-  }
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) off[n] = partial[gridDim.x];
 }
 
 __global__ __launch_bounds__(256) void co_scatter(CoP P, CoD D) {
@@ -1380,44 +1306,6 @@ __global__ __launch_bounds__(256) void co_classify_all(CoP P, CoD D) {
   }
 }
 
-// ordered list of the set bits: counts per 1024-word block, then positions
-__global__ __launch_bounds__(1024) void bits_count(const unsigned* bits, long long nwords, int* partial) {
-  __shared__ unsigned long long s[17];
-  long long w = (long long)blockIdx.x * 1024 + threadIdx.x;
-  unsigned long long v = w < nwords ? __popc(bits[w]) : 0;
-  unsigned long long tot;
-  block_excl_scan64(v, s, tot);
-  if (threadIdx.x == 0) partial[blockIdx.x] = (int)tot;
-}
-__global__ __launch_bounds__(1024) void bits_emit(const unsigned* bits, long long nwords, const int* partial,
-                                                  int* list, int cap, int NT, CoScalars* sc, int nblocks) {
-  __shared__ unsigned long long s[17];
-  long long w = (long long)blockIdx.x * 1024 + threadIdx.x;
-  unsigned word = w < nwords ? bits[w] : 0u;
-  unsigned long long tot;
-  unsigned long long e = block_excl_scan64(__popc(word), s, tot);
-  int pos = partial[blockIdx.x] + (int)e;
-  while (word) {
-    int b = __ffs(word) - 1;
-    word &= word - 1;
-    int idx = (int)(w * 32 + b);
-    if (idx < NT) {
-      if (pos < cap) list[pos] = idx;
-      pos++;
-    }
-  }
-  if (blockIdx.x == nblocks - 1 && threadIdx.x == 0) {
-    int n = partial[nblocks];
-    sc->near_num = n < cap ? n : cap;
-    if (n > cap) raise_error(&sc->err, ERR_NEAR_CAP);
-  }
-}
-
-template <typename T>
-__global__ void fill_kernel(T* p, long long n, T v) {
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    p[i] = v;
-}
 __global__ void reset_far(CoScalars* sc) { sc->far_maxv2 = 0ull; }
 
 // --------------------------------------------------------------- Hilbert sort
@@ -1438,16 +1326,6 @@ __global__ __launch_bounds__(256) void sort_scatter(int NT, const int* key, cons
                                                     int* sb) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < NT; i += gridDim.x * blockDim.x)
     if (rk[i] < BMAX) sb[soff[key[i]] + rk[i]] = i;
-}
-__global__ __launch_bounds__(256) void sort_order(long long ncell, const int* scnt, const int* soff, int* sb) {
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ncell) return;
-  int n = min(scnt[t], BMAX), o = soff[t];
-  for (int j = 1; j < n; j++) {   // insertion sort: ascending original index
-    int v = sb[o + j], k = j;
-    while (k > 0 && sb[o + k - 1] > v) { sb[o + k] = sb[o + k - 1]; k--; }
-    sb[o + k] = v;
-  }
 }
 __global__ __launch_bounds__(256) void sort_gather(CoP P, CoD D, const int* sb, double* nx, double* ny, double* nz,
                                                    double* nvx, double* nvy, double* nvz, int* nf,
@@ -1514,21 +1392,6 @@ static T* dalloc(mpcx_co* e, size_t n) {
   if (hipMalloc(&p, n * sizeof(T) + 64) != hipSuccess) return nullptr;
   e->allocs.push_back(p);
   return (T*)p;
-}
-
-static int grid_for(long long n, int threads, int cap = 4096) {
-  long long g = (n + threads - 1) / threads;
-  if (g < 1) g = 1;
-  if (g > cap) g = cap;
-  return (int)g;
-}
-
-static void launch_scan(hipStream_t st, const int* cnt, long long n, int* partial, int* off) {
-  int nb = (int)((n + 4095) / 4096);
-  if (nb < 1) nb = 1;
-  hipLaunchKernelGGL(scan_partial, dim3(nb), dim3(1024), 0, st, cnt, n, partial);
-  hipLaunchKernelGGL(scan_top, dim3(1), dim3(1024), 0, st, partial, nb);
-  hipLaunchKernelGGL(scan_final, dim3(nb), dim3(1024), 0, st, cnt, n, (const int*)partial, off);
 }
 
 static int check_device_error(mpcx_co* e) {
@@ -1795,7 +1658,7 @@ static int co_sort(mpcx_co* e) {
   hipLaunchKernelGGL(sort_scatter, dim3(grid_for(NT, 256)), dim3(256), 0, st, (int)NT, (const int*)e->d_skey,
                      (const unsigned short*)e->d_srk, (const int*)e->d_soff, e->d_sb);
   hipLaunchKernelGGL(sort_order, dim3((unsigned)((vol + 255) / 256)), dim3(256), 0, st, vol, (const int*)e->d_scnt,
-                     (const int*)e->d_soff, e->d_sb);
+                     (const int*)e->d_soff, e->d_sb, BMAX);
   hipLaunchKernelGGL(fill_kernel<unsigned>, dim3(grid_for(e->nwords + 1, 256)), dim3(256), 0, st, e->alt_bits,
                      e->nwords + 1, 0u);
   hipLaunchKernelGGL(sort_gather, dim3(grid_for(NT, 256)), dim3(256), 0, st, P, D, (const int*)e->d_sb, e->alt[0],
@@ -1807,13 +1670,8 @@ static int co_sort(mpcx_co* e) {
 }
 
 static void co_near_list(mpcx_co* e) {
-  const long long nw = e->nwords;
-  int nb = (int)((nw + 1023) / 1024);
-  if (nb < 1) nb = 1;
-  hipLaunchKernelGGL(bits_count, dim3(nb), dim3(1024), 0, e->st, (const unsigned*)e->D.near_bits, nw, e->D.partial);
-  hipLaunchKernelGGL(scan_top, dim3(1), dim3(1024), 0, e->st, e->D.partial, nb);
-  hipLaunchKernelGGL(bits_emit, dim3(nb), dim3(1024), 0, e->st, (const unsigned*)e->D.near_bits, nw,
-                     (const int*)e->D.partial, e->D.near_list, e->D.near_cap, e->P.NT, e->D.sc, nb);
+  launch_bits_list(e->st, (const unsigned*)e->D.near_bits, e->nwords, e->D.partial, e->D.near_list, e->D.near_cap,
+                   e->P.NT, &e->D.sc->near_num, &e->D.sc->err);
 }
 
 static void launch_near(mpcx_co* e, int nsteps, int mode) {

@@ -25,6 +25,7 @@ enum DevError : int {
   ERR_EVENTS = 9,          // more rebuild events in one interval than recorded
   ERR_RATTLE = 10,         // the bond constraint did not converge
   ERR_INSIDE_LOST = 11,    // a list member missing from the next interval's near set
+  ERR_DRAWS = 12,          // the host draw buffer ran out before the run ended
 };
 
 __device__ __forceinline__ void raise_error(int* err, int code) { atomicCAS(err, 0, code); }

@@ -1162,7 +1162,7 @@ int or_sk_time_step(or_sk_params *p, or_sk_state *s, or_mt95_t *rng, int64_t cst
     or_sk_forces(p, s, rng);
     return 0;
   }
-  if (cstep % 10000 == 0 && or_sk_sort(p, s)) return -1;
+  if (cstep % (p->sort_step > 0 ? p->sort_step : 10000) == 0 && or_sk_sort(p, s)) return -1;
   or_sk_motion(p, s);
   if (or_sk_distribute(p, s)) return -1;
   or_sk_forces(p, s, rng);
@@ -1226,6 +1226,7 @@ int or_sk_run(int64_t *ip, double *rp, int32_t seed, int64_t cstep0, int64_t nst
   memset(&p, 0, sizeof p);
   const int64_t space = ip[1];
   p.NT = ip[0];
+  p.sort_step = ip[2];
   for (int i = 0; i < 3; i++) {
     p.box[i] = rp[i]; p.hbox[i] = 0.5 * rp[i]; p.ibox[i] = (int64_t)rp[i];
   }

@@ -125,6 +125,7 @@ typedef struct {
   double cutoffC, cutoffN, cutoffC2, cutoffN2, A0;
   double k1, k2, k3, k4, k5, k6;
   double sinrot, cosrot;
+  int64_t sort_step;   /* Main_SelkovDimer.f90:131 sorts every 10000 steps; other values only in tests */
 } or_sk_params;
 typedef struct {
   int64_t *flag;

@@ -172,7 +172,7 @@ def co_run(sysd: dict, nsteps: int, do_setup: int = 0, use_ref: bool = False, to
 
 # --------------------------------------------------------------- selkov driver
 def sk_blocks(sysd: dict):
-    ip = np.array([sysd["NT"], sysd["space"]], dtype=np.int64)
+    ip = np.array([sysd["NT"], sysd["space"], sysd.get("sort_step", 10000)], dtype=np.int64)
     rp = np.zeros(29, np.float64)
     rp[0:3] = sysd["box"]
     keys = ["Mf", "Mg", "Mi", "Rc", "Rn", "Dcn", "EfC", "EgC", "EiC", "EfN", "EgN", "EiN", "RPfg", "RPgf",
