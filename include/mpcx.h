@@ -223,6 +223,37 @@ int mpcx_sk_get_cells(mpcx_sk* e, double* rec8, int32_t* cell_of, int32_t* censu
  * sk_collide, sk_rotate; and the run's wall time. */
 int mpcx_sk_kernel_ms(mpcx_sk* e, double* ms4, double* run_ms);
 
+/* ------------------------------------------------------------------------
+ * Diagnostics on the device-resident state (either engine)
+ * ---------------------------------------------------------------------- */
+/* out16: [0] particles counted, [1..3] per species (A B S / F G I), [4..6]
+ * solvent momentum, [7..9] total momentum with the dimers, [10] solvent and
+ * [11] dimer kinetic energy, [12] solvent temperature (2/3)KE/N, [13] dimer
+ * (2/3)KE, [14] total energy with TPotential (collective_code/output_mod.f95:
+ * 681-805), [15] propulsion velocity of dimer 0: centre-of-mass velocity
+ * along the N->C unit vector (selkov_code/Output_SelkovDimer.f90:147-173). */
+int mpcx_co_observables(mpcx_co* e, double* out16);
+int mpcx_sk_observables(mpcx_sk* e, double* out16);
+/* A/B counts in the concentration slabs of collective_code/output_mod.f95:
+ * 486-568 (bins ceiling(x/width); divide by the slab volume for densities):
+ * A1d, B1d (nx), Axy, Bxy (nx,ny), Axz, Bxz (nx,nz), Ayz, Byz (ny,nz), each
+ * column-major, with n? = int(box/width). */
+int mpcx_co_conc_slabs(mpcx_co* e, double x_slab, double y_slab, double z_slab, double half_width,
+                       double x_width, double y_width, double z_width, int32_t* counts);
+/* F, G, I counts (3, 8) in the shells around the C sphere of
+ * selkov_code/Output_Concentration_Catalytic.f90:50-156 (R^2 bounds 4, 9, 16,
+ * 36, 64, 100, 144, 225, 400). */
+int mpcx_sk_shells(mpcx_sk* e, int64_t* counts24);
+/* Counts per (species, z bin, r bin), (3, nz, nr) row-major, in the body frame
+ * of a dimer: origin at its C sphere, z along the N->C unit vector, r the
+ * distance from that axis; z in [zmin, zmax), r in [0, rmax).  No reference
+ * counterpart (its analogues are the shells above); the time average of these
+ * fields trains the surrogate. */
+int mpcx_co_body_density(mpcx_co* e, int32_t dimer, int32_t nr, int32_t nz, double rmax, double zmin, double zmax,
+                         uint32_t* counts);
+int mpcx_sk_body_density(mpcx_sk* e, int32_t nr, int32_t nz, double rmax, double zmin, double zmax,
+                         uint32_t* counts);
+
 #ifdef __cplusplus
 }
 #endif

@@ -183,6 +183,39 @@ class CollectiveEngine:
                     particle_us=us(a[2]) / n, serial_us=us(a[3]) / n, rebuilds=int(a[4]) / n,
                     near_num=int(a[5]), inside_num=int(a[6]))
 
+    OBSERVABLES = ("n", "nA", "nB", "nS", "psol_x", "psol_y", "psol_z", "p_x", "p_y", "p_z", "ke_sol", "ke_dim",
+                   "T_sol", "T_dim", "E_tot", "v_prop")
+
+    def observables(self) -> dict:
+        """System observables of output_mod.f95:681-805 and the propulsion velocity."""
+        a = np.zeros(16)
+        check(lib().mpcx_co_observables(self.h, ptr(a)), "mpcx_co_observables")
+        return dict(zip(self.OBSERVABLES, a.tolist()))
+
+    def conc_slabs(self, x_slab, y_slab, z_slab, half_width, width=0.5) -> dict:
+        """A/B counts of the concentration slabs (output_mod.f95:486-568)."""
+        L = [int(b) for b in self.p["box"]]
+        nx, ny, nz = (int(v / width) for v in L)
+        sizes = [("A1d", (nx,)), ("B1d", (nx,)), ("Axy", (nx, ny)), ("Bxy", (nx, ny)), ("Axz", (nx, nz)),
+                 ("Bxz", (nx, nz)), ("Ayz", (ny, nz)), ("Byz", (ny, nz))]
+        buf = np.zeros(sum(int(np.prod(sh)) for _, sh in sizes), np.int32)
+        check(lib().mpcx_co_conc_slabs(self.h, C.c_double(x_slab), C.c_double(y_slab), C.c_double(z_slab),
+                                       C.c_double(half_width), C.c_double(width), C.c_double(width),
+                                       C.c_double(width), ptr(buf)), "mpcx_co_conc_slabs")
+        out, o = {}, 0
+        for k, sh in sizes:
+            n = int(np.prod(sh))
+            out[k] = buf[o:o + n].reshape(sh, order="F")
+            o += n
+        return out
+
+    def body_density(self, nr=32, nz=64, rmax=16.0, zmin=-24.0, zmax=16.0, dimer=0) -> np.ndarray:
+        """Counts (3, nz, nr) per species in the body frame of a dimer."""
+        a = np.zeros((3, nz, nr), np.uint32)
+        check(lib().mpcx_co_body_density(self.h, C.c_int32(dimer), C.c_int32(nr), C.c_int32(nz), C.c_double(rmax),
+                                         C.c_double(zmin), C.c_double(zmax), ptr(a)), "mpcx_co_body_density")
+        return a
+
     KERNELS = ("co_near_engine", "co_bulk", "co_collide", "co_rotate", "co_scatter")
 
     def kernel_ms(self) -> dict:

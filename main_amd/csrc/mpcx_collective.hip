@@ -1884,6 +1884,32 @@ extern "C" int mpcx_co_near_profile(mpcx_co* e, uint64_t* prof8) {
   return 0;
 }
 
+namespace mpcx {
+int co_view(mpcx_co* e, DevView* v) {
+  if (!e || !v) return set_error(MPCX_E_ARG, "null argument");
+  if (hipStreamSynchronize(e->st) != hipSuccess) return set_error(MPCX_E_HIP, "stream synchronize failed");
+  memset(v, 0, sizeof *v);
+  const CoD& D = e->D;
+  v->x = D.x; v->y = D.y; v->z = D.z; v->vx = D.vx; v->vy = D.vy; v->vz = D.vz; v->flag = D.flag;
+  v->n = e->P.NT;
+  for (int i = 0; i < 3; i++) v->box[i] = e->P.box[i];
+  v->mass[0] = e->cfg.Ma; v->mass[1] = e->cfg.Mb; v->mass[2] = e->cfg.Ms;
+  v->species[0] = 0; v->species[1] = 1; v->species[2] = 2; v->species[3] = -1;
+  v->periodic_z = 0;
+  v->nd = e->nd;
+  std::vector<double> dim(12 * MAXS);
+  if (hipMemcpy(dim.data(), D.dim, dim.size() * 8, hipMemcpyDeviceToHost) != hipSuccess)
+    return set_error(MPCX_E_HIP, "hipMemcpy failed");
+  for (int f = 0; f < 12; f++)
+    for (int s = 0; s < 2 * e->nd; s++) v->dim[f][s] = dim[f * MAXS + s];
+  v->msph[0] = e->cfg.Mc; v->msph[1] = e->cfg.Mn;
+  if (hipMemcpy(&v->tpotential, &D.sc->tpotential, 8, hipMemcpyDeviceToHost) != hipSuccess)
+    return set_error(MPCX_E_HIP, "hipMemcpy failed");
+  v->stream = e->st;
+  return 0;
+}
+}  // namespace mpcx
+
 extern "C" int mpcx_co_kernel_ms(mpcx_co* e, double* ms5, int64_t* count) {
   if (!e || !ms5) return set_error(MPCX_E_ARG, "null argument");
   for (int k = 0; k < mpcx_co::NKT; k++) ms5[k] = e->kt_ms[k];

@@ -2,6 +2,9 @@
 #pragma once
 #include <stdint.h>
 
+struct mpcx_co;
+struct mpcx_sk;
+
 namespace mpcx {
 // records the message of the last failure (per calling thread) and returns code
 int set_error(int code, const char* msg);
@@ -12,4 +15,22 @@ inline double ipow(double x, int n) {
   for (int i = 1; i < n; i++) r = r * x;
   return r;
 }
+
+// Device arrays of one engine's system, for the diagnostics kernels.
+struct DevView {
+  const double *x, *y, *z, *vx, *vy, *vz;
+  const int* flag;
+  long long n;            // particle slots
+  double box[3];
+  double mass[4];         // particle mass by flag value 0..3 (0 for flags that are not particles)
+  int species[4];         // species column of each flag value (-1: not counted)
+  int periodic_z;         // collective_code has walls in z
+  double dim[12][4];      // sphere fields (x y z vx vy vz fx fy fz ...) of up to two dimers, s = k + i*nd
+  int nd;                 // dimers
+  double msph[2];         // C and N sphere masses
+  double tpotential;
+  void* stream;           // hipStream_t of the engine
+};
+int co_view(struct ::mpcx_co* e, DevView* v);
+int sk_view(struct ::mpcx_sk* e, DevView* v);
 }  // namespace mpcx

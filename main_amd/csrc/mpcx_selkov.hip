@@ -1327,6 +1327,34 @@ extern "C" int mpcx_sk_get_cells(mpcx_sk* e, double* rec8, int32_t* cell_of, int
   return 0;
 }
 
+namespace mpcx {
+int sk_view(mpcx_sk* e, DevView* v) {
+  if (!e || !v) return set_error(MPCX_E_ARG, "null argument");
+  if (hipStreamSynchronize(e->st) != hipSuccess) return set_error(MPCX_E_HIP, "stream synchronize failed");
+  memset(v, 0, sizeof *v);
+  const SkD& D = e->D;
+  v->x = D.x; v->y = D.y; v->z = D.z; v->vx = D.vx; v->vy = D.vy; v->vz = D.vz; v->flag = D.flag;
+  v->n = e->P.space;
+  for (int i = 0; i < 3; i++) v->box[i] = e->P.box[i];
+  v->mass[0] = e->cfg.Mf; v->mass[1] = e->cfg.Mg; v->mass[3] = e->cfg.Mi;
+  v->species[0] = 0; v->species[1] = 1; v->species[2] = -1; v->species[3] = 2;
+  v->periodic_z = 1;
+  v->nd = 1;
+  std::vector<double> dim(24);
+  if (hipMemcpy(dim.data(), D.dim, 24 * 8, hipMemcpyDeviceToHost) != hipSuccess)
+    return set_error(MPCX_E_HIP, "hipMemcpy failed");
+  for (int f = 0; f < 12; f++)
+    for (int s = 0; s < 2; s++) v->dim[f][s] = dim[f * 2 + s];
+  v->msph[0] = e->cfg.Mc; v->msph[1] = e->cfg.Mn;
+  SkScalars sc;
+  if (hipMemcpy(&sc, D.sc, sizeof sc, hipMemcpyDeviceToHost) != hipSuccess)
+    return set_error(MPCX_E_HIP, "hipMemcpy failed");
+  v->tpotential = sc.tpotential;
+  v->stream = e->st;
+  return 0;
+}
+}  // namespace mpcx
+
 extern "C" int mpcx_sk_kernel_ms(mpcx_sk* e, double* ms4, double* run_ms) {
   if (!e || !ms4) return set_error(MPCX_E_ARG, "null argument");
   for (int k = 0; k < mpcx_sk::NKT; k++) ms4[k] = e->kt_ms[k];

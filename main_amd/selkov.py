@@ -127,6 +127,26 @@ class SelkovEngine:
                     ncn=cell.astype(np.int64) + 1, tF=census[0].astype(np.int64), tG=census[1].astype(np.int64),
                     tI=census[2].astype(np.int64), affected=(unaff == 0))
 
+    OBSERVABLES = ("n", "nF", "nG", "nI", "psol_x", "psol_y", "psol_z", "p_x", "p_y", "p_z", "ke_sol", "ke_dim",
+                   "T_sol", "T_dim", "E_tot", "v_prop")
+
+    def observables(self) -> dict:
+        a = np.zeros(16)
+        check(lib().mpcx_sk_observables(self.h, ptr(a)), "mpcx_sk_observables")
+        return dict(zip(self.OBSERVABLES, a.tolist()))
+
+    def shells(self) -> np.ndarray:
+        """F, G, I counts (3, 8) in the shells of Output_Concentration_Catalytic.f90."""
+        a = np.zeros((3, 8), np.int64)
+        check(lib().mpcx_sk_shells(self.h, ptr(a)), "mpcx_sk_shells")
+        return a
+
+    def body_density(self, nr=32, nz=64, rmax=16.0, zmin=-24.0, zmax=16.0) -> np.ndarray:
+        a = np.zeros((3, nz, nr), np.uint32)
+        check(lib().mpcx_sk_body_density(self.h, C.c_int32(nr), C.c_int32(nz), C.c_double(rmax), C.c_double(zmin),
+                                         C.c_double(zmax), ptr(a)), "mpcx_sk_body_density")
+        return a
+
     KERNELS = ("sk_near_engine", "sk_bulk", "sk_collide", "sk_rotate")
 
     def kernel_ms(self) -> dict:
