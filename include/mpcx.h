@@ -254,6 +254,31 @@ int mpcx_co_body_density(mpcx_co* e, int32_t dimer, int32_t nr, int32_t nz, doub
 int mpcx_sk_body_density(mpcx_sk* e, int32_t nr, int32_t nz, double rmax, double zmin, double zmax,
                          uint32_t* counts);
 
+/* ------------------------------------------------------------------------
+ * MLP surrogate (no reference counterpart): parameters of a run -> the
+ * time-averaged body-frame density field and propulsion velocity.  FP32
+ * master weights; every GEMM runs on the MFMA units with BF16 operands and
+ * FP32 accumulation (forward, both backward products); ReLU hidden layers,
+ * linear output, weighted squared error, Adam.
+ * ---------------------------------------------------------------------- */
+typedef struct mpcx_mlp mpcx_mlp;
+int mpcx_mlp_create(mpcx_mlp** out, int32_t n_in, int32_t n_hidden, int32_t n_layers, int32_t n_out, uint64_t seed);
+int mpcx_mlp_destroy(mpcx_mlp* e);
+/* parameters, flat FP32: for each layer W (n_next x n_prev, row-major) then b */
+int64_t mpcx_mlp_num_params(mpcx_mlp* e);
+int mpcx_mlp_get_params(mpcx_mlp* e, float* flat);
+int mpcx_mlp_set_params(mpcx_mlp* e, const float* flat);
+int mpcx_mlp_forward(mpcx_mlp* e, int32_t rows, const float* x, float* y);
+/* loss = sum_c wout[c] (y - target)^2 averaged over rows; grad (flat, may be NULL) */
+int mpcx_mlp_loss_grad(mpcx_mlp* e, int32_t rows, const float* x, const float* target, const float* wout,
+                       double* loss, float* grad);
+int mpcx_mlp_train(mpcx_mlp* e, int64_t n, const float* x, const float* target, const float* wout, int32_t batch,
+                   int32_t epochs, float lr, uint64_t seed, double* epoch_loss);
+int mpcx_mlp_train_stats(mpcx_mlp* e, double* ms, double* gemm_flops);
+/* C = A B (row-major, M x K and K x N) with BF16 operands on the MFMA units */
+int mpcx_gemm_bf16(int32_t M, int32_t N, int32_t K, const float* A, const float* B, float* C, int32_t reps,
+                   double* ms);
+
 #ifdef __cplusplus
 }
 #endif
