@@ -177,6 +177,24 @@ int mpcx_co_near_profile(mpcx_co* e, uint64_t* prof8);
 int mpcx_co_kernel_ms(mpcx_co* e, double* ms5, int64_t* count);
 
 /* ------------------------------------------------------------------------
+ * Batches of collective systems (no reference counterpart: the reference
+ * runs one system per process).  A batch advances R engines created and set
+ * up as above (rng_mode 1, same numdims, MPCtstep and sort_step; boxes,
+ * parameters and keys may differ) with one launch per kernel of an interval
+ * for all of them: the near engine with one workgroup per replica, the
+ * particle and cell kernels over tiles of `tile` particles / 256 cells.
+ * Each replica ends bit-identical to running it alone; its state is read
+ * with mpcx_co_get_state.  With use_graph, the launches of an interval
+ * without a re-sort are captured once into a HIP graph and replayed.
+ * ---------------------------------------------------------------------- */
+typedef struct mpcx_cob mpcx_cob;
+int mpcx_cob_create(mpcx_cob** out, int32_t R, mpcx_co* const* engines, int32_t tile);
+int mpcx_cob_destroy(mpcx_cob* b);
+int mpcx_cob_run(mpcx_cob* b, int64_t ncoll, int32_t use_graph);
+int mpcx_cob_set_tile(mpcx_cob* b, int32_t tile);
+int mpcx_cob_info(mpcx_cob* b, int64_t* particles, int64_t* cells, double* run_ms);
+
+/* ------------------------------------------------------------------------
  * Device-resident Selkov engine (selkov_code)
  * ---------------------------------------------------------------------- */
 typedef struct mpcx_sk mpcx_sk;

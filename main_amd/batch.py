@@ -1,0 +1,58 @@
+"""Batches of collective systems advanced together on one MI355X (see
+mpcx_cob_* in include/mpcx.h): independent replicas with their own boxes,
+parameters and Philox keys, one launch per kernel of an interval for all of
+them, the normal intervals replayed from a HIP graph.  The tile size of the
+particle kernels is chosen by `autotune`, which times short runs.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+from ._lib import check, lib
+from .collective import CollectiveEngine
+
+
+class CollectiveBatch:
+    def __init__(self, engines: list[CollectiveEngine], tile: int = 2048):
+        self.engines = engines     # kept alive: the batch works on their device state
+        arr = (C.c_void_p * len(engines))(*[e.h.value for e in engines])
+        self.h = C.c_void_p()
+        check(lib().mpcx_cob_create(C.byref(self.h), C.c_int32(len(engines)), arr, C.c_int32(tile)),
+              "mpcx_cob_create")
+        self.tile = tile
+
+    def close(self):
+        if self.h:
+            lib().mpcx_cob_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, ncoll: int, use_graph: bool = True):
+        check(lib().mpcx_cob_run(self.h, C.c_int64(ncoll), C.c_int32(1 if use_graph else 0)), "mpcx_cob_run")
+
+    def set_tile(self, tile: int):
+        check(lib().mpcx_cob_set_tile(self.h, C.c_int32(tile)), "mpcx_cob_set_tile")
+        self.tile = tile
+
+    def info(self) -> dict:
+        p, c, ms = C.c_int64(0), C.c_int64(0), C.c_double(0)
+        check(lib().mpcx_cob_info(self.h, C.byref(p), C.byref(c), C.byref(ms)), "mpcx_cob_info")
+        return dict(particles=p.value, cells=c.value, run_ms=ms.value)
+
+    def autotune(self, tiles=(512, 1024, 2048, 4096, 8192), ncoll: int = 4, use_graph: bool = True) -> dict:
+        """Time `ncoll` intervals at each tile size (advancing the systems) and
+        keep the fastest; returns {tile: ms per interval}."""
+        times = {}
+        for t in tiles:
+            self.set_tile(t)
+            self.run(1, use_graph)          # capture / warm
+            self.run(ncoll, use_graph)
+            times[t] = self.info()["run_ms"] / ncoll
+        best = min(times, key=times.get)
+        self.set_tile(best)
+        return times

@@ -59,6 +59,7 @@ struct CoScalars {
   double tpotential;
   double x0[MAXS], y0[MAXS], z0[MAXS];
   unsigned long long prof[8];   // near engine: total, rebuild, particle phase, serial (100 MHz ticks), rebuilds
+  unsigned long long coll;      // collision counter of the current interval (Philox counter)
 };
 
 struct CoP {
@@ -110,7 +111,6 @@ struct CoD {
   unsigned* firstB;       // [vol]
   const double* U;        // host draws of this collision (rng_mode 0)
   const double* ptab;     // reaction probabilities, 101 entries
-  unsigned long long coll;  // collision counter for Philox
 };
 
 // ------------------------------------------------------------- streaming
@@ -855,8 +855,11 @@ __device__ __forceinline__ void ne_save_scalars(CoScalars* sc) {
   sc->n_ev = g_ne.nev; sc->near_vmax2 = g_ne.vmax2;
 }
 
+// One interval of the near set of one system.  P and D are the launch
+// parameters (kernel arguments, or the LDS copies g_ne.P/g_ne.D when
+// `in_lds`, which then need no copy).
 template <int ND>
-__global__ __launch_bounds__(NE_T) void co_near_engine(CoP P, CoD D, int nsteps, int mode) {
+__device__ __forceinline__ void near_body(const CoP& P, const CoD& D, int nsteps, int mode, bool in_lds) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   constexpr int NSPH = 2 * ND;
   constexpr int NV = 4 * NSPH + 1;
@@ -864,8 +867,10 @@ __global__ __launch_bounds__(NE_T) void co_near_engine(CoP P, CoD D, int nsteps,
 
   for (int j = tid; j < 12 * MAXS; j += NE_T) g_ne.sd[j / MAXS][j % MAXS] = D.dim[j];
   if (tid == 0) {
-    g_ne.P = P;
-    g_ne.D = D;
+    if (!in_lds) {
+      g_ne.P = P;
+      g_ne.D = D;
+    }
     g_ne.cstep = sc->cstep; g_ne.tc = sc->time_count; g_ne.tp = sc->time_penetrate; g_ne.in_num = sc->in_num;
     g_ne.nev = (mode & M_RESUME) ? sc->n_ev : 0;
     g_ne.near_num = sc->near_num;
@@ -1002,19 +1007,40 @@ __global__ __launch_bounds__(NE_T) void co_near_engine(CoP P, CoD D, int nsteps,
   }
   for (int j = tid; j < 12 * MAXS; j += NE_T) D.dim[j] = g_ne.sd[j / MAXS][j % MAXS];
 }
+
+template <int ND>
+__global__ __launch_bounds__(NE_T) void co_near_engine(CoP P, CoD D, int nsteps, int mode) {
+  near_body<ND>(P, D, nsteps, mode, false);
+}
+// one workgroup per system of a batch
+template <int ND>
+__global__ __launch_bounds__(NE_T) void co_near_engine_b(const CoP* Ps, const CoD* Ds, int nsteps, int mode) {
+  if (threadIdx.x == 0) {
+    g_ne.P = Ps[blockIdx.x];
+    g_ne.D = Ds[blockIdx.x];
+  }
+  __syncthreads();
+  near_body<ND>(g_ne.P, g_ne.D, nsteps, mode, true);
+}
 #undef NE_KICK
 #undef NE_STREAM
 #undef SLOT
 
 // ------------------------------------------------------------- bulk kernels
+// A block of a batched launch works on elements [begin, end) of replica r.
+struct Tile {
+  int r, pad;
+  long long begin, end;
+};
+
 __device__ __forceinline__ void collision_shift(const CoP& P, const CoD& D, double& xs, double& ys, double& zs) {
   double u0, u1, u2;
   if (P.rng_mode == 0) {
     u0 = D.U[0]; u1 = D.U[1]; u2 = D.U[2];
   } else {
     double t;
-    philox_uniform2(P.seed, 0, D.coll, 0, u0, u1);
-    philox_uniform2(P.seed, 0, D.coll, 1, u2, t);
+    philox_uniform2(P.seed, 0, D.sc->coll, 0, u0, u1);
+    philox_uniform2(P.seed, 0, D.sc->coll, 1, u2, t);
   }
   xs = P.a0 * (u0 - 0.5);
   ys = P.a0 * (u1 - 0.5);
@@ -1040,7 +1066,8 @@ __device__ __forceinline__ long long shifted_cell(const CoP& P, double x, double
 // particle into the shifted grid (count + rank, by the cell's tour position)
 // and into the census; record its distance to the nearest sphere for the
 // selection of the next interval's near set.
-__global__ __launch_bounds__(256) void co_bulk(CoP P, CoD D, int do_stream, int do_bin) {
+__device__ __forceinline__ void bulk_body(const CoP& P, const CoD& D, int do_stream, int do_bin, long long first,
+                                          long long end, long long step) {
   const int NT = P.NT;
   const int nev = D.sc->n_ev;
   __shared__ int ev[MAXEV];
@@ -1055,10 +1082,9 @@ __global__ __launch_bounds__(256) void co_bulk(CoP P, CoD D, int do_stream, int 
   double xs = 0, ys = 0, zs = 0;
   if (do_bin) collision_shift(P, D, xs, ys, zs);
   const int lane = threadIdx.x & 63;
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long base = (long long)blockIdx.x * blockDim.x; base < NT; base += stride) {
+  for (long long base = first; base < end; base += step) {
     const int i = (int)(base + threadIdx.x);
-    const bool valid = i < NT;
+    const bool valid = i < end;
     bool near = false;
     double x = 0, y = 0, z = 0;
     int f = -1;
@@ -1145,12 +1171,30 @@ __global__ __launch_bounds__(256) void co_bulk(CoP P, CoD D, int do_stream, int 
     }
   }
 }
+__global__ __launch_bounds__(256) void co_bulk(CoP P, CoD D, int do_stream, int do_bin) {
+  bulk_body(P, D, do_stream, do_bin, (long long)blockIdx.x * blockDim.x, P.NT, (long long)gridDim.x * blockDim.x);
+}
+__global__ __launch_bounds__(256) void co_bulk_b(const CoP* Ps, const CoD* Ds, const Tile* tiles, int do_stream,
+                                                 int do_bin) {
+  const Tile t = tiles[blockIdx.x];
+  const CoP P = Ps[t.r];
+  const CoD D = Ds[t.r];
+  bulk_body(P, D, do_stream, do_bin, t.begin, t.end, 256);
+}
 
-__global__ __launch_bounds__(256) void co_scatter(CoP P, CoD D) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.NT; i += gridDim.x * blockDim.x) {
+__device__ __forceinline__ void scatter_body(const CoD& D, long long first, long long end, long long step) {
+  for (long long i = first + threadIdx.x; i < end; i += step) {
     int r = D.rank_of[i];
-    if (r < BMAX) D.bucket[D.off[D.cell_of[i]] + r] = i;
+    if (r < BMAX) D.bucket[D.off[D.cell_of[i]] + r] = (int)i;
   }
+}
+__global__ __launch_bounds__(256) void co_scatter(CoP P, CoD D) {
+  scatter_body(D, (long long)blockIdx.x * blockDim.x, P.NT, (long long)gridDim.x * blockDim.x);
+}
+__global__ __launch_bounds__(256) void co_scatter_b(const CoD* Ds, const Tile* tiles) {
+  const Tile t = tiles[blockIdx.x];
+  const CoD D = Ds[t.r];
+  scatter_body(D, t.begin, t.end, 256);
 }
 
 // Per collision cell, in tour order (thread t <-> t-th cell of the tour, so a
@@ -1158,10 +1202,8 @@ __global__ __launch_bounds__(256) void co_scatter(CoP P, CoD D) {
 // members in ascending particle order, centre-of-mass velocity
 // (mpc_rotation_mod.f95:153-182), random axis (:184-193).  Thread t also does
 // the B->A reaction of unshifted cell t (reaction_mod.f95:41-62).
-__global__ __launch_bounds__(CT) void co_collide(CoP P, CoD D) {
-  constexpr int LB = 32;   // members sorted in LDS; larger cells sort in place in the bucket
-  __shared__ int sidx[LB * CT];
-  const long long t = (long long)blockIdx.x * CT + threadIdx.x;
+constexpr int LB = 32;   // members sorted in LDS; larger cells sort in place in the bucket
+__device__ __forceinline__ void collide_body(const CoP& P, const CoD& D, long long t, int* sidx) {
   if (t < P.mpcnumcell) {
     const int c = D.mpc_cell[t];
     int n = D.cnt[t];
@@ -1212,7 +1254,7 @@ __global__ __launch_bounds__(CT) void co_collide(CoP P, CoD D) {
       u1 = D.U[3 + 2 * (long long)c];
       u2 = D.U[4 + 2 * (long long)c];
     } else {
-      philox_uniform2(P.seed, 1, D.coll, (unsigned long long)c, u1, u2);
+      philox_uniform2(P.seed, 1, D.sc->coll, (unsigned long long)c, u1, u2);
     }
     double phi = P.twopi * u1;
     double th = 2.0 * u2 - 1.0;
@@ -1228,7 +1270,7 @@ __global__ __launch_bounds__(CT) void co_collide(CoP P, CoD D) {
     double prob = D.ptab[min(nb, CENSUS_MAX)];
     double r;
     if (P.rng_mode == 0) r = D.U[3 + 2 * P.mpcnumcell + t];
-    else { double q; philox_uniform2(P.seed, 2, D.coll, (unsigned long long)t, r, q); }
+    else { double q; philox_uniform2(P.seed, 2, D.sc->coll, (unsigned long long)t, r, q); }
     if (0.0 < r && r <= prob) {
       unsigned key = D.firstB[t];
       int i = key >= (unsigned)P.NT ? (int)(key - (unsigned)P.NT) : (int)key;
@@ -1237,13 +1279,27 @@ __global__ __launch_bounds__(CT) void co_collide(CoP P, CoD D) {
   }
 }
 
+__global__ __launch_bounds__(CT) void co_collide(CoP P, CoD D) {
+  __shared__ int sidx[LB * CT];
+  collide_body(P, D, (long long)blockIdx.x * CT + threadIdx.x, sidx);
+}
+__global__ __launch_bounds__(CT) void co_collide_b(const CoP* Ps, const CoD* Ds, const Tile* tiles) {
+  __shared__ int sidx[LB * CT];
+  const Tile t = tiles[blockIdx.x];
+  const CoP P = Ps[t.r];
+  const CoD D = Ds[t.r];
+  const long long c = t.begin + threadIdx.x;
+  if (c < t.end) collide_body(P, D, c, sidx);
+}
+
 // Rotation of mpc_rotation_mod.f95:205-230 and selection of the next
 // interval's near set: a particle is near when its distance beyond a list
 // radius (recorded by co_bulk, rounded down) is within |v| tau + guard.
-__global__ __launch_bounds__(256) void co_rotate(CoP P, CoD D, int classify) {
+__device__ __forceinline__ void rotate_body(const CoP& P, const CoD& D, int classify, long long first, long long end,
+                                            long long step) {
   double fmax2 = 0.0;
   const double tau = P.tau * (1.0 + 1e-9);
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.NT; i += gridDim.x * blockDim.x) {
+  for (long long i = first + threadIdx.x; i < end; i += step) {
     int t = D.cell_of[i];
     const double2* rec = (const double2*)(D.cellrec + (long long)t * 8);
     double2 r0 = rec[0], r1 = rec[1], r2 = rec[2];
@@ -1271,6 +1327,15 @@ __global__ __launch_bounds__(256) void co_rotate(CoP P, CoD D, int classify) {
     double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
     atomicMax(&D.sc->far_maxv2, (unsigned long long)__double_as_longlong(m));
   }
+}
+__global__ __launch_bounds__(256) void co_rotate(CoP P, CoD D, int classify) {
+  rotate_body(P, D, classify, (long long)blockIdx.x * blockDim.x, P.NT, (long long)gridDim.x * blockDim.x);
+}
+__global__ __launch_bounds__(256) void co_rotate_b(const CoP* Ps, const CoD* Ds, const Tile* tiles) {
+  const Tile t = tiles[blockIdx.x];
+  const CoP P = Ps[t.r];
+  const CoD D = Ds[t.r];
+  rotate_body(P, D, 1, t.begin, t.end, 256);
 }
 
 // Near set from particle positions (set-up, before any collision).
@@ -1307,6 +1372,7 @@ __global__ __launch_bounds__(256) void co_classify_all(CoP P, CoD D) {
 }
 
 __global__ void reset_far(CoScalars* sc) { sc->far_maxv2 = 0ull; }
+__global__ void set_coll(CoScalars* sc, unsigned long long v) { sc->coll = v; }
 
 // --------------------------------------------------------------- Hilbert sort
 // sort_hilbert_mod.f95:50-92: stable bin by tour position of the unshifted cell
@@ -1555,9 +1621,7 @@ extern "C" int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg) {
   HIPCHK(hipEventCreate(&e->e0)); HIPCHK(hipEventCreate(&e->e1));
   HIPCHK(hipEventCreate(&e->b0)); HIPCHK(hipEventCreate(&e->b1));
   HIPCHK(hipEventCreate(&e->n0)); HIPCHK(hipEventCreate(&e->n1));
-  e->tev.assign((size_t)mpcx_co::TMAX * mpcx_co::NKT * 2, nullptr);
   e->tev_on.assign((size_t)mpcx_co::TMAX * mpcx_co::NKT, 0);
-  for (auto& ev : e->tev) HIPCHK(hipEventCreate(&ev));
   HIPCHK(hipMemcpy(ptab, cfg->reaction_p, 101 * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemset(D.sc, 0, sizeof(CoScalars)));
   HIPCHK(hipMemset(D.status, 0, NT));
@@ -1644,12 +1708,11 @@ extern "C" int mpcx_co_set_state(mpcx_co* e, int32_t cstep, const double* x, con
   return 0;
 }
 
-static int co_sort(mpcx_co* e) {
+static int co_sort(mpcx_co* e, hipStream_t st) {
   CoD& D = e->D;
   CoP& P = e->P;
   const long long NT = P.NT, vol = P.vol;
   if (!e->has_tour) return set_error(MPCX_E_ARG, "a sort step needs the cell tour (pass it to mpcx_co_set_state)");
-  hipStream_t st = e->st;
   hipLaunchKernelGGL(fill_kernel<int>, dim3(grid_for(vol, 256)), dim3(256), 0, st, e->d_scnt, vol, 0);
   hipLaunchKernelGGL(sort_bin, dim3(grid_for(NT, 256)), dim3(256), 0, st, P, D, (const int*)e->d_tourpos, e->d_scnt,
                      e->d_skey, e->d_srk);
@@ -1697,7 +1760,7 @@ extern "C" int mpcx_co_setup(mpcx_co* e, int32_t do_sort) {
   hipLaunchKernelGGL(fill_kernel<unsigned>, dim3(grid_for(e->nwords + 1, 256)), dim3(256), 0, st, D.near_bits,
                      e->nwords + 1, 0u);
   if (do_sort) {
-    int r = co_sort(e);
+    int r = co_sort(e, st);
     if (r) return r;
   }
   hipLaunchKernelGGL(fill_kernel<unsigned>, dim3(grid_for(e->nwords + 1, 256)), dim3(256), 0, st, D.near_bits,
@@ -1726,12 +1789,16 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
   int cstep = 0;
   HIPCHK(hipMemcpy(&cstep, &D.sc->cstep, sizeof(int), hipMemcpyDeviceToHost));
   e->bulk_ms = 0; e->near_ms = 0;
+  if (e->tev.empty()) {   // per-kernel timing events, made on the first run
+    e->tev.assign((size_t)mpcx_co::TMAX * mpcx_co::NKT * 2, nullptr);
+    for (auto& ev : e->tev) HIPCHK(hipEventCreate(&ev));
+  }
   HIPCHK(hipEventRecord(e->e0, st));
   const int gN = grid_for(NT, 256, 2048);
   for (int64_t c = 0; c < ncoll; c++) {
     const int cnext = cstep + P.MPCtstep;
     const bool sort_due = (cnext % P.sort_step) == 0;
-    D.coll = (unsigned long long)(cnext / P.MPCtstep);
+    hipLaunchKernelGGL(set_coll, dim3(1), dim3(1), 0, st, D.sc, (unsigned long long)(cnext / P.MPCtstep));
     if (P.rng_mode == 0)
       HIPCHK(hipMemcpyAsync(e->d_U, draws + c * e->ndraw, e->ndraw * 8, hipMemcpyHostToDevice, st));
     const bool timed = c < mpcx_co::TMAX;
@@ -1752,7 +1819,7 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
     hipLaunchKernelGGL(fill_kernel<unsigned>, dim3(grid_for(vol, 256)), dim3(256), 0, st, D.firstB, vol, 0xffffffffu);
     if (sort_due) {
       hipLaunchKernelGGL(co_bulk, dim3(gN), dim3(256), 0, st, P, D, 1, 0);
-      int r = co_sort(e);
+      int r = co_sort(e, st);
       if (r) return r;
       co_near_list(e);
       launch_near(e, 1, M_RESUME);
@@ -1923,4 +1990,312 @@ extern "C" int mpcx_co_timing(mpcx_co* e, double* run_ms, double* bulk_ms, doubl
   if (bulk_ms) *bulk_ms = e->bulk_ms;
   if (near_ms) *near_ms = e->near_ms;
   return 0;
+}
+
+// =====================================================================================
+// Batches: many independent collective systems ("replicas") advanced together.
+// Each replica is an ordinary engine (its own device arrays, parameters and
+// Philox key); the batch launches every kernel of an interval once for all of
+// them: the near engine with one workgroup per replica, the particle and cell
+// kernels over a table of tiles (replica, first, last).  A replica's results
+// are bit-identical to running it alone.
+// =====================================================================================
+namespace mpcx {
+
+// per replica: exclusive scan of the collision-cell counts (one workgroup each)
+__global__ __launch_bounds__(1024) void scan_cells_b(const CoP* Ps, const CoD* Ds) {
+  const CoD D = Ds[blockIdx.x];
+  const long long n = Ps[blockIdx.x].mpcnumcell;
+  __shared__ unsigned long long s[17];
+  __shared__ long long run;
+  if (threadIdx.x == 0) run = 0;
+  __syncthreads();
+  for (long long b0 = 0; b0 < n; b0 += 4096) {
+    int v[4];
+    unsigned long long sum = 0;
+    for (int j = 0; j < 4; j++) {
+      const long long i = b0 + threadIdx.x * 4 + j;
+      v[j] = i < n ? D.cnt[i] : 0;
+      sum += v[j];
+    }
+    unsigned long long tot;
+    const unsigned long long e = block_excl_scan64(sum, s, tot);
+    long long r = run + (long long)e;
+    for (int j = 0; j < 4; j++) {
+      const long long i = b0 + threadIdx.x * 4 + j;
+      if (i < n) D.off[i] = (int)r;
+      r += v[j];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) run += (long long)tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) D.off[n] = (int)run;
+}
+
+// per replica: the ordered near list from its bitmap (one workgroup each)
+__global__ __launch_bounds__(1024) void near_list_b(const CoP* Ps, const CoD* Ds) {
+  const CoD D = Ds[blockIdx.x];
+  const int NT = Ps[blockIdx.x].NT;
+  const long long nwords = (NT + 31) / 32;
+  __shared__ unsigned long long s[17];
+  __shared__ long long run;
+  if (threadIdx.x == 0) run = 0;
+  __syncthreads();
+  for (long long w0 = 0; w0 < nwords; w0 += 1024) {
+    const long long w = w0 + threadIdx.x;
+    unsigned word = w < nwords ? D.near_bits[w] : 0u;
+    unsigned long long tot;
+    const unsigned long long e = block_excl_scan64(__popc(word), s, tot);
+    long long pos = run + (long long)e;
+    while (word) {
+      const int b = __ffs(word) - 1;
+      word &= word - 1;
+      const long long idx = w * 32 + b;
+      if (idx < NT) {
+        if (pos < D.near_cap) D.near_list[pos] = (int)idx;
+        pos++;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) run += (long long)tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    D.sc->near_num = run < D.near_cap ? (int)run : D.near_cap;
+    if (run > D.near_cap) raise_error(&D.sc->err, ERR_NEAR_CAP);
+  }
+}
+
+// per replica (blockIdx.y): clear the collision-cell and census counters, next collision
+__global__ __launch_bounds__(256) void fills_begin_b(const CoP* Ps, const CoD* Ds) {
+  const CoD D = Ds[blockIdx.y];
+  const CoP& P = Ps[blockIdx.y];
+  const long long mc = P.mpcnumcell, vol = P.vol;
+  const long long i0 = blockIdx.x * 256ll + threadIdx.x, st = (long long)gridDim.x * 256;
+  for (long long i = i0; i < mc; i += st) D.cnt[i] = 0;
+  for (long long i = i0; i < 3 * vol; i += st) D.tcount[i] = 0;
+  for (long long i = i0; i < vol; i += st) D.firstB[i] = 0xffffffffu;
+  if (blockIdx.x == 0 && threadIdx.x == 0) D.sc->coll += 1;
+}
+// per replica: clear the near bitmap and the far speed maximum before the rotation
+__global__ __launch_bounds__(256) void fills_rotate_b(const CoP* Ps, const CoD* Ds) {
+  const CoD D = Ds[blockIdx.y];
+  const long long nw = (Ps[blockIdx.y].NT + 31) / 32 + 1;
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < nw; i += (long long)gridDim.x * 256) D.near_bits[i] = 0u;
+  if (blockIdx.x == 0 && threadIdx.x == 0) D.sc->far_maxv2 = 0ull;
+}
+
+}  // namespace mpcx
+
+struct mpcx_cob {
+  int R = 0, nd = 1, S = 0, sort_step = 0;
+  std::vector<mpcx_co*> eng;
+  CoP* d_P = nullptr;
+  CoD* d_D = nullptr;
+  std::vector<CoD> h_D;
+  Tile *d_pt = nullptr, *d_ct = nullptr;
+  int n_pt = 0, n_ct = 0, tile = 2048;
+  hipStream_t st = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  double run_ms = 0;
+  long long particles = 0, cells = 0;
+  int fill_blocks = 64;
+};
+
+static int cob_tiles(mpcx_cob* b) {
+  std::vector<Tile> pt, ct;
+  for (int r = 0; r < b->R; r++) {
+    const CoP& P = b->eng[r]->P;
+    for (long long i = 0; i < P.NT; i += b->tile) pt.push_back(Tile{r, 0, i, std::min<long long>(i + b->tile, P.NT)});
+    for (long long t = 0; t < P.mpcnumcell; t += CT)
+      ct.push_back(Tile{r, 0, t, std::min<long long>(t + CT, P.mpcnumcell)});
+  }
+  if (b->d_pt) (void)hipFree(b->d_pt);
+  if (b->d_ct) (void)hipFree(b->d_ct);
+  b->d_pt = b->d_ct = nullptr;
+  HIPCHK(hipMalloc(&b->d_pt, pt.size() * sizeof(Tile)));
+  HIPCHK(hipMalloc(&b->d_ct, ct.size() * sizeof(Tile)));
+  HIPCHK(hipMemcpy(b->d_pt, pt.data(), pt.size() * sizeof(Tile), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b->d_ct, ct.data(), ct.size() * sizeof(Tile), hipMemcpyHostToDevice));
+  b->n_pt = (int)pt.size();
+  b->n_ct = (int)ct.size();
+  return 0;
+}
+
+static int cob_upload_D(mpcx_cob* b) {
+  for (int r = 0; r < b->R; r++) b->h_D[r] = b->eng[r]->D;
+  HIPCHK(hipMemcpyAsync(b->d_D, b->h_D.data(), b->R * sizeof(CoD), hipMemcpyHostToDevice, b->st));
+  HIPCHK(hipStreamSynchronize(b->st));
+  return 0;
+}
+
+static void cob_launch_near(mpcx_cob* b, int nsteps, int mode) {
+  if (b->nd == 1)
+    hipLaunchKernelGGL(co_near_engine_b<1>, dim3(b->R), dim3(NE_T), 0, b->st, b->d_P, b->d_D, nsteps, mode);
+  else
+    hipLaunchKernelGGL(co_near_engine_b<2>, dim3(b->R), dim3(NE_T), 0, b->st, b->d_P, b->d_D, nsteps, mode);
+}
+
+// the launches of one interval without a re-sort (captured into the graph)
+static void cob_interval(mpcx_cob* b) {
+  const dim3 fg(b->fill_blocks, b->R);
+  cob_launch_near(b, b->S, 0);
+  hipLaunchKernelGGL(fills_begin_b, fg, dim3(256), 0, b->st, b->d_P, b->d_D);
+  hipLaunchKernelGGL(co_bulk_b, dim3(b->n_pt), dim3(256), 0, b->st, b->d_P, b->d_D, b->d_pt, 1, 1);
+  hipLaunchKernelGGL(scan_cells_b, dim3(b->R), dim3(1024), 0, b->st, b->d_P, b->d_D);
+  hipLaunchKernelGGL(co_scatter_b, dim3(b->n_pt), dim3(256), 0, b->st, b->d_D, b->d_pt);
+  hipLaunchKernelGGL(co_collide_b, dim3(b->n_ct), dim3(CT), 0, b->st, b->d_P, b->d_D, b->d_ct);
+  hipLaunchKernelGGL(fills_rotate_b, fg, dim3(256), 0, b->st, b->d_P, b->d_D);
+  hipLaunchKernelGGL(co_rotate_b, dim3(b->n_pt), dim3(256), 0, b->st, b->d_P, b->d_D, b->d_pt);
+  hipLaunchKernelGGL(near_list_b, dim3(b->R), dim3(1024), 0, b->st, b->d_P, b->d_D);
+}
+
+// an interval that ends in a re-sort: the near engines stop before the last
+// step's list rebuild, every replica is sorted, then they resume
+static int cob_sort_interval(mpcx_cob* b) {
+  const dim3 fg(b->fill_blocks, b->R);
+  cob_launch_near(b, b->S, M_STOP);
+  hipLaunchKernelGGL(fills_begin_b, fg, dim3(256), 0, b->st, b->d_P, b->d_D);
+  hipLaunchKernelGGL(co_bulk_b, dim3(b->n_pt), dim3(256), 0, b->st, b->d_P, b->d_D, b->d_pt, 1, 0);
+  for (int r = 0; r < b->R; r++) {
+    int rc = co_sort(b->eng[r], b->st);
+    if (rc) return rc;
+  }
+  int rc = cob_upload_D(b);
+  if (rc) return rc;
+  hipLaunchKernelGGL(near_list_b, dim3(b->R), dim3(1024), 0, b->st, b->d_P, b->d_D);
+  cob_launch_near(b, 1, M_RESUME);
+  hipLaunchKernelGGL(co_bulk_b, dim3(b->n_pt), dim3(256), 0, b->st, b->d_P, b->d_D, b->d_pt, 0, 1);
+  hipLaunchKernelGGL(scan_cells_b, dim3(b->R), dim3(1024), 0, b->st, b->d_P, b->d_D);
+  hipLaunchKernelGGL(co_scatter_b, dim3(b->n_pt), dim3(256), 0, b->st, b->d_D, b->d_pt);
+  hipLaunchKernelGGL(co_collide_b, dim3(b->n_ct), dim3(CT), 0, b->st, b->d_P, b->d_D, b->d_ct);
+  hipLaunchKernelGGL(fills_rotate_b, fg, dim3(256), 0, b->st, b->d_P, b->d_D);
+  hipLaunchKernelGGL(co_rotate_b, dim3(b->n_pt), dim3(256), 0, b->st, b->d_P, b->d_D, b->d_pt);
+  hipLaunchKernelGGL(near_list_b, dim3(b->R), dim3(1024), 0, b->st, b->d_P, b->d_D);
+  return 0;
+}
+
+extern "C" int mpcx_cob_destroy(mpcx_cob* b) {
+  if (!b) return 0;
+  if (b->st) (void)hipStreamSynchronize(b->st);
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->d_P) (void)hipFree(b->d_P);
+  if (b->d_D) (void)hipFree(b->d_D);
+  if (b->d_pt) (void)hipFree(b->d_pt);
+  if (b->d_ct) (void)hipFree(b->d_ct);
+  if (b->e0) (void)hipEventDestroy(b->e0);
+  if (b->e1) (void)hipEventDestroy(b->e1);
+  if (b->st) (void)hipStreamDestroy(b->st);
+  delete b;
+  return 0;
+}
+
+extern "C" int mpcx_cob_create(mpcx_cob** out, int32_t R, mpcx_co* const* engines, int32_t tile) {
+  if (!out || R < 1 || !engines) return set_error(MPCX_E_ARG, "bad arguments");
+  *out = nullptr;
+  const mpcx_co* e0 = engines[0];
+  for (int r = 0; r < R; r++) {
+    const mpcx_co* e = engines[r];
+    if (!e) return set_error(MPCX_E_ARG, "null engine in the batch");
+    if (e->P.rng_mode != 1) return set_error(MPCX_E_UNSUPPORTED, "a batch runs in rng_mode 1 (device Philox)");
+    if (e->nd != e0->nd || e->P.MPCtstep != e0->P.MPCtstep || e->P.sort_step != e0->P.sort_step)
+      return set_error(MPCX_E_UNSUPPORTED, "replicas of a batch share numdims, MPCtstep and sort_step");
+  }
+  if (tile < 256 || tile % 256 != 0) return set_error(MPCX_E_ARG, "tile must be a positive multiple of 256");
+  mpcx_cob* b = new mpcx_cob();
+  b->R = R;
+  b->nd = e0->nd;
+  b->S = e0->P.MPCtstep;
+  b->sort_step = e0->P.sort_step;
+  b->tile = tile;
+  b->eng.assign(engines, engines + R);
+  b->h_D.resize(R);
+  std::vector<CoP> hp(R);
+  for (int r = 0; r < R; r++) {
+    hp[r] = engines[r]->P;
+    b->particles += engines[r]->P.NT;
+    b->cells += engines[r]->P.mpcnumcell;
+  }
+  int rc = 0;
+  if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess || hipMalloc(&b->d_P, R * sizeof(CoP)) != hipSuccess ||
+      hipMalloc(&b->d_D, R * sizeof(CoD)) != hipSuccess || hipEventCreate(&b->e0) != hipSuccess ||
+      hipEventCreate(&b->e1) != hipSuccess)
+    rc = set_error(MPCX_E_HIP, "batch allocation failed");
+  if (!rc && hipMemcpy(b->d_P, hp.data(), R * sizeof(CoP), hipMemcpyHostToDevice) != hipSuccess)
+    rc = set_error(MPCX_E_HIP, "hipMemcpy failed");
+  if (!rc) rc = cob_tiles(b);
+  if (rc) {
+    mpcx_cob_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return 0;
+}
+
+extern "C" int mpcx_cob_run(mpcx_cob* b, int64_t ncoll, int32_t use_graph) {
+  if (!b || ncoll < 0) return set_error(MPCX_E_ARG, "bad arguments");
+  // every replica at the same collision step; engines idle on their own streams
+  int cstep = -1;
+  for (int r = 0; r < b->R; r++) {
+    mpcx_co* e = b->eng[r];
+    HIPCHK(hipStreamSynchronize(e->st));
+    int cs = 0;
+    HIPCHK(hipMemcpy(&cs, &e->D.sc->cstep, sizeof(int), hipMemcpyDeviceToHost));
+    if (cs % b->S != 0) return set_error(MPCX_E_ARG, "a batch runs from a collision step");
+    if (cstep >= 0 && cs != cstep) return set_error(MPCX_E_ARG, "replicas of a batch must be at the same step");
+    cstep = cs;
+    const unsigned long long coll = (unsigned long long)(cs / b->S);
+    HIPCHK(hipMemcpy(&e->D.sc->coll, &coll, sizeof coll, hipMemcpyHostToDevice));
+  }
+  int rc = cob_upload_D(b);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(b->e0, b->st));
+  for (int64_t c = 0; c < ncoll; c++) {
+    const int cnext = cstep + b->S;
+    if (cnext % b->sort_step == 0) {
+      if ((rc = cob_sort_interval(b))) return rc;
+    } else if (use_graph) {
+      if (!b->gexec) {
+        HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+        cob_interval(b);
+        HIPCHK(hipStreamEndCapture(b->st, &b->graph));
+        HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+      }
+      HIPCHK(hipGraphLaunch(b->gexec, b->st));
+    } else {
+      cob_interval(b);
+    }
+    cstep = cnext;
+  }
+  HIPCHK(hipEventRecord(b->e1, b->st));
+  HIPCHK(hipEventSynchronize(b->e1));
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, b->e0, b->e1);
+  b->run_ms = ms;
+  for (int r = 0; r < b->R; r++)
+    if ((rc = check_device_error(b->eng[r]))) return rc;
+  return 0;
+}
+
+// particles and collision cells of the batch, and the wall time of the last run (ms)
+extern "C" int mpcx_cob_info(mpcx_cob* b, int64_t* particles, int64_t* cells, double* run_ms) {
+  if (!b) return set_error(MPCX_E_ARG, "null batch");
+  if (particles) *particles = b->particles;
+  if (cells) *cells = b->cells;
+  if (run_ms) *run_ms = b->run_ms;
+  return 0;
+}
+
+// re-tile the particle kernels (the autotuner's knob); drops a captured graph
+extern "C" int mpcx_cob_set_tile(mpcx_cob* b, int32_t tile) {
+  if (!b || tile < 256 || tile % 256 != 0) return set_error(MPCX_E_ARG, "tile must be a positive multiple of 256");
+  HIPCHK(hipStreamSynchronize(b->st));
+  if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+  if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+  b->tile = tile;
+  return cob_tiles(b);
 }
