@@ -78,6 +78,52 @@ def cpu_baseline(L=100, budget_s=15.0):
                       f"{dt:.1f} s on one host core"}
 
 
+def batched(R=256, steps=20, warmup=4, seed=4357787, tiles=(1024, 2048, 4096, 8192)):
+    """Replica ensemble: R single-dimer systems with cubic boxes of 20-40 cells
+    (edges uniform), 10 solvent per cell, RPab uniform in [0.2, 1], EbN uniform
+    in [0.1, 1], one Philox key per replica, the other physics as the headline
+    system; advanced together by one batch (HIP-graph replay, autotuned tile)."""
+    from main_amd import systems as S
+    from main_amd.batch import CollectiveBatch
+    from main_amd.collective import CollectiveEngine
+    from main_amd._lib import device_synchronize
+    from main_amd.tour import cell_tour
+    rng = np.random.default_rng(seed)
+    engines, tours, t0 = [], {}, time.time()
+    for r in range(R):
+        L = int(rng.integers(20, 41))
+        p = S.collective_params(Lx=float(L), Ly=float(L), Lz=float(L), numdims=1, rhoa=10.0, Rc=2.0, Rn=4.0,
+                                Dcn=6.5, EbN=float(rng.uniform(0.1, 1.0)), RPab=float(rng.uniform(0.2, 1.0)),
+                                T=1.0 / 6.0, skinrad=0.3, MDt=0.01, MPCt=0.5, sort_step=500, k1=1e-3)
+        s = S.collective_initial_state(p, seed=seed + r)
+        if L not in tours:
+            tours[L] = cell_tour(L, L, L)
+        e = CollectiveEngine(p, rng_mode=1, seed=seed + r)
+        e.set_state(s, tour=tours[L])
+        e.setup(do_sort=True)
+        engines.append(e)
+    print(f"# batched: {R} replicas built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    batch = CollectiveBatch(engines, tile=2048)
+    info = batch.info()
+    times = batch.autotune(tiles=tiles, ncoll=2)
+    batch.run(warmup)
+    device_synchronize()
+    t0 = time.perf_counter()
+    batch.run(steps)
+    device_synchronize()
+    dt = time.perf_counter() - t0
+    ms = dt * 1000.0 / steps
+    out = {"replicas": R, "particles": info["particles"], "collision_cells": info["cells"],
+           "ms_per_interval": ms, "particle_updates_per_s": info["particles"] * steps / dt,
+           "replicas_per_hour_2000_collisions": R * 3600.0 / (2000 * ms * 1e-3),
+           "tile": batch.tile, "autotune_ms_per_interval": times, "rng": "device Philox4x32-10",
+           "graph": True, "dtype": "f64"}
+    batch.close()
+    for e in engines:
+        e.close()
+    return out
+
+
 def read_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed counter summary."""
     if not os.path.exists(TRAFFIC_FILE):
@@ -136,6 +182,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--probe-copy", action="store_true", help="also run the copy probe (counter calibration)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
+    ap.add_argument("--batched", type=int, default=256,
+                    help="replicas of the batched ensemble measured beside the headline (0: skip)")
+    ap.add_argument("--batched-only", action="store_true", help="measure only the replica ensemble")
     args = ap.parse_args()
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc)
@@ -150,6 +199,10 @@ def main():
     from main_amd.collective import CollectiveEngine
     from main_amd.tour import cell_tour
 
+    if args.batched_only:
+        print(json.dumps({"metric": "replicas/hour (2000-collision runs)", **batched(args.batched, args.steps,
+                                                                                   args.warmup)}), flush=True)
+        return
     if args.probe_copy:
         ms = C.c_double(0)
         check(lib().mpcx_probe_copy(C.c_int64(PROBE_BYTES), C.c_int32(3), C.byref(ms)), "mpcx_probe_copy")
@@ -207,6 +260,8 @@ def main():
                      "kernel": dom, "algorithmic_bytes_per_particle": ALGO_BYTES[dom],
                      "launch_ms": hbm[dom], "kernel_ms": kms},
     }
+    if args.batched > 0:
+        out["batched"] = batched(args.batched, steps=10, warmup=2)
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(args.size)

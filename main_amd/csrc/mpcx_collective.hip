@@ -859,7 +859,7 @@ __device__ __forceinline__ void ne_save_scalars(CoScalars* sc) {
 // parameters (kernel arguments, or the LDS copies g_ne.P/g_ne.D when
 // `in_lds`, which then need no copy).
 template <int ND>
-__device__ __forceinline__ void near_body(const CoP& P, const CoD& D, int nsteps, int mode, bool in_lds) {
+__device__ __forceinline__ void near_body(const CoP P, const CoD D, int nsteps, int mode, bool in_lds) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   constexpr int NSPH = 2 * ND;
   constexpr int NV = 4 * NSPH + 1;
@@ -1014,13 +1014,9 @@ __global__ __launch_bounds__(NE_T) void co_near_engine(CoP P, CoD D, int nsteps,
 }
 // one workgroup per system of a batch
 template <int ND>
-__global__ __launch_bounds__(NE_T) void co_near_engine_b(const CoP* Ps, const CoD* Ds, int nsteps, int mode) {
-  if (threadIdx.x == 0) {
-    g_ne.P = Ps[blockIdx.x];
-    g_ne.D = Ds[blockIdx.x];
-  }
-  __syncthreads();
-  near_body<ND>(g_ne.P, g_ne.D, nsteps, mode, true);
+__global__ __launch_bounds__(NE_T) void co_near_engine_b(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds,
+                                                         int nsteps, int mode) {
+  near_body<ND>(Ps[blockIdx.x], Ds[blockIdx.x], nsteps, mode, false);
 }
 #undef NE_KICK
 #undef NE_STREAM
@@ -1032,6 +1028,21 @@ struct Tile {
   int r, pad;
   long long begin, end;
 };
+
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b to XCD b % 8).
+// The logical block of workgroup b makes each XCD's workgroups cover one
+// contiguous stretch of the arrays, so neighbouring data shares an L2.
+__device__ __forceinline__ int xcd_block(int b, int G) {
+  const int q = G >> 3, r = G & 7, x = b & 7, k = b >> 3;
+  return x * q + min(x, r) + k;
+}
+// contiguous range [first, end) of logical block lb when n items are split over
+// G blocks in multiples of 256
+__device__ __forceinline__ void block_range(long long n, int G, int lb, long long& first, long long& end) {
+  const long long chunk = (((n + G - 1) / G) + 255) / 256 * 256;
+  first = (long long)lb * chunk;
+  end = min(first + chunk, n);
+}
 
 __device__ __forceinline__ void collision_shift(const CoP& P, const CoD& D, double& xs, double& ys, double& zs) {
   double u0, u1, u2;
@@ -1281,7 +1292,7 @@ __device__ __forceinline__ void collide_body(const CoP& P, const CoD& D, long lo
 
 __global__ __launch_bounds__(CT) void co_collide(CoP P, CoD D) {
   __shared__ int sidx[LB * CT];
-  collide_body(P, D, (long long)blockIdx.x * CT + threadIdx.x, sidx);
+  collide_body(P, D, (long long)xcd_block(blockIdx.x, gridDim.x) * CT + threadIdx.x, sidx);
 }
 __global__ __launch_bounds__(CT) void co_collide_b(const CoP* Ps, const CoD* Ds, const Tile* tiles) {
   __shared__ int sidx[LB * CT];
@@ -1376,9 +1387,9 @@ __global__ void set_coll(CoScalars* sc, unsigned long long v) { sc->coll = v; }
 
 // --------------------------------------------------------------- Hilbert sort
 // sort_hilbert_mod.f95:50-92: stable bin by tour position of the unshifted cell
-__global__ __launch_bounds__(256) void sort_bin(CoP P, CoD D, const int* tourpos, int* scnt, int* key,
-                                                unsigned short* rk) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.NT; i += gridDim.x * blockDim.x) {
+__device__ __forceinline__ void sort_bin_body(const CoP& P, const CoD& D, const int* tourpos, int* scnt, int* key,
+                                              unsigned short* rk, long long first, long long end, long long step) {
+  for (long long i = first + threadIdx.x; i < end; i += step) {
     long long u = cell_unshifted0(D.x[i], D.y[i], D.z[i], P.box[0], P.box[1]);
     if (u < 0 || u >= P.vol || tourpos[u] < 0) { raise_error(&D.sc->err, ERR_CELL_RANGE); key[i] = 0; rk[i] = 0; continue; }
     int t = tourpos[u];
@@ -1388,20 +1399,28 @@ __global__ __launch_bounds__(256) void sort_bin(CoP P, CoD D, const int* tourpos
     rk[i] = (unsigned short)min(r, 65535);
   }
 }
+__global__ __launch_bounds__(256) void sort_bin(CoP P, CoD D, const int* tourpos, int* scnt, int* key,
+                                                unsigned short* rk) {
+  sort_bin_body(P, D, tourpos, scnt, key, rk, (long long)blockIdx.x * blockDim.x, P.NT,
+                (long long)gridDim.x * blockDim.x);
+}
+__device__ __forceinline__ void sort_scatter_body(const int* key, const unsigned short* rk, const int* soff, int* sb,
+                                                  long long first, long long end, long long step) {
+  for (long long i = first + threadIdx.x; i < end; i += step)
+    if (rk[i] < BMAX) sb[soff[key[i]] + rk[i]] = (int)i;
+}
 __global__ __launch_bounds__(256) void sort_scatter(int NT, const int* key, const unsigned short* rk, const int* soff,
                                                     int* sb) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < NT; i += gridDim.x * blockDim.x)
-    if (rk[i] < BMAX) sb[soff[key[i]] + rk[i]] = i;
+  sort_scatter_body(key, rk, soff, sb, (long long)blockIdx.x * blockDim.x, NT, (long long)gridDim.x * blockDim.x);
 }
-__global__ __launch_bounds__(256) void sort_gather(CoP P, CoD D, const int* sb, double* nx, double* ny, double* nz,
-                                                   double* nvx, double* nvy, double* nvz, int* nf,
-                                                   unsigned char* nst, unsigned* nbits) {
+__device__ __forceinline__ void sort_gather_body(const CoD& D, const int* sb, double* nx, double* ny, double* nz,
+                                                 double* nvx, double* nvy, double* nvz, int* nf, unsigned char* nst,
+                                                 unsigned* nbits, long long first, long long end, long long step) {
   const int lane = threadIdx.x & 63;
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long base = (long long)blockIdx.x * blockDim.x; base < P.NT; base += stride) {
+  for (long long base = first; base < end; base += step) {
     const long long j = base + threadIdx.x;
     bool bit = false;
-    if (j < P.NT) {
+    if (j < end) {
       int i = sb[j];
       nx[j] = D.x[i]; ny[j] = D.y[i]; nz[j] = D.z[i];
       nvx[j] = D.vx[i]; nvy[j] = D.vy[i]; nvz[j] = D.vz[i];
@@ -1414,6 +1433,13 @@ __global__ __launch_bounds__(256) void sort_gather(CoP P, CoD D, const int* sb, 
     if (lane == 0) nbits[w0] = (unsigned)m;
     if (lane == 32) nbits[w0 + 1] = (unsigned)(m >> 32);
   }
+}
+__global__ __launch_bounds__(256) void sort_gather(CoP P, CoD D, const int* sb, double* nx, double* ny, double* nz,
+                                                   double* nvx, double* nvy, double* nvz, int* nf,
+                                                   unsigned char* nst, unsigned* nbits) {
+  long long first, end;
+  block_range(P.NT, gridDim.x, xcd_block(blockIdx.x, gridDim.x), first, end);
+  sort_gather_body(D, sb, nx, ny, nz, nvx, nvy, nvz, nf, nst, nbits, first, end, 256);
 }
 
 }  // namespace mpcx
@@ -2086,6 +2112,89 @@ __global__ __launch_bounds__(256) void fills_rotate_b(const CoP* Ps, const CoD* 
   if (blockIdx.x == 0 && threadIdx.x == 0) D.sc->far_maxv2 = 0ull;
 }
 
+// sort buffers of one replica (the engine's own)
+struct CoSortD {
+  const int* tourpos;
+  int *scnt, *soff, *sb, *key;
+  unsigned short* rk;
+  double* alt[6];
+  int* aflag;
+  unsigned char* astat;
+  unsigned* abits;
+};
+
+__global__ __launch_bounds__(256) void sort_fills_b(const CoP* Ps, const CoSortD* Ss) {
+  const CoSortD S = Ss[blockIdx.y];
+  const CoP& P = Ps[blockIdx.y];
+  const long long nw = (P.NT + 31) / 32 + 1;
+  const long long i0 = blockIdx.x * 256ll + threadIdx.x, st = (long long)gridDim.x * 256;
+  for (long long i = i0; i < P.vol; i += st) S.scnt[i] = 0;
+  for (long long i = i0; i < P.NT; i += st) S.sb[i] = 0;
+  for (long long i = i0; i < nw; i += st) S.abits[i] = 0u;
+}
+__global__ __launch_bounds__(256) void sort_bin_b(const CoP* Ps, const CoD* Ds, const CoSortD* Ss, const Tile* tiles) {
+  const Tile t = tiles[blockIdx.x];
+  const CoP P = Ps[t.r];
+  const CoD D = Ds[t.r];
+  const CoSortD S = Ss[t.r];
+  sort_bin_body(P, D, S.tourpos, S.scnt, S.key, S.rk, t.begin, t.end, 256);
+}
+// per replica: exclusive scan of the sort counts over the vol cells
+__global__ __launch_bounds__(1024) void sort_scan_b(const CoP* Ps, const CoSortD* Ss) {
+  const CoSortD S = Ss[blockIdx.x];
+  const long long n = Ps[blockIdx.x].vol;
+  __shared__ unsigned long long sh[17];
+  __shared__ long long run;
+  if (threadIdx.x == 0) run = 0;
+  __syncthreads();
+  for (long long b0 = 0; b0 < n; b0 += 4096) {
+    int v[4];
+    unsigned long long sum = 0;
+    for (int j = 0; j < 4; j++) {
+      const long long i = b0 + threadIdx.x * 4 + j;
+      v[j] = i < n ? S.scnt[i] : 0;
+      sum += v[j];
+    }
+    unsigned long long tot;
+    const unsigned long long e = block_excl_scan64(sum, sh, tot);
+    long long r = run + (long long)e;
+    for (int j = 0; j < 4; j++) {
+      const long long i = b0 + threadIdx.x * 4 + j;
+      if (i < n) S.soff[i] = (int)r;
+      r += v[j];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) run += (long long)tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) S.soff[n] = (int)run;
+}
+__global__ __launch_bounds__(256) void sort_scatter_b(const CoSortD* Ss, const Tile* tiles) {
+  const Tile t = tiles[blockIdx.x];
+  const CoSortD S = Ss[t.r];
+  sort_scatter_body(S.key, S.rk, S.soff, S.sb, t.begin, t.end, 256);
+}
+// per tour cell of a tile: its members in ascending original index
+__global__ __launch_bounds__(256) void sort_order_b(const CoSortD* Ss, const Tile* tiles) {
+  const Tile t = tiles[blockIdx.x];
+  const CoSortD S = Ss[t.r];
+  const long long c = t.begin + threadIdx.x;
+  if (c >= t.end) return;
+  const int n = min(S.scnt[c], BMAX), o = S.soff[c];
+  for (int j = 1; j < n; j++) {
+    int v = S.sb[o + j], k = j;
+    while (k > 0 && S.sb[o + k - 1] > v) { S.sb[o + k] = S.sb[o + k - 1]; k--; }
+    S.sb[o + k] = v;
+  }
+}
+__global__ __launch_bounds__(256) void sort_gather_b(const CoD* Ds, const CoSortD* Ss, const Tile* tiles) {
+  const Tile t = tiles[blockIdx.x];
+  const CoD D = Ds[t.r];
+  const CoSortD S = Ss[t.r];
+  sort_gather_body(D, S.sb, S.alt[0], S.alt[1], S.alt[2], S.alt[3], S.alt[4], S.alt[5], S.aflag, S.astat, S.abits,
+                   t.begin, t.end, 256);
+}
+
 }  // namespace mpcx
 
 struct mpcx_cob {
@@ -2094,8 +2203,10 @@ struct mpcx_cob {
   CoP* d_P = nullptr;
   CoD* d_D = nullptr;
   std::vector<CoD> h_D;
-  Tile *d_pt = nullptr, *d_ct = nullptr;
-  int n_pt = 0, n_ct = 0, tile = 2048;
+  Tile *d_pt = nullptr, *d_ct = nullptr, *d_vt = nullptr, *d_gt = nullptr;
+  int n_pt = 0, n_ct = 0, n_vt = 0, tile = 2048;
+  CoSortD* d_S = nullptr;
+  std::vector<CoSortD> h_S;
   hipStream_t st = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -2106,30 +2217,80 @@ struct mpcx_cob {
 };
 
 static int cob_tiles(mpcx_cob* b) {
-  std::vector<Tile> pt, ct;
+  std::vector<Tile> pt, ct, vt;
   for (int r = 0; r < b->R; r++) {
     const CoP& P = b->eng[r]->P;
     for (long long i = 0; i < P.NT; i += b->tile) pt.push_back(Tile{r, 0, i, std::min<long long>(i + b->tile, P.NT)});
     for (long long t = 0; t < P.mpcnumcell; t += CT)
       ct.push_back(Tile{r, 0, t, std::min<long long>(t + CT, P.mpcnumcell)});
+    for (long long t = 0; t < P.vol; t += 256) vt.push_back(Tile{r, 0, t, std::min<long long>(t + 256, P.vol)});
   }
-  if (b->d_pt) (void)hipFree(b->d_pt);
-  if (b->d_ct) (void)hipFree(b->d_ct);
-  b->d_pt = b->d_ct = nullptr;
+  // workgroup b runs on XCD b % 8: lay the tiles out so that each XCD's
+  // workgroups take one contiguous run of tiles
+  auto xcd_order = [](std::vector<Tile>& v) {
+    const size_t n = v.size(), q = n / 8, r = n % 8;
+    std::vector<Tile> o(n);
+    for (size_t b = 0; b < n; b++) {
+      const size_t x = b % 8, k = b / 8;
+      o[b] = v[x * q + std::min(x, r) + k];
+    }
+    v.swap(o);
+  };
+  std::vector<Tile> gt = pt;   // gather kernels (sort gather) take the XCD-contiguous order
+  xcd_order(gt);
+  xcd_order(ct);
+  for (Tile** d : {&b->d_pt, &b->d_ct, &b->d_vt, &b->d_gt})
+    if (*d) { (void)hipFree(*d); *d = nullptr; }
   HIPCHK(hipMalloc(&b->d_pt, pt.size() * sizeof(Tile)));
+  HIPCHK(hipMalloc(&b->d_gt, gt.size() * sizeof(Tile)));
   HIPCHK(hipMalloc(&b->d_ct, ct.size() * sizeof(Tile)));
+  HIPCHK(hipMalloc(&b->d_vt, vt.size() * sizeof(Tile)));
+  HIPCHK(hipMemcpy(b->d_gt, gt.data(), gt.size() * sizeof(Tile), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(b->d_pt, pt.data(), pt.size() * sizeof(Tile), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(b->d_ct, ct.data(), ct.size() * sizeof(Tile), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b->d_vt, vt.data(), vt.size() * sizeof(Tile), hipMemcpyHostToDevice));
   b->n_pt = (int)pt.size();
   b->n_ct = (int)ct.size();
+  b->n_vt = (int)vt.size();
   return 0;
 }
 
 static int cob_upload_D(mpcx_cob* b) {
-  for (int r = 0; r < b->R; r++) b->h_D[r] = b->eng[r]->D;
+  for (int r = 0; r < b->R; r++) {
+    mpcx_co* e = b->eng[r];
+    b->h_D[r] = e->D;
+    CoSortD& S = b->h_S[r];
+    S.tourpos = e->d_tourpos; S.scnt = e->d_scnt; S.soff = e->d_soff; S.sb = e->d_sb; S.key = e->d_skey;
+    S.rk = e->d_srk;
+    for (int a = 0; a < 6; a++) S.alt[a] = e->alt[a];
+    S.aflag = e->alt_flag; S.astat = e->alt_status; S.abits = e->alt_bits;
+  }
   HIPCHK(hipMemcpyAsync(b->d_D, b->h_D.data(), b->R * sizeof(CoD), hipMemcpyHostToDevice, b->st));
+  HIPCHK(hipMemcpyAsync(b->d_S, b->h_S.data(), b->R * sizeof(CoSortD), hipMemcpyHostToDevice, b->st));
   HIPCHK(hipStreamSynchronize(b->st));
   return 0;
+}
+
+// the Hilbert re-sort of every replica (sort_hilbert_mod.f95:50-92), then the
+// double buffers of each engine swap roles
+static int cob_sort_all(mpcx_cob* b) {
+  for (int r = 0; r < b->R; r++)
+    if (!b->eng[r]->has_tour) return set_error(MPCX_E_ARG, "a sort step needs the cell tour of every replica");
+  const dim3 fg(b->fill_blocks, b->R);
+  hipLaunchKernelGGL(sort_fills_b, fg, dim3(256), 0, b->st, b->d_P, b->d_S);
+  hipLaunchKernelGGL(sort_bin_b, dim3(b->n_pt), dim3(256), 0, b->st, b->d_P, b->d_D, b->d_S, b->d_pt);
+  hipLaunchKernelGGL(sort_scan_b, dim3(b->R), dim3(1024), 0, b->st, b->d_P, b->d_S);
+  hipLaunchKernelGGL(sort_scatter_b, dim3(b->n_pt), dim3(256), 0, b->st, b->d_S, b->d_pt);
+  hipLaunchKernelGGL(sort_order_b, dim3(b->n_vt), dim3(256), 0, b->st, b->d_S, b->d_vt);
+  hipLaunchKernelGGL(sort_gather_b, dim3(b->n_pt), dim3(256), 0, b->st, b->d_D, b->d_S, b->d_gt);
+  for (int r = 0; r < b->R; r++) {
+    mpcx_co* e = b->eng[r];
+    CoD& D = e->D;
+    std::swap(D.x, e->alt[0]); std::swap(D.y, e->alt[1]); std::swap(D.z, e->alt[2]);
+    std::swap(D.vx, e->alt[3]); std::swap(D.vy, e->alt[4]); std::swap(D.vz, e->alt[5]);
+    std::swap(D.flag, e->alt_flag); std::swap(D.status, e->alt_status); std::swap(D.near_bits, e->alt_bits);
+  }
+  return cob_upload_D(b);
 }
 
 static void cob_launch_near(mpcx_cob* b, int nsteps, int mode) {
@@ -2160,11 +2321,7 @@ static int cob_sort_interval(mpcx_cob* b) {
   cob_launch_near(b, b->S, M_STOP);
   hipLaunchKernelGGL(fills_begin_b, fg, dim3(256), 0, b->st, b->d_P, b->d_D);
   hipLaunchKernelGGL(co_bulk_b, dim3(b->n_pt), dim3(256), 0, b->st, b->d_P, b->d_D, b->d_pt, 1, 0);
-  for (int r = 0; r < b->R; r++) {
-    int rc = co_sort(b->eng[r], b->st);
-    if (rc) return rc;
-  }
-  int rc = cob_upload_D(b);
+  int rc = cob_sort_all(b);
   if (rc) return rc;
   hipLaunchKernelGGL(near_list_b, dim3(b->R), dim3(1024), 0, b->st, b->d_P, b->d_D);
   cob_launch_near(b, 1, M_RESUME);
@@ -2187,6 +2344,9 @@ extern "C" int mpcx_cob_destroy(mpcx_cob* b) {
   if (b->d_D) (void)hipFree(b->d_D);
   if (b->d_pt) (void)hipFree(b->d_pt);
   if (b->d_ct) (void)hipFree(b->d_ct);
+  if (b->d_vt) (void)hipFree(b->d_vt);
+  if (b->d_gt) (void)hipFree(b->d_gt);
+  if (b->d_S) (void)hipFree(b->d_S);
   if (b->e0) (void)hipEventDestroy(b->e0);
   if (b->e1) (void)hipEventDestroy(b->e1);
   if (b->st) (void)hipStreamDestroy(b->st);
@@ -2214,6 +2374,7 @@ extern "C" int mpcx_cob_create(mpcx_cob** out, int32_t R, mpcx_co* const* engine
   b->tile = tile;
   b->eng.assign(engines, engines + R);
   b->h_D.resize(R);
+  b->h_S.resize(R);
   std::vector<CoP> hp(R);
   for (int r = 0; r < R; r++) {
     hp[r] = engines[r]->P;
@@ -2222,7 +2383,8 @@ extern "C" int mpcx_cob_create(mpcx_cob** out, int32_t R, mpcx_co* const* engine
   }
   int rc = 0;
   if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess || hipMalloc(&b->d_P, R * sizeof(CoP)) != hipSuccess ||
-      hipMalloc(&b->d_D, R * sizeof(CoD)) != hipSuccess || hipEventCreate(&b->e0) != hipSuccess ||
+      hipMalloc(&b->d_D, R * sizeof(CoD)) != hipSuccess || hipMalloc(&b->d_S, R * sizeof(CoSortD)) != hipSuccess ||
+      hipEventCreate(&b->e0) != hipSuccess ||
       hipEventCreate(&b->e1) != hipSuccess)
     rc = set_error(MPCX_E_HIP, "batch allocation failed");
   if (!rc && hipMemcpy(b->d_P, hp.data(), R * sizeof(CoP), hipMemcpyHostToDevice) != hipSuccess)

@@ -63,7 +63,8 @@ def place_dimers(p, rng):
     yd = np.zeros((nd, 2), order="F")
     zd = np.zeros((nd, 2), order="F")
     for i in range(nd):
-        for _ in range(100000):
+        placed = False
+        for _ in range(20000 if nd > 1 else 2000):
             x, y, z, ph, th = rng.random(5)
             ph = 2.0 * PI * ph
             th = 2.0 * th - 1.0
@@ -83,8 +84,21 @@ def place_dimers(p, rng):
                     if d @ d < ((ra + rb) * cut + p.get("dimer_gap", 0.0)) ** 2:
                         ok = False
             if ok:
+                placed = True
                 break
-        else:
+        if not placed and nd == 1:
+            # boxes too low for the reference's wall rejection (it needs
+            # (Rw+Rn)*3^(1/6) + Rn*2^(1/6) of clearance on both sides): the
+            # dimer at mid-height with its bond in the x-y plane, beyond the
+            # range of the wall potential when Lz/2 > (Rw+Rn)*3^(1/6)
+            x, y, ph = rng.random(3)
+            c = np.array([box[0] * x, box[1] * y, 0.5 * box[2]])
+            u = np.array([math.cos(2.0 * PI * ph), math.sin(2.0 * PI * ph), 0.0])
+            pc, pn = c + 0.5 * Dcn * u, c - 0.5 * Dcn * u
+            if 0.5 * box[2] <= max(wc, wn):
+                raise RuntimeError("box too low for a dimer")
+            placed = True
+        if not placed:
             raise RuntimeError("could not place dimer %d" % i)
         pc[:2] -= box[:2] * np.floor(pc[:2] / box[:2])
         pn[:2] -= box[:2] * np.floor(pn[:2] / box[:2])
