@@ -193,6 +193,12 @@ int mpcx_cob_destroy(mpcx_cob* b);
 int mpcx_cob_run(mpcx_cob* b, int64_t ncoll, int32_t use_graph);
 int mpcx_cob_set_tile(mpcx_cob* b, int32_t tile);
 int mpcx_cob_info(mpcx_cob* b, int64_t* particles, int64_t* cells, double* run_ms);
+/* Advance n intervals and, after each, add every replica's body-frame B
+ * counts (R, nz, nr) (frame of its dimer 0, as mpcx_co_body_density) to
+ * `counts` and its propulsion velocity to vsum[r]: the time averages that
+ * train the surrogate. */
+int mpcx_cob_body_average(mpcx_cob* b, int64_t n, int32_t nr, int32_t nz, double rmax, double zmin, double zmax,
+                          uint32_t* counts, double* vsum);
 
 /* ------------------------------------------------------------------------
  * Device-resident Selkov engine (selkov_code)

@@ -44,6 +44,19 @@ class CollectiveBatch:
         check(lib().mpcx_cob_info(self.h, C.byref(p), C.byref(c), C.byref(ms)), "mpcx_cob_info")
         return dict(particles=p.value, cells=c.value, run_ms=ms.value)
 
+    def body_average(self, n: int, nr=32, nz=64, rmax=12.0, zmin=-20.0, zmax=12.0):
+        """Run n intervals; per replica the mean body-frame B counts (nz, nr)
+        and the mean propulsion velocity over the n samples."""
+        import numpy as np
+        R = len(self.engines)
+        counts = np.zeros((R, nz, nr), np.uint32)
+        vsum = np.zeros(R)
+        from ._lib import ptr
+        check(lib().mpcx_cob_body_average(self.h, C.c_int64(n), C.c_int32(nr), C.c_int32(nz), C.c_double(rmax),
+                                          C.c_double(zmin), C.c_double(zmax), ptr(counts), ptr(vsum)),
+              "mpcx_cob_body_average")
+        return counts.astype(np.float64) / max(n, 1), vsum / max(n, 1)
+
     def autotune(self, tiles=(512, 1024, 2048, 4096, 8192), ncoll: int = 4, use_graph: bool = True) -> dict:
         """Time `ncoll` intervals at each tile size (advancing the systems) and
         keep the fastest; returns {tile: ms per interval}."""

@@ -2398,9 +2398,9 @@ extern "C" int mpcx_cob_create(mpcx_cob** out, int32_t R, mpcx_co* const* engine
   return 0;
 }
 
-extern "C" int mpcx_cob_run(mpcx_cob* b, int64_t ncoll, int32_t use_graph) {
-  if (!b || ncoll < 0) return set_error(MPCX_E_ARG, "bad arguments");
-  // every replica at the same collision step; engines idle on their own streams
+// checks that every replica sits at the same collision step, sets the
+// collision counters and uploads the replica descriptors
+static int cob_begin(mpcx_cob* b, int* cstep_out) {
   int cstep = -1;
   for (int r = 0; r < b->R; r++) {
     mpcx_co* e = b->eng[r];
@@ -2413,34 +2413,54 @@ extern "C" int mpcx_cob_run(mpcx_cob* b, int64_t ncoll, int32_t use_graph) {
     const unsigned long long coll = (unsigned long long)(cs / b->S);
     HIPCHK(hipMemcpy(&e->D.sc->coll, &coll, sizeof coll, hipMemcpyHostToDevice));
   }
-  int rc = cob_upload_D(b);
+  *cstep_out = cstep;
+  return cob_upload_D(b);
+}
+
+// launches one interval (graph replay when possible)
+static int cob_step(mpcx_cob* b, int* cstep, int use_graph) {
+  const int cnext = *cstep + b->S;
+  int rc = 0;
+  if (cnext % b->sort_step == 0) {
+    rc = cob_sort_interval(b);
+  } else if (use_graph) {
+    if (!b->gexec) {
+      HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+      cob_interval(b);
+      HIPCHK(hipStreamEndCapture(b->st, &b->graph));
+      HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+    }
+    HIPCHK(hipGraphLaunch(b->gexec, b->st));
+  } else {
+    cob_interval(b);
+  }
+  *cstep = cnext;
+  return rc;
+}
+
+static int cob_end(mpcx_cob* b) {
+  HIPCHK(hipStreamSynchronize(b->st));
+  for (int r = 0; r < b->R; r++) {
+    int rc = check_device_error(b->eng[r]);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+extern "C" int mpcx_cob_run(mpcx_cob* b, int64_t ncoll, int32_t use_graph) {
+  if (!b || ncoll < 0) return set_error(MPCX_E_ARG, "bad arguments");
+  int cstep = 0;
+  int rc = cob_begin(b, &cstep);
   if (rc) return rc;
   HIPCHK(hipEventRecord(b->e0, b->st));
-  for (int64_t c = 0; c < ncoll; c++) {
-    const int cnext = cstep + b->S;
-    if (cnext % b->sort_step == 0) {
-      if ((rc = cob_sort_interval(b))) return rc;
-    } else if (use_graph) {
-      if (!b->gexec) {
-        HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-        cob_interval(b);
-        HIPCHK(hipStreamEndCapture(b->st, &b->graph));
-        HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
-      }
-      HIPCHK(hipGraphLaunch(b->gexec, b->st));
-    } else {
-      cob_interval(b);
-    }
-    cstep = cnext;
-  }
+  for (int64_t c = 0; c < ncoll && !rc; c++) rc = cob_step(b, &cstep, use_graph);
+  if (rc) return rc;
   HIPCHK(hipEventRecord(b->e1, b->st));
   HIPCHK(hipEventSynchronize(b->e1));
   float ms = 0;
   (void)hipEventElapsedTime(&ms, b->e0, b->e1);
   b->run_ms = ms;
-  for (int r = 0; r < b->R; r++)
-    if ((rc = check_device_error(b->eng[r]))) return rc;
-  return 0;
+  return cob_end(b);
 }
 
 // particles and collision cells of the batch, and the wall time of the last run (ms)
@@ -2460,4 +2480,81 @@ extern "C" int mpcx_cob_set_tile(mpcx_cob* b, int32_t tile) {
   if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
   b->tile = tile;
   return cob_tiles(b);
+}
+
+// ---------------------------------------------------------------- batch diagnostics
+// Per replica (blockIdx.y): counts of B (flag 1) particles per (z, r) bin in
+// the body frame of its dimer 0 (origin at the C sphere, z along the N->C unit
+// vector), added to acc[r][nz][nr]; and its dimer's centre-of-mass velocity
+// along that axis, added to vacc[r] (one thread).
+namespace mpcx {
+struct BodyGrid {
+  int nr, nz;
+  double rmax, zmin, zmax;
+};
+__global__ __launch_bounds__(256) void body_acc_b(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds,
+                                                  BodyGrid G, unsigned* acc, double* vacc) {
+  const int r = blockIdx.y;
+  const CoP P = Ps[r];
+  const CoD D = Ds[r];
+  const int nd = P.nd;
+  const double cx = D.dim[DX * MAXS + 0], cy = D.dim[DY * MAXS + 0], cz = D.dim[DZ * MAXS + 0];
+  double ux = cx - D.dim[DX * MAXS + nd], uy = cy - D.dim[DY * MAXS + nd], uz = cz - D.dim[DZ * MAXS + nd];
+  ux = pbc_half(ux, P.box[0]);
+  uy = pbc_half(uy, P.box[1]);
+  const double len = sqrt(ux * ux + uy * uy + uz * uz);
+  ux = ux / len; uy = uy / len; uz = uz / len;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const double M = P.Mc + P.Mn;
+    const double vx = (P.Mc * D.dim[DVX * MAXS + 0] + P.Mn * D.dim[DVX * MAXS + nd]) / M;
+    const double vy = (P.Mc * D.dim[DVY * MAXS + 0] + P.Mn * D.dim[DVY * MAXS + nd]) / M;
+    const double vz = (P.Mc * D.dim[DVZ * MAXS + 0] + P.Mn * D.dim[DVZ * MAXS + nd]) / M;
+    vacc[r] += (vx * ux + vy * uy) + vz * uz;
+  }
+  const double dr = G.rmax / G.nr, dz = (G.zmax - G.zmin) / G.nz;
+  unsigned* a = acc + (size_t)r * G.nz * G.nr;
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < P.NT; i += (long long)gridDim.x * 256) {
+    if (D.flag[i] != 1) continue;
+    const double ax = pbc_half(D.x[i] - cx, P.box[0]), ay = pbc_half(D.y[i] - cy, P.box[1]), az = D.z[i] - cz;
+    const double zr = (ax * ux + ay * uy) + az * uz;
+    const double px = ax - zr * ux, py = ay - zr * uy, pz = az - zr * uz;
+    const double rr = sqrt((px * px + py * py) + pz * pz);
+    if (!(zr >= G.zmin && zr < G.zmax && rr < G.rmax)) continue;
+    const int iz = min((int)floor((zr - G.zmin) / dz), G.nz - 1), ir = min((int)floor(rr / dr), G.nr - 1);
+    atomicAdd(&a[iz * G.nr + ir], 1u);
+  }
+}
+}  // namespace mpcx
+
+// Accumulate the body-frame B density and propulsion velocity of every
+// replica over n intervals (one sample after each): counts (R, nz, nr) and
+// velocity sums (R) are added to the host arrays.
+extern "C" int mpcx_cob_body_average(mpcx_cob* b, int64_t n, int32_t nr, int32_t nz, double rmax, double zmin,
+                                     double zmax, uint32_t* counts, double* vsum) {
+  if (!b || n < 0 || nr < 1 || nz < 1 || !(rmax > 0) || !(zmax > zmin) || !counts || !vsum)
+    return set_error(MPCX_E_ARG, "bad arguments");
+  const size_t bins = (size_t)b->R * nz * nr;
+  unsigned* acc = nullptr;
+  double* vacc = nullptr;
+  int rc = 0;
+  if (hipMalloc(&acc, bins * 4) != hipSuccess || hipMalloc(&vacc, b->R * 8) != hipSuccess ||
+      hipMemcpy(acc, counts, bins * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(vacc, vsum, b->R * 8, hipMemcpyHostToDevice) != hipSuccess)
+    rc = set_error(MPCX_E_HIP, "body accumulator allocation failed");
+  BodyGrid G{nr, nz, rmax, zmin, zmax};
+  int cstep = 0;
+  if (!rc) rc = cob_begin(b, &cstep);
+  for (int64_t i = 0; i < n && !rc; i++) {
+    rc = cob_step(b, &cstep, 1);
+    if (!rc) hipLaunchKernelGGL(body_acc_b, dim3(64, b->R), dim3(256), 0, b->st, b->d_P, b->d_D, G, acc, vacc);
+  }
+  if (!rc) rc = cob_end(b);
+  if (!rc) {
+    if (hipMemcpy(counts, acc, bins * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(vsum, vacc, b->R * 8, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = set_error(MPCX_E_HIP, "hipMemcpy failed");
+  }
+  (void)hipFree(acc);
+  (void)hipFree(vacc);
+  return rc;
 }

@@ -42,6 +42,7 @@ class Surrogate:
         self.nparam = int(lib().mpcx_mlp_num_params(self.h))
         self.x_mean = np.zeros(n_in)
         self.x_std = np.ones(n_in)
+        self.y_mean = np.zeros(n_out)
         self.y_scale = np.ones(n_out)
 
     def close(self):
@@ -100,15 +101,23 @@ class Surrogate:
         return ms.value, fl.value
 
     # physical-units interface
-    def fit(self, params: np.ndarray, targets: np.ndarray, out_weight: np.ndarray | None = None, **kw) -> np.ndarray:
+    def fit(self, params: np.ndarray, targets: np.ndarray, out_weight: np.ndarray | None = None,
+            scale_floor: float = 1e-2, **kw) -> np.ndarray:
+        """Train on (params, targets) in physical units.  Targets are centred
+        on the training mean and divided by their spread, floored at
+        `scale_floor` times the median spread so that outputs which hardly vary
+        (bins inside a sphere) do not turn sampling noise into large values."""
         params = np.asarray(params, np.float64)
         targets = np.asarray(targets, np.float64)
         self.x_mean = params.mean(axis=0)
         self.x_std = params.std(axis=0) + 1e-12
-        self.y_scale = targets.std(axis=0) + 1e-6
-        w = np.ones(self.n_out) if out_weight is None else np.asarray(out_weight)
-        return self.train_raw((params - self.x_mean) / self.x_std, targets / self.y_scale, w / self.n_out, **kw)
+        self.y_mean = targets.mean(axis=0)
+        sd = targets.std(axis=0)
+        self.y_scale = np.maximum(sd, scale_floor * np.median(sd) + 1e-12)
+        w = np.ones(self.n_out) if out_weight is None else np.asarray(out_weight, np.float64)
+        return self.train_raw((params - self.x_mean) / self.x_std, (targets - self.y_mean) / self.y_scale,
+                              w / w.sum(), **kw)
 
     def predict(self, params: np.ndarray) -> np.ndarray:
         x = (np.asarray(params, np.float64) - self.x_mean) / self.x_std
-        return self.forward_raw(x).astype(np.float64) * self.y_scale
+        return self.forward_raw(x).astype(np.float64) * self.y_scale + self.y_mean
