@@ -153,10 +153,28 @@ int mpcx_co_get_state(mpcx_co* e, double* x, double* y, double* z, double* vx, d
                       int32_t* scalars /* cstep, time_count, time_penetrate, inside_num */);
 /* Inside list (ascending 0-based particle indices), the neighbour-list
  * membership bit mask of each entry (bit 2k: C sphere of dimer k, 2k+1: N),
- * and the force on each entry from each sphere it belongs to (up to 4,
- * in sphere order: fp[m*3*cap + c*cap + q]).  Returns the count in *n. */
+ * and the force on each entry from each sphere it belongs to (up to 8,
+ * by bit: fp[b*3*cap + c*cap + q]).  Returns the count in *n.  One-workgroup
+ * mode only (up to four dimers); mpcx_co_get_lists works in both modes. */
 int mpcx_co_get_inside(mpcx_co* e, int32_t* list, uint32_t* memb, double* fp, double* pot, int64_t cap,
                        int64_t* n);
+/* Neighbour lists as neighbour_list_mod.f95:38-57 leaves them (neighbour,
+ * neighbour_num) with the forces_mod.f95:55-136 pair forces and potentials
+ * of the last MD step: nb/fpx/fpy/fpz/pot are (numdims, 2, max_nb)
+ * column-major with 1-based particle indices, ascending; nbnum (numdims, 2);
+ * in_list the inside list (0-based, ascending, inout_particle_list), *in_num
+ * its length.  Any pointer but nbnum and in_num may be NULL. */
+int mpcx_co_get_lists(mpcx_co* e, int64_t max_nb, int64_t* nb, int64_t* nbnum, double* fpx, double* fpy,
+                      double* fpz, double* pot, int32_t* in_list, int64_t in_cap, int64_t* in_num);
+/* Near-engine layout for two or more dimers.  on = 1 (the default above two
+ * dimers): dimers whose near regions can meet within an interval are grouped
+ * into clusters of at most four, each advanced by its own workgroup, the
+ * workgroups meeting at a grid barrier every MD step to take the
+ * system-wide list-rebuild decision of evolve_mod.f95:457-465.  on = 0: one
+ * workgroup for the whole system (up to four dimers).  Call before
+ * mpcx_co_setup.  No counterpart in the reference (the OpenMP loops of
+ * evolve_mod.f95 run over all dimers in one address space). */
+int mpcx_co_set_clusters(mpcx_co* e, int32_t on);
 /* Collision-cell data of the last collision: (mpcnumcell) arrays, 0-based
  * cell of each particle, and unshifted census counts (3, vol). */
 int mpcx_co_get_cells(mpcx_co* e, double* vcm_uv /* (mpcnumcell, 8) */, int32_t* cell_of,

@@ -16,7 +16,7 @@ import numpy as np
 
 from ._lib import MtmodStream, check, lib, ptr, require_gpu
 
-MAXM = 4            # force slots stored per inside particle by the engine
+MAXM = 8            # force slots stored per inside entry by the engine (one per list of up to four dimers)
 MAX_INSIDE = 16384  # inside-list capacity of the near engine
 
 
@@ -41,7 +41,7 @@ def reaction_table(k1: float, MPCt: float) -> np.ndarray:
     return t
 
 
-def make_config(p: dict, rng_mode: int = 0, seed: int = 4357787, sphere_guard: float = 1.0) -> CoConfig:
+def make_config(p: dict, rng_mode: int = 0, seed: int = 4357787, sphere_guard: float = 0.3) -> CoConfig:
     c = CoConfig()
     for k in ("NT", "numdims", "vol", "mpcnumcell", "rxnnumcell"):
         setattr(c, k, int(p[k]))
@@ -72,13 +72,19 @@ def draws_per_collision(p: dict) -> int:
 class CollectiveEngine:
     """Device-resident collective_code system (one GPU)."""
 
-    def __init__(self, p: dict, rng_mode: int = 0, seed: int = 4357787, sphere_guard: float = 1.0):
+    def __init__(self, p: dict, rng_mode: int = 0, seed: int = 4357787, sphere_guard: float = 0.3,
+                 clusters: bool | None = None):
+        """clusters: one near-engine workgroup per cluster of dimers (the
+        default above two dimers) or one for the whole system (False, up to
+        four dimers); see mpcx_co_set_clusters."""
         require_gpu()
         self.p = p
         self.cfg = make_config(p, rng_mode, seed, sphere_guard)
         self.h = C.c_void_p()
         check(lib().mpcx_co_create(C.byref(self.h), C.byref(self.cfg)), "mpcx_co_create")
         self.rng_mode = rng_mode
+        if clusters is not None:
+            check(lib().mpcx_co_set_clusters(self.h, C.c_int32(1 if clusters else 0)), "mpcx_co_set_clusters")
 
     def close(self):
         if self.h:
@@ -132,33 +138,16 @@ class CollectiveEngine:
     def get_lists(self) -> dict:
         """Inside list, neighbour lists (reference layout, 1-based) and Fpair."""
         nd, NT, mnb = self.p["numdims"], self.p["NT"], int(self.p.get("max_nb", 30000))
-        cap = MAX_INSIDE
+        cap = MAX_INSIDE * max(1, nd)
         lst = np.zeros(cap, np.int32)
-        memb = np.zeros(cap, np.uint32)
-        fp = np.zeros(MAXM * 3 * cap)
-        pot = np.zeros(MAXM * cap)
-        n = C.c_int64(0)
-        check(lib().mpcx_co_get_inside(self.h, ptr(lst), ptr(memb), ptr(fp), ptr(pot), C.c_int64(cap),
-                                       C.byref(n)), "mpcx_co_get_inside")
-        n = n.value
-        lst, memb = lst[:n], memb[:n]
-        fp = fp.reshape(MAXM, 3, cap)[:, :, :n]
-        pot = pot.reshape(MAXM, cap)[:, :n]
         nb = np.zeros((nd, 2, mnb), np.int64, order="F")
         nbnum = np.zeros((nd, 2), np.int64, order="F")
         fpx = np.zeros((nd, 2, mnb), order="F")
         fpy, fpz, potn = fpx.copy(order="F"), fpx.copy(order="F"), fpx.copy(order="F")
-        for b in range(2 * nd):   # forces are stored by list b = 2k + i
-            k, i = b >> 1, b & 1
-            sel = np.nonzero((memb >> b) & 1)[0]
-            cnt = sel.size
-            nb[k, i, :cnt] = lst[sel] + 1
-            nbnum[k, i] = cnt
-            fpx[k, i, :cnt] = fp[b, 0, sel]
-            fpy[k, i, :cnt] = fp[b, 1, sel]
-            fpz[k, i, :cnt] = fp[b, 2, sel]
-            potn[k, i, :cnt] = pot[b, sel]
-        inside = lst.astype(np.int64) + 1
+        n = C.c_int64(0)
+        check(lib().mpcx_co_get_lists(self.h, C.c_int64(mnb), ptr(nb), ptr(nbnum), ptr(fpx), ptr(fpy), ptr(fpz),
+                                      ptr(potn), ptr(lst), C.c_int64(cap), C.byref(n)), "mpcx_co_get_lists")
+        inside = lst[:n.value].astype(np.int64) + 1
         outside = np.setdiff1d(np.arange(1, NT + 1, dtype=np.int64), inside, assume_unique=True)
         return dict(in_list=inside, out_list=outside, nb=nb, nbnum=nbnum, fpx=fpx, fpy=fpy, fpz=fpz, pot=potn)
 

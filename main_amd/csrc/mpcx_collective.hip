@@ -35,9 +35,11 @@ namespace mpcx {
 
 constexpr int NE_T = 512;    // near-engine threads (one workgroup, 256 VGPRs per lane)
 constexpr int NE_CAP = 16384;  // inside-list capacity of the near engine
-constexpr int MAXD = 2;      // dimers the near engine handles
+constexpr int MAXD = 4;      // dimers one near-engine workgroup handles (a cluster)
 constexpr int MAXS = 2 * MAXD;
-constexpr int MAXM = 4;      // neighbour lists one particle may belong to
+constexpr int MAXM = MAXS;   // force slots kept per inside entry (one per list of its cluster)
+constexpr int MAXDG = 120;   // dimers of one system
+constexpr int MAXSG = 2 * MAXDG;   // stride of the sphere fields in CoD::dim
 constexpr int MAXEV = 64;    // rebuilds recorded per interval
 constexpr int BMAX = 64;     // particles per collision cell in the ordered sum
 constexpr int CENSUS_MAX = 100;  // MaxA/MaxB/MaxS, global_mod.f95:114-116
@@ -47,7 +49,7 @@ constexpr int CT = 256;      // threads of the per-cell kernel
 constexpr long long RATTLE_CAP = 100000;
 
 enum : uint8_t { ST_INSIDE = 1, ST_EXCL = 2 };
-enum : int { M_SETUP = 1, M_STOP = 2, M_RESUME = 4 };
+enum : int { M_SETUP = 1, M_STOP = 2, M_RESUME = 4, M_CLUSTER = 8 };
 enum : int { DX = 0, DY, DZ, DVX, DVY, DVZ, DFX, DFY, DFZ, DQX, DQY, DQZ };
 
 struct CoScalars {
@@ -57,9 +59,24 @@ struct CoScalars {
   unsigned long long far_maxv2;    // bits of a non-negative double
   double near_vmax2;
   double tpotential;
-  double x0[MAXS], y0[MAXS], z0[MAXS];
+  double x0[MAXSG], y0[MAXSG], z0[MAXSG];   // spheres at the interval start
   unsigned long long prof[8];   // near engine: total, rebuild, particle phase, serial (100 MHz ticks), rebuilds
   unsigned long long coll;      // collision counter of the current interval (Philox counter)
+};
+
+// Dimers whose near regions can overlap within an interval form a cluster;
+// each cluster is advanced by its own near-engine workgroup, and the
+// workgroups of one system meet at a grid barrier once per MD step (and per
+// list rebuild) to take the system-wide rebuild decisions of the reference.
+struct CluInfo {
+  int nclu, abort;
+  int cnt[MAXDG];            // dimers per cluster
+  int dims[MAXDG][MAXD];     // their global indices, ascending
+  int clu_of[MAXDG];         // cluster of each dimer
+  int loc[MAXDG];            // position of each dimer within its cluster
+  unsigned count, gen;       // barrier arrivals and generation
+  unsigned long long pub[2][MAXDG];   // exchanged values (double bits), by generation parity
+  unsigned adj[MAXDG][(MAXDG + 31) / 32];   // dimers linked through a near particle
 };
 
 struct CoP {
@@ -79,7 +96,7 @@ struct CoP {
 struct CoD {
   double *x, *y, *z, *vx, *vy, *vz;
   int* flag;
-  double* dim;            // [12][MAXS], sphere s = k + i*nd
+  double* dim;            // [12][MAXSG], sphere s = k + i*nd
   CoScalars* sc;
   const int* react_box;
   int* near_list;         // near set of the interval, ascending particle index
@@ -111,6 +128,16 @@ struct CoD {
   unsigned* firstB;       // [vol]
   const double* U;        // host draws of this collision (rng_mode 0)
   const double* ptab;     // reaction probabilities, 101 entries
+  // clusters (many dimers): the system's cluster table, this workgroup's
+  // cluster, and per-particle records of list memberships carried between
+  // intervals (up to four global list ids 2k+i as bytes, their pair forces
+  // and potentials, max_solvent_dist)
+  CluInfo* clu;
+  int cid, pad_c;
+  unsigned* rec_ids;
+  double* rec_msd;
+  double* rec_fp;         // [p][4][3]
+  double* rec_pot;        // [p][4]
 };
 
 // ------------------------------------------------------------- streaming
@@ -394,36 +421,81 @@ struct NearShared {
   double fin[4 * MAXS + 1];
   int wsum[NE_W + 1];
   double vmax2;
-  int cstep, tc, tp, in_num, nev, near_num, rebuild;
+  int cstep, tc, tp, in_num, nev, near_num, rebuild, clustered;
   CoP P;   // launch parameters, read from LDS by the helpers below
   CoD D;
 };
 __shared__ NearShared g_ne;
 
-// 4-bit memberships of a lane's near entries (entry j in nibble j)
+// memberships of a lane's near entries, B bits each (B = 4 for up to two
+// dimers, 8 for up to four): entry j in field j
+template <int B>
 struct Memb {
-  unsigned long long w[NE_EMAX / 16];
+  static constexpr int PER = 64 / B;
+  static constexpr unsigned MASK = (1u << B) - 1u;
+  unsigned long long w[NE_EMAX / PER];
   __device__ __forceinline__ void clear() {
 #pragma unroll
-    for (int i = 0; i < NE_EMAX / 16; i++) w[i] = 0ull;
+    for (int i = 0; i < NE_EMAX / PER; i++) w[i] = 0ull;
   }
   __device__ __forceinline__ void set(int j, unsigned m) {
 #pragma unroll
-    for (int i = 0; i < NE_EMAX / 16; i++)
-      if ((j >> 4) == i) w[i] |= (unsigned long long)(m & 15u) << (4 * (j & 15));
+    for (int i = 0; i < NE_EMAX / PER; i++)
+      if (j / PER == i) w[i] |= (unsigned long long)(m & MASK) << (B * (j % PER));
   }
   __device__ __forceinline__ unsigned get(int j) const {
     unsigned r = 0;
 #pragma unroll
-    for (int i = 0; i < NE_EMAX / 16; i++)
-      if ((j >> 4) == i) r = (unsigned)(w[i] >> (4 * (j & 15))) & 15u;
+    for (int i = 0; i < NE_EMAX / PER; i++)
+      if (j / PER == i) r = (unsigned)(w[i] >> (B * (j % PER))) & MASK;
     return r;
   }
 };
+template <int ND>
+using MembN = Memb<(ND <= 2 ? 4 : 8)>;
 
 __device__ __forceinline__ unsigned long long lanes_below() {
   const int lane = threadIdx.x & 63;
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Grid barrier of the clusters of one system (thread 0 of each cluster's
+// workgroup): publishes v, waits for every cluster and returns the maximum.
+// Values are double-buffered by the generation's parity, so a fast cluster
+// entering the next barrier cannot overwrite what a slow one still reads.  A
+// bounded wait: a cluster that stopped early sets `abort`, and every wait
+// gives up after SPIN_CAP polls, so all workgroups always finish.
+constexpr long long SPIN_CAP = 1ll << 24;
+__device__ __attribute__((noinline)) double clu_exchange_max(CluInfo* ci, int cid, double v, int* err) {
+  const int n = ci->nclu;
+  if (n <= 1) return v;
+  const unsigned gen = __hip_atomic_load(&ci->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  unsigned long long* pub = ci->pub[gen & 1u];
+  __hip_atomic_store(&pub[cid], (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+  __threadfence();
+  const unsigned arrived = atomicAdd(&ci->count, 1u);
+  if (arrived == (unsigned)n - 1u) {
+    atomicExch(&ci->count, 0u);
+    __threadfence();
+    atomicAdd(&ci->gen, 1u);
+  } else {
+    long long spins = 0;
+    while (__hip_atomic_load(&ci->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+      if (__hip_atomic_load(&ci->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || ++spins > SPIN_CAP) {
+        raise_error(err, ERR_BARRIER);
+        atomicExch(&ci->abort, 1);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+  }
+  __threadfence();
+  double m = v;
+  for (int c = 0; c < n; c++)
+    m = fmax(m, __longlong_as_double((long long)__hip_atomic_load(&pub[c], __ATOMIC_RELAXED,
+                                                                   __HIP_MEMORY_SCOPE_AGENT)));
+  return m;
 }
 
 // exclusive scan of the per-wave counts in g_ne.wsum (thread 0), total in wsum[NE_W]
@@ -512,7 +584,7 @@ __device__ __forceinline__ void ne_store_slots(const CoD& D, int n) {
 // 38-57, 129-167) when do_memb, and max v^2 over the near set
 // (evolve_mod.f95:555-562).  Returns the wave's count of list members.
 template <int ND>
-__device__ __forceinline__ int ne_memb_pass(const CoP& P, const CoD& D, bool do_stream, bool do_memb, Memb& mk) {
+__device__ __forceinline__ int ne_memb_pass(const CoP& P, const CoD& D, bool do_stream, bool do_memb, MembN<ND>& mk) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int nn = g_ne.near_num;
   const int E = (nn + NE_T - 1) / NE_T;
@@ -573,7 +645,7 @@ __device__ __forceinline__ int ne_memb_pass(const CoP& P, const CoD& D, bool do_
 // inout_particle_list builds it at one thread), its memberships, its slot
 // state, and time_penetrate (evolve_mod.f95:563).
 template <int ND>
-__device__ __forceinline__ void ne_compact(const CoP& P, const CoD& D, const Memb& mk, int wcount) {
+__device__ __forceinline__ void ne_compact(const CoP& P, const CoD& D, const MembN<ND>& mk, int wcount) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int nn = g_ne.near_num;
   const int E = (nn + NE_T - 1) / NE_T;
@@ -606,7 +678,9 @@ __device__ __forceinline__ void ne_compact(const CoP& P, const CoD& D, const Mem
   if (threadIdx.x == 0) {
     g_ne.in_num = total;
     double fm = __longlong_as_double((long long)D.sc->far_maxv2);
-    double maxt = fmax(fm, g_ne.vmax2);
+    double vm = g_ne.vmax2;
+    if (g_ne.clustered) vm = clu_exchange_max(D.clu, D.cid, vm, &D.sc->err);
+    double maxt = fmax(fm, vm);
     g_ne.tp = (int)floor(((0.5 * P.skinrad) / sqrt(maxt)) / P.MDt);
     g_ne.tc = 0;
   }
@@ -625,9 +699,53 @@ __device__ __attribute__((noinline)) void ne_rebuild(bool store_first, bool do_s
     __threadfence_block();
     __syncthreads();
   }
-  Memb mk;
+  MembN<ND> mk;
   int c = ne_memb_pass<ND>(P, D, do_stream, compact, mk);
   if (compact) ne_compact<ND>(P, D, mk, c);
+}
+
+// Cluster mode: the list memberships, stored pair forces and
+// max_solvent_dist of an inside particle live in per-particle records between
+// intervals (the clusters may be regrouped), keyed by global list 2k+i.
+__device__ __forceinline__ void ne_load_record(const CoD& D, int k, int p) {
+  const int cap = D.in_cap;
+  const unsigned ids = D.rec_ids[p];
+  unsigned mb = 0;
+  for (int m = 0; m < 4; m++) {
+    const unsigned gid = (ids >> (8 * m)) & 0xffu;
+    if (gid == 0xffu) break;
+    const int kg = (int)(gid >> 1), ii = (int)(gid & 1u);
+    if (kg >= MAXDG || D.clu->clu_of[kg] != D.cid || D.clu->loc[kg] >= MAXD) {
+      raise_error(&D.sc->err, ERR_CLUSTER_MEMB);
+      continue;
+    }
+    const int b = 2 * D.clu->loc[kg] + ii;
+    mb |= 1u << b;
+    for (int c = 0; c < 3; c++) D.in_fp[(b * 3 + c) * cap + k] = D.rec_fp[(size_t)p * 12 + m * 3 + c];
+    D.in_pot[b * cap + k] = D.rec_pot[(size_t)p * 4 + m];
+  }
+  D.in_memb[k] = mb;
+  D.in_msd[k] = D.rec_msd[p];
+}
+__device__ __forceinline__ void ne_write_records(const CoD& D, int n) {
+  const int cap = D.in_cap, nb = 2 * min(D.clu->cnt[D.cid], MAXD);
+  for (int k = threadIdx.x; k < n; k += NE_T) {
+    const int p = D.in_plist[k];
+    const unsigned mb = D.in_memb[k];
+    unsigned ids = 0xffffffffu;
+    int m = 0;
+    for (int b = 0; b < nb; b++) {
+      if (!((mb >> b) & 1u)) continue;
+      if (m == 4) { raise_error(&D.sc->err, ERR_MEMBERSHIPS); break; }
+      const unsigned gid = 2u * (unsigned)D.clu->dims[D.cid][b >> 1] + (unsigned)(b & 1);
+      ids = (ids & ~(0xffu << (8 * m))) | (gid << (8 * m));
+      for (int c = 0; c < 3; c++) D.rec_fp[(size_t)p * 12 + m * 3 + c] = D.in_fp[(b * 3 + c) * cap + k];
+      D.rec_pot[(size_t)p * 4 + m] = D.in_pot[b * cap + k];
+      m++;
+    }
+    D.rec_ids[p] = ids;
+    D.rec_msd[p] = D.in_msd[k];
+  }
 }
 
 // Interval start: the inside list (unchanged since the last rebuild, same
@@ -661,6 +779,7 @@ __device__ __attribute__((noinline)) void ne_map_inside() {
     if (in && k < n) {
       D.in_list[k] = q;
       slot_from_ns(D, k, q);
+      if (g_ne.clustered) ne_load_record(D, k, D.near_list[q]);
       if (k < LCAP) { g_ne.lmemb[k] = D.in_memb[k]; g_ne.lmsd[k] = D.in_msd[k]; }
     }
   }
@@ -674,6 +793,7 @@ __device__ __forceinline__ void ne_begin_step_regs(const CoP& P, const CoD& D, D
   g_ne.cstep += 1;
   g_ne.tc += 1;
   rattle_positions<ND>(P, d, &D.sc->err);
+  if (g_ne.clustered) maxv = clu_exchange_max(D.clu, D.cid, maxv, &D.sc->err);
   const int rb = (maxv > P.hs2 || g_ne.tc == g_ne.tp || g_ne.cstep % P.MPCtstep == 0);
   g_ne.rebuild = rb;
   if (rb) {
@@ -865,7 +985,7 @@ __device__ __forceinline__ void near_body(const CoP P, const CoD D, int nsteps, 
   constexpr int NV = 4 * NSPH + 1;
   CoScalars* sc = D.sc;
 
-  for (int j = tid; j < 12 * MAXS; j += NE_T) g_ne.sd[j / MAXS][j % MAXS] = D.dim[j];
+  for (int j = tid; j < 12 * MAXS; j += NE_T) g_ne.sd[j / MAXS][j % MAXS] = D.dim[(j / MAXS) * MAXSG + j % MAXS];
   if (tid == 0) {
     if (!in_lds) {
       g_ne.P = P;
@@ -876,10 +996,14 @@ __device__ __forceinline__ void near_body(const CoP P, const CoD D, int nsteps, 
     g_ne.near_num = sc->near_num;
     g_ne.vmax2 = sc->near_vmax2;
     g_ne.rebuild = 0;
+    g_ne.clustered = (mode & M_CLUSTER) != 0;
   }
   __syncthreads();
   if (g_ne.near_num > NE_NEAR) {   // the set does not fit the lanes' membership registers
-    if (tid == 0) raise_error(&sc->err, ERR_NEAR_CAP);
+    if (tid == 0) {
+      raise_error(&sc->err, ERR_NEAR_CAP);
+      if (mode & M_CLUSTER) atomicExch(&D.clu->abort, 1);
+    }
     return;
   }
   ne_gather(D, g_ne.near_num);
@@ -898,6 +1022,7 @@ __device__ __forceinline__ void near_body(const CoP P, const CoD D, int nsteps, 
     }
     __threadfence_block();
     __syncthreads();
+    if (mode & M_CLUSTER) ne_write_records(D, g_ne.in_num);
     ne_scatter(D, g_ne.near_num);
     if (tid == 0) {
       ne_save_scalars(sc);
@@ -942,7 +1067,7 @@ __device__ __forceinline__ void near_body(const CoP P, const CoD D, int nsteps, 
           __syncthreads();
           ne_scatter(D, g_ne.near_num);
           if (tid == 0) ne_save_scalars(sc);
-          for (int j = tid; j < 12 * MAXS; j += NE_T) D.dim[j] = g_ne.sd[j / MAXS][j % MAXS];
+          for (int j = tid; j < 12 * NSPH; j += NE_T) D.dim[(j / NSPH) * MAXSG + j % NSPH] = g_ne.sd[j / NSPH][j % NSPH];
           return;
         }
       } else {
@@ -994,6 +1119,11 @@ __device__ __forceinline__ void near_body(const CoP P, const CoD D, int nsteps, 
     D.in_plist[k] = D.near_list[q];
     if (k < LCAP) D.in_msd[k] = g_ne.lmsd[k];
   }
+  if (mode & M_CLUSTER) {
+    __threadfence_block();
+    __syncthreads();
+    ne_write_records(D, n);
+  }
   ne_store_slots(D, n);
   __threadfence_block();
   __syncthreads();
@@ -1005,7 +1135,7 @@ __device__ __forceinline__ void near_body(const CoP P, const CoD D, int nsteps, 
     ne_save_scalars(sc);
     for (int s = 0; s < NSPH; s++) { sc->x0[s] = g_ne.sd[DX][s]; sc->y0[s] = g_ne.sd[DY][s]; sc->z0[s] = g_ne.sd[DZ][s]; }
   }
-  for (int j = tid; j < 12 * MAXS; j += NE_T) D.dim[j] = g_ne.sd[j / MAXS][j % MAXS];
+  for (int j = tid; j < 12 * NSPH; j += NE_T) D.dim[(j / NSPH) * MAXSG + j % NSPH] = g_ne.sd[j / NSPH][j % NSPH];
 }
 
 template <int ND>
@@ -1018,9 +1148,289 @@ __global__ __launch_bounds__(NE_T) void co_near_engine_b(const CoP* __restrict__
                                                          int nsteps, int mode) {
   near_body<ND>(Ps[blockIdx.x], Ds[blockIdx.x], nsteps, mode, false);
 }
+// one workgroup per cluster of a many-dimer system (a cooperative launch:
+// every workgroup is resident, so the clusters' grid barriers complete)
+__global__ __launch_bounds__(NE_T) void co_near_engine_c(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds,
+                                                         const CluInfo* __restrict__ ci, int nsteps, int mode) {
+  const int c = blockIdx.x;
+  if (c >= ci->nclu) return;
+  switch (Ps[c].nd) {
+    case 1: near_body<1>(Ps[c], Ds[c], nsteps, mode | M_CLUSTER, false); break;
+    case 2: near_body<2>(Ps[c], Ds[c], nsteps, mode | M_CLUSTER, false); break;
+    case 3: near_body<3>(Ps[c], Ds[c], nsteps, mode | M_CLUSTER, false); break;
+    default: near_body<4>(Ps[c], Ds[c], nsteps, mode | M_CLUSTER, false); break;
+  }
+}
 #undef NE_KICK
 #undef NE_STREAM
 #undef SLOT
+
+// ------------------------------------------------------------- clusters
+// Segment bases of the per-cluster near engines (cluster c owns entries
+// [c*cap, (c+1)*cap) of each array) and their descriptors.
+struct CluBase {
+  CoP* Ps;
+  CoD* Ds;
+  CoScalars* scs;
+  double* dimc;                 // [c][12*MAXSG]
+  int* near_list;
+  double *ns_x, *ns_y, *ns_z, *ns_vx, *ns_vy, *ns_vz;
+  int* ns_flag;
+  unsigned char* ns_stat;
+  int *in_list, *in_plist;
+  unsigned* in_memb;
+  double *in_fp, *in_pot, *in_msd;
+  double *sl_x, *sl_y, *sl_z, *sl_vx, *sl_vy, *sl_vz;
+  int* sl_flag;
+  unsigned char* assign;        // dimer of the nearest sphere of each entry of the system's near list
+};
+
+__device__ __forceinline__ void sphere_xyz(const CoD& D, int s, double& x, double& y, double& z) {
+  x = D.dim[DX * MAXSG + s]; y = D.dim[DY * MAXSG + s]; z = D.dim[DZ * MAXSG + s];
+}
+
+// Links between dimers, from the near set: a near particle within reach of
+// spheres of two dimers (|v| tau + guard, plus the skin for list members
+// carried over and 1/32 for the quantized near test) links them, so two
+// clusters never share a particle that could enter a list of either within
+// the interval.  Also the dimer of each entry's nearest sphere (beyond its
+// list radius, the measure of the near/far split), which picks its cluster.
+__global__ __launch_bounds__(256) void clu_links(CoP P, CoD G, unsigned char* near_dimer) {
+  __shared__ double sx[MAXSG], sy[MAXSG], sz[MAXSG];
+  const int nd = P.nd;
+  for (int s = threadIdx.x; s < 2 * nd; s += 256) sphere_xyz(G, s, sx[s], sy[s], sz[s]);
+  __syncthreads();
+  const double tau = P.tau * (1.0 + 1e-9), extra = P.guard + 1e-6 + P.skinrad + 1.0 / 32.0;
+  const int nn = min(G.sc->near_num, G.near_cap);
+  for (int q = blockIdx.x * 256 + threadIdx.x; q < nn; q += gridDim.x * 256) {
+    const int p = G.near_list[q];
+    const double x = G.x[p], y = G.y[p], z = G.z[p], vx = G.vx[p], vy = G.vy[p], vz = G.vz[p];
+    const double reach = sqrt((vx * vx + vy * vy) + vz * vz) * tau + extra;
+    double best = 1e300;
+    int bs = 0, k0 = -1;
+    for (int s = 0; s < 2 * nd; s++) {
+      const double a = pbc_half(x - sx[s], P.box[0]), b = pbc_half(y - sy[s], P.box[1]), c = z - sz[s];
+      const double d = sqrt(a * a + b * b + c * c) - P.full[s >= nd];
+      if (d < best) { best = d; bs = s; }
+      if (d <= reach) {
+        const int k = s < nd ? s : s - nd;
+        if (k0 < 0) k0 = k;
+        else if (k != k0) {
+          atomicOr(&G.clu->adj[k0][k >> 5], 1u << (k & 31));
+          atomicOr(&G.clu->adj[k][k0 >> 5], 1u << (k0 & 31));
+        }
+      }
+    }
+    near_dimer[q] = (unsigned char)(bs < nd ? bs : bs - nd);
+  }
+}
+
+__global__ void clu_clear(CluInfo* ci) {
+  for (int j = threadIdx.x; j < MAXDG * ((MAXDG + 31) / 32); j += blockDim.x) ci->adj[j / ((MAXDG + 31) / 32)][j % ((MAXDG + 31) / 32)] = 0u;
+}
+
+// Clusters: the connected groups of the near-particle links and of spheres
+// within the sphere-sphere cutoff of each other plus twice the guard (each may
+// move that far in the interval), numbered by their smallest dimer index.
+// Also writes each cluster's parameters, descriptor, scalars and local
+// sphere fields.
+__global__ __launch_bounds__(256) void clu_build(CoP P, CoD G, CluBase B) {
+  CluInfo* ci = G.clu;
+  const int nd = P.nd, tid = threadIdx.x;
+  __shared__ unsigned adj[MAXDG][(MAXDG + 31) / 32];
+  __shared__ int lab[MAXDG], nlab[MAXDG];
+  __shared__ int changed;
+  for (int j = tid; j < MAXDG * ((MAXDG + 31) / 32); j += 256)
+    adj[j / ((MAXDG + 31) / 32)][j % ((MAXDG + 31) / 32)] = ci->adj[j / ((MAXDG + 31) / 32)][j % ((MAXDG + 31) / 32)];
+  for (int k = tid; k < nd; k += 256) lab[k] = k;
+  __syncthreads();
+  for (int pr = tid; pr < nd * nd; pr += 256) {
+    const int k = pr / nd, l = pr % nd;
+    if (k >= l) continue;
+    bool link = false;
+    for (int i = 0; i < 2; i++)
+      for (int j = 0; j < 2; j++) {
+        double ax, ay, az, bx, by, bz;
+        sphere_xyz(G, k + i * nd, ax, ay, az);
+        sphere_xyz(G, l + j * nd, bx, by, bz);
+        const double dx = pbc_half(ax - bx, P.box[0]), dy = pbc_half(ay - by, P.box[1]), dz = az - bz;
+        const int kind = (i == 0 && j == 0) ? 0 : (i == 1 && j == 1) ? 2 : 1;
+        const double lim = sqrt(P.mm_cut2[kind]) + 2.0 * P.guard;
+        if (dx * dx + dy * dy + dz * dz <= lim * lim) link = true;
+      }
+    if (link) {
+      atomicOr(&adj[k][l >> 5], 1u << (l & 31));
+      atomicOr(&adj[l][k >> 5], 1u << (k & 31));
+    }
+  }
+  __syncthreads();
+  for (int it = 0; it <= nd; it++) {   // minimum-label propagation over the links
+    if (tid == 0) changed = 0;
+    __syncthreads();
+    for (int k = tid; k < nd; k += 256) {
+      int m = lab[k];
+      for (int l = 0; l < nd; l++)
+        if ((adj[k][l >> 5] >> (l & 31)) & 1u) m = min(m, lab[l]);
+      nlab[k] = m;
+      if (m != lab[k]) changed = 1;
+    }
+    __syncthreads();
+    for (int k = tid; k < nd; k += 256) lab[k] = nlab[k];
+    __syncthreads();
+    if (!changed) break;
+  }
+  if (tid == 0) {
+    int n = 0;
+    for (int k = 0; k < nd; k++)
+      if (lab[k] == k) { nlab[k] = n; ci->cnt[n] = 0; n++; }
+    for (int k = 0; k < nd; k++) {
+      const int c = nlab[lab[k]];
+      ci->clu_of[k] = c;
+      const int j = ci->cnt[c]++;
+      ci->loc[k] = j;
+      if (j < MAXD) ci->dims[c][j] = k; else raise_error(&G.sc->err, ERR_CLUSTER);
+    }
+    ci->nclu = n;
+    ci->count = 0u;
+    ci->abort = 0;
+  }
+  __syncthreads();
+  const int nclu = ci->nclu;
+  for (int c = tid; c < nclu; c += 256) {
+    const int nl = min(ci->cnt[c], MAXD);
+    CoP Pc = P;
+    Pc.nd = nl;
+    B.Ps[c] = Pc;
+    CoD D = G;
+    const size_t nq = (size_t)NE_NEAR * c, ni = (size_t)NE_CAP * c;
+    D.sc = B.scs + c;
+    D.dim = B.dimc + (size_t)c * 12 * MAXSG;
+    D.near_list = B.near_list + nq;
+    D.ns_x = B.ns_x + nq; D.ns_y = B.ns_y + nq; D.ns_z = B.ns_z + nq;
+    D.ns_vx = B.ns_vx + nq; D.ns_vy = B.ns_vy + nq; D.ns_vz = B.ns_vz + nq;
+    D.ns_flag = B.ns_flag + nq; D.ns_stat = B.ns_stat + nq;
+    D.in_list = B.in_list + ni; D.in_plist = B.in_plist + ni; D.in_memb = B.in_memb + ni;
+    D.in_fp = B.in_fp + ni * MAXM * 3; D.in_pot = B.in_pot + ni * MAXM; D.in_msd = B.in_msd + ni;
+    D.sl_x = B.sl_x + ni; D.sl_y = B.sl_y + ni; D.sl_z = B.sl_z + ni;
+    D.sl_vx = B.sl_vx + ni; D.sl_vy = B.sl_vy + ni; D.sl_vz = B.sl_vz + ni; D.sl_flag = B.sl_flag + ni;
+    D.in_cap = NE_CAP;
+    D.near_cap = NE_NEAR;
+    D.cid = c;
+    B.Ds[c] = D;
+    // scalars: the system's, with this cluster's spheres at the interval start
+    CoScalars* cs = B.scs + c;
+    const CoScalars* gs = G.sc;
+    cs->cstep = gs->cstep; cs->time_count = gs->time_count; cs->time_penetrate = gs->time_penetrate;
+    cs->in_num = 0; cs->near_num = 0; cs->n_ev = gs->n_ev; cs->err = 0;
+    for (int e = 0; e < MAXEV; e++) cs->events[e] = gs->events[e];
+    cs->far_maxv2 = gs->far_maxv2; cs->near_vmax2 = gs->near_vmax2; cs->tpotential = 0.0;
+    cs->coll = gs->coll;
+    for (int e = 0; e < 8; e++) cs->prof[e] = gs->prof[e];
+    for (int j = 0; j < nl; j++)
+      for (int i = 0; i < 2; i++) {
+        const int sg = ci->dims[c][j] + i * nd, sl = j + i * nl;
+        cs->x0[sl] = gs->x0[sg]; cs->y0[sl] = gs->y0[sg]; cs->z0[sl] = gs->z0[sg];
+      }
+  }
+  // local sphere fields
+  for (int w = tid; w < nclu * 12 * MAXS; w += 256) {
+    const int c = w / (12 * MAXS), f = (w / MAXS) % 12, sl = w % MAXS;
+    const int nl = min(ci->cnt[c], MAXD);
+    if (sl >= 2 * nl) continue;
+    const int j = sl % nl, i = sl / nl;
+    B.dimc[(size_t)c * 12 * MAXSG + f * MAXSG + sl] = G.dim[f * MAXSG + ci->dims[c][j] + i * nd];
+  }
+}
+
+// per cluster (one workgroup each): its entries of the near list in ascending
+// particle order, and the number of them that are list members
+__global__ __launch_bounds__(1024) void clu_lists(CoD G, CluBase B) {
+  const int c = blockIdx.x;
+  if (c >= G.clu->nclu) return;
+  __shared__ int wcnt[16], wins[16];
+  __shared__ int base;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (threadIdx.x == 0) base = 0;
+  int ins = 0;
+  const int nn = min(G.sc->near_num, G.near_cap);
+  int* out = B.near_list + (size_t)NE_NEAR * c;
+  __syncthreads();
+  for (int q0 = 0; q0 < nn; q0 += 1024) {
+    const int q = q0 + threadIdx.x;
+    const bool mine = q < nn && G.clu->clu_of[B.assign[q]] == c;
+    int p = mine ? G.near_list[q] : 0;
+    const unsigned long long bl = __ballot(mine);
+    if (lane == 0) wcnt[wv] = __popcll(bl);
+    if (mine && (G.status[p] & ST_INSIDE)) ins++;
+    __syncthreads();
+    int off = base;
+    for (int w = 0; w < wv; w++) off += wcnt[w];
+    const int r = off + __popcll(bl & lanes_below());
+    if (mine && r < NE_NEAR) out[r] = p;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int t = 0;
+      for (int w = 0; w < 16; w++) t += wcnt[w];
+      base += t;
+    }
+    __syncthreads();
+  }
+  ins = wave_sum_i(ins);
+  if (lane == 0) wins[wv] = ins;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < 16; w++) t += wins[w];
+    B.scs[c].near_num = base;
+    B.scs[c].in_num = t;
+  }
+}
+
+// after the clusters' launch: their sphere fields and scalars back into the
+// system's (all clusters agree on the step counters and the rebuild events)
+__global__ __launch_bounds__(256) void clu_finish(CoP P, CoD G, CluBase B, int mode) {
+  CluInfo* ci = G.clu;
+  const int nclu = ci->nclu, nd = P.nd;
+  for (int w = threadIdx.x; w < nclu * 12 * MAXS; w += 256) {
+    const int c = w / (12 * MAXS), f = (w / MAXS) % 12, sl = w % MAXS;
+    const int nl = min(ci->cnt[c], MAXD);
+    if (sl >= 2 * nl) continue;
+    const int j = sl % nl, i = sl / nl;
+    G.dim[f * MAXSG + ci->dims[c][j] + i * nd] = B.dimc[(size_t)c * 12 * MAXSG + f * MAXSG + sl];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    CoScalars* gs = G.sc;
+    const CoScalars* s0 = B.scs;
+    int err = 0, in_num = 0, near_num = 0;
+    double tp = 0.0, nv = 0.0;
+    unsigned long long pf[5] = {0, 0, 0, 0, 0};
+    for (int c = 0; c < nclu; c++) {
+      const CoScalars* cs = B.scs + c;
+      if (!err && cs->err) err = cs->err;
+      if (cs->cstep != s0->cstep || cs->time_count != s0->time_count || cs->time_penetrate != s0->time_penetrate ||
+          cs->n_ev != s0->n_ev)
+        if (!err) err = ERR_BARRIER;
+      in_num += cs->in_num;
+      near_num += cs->near_num;
+      tp = tp + cs->tpotential;
+      nv = fmax(nv, cs->near_vmax2);
+      for (int e = 0; e < 5; e++) pf[e] = max(pf[e], cs->prof[e]);
+    }
+    if (!err && ci->abort) err = ERR_BARRIER;
+    if (err) raise_error(&gs->err, err);
+    gs->cstep = s0->cstep; gs->time_count = s0->time_count; gs->time_penetrate = s0->time_penetrate;
+    gs->n_ev = s0->n_ev;
+    for (int e = 0; e < MAXEV; e++) gs->events[e] = s0->events[e];
+    gs->in_num = in_num;
+    gs->near_vmax2 = nv;
+    gs->tpotential = tp;
+    for (int e = 0; e < 5; e++) gs->prof[e] = pf[e];
+    gs->prof[5] = near_num; gs->prof[6] = in_num; gs->prof[7] = s0->prof[7];
+  }
+  if (!(mode & M_STOP))
+    for (int s = threadIdx.x; s < 2 * nd; s += 256) sphere_xyz(G, s, G.sc->x0[s], G.sc->y0[s], G.sc->z0[s]);
+}
 
 // ------------------------------------------------------------- bulk kernels
 // A block of a batched launch works on elements [begin, end) of replica r.
@@ -1082,12 +1492,12 @@ __device__ __forceinline__ void bulk_body(const CoP& P, const CoD& D, int do_str
   const int NT = P.NT;
   const int nev = D.sc->n_ev;
   __shared__ int ev[MAXEV];
-  __shared__ double sph[3][MAXS];
+  __shared__ double sph[3][MAXSG];
   for (int j = threadIdx.x; j < nev && j < MAXEV; j += blockDim.x) ev[j] = D.sc->events[j];
-  if (threadIdx.x < 2 * P.nd) {
-    sph[0][threadIdx.x] = D.sc->x0[threadIdx.x];
-    sph[1][threadIdx.x] = D.sc->y0[threadIdx.x];
-    sph[2][threadIdx.x] = D.sc->z0[threadIdx.x];
+  for (int j = threadIdx.x; j < 2 * P.nd; j += blockDim.x) {
+    sph[0][j] = D.sc->x0[j];
+    sph[1][j] = D.sc->y0[j];
+    sph[2][j] = D.sc->z0[j];
   }
   __syncthreads();
   double xs = 0, ys = 0, zs = 0;
@@ -1476,6 +1886,9 @@ struct mpcx_co {
   long long nwords = 0;
   int nd = 1;
   std::vector<int> mpc_pos, mpc_cell;   // host copies of the collision-cell tour
+  // clusters (two or more dimers): one near-engine workgroup per cluster
+  bool clustered = false, can_cluster = false;
+  CluBase cb;
 };
 
 template <typename T>
@@ -1502,8 +1915,12 @@ static int check_device_error(mpcx_co* e) {
                                 "a neighbour list is longer than max_neighbour",
                                 "more than 64 list rebuilds in one interval",
                                 "the dimer bond constraint did not converge",
-                                "a neighbour-list member is missing from the near set"};
-    std::string m = std::string("device check failed: ") + (err > 0 && err < 12 ? msg[err] : "unknown");
+                                "a neighbour-list member is missing from the near set",
+                                "the host draw buffer ran out",
+                                "more than four dimers lie within reach of each other (one cluster)",
+                                "the clusters of the system fell out of step",
+                                "a neighbour-list member was carried to a cluster without its dimer"};
+    std::string m = std::string("device check failed: ") + (err > 0 && err < 16 ? msg[err] : "unknown");
     int zero = 0;
     (void)hipMemcpy(&e->D.sc->err, &zero, sizeof(int), hipMemcpyHostToDevice);
     return set_error(MPCX_E_DEVICE, m.c_str());
@@ -1520,8 +1937,8 @@ static int check_device_error(mpcx_co* e) {
 extern "C" int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg) {
   if (!out || !cfg) return set_error(MPCX_E_ARG, "null argument");
   *out = nullptr;
-  if (cfg->numdims < 1 || cfg->numdims > MAXD)
-    return set_error(MPCX_E_UNSUPPORTED, "the device-resident collective engine handles 1 or 2 dimers");
+  if (cfg->numdims < 1 || cfg->numdims > MAXDG)
+    return set_error(MPCX_E_UNSUPPORTED, "the device-resident collective engine handles 1 to 120 dimers");
   if (cfg->NT < 1 || cfg->NT > 0x7fffff00LL || cfg->vol < 1) return set_error(MPCX_E_ARG, "bad sizes");
   if (cfg->sort_step % cfg->MPCtstep != 0)
     return set_error(MPCX_E_UNSUPPORTED, "sort_step must be a multiple of MPCtstep for the device-resident engine");
@@ -1578,7 +1995,7 @@ extern "C" int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg) {
   P.a0 = cfg->a0; P.cosrot = cfg->cosrot; P.sinrot = cfg->sinrot; P.omc = 1.0 - cfg->cosrot;
   P.twopi = 2.0 * (4.0 * atan(1.0));
   P.tau = (double)cfg->MPCtstep * MDt;
-  P.guard = cfg->sphere_guard > 0 ? cfg->sphere_guard : 1.0;
+  P.guard = cfg->sphere_guard > 0 ? cfg->sphere_guard : 0.3;
   P.skinrad = cfg->skinrad;
 
   const long long NT = cfg->NT, vol = cfg->vol, mc = cfg->mpcnumcell;
@@ -1587,17 +2004,44 @@ extern "C" int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg) {
   D.x = dalloc<double>(e, NT); D.y = dalloc<double>(e, NT); D.z = dalloc<double>(e, NT);
   D.vx = dalloc<double>(e, NT); D.vy = dalloc<double>(e, NT); D.vz = dalloc<double>(e, NT);
   D.flag = dalloc<int>(e, NT);
-  D.dim = dalloc<double>(e, 12 * MAXS);
+  D.dim = dalloc<double>(e, 12 * MAXSG);
   D.sc = dalloc<CoScalars>(e, 1);
   D.react_box = dalloc<int>(e, std::max(1, P.nrb));
   D.in_cap = NE_CAP;
-  D.near_cap = (int)std::min<long long>(NT, NE_NEAR);
+  e->can_cluster = e->nd >= 2;
+  e->clustered = e->nd > 2;
+  // the system's near list holds every cluster's near set
+  const long long nsq = std::min<long long>(NT, NE_NEAR);
+  D.near_cap = (int)std::min<long long>(NT, (long long)NE_NEAR * (e->can_cluster ? e->nd : 1));
   D.near_list = dalloc<int>(e, D.near_cap);
-  D.ns_x = dalloc<double>(e, D.near_cap); D.ns_y = dalloc<double>(e, D.near_cap); D.ns_z = dalloc<double>(e, D.near_cap);
-  D.ns_vx = dalloc<double>(e, D.near_cap); D.ns_vy = dalloc<double>(e, D.near_cap);
-  D.ns_vz = dalloc<double>(e, D.near_cap);
-  D.ns_flag = dalloc<int>(e, D.near_cap);
-  D.ns_stat = dalloc<unsigned char>(e, D.near_cap);
+  D.ns_x = dalloc<double>(e, nsq); D.ns_y = dalloc<double>(e, nsq); D.ns_z = dalloc<double>(e, nsq);
+  D.ns_vx = dalloc<double>(e, nsq); D.ns_vy = dalloc<double>(e, nsq);
+  D.ns_vz = dalloc<double>(e, nsq);
+  D.ns_flag = dalloc<int>(e, nsq);
+  D.ns_stat = dalloc<unsigned char>(e, nsq);
+  if (e->can_cluster) {
+    const size_t nc = (size_t)e->nd, nq = nc * NE_NEAR, ni = nc * NE_CAP;
+    CluBase& B = e->cb;
+    memset(&B, 0, sizeof B);
+    D.clu = dalloc<CluInfo>(e, 1);
+    B.Ps = dalloc<CoP>(e, nc); B.Ds = dalloc<CoD>(e, nc); B.scs = dalloc<CoScalars>(e, nc);
+    B.dimc = dalloc<double>(e, nc * 12 * MAXSG);
+    B.near_list = dalloc<int>(e, nq);
+    B.ns_x = dalloc<double>(e, nq); B.ns_y = dalloc<double>(e, nq); B.ns_z = dalloc<double>(e, nq);
+    B.ns_vx = dalloc<double>(e, nq); B.ns_vy = dalloc<double>(e, nq); B.ns_vz = dalloc<double>(e, nq);
+    B.ns_flag = dalloc<int>(e, nq); B.ns_stat = dalloc<unsigned char>(e, nq);
+    B.in_list = dalloc<int>(e, ni); B.in_plist = dalloc<int>(e, ni); B.in_memb = dalloc<unsigned>(e, ni);
+    B.in_fp = dalloc<double>(e, ni * MAXM * 3); B.in_pot = dalloc<double>(e, ni * MAXM);
+    B.in_msd = dalloc<double>(e, ni);
+    B.sl_x = dalloc<double>(e, ni); B.sl_y = dalloc<double>(e, ni); B.sl_z = dalloc<double>(e, ni);
+    B.sl_vx = dalloc<double>(e, ni); B.sl_vy = dalloc<double>(e, ni); B.sl_vz = dalloc<double>(e, ni);
+    B.sl_flag = dalloc<int>(e, ni);
+    B.assign = dalloc<unsigned char>(e, (size_t)D.near_cap);
+    D.rec_ids = dalloc<unsigned>(e, NT);
+    D.rec_msd = dalloc<double>(e, NT);
+    D.rec_fp = dalloc<double>(e, (size_t)NT * 12);
+    D.rec_pot = dalloc<double>(e, (size_t)NT * 4);
+  }
   e->nwords = (NT + 31) / 32;
   D.near_bits = dalloc<unsigned>(e, e->nwords + 1);
   D.status = dalloc<unsigned char>(e, NT);
@@ -1655,7 +2099,22 @@ extern "C" int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg) {
   HIPCHK(hipMemset(D.in_fp, 0, (size_t)MAXM * 3 * D.in_cap * 8));
   HIPCHK(hipMemset(D.in_pot, 0, (size_t)MAXM * D.in_cap * 8));
   HIPCHK(hipMemset(D.in_msd, 0, (size_t)D.in_cap * 8));
+  if (e->can_cluster) {
+    HIPCHK(hipMemset(D.clu, 0, sizeof(CluInfo)));
+    HIPCHK(hipMemset(D.rec_ids, 0xff, (size_t)NT * 4));
+    HIPCHK(hipMemset(D.rec_msd, 0, (size_t)NT * 8));
+  }
   *out = e;
+  return 0;
+}
+
+// 1: one near-engine workgroup per cluster of dimers (two or more dimers);
+// 0: one workgroup for the whole system (up to four dimers)
+extern "C" int mpcx_co_set_clusters(mpcx_co* e, int32_t on) {
+  if (!e) return set_error(MPCX_E_ARG, "null engine");
+  if (on && !e->can_cluster) return set_error(MPCX_E_UNSUPPORTED, "clusters need two or more dimers");
+  if (!on && e->nd > MAXD) return set_error(MPCX_E_UNSUPPORTED, "one workgroup handles up to four dimers");
+  e->clustered = on != 0;
   return 0;
 }
 
@@ -1689,10 +2148,10 @@ extern "C" int mpcx_co_set_state(mpcx_co* e, int32_t cstep, const double* x, con
   HIPCHK(hipMemcpy(D.vy, vy, NT * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(D.vz, vz, NT * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(D.flag, flag, NT * 4, hipMemcpyHostToDevice));
-  std::vector<double> dim(12 * MAXS, 0.0);
+  std::vector<double> dim(12 * MAXSG, 0.0);
   const double* src[9] = {xd, yd, zd, vxd, vyd, vzd, fxd, fyd, fzd};
   for (int f = 0; f < 9; f++)
-    for (size_t s = 0; s < 2 * nd; s++) dim[f * MAXS + s] = src[f] ? src[f][s] : 0.0;
+    for (size_t s = 0; s < 2 * nd; s++) dim[f * MAXSG + s] = src[f] ? src[f][s] : 0.0;
   HIPCHK(hipMemcpy(D.dim, dim.data(), dim.size() * 8, hipMemcpyHostToDevice));
   std::vector<int> rb((size_t)std::max(1, e->P.nrb), 0);
   if (react_box)
@@ -1763,11 +2222,31 @@ static void co_near_list(mpcx_co* e) {
                    e->P.NT, &e->D.sc->near_num, &e->D.sc->err);
 }
 
-static void launch_near(mpcx_co* e, int nsteps, int mode) {
-  if (e->nd == 1)
-    hipLaunchKernelGGL(co_near_engine<1>, dim3(1), dim3(NE_T), 0, e->st, e->P, e->D, nsteps, mode);
-  else
-    hipLaunchKernelGGL(co_near_engine<2>, dim3(1), dim3(NE_T), 0, e->st, e->P, e->D, nsteps, mode);
+static hipError_t launch_near(mpcx_co* e, int nsteps, int mode) {
+  if (e->clustered) {
+    CoD& D = e->D;
+    CluBase& B = e->cb;
+    hipLaunchKernelGGL(clu_clear, dim3(1), dim3(256), 0, e->st, D.clu);
+    hipLaunchKernelGGL(clu_links, dim3(grid_for(D.near_cap, 256, 1024)), dim3(256), 0, e->st, e->P, D, B.assign);
+    hipLaunchKernelGGL(clu_build, dim3(1), dim3(256), 0, e->st, e->P, D, B);
+    hipLaunchKernelGGL(clu_lists, dim3(e->nd), dim3(1024), 0, e->st, D, B);
+    const CoP* Ps = B.Ps;
+    const CoD* Ds = B.Ds;
+    const CluInfo* ci = D.clu;
+    void* args[] = {(void*)&Ps, (void*)&Ds, (void*)&ci, (void*)&nsteps, (void*)&mode};
+    hipError_t r = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(co_near_engine_c), dim3(e->nd),
+                                              dim3(NE_T), args, 0, e->st);
+    if (r != hipSuccess) return r;
+    hipLaunchKernelGGL(clu_finish, dim3(1), dim3(256), 0, e->st, e->P, D, B, mode);
+    return hipGetLastError();
+  }
+  switch (e->nd) {
+    case 1: hipLaunchKernelGGL(co_near_engine<1>, dim3(1), dim3(NE_T), 0, e->st, e->P, e->D, nsteps, mode); break;
+    case 2: hipLaunchKernelGGL(co_near_engine<2>, dim3(1), dim3(NE_T), 0, e->st, e->P, e->D, nsteps, mode); break;
+    case 3: hipLaunchKernelGGL(co_near_engine<3>, dim3(1), dim3(NE_T), 0, e->st, e->P, e->D, nsteps, mode); break;
+    default: hipLaunchKernelGGL(co_near_engine<4>, dim3(1), dim3(NE_T), 0, e->st, e->P, e->D, nsteps, mode); break;
+  }
+  return hipGetLastError();
 }
 
 extern "C" int mpcx_co_setup(mpcx_co* e, int32_t do_sort) {
@@ -1776,11 +2255,11 @@ extern "C" int mpcx_co_setup(mpcx_co* e, int32_t do_sort) {
   const long long NT = e->P.NT;
   hipStream_t st = e->st;
   // sphere positions of this interval start
-  std::vector<double> dim(12 * MAXS);
+  std::vector<double> dim(12 * MAXSG);
   HIPCHK(hipMemcpy(dim.data(), D.dim, dim.size() * 8, hipMemcpyDeviceToHost));
   CoScalars sc;
   HIPCHK(hipMemcpy(&sc, D.sc, sizeof sc, hipMemcpyDeviceToHost));
-  for (int s = 0; s < 2 * e->nd; s++) { sc.x0[s] = dim[DX * MAXS + s]; sc.y0[s] = dim[DY * MAXS + s]; sc.z0[s] = dim[DZ * MAXS + s]; }
+  for (int s = 0; s < 2 * e->nd; s++) { sc.x0[s] = dim[DX * MAXSG + s]; sc.y0[s] = dim[DY * MAXSG + s]; sc.z0[s] = dim[DZ * MAXSG + s]; }
   sc.far_maxv2 = 0; sc.near_vmax2 = 0.0; sc.n_ev = 0; sc.err = 0;
   HIPCHK(hipMemcpy(D.sc, &sc, sizeof sc, hipMemcpyHostToDevice));
   hipLaunchKernelGGL(fill_kernel<unsigned>, dim3(grid_for(e->nwords + 1, 256)), dim3(256), 0, st, D.near_bits,
@@ -1795,7 +2274,7 @@ extern "C" int mpcx_co_setup(mpcx_co* e, int32_t do_sort) {
   co_near_list(e);
   hipLaunchKernelGGL(fill_kernel<unsigned char>, dim3(grid_for(NT, 256)), dim3(256), 0, st, D.status, NT,
                      (unsigned char)0);
-  launch_near(e, 0, M_SETUP);
+  HIPCHK(launch_near(e, 0, M_SETUP));
   HIPCHK(hipStreamSynchronize(st));
   return check_device_error(e);
 }
@@ -1837,7 +2316,7 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
       for (int k = 0; k < mpcx_co::NKT; k++) e->tev_on[(size_t)c * mpcx_co::NKT + k] = 0;
     HIPCHK(hipEventRecord(e->n0, st));
     HIPCHK(tmark(0, 0));
-    launch_near(e, P.MPCtstep, sort_due ? M_STOP : 0);
+    HIPCHK(launch_near(e, P.MPCtstep, sort_due ? M_STOP : 0));
     HIPCHK(tmark(0, 1));
     HIPCHK(hipEventRecord(e->n1, st));
     hipLaunchKernelGGL(fill_kernel<int>, dim3(grid_for(mc, 256)), dim3(256), 0, st, D.cnt, mc, 0);
@@ -1848,7 +2327,7 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
       int r = co_sort(e, st);
       if (r) return r;
       co_near_list(e);
-      launch_near(e, 1, M_RESUME);
+      HIPCHK(launch_near(e, 1, M_RESUME));
       HIPCHK(hipEventRecord(e->b0, st));
       hipLaunchKernelGGL(co_bulk, dim3(gN), dim3(256), 0, st, P, D, 0, 1);
     } else {
@@ -1917,11 +2396,11 @@ extern "C" int mpcx_co_get_state(mpcx_co* e, double* x, double* y, double* z, do
   if (vz) HIPCHK(hipMemcpy(vz, D.vz, NT * 8, hipMemcpyDeviceToHost));
   if (flag) HIPCHK(hipMemcpy(flag, D.flag, NT * 4, hipMemcpyDeviceToHost));
   if (dimer12) {
-    std::vector<double> dim(12 * MAXS);
+    std::vector<double> dim(12 * MAXSG);
     HIPCHK(hipMemcpy(dim.data(), D.dim, dim.size() * 8, hipMemcpyDeviceToHost));
     const int ns = 2 * e->nd;
     for (int f = 0; f < 12; f++)
-      for (int s = 0; s < ns; s++) dimer12[f * ns + s] = dim[f * MAXS + s];
+      for (int s = 0; s < ns; s++) dimer12[f * ns + s] = dim[f * MAXSG + s];
   }
   if (scalars) {
     CoScalars sc;
@@ -1934,6 +2413,7 @@ extern "C" int mpcx_co_get_state(mpcx_co* e, double* x, double* y, double* z, do
 extern "C" int mpcx_co_get_inside(mpcx_co* e, int32_t* list, uint32_t* memb, double* fp, double* pot, int64_t cap,
                                   int64_t* n) {
   if (!e || !n) return set_error(MPCX_E_ARG, "null argument");
+  if (e->clustered) return set_error(MPCX_E_UNSUPPORTED, "with clusters the lists are read with mpcx_co_get_lists");
   HIPCHK(hipStreamSynchronize(e->st));
   CoScalars sc;
   HIPCHK(hipMemcpy(&sc, e->D.sc, sizeof sc, hipMemcpyDeviceToHost));
@@ -1948,6 +2428,84 @@ extern "C" int mpcx_co_get_inside(mpcx_co* e, int32_t* list, uint32_t* memb, dou
   if (pot)
     for (int j = 0; j < MAXM; j++)
       HIPCHK(hipMemcpy(pot + j * cap, e->D.in_pot + j * ic, m * 8, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// Neighbour lists in the reference's layout (neighbour_list_mod.f95:38-57):
+// nb/fpx/fpy/fpz/pot as (numdims, 2, max_nb) column-major with 1-based
+// particle indices in ascending order, nbnum as (numdims, 2); the inside list
+// (0-based, ascending) in in_list.  Works with and without clusters.
+extern "C" int mpcx_co_get_lists(mpcx_co* e, int64_t max_nb, int64_t* nb, int64_t* nbnum, double* fpx, double* fpy,
+                                 double* fpz, double* pot, int32_t* in_list, int64_t in_cap, int64_t* in_num) {
+  if (!e || !nbnum || !in_num || max_nb < 0) return set_error(MPCX_E_ARG, "bad arguments");
+  HIPCHK(hipStreamSynchronize(e->st));
+  const int nd = e->nd;
+  std::vector<std::vector<int>> lp(2 * nd);
+  std::vector<std::vector<double>> lf(2 * nd);   // fx fy fz pot per member
+  std::vector<int> all;
+  auto take = [&](const CoD& D, int n, const int* dims) -> int {
+    const size_t cap = (size_t)D.in_cap;
+    std::vector<int> pl(n);
+    std::vector<unsigned> mb(n);
+    std::vector<double> fp((size_t)MAXM * 3 * n), pt((size_t)MAXM * n);
+    if (n) {
+      HIPCHK(hipMemcpy(pl.data(), D.in_plist, (size_t)n * 4, hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(mb.data(), D.in_memb, (size_t)n * 4, hipMemcpyDeviceToHost));
+      for (int j = 0; j < MAXM * 3; j++)
+        HIPCHK(hipMemcpy(fp.data() + (size_t)j * n, D.in_fp + j * cap, (size_t)n * 8, hipMemcpyDeviceToHost));
+      for (int j = 0; j < MAXM; j++)
+        HIPCHK(hipMemcpy(pt.data() + (size_t)j * n, D.in_pot + j * cap, (size_t)n * 8, hipMemcpyDeviceToHost));
+    }
+    for (int k = 0; k < n; k++) {
+      all.push_back(pl[k]);
+      for (int b = 0; b < MAXM; b++) {
+        if (!((mb[k] >> b) & 1u)) continue;
+        const int kg = dims ? dims[b >> 1] : (b >> 1), i = b & 1;
+        if (kg < 0 || kg >= nd) return set_error(MPCX_E_DEVICE, "list membership outside the system's dimers");
+        const int L = kg + nd * i;
+        lp[L].push_back(pl[k]);
+        for (int c = 0; c < 3; c++) lf[L].push_back(fp[(size_t)(b * 3 + c) * n + k]);
+        lf[L].push_back(pt[(size_t)b * n + k]);
+      }
+    }
+    return 0;
+  };
+  if (!e->clustered) {
+    CoScalars sc;
+    HIPCHK(hipMemcpy(&sc, e->D.sc, sizeof sc, hipMemcpyDeviceToHost));
+    int rc = take(e->D, sc.in_num, nullptr);
+    if (rc) return rc;
+  } else {
+    CluInfo ci;
+    HIPCHK(hipMemcpy(&ci, e->D.clu, sizeof ci, hipMemcpyDeviceToHost));
+    std::vector<CoD> Ds(ci.nclu);
+    std::vector<CoScalars> scs(ci.nclu);
+    HIPCHK(hipMemcpy(Ds.data(), e->cb.Ds, sizeof(CoD) * ci.nclu, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(scs.data(), e->cb.scs, sizeof(CoScalars) * ci.nclu, hipMemcpyDeviceToHost));
+    for (int c = 0; c < ci.nclu; c++) {
+      int rc = take(Ds[c], scs[c].in_num, ci.dims[c]);
+      if (rc) return rc;
+    }
+  }
+  std::sort(all.begin(), all.end());
+  *in_num = (int64_t)all.size();
+  if (in_list) {
+    if ((int64_t)all.size() > in_cap) return set_error(MPCX_E_ARG, "buffer smaller than the inside list");
+    for (size_t j = 0; j < all.size(); j++) in_list[j] = all[j];
+  }
+  for (int L = 0; L < 2 * nd; L++) {
+    const int64_t cnt = (int64_t)lp[L].size();
+    nbnum[L] = cnt;
+    if (cnt > max_nb) return set_error(MPCX_E_DEVICE, "a neighbour list is longer than max_nb");
+    for (int64_t j = 0; j < cnt; j++) {
+      const size_t o = (size_t)L + (size_t)2 * nd * (size_t)j;
+      if (nb) nb[o] = (int64_t)lp[L][j] + 1;
+      if (fpx) fpx[o] = lf[L][4 * j + 0];
+      if (fpy) fpy[o] = lf[L][4 * j + 1];
+      if (fpz) fpz[o] = lf[L][4 * j + 2];
+      if (pot) pot[o] = lf[L][4 * j + 3];
+    }
+  }
   return 0;
 }
 
@@ -1990,11 +2548,11 @@ int co_view(mpcx_co* e, DevView* v) {
   v->species[0] = 0; v->species[1] = 1; v->species[2] = 2; v->species[3] = -1;
   v->periodic_z = 0;
   v->nd = e->nd;
-  std::vector<double> dim(12 * MAXS);
+  std::vector<double> dim(12 * MAXSG);
   if (hipMemcpy(dim.data(), D.dim, dim.size() * 8, hipMemcpyDeviceToHost) != hipSuccess)
     return set_error(MPCX_E_HIP, "hipMemcpy failed");
   for (int f = 0; f < 12; f++)
-    for (int s = 0; s < 2 * e->nd; s++) v->dim[f][s] = dim[f * MAXS + s];
+    for (int s = 0; s < 2 * e->nd; s++) v->dim[f][s] = dim[f * MAXSG + s];
   v->msph[0] = e->cfg.Mc; v->msph[1] = e->cfg.Mn;
   if (hipMemcpy(&v->tpotential, &D.sc->tpotential, 8, hipMemcpyDeviceToHost) != hipSuccess)
     return set_error(MPCX_E_HIP, "hipMemcpy failed");
@@ -2364,6 +2922,7 @@ extern "C" int mpcx_cob_create(mpcx_cob** out, int32_t R, mpcx_co* const* engine
     if (e->P.rng_mode != 1) return set_error(MPCX_E_UNSUPPORTED, "a batch runs in rng_mode 1 (device Philox)");
     if (e->nd != e0->nd || e->P.MPCtstep != e0->P.MPCtstep || e->P.sort_step != e0->P.sort_step)
       return set_error(MPCX_E_UNSUPPORTED, "replicas of a batch share numdims, MPCtstep and sort_step");
+    if (e->nd > 2) return set_error(MPCX_E_UNSUPPORTED, "batched replicas hold one or two dimers");
   }
   if (tile < 256 || tile % 256 != 0) return set_error(MPCX_E_ARG, "tile must be a positive multiple of 256");
   mpcx_cob* b = new mpcx_cob();
@@ -2498,17 +3057,17 @@ __global__ __launch_bounds__(256) void body_acc_b(const CoP* __restrict__ Ps, co
   const CoP P = Ps[r];
   const CoD D = Ds[r];
   const int nd = P.nd;
-  const double cx = D.dim[DX * MAXS + 0], cy = D.dim[DY * MAXS + 0], cz = D.dim[DZ * MAXS + 0];
-  double ux = cx - D.dim[DX * MAXS + nd], uy = cy - D.dim[DY * MAXS + nd], uz = cz - D.dim[DZ * MAXS + nd];
+  const double cx = D.dim[DX * MAXSG + 0], cy = D.dim[DY * MAXSG + 0], cz = D.dim[DZ * MAXSG + 0];
+  double ux = cx - D.dim[DX * MAXSG + nd], uy = cy - D.dim[DY * MAXSG + nd], uz = cz - D.dim[DZ * MAXSG + nd];
   ux = pbc_half(ux, P.box[0]);
   uy = pbc_half(uy, P.box[1]);
   const double len = sqrt(ux * ux + uy * uy + uz * uz);
   ux = ux / len; uy = uy / len; uz = uz / len;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     const double M = P.Mc + P.Mn;
-    const double vx = (P.Mc * D.dim[DVX * MAXS + 0] + P.Mn * D.dim[DVX * MAXS + nd]) / M;
-    const double vy = (P.Mc * D.dim[DVY * MAXS + 0] + P.Mn * D.dim[DVY * MAXS + nd]) / M;
-    const double vz = (P.Mc * D.dim[DVZ * MAXS + 0] + P.Mn * D.dim[DVZ * MAXS + nd]) / M;
+    const double vx = (P.Mc * D.dim[DVX * MAXSG + 0] + P.Mn * D.dim[DVX * MAXSG + nd]) / M;
+    const double vy = (P.Mc * D.dim[DVY * MAXSG + 0] + P.Mn * D.dim[DVY * MAXSG + nd]) / M;
+    const double vz = (P.Mc * D.dim[DVZ * MAXSG + 0] + P.Mn * D.dim[DVZ * MAXSG + nd]) / M;
     vacc[r] += (vx * ux + vy * uy) + vz * uz;
   }
   const double dr = G.rmax / G.nr, dz = (G.zmax - G.zmin) / G.nz;

@@ -26,6 +26,9 @@ enum DevError : int {
   ERR_RATTLE = 10,         // the bond constraint did not converge
   ERR_INSIDE_LOST = 11,    // a list member missing from the next interval's near set
   ERR_DRAWS = 12,          // the host draw buffer ran out before the run ended
+  ERR_CLUSTER = 13,        // more than four dimers whose near regions overlap
+  ERR_BARRIER = 14,        // the clusters of one system fell out of step
+  ERR_CLUSTER_MEMB = 15,   // a list member carried over to a cluster without its dimer
 };
 
 __device__ __forceinline__ void raise_error(int* err, int code) { atomicCAS(err, 0, code); }

@@ -25,7 +25,7 @@ struct DevView {
   double mass[4];         // particle mass by flag value 0..3 (0 for flags that are not particles)
   int species[4];         // species column of each flag value (-1: not counted)
   int periodic_z;         // collective_code has walls in z
-  double dim[12][4];      // sphere fields (x y z vx vy vz fx fy fz ...) of up to two dimers, s = k + i*nd
+  double dim[12][240];    // sphere fields (x y z vx vy vz fx fy fz ...) of up to 120 dimers, s = k + i*nd
   int nd;                 // dimers
   double msph[2];         // C and N sphere masses
   double tpotential;

@@ -48,6 +48,18 @@ CASES = {
     "two_dimers_sort": dict(ncoll=3, seed=7, flags_mix=True,
                             p=("small", dict(numdims=2, Lx=20.0, Ly=20.0, Lz=24.0, sort_step=20, EbN=0.1)),
                             tour="hilbert"),
+    "three_dimers_sort": dict(ncoll=3, seed=11, flags_mix=True,
+                              p=("small", dict(numdims=3, Lx=18.0, Ly=18.0, Lz=16.0, sort_step=20, EbN=0.1)),
+                              tour="hilbert"),
+    "four_dimers": dict(ncoll=3, seed=12, p=("small", dict(numdims=4, Lx=18.0, Ly=18.0, Lz=14.0, EbN=0.3))),
+    "four_dimers_one_workgroup": dict(ncoll=3, seed=12, clusters=False,
+                                      p=("small", dict(numdims=4, Lx=18.0, Ly=18.0, Lz=14.0, EbN=0.3))),
+    "two_dimers_clusters": dict(ncoll=3, seed=7, flags_mix=True, clusters=True,
+                                p=("small", dict(numdims=2, Lx=20.0, Ly=20.0, Lz=24.0, sort_step=20, EbN=0.1)),
+                                tour="hilbert"),
+    "eight_dimers_clusters_sort": dict(ncoll=4, seed=13, flags_mix=True,
+                                       p=("small", dict(numdims=8, Lx=28.0, Ly=28.0, Lz=16.0, sort_step=20)),
+                                       tour="hilbert"),
     "reactive_wall": dict(ncoll=3, seed=8, p=("small", dict(k1=0.2, Lz=12.0)), react="wall2"),
     "dense_skin05": dict(ncoll=2, seed=9, flags_mix=True, p=("small", dict(rhoa=8.0, skinrad=0.5, Rn=2.5))),
 }
@@ -85,9 +97,11 @@ def _h(a):
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
 
-@pytest.mark.parametrize("case", ["co_one_dimer", "co_two_dimers_sort_wall"])
-def test_engine_integer_state_equals_reference_fixture(golden, case):
-    """Flags and cell arrays hashed from the reference's Fortran run."""
+@pytest.mark.parametrize("case,clusters", [("co_one_dimer", None), ("co_two_dimers_sort_wall", None),
+                                           ("co_two_dimers_sort_wall", True), ("co_six_dimers_sort", None)])
+def test_engine_integer_state_equals_reference_fixture(golden, case, clusters):
+    """Flags and cell arrays hashed from the reference's Fortran run (six
+    dimers: clusters, each advanced by its own workgroup)."""
     from tests.golden.make_golden import collective_inputs
     rec = golden["cases"][case]
     cs = rec["case"]
@@ -97,7 +111,7 @@ def test_engine_integer_state_equals_reference_fixture(golden, case):
     ncoll = cs["steps"] // p["MPCtstep"]
     assert ncoll * p["MPCtstep"] == cs["steps"]
     draws = host_draws(MtmodStream(mt=s["mt"], mti=s["mti"]), p, ncoll)
-    eng = CollectiveEngine(p, rng_mode=0)
+    eng = CollectiveEngine(p, rng_mode=0, clusters=clusters)
     try:
         eng.set_state(s, tour=tour)
         eng.setup(do_sort=cs["sort"])
