@@ -166,15 +166,18 @@ int mpcx_co_get_inside(mpcx_co* e, int32_t* list, uint32_t* memb, double* fp, do
  * its length.  Any pointer but nbnum and in_num may be NULL. */
 int mpcx_co_get_lists(mpcx_co* e, int64_t max_nb, int64_t* nb, int64_t* nbnum, double* fpx, double* fpy,
                       double* fpz, double* pot, int32_t* in_list, int64_t in_cap, int64_t* in_num);
-/* Near-engine layout for two or more dimers.  on = 1 (the default above two
- * dimers): dimers whose near regions can meet within an interval are grouped
- * into clusters of at most four, each advanced by its own workgroup, the
- * workgroups meeting at a grid barrier every MD step to take the
- * system-wide list-rebuild decision of evolve_mod.f95:457-465.  on = 0: one
- * workgroup for the whole system (up to four dimers).  Call before
- * mpcx_co_setup.  No counterpart in the reference (the OpenMP loops of
- * evolve_mod.f95 run over all dimers in one address space). */
-int mpcx_co_set_clusters(mpcx_co* e, int32_t on);
+/* Near-engine layout for two or more dimers (call before mpcx_co_setup).
+ *  mode 0: one workgroup for the whole system (up to four dimers; the default
+ *          for one or two dimers);
+ *  mode 1: dimers whose near regions can meet within an interval are grouped
+ *          into clusters of at most four, one workgroup each;
+ *  mode 2: the system's near set split over cooperative workgroups, any number
+ *          of dimers at any density (the default above two dimers).
+ * In modes 1 and 2 the workgroups meet at a grid barrier every MD step to take
+ * the system-wide list-rebuild decision of evolve_mod.f95:457-465.  No
+ * counterpart in the reference (its OpenMP loops over dimers and particles in
+ * evolve_mod.f95 run in one address space). */
+int mpcx_co_set_near_mode(mpcx_co* e, int32_t mode);
 /* Collision-cell data of the last collision: (mpcnumcell) arrays, 0-based
  * cell of each particle, and unshifted census counts (3, vol). */
 int mpcx_co_get_cells(mpcx_co* e, double* vcm_uv /* (mpcnumcell, 8) */, int32_t* cell_of,

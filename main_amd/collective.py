@@ -73,18 +73,19 @@ class CollectiveEngine:
     """Device-resident collective_code system (one GPU)."""
 
     def __init__(self, p: dict, rng_mode: int = 0, seed: int = 4357787, sphere_guard: float = 0.3,
-                 clusters: bool | None = None):
-        """clusters: one near-engine workgroup per cluster of dimers (the
-        default above two dimers) or one for the whole system (False, up to
-        four dimers); see mpcx_co_set_clusters."""
+                 near_mode: int | None = None):
+        """near_mode (two or more dimers): 0 one near-engine workgroup for the
+        system (up to four dimers), 1 one per cluster of dimers, 2 the near
+        set over cooperative workgroups (default above two dimers); see
+        mpcx_co_set_near_mode."""
         require_gpu()
         self.p = p
         self.cfg = make_config(p, rng_mode, seed, sphere_guard)
         self.h = C.c_void_p()
         check(lib().mpcx_co_create(C.byref(self.h), C.byref(self.cfg)), "mpcx_co_create")
         self.rng_mode = rng_mode
-        if clusters is not None:
-            check(lib().mpcx_co_set_clusters(self.h, C.c_int32(1 if clusters else 0)), "mpcx_co_set_clusters")
+        if near_mode is not None:
+            check(lib().mpcx_co_set_near_mode(self.h, C.c_int32(int(near_mode))), "mpcx_co_set_near_mode")
 
     def close(self):
         if self.h:

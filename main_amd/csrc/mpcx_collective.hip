@@ -1386,6 +1386,777 @@ __global__ __launch_bounds__(1024) void clu_lists(CoD G, CluBase B) {
   }
 }
 
+// ------------------------------------------------------------- global near engine
+// Many dimers at any density: the near set of one system is split into
+// contiguous chunks (ascending particle index) over G cooperative
+// workgroups.  Every workgroup knows every sphere; a particle's list
+// memberships are up to four global list ids 2k+i (ascending, 0xff = none),
+// so the number of dimers is a run-time value.  Per MD step: each workgroup
+// advances its own inside entries and sums their pair forces per sphere in a
+// fixed order (per wave, then over waves); at a grid barrier workgroup 0 adds
+// the workgroups' sums in workgroup order, integrates every dimer (one thread
+// per sphere for the forces, one per dimer for RATTLE) and takes the rebuild
+// decision; a second barrier hands the new sphere positions and the decision
+// to all.  A rebuild compacts the inside list with a prefix over the
+// workgroups' counts, so lists keep the reference's ascending order.
+constexpr int GE_T = 512;
+constexpr int GE_W = GE_T / 64;
+constexpr int GE_LCAP = 768;    // inside entries of a workgroup staged in LDS
+constexpr int GE_GMAX = 128;    // workgroups of one launch
+struct GeSync {
+  unsigned count, gen;
+  int abort, pad;
+  int cnt[GE_GMAX];
+  unsigned long long vmax2[GE_GMAX];
+  // decision block written by workgroup 0
+  int rebuild, tc, tp, cstep, nev, pad2;
+};
+struct GeD {
+  GeSync* sy;
+  unsigned* memb_q;      // [near_cap] memberships of the near entries at the last rebuild
+  unsigned* in_memb4;    // [in_cap] memberships of the inside entries
+  double* in_fp4;        // [in_cap][4][3] pair forces by membership slot
+  double* in_pot4;       // [in_cap][4]
+  double* part;          // [GE_GMAX][4*MAXSG+1] per-workgroup sphere sums, max displacement
+  int in_cap, pad;
+};
+struct GeShared {
+  double lx[GE_LCAP], ly[GE_LCAP], lz[GE_LCAP], lvx[GE_LCAP], lvy[GE_LCAP], lvz[GE_LCAP], lmsd[GE_LCAP];
+  int lflag[GE_LCAP];
+  unsigned lmemb[GE_LCAP];
+  double dim[12][MAXSG];
+  double wacc[GE_W][4 * MAXSG];
+  double fin[4 * MAXSG];
+  double tpp[MAXSG];
+  int wsum[GE_W + 1];
+  int cstep, tc, tp, in_num, nev, rebuild;
+  int qb, qe, ib, ie;
+  double vmax2, red[GE_W];
+};
+__shared__ GeShared g_ge;
+
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, MPCX_WAVE));
+  return v;
+}
+
+// grid barrier of the G workgroups (thread 0); bounded like clu_exchange_max
+__device__ __attribute__((noinline)) void ge_barrier_t0(GeSync* sy, int G, int* err) {
+  if (G <= 1) return;
+  const unsigned gen = __hip_atomic_load(&sy->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  __threadfence();
+  if (atomicAdd(&sy->count, 1u) == (unsigned)G - 1u) {
+    atomicExch(&sy->count, 0u);
+    __threadfence();
+    atomicAdd(&sy->gen, 1u);
+  } else {
+    long long spins = 0;
+    while (__hip_atomic_load(&sy->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+      if (__hip_atomic_load(&sy->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || ++spins > SPIN_CAP) {
+        raise_error(err, ERR_BARRIER);
+        atomicExch(&sy->abort, 1);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+  }
+  __threadfence();
+}
+__device__ __forceinline__ void ge_sync(const GeD& E, int* err) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) ge_barrier_t0(E.sy, (int)gridDim.x, err);
+  __syncthreads();
+}
+
+__device__ __forceinline__ unsigned ge_memberships(double x, double y, double z, const CoP& P, int* err) {
+  unsigned ids = 0xffffffffu;
+  int m = 0;
+  const int nd = P.nd;
+  for (int k = 0; k < nd; k++)
+    for (int i = 0; i < 2; i++) {
+      const int s = k + i * nd;
+      double d1 = pbc_half(x - g_ge.dim[DX][s], P.box[0]);
+      double d2 = pbc_half(y - g_ge.dim[DY][s], P.box[1]);
+      double d3 = z - g_ge.dim[DZ][s];
+      if (d1 * d1 + d2 * d2 + d3 * d3 <= P.full2[i]) {
+        if (m == 4) { raise_error(err, ERR_MEMBERSHIPS); return ids; }
+        ids = (ids & ~(0xffu << (8 * m))) | ((unsigned)(2 * k + i) << (8 * m));
+        m++;
+      }
+    }
+  return ids;
+}
+
+__device__ __forceinline__ void ge_slot_load(const CoD& D, int k, double& x, double& y, double& z, double& vx,
+                                             double& vy, double& vz, int& fl, double& ms) {
+  const int kl = k - g_ge.ib;
+  if (kl < GE_LCAP) {
+    x = g_ge.lx[kl]; y = g_ge.ly[kl]; z = g_ge.lz[kl]; vx = g_ge.lvx[kl]; vy = g_ge.lvy[kl]; vz = g_ge.lvz[kl];
+    fl = g_ge.lflag[kl]; ms = g_ge.lmsd[kl];
+  } else {
+    x = D.sl_x[k]; y = D.sl_y[k]; z = D.sl_z[k]; vx = D.sl_vx[k]; vy = D.sl_vy[k]; vz = D.sl_vz[k];
+    fl = D.sl_flag[k]; ms = D.in_msd[k];
+  }
+}
+__device__ __forceinline__ void ge_slot_store(const CoD& D, int k, double x, double y, double z, double vx,
+                                              double vy, double vz, int fl, double ms) {
+  const int kl = k - g_ge.ib;
+  if (kl < GE_LCAP) {
+    g_ge.lx[kl] = x; g_ge.ly[kl] = y; g_ge.lz[kl] = z; g_ge.lvx[kl] = vx; g_ge.lvy[kl] = vy; g_ge.lvz[kl] = vz;
+    g_ge.lflag[kl] = fl; g_ge.lmsd[kl] = ms;
+  } else {
+    D.sl_x[k] = x; D.sl_y[k] = y; D.sl_z[k] = z; D.sl_vx[k] = vx; D.sl_vy[k] = vy; D.sl_vz[k] = vz;
+    D.sl_flag[k] = fl; D.in_msd[k] = ms;
+  }
+}
+// near slot q -> entry k of this workgroup
+__device__ __forceinline__ void ge_stage(const CoD& D, int k, int q, double ms) {
+  ge_slot_store(D, k, D.ns_x[q], D.ns_y[q], D.ns_z[q], D.ns_vx[q], D.ns_vy[q], D.ns_vz[q], D.ns_flag[q], ms);
+}
+// this workgroup's entries -> their near slots
+__device__ __forceinline__ void ge_unstage(const CoD& D) {
+  for (int k = g_ge.ib + threadIdx.x; k < g_ge.ie; k += GE_T) {
+    double x, y, z, vx, vy, vz, ms;
+    int fl;
+    ge_slot_load(D, k, x, y, z, vx, vy, vz, fl, ms);
+    const int q = D.in_list[k];
+    D.ns_x[q] = x; D.ns_y[q] = y; D.ns_z[q] = z; D.ns_vx[q] = vx; D.ns_vy[q] = vy; D.ns_vz[q] = vz;
+    D.ns_flag[q] = fl;
+    if (k - g_ge.ib < GE_LCAP) D.in_msd[k] = ms;
+  }
+}
+
+// exclusive prefix of a per-workgroup count over the grid (after a barrier)
+__device__ __forceinline__ void ge_publish_count(const GeD& E, int c, double vm) {
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(&E.sy->cnt[blockIdx.x], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&E.sy->vmax2[blockIdx.x], (unsigned long long)__double_as_longlong(vm), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+__device__ __forceinline__ void ge_read_counts(const GeD& E, int& before, int& total, double& vmax) {
+  before = 0; total = 0; vmax = 0.0;
+  for (int w = 0; w < (int)gridDim.x; w++) {
+    const int c = __hip_atomic_load(&E.sy->cnt[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (w < (int)blockIdx.x) before += c;
+    total += c;
+    vmax = fmax(vmax, __longlong_as_double((long long)__hip_atomic_load(&E.sy->vmax2[w], __ATOMIC_RELAXED,
+                                                                      __HIP_MEMORY_SCOPE_AGENT)));
+  }
+}
+
+// ordered compaction of this workgroup's near entries with pred(q) into
+// entries [base, ...): wave w owns [qb + w*64*E, ...), lanes interleaved
+template <typename Pred, typename Emit>
+__device__ __forceinline__ void ge_compact_chunk(int base, Pred pred, Emit emit) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int n = g_ge.qe - g_ge.qb;
+  const int E = (n + GE_T - 1) / GE_T;
+  const int q0 = g_ge.qb + wv * 64 * E + lane;
+  int cnt = 0;
+  for (int j = 0; j < E; j++) {
+    const int q = q0 + j * 64;
+    cnt += __popcll(__ballot(q < g_ge.qe && pred(q)));
+  }
+  if (lane == 0) g_ge.wsum[wv] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int w = 0; w < GE_W; w++) { int t = g_ge.wsum[w]; g_ge.wsum[w] = run; run += t; }
+    g_ge.wsum[GE_W] = run;
+  }
+  __syncthreads();
+  int b = base + g_ge.wsum[wv];
+  const unsigned long long lt = lanes_below();
+  for (int j = 0; j < E; j++) {
+    const int q = q0 + j * 64;
+    const bool in = q < g_ge.qe && pred(q);
+    const unsigned long long bl = __ballot(in);
+    const int k = b + __popcll(bl & lt);
+    b += __popcll(bl);
+    if (in) emit(q, k);
+  }
+  __syncthreads();
+}
+__device__ __forceinline__ int ge_count_chunk_pred_total() { return g_ge.wsum[GE_W]; }
+
+// Rebuild over this workgroup's chunk: lazy streaming of its near particles
+// outside the lists (when do_stream), memberships, then the ordered
+// compaction of all workgroups' members into the inside list.
+__device__ __attribute__((noinline)) void ge_rebuild(const CoP& P, const CoD& D, const GeD& E, bool store_first,
+                                                     bool do_stream, bool compact) {
+  if (store_first) {
+    ge_unstage(D);
+    __threadfence_block();
+    __syncthreads();
+  }
+  const double tc = (double)g_ge.tc;
+  double vm = 0.0;
+  int* err = &D.sc->err;
+  for (int q = g_ge.qb + threadIdx.x; q < g_ge.qe; q += GE_T) {
+    double x = D.ns_x[q], y = D.ns_y[q], z = D.ns_z[q], vx = D.ns_vx[q], vy = D.ns_vy[q], vz = D.ns_vz[q];
+    int f = D.ns_flag[q];
+    if (do_stream && !(D.ns_stat[q] & ST_INSIDE)) {
+      const int f0 = f;
+      bool b = stream_particle(x, y, z, vx, vy, vz, f, tc, P, D.react_box, err);
+      D.ns_x[q] = x; D.ns_y[q] = y; D.ns_z[q] = z;
+      if (b) { D.ns_vx[q] = vx; D.ns_vy[q] = vy; D.ns_vz[q] = vz; }
+      if (f != f0) D.ns_flag[q] = f;
+    }
+    vm = fmax(vm, (vx * vx + vy * vy) + vz * vz);
+    if (compact) E.memb_q[q] = ge_memberships(x, y, z, P, err);
+  }
+  vm = wave_max(vm);
+  if ((threadIdx.x & 63) == 0) g_ge.red[threadIdx.x >> 6] = vm;
+  __threadfence_block();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int w = 0; w < GE_W; w++) m = fmax(m, g_ge.red[w]);
+    if (do_stream) g_ge.vmax2 = m;
+  }
+  if (!compact) {   // stopping for a sort: the system-wide max v^2 is kept for the resumed rebuild
+    __syncthreads();
+    if (threadIdx.x == 0) ge_publish_count(E, 0, g_ge.vmax2);
+    ge_sync(E, err);
+    int before, total;
+    double vmax;
+    ge_read_counts(E, before, total, vmax);
+    if (threadIdx.x == 0) g_ge.vmax2 = vmax;
+    __syncthreads();
+    return;
+  }
+  // members of this chunk, then their place in the system's inside list
+  const auto member = [&](int q) { return E.memb_q[q] != 0xffffffffu; };
+  int cnt = 0;
+  for (int q = g_ge.qb + threadIdx.x; q < g_ge.qe; q += GE_T) cnt += member(q) ? 1 : 0;
+  cnt = wave_sum_i(cnt);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) g_ge.wsum[threadIdx.x >> 6] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < GE_W; w++) t += g_ge.wsum[w];
+    ge_publish_count(E, t, g_ge.vmax2);
+  }
+  ge_sync(E, err);
+  int before, total;
+  double vmax;
+  ge_read_counts(E, before, total, vmax);
+  if (total > E.in_cap) {
+    if (threadIdx.x == 0) raise_error(err, ERR_INSIDE_CAP);
+    total = E.in_cap;
+  }
+  if (threadIdx.x == 0) {
+    g_ge.ib = min(before, E.in_cap);
+    g_ge.in_num = total;
+    g_ge.vmax2 = vmax;
+    double fm = __longlong_as_double((long long)D.sc->far_maxv2);
+    double maxt = fmax(fm, vmax);
+    g_ge.tp = (int)floor(((0.5 * P.skinrad) / sqrt(maxt)) / P.MDt);
+    g_ge.tc = 0;
+  }
+  __syncthreads();
+  ge_compact_chunk(g_ge.ib, member, [&](int q, int k) {
+    if (k < E.in_cap) {
+      D.in_list[k] = q;
+      E.in_memb4[k] = E.memb_q[q];
+      if (k - g_ge.ib < GE_LCAP) g_ge.lmemb[k - g_ge.ib] = E.memb_q[q];
+      ge_stage(D, k, q, 0.0);
+    }
+  });
+  for (int q = g_ge.qb + threadIdx.x; q < g_ge.qe; q += GE_T) D.ns_stat[q] = member(q) ? ST_INSIDE : 0;
+  if (threadIdx.x == 0) g_ge.ie = min(g_ge.ib + g_ge.wsum[GE_W], E.in_cap);
+  __threadfence_block();
+  __syncthreads();
+}
+
+// Interval start: the inside list (unchanged since the last rebuild) as near
+// indices of this interval's set; entry k keeps its memberships, forces and
+// max_solvent_dist in the system's entry arrays.
+__device__ __attribute__((noinline)) void ge_map_inside(const CoD& D, const GeD& E) {
+  const auto ins = [&](int q) { return (D.ns_stat[q] & ST_INSIDE) != 0; };
+  int cnt = 0;
+  for (int q = g_ge.qb + threadIdx.x; q < g_ge.qe; q += GE_T) cnt += ins(q) ? 1 : 0;
+  cnt = wave_sum_i(cnt);
+  if ((threadIdx.x & 63) == 0) g_ge.wsum[threadIdx.x >> 6] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < GE_W; w++) t += g_ge.wsum[w];
+    ge_publish_count(E, t, 0.0);
+  }
+  ge_sync(E, &D.sc->err);
+  int before, total;
+  double vm;
+  ge_read_counts(E, before, total, vm);
+  if (threadIdx.x == 0) {
+    if (total != g_ge.in_num) raise_error(&D.sc->err, ERR_INSIDE_LOST);
+    g_ge.ib = min(before, E.in_cap);
+  }
+  __syncthreads();
+  ge_compact_chunk(g_ge.ib, ins, [&](int q, int k) {
+    if (k < E.in_cap) {
+      D.in_list[k] = q;
+      const int kl = k - g_ge.ib;
+      ge_stage(D, k, q, kl < GE_LCAP ? D.in_msd[k] : D.in_msd[k]);
+      if (kl < GE_LCAP) g_ge.lmemb[kl] = E.in_memb4[k];
+    }
+  });
+  if (threadIdx.x == 0) g_ge.ie = min(g_ge.ib + g_ge.wsum[GE_W], E.in_cap);
+  __syncthreads();
+}
+
+__device__ __forceinline__ unsigned ge_memb_of(const GeD& E, int k) {
+  const int kl = k - g_ge.ib;
+  return kl < GE_LCAP ? g_ge.lmemb[kl] : E.in_memb4[k];
+}
+
+// kick with up to four pair forces (list order)
+__device__ __forceinline__ void ge_kick(const CoP& P, int fl, unsigned ids, const double* fx, const double* fy,
+                                        const double* fz, double& vx, double& vy, double& vz) {
+  bool ok;
+  const double h = kick_factor(P, fl, ok);
+  if (!ok) return;
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    if (((ids >> (8 * m)) & 0xffu) == 0xffu) break;
+    vx = vx + h * fx[m];
+    vy = vy + h * fy[m];
+    vz = vz + h * fz[m];
+  }
+}
+
+// Particle phase of this workgroup's entries (forces_mod.f95:62-136, kick,
+// and unless the last step the next kick and streaming); per-sphere sums
+// into wacc by wave in a fixed order.
+__device__ __attribute__((noinline)) double ge_particles(const CoP& P, const CoD& D, const GeD& E, bool last) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int nd = P.nd;
+  double mx = 0.0;
+  const int ib = g_ge.ib, ie = g_ge.ie;
+  for (int k0 = ib + wv * 64; k0 < ie; k0 += GE_T) {
+    const int k = k0 + lane;
+    const bool valid = k < ie;
+    double fx[4] = {0, 0, 0, 0}, fy[4] = {0, 0, 0, 0}, fz[4] = {0, 0, 0, 0}, pot[4] = {0, 0, 0, 0};
+    unsigned ids = 0xffffffffu;
+    if (valid) {
+      double x, y, z, vx, vy, vz, ms;
+      int fl;
+      ge_slot_load(D, k, x, y, z, vx, vy, vz, fl, ms);
+      ids = ge_memb_of(E, k);
+#pragma unroll
+      for (int m = 0; m < 4; m++) {
+        const unsigned b = (ids >> (8 * m)) & 0xffu;
+        if (b == 0xffu) break;
+        const int kk = (int)(b >> 1), ii = (int)(b & 1u), s = kk + ii * nd;
+        double d1 = pbc_half(x - g_ge.dim[DX][s], P.box[0]);
+        double d2 = pbc_half(y - g_ge.dim[DY][s], P.box[1]);
+        double d3 = z - g_ge.dim[DZ][s];
+        double dRsq = d1 * d1 + d2 * d2 + d3 * d3;
+        if (dRsq <= P.cut2[ii]) {
+          double En = 0.0;
+          bool on = false;
+          if (fl == 0) {
+            if (ii == 0) { fl = 1; En = P.Eb[0]; } else En = P.Ea[1];
+            on = true;
+          } else if (fl == 1) { En = P.Eb[ii]; on = true; }
+          else if (fl == 2) { En = P.Es[ii]; on = true; }
+          if (on) {
+            double F, Ub;
+            lj_terms(dRsq, P.R6[ii], P.R12[ii], P.r12R12[ii], P.r6R6[ii], F, Ub);
+            double EF = En * F;
+            fx[m] = EF * d1; fy[m] = EF * d2; fz[m] = EF * d3; pot[m] = En * Ub;
+          }
+        }
+      }
+      ge_kick(P, fl, ids, fx, fy, fz, vx, vy, vz);
+      if (!last) {
+        ge_kick(P, fl, ids, fx, fy, fz, vx, vy, vz);
+        stream_particle(x, y, z, vx, vy, vz, fl, 1.0, P, D.react_box, &D.sc->err);
+        double v2 = (vx * vx + vy * vy) + vz * vz;
+        ms = ms + v2 * (P.MDt * P.MDt);
+        mx = fmax(mx, ms);
+      } else {
+        for (int m = 0; m < 4; m++) {
+          E.in_fp4[(size_t)k * 12 + m * 3 + 0] = fx[m];
+          E.in_fp4[(size_t)k * 12 + m * 3 + 1] = fy[m];
+          E.in_fp4[(size_t)k * 12 + m * 3 + 2] = fz[m];
+          E.in_pot4[(size_t)k * 4 + m] = pot[m];
+        }
+      }
+      ge_slot_store(D, k, x, y, z, vx, vy, vz, fl, ms);
+    }
+    // per-sphere sums of this wave's 64 entries, lists in ascending order
+    int ptr = 0;
+    for (;;) {
+      const unsigned b = ptr < 4 ? (ids >> (8 * ptr)) & 0xffu : 0xffu;
+      const int key = b == 0xffu ? 0x7fffffff : (int)b;
+      const int bmin = wave_min_i(key);
+      if (bmin == 0x7fffffff) break;
+      const int smin = (bmin >> 1) + (bmin & 1) * nd;
+      const bool mine = key == bmin;
+      double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
+      if (mine) {
+#pragma unroll
+        for (int m = 0; m < 4; m++)
+          if (m == ptr) { c0 = fx[m]; c1 = fy[m]; c2 = fz[m]; c3 = pot[m]; }
+      }
+      c0 = wave_sum(c0); c1 = wave_sum(c1); c2 = wave_sum(c2); c3 = wave_sum(c3);
+      if (lane == 0) {
+        double* a = &g_ge.wacc[wv][4 * smin];
+        a[0] = a[0] + c0; a[1] = a[1] + c1; a[2] = a[2] + c2; a[3] = a[3] + c3;
+      }
+      if (mine) ptr++;
+    }
+  }
+  return mx;
+}
+
+// thread s < 2 nd of workgroup 0: forces on sphere s (evolve_mod.f95:605-636,
+// the order of dimer_forces_and_rattle_v)
+__device__ __forceinline__ void ge_sphere_force(const CoP& P, int s) {
+  const int nd = P.nd, k = s < nd ? s : s - nd, i = s < nd ? 0 : 1;
+  double fw, wp;
+  lj_mot_wall(P, g_ge.dim[DZ][s], i, fw, wp);
+  double fx = 0.0 - g_ge.fin[4 * s + 0], fy = 0.0 - g_ge.fin[4 * s + 1], fz = 0.0 - g_ge.fin[4 * s + 2];
+  double tp = 0.0 + g_ge.fin[4 * s + 3];
+  for (int j = 0; j < nd; j++)
+    for (int l = 0; l < 2; l++) {
+      if (j == k) continue;
+      const int b = j + l * nd;
+      const int kind = (i == 0 && l == 0) ? 0 : (i == 1 && l == 1) ? 2 : 1;
+      double mx, my, mz, mp;
+      lj_mot_mot(P, g_ge.dim[DX][s], g_ge.dim[DY][s], g_ge.dim[DZ][s], g_ge.dim[DX][b], g_ge.dim[DY][b],
+                 g_ge.dim[DZ][b], kind, mx, my, mz, mp);
+      fx = fx + mx; fy = fy + my; fz = fz + mz;
+      if (j < k) tp = tp + mp;
+    }
+  fz = fz + fw;
+  tp = tp + wp;
+  g_ge.dim[DFX][s] = fx; g_ge.dim[DFY][s] = fy; g_ge.dim[DFZ][s] = fz;
+  g_ge.tpp[s] = tp;
+}
+// thread k < nd of workgroup 0: one dimer through rattle_positions / the
+// velocity RATTLE of dimer_forces_and_rattle_v (the same arithmetic)
+__device__ __forceinline__ void ge_dimer_load(Dimer<1>& d, int k, int nd) {
+  for (int f = 0; f < 12; f++) { d.v[f][0] = g_ge.dim[f][k]; d.v[f][1] = g_ge.dim[f][k + nd]; }
+}
+__device__ __forceinline__ void ge_dimer_store(const Dimer<1>& d, int k, int nd) {
+  for (int f = 0; f < 12; f++) { g_ge.dim[f][k] = d.v[f][0]; g_ge.dim[f][k + nd] = d.v[f][1]; }
+}
+__device__ __forceinline__ void ge_rattle_v(const CoP& P, Dimer<1>& d, int* err) {
+  const int s0 = 0, s1 = 1;
+  d.v[DVX][s0] = d.v[DQX][s0] + P.hc * d.v[DFX][s0];
+  d.v[DVY][s0] = d.v[DQY][s0] + P.hc * d.v[DFY][s0];
+  d.v[DVZ][s0] = d.v[DQZ][s0] + P.hc * d.v[DFZ][s0];
+  d.v[DVX][s1] = d.v[DQX][s1] + P.hn * d.v[DFX][s1];
+  d.v[DVY][s1] = d.v[DQY][s1] + P.hn * d.v[DFY][s1];
+  d.v[DVZ][s1] = d.v[DQZ][s1] + P.hn * d.v[DFZ][s1];
+  for (long long acc = 0;; acc++) {
+    if (acc == RATTLE_CAP) { raise_error(err, ERR_RATTLE); break; }
+    double dvx = d.v[DVX][s0] - d.v[DVX][s1], dvy = d.v[DVY][s0] - d.v[DVY][s1], dvz = d.v[DVZ][s0] - d.v[DVZ][s1];
+    double d0 = pbc_half(d.v[DX][s0] - d.v[DX][s1], P.box[0]);
+    double d1 = pbc_half(d.v[DY][s0] - d.v[DY][s1], P.box[1]);
+    double d2 = d.v[DZ][s0] - d.v[DZ][s1];
+    double c2 = 2.0 * (dvx * d0 + dvy * d1 + dvz * d2);
+    if (fabs(c2) < 1e-8) break;
+    double O = -((0.25 * (P.IIMcIMn * c2)) / P.Dcn2);
+    double o2 = 2.0 * O;
+    d.v[DVX][s0] = d.v[DVX][s0] + (o2 * d0) / P.Mc;
+    d.v[DVY][s0] = d.v[DVY][s0] + (o2 * d1) / P.Mc;
+    d.v[DVZ][s0] = d.v[DVZ][s0] + (o2 * d2) / P.Mc;
+    d.v[DVX][s1] = d.v[DVX][s1] - (o2 * d0) / P.Mn;
+    d.v[DVY][s1] = d.v[DVY][s1] - (o2 * d1) / P.Mn;
+    d.v[DVZ][s1] = d.v[DVZ][s1] - (o2 * d2) / P.Mn;
+  }
+}
+
+// workgroup 0: the dimers at the end of step t (forces, velocity RATTLE) and,
+// unless last, the start of step t+1 (positions RATTLE, rebuild decision)
+__device__ __attribute__((noinline)) void ge_dimers(const CoP& P, const CoD& D, const GeD& E, bool end_step,
+                                                    bool begin_step, double maxv) {
+  const int nd = P.nd, tid = threadIdx.x;
+  int* err = &D.sc->err;
+  if (end_step) {
+    for (int s = tid; s < 2 * nd; s += GE_T) ge_sphere_force(P, s);
+    __syncthreads();
+    if (tid == 0) {
+      double tpot = 0.0;
+      for (int k = 0; k < nd; k++)
+        for (int i = 0; i < 2; i++) tpot = tpot + g_ge.tpp[k + i * nd];
+      D.sc->tpotential = tpot;
+    }
+    for (int k = tid; k < nd; k += GE_T) {
+      Dimer<1> d;
+      ge_dimer_load(d, k, nd);
+      ge_rattle_v(P, d, err);
+      ge_dimer_store(d, k, nd);
+    }
+    __syncthreads();
+  }
+  if (!begin_step) return;
+  if (tid == 0) { g_ge.cstep += 1; g_ge.tc += 1; }
+  for (int k = tid; k < nd; k += GE_T) {
+    Dimer<1> d;
+    ge_dimer_load(d, k, nd);
+    rattle_positions<1>(P, d, err);
+    ge_dimer_store(d, k, nd);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    const int rb = (maxv > P.hs2 || g_ge.tc == g_ge.tp || g_ge.cstep % P.MPCtstep == 0);
+    g_ge.rebuild = rb;
+    if (rb) {
+      if (g_ge.nev < MAXEV) D.sc->events[g_ge.nev] = g_ge.tc; else raise_error(err, ERR_EVENTS);
+      g_ge.nev += 1;
+    }
+  }
+  __syncthreads();
+  if (g_ge.rebuild)
+    for (int s = tid; s < 2 * nd; s += GE_T) {
+      double a = pbc_half(g_ge.dim[DX][s] - D.sc->x0[s], P.box[0]);
+      double b = pbc_half(g_ge.dim[DY][s] - D.sc->y0[s], P.box[1]);
+      double c = g_ge.dim[DZ][s] - D.sc->z0[s];
+      if (a * a + b * b + c * c > P.guard * P.guard) raise_error(err, ERR_NEAR_GUARD);
+    }
+}
+
+// workgroup 0 publishes the dimer state and the decision; the others read them
+__device__ __forceinline__ void ge_publish(const CoP& P, const CoD& D, const GeD& E) {
+  const int nd = P.nd;
+  if (blockIdx.x == 0) {
+    for (int j = threadIdx.x; j < 12 * 2 * nd; j += GE_T) {
+      const int f = j / (2 * nd), s = j % (2 * nd);
+      D.dim[f * MAXSG + s] = g_ge.dim[f][s];
+    }
+    if (threadIdx.x == 0) {
+      GeSync* y = E.sy;
+      y->rebuild = g_ge.rebuild; y->tc = g_ge.tc; y->tp = g_ge.tp; y->cstep = g_ge.cstep; y->nev = g_ge.nev;
+    }
+  }
+  ge_sync(E, &D.sc->err);
+  if (blockIdx.x != 0) {
+    for (int j = threadIdx.x; j < 12 * 2 * nd; j += GE_T) {
+      const int f = j / (2 * nd), s = j % (2 * nd);
+      g_ge.dim[f][s] = __hip_atomic_load(&D.dim[f * MAXSG + s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (threadIdx.x == 0) {
+      const GeSync* y = E.sy;
+      g_ge.rebuild = __hip_atomic_load(&y->rebuild, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      g_ge.tc = __hip_atomic_load(&y->tc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      g_ge.tp = __hip_atomic_load(&y->tp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      g_ge.cstep = __hip_atomic_load(&y->cstep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      g_ge.nev = __hip_atomic_load(&y->nev, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+  __syncthreads();
+}
+
+// per-workgroup sums of the wave accumulators into part[wg] (and the max
+// displacement), then workgroup 0 adds the workgroups' sums in order
+__device__ __forceinline__ double ge_reduce(const CoP& P, const GeD& E, double mx) {
+  const int nd = P.nd, nv = 4 * 2 * nd;
+  mx = wave_max(mx);
+  if ((threadIdx.x & 63) == 0) g_ge.red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  double* part = E.part + (size_t)blockIdx.x * (4 * MAXSG + 1);
+  for (int v = threadIdx.x; v < nv; v += GE_T) {
+    double a = g_ge.wacc[0][v];
+    g_ge.wacc[0][v] = 0.0;
+    for (int w = 1; w < GE_W; w++) { a = a + g_ge.wacc[w][v]; g_ge.wacc[w][v] = 0.0; }
+    part[v] = a;
+  }
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int w = 0; w < GE_W; w++) m = fmax(m, g_ge.red[w]);
+    part[4 * MAXSG] = m;
+  }
+  return 0.0;
+}
+
+__device__ __forceinline__ double ge_gather_sums(const CoP& P, const GeD& E) {
+  // workgroup 0, after the barrier
+  const int nd = P.nd, nv = 4 * 2 * nd, G = (int)gridDim.x;
+  for (int v = threadIdx.x; v < nv; v += GE_T) {
+    double a = 0.0;
+    for (int w = 0; w < G; w++)
+      a = a + __longlong_as_double((long long)__hip_atomic_load(
+                  (unsigned long long*)&E.part[(size_t)w * (4 * MAXSG + 1) + v], __ATOMIC_RELAXED,
+                  __HIP_MEMORY_SCOPE_AGENT));
+    g_ge.fin[v] = a;
+  }
+  double m = 0.0;
+  for (int w = 0; w < G; w++)
+    m = fmax(m, __longlong_as_double((long long)__hip_atomic_load(
+                    (unsigned long long*)&E.part[(size_t)w * (4 * MAXSG + 1) + 4 * MAXSG], __ATOMIC_RELAXED,
+                    __HIP_MEMORY_SCOPE_AGENT)));
+  __syncthreads();
+  return m;
+}
+
+__global__ __launch_bounds__(GE_T) void co_ge_engine(CoP P, CoD D, GeD E, int nsteps, int mode) {
+  const int tid = threadIdx.x, nd = P.nd, G = (int)gridDim.x;
+  CoScalars* sc = D.sc;
+  int* err = &sc->err;
+  for (int j = tid; j < 12 * 2 * nd; j += GE_T) {
+    const int f = j / (2 * nd), s = j % (2 * nd);
+    g_ge.dim[f][s] = D.dim[f * MAXSG + s];
+  }
+  for (int j = tid; j < GE_W * 4 * MAXSG; j += GE_T) g_ge.wacc[j / (4 * MAXSG)][j % (4 * MAXSG)] = 0.0;
+  const int nn = min(sc->near_num, D.near_cap);
+  const int chunk = (nn + G - 1) / G;
+  if (tid == 0) {
+    g_ge.cstep = sc->cstep; g_ge.tc = sc->time_count; g_ge.tp = sc->time_penetrate; g_ge.in_num = sc->in_num;
+    g_ge.nev = (mode & M_RESUME) ? sc->n_ev : 0;
+    g_ge.vmax2 = sc->near_vmax2;
+    g_ge.rebuild = 0;
+    g_ge.qb = min(nn, (int)blockIdx.x * chunk);
+    g_ge.qe = min(nn, g_ge.qb + chunk);
+    g_ge.ib = 0; g_ge.ie = 0;
+  }
+  __syncthreads();
+  // near particles of this chunk into their slots
+  for (int q = g_ge.qb + tid; q < g_ge.qe; q += GE_T) {
+    const int p = D.near_list[q];
+    D.ns_x[q] = D.x[p]; D.ns_y[q] = D.y[p]; D.ns_z[q] = D.z[p];
+    D.ns_vx[q] = D.vx[p]; D.ns_vy[q] = D.vy[p]; D.ns_vz[q] = D.vz[p];
+    D.ns_flag[q] = D.flag[p]; D.ns_stat[q] = D.status[p];
+  }
+  __threadfence_block();
+  __syncthreads();
+  const auto scatter_chunk = [&]() {
+    for (int q = g_ge.qb + tid; q < g_ge.qe; q += GE_T) {
+      const int p = D.near_list[q];
+      D.x[p] = D.ns_x[q]; D.y[p] = D.ns_y[q]; D.z[p] = D.ns_z[q];
+      D.vx[p] = D.ns_vx[q]; D.vy[p] = D.ns_vy[q]; D.vz[p] = D.ns_vz[q];
+      D.flag[p] = D.ns_flag[q]; D.status[p] = D.ns_stat[q];
+    }
+  };
+  const auto save_scalars = [&](bool with_x0) {
+    if (blockIdx.x == 0 && tid == 0) {
+      sc->cstep = g_ge.cstep; sc->time_count = g_ge.tc; sc->time_penetrate = g_ge.tp; sc->in_num = g_ge.in_num;
+      sc->n_ev = g_ge.nev; sc->near_vmax2 = g_ge.vmax2;
+    }
+    if (blockIdx.x == 0) {
+      for (int j = tid; j < 12 * 2 * nd; j += GE_T) {
+        const int f = j / (2 * nd), s = j % (2 * nd);
+        D.dim[f * MAXSG + s] = g_ge.dim[f][s];
+      }
+      if (with_x0)
+        for (int s = tid; s < 2 * nd; s += GE_T) { sc->x0[s] = g_ge.dim[DX][s]; sc->y0[s] = g_ge.dim[DY][s]; sc->z0[s] = g_ge.dim[DZ][s]; }
+    }
+  };
+
+  if (mode & M_SETUP) {
+    ge_rebuild(P, D, E, false, false, true);
+    for (int k = g_ge.ib + tid; k < g_ge.ie; k += GE_T) {
+      D.in_plist[k] = D.near_list[D.in_list[k]];
+      for (int j = 0; j < 12; j++) E.in_fp4[(size_t)k * 12 + j] = 0.0;
+      for (int j = 0; j < 4; j++) E.in_pot4[(size_t)k * 4 + j] = 0.0;
+      D.in_msd[k] = 0.0;
+    }
+    __threadfence_block();
+    __syncthreads();
+    scatter_chunk();
+    if (tid == 0 && blockIdx.x == 0) sc->n_ev = 0;
+    save_scalars(true);
+    if (blockIdx.x == 0 && tid == 0) sc->n_ev = 0;
+    return;
+  }
+
+  const bool resume = (mode & M_RESUME) != 0;
+  if (!resume) {
+    ge_map_inside(D, E);
+    double mx = 0.0;
+    for (int k = g_ge.ib + tid; k < g_ge.ie; k += GE_T) {
+      double x, y, z, vx, vy, vz, ms;
+      int fl;
+      ge_slot_load(D, k, x, y, z, vx, vy, vz, fl, ms);
+      const unsigned ids = ge_memb_of(E, k);
+      double fx[4], fy[4], fz[4];
+      for (int m = 0; m < 4; m++) {
+        fx[m] = E.in_fp4[(size_t)k * 12 + m * 3 + 0];
+        fy[m] = E.in_fp4[(size_t)k * 12 + m * 3 + 1];
+        fz[m] = E.in_fp4[(size_t)k * 12 + m * 3 + 2];
+      }
+      ge_kick(P, fl, ids, fx, fy, fz, vx, vy, vz);
+      stream_particle(x, y, z, vx, vy, vz, fl, 1.0, P, D.react_box, err);
+      double v2 = (vx * vx + vy * vy) + vz * vz;
+      ms = ms + v2 * (P.MDt * P.MDt);
+      mx = fmax(mx, ms);
+      ge_slot_store(D, k, x, y, z, vx, vy, vz, fl, ms);
+    }
+    ge_reduce(P, E, mx);
+    ge_sync(E, err);
+    if (blockIdx.x == 0) {
+      const double m = ge_gather_sums(P, E);
+      ge_dimers(P, D, E, false, true, m);
+    }
+    ge_publish(P, D, E);
+  } else {
+    if (tid == 0) g_ge.rebuild = 1;
+    __syncthreads();
+  }
+
+  for (int t = 1; t <= nsteps; t++) {
+    const bool last = (t == nsteps);
+    if (g_ge.rebuild) {
+      if (!(resume && last)) {
+        const bool stop = (mode & M_STOP) && last;
+        ge_rebuild(P, D, E, true, true, !stop);
+        if (stop) {
+          __threadfence_block();
+          __syncthreads();
+          scatter_chunk();
+          save_scalars(false);
+          return;
+        }
+      } else {
+        ge_rebuild(P, D, E, false, false, true);
+      }
+    }
+    const double mx = ge_particles(P, D, E, last);
+    __syncthreads();
+    ge_reduce(P, E, mx);
+    ge_sync(E, err);
+    if (blockIdx.x == 0) {
+      const double m = ge_gather_sums(P, E);
+      ge_dimers(P, D, E, true, !last, m);
+    }
+    if (!last) ge_publish(P, D, E);
+  }
+
+  // end of the interval: census exclusion, inside list as particle indices,
+  // entries back into their slots and the particle arrays
+  for (int k = g_ge.ib + tid; k < g_ge.ie; k += GE_T) {
+    double x, y, z, vx, vy, vz, ms;
+    int fl;
+    ge_slot_load(D, k, x, y, z, vx, vy, vz, fl, ms);
+    const unsigned ids = ge_memb_of(E, k);
+    bool ex = false;
+    for (int m = 0; m < 4; m++) {
+      const unsigned b = (ids >> (8 * m)) & 0xffu;
+      if (b == 0xffu) break;
+      const int kk = (int)(b >> 1), ii = (int)(b & 1u), s = kk + ii * nd;
+      double d1 = pbc_half(x - g_ge.dim[DX][s], P.box[0]);
+      double d2 = pbc_half(y - g_ge.dim[DY][s], P.box[1]);
+      double d3 = z - g_ge.dim[DZ][s];
+      if (d1 * d1 + d2 * d2 + d3 * d3 <= P.cut2[ii]) ex = true;
+    }
+    const int q = D.in_list[k];
+    D.ns_stat[q] = ST_INSIDE | (ex ? ST_EXCL : 0);
+    D.in_plist[k] = D.near_list[q];
+  }
+  ge_unstage(D);
+  __threadfence_block();
+  __syncthreads();
+  scatter_chunk();
+  save_scalars(true);
+}
+
 // after the clusters' launch: their sphere fields and scalars back into the
 // system's (all clusters agree on the step counters and the rebuild events)
 __global__ __launch_bounds__(256) void clu_finish(CoP P, CoD G, CluBase B, int mode) {
@@ -1886,9 +2657,15 @@ struct mpcx_co {
   long long nwords = 0;
   int nd = 1;
   std::vector<int> mpc_pos, mpc_cell;   // host copies of the collision-cell tour
-  // clusters (two or more dimers): one near-engine workgroup per cluster
+  // near engine with two or more dimers: 0 one workgroup (up to four
+  // dimers), 1 one workgroup per cluster of dimers, 2 the system's near set
+  // over G cooperative workgroups
+  int near_mode = 0;
   bool clustered = false, can_cluster = false;
   CluBase cb;
+  CoD Dg;          // the system's descriptor with the global engine's entry arrays
+  GeD E;
+  int geG = 1;
 };
 
 template <typename T>
@@ -2009,7 +2786,8 @@ extern "C" int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg) {
   D.react_box = dalloc<int>(e, std::max(1, P.nrb));
   D.in_cap = NE_CAP;
   e->can_cluster = e->nd >= 2;
-  e->clustered = e->nd > 2;
+  e->near_mode = e->nd > 2 ? 2 : 0;
+  e->clustered = false;
   // the system's near list holds every cluster's near set
   const long long nsq = std::min<long long>(NT, NE_NEAR);
   D.near_cap = (int)std::min<long long>(NT, (long long)NE_NEAR * (e->can_cluster ? e->nd : 1));
@@ -2054,6 +2832,29 @@ extern "C" int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg) {
   D.sl_x = dalloc<double>(e, D.in_cap); D.sl_y = dalloc<double>(e, D.in_cap); D.sl_z = dalloc<double>(e, D.in_cap);
   D.sl_vx = dalloc<double>(e, D.in_cap); D.sl_vy = dalloc<double>(e, D.in_cap); D.sl_vz = dalloc<double>(e, D.in_cap);
   D.sl_flag = dalloc<int>(e, D.in_cap);
+  if (e->can_cluster) {   // the global engine's arrays (entries of the whole system)
+    CoD& G = e->Dg;
+    G = D;   // particle arrays and scalars are filled in below / shared
+    const size_t nq = (size_t)D.near_cap;
+    const int icap = (int)std::min<long long>(NT, (long long)NE_CAP + 8192LL * e->nd);
+    G.ns_x = dalloc<double>(e, nq); G.ns_y = dalloc<double>(e, nq); G.ns_z = dalloc<double>(e, nq);
+    G.ns_vx = dalloc<double>(e, nq); G.ns_vy = dalloc<double>(e, nq); G.ns_vz = dalloc<double>(e, nq);
+    G.ns_flag = dalloc<int>(e, nq); G.ns_stat = dalloc<unsigned char>(e, nq);
+    G.in_cap = icap;
+    G.in_list = dalloc<int>(e, icap); G.in_plist = dalloc<int>(e, icap); G.in_msd = dalloc<double>(e, icap);
+    G.sl_x = dalloc<double>(e, icap); G.sl_y = dalloc<double>(e, icap); G.sl_z = dalloc<double>(e, icap);
+    G.sl_vx = dalloc<double>(e, icap); G.sl_vy = dalloc<double>(e, icap); G.sl_vz = dalloc<double>(e, icap);
+    G.sl_flag = dalloc<int>(e, icap);
+    GeD& E = e->E;
+    E.sy = dalloc<GeSync>(e, 1);
+    E.memb_q = dalloc<unsigned>(e, nq);
+    E.in_memb4 = dalloc<unsigned>(e, icap);
+    E.in_fp4 = dalloc<double>(e, (size_t)icap * 12);
+    E.in_pot4 = dalloc<double>(e, (size_t)icap * 4);
+    E.part = dalloc<double>(e, (size_t)GE_GMAX * (4 * MAXSG + 1));
+    E.in_cap = icap;
+    e->geG = std::min(GE_GMAX, std::max(2, 2 * e->nd));
+  }
   D.cell_of = dalloc<int>(e, NT);
   D.rank_of = dalloc<unsigned short>(e, NT);
   D.cnt = dalloc<int>(e, mc);
@@ -2100,6 +2901,15 @@ extern "C" int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg) {
   HIPCHK(hipMemset(D.in_pot, 0, (size_t)MAXM * D.in_cap * 8));
   HIPCHK(hipMemset(D.in_msd, 0, (size_t)D.in_cap * 8));
   if (e->can_cluster) {
+    // the global engine shares the particle arrays, scalars and cell data
+    CoD& G = e->Dg;
+    G.x = D.x; G.y = D.y; G.z = D.z; G.vx = D.vx; G.vy = D.vy; G.vz = D.vz; G.flag = D.flag; G.dim = D.dim;
+    G.sc = D.sc; G.react_box = D.react_box; G.near_list = D.near_list; G.near_bits = D.near_bits;
+    G.status = D.status; G.near_cap = D.near_cap;
+    HIPCHK(hipMemset(e->E.sy, 0, sizeof(GeSync)));
+    HIPCHK(hipMemset(e->E.in_fp4, 0, (size_t)e->E.in_cap * 12 * 8));
+    HIPCHK(hipMemset(e->E.in_pot4, 0, (size_t)e->E.in_cap * 4 * 8));
+    HIPCHK(hipMemset(G.in_msd, 0, (size_t)e->E.in_cap * 8));
     HIPCHK(hipMemset(D.clu, 0, sizeof(CluInfo)));
     HIPCHK(hipMemset(D.rec_ids, 0xff, (size_t)NT * 4));
     HIPCHK(hipMemset(D.rec_msd, 0, (size_t)NT * 8));
@@ -2108,13 +2918,16 @@ extern "C" int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg) {
   return 0;
 }
 
-// 1: one near-engine workgroup per cluster of dimers (two or more dimers);
-// 0: one workgroup for the whole system (up to four dimers)
-extern "C" int mpcx_co_set_clusters(mpcx_co* e, int32_t on) {
+// 0: one workgroup for the whole system (up to four dimers); 1: one
+// workgroup per cluster of dimers (clusters of up to four); 2: the system's
+// near set over cooperative workgroups (any number of dimers)
+extern "C" int mpcx_co_set_near_mode(mpcx_co* e, int32_t mode) {
   if (!e) return set_error(MPCX_E_ARG, "null engine");
-  if (on && !e->can_cluster) return set_error(MPCX_E_UNSUPPORTED, "clusters need two or more dimers");
-  if (!on && e->nd > MAXD) return set_error(MPCX_E_UNSUPPORTED, "one workgroup handles up to four dimers");
-  e->clustered = on != 0;
+  if (mode < 0 || mode > 2) return set_error(MPCX_E_ARG, "near mode is 0, 1 or 2");
+  if (mode && !e->can_cluster) return set_error(MPCX_E_UNSUPPORTED, "modes 1 and 2 need two or more dimers");
+  if (!mode && e->nd > MAXD) return set_error(MPCX_E_UNSUPPORTED, "one workgroup handles up to four dimers");
+  e->near_mode = mode;
+  e->clustered = mode == 1;
   return 0;
 }
 
@@ -2222,7 +3035,25 @@ static void co_near_list(mpcx_co* e) {
                    e->P.NT, &e->D.sc->near_num, &e->D.sc->err);
 }
 
+__global__ void ge_reset(GeSync* sy) { sy->count = 0u; sy->abort = 0; }
+
 static hipError_t launch_near(mpcx_co* e, int nsteps, int mode) {
+  if (e->near_mode == 2) {
+    hipLaunchKernelGGL(ge_reset, dim3(1), dim3(1), 0, e->st, e->E.sy);
+    const CoP P = e->P;
+    CoD G = e->Dg;
+    {   // the particle arrays the system holds now (a sort swaps them with its buffers)
+      const CoD& D = e->D;
+      G.x = D.x; G.y = D.y; G.z = D.z; G.vx = D.vx; G.vy = D.vy; G.vz = D.vz; G.flag = D.flag;
+      G.status = D.status; G.near_bits = D.near_bits; G.near_list = D.near_list; G.dim = D.dim; G.sc = D.sc;
+    }
+    const GeD E = e->E;
+    void* args[] = {(void*)&P, (void*)&G, (void*)&E, (void*)&nsteps, (void*)&mode};
+    hipError_t r = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(co_ge_engine), dim3(e->geG), dim3(GE_T),
+                                              args, 0, e->st);
+    if (r != hipSuccess) return r;
+    return hipGetLastError();
+  }
   if (e->clustered) {
     CoD& D = e->D;
     CluBase& B = e->cb;
@@ -2413,7 +3244,7 @@ extern "C" int mpcx_co_get_state(mpcx_co* e, double* x, double* y, double* z, do
 extern "C" int mpcx_co_get_inside(mpcx_co* e, int32_t* list, uint32_t* memb, double* fp, double* pot, int64_t cap,
                                   int64_t* n) {
   if (!e || !n) return set_error(MPCX_E_ARG, "null argument");
-  if (e->clustered) return set_error(MPCX_E_UNSUPPORTED, "with clusters the lists are read with mpcx_co_get_lists");
+  if (e->near_mode != 0) return set_error(MPCX_E_UNSUPPORTED, "in this near mode the lists are read with mpcx_co_get_lists");
   HIPCHK(hipStreamSynchronize(e->st));
   CoScalars sc;
   HIPCHK(hipMemcpy(&sc, e->D.sc, sizeof sc, hipMemcpyDeviceToHost));
@@ -2470,7 +3301,33 @@ extern "C" int mpcx_co_get_lists(mpcx_co* e, int64_t max_nb, int64_t* nb, int64_
     }
     return 0;
   };
-  if (!e->clustered) {
+  if (e->near_mode == 2) {
+    CoScalars sc;
+    HIPCHK(hipMemcpy(&sc, e->D.sc, sizeof sc, hipMemcpyDeviceToHost));
+    const int n = std::min(sc.in_num, e->E.in_cap);
+    std::vector<int> pl(n);
+    std::vector<unsigned> ids(n);
+    std::vector<double> fp((size_t)n * 12), pt((size_t)n * 4);
+    if (n) {
+      HIPCHK(hipMemcpy(pl.data(), e->Dg.in_plist, (size_t)n * 4, hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(ids.data(), e->E.in_memb4, (size_t)n * 4, hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(fp.data(), e->E.in_fp4, (size_t)n * 96, hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(pt.data(), e->E.in_pot4, (size_t)n * 32, hipMemcpyDeviceToHost));
+    }
+    for (int k = 0; k < n; k++) {
+      all.push_back(pl[k]);
+      for (int m = 0; m < 4; m++) {
+        const unsigned b = (ids[k] >> (8 * m)) & 0xffu;
+        if (b == 0xffu) break;
+        const int kg = (int)(b >> 1), i = (int)(b & 1u);
+        if (kg >= nd) return set_error(MPCX_E_DEVICE, "list membership outside the system's dimers");
+        const int L = kg + nd * i;
+        lp[L].push_back(pl[k]);
+        for (int c = 0; c < 3; c++) lf[L].push_back(fp[(size_t)k * 12 + m * 3 + c]);
+        lf[L].push_back(pt[(size_t)k * 4 + m]);
+      }
+    }
+  } else if (!e->clustered) {
     CoScalars sc;
     HIPCHK(hipMemcpy(&sc, e->D.sc, sizeof sc, hipMemcpyDeviceToHost));
     int rc = take(e->D, sc.in_num, nullptr);

@@ -52,14 +52,21 @@ CASES = {
                               p=("small", dict(numdims=3, Lx=18.0, Ly=18.0, Lz=16.0, sort_step=20, EbN=0.1)),
                               tour="hilbert"),
     "four_dimers": dict(ncoll=3, seed=12, p=("small", dict(numdims=4, Lx=18.0, Ly=18.0, Lz=14.0, EbN=0.3))),
-    "four_dimers_one_workgroup": dict(ncoll=3, seed=12, clusters=False,
+    "four_dimers_one_workgroup": dict(ncoll=3, seed=12, near_mode=0,
                                       p=("small", dict(numdims=4, Lx=18.0, Ly=18.0, Lz=14.0, EbN=0.3))),
-    "two_dimers_clusters": dict(ncoll=3, seed=7, flags_mix=True, clusters=True,
+    "two_dimers_clusters": dict(ncoll=3, seed=7, flags_mix=True, near_mode=1,
                                 p=("small", dict(numdims=2, Lx=20.0, Ly=20.0, Lz=24.0, sort_step=20, EbN=0.1)),
                                 tour="hilbert"),
-    "eight_dimers_clusters_sort": dict(ncoll=4, seed=13, flags_mix=True,
+    "eight_dimers_clusters_sort": dict(ncoll=4, seed=13, flags_mix=True, near_mode=1,
                                        p=("small", dict(numdims=8, Lx=28.0, Ly=28.0, Lz=16.0, sort_step=20)),
                                        tour="hilbert"),
+    "eight_dimers_global_sort": dict(ncoll=4, seed=13, flags_mix=True,
+                                     p=("small", dict(numdims=8, Lx=28.0, Ly=28.0, Lz=16.0, sort_step=20)),
+                                     tour="hilbert"),
+    "two_dimers_global": dict(ncoll=3, seed=7, flags_mix=True, near_mode=2,
+                              p=("small", dict(numdims=2, Lx=20.0, Ly=20.0, Lz=24.0, sort_step=20, EbN=0.1)),
+                              tour="hilbert"),
+    "twenty_dimers_dense": dict(ncoll=3, seed=14, p=("small", dict(numdims=20, Lx=40.0, Ly=40.0, Lz=16.0, rhoa=4.0))),
     "reactive_wall": dict(ncoll=3, seed=8, p=("small", dict(k1=0.2, Lz=12.0)), react="wall2"),
     "dense_skin05": dict(ncoll=2, seed=9, flags_mix=True, p=("small", dict(rhoa=8.0, skinrad=0.5, Rn=2.5))),
 }
@@ -97,11 +104,13 @@ def _h(a):
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
 
-@pytest.mark.parametrize("case,clusters", [("co_one_dimer", None), ("co_two_dimers_sort_wall", None),
-                                           ("co_two_dimers_sort_wall", True), ("co_six_dimers_sort", None)])
-def test_engine_integer_state_equals_reference_fixture(golden, case, clusters):
-    """Flags and cell arrays hashed from the reference's Fortran run (six
-    dimers: clusters, each advanced by its own workgroup)."""
+@pytest.mark.parametrize("case,near_mode", [("co_one_dimer", None), ("co_two_dimers_sort_wall", None),
+                                            ("co_six_dimers_sort", 1),
+                                           ("co_two_dimers_sort_wall", 1), ("co_two_dimers_sort_wall", 2), ("co_six_dimers_sort", None)])
+def test_engine_integer_state_equals_reference_fixture(golden, case, near_mode):
+    """Flags and cell arrays hashed from the reference's Fortran run, in
+    every near-engine mode (0 one workgroup, 1 clusters, 2 cooperative
+    workgroups over the near set; None the default for the dimer count)."""
     from tests.golden.make_golden import collective_inputs
     rec = golden["cases"][case]
     cs = rec["case"]
@@ -111,7 +120,7 @@ def test_engine_integer_state_equals_reference_fixture(golden, case, clusters):
     ncoll = cs["steps"] // p["MPCtstep"]
     assert ncoll * p["MPCtstep"] == cs["steps"]
     draws = host_draws(MtmodStream(mt=s["mt"], mti=s["mti"]), p, ncoll)
-    eng = CollectiveEngine(p, rng_mode=0, clusters=clusters)
+    eng = CollectiveEngine(p, rng_mode=0, near_mode=near_mode)
     try:
         eng.set_state(s, tour=tour)
         eng.setup(do_sort=cs["sort"])
