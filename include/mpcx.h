@@ -130,7 +130,8 @@ int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg);
 int mpcx_co_destroy(mpcx_co* e);
 
 /* Upload solvent and dimer state; dimer arrays are (numdims,2) column-major.
- * react_box is (Lx*Ly) from reactive_grid_setup; tour as in mpcx_sort_hilbert. */
+ * react_box is (Lx*Ly) from reactive_grid_setup; tour as in mpcx_sort_hilbert.
+ * The seven solvent arrays may all be NULL (then mpcx_co_init_solvent). */
 int mpcx_co_set_state(mpcx_co* e, int32_t cstep, const double* x, const double* y, const double* z,
                       const double* vx, const double* vy, const double* vz, const int32_t* flag,
                       const double* xd, const double* yd, const double* zd,
@@ -146,6 +147,13 @@ int mpcx_co_setup(mpcx_co* e, int32_t do_sort);
  * each).  In rng_mode 0, draws holds ncoll blocks of
  * 3 + 2*mpcnumcell + rxnnumcell uniforms (grnd() order of one collision). */
 int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws);
+
+/* Initial solvent on the device (initial_mod.f95:122-345 with Philox draws
+ * from `seed` instead of the Mersenne Twister): flags A below Na0, B below
+ * Na0+Nb0, S above; positions uniform in the box outside every sphere's LJ
+ * cutoff (the dimers set by mpcx_co_set_state, which then takes NULL solvent
+ * arrays); Gaussian velocities of variance T/m; total momentum removed. */
+int mpcx_co_init_solvent(mpcx_co* e, int64_t Na0, int64_t Nb0, double T, uint64_t seed);
 
 /* Download state.  Arrays may be NULL to skip them. */
 int mpcx_co_get_state(mpcx_co* e, double* x, double* y, double* z, double* vx, double* vy, double* vz,

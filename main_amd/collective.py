@@ -111,6 +111,26 @@ class CollectiveEngine:
                                                                            "fxd", "fyd", "fzd")],
                                       ptr(rb), ptr(t)), "mpcx_co_set_state")
 
+    def set_dimers(self, s: dict, tour=None, cstep: int = 0):
+        """Dimers, reactive grid and tour only; the solvent then comes from init_solvent."""
+        dims = {k: np.asfortranarray(s[k], dtype=np.float64) for k in
+                ("xd", "yd", "zd", "vxd", "vyd", "vzd", "fxd", "fyd", "fzd")}
+        self._keep = list(dims.values())
+        rb = np.ascontiguousarray(s["react_box"], dtype=np.int32)
+        t = None if tour is None else np.ascontiguousarray(tour, dtype=np.int64)
+        self._keep += [rb, t]
+        null = None
+        check(lib().mpcx_co_set_state(self.h, C.c_int32(cstep), *([null] * 7),
+                                      *[ptr(dims[k]) for k in ("xd", "yd", "zd", "vxd", "vyd", "vzd",
+                                                               "fxd", "fyd", "fzd")],
+                                      ptr(rb), ptr(t)), "mpcx_co_set_state")
+
+    def init_solvent(self, seed: int):
+        """Solvent of initial_mod.f95:122-345 generated on the device (Philox draws)."""
+        p = self.p
+        check(lib().mpcx_co_init_solvent(self.h, C.c_int64(int(p["Na0"])), C.c_int64(int(p["Nb0"])),
+                                         C.c_double(float(p["T"])), C.c_uint64(int(seed))), "mpcx_co_init_solvent")
+
     def setup(self, do_sort: bool = True):
         check(lib().mpcx_co_setup(self.h, C.c_int32(1 if do_sort else 0)), "mpcx_co_setup")
 

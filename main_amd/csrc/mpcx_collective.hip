@@ -2696,8 +2696,9 @@ static int check_device_error(mpcx_co* e) {
                                 "the host draw buffer ran out",
                                 "more than four dimers lie within reach of each other (one cluster)",
                                 "the clusters of the system fell out of step",
-                                "a neighbour-list member was carried to a cluster without its dimer"};
-    std::string m = std::string("device check failed: ") + (err > 0 && err < 16 ? msg[err] : "unknown");
+                                "a neighbour-list member was carried to a cluster without its dimer",
+                                "no place outside the spheres found for a solvent particle"};
+    std::string m = std::string("device check failed: ") + (err > 0 && err < 17 ? msg[err] : "unknown");
     int zero = 0;
     (void)hipMemcpy(&e->D.sc->err, &zero, sizeof(int), hipMemcpyHostToDevice);
     return set_error(MPCX_E_DEVICE, m.c_str());
@@ -2950,17 +2951,21 @@ extern "C" int mpcx_co_set_state(mpcx_co* e, int32_t cstep, const double* x, con
                                  const double* xd, const double* yd, const double* zd, const double* vxd,
                                  const double* vyd, const double* vzd, const double* fxd, const double* fyd,
                                  const double* fzd, const int32_t* react_box, const int64_t* tour) {
-  if (!e || !x || !y || !z || !vx || !vy || !vz || !flag || !xd || !yd || !zd)
+  // the solvent arrays may all be NULL: mpcx_co_init_solvent then fills them on the device
+  const bool solvent = x || y || z || vx || vy || vz || flag;
+  if ((solvent && (!x || !y || !z || !vx || !vy || !vz || !flag)) || !xd || !yd || !zd)
     return set_error(MPCX_E_ARG, "null argument");
   const size_t NT = (size_t)e->P.NT, nd = (size_t)e->nd;
   CoD& D = e->D;
-  HIPCHK(hipMemcpy(D.x, x, NT * 8, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(D.y, y, NT * 8, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(D.z, z, NT * 8, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(D.vx, vx, NT * 8, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(D.vy, vy, NT * 8, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(D.vz, vz, NT * 8, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(D.flag, flag, NT * 4, hipMemcpyHostToDevice));
+  if (solvent) {
+    HIPCHK(hipMemcpy(D.x, x, NT * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(D.y, y, NT * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(D.z, z, NT * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(D.vx, vx, NT * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(D.vy, vy, NT * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(D.vz, vz, NT * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(D.flag, flag, NT * 4, hipMemcpyHostToDevice));
+  }
   std::vector<double> dim(12 * MAXSG, 0.0);
   const double* src[9] = {xd, yd, zd, vxd, vyd, vzd, fxd, fyd, fzd};
   for (int f = 0; f < 9; f++)
@@ -3210,6 +3215,93 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
     }
     e->kt_ms[k] = n ? sum / (double)n : 0.0;
   }
+  return check_device_error(e);
+}
+
+// ------------------------------------------------------------- initial solvent
+// initial_mod.f95:122-345 on the device: species by index (A below Na0, then
+// B below Na0+Nb0, then S), positions uniform in the box and redrawn while
+// within the cutoff of any sphere, Gaussian velocities with variance T/m of
+// the species (Box-Muller), then the mean velocity subtracted.  Draws come
+// from Philox streams 8 (positions, one counter per attempt) and 9
+// (velocities), keyed by the seed and the particle index.
+namespace mpcx {
+constexpr int INIT_TRIES = 100000;
+__global__ __launch_bounds__(256) void init_solvent(CoP P, CoD D, long long na, long long nb, double sa, double sb,
+                                                    double ss, unsigned long long seed, double* part) {
+  __shared__ double sx[MAXSG], sy[MAXSG], sz[MAXSG];
+  __shared__ double red[4][3];
+  const int nd = P.nd;
+  for (int s = threadIdx.x; s < 2 * nd; s += 256) {
+    sx[s] = D.dim[DX * MAXSG + s]; sy[s] = D.dim[DY * MAXSG + s]; sz[s] = D.dim[DZ * MAXSG + s];
+  }
+  __syncthreads();
+  const double twopi = 2.0 * (4.0 * atan(1.0));
+  double acc[3] = {0.0, 0.0, 0.0};
+  const long long per = (P.NT + gridDim.x - 1) / gridDim.x;
+  const long long i0 = (long long)blockIdx.x * per, i1 = min((long long)P.NT, i0 + per);
+  for (long long i = i0 + threadIdx.x; i < i1; i += 256) {
+    double x = 0, y = 0, z = 0;
+    bool ok = false;
+    for (int a = 0; a < INIT_TRIES && !ok; a++) {
+      double u, v, w, unused;
+      philox_uniform2(seed, 8, 2ull * a, (uint64_t)i, u, v);
+      philox_uniform2(seed, 8, 2ull * a + 1, (uint64_t)i, w, unused);
+      x = P.box[0] * u; y = P.box[1] * v; z = P.box[2] * w;
+      ok = true;
+      for (int s = 0; s < 2 * nd && ok; s++) {
+        const double a1 = pbc_half(x - sx[s], P.box[0]), a2 = pbc_half(y - sy[s], P.box[1]), a3 = z - sz[s];
+        if (a1 * a1 + a2 * a2 + a3 * a3 < P.cut2[s >= nd]) ok = false;
+      }
+    }
+    if (!ok) raise_error(&D.sc->err, ERR_INIT);
+    const int f = i < na ? 0 : (i < na + nb ? 1 : 2);
+    const double sd = f == 0 ? sa : (f == 1 ? sb : ss);
+    double u1, u2, u3, u4;
+    philox_uniform2(seed, 9, 0, (uint64_t)i, u1, u2);
+    philox_uniform2(seed, 9, 1, (uint64_t)i, u3, u4);
+    const double r1 = sqrt(-2.0 * log(1.0 - u1)), r2 = sqrt(-2.0 * log(1.0 - u3));
+    const double vx = sd * (r1 * cos(twopi * u2)), vy = sd * (r1 * sin(twopi * u2)), vz = sd * (r2 * cos(twopi * u4));
+    D.x[i] = x; D.y[i] = y; D.z[i] = z;
+    D.vx[i] = vx; D.vy[i] = vy; D.vz[i] = vz;
+    D.flag[i] = f;
+    acc[0] += vx; acc[1] += vy; acc[2] += vz;
+  }
+  for (int c = 0; c < 3; c++) {
+    const double r = wave_sum(acc[c]);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][c] = r;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) part[blockIdx.x * 3 + threadIdx.x] =
+      ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
+}
+__global__ __launch_bounds__(256) void init_zero_momentum(CoP P, CoD D, const double* part, int nblk) {
+  __shared__ double mean[3];
+  if (threadIdx.x < 3) {
+    double s = 0.0;
+    for (int b = 0; b < nblk; b++) s = s + part[b * 3 + threadIdx.x];
+    mean[threadIdx.x] = s / (double)P.NT;
+  }
+  __syncthreads();
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < P.NT; i += (long long)gridDim.x * 256) {
+    D.vx[i] = D.vx[i] - mean[0];
+    D.vy[i] = D.vy[i] - mean[1];
+    D.vz[i] = D.vz[i] - mean[2];
+  }
+}
+}  // namespace mpcx
+
+extern "C" int mpcx_co_init_solvent(mpcx_co* e, int64_t Na0, int64_t Nb0, double T, uint64_t seed) {
+  if (!e || Na0 < 0 || Nb0 < 0 || Na0 + Nb0 > e->P.NT || !(T >= 0)) return set_error(MPCX_E_ARG, "bad arguments");
+  const int nblk = 2048;
+  double* part = nullptr;
+  HIPCHK(hipMalloc(&part, (size_t)nblk * 3 * 8));
+  hipLaunchKernelGGL(init_solvent, dim3(nblk), dim3(256), 0, e->st, e->P, e->D, (long long)Na0, (long long)Nb0,
+                     sqrt(T / e->cfg.Ma), sqrt(T / e->cfg.Mb), sqrt(T / e->cfg.Ms), (unsigned long long)seed, part);
+  hipLaunchKernelGGL(init_zero_momentum, dim3(1024), dim3(256), 0, e->st, e->P, e->D, (const double*)part, nblk);
+  hipError_t r = hipStreamSynchronize(e->st);
+  (void)hipFree(part);
+  if (r != hipSuccess) return set_error(MPCX_E_HIP, hipGetErrorString(r));
   return check_device_error(e);
 }
 

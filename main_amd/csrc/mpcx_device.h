@@ -29,6 +29,7 @@ enum DevError : int {
   ERR_CLUSTER = 13,        // more than four dimers whose near regions overlap
   ERR_BARRIER = 14,        // the clusters of one system fell out of step
   ERR_CLUSTER_MEMB = 15,   // a list member carried over to a cluster without its dimer
+  ERR_INIT = 16,           // no place outside the spheres found for a solvent particle
 };
 
 __device__ __forceinline__ void raise_error(int* err, int code) { atomicCAS(err, 0, code); }

@@ -62,9 +62,30 @@ def place_dimers(p, rng):
     xd = np.zeros((nd, 2), order="F")
     yd = np.zeros((nd, 2), order="F")
     zd = np.zeros((nd, 2), order="F")
+    # boxes too low for the reference's wall rejection (it needs
+    # (Rw+R)*3^(1/6) + R*2^(1/6) of clearance on both sides of each sphere,
+    # and initial_mod.f95:80-119 then never ends): every dimer at mid-height
+    # with its bond in the x-y plane, rejecting overlaps only
+    low = box[2] < 2.0 * max(wc + Rc * cut, wn + Rn * cut)
+    if low and 0.5 * box[2] <= max(Rc, Rn) * cut:
+        raise RuntimeError("box too low for a dimer")
+
+    def overlaps(pc, pn, i):
+        for j in range(i):
+            for (a, ra), (b, rb) in [((pc, Rc), ((xd[j, 0], yd[j, 0], zd[j, 0]), Rc)),
+                                     ((pc, Rc), ((xd[j, 1], yd[j, 1], zd[j, 1]), Rn)),
+                                     ((pn, Rn), ((xd[j, 0], yd[j, 0], zd[j, 0]), Rc)),
+                                     ((pn, Rn), ((xd[j, 1], yd[j, 1], zd[j, 1]), Rn))]:
+                d = np.array(a) - np.array(b)
+                d[:2] = _pbc(d[:2], box[:2])
+                if d @ d < ((ra + rb) * cut + p.get("dimer_gap", 0.0)) ** 2:
+                    return True
+        return False
+
     for i in range(nd):
         placed = False
-        for _ in range(20000 if nd > 1 else 2000):
+        tries = 0 if low else (20000 if nd > 1 else 2000)
+        for _ in range(tries):
             x, y, z, ph, th = rng.random(5)
             ph = 2.0 * PI * ph
             th = 2.0 * th - 1.0
@@ -74,30 +95,18 @@ def place_dimers(p, rng):
             pc, pn = c + 0.5 * Dcn * u, c - 0.5 * Dcn * u
             ok = (wc + Rc * cut <= pc[2] <= box[2] - wc - Rc * cut) and \
                  (wn + Rn * cut <= pn[2] <= box[2] - wn - Rn * cut)
-            for j in range(i):
-                for (a, ra), (b, rb) in [((pc, Rc), ((xd[j, 0], yd[j, 0], zd[j, 0]), Rc)),
-                                         ((pc, Rc), ((xd[j, 1], yd[j, 1], zd[j, 1]), Rn)),
-                                         ((pn, Rn), ((xd[j, 0], yd[j, 0], zd[j, 0]), Rc)),
-                                         ((pn, Rn), ((xd[j, 1], yd[j, 1], zd[j, 1]), Rn))]:
-                    d = np.array(a) - np.array(b)
-                    d[:2] = _pbc(d[:2], box[:2])
-                    if d @ d < ((ra + rb) * cut + p.get("dimer_gap", 0.0)) ** 2:
-                        ok = False
-            if ok:
+            if ok and not overlaps(pc, pn, i):
                 placed = True
                 break
-        if not placed and nd == 1:
-            # boxes too low for the reference's wall rejection (it needs
-            # (Rw+Rn)*3^(1/6) + Rn*2^(1/6) of clearance on both sides): the
-            # dimer at mid-height with its bond in the x-y plane, beyond the
-            # range of the wall potential when Lz/2 > (Rw+Rn)*3^(1/6)
-            x, y, ph = rng.random(3)
-            c = np.array([box[0] * x, box[1] * y, 0.5 * box[2]])
-            u = np.array([math.cos(2.0 * PI * ph), math.sin(2.0 * PI * ph), 0.0])
-            pc, pn = c + 0.5 * Dcn * u, c - 0.5 * Dcn * u
-            if 0.5 * box[2] <= max(wc, wn):
-                raise RuntimeError("box too low for a dimer")
-            placed = True
+        if not placed and (low or nd == 1):
+            for _ in range(20000):
+                x, y, ph = rng.random(3)
+                c = np.array([box[0] * x, box[1] * y, 0.5 * box[2]])
+                u = np.array([math.cos(2.0 * PI * ph), math.sin(2.0 * PI * ph), 0.0])
+                pc, pn = c + 0.5 * Dcn * u, c - 0.5 * Dcn * u
+                if not overlaps(pc, pn, i):
+                    placed = True
+                    break
         if not placed:
             raise RuntimeError("could not place dimer %d" % i)
         pc[:2] -= box[:2] * np.floor(pc[:2] / box[:2])
@@ -129,6 +138,18 @@ def solvent_outside_spheres(n, box, spheres, rng, periodic_z=False):
 def maxwell_velocities(n, kT, m, rng):
     v = rng.standard_normal((n, 3)) * math.sqrt(kT / m)
     return v - v.mean(0)
+
+
+def collective_dimers(p, seed=4357787, reactive_box=None):
+    """Dimer part of the initial state (placement as collective_initial_state)
+    for engines that generate the solvent on the device."""
+    rng = np.random.default_rng(seed)
+    xd, yd, zd = place_dimers(p, rng)
+    nd = p["numdims"]
+    z = lambda: np.zeros((nd, 2), order="F")
+    nrb = int(p["box"][0]) * int(p["box"][1])
+    rb = np.zeros(nrb, np.int32) if reactive_box is None else np.asarray(reactive_box, np.int32)
+    return dict(xd=xd, yd=yd, zd=zd, vxd=z(), vyd=z(), vzd=z(), fxd=z(), fyd=z(), fzd=z(), react_box=rb)
 
 
 def collective_initial_state(p, seed=4357787, reactive_box=None):
