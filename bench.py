@@ -124,6 +124,51 @@ def batched(R=256, steps=20, warmup=4, seed=4357787, tiles=(1024, 2048, 4096, 81
     return out
 
 
+MANY_DIMER = {
+    # 200x200x10 slab, 100 dimers in the mid-plane, plain FP64 (the split-precision
+    # configuration's geometry and physics)
+    "slab_100_dimers_200x200x10": dict(Lx=200.0, Ly=200.0, Lz=10.0, numdims=100),
+    # collective_code/input.txt: 64x64x32 box, 20 dimers
+    "reference_input_20_dimers_64x64x32": dict(Lx=64.0, Ly=64.0, Lz=32.0, numdims=20),
+}
+
+
+def many_dimers(name, steps=20, warmup=5, seed=4357787):
+    """One many-dimer system (device-generated initial solvent, headline
+    physics, sort every 10 collisions), advanced by the global near engine."""
+    from main_amd import systems as S
+    from main_amd._lib import device_synchronize
+    from main_amd.collective import CollectiveEngine
+    from main_amd.tour import cell_tour
+    p = S.collective_params(**MANY_DIMER[name], rhoa=10.0, Rc=2.0, Rn=4.0, Dcn=6.5, EbN=0.1, T=1.0 / 6.0,
+                            skinrad=0.3, MDt=0.01, MPCt=0.5, sort_step=500, k1=1e-3)
+    t0 = time.time()
+    d = S.collective_dimers(p, seed=seed)
+    eng = CollectiveEngine(p, rng_mode=1, seed=seed)
+    eng.set_dimers(d, tour=cell_tour(*[int(v) for v in p["box"]]))
+    eng.init_solvent(seed)
+    eng.setup(do_sort=True)
+    device_synchronize()
+    t_init = time.time() - t0
+    eng.run(warmup)
+    device_synchronize()
+    t0 = time.perf_counter()
+    eng.run(steps)
+    device_synchronize()
+    dt = time.perf_counter() - t0
+    kms = eng.kernel_ms()
+    prof = eng.near_profile()
+    eng.close()
+    print(f"# {name}: workgroup-0 us per launch: total {prof['total_us']:.0f} rebuild {prof['rebuild_us']:.0f} "
+          f"particles {prof['particle_us']:.0f} dimers {prof['serial_us']:.0f} rebuilds {prof['rebuilds']:.1f} "
+          f"barrier waits {prof['near_num'] / 100.0 / max(prof['launches'], 1):.0f} "
+          f"totals {prof['inside_num'] / 100.0 / max(prof['launches'], 1):.0f}", file=sys.stderr, flush=True)
+    print(f"# {name}: {dt * 1000.0 / steps:.2f} ms/interval", file=sys.stderr, flush=True)
+    return {"particles": p["NT"], "dimers": p["numdims"], "ms_per_interval": dt * 1000.0 / steps,
+            "particle_updates_per_s": p["NT"] * steps / dt, "init_s": t_init, "kernel_ms": kms,
+            "near_engine": "global (cooperative workgroups)", "dtype": "f64", "rng": "device Philox4x32-10"}
+
+
 def read_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed counter summary."""
     if not os.path.exists(TRAFFIC_FILE):
@@ -185,6 +230,8 @@ def main():
     ap.add_argument("--batched", type=int, default=256,
                     help="replicas of the batched ensemble measured beside the headline (0: skip)")
     ap.add_argument("--batched-only", action="store_true", help="measure only the replica ensemble")
+    ap.add_argument("--many-dimers", type=int, default=1, help="measure the many-dimer systems beside the headline")
+    ap.add_argument("--many-dimers-only", action="store_true")
     args = ap.parse_args()
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc)
@@ -202,6 +249,10 @@ def main():
     if args.batched_only:
         print(json.dumps({"metric": "replicas/hour (2000-collision runs)", **batched(args.batched, args.steps,
                                                                                    args.warmup)}), flush=True)
+        return
+    if args.many_dimers_only:
+        out = {name: many_dimers(name, args.steps, args.warmup) for name in MANY_DIMER}
+        print(json.dumps({"metric": "particle-updates/s (MD+MPC), many dimers", **out}), flush=True)
         return
     if args.probe_copy:
         ms = C.c_double(0)
@@ -262,6 +313,8 @@ def main():
     }
     if args.batched > 0:
         out["batched"] = batched(args.batched, steps=10, warmup=2)
+    if args.many_dimers:
+        out["many_dimers"] = {name: many_dimers(name) for name in MANY_DIMER}
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(args.size)

@@ -1403,6 +1403,9 @@ constexpr int GE_T = 512;
 constexpr int GE_W = GE_T / 64;
 constexpr int GE_LCAP = 768;    // inside entries of a workgroup staged in LDS
 constexpr int GE_GMAX = 128;    // workgroups of one launch
+constexpr int SG_CELLS = 1024;  // coarse cells of the sphere grid
+constexpr int SG_CAP = 8;       // spheres listed per coarse cell (more: the cell scans all spheres)
+constexpr int MM_CAP = 8;       // sphere-sphere candidates per sphere (more: all spheres)
 struct GeSync {
   unsigned count, gen;
   int abort, pad;
@@ -1417,7 +1420,8 @@ struct GeD {
   unsigned* in_memb4;    // [in_cap] memberships of the inside entries
   double* in_fp4;        // [in_cap][4][3] pair forces by membership slot
   double* in_pot4;       // [in_cap][4]
-  double* part;          // [GE_GMAX][4*MAXSG+1] per-workgroup sphere sums, max displacement
+  double* part;          // [2][GE_GMAX][4*MAXSG+1] per-workgroup sphere sums, max displacement (by step parity)
+  unsigned* touch;       // [2][MAXSG][GE_GMAX/32] workgroups holding members of each sphere's list
   int in_cap, pad;
 };
 struct GeShared {
@@ -1432,8 +1436,21 @@ struct GeShared {
   int cstep, tc, tp, in_num, nev, rebuild;
   int qb, qe, ib, ie;
   double vmax2, red[GE_W];
+  unsigned long long tm[8];   // real-time ticks by phase (workgroup 0 reports them)
+  unsigned touched[2][(MAXSG + 31) / 32];   // spheres this workgroup published, by step parity
+  // coarse grid of the spheres (positions at the launch start, radius
+  // full + guard) and each sphere's sphere-sphere candidates
+  unsigned char sg_ids[SG_CELLS][SG_CAP];
+  unsigned char sg_n[SG_CELLS];
+  unsigned char mm_ids[MAXSG][MM_CAP];
+  unsigned char mm_n[MAXSG];
+  int sg_nx, sg_ny, sg_nz;
+  double sg_h;
 };
 __shared__ GeShared g_ge;
+// phase timers: 1 rebuild, 2 particle phase, 3 dimers, 5 barrier waits, 6 totals over workgroups
+#define GE_T0() const unsigned long long t0_ = __builtin_amdgcn_s_memrealtime()
+#define GE_T1(k) if (threadIdx.x == 0) g_ge.tm[k] += __builtin_amdgcn_s_memrealtime() - t0_
 
 __device__ __forceinline__ int wave_min_i(int v) {
 #pragma unroll
@@ -1466,27 +1483,98 @@ __device__ __attribute__((noinline)) void ge_barrier_t0(GeSync* sy, int G, int* 
 __device__ __forceinline__ void ge_sync(const GeD& E, int* err) {
   __threadfence();
   __syncthreads();
+  GE_T0();
   if (threadIdx.x == 0) ge_barrier_t0(E.sy, (int)gridDim.x, err);
+  GE_T1(5);
   __syncthreads();
 }
 
+__device__ __forceinline__ int sg_cell(double x, double y, double z) {
+  const double h = g_ge.sg_h;
+  const int ix = min(max((int)floor(x / h), 0), g_ge.sg_nx - 1);
+  const int iy = min(max((int)floor(y / h), 0), g_ge.sg_ny - 1);
+  const int iz = min(max((int)floor(z / h), 0), g_ge.sg_nz - 1);
+  return ix + g_ge.sg_nx * (iy + g_ge.sg_ny * iz);
+}
+// list b = 2k+i holds the particle (neighbour_list_mod.f95:38-57)
+__device__ __forceinline__ bool ge_in_list(double x, double y, double z, const CoP& P, int b) {
+  const int k = b >> 1, i = b & 1, s = k + i * P.nd;
+  double d1 = pbc_half(x - g_ge.dim[DX][s], P.box[0]);
+  double d2 = pbc_half(y - g_ge.dim[DY][s], P.box[1]);
+  double d3 = z - g_ge.dim[DZ][s];
+  return d1 * d1 + d2 * d2 + d3 * d3 <= P.full2[i];
+}
+// the lists holding a particle, ascending b, from its coarse cell's candidates
 __device__ __forceinline__ unsigned ge_memberships(double x, double y, double z, const CoP& P, int* err) {
   unsigned ids = 0xffffffffu;
   int m = 0;
-  const int nd = P.nd;
-  for (int k = 0; k < nd; k++)
-    for (int i = 0; i < 2; i++) {
-      const int s = k + i * nd;
-      double d1 = pbc_half(x - g_ge.dim[DX][s], P.box[0]);
-      double d2 = pbc_half(y - g_ge.dim[DY][s], P.box[1]);
-      double d3 = z - g_ge.dim[DZ][s];
-      if (d1 * d1 + d2 * d2 + d3 * d3 <= P.full2[i]) {
-        if (m == 4) { raise_error(err, ERR_MEMBERSHIPS); return ids; }
-        ids = (ids & ~(0xffu << (8 * m))) | ((unsigned)(2 * k + i) << (8 * m));
-        m++;
+  const int c = sg_cell(x, y, z);
+  const int n = g_ge.sg_n[c];
+  const int nb = n <= SG_CAP ? n : 2 * P.nd;
+  for (int j = 0; j < nb; j++) {
+    const int b = n <= SG_CAP ? (int)g_ge.sg_ids[c][j] : j;
+    if (ge_in_list(x, y, z, P, b)) {
+      if (m == 4) { raise_error(err, ERR_MEMBERSHIPS); return ids; }
+      ids = (ids & ~(0xffu << (8 * m))) | ((unsigned)b << (8 * m));
+      m++;
+    }
+  }
+  return ids;
+}
+// per launch, in every workgroup: the coarse grid and the sphere-sphere
+// candidates; spheres move at most the guard in an interval
+__device__ __forceinline__ void ge_build_grids(const CoP& P) {
+  const int nd = P.nd, tid = threadIdx.x;
+  if (tid == 0) {
+    double h = 8.0;
+    int nx, ny, nz;
+    for (;;) {
+      nx = (int)ceil(P.box[0] / h); ny = (int)ceil(P.box[1] / h); nz = (int)ceil(P.box[2] / h);
+      if ((long long)nx * ny * nz <= SG_CELLS) break;
+      h *= 1.25;
+    }
+    g_ge.sg_h = h; g_ge.sg_nx = nx; g_ge.sg_ny = ny; g_ge.sg_nz = nz;
+  }
+  __syncthreads();
+  const double h = g_ge.sg_h;
+  const int ncell = g_ge.sg_nx * g_ge.sg_ny * g_ge.sg_nz;
+  for (int c = tid; c < ncell; c += GE_T) {
+    const int ix = c % g_ge.sg_nx, iy = (c / g_ge.sg_nx) % g_ge.sg_ny, iz = c / (g_ge.sg_nx * g_ge.sg_ny);
+    const double cx = (ix + 0.5) * h, cy = (iy + 0.5) * h, cz = (iz + 0.5) * h;
+    int n = 0;
+    for (int b = 0; b < 2 * nd; b++) {
+      const int k = b >> 1, i = b & 1, s = k + i * nd;
+      const double r = P.full[i] + P.guard + 1e-6;
+      const double ax = fmax(fabs(pbc_half(g_ge.dim[DX][s] - cx, P.box[0])) - 0.5 * h, 0.0);
+      const double ay = fmax(fabs(pbc_half(g_ge.dim[DY][s] - cy, P.box[1])) - 0.5 * h, 0.0);
+      const double az = fmax(fabs(g_ge.dim[DZ][s] - cz) - 0.5 * h, 0.0);
+      if (ax * ax + ay * ay + az * az <= r * r) {
+        if (n < SG_CAP) g_ge.sg_ids[c][n] = (unsigned char)b;
+        n++;
       }
     }
-  return ids;
+    g_ge.sg_n[c] = (unsigned char)min(n, 255);
+  }
+  for (int s = tid; s < 2 * nd; s += GE_T) {   // every workgroup integrates the dimers
+      const int k = s < nd ? s : s - nd, i = s < nd ? 0 : 1;
+      int n = 0;
+      for (int j = 0; j < nd; j++)
+        for (int l = 0; l < 2; l++) {
+          if (j == k) continue;
+          const int b = j + l * nd;
+          const int kind = (i == 0 && l == 0) ? 0 : (i == 1 && l == 1) ? 2 : 1;
+          const double r = sqrt(P.mm_cut2[kind]) + 2.0 * P.guard + 1e-6;
+          const double dx = pbc_half(g_ge.dim[DX][s] - g_ge.dim[DX][b], P.box[0]);
+          const double dy = pbc_half(g_ge.dim[DY][s] - g_ge.dim[DY][b], P.box[1]);
+          const double dz = g_ge.dim[DZ][s] - g_ge.dim[DZ][b];
+          if (dx * dx + dy * dy + dz * dz <= r * r) {
+            if (n < MM_CAP) g_ge.mm_ids[s][n] = (unsigned char)(2 * j + l);
+            n++;
+          }
+        }
+      g_ge.mm_n[s] = (unsigned char)min(n, 255);
+    }
+  __syncthreads();
 }
 
 __device__ __forceinline__ void ge_slot_load(const CoD& D, int k, double& x, double& y, double& z, double& vx,
@@ -1823,8 +1911,13 @@ __device__ __forceinline__ void ge_sphere_force(const CoP& P, int s) {
   lj_mot_wall(P, g_ge.dim[DZ][s], i, fw, wp);
   double fx = 0.0 - g_ge.fin[4 * s + 0], fy = 0.0 - g_ge.fin[4 * s + 1], fz = 0.0 - g_ge.fin[4 * s + 2];
   double tp = 0.0 + g_ge.fin[4 * s + 3];
-  for (int j = 0; j < nd; j++)
-    for (int l = 0; l < 2; l++) {
+  // partners beyond the sphere-sphere cutoff add exact zeros; only the
+  // candidates within it (plus the guard each may move) are visited, in order
+  const int nc = g_ge.mm_n[s];
+  const int nj = nc <= MM_CAP ? nc : 2 * nd;
+  for (int q = 0; q < nj; q++) {
+      const int jl = nc <= MM_CAP ? (int)g_ge.mm_ids[s][q] : q;
+      const int j = jl >> 1, l = jl & 1;
       if (j == k) continue;
       const int b = j + l * nd;
       const int kind = (i == 0 && l == 0) ? 0 : (i == 1 && l == 1) ? 2 : 1;
@@ -1883,7 +1976,7 @@ __device__ __attribute__((noinline)) void ge_dimers(const CoP& P, const CoD& D, 
   if (end_step) {
     for (int s = tid; s < 2 * nd; s += GE_T) ge_sphere_force(P, s);
     __syncthreads();
-    if (tid == 0) {
+    if (tid == 0 && blockIdx.x == 0) {
       double tpot = 0.0;
       for (int k = 0; k < nd; k++)
         for (int i = 0; i < 2; i++) tpot = tpot + g_ge.tpp[k + i * nd];
@@ -1910,7 +2003,8 @@ __device__ __attribute__((noinline)) void ge_dimers(const CoP& P, const CoD& D, 
     const int rb = (maxv > P.hs2 || g_ge.tc == g_ge.tp || g_ge.cstep % P.MPCtstep == 0);
     g_ge.rebuild = rb;
     if (rb) {
-      if (g_ge.nev < MAXEV) D.sc->events[g_ge.nev] = g_ge.tc; else raise_error(err, ERR_EVENTS);
+      if (g_ge.nev >= MAXEV) raise_error(err, ERR_EVENTS);
+      else if (blockIdx.x == 0) D.sc->events[g_ge.nev] = g_ge.tc;
       g_ge.nev += 1;
     }
   }
@@ -1924,77 +2018,84 @@ __device__ __attribute__((noinline)) void ge_dimers(const CoP& P, const CoD& D, 
     }
 }
 
-// workgroup 0 publishes the dimer state and the decision; the others read them
-__device__ __forceinline__ void ge_publish(const CoP& P, const CoD& D, const GeD& E) {
-  const int nd = P.nd;
-  if (blockIdx.x == 0) {
-    for (int j = threadIdx.x; j < 12 * 2 * nd; j += GE_T) {
-      const int f = j / (2 * nd), s = j % (2 * nd);
-      D.dim[f * MAXSG + s] = g_ge.dim[f][s];
-    }
-    if (threadIdx.x == 0) {
-      GeSync* y = E.sy;
-      y->rebuild = g_ge.rebuild; y->tc = g_ge.tc; y->tp = g_ge.tp; y->cstep = g_ge.cstep; y->nev = g_ge.nev;
-    }
-  }
-  ge_sync(E, &D.sc->err);
-  if (blockIdx.x != 0) {
-    for (int j = threadIdx.x; j < 12 * 2 * nd; j += GE_T) {
-      const int f = j / (2 * nd), s = j % (2 * nd);
-      g_ge.dim[f][s] = __hip_atomic_load(&D.dim[f * MAXSG + s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    if (threadIdx.x == 0) {
-      const GeSync* y = E.sy;
-      g_ge.rebuild = __hip_atomic_load(&y->rebuild, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      g_ge.tc = __hip_atomic_load(&y->tc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      g_ge.tp = __hip_atomic_load(&y->tp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      g_ge.cstep = __hip_atomic_load(&y->cstep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      g_ge.nev = __hip_atomic_load(&y->nev, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
-  __syncthreads();
-}
-
-// per-workgroup sums of the wave accumulators into part[wg] (and the max
-// displacement), then workgroup 0 adds the workgroups' sums in order
-__device__ __forceinline__ double ge_reduce(const CoP& P, const GeD& E, double mx) {
-  const int nd = P.nd, nv = 4 * 2 * nd;
+// Every workgroup integrates every dimer (the same arithmetic on the same
+// totals, so all hold the same dimer state and decisions) after one barrier
+// per MD step.  A workgroup publishes the per-sphere sums of its entries only
+// for the spheres it holds members of, with a bit in that sphere's word of
+// workgroups; the totals add the published sums in workgroup order.  Sums,
+// bits and the max displacement are double-buffered by step parity: a buffer
+// is rewritten two barriers after it was read, and a workgroup clears its
+// bits of a buffer after the barrier that follows their readers.
+__device__ __forceinline__ void ge_post(const CoP& P, const GeD& E, double mx, int par) {
+  const int nd = P.nd, G = (int)gridDim.x;
   mx = wave_max(mx);
   if ((threadIdx.x & 63) == 0) g_ge.red[threadIdx.x >> 6] = mx;
   __syncthreads();
-  double* part = E.part + (size_t)blockIdx.x * (4 * MAXSG + 1);
-  for (int v = threadIdx.x; v < nv; v += GE_T) {
-    double a = g_ge.wacc[0][v];
-    g_ge.wacc[0][v] = 0.0;
-    for (int w = 1; w < GE_W; w++) { a = a + g_ge.wacc[w][v]; g_ge.wacc[w][v] = 0.0; }
-    part[v] = a;
+  double* part = E.part + ((size_t)par * GE_GMAX + blockIdx.x) * (4 * MAXSG + 1);
+  unsigned* tb = E.touch + (size_t)par * MAXSG * (GE_GMAX / 32);
+  for (int sph = threadIdx.x; sph < 2 * nd; sph += GE_T) {
+    double a[4];
+    bool any = false;
+    for (int c = 0; c < 4; c++) {
+      const int v = 4 * sph + c;
+      double t = g_ge.wacc[0][v];
+      g_ge.wacc[0][v] = 0.0;
+      for (int w = 1; w < GE_W; w++) { t = t + g_ge.wacc[w][v]; g_ge.wacc[w][v] = 0.0; }
+      a[c] = t;
+      any = any || t != 0.0;
+    }
+    if (any) {
+      for (int c = 0; c < 4; c++) part[4 * sph + c] = a[c];
+      atomicOr(&tb[sph * (GE_GMAX / 32) + (blockIdx.x >> 5)], 1u << (blockIdx.x & 31));
+      atomicOr(&g_ge.touched[par][sph >> 5], 1u << (sph & 31));
+    }
   }
   if (threadIdx.x == 0) {
     double m = 0.0;
     for (int w = 0; w < GE_W; w++) m = fmax(m, g_ge.red[w]);
     part[4 * MAXSG] = m;
   }
-  return 0.0;
+  (void)G;
 }
-
-__device__ __forceinline__ double ge_gather_sums(const CoP& P, const GeD& E) {
-  // workgroup 0, after the barrier
+// after the barrier: totals of every sphere into fin, this workgroup's bits of
+// the other parity cleared; returns the system's max displacement
+__device__ __forceinline__ double ge_totals(const CoP& P, const GeD& E, int par) {
   const int nd = P.nd, nv = 4 * 2 * nd, G = (int)gridDim.x;
+  const double* part = E.part + (size_t)par * GE_GMAX * (4 * MAXSG + 1);
+  const unsigned* tb = E.touch + (size_t)par * MAXSG * (GE_GMAX / 32);
   for (int v = threadIdx.x; v < nv; v += GE_T) {
+    const int sph = v >> 2;
     double a = 0.0;
-    for (int w = 0; w < G; w++)
-      a = a + __longlong_as_double((long long)__hip_atomic_load(
-                  (unsigned long long*)&E.part[(size_t)w * (4 * MAXSG + 1) + v], __ATOMIC_RELAXED,
-                  __HIP_MEMORY_SCOPE_AGENT));
+    for (int wd = 0; wd < GE_GMAX / 32; wd++) {
+      unsigned bits = __hip_atomic_load(&tb[sph * (GE_GMAX / 32) + wd], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      while (bits) {
+        const int w = wd * 32 + __ffs(bits) - 1;
+        bits &= bits - 1u;
+        a = a + __longlong_as_double((long long)__hip_atomic_load(
+                    (unsigned long long*)&part[(size_t)w * (4 * MAXSG + 1) + v], __ATOMIC_RELAXED,
+                    __HIP_MEMORY_SCOPE_AGENT));
+      }
+    }
     g_ge.fin[v] = a;
   }
   double m = 0.0;
-  for (int w = 0; w < G; w++)
+  for (int w = threadIdx.x; w < G; w += GE_T)
     m = fmax(m, __longlong_as_double((long long)__hip_atomic_load(
-                    (unsigned long long*)&E.part[(size_t)w * (4 * MAXSG + 1) + 4 * MAXSG], __ATOMIC_RELAXED,
+                    (unsigned long long*)&part[(size_t)w * (4 * MAXSG + 1) + 4 * MAXSG], __ATOMIC_RELAXED,
                     __HIP_MEMORY_SCOPE_AGENT)));
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) g_ge.red[threadIdx.x >> 6] = m;
+  // this workgroup's bits of the other buffer (read before this barrier) are cleared
+  unsigned* to = E.touch + (size_t)(par ^ 1) * MAXSG * (GE_GMAX / 32);
+  for (int sph = threadIdx.x; sph < 2 * nd; sph += GE_T)
+    if ((g_ge.touched[par ^ 1][sph >> 5] >> (sph & 31)) & 1u)
+      atomicAnd(&to[sph * (GE_GMAX / 32) + (blockIdx.x >> 5)], ~(1u << (blockIdx.x & 31)));
   __syncthreads();
-  return m;
+  if (threadIdx.x < (MAXSG + 31) / 32) g_ge.touched[par ^ 1][threadIdx.x] = 0u;
+  double r = 0.0;
+  for (int w = 0; w < GE_W; w++) r = fmax(r, g_ge.red[w]);
+  __syncthreads();
+  return r;
 }
 
 __global__ __launch_bounds__(GE_T) void co_ge_engine(CoP P, CoD D, GeD E, int nsteps, int mode) {
@@ -2006,6 +2107,12 @@ __global__ __launch_bounds__(GE_T) void co_ge_engine(CoP P, CoD D, GeD E, int ns
     g_ge.dim[f][s] = D.dim[f * MAXSG + s];
   }
   for (int j = tid; j < GE_W * 4 * MAXSG; j += GE_T) g_ge.wacc[j / (4 * MAXSG)][j % (4 * MAXSG)] = 0.0;
+  if (tid < 8) g_ge.tm[tid] = 0ull;
+  if (tid < 2 * ((MAXSG + 31) / 32)) g_ge.touched[tid / ((MAXSG + 31) / 32)][tid % ((MAXSG + 31) / 32)] = 0u;
+  int par = 0;
+  const unsigned long long t_start = __builtin_amdgcn_s_memrealtime();
+  __syncthreads();
+  ge_build_grids(P);
   const int nn = min(sc->near_num, D.near_cap);
   const int chunk = (nn + G - 1) / G;
   if (tid == 0) {
@@ -2089,13 +2196,11 @@ __global__ __launch_bounds__(GE_T) void co_ge_engine(CoP P, CoD D, GeD E, int ns
       mx = fmax(mx, ms);
       ge_slot_store(D, k, x, y, z, vx, vy, vz, fl, ms);
     }
-    ge_reduce(P, E, mx);
+    ge_post(P, E, mx, par);
     ge_sync(E, err);
-    if (blockIdx.x == 0) {
-      const double m = ge_gather_sums(P, E);
-      ge_dimers(P, D, E, false, true, m);
-    }
-    ge_publish(P, D, E);
+    const double m = ge_totals(P, E, par);
+    par ^= 1;
+    ge_dimers(P, D, E, false, true, m);
   } else {
     if (tid == 0) g_ge.rebuild = 1;
     __syncthreads();
@@ -2104,6 +2209,8 @@ __global__ __launch_bounds__(GE_T) void co_ge_engine(CoP P, CoD D, GeD E, int ns
   for (int t = 1; t <= nsteps; t++) {
     const bool last = (t == nsteps);
     if (g_ge.rebuild) {
+      if (tid == 0) g_ge.tm[4] += 1;
+      GE_T0();
       if (!(resume && last)) {
         const bool stop = (mode & M_STOP) && last;
         ge_rebuild(P, D, E, true, true, !stop);
@@ -2117,16 +2224,28 @@ __global__ __launch_bounds__(GE_T) void co_ge_engine(CoP P, CoD D, GeD E, int ns
       } else {
         ge_rebuild(P, D, E, false, false, true);
       }
+      GE_T1(1);
     }
-    const double mx = ge_particles(P, D, E, last);
-    __syncthreads();
-    ge_reduce(P, E, mx);
+    {
+      GE_T0();
+      const double mx = ge_particles(P, D, E, last);
+      __syncthreads();
+      ge_post(P, E, mx, par);
+      GE_T1(2);
+    }
     ge_sync(E, err);
-    if (blockIdx.x == 0) {
-      const double m = ge_gather_sums(P, E);
-      ge_dimers(P, D, E, true, !last, m);
+    double m;
+    {
+      GE_T0();
+      m = ge_totals(P, E, par);
+      par ^= 1;
+      GE_T1(6);
     }
-    if (!last) ge_publish(P, D, E);
+    {
+      GE_T0();
+      ge_dimers(P, D, E, true, !last, m);
+      GE_T1(3);
+    }
   }
 
   // end of the interval: census exclusion, inside list as particle indices,
@@ -2155,6 +2274,11 @@ __global__ __launch_bounds__(GE_T) void co_ge_engine(CoP P, CoD D, GeD E, int ns
   __syncthreads();
   scatter_chunk();
   save_scalars(true);
+  if (blockIdx.x == 0 && tid == 0) {
+    sc->prof[0] += __builtin_amdgcn_s_memrealtime() - t_start;
+    sc->prof[1] += g_ge.tm[1]; sc->prof[2] += g_ge.tm[2]; sc->prof[3] += g_ge.tm[3]; sc->prof[4] += g_ge.tm[4];
+    sc->prof[5] += g_ge.tm[5]; sc->prof[6] += g_ge.tm[6]; sc->prof[7] += 1;
+  }
 }
 
 // after the clusters' launch: their sphere fields and scalars back into the
@@ -2852,7 +2976,8 @@ extern "C" int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg) {
     E.in_memb4 = dalloc<unsigned>(e, icap);
     E.in_fp4 = dalloc<double>(e, (size_t)icap * 12);
     E.in_pot4 = dalloc<double>(e, (size_t)icap * 4);
-    E.part = dalloc<double>(e, (size_t)GE_GMAX * (4 * MAXSG + 1));
+    E.part = dalloc<double>(e, (size_t)2 * GE_GMAX * (4 * MAXSG + 1));
+    E.touch = dalloc<unsigned>(e, (size_t)2 * MAXSG * (GE_GMAX / 32));
     E.in_cap = icap;
     e->geG = std::min(GE_GMAX, std::max(2, 2 * e->nd));
   }
@@ -2908,6 +3033,7 @@ extern "C" int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg) {
     G.sc = D.sc; G.react_box = D.react_box; G.near_list = D.near_list; G.near_bits = D.near_bits;
     G.status = D.status; G.near_cap = D.near_cap;
     HIPCHK(hipMemset(e->E.sy, 0, sizeof(GeSync)));
+    HIPCHK(hipMemset(e->E.touch, 0, (size_t)2 * MAXSG * (GE_GMAX / 32) * 4));
     HIPCHK(hipMemset(e->E.in_fp4, 0, (size_t)e->E.in_cap * 12 * 8));
     HIPCHK(hipMemset(e->E.in_pot4, 0, (size_t)e->E.in_cap * 4 * 8));
     HIPCHK(hipMemset(G.in_msd, 0, (size_t)e->E.in_cap * 8));
@@ -3040,11 +3166,16 @@ static void co_near_list(mpcx_co* e) {
                    e->P.NT, &e->D.sc->near_num, &e->D.sc->err);
 }
 
-__global__ void ge_reset(GeSync* sy) { sy->count = 0u; sy->abort = 0; }
+// before every launch of the global engine: barrier state and the
+// workgroup bits of the sphere sums (the last step's bits stay set)
+__global__ void ge_reset(GeSync* sy, unsigned* touch) {
+  if (threadIdx.x == 0) { sy->count = 0u; sy->abort = 0; }
+  for (int j = threadIdx.x; j < 2 * MAXSG * (GE_GMAX / 32); j += blockDim.x) touch[j] = 0u;
+}
 
 static hipError_t launch_near(mpcx_co* e, int nsteps, int mode) {
   if (e->near_mode == 2) {
-    hipLaunchKernelGGL(ge_reset, dim3(1), dim3(1), 0, e->st, e->E.sy);
+    hipLaunchKernelGGL(ge_reset, dim3(1), dim3(256), 0, e->st, e->E.sy, e->E.touch);
     const CoP P = e->P;
     CoD G = e->Dg;
     {   // the particle arrays the system holds now (a sort swaps them with its buffers)
