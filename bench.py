@@ -169,6 +169,30 @@ def many_dimers(name, steps=20, warmup=5, seed=4357787):
             "near_engine": "global (cooperative workgroups)", "dtype": "f64", "rng": "device Philox4x32-10"}
 
 
+def selkov(steps=20, warmup=5, seed=8154543):
+    """selkov_code at 50^3 cells, 10 per cell (1.25e6 particles), one dimer,
+    the reference input's rates, tau 0.1 = 20 x 0.005, device Philox draws."""
+    from main_amd import systems as S
+    from main_amd._lib import device_synchronize
+    from main_amd.selkov import SelkovEngine
+    p = S.selkov_params(L=50.0, rho=10.0, NG=100000)
+    s = S.selkov_initial_state(p, seed=seed)
+    eng = SelkovEngine(p, rng_mode=1, seed=seed)
+    eng.set_state(s, cstep=0)
+    eng.run(warmup)
+    device_synchronize()
+    t0 = time.perf_counter()
+    eng.run(steps)
+    device_synchronize()
+    dt = time.perf_counter() - t0
+    kms = eng.kernel_ms()
+    eng.close()
+    print(f"# selkov 50^3: {dt * 1000.0 / steps:.3f} ms/interval", file=sys.stderr, flush=True)
+    return {"particles": p["NT"], "md_steps_per_collision": p["MPCtstep"], "ms_per_interval": dt * 1000.0 / steps,
+            "particle_updates_per_s": p["NT"] * steps / dt, "kernel_ms": kms, "dtype": "f64",
+            "rng": "device Philox4x32-10"}
+
+
 def read_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed counter summary."""
     if not os.path.exists(TRAFFIC_FILE):
@@ -252,6 +276,7 @@ def main():
         return
     if args.many_dimers_only:
         out = {name: many_dimers(name, args.steps, args.warmup) for name in MANY_DIMER}
+        out["selkov_50cube"] = selkov(args.steps, args.warmup)
         print(json.dumps({"metric": "particle-updates/s (MD+MPC), many dimers", **out}), flush=True)
         return
     if args.probe_copy:
@@ -315,6 +340,7 @@ def main():
         out["batched"] = batched(args.batched, steps=10, warmup=2)
     if args.many_dimers:
         out["many_dimers"] = {name: many_dimers(name) for name in MANY_DIMER}
+        out["selkov_50cube"] = selkov()
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(args.size)
