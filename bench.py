@@ -193,6 +193,48 @@ def selkov(steps=20, warmup=5, seed=8154543):
             "rng": "device Philox4x32-10"}
 
 
+# algorithmic bytes per particle of the split-precision kernels (29-byte state: int32 cell, 3 x FP32
+# offsets, 3 x FP32 velocities, int8 species)
+ALGO_BYTES_SPLIT = {
+    "co_bulk": 53.0,     # sp_bulk: cell r/w 8, offsets r/w 24, species 1, velocities read 12 (far replay),
+                         # collision cell 4 + rank 2, distance byte 1, status 1
+    "co_rotate": 35.0,   # sp_rotate: velocities r/w 24, collision cell 4, cell record share 6, distance byte 1
+    "co_collide": 23.0,  # sp_collide: velocities 12, bucket 4, cell record + counts share 7
+    "co_scatter": 14.0,
+}
+
+
+def split_headline(L=100, steps=60, warmup=20, seed=4357787):
+    """The headline system in split-precision storage (same physics, device Philox)."""
+    from main_amd import systems as S
+    from main_amd._lib import device_synchronize
+    from main_amd.collective import CollectiveEngine
+    from main_amd.tour import cell_tour
+    p = headline_params(L)
+    d = S.collective_dimers(p, seed=seed)
+    eng = CollectiveEngine(p, rng_mode=1, seed=seed, split=True)
+    eng.set_dimers(d, tour=cell_tour(L, L, L))
+    eng.init_solvent(seed)
+    eng.setup(do_sort=True)
+    eng.run(warmup)
+    device_synchronize()
+    t0 = time.perf_counter()
+    eng.run(steps)
+    device_synchronize()
+    dt = time.perf_counter() - t0
+    kms = eng.kernel_ms()
+    eng.close()
+    NT = p["NT"]
+    hbm = {k: v for k, v in kms.items() if k in ALGO_BYTES_SPLIT}
+    dom = max(hbm, key=hbm.get)
+    ach = ALGO_BYTES_SPLIT[dom] * NT / (hbm[dom] * 1e-3) / 1e9
+    print(f"# split {L}^3: {dt * 1000.0 / steps:.3f} ms/interval {kms}", file=sys.stderr, flush=True)
+    return {"particles": NT, "ms_per_interval": dt * 1000.0 / steps, "particle_updates_per_s": NT * steps / dt,
+            "kernel_ms": kms, "dtype": "f32 solvent (cell + offsets), f64 near set and dimer",
+            "roofline": {"kernel": dom.replace("co_", "sp_"), "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": ach / HBM_PEAK_GBS, "algorithmic_bytes_per_particle": ALGO_BYTES_SPLIT[dom]}}
+
+
 def read_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed counter summary."""
     if not os.path.exists(TRAFFIC_FILE):
@@ -256,6 +298,8 @@ def main():
     ap.add_argument("--batched-only", action="store_true", help="measure only the replica ensemble")
     ap.add_argument("--many-dimers", type=int, default=1, help="measure the many-dimer systems beside the headline")
     ap.add_argument("--many-dimers-only", action="store_true")
+    ap.add_argument("--split", type=int, default=1, help="measure the split-precision headline beside the FP64 one")
+    ap.add_argument("--split-only", action="store_true")
     args = ap.parse_args()
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc)
@@ -273,6 +317,10 @@ def main():
     if args.batched_only:
         print(json.dumps({"metric": "replicas/hour (2000-collision runs)", **batched(args.batched, args.steps,
                                                                                    args.warmup)}), flush=True)
+        return
+    if args.split_only:
+        print(json.dumps({"metric": "particle-updates/s (MD+MPC), split precision",
+                          **split_headline(args.size, args.steps, args.warmup)}), flush=True)
         return
     if args.many_dimers_only:
         out = {name: many_dimers(name, args.steps, args.warmup) for name in MANY_DIMER}
@@ -338,6 +386,8 @@ def main():
     }
     if args.batched > 0:
         out["batched"] = batched(args.batched, steps=10, warmup=2)
+    if args.split:
+        out["split_precision"] = split_headline(args.size)
     if args.many_dimers:
         out["many_dimers"] = {name: many_dimers(name) for name in MANY_DIMER}
         out["selkov_50cube"] = selkov()

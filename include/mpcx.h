@@ -177,7 +177,7 @@ int mpcx_co_get_lists(mpcx_co* e, int64_t max_nb, int64_t* nb, int64_t* nbnum, d
 /* Near-engine layout for two or more dimers (call before mpcx_co_setup).
  *  mode 0: one workgroup for the whole system (up to four dimers; the default
  *          for one or two dimers);
- *  mode 1: dimers whose near regions can meet within an interval are grouped
+ *  mode 1 (two or more dimers): dimers whose near regions can meet within an interval are grouped
  *          into clusters of at most four, one workgroup each;
  *  mode 2: the system's near set split over cooperative workgroups, any number
  *          of dimers at any density (the default above two dimers).
@@ -186,6 +186,15 @@ int mpcx_co_get_lists(mpcx_co* e, int64_t max_nb, int64_t* nb, int64_t* nbnum, d
  * counterpart in the reference (its OpenMP loops over dimers and particles in
  * evolve_mod.f95 run in one address space). */
 int mpcx_co_set_near_mode(mpcx_co* e, int32_t mode);
+/* Workgroups of mode 2 (1..128; default 2 per dimer, at least 2).  Mode 2 also
+ * runs one-dimer systems (its near set split over the workgroups). */
+int mpcx_co_set_near_workgroups(mpcx_co* e, int32_t G);
+/* split = 1: split-precision solvent (the 0-based unshifted cell and FP32
+ * offsets in it, FP32 velocities, 8-bit species; far particles stream, bin,
+ * collide with Kahan-compensated FP32 cell sums and rotate in FP32; near
+ * particles and the dimers stay FP64).  Device Philox draws only; call before
+ * mpcx_co_setup.  No counterpart in the reference (FP64 throughout). */
+int mpcx_co_set_precision(mpcx_co* e, int32_t split);
 /* Collision-cell data of the last collision: (mpcnumcell) arrays, 0-based
  * cell of each particle, and unshifted census counts (3, vol). */
 int mpcx_co_get_cells(mpcx_co* e, double* vcm_uv /* (mpcnumcell, 8) */, int32_t* cell_of,

@@ -73,7 +73,7 @@ class CollectiveEngine:
     """Device-resident collective_code system (one GPU)."""
 
     def __init__(self, p: dict, rng_mode: int = 0, seed: int = 4357787, sphere_guard: float = 0.3,
-                 near_mode: int | None = None):
+                 near_mode: int | None = None, near_workgroups: int | None = None, split: bool = False):
         """near_mode (two or more dimers): 0 one near-engine workgroup for the
         system (up to four dimers), 1 one per cluster of dimers, 2 the near
         set over cooperative workgroups (default above two dimers); see
@@ -86,6 +86,12 @@ class CollectiveEngine:
         self.rng_mode = rng_mode
         if near_mode is not None:
             check(lib().mpcx_co_set_near_mode(self.h, C.c_int32(int(near_mode))), "mpcx_co_set_near_mode")
+        if split:
+            check(lib().mpcx_co_set_precision(self.h, C.c_int32(1)), "mpcx_co_set_precision")
+        self.split = split
+        if near_workgroups is not None:
+            check(lib().mpcx_co_set_near_workgroups(self.h, C.c_int32(int(near_workgroups))),
+                  "mpcx_co_set_near_workgroups")
 
     def close(self):
         if self.h:

@@ -2749,6 +2749,378 @@ __global__ __launch_bounds__(256) void sort_gather(CoP P, CoD D, const int* sb, 
 
 }  // namespace mpcx
 
+namespace mpcx {
+// ------------------------------------------------------------- split precision
+// Solvent stored as the 0-based unshifted cell (x + Lx y + Lx Ly z) with FP32
+// offsets in [0,1) inside it, FP32 velocities and an 8-bit species: 29 bytes
+// read and written per particle and interval instead of 52.  Far particles
+// stream, bounce, bin, collide and rotate in FP32 cell coordinates (no large
+// coordinates ever formed, walls handled relative to the wall's cell); the
+// per-cell velocity sums are Kahan-compensated FP32 in ascending particle
+// order.  Near particles and the dimers keep the FP64 near engines: their
+// state is expanded into the FP64 staging arrays before each near-engine
+// launch and stored back after it.
+struct SpD {
+  int* cell;
+  float *ox, *oy, *oz, *vx, *vy, *vz;
+  signed char* flag;
+};
+
+__device__ __forceinline__ void sp_split(double x, double y, double z, const CoP& P, int& cell, float& ox, float& oy,
+                                         float& oz) {
+  int ix = (int)floor(x), iy = (int)floor(y), iz = (int)floor(z);
+  float fx = (float)(x - (double)ix), fy = (float)(y - (double)iy), fz = (float)(z - (double)iz);
+  if (fx >= 1.0f) { fx = 0.0f; ix += 1; }
+  if (fy >= 1.0f) { fy = 0.0f; iy += 1; }
+  if (fz >= 1.0f) { fz = 0.0f; iz += 1; }
+  if (ix >= P.Lx) ix -= P.Lx; else if (ix < 0) ix += P.Lx;
+  if (iy >= P.Ly) iy -= P.Ly; else if (iy < 0) iy += P.Ly;
+  cell = ix + P.Lx * (iy + P.Ly * iz);
+  ox = fx; oy = fy; oz = fz;
+}
+__device__ __forceinline__ void sp_cell_xyz(const CoP& P, int cell, int& ix, int& iy, int& iz) {
+  ix = cell % P.Lx;
+  const int r = cell / P.Lx;
+  iy = r % P.Ly;
+  iz = r / P.Ly;
+}
+
+// FP64 staging <-> split storage, for all particles or the near list
+__global__ __launch_bounds__(256) void sp_store(CoP P, CoD D, SpD S, int near_only) {
+  const long long n = near_only ? (long long)min(D.sc->near_num, D.near_cap) : (long long)P.NT;
+  for (long long j = (long long)blockIdx.x * 256 + threadIdx.x; j < n; j += (long long)gridDim.x * 256) {
+    const int i = near_only ? D.near_list[j] : (int)j;
+    int c;
+    float ox, oy, oz;
+    sp_split(D.x[i], D.y[i], D.z[i], P, c, ox, oy, oz);
+    S.cell[i] = c; S.ox[i] = ox; S.oy[i] = oy; S.oz[i] = oz;
+    S.vx[i] = (float)D.vx[i]; S.vy[i] = (float)D.vy[i]; S.vz[i] = (float)D.vz[i];
+    S.flag[i] = (signed char)D.flag[i];
+  }
+}
+__global__ __launch_bounds__(256) void sp_load(CoP P, CoD D, SpD S, int near_only) {
+  const long long n = near_only ? (long long)min(D.sc->near_num, D.near_cap) : (long long)P.NT;
+  for (long long j = (long long)blockIdx.x * 256 + threadIdx.x; j < n; j += (long long)gridDim.x * 256) {
+    const int i = near_only ? D.near_list[j] : (int)j;
+    int ix, iy, iz;
+    sp_cell_xyz(P, S.cell[i], ix, iy, iz);
+    D.x[i] = (double)ix + (double)S.ox[i]; D.y[i] = (double)iy + (double)S.oy[i]; D.z[i] = (double)iz + (double)S.oz[i];
+    D.vx[i] = (double)S.vx[i]; D.vy[i] = (double)S.vy[i]; D.vz[i] = (double)S.vz[i];
+    D.flag[i] = (int)S.flag[i];
+  }
+}
+
+__device__ __forceinline__ int sp_wrap(int i, int L) { return i >= L ? i - L : (i < 0 ? i + L : i); }
+__device__ __forceinline__ void sp_carry(int& i, float& o, int L) {
+  const float k = floorf(o);
+  o = o - k;
+  if (o >= 1.0f) { o = 0.0f; i += 1; }
+  i = sp_wrap(i + (int)k, L);
+}
+// evolve_mod.f95:259-312 / :474-527 in cell coordinates (FP32): tc MD steps
+// of free flight, bounce-back at the z walls, A<->B on a reactive patch of z = 0
+__device__ __forceinline__ bool sp_stream(int& ix, int& iy, int& iz, float& ox, float& oy, float& oz, float& vx,
+                                          float& vy, float& vz, int& flag, float dtc, const CoP& P,
+                                          const int* react_box, int* err) {
+  ox = ox + vx * dtc;
+  oy = oy + vy * dtc;
+  oz = oz + vz * dtc;
+  sp_carry(ix, ox, P.Lx);
+  sp_carry(iy, oy, P.Ly);
+  const float kz = floorf(oz);
+  const int nz = iz + (int)kz;
+  if (nz < 0) {   // below z = 0: z = iz + oz < 0
+    const float z = (float)iz + oz;
+    const float t = z / vz;
+    float ax = ox - vx * t, ay = oy - vy * t;
+    int hx = ix, hy = iy;
+    sp_carry(hx, ax, P.Lx);
+    sp_carry(hy, ay, P.Ly);
+    const int hit = hx + P.Lx * hy + 1;
+    if (hit < 1 || hit > P.nrb) raise_error(err, ERR_CELL_RANGE);
+    else if (react_box[hit - 1] == 1) {
+      if (P.react_dir == 0 && flag == 0) flag = 1;
+      else if (P.react_dir == 1 && flag == 1) flag = 0;
+    }
+    vx = -vx; vy = -vy; vz = -vz;
+    ox = ox + (2.0f * vx) * t; oy = oy + (2.0f * vy) * t;
+    float zn = z + (2.0f * vz) * t;
+    sp_carry(ix, ox, P.Lx);
+    sp_carry(iy, oy, P.Ly);
+    const float kk = floorf(zn);
+    iz = (int)kk; oz = zn - kk;
+    if (oz >= 1.0f) { oz = 0.0f; iz += 1; }
+    return true;
+  }
+  const float over = (float)(nz - P.Lz) + (oz - kz);   // z - Lz
+  if (nz >= P.Lz && over > 0.0f) {
+    const float t = over / vz;
+    vx = -vx; vy = -vy; vz = -vz;
+    ox = ox + (2.0f * vx) * t; oy = oy + (2.0f * vy) * t;
+    const float zr = over + (2.0f * vz) * t;   // z - Lz after the bounce
+    sp_carry(ix, ox, P.Lx);
+    sp_carry(iy, oy, P.Ly);
+    const float kk = floorf(zr);
+    iz = P.Lz + (int)kk; oz = zr - kk;
+    if (oz >= 1.0f) { oz = 0.0f; iz += 1; }
+    return true;
+  }
+  iz = nz;
+  oz = oz - kz;
+  if (oz >= 1.0f) { oz = 0.0f; iz += 1; }
+  return false;
+}
+
+// co_bulk in split storage
+__global__ __launch_bounds__(256) void sp_bulk(CoP P, CoD D, SpD S, int do_stream, int do_bin) {
+  const int NT = P.NT;
+  const int nev = D.sc->n_ev;
+  __shared__ int ev[MAXEV];
+  __shared__ double sph[3][MAXSG];
+  for (int j = threadIdx.x; j < nev && j < MAXEV; j += blockDim.x) ev[j] = D.sc->events[j];
+  for (int j = threadIdx.x; j < 2 * P.nd; j += blockDim.x) {
+    sph[0][j] = D.sc->x0[j]; sph[1][j] = D.sc->y0[j]; sph[2][j] = D.sc->z0[j];
+  }
+  __syncthreads();
+  int ev_total = 0;
+  for (int e = 0; e < nev && e < MAXEV; e++) ev_total += ev[e];
+  double xs = 0, ys = 0, zs = 0;
+  if (do_bin) collision_shift(P, D, xs, ys, zs);
+  const float fxs = (float)xs, fys = (float)ys, fzs = (float)zs;
+  const float mdt = (float)P.MDt;
+  const int lane = threadIdx.x & 63;
+  const long long step = (long long)gridDim.x * blockDim.x;
+  for (long long base = (long long)blockIdx.x * blockDim.x; base < NT; base += step) {
+    const int i = (int)(base + threadIdx.x);
+    const bool valid = i < NT;
+    bool near = false;
+    int ix = 0, iy = 0, iz = 0, f = -1;
+    float ox = 0, oy = 0, oz = 0;
+    unsigned char st = 0;
+    if (valid) {
+      near = (D.near_bits[i >> 5] >> (i & 31)) & 1u;
+      st = D.status[i];
+      if (!near && st != 0) { D.status[i] = 0; st = 0; }
+      sp_cell_xyz(P, S.cell[i], ix, iy, iz);
+      ox = S.ox[i]; oy = S.oy[i]; oz = S.oz[i];
+      f = S.flag[i];
+      if (do_stream && !near && nev > 0) {
+        // one free flight over all the interval's steps: the reference streams
+        // such a particle lazily at each rebuild, the same path (a far particle
+        // moves at most |v| tau, about a cell, so it meets at most one wall)
+        float vx = S.vx[i], vy = S.vy[i], vz = S.vz[i];
+        const int f0 = f;
+        const bool bounced = sp_stream(ix, iy, iz, ox, oy, oz, vx, vy, vz, f, mdt * (float)ev_total, P, D.react_box,
+                                       &D.sc->err);
+        S.cell[i] = ix + P.Lx * (iy + P.Ly * iz);
+        S.ox[i] = ox; S.oy[i] = oy; S.oz[i] = oz;
+        if (bounced) { S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz; }
+        if (f != f0) S.flag[i] = (signed char)f;
+      }
+    }
+    if (!do_bin) continue;
+    bool ok = valid;
+    int t = 0;
+    if (valid) {   // shifted cell (mpc_rotation_mod.f95:116-145) from the cell and offsets
+      const int cx = sp_wrap(ix + (int)floorf(ox + fxs), P.Lx);
+      const int cy = sp_wrap(iy + (int)floorf(oy + fys), P.Ly);
+      const int cz = iz + (int)floorf(oz + fzs) + (zs < 0.0 ? 1 : 0);
+      const long long c = (long long)cx + (long long)P.Lx * cy + (long long)P.Lx * P.Ly * cz;
+      if (cz < 0 || c >= P.mpcnumcell) { raise_error(&D.sc->err, ERR_CELL_RANGE); ok = false; }
+      else t = D.mpc_pos[c];
+    }
+    int r = wave_atomic_rank(D.cnt, t, ok);
+    if (valid) {
+      if (ok && r >= BMAX) raise_error(&D.sc->err, ERR_BUCKET);
+      D.cell_of[i] = t;
+      D.rank_of[i] = ok ? (unsigned short)min(r, 65535) : (unsigned short)0xffff;
+      const double x = (double)ix + (double)ox, y = (double)iy + (double)oy, z = (double)iz + (double)oz;
+      double dmin = 1e30;
+      for (int s = 0; s < 2 * P.nd; s++) {
+        double a = pbc_half(x - sph[0][s], P.box[0]);
+        double b = pbc_half(y - sph[1][s], P.box[1]);
+        double c = z - sph[2][s];
+        dmin = fmin(dmin, sqrt(a * a + b * b + c * c) - P.full[s >= P.nd]);
+      }
+      // 1e-5 below the measured distance: the FP32 position must not make a particle look farther
+      double qd = floor(fmax(dmin - 1e-5, 0.0) * 32.0);
+      if (st & ST_INSIDE) qd = 0.0;
+      D.dq[i] = (unsigned char)fmin(qd, 255.0);
+    }
+    unsigned key = (unsigned)i;
+    bool cen = false;
+    int ck = 0;
+    if (valid && ok) {
+      bool skip = false;
+      if (near && (st & ST_INSIDE)) { key = (unsigned)NT + (unsigned)i; skip = (st & ST_EXCL) != 0; }
+      int sp = f == 0 ? 0 : f == 1 ? 1 : f == 2 ? 2 : -1;
+      if (!skip && sp >= 0) {
+        const long long u = (long long)ix + (long long)P.Lx * iy + (long long)P.Lx * P.Ly * iz;
+        if (u < 0 || u >= P.vol) raise_error(&D.sc->err, ERR_CELL_RANGE);
+        else { cen = true; ck = (int)(sp * P.vol + u); }
+      }
+    }
+    {
+      unsigned long long todo = __ballot(cen);
+      const unsigned long long outside = __ballot(cen && key < (unsigned)NT);
+      int gsize = 0;
+      unsigned gkey = 0;
+      bool leader = false;
+      while (todo) {
+        int ld = __ffsll((unsigned long long)todo) - 1;
+        int k = __shfl(ck, ld);
+        unsigned long long grp = __ballot(cen && ck == k);
+        unsigned long long go = grp & outside;
+        int src = __ffsll((unsigned long long)(go ? go : grp)) - 1;
+        unsigned mk = (unsigned)__shfl((int)key, src);
+        if (lane == ld) { leader = true; gsize = __popcll(grp); gkey = mk; }
+        todo &= ~grp;
+      }
+      if (leader) {
+        int old = atomicAdd(&D.tcount[ck], gsize);
+        if (old + gsize > CENSUS_MAX) raise_error(&D.sc->err, ERR_MEMBERS);
+        if (ck >= P.vol && ck < 2 * P.vol) atomicMin(&D.firstB[ck - P.vol], gkey);
+      }
+    }
+  }
+}
+
+// co_collide in split storage: Kahan-compensated FP32 sums in ascending order
+__global__ __launch_bounds__(CT) void sp_collide(CoP P, CoD D, SpD S) {
+  __shared__ int sidx[LB * CT];
+  const long long t = (long long)xcd_block(blockIdx.x, gridDim.x) * CT + threadIdx.x;
+  if (t < P.mpcnumcell) {
+    const int c = D.mpc_cell[t];
+    int n = D.cnt[t];
+    if (n > BMAX) n = BMAX;
+    const int* b = D.bucket + D.off[t];
+    int* idx;
+    int istride;
+    if (n <= LB) {
+      idx = sidx + threadIdx.x;
+      istride = CT;
+      for (int j = 0; j < n; j++) {
+        int v = b[j], k = j;
+        while (k > 0 && idx[(k - 1) * CT] > v) { idx[k * CT] = idx[(k - 1) * CT]; k--; }
+        idx[k * CT] = v;
+      }
+    } else {
+      idx = (int*)b;
+      istride = 1;
+      for (int j = 1; j < n; j++) {
+        int v = idx[j], k = j;
+        while (k > 0 && idx[k - 1] > v) { idx[k] = idx[k - 1]; k--; }
+        idx[k] = v;
+      }
+    }
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
+    for (int j = 0; j < n; j++) {
+      const int i = idx[j * istride];
+      const float yx = S.vx[i] - cx, yy = S.vy[i] - cy, yz = S.vz[i] - cz;
+      const float tx = sx + yx, ty = sy + yy, tz = sz + yz;
+      cx = (tx - sx) - yx; cy = (ty - sy) - yy; cz = (tz - sz) - yz;
+      sx = tx; sy = ty; sz = tz;
+    }
+    double mx = 0.0, my = 0.0, mz = 0.0;
+    if (n != 0) {
+      const float inv = 1.0f / (float)n;
+      mx = (double)(sx * inv); my = (double)(sy * inv); mz = (double)(sz * inv);
+    }
+    double u1, u2;
+    philox_uniform2(P.seed, 1, D.sc->coll, (unsigned long long)c, u1, u2);
+    double phi = P.twopi * u1;
+    double th = 2.0 * u2 - 1.0;
+    double st = sqrt(1.0 - th * th);
+    double2* rec = (double2*)(D.cellrec + t * 8);
+    rec[0] = make_double2(mx, my);
+    rec[1] = make_double2(mz, st * cos(phi));
+    rec[2] = make_double2(st * sin(phi), th);
+    rec[3] = make_double2((double)n, 0.0);
+  }
+  if (t < P.rxnnumcell) {
+    int nb = D.tcount[1 * P.vol + t];
+    double prob = D.ptab[min(nb, CENSUS_MAX)];
+    double r, q;
+    philox_uniform2(P.seed, 2, D.sc->coll, (unsigned long long)t, r, q);
+    if (0.0 < r && r <= prob) {
+      unsigned key = D.firstB[t];
+      int i = key >= (unsigned)P.NT ? (int)(key - (unsigned)P.NT) : (int)key;
+      S.flag[i] = 0;
+    }
+  }
+}
+
+// co_rotate in split storage (FP32 rotation) and the next near set
+__global__ __launch_bounds__(256) void sp_rotate(CoP P, CoD D, SpD S) {
+  float fmax2 = 0.0f;
+  const double tau = P.tau * (1.0 + 1e-9);
+  const float cr = (float)P.cosrot, sr = (float)P.sinrot, omc = (float)P.omc;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < P.NT; i += (long long)gridDim.x * 256) {
+    const int t = D.cell_of[i];
+    const double2* rec = (const double2*)(D.cellrec + (long long)t * 8);
+    const double2 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+    const float cx = (float)r0.x, cy = (float)r0.y, cz = (float)r1.x;
+    const float ux = (float)r1.y, uy = (float)r2.x, uz = (float)r2.y;
+    float vx = S.vx[i], vy = S.vy[i], vz = S.vz[i];
+    const float dx = vx - cx, dy = vy - cy, dz = vz - cz;
+    const float rx = (dx * (cr + (ux * ux) * omc) + dy * ((ux * uy) * omc - uz * sr)) + dz * ((ux * uz) * omc + uy * sr);
+    const float ry = (dx * ((ux * uy) * omc + uz * sr) + dy * (cr + (uy * uy) * omc)) + dz * ((uy * uz) * omc - ux * sr);
+    const float rz = (dx * ((ux * uz) * omc - uy * sr) + dy * ((uy * uz) * omc + ux * sr)) + dz * (cr + (uz * uz) * omc);
+    vx = cx + rx; vy = cy + ry; vz = cz + rz;
+    S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz;
+    const float v2 = (vx * vx + vy * vy) + vz * vz;
+    const double reach = sqrt((double)v2) * tau + P.guard + 1e-6;
+    if ((double)D.dq[i] * (1.0 / 32.0) <= reach) atomicOr(&D.near_bits[i >> 5], 1u << (i & 31));
+    else fmax2 = fmaxf(fmax2, v2);
+  }
+  double m = wave_max((double)fmax2);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double mm = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    // FP32 velocities: the far maximum is raised by a relative 1e-6 so time_penetrate stays conservative
+    atomicMax(&D.sc->far_maxv2, (unsigned long long)__double_as_longlong(mm * (1.0 + 1e-6)));
+  }
+}
+
+// Hilbert re-sort of the split arrays (sort_hilbert_mod.f95:50-92: stable bin
+// by the tour position of the unshifted cell, which split storage holds)
+__global__ __launch_bounds__(256) void sp_sort_bin(CoP P, CoD D, SpD S, const int* tourpos, int* scnt, int* key,
+                                                   unsigned short* rk) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < P.NT; i += (long long)gridDim.x * 256) {
+    const long long u = S.cell[i];
+    if (u < 0 || u >= P.vol || tourpos[u] < 0) { raise_error(&D.sc->err, ERR_CELL_RANGE); key[i] = 0; rk[i] = 0; continue; }
+    const int t = tourpos[u];
+    const int r = atomicAdd(&scnt[t], 1);
+    if (r >= BMAX) raise_error(&D.sc->err, ERR_BUCKET);
+    key[i] = t;
+    rk[i] = (unsigned short)min(r, 65535);
+  }
+}
+__global__ __launch_bounds__(256) void sp_sort_gather(CoP P, CoD D, SpD S, SpD N, const int* sb, unsigned char* nst,
+                                                      unsigned* nbits) {
+  long long first, end;
+  block_range(P.NT, gridDim.x, xcd_block(blockIdx.x, gridDim.x), first, end);
+  const int lane = threadIdx.x & 63;
+  for (long long base = first; base < end; base += 256) {
+    const long long j = base + threadIdx.x;
+    bool bit = false;
+    if (j < end) {
+      const int i = sb[j];
+      N.cell[j] = S.cell[i]; N.ox[j] = S.ox[i]; N.oy[j] = S.oy[i]; N.oz[j] = S.oz[i];
+      N.vx[j] = S.vx[i]; N.vy[j] = S.vy[i]; N.vz[j] = S.vz[i]; N.flag[j] = S.flag[i];
+      nst[j] = D.status[i];
+      bit = (D.near_bits[i >> 5] >> (i & 31)) & 1u;
+    }
+    unsigned long long m = __ballot(bit);
+    const long long w0 = (base + (threadIdx.x & ~63)) >> 5;
+    if (lane == 0) nbits[w0] = (unsigned)m;
+    if (lane == 32) nbits[w0 + 1] = (unsigned)(m >> 32);
+  }
+}
+}  // namespace mpcx
+
 // =====================================================================================
 // Host side
 // =====================================================================================
@@ -2785,11 +3157,14 @@ struct mpcx_co {
   // dimers), 1 one workgroup per cluster of dimers, 2 the system's near set
   // over G cooperative workgroups
   int near_mode = 0;
-  bool clustered = false, can_cluster = false;
+  bool clustered = false, can_cluster = false, has_ge = false;
   CluBase cb;
   CoD Dg;          // the system's descriptor with the global engine's entry arrays
   GeD E;
   int geG = 1;
+  // split precision: solvent in cell + FP32 offsets / FP32 velocities / 8-bit species
+  bool split = false;
+  SpD S, Salt;
 };
 
 template <typename T>
@@ -2835,6 +3210,44 @@ static int check_device_error(mpcx_co* e) {
     hipError_t _r = (x);                                                 \
     if (_r != hipSuccess) return set_error(MPCX_E_HIP, hipGetErrorString(_r)); \
   } while (0)
+
+// The global near engine's arrays (entries of the whole system): made at
+// creation above two dimers, or when the mode is selected.
+static int ge_alloc(mpcx_co* e) {
+  if (e->has_ge) return 0;
+  CoD& D = e->D;
+  const long long NT = e->P.NT;
+  CoD& G = e->Dg;
+  G = D;   // particle arrays, scalars and cell data are shared (refreshed at each launch)
+  const size_t nq = (size_t)D.near_cap;
+  const int icap = (int)std::min<long long>(NT, (long long)NE_CAP + 8192LL * e->nd);
+  G.ns_x = dalloc<double>(e, nq); G.ns_y = dalloc<double>(e, nq); G.ns_z = dalloc<double>(e, nq);
+  G.ns_vx = dalloc<double>(e, nq); G.ns_vy = dalloc<double>(e, nq); G.ns_vz = dalloc<double>(e, nq);
+  G.ns_flag = dalloc<int>(e, nq); G.ns_stat = dalloc<unsigned char>(e, nq);
+  G.in_cap = icap;
+  G.in_list = dalloc<int>(e, icap); G.in_plist = dalloc<int>(e, icap); G.in_msd = dalloc<double>(e, icap);
+  G.sl_x = dalloc<double>(e, icap); G.sl_y = dalloc<double>(e, icap); G.sl_z = dalloc<double>(e, icap);
+  G.sl_vx = dalloc<double>(e, icap); G.sl_vy = dalloc<double>(e, icap); G.sl_vz = dalloc<double>(e, icap);
+  G.sl_flag = dalloc<int>(e, icap);
+  GeD& E = e->E;
+  E.sy = dalloc<GeSync>(e, 1);
+  E.memb_q = dalloc<unsigned>(e, nq);
+  E.in_memb4 = dalloc<unsigned>(e, icap);
+  E.in_fp4 = dalloc<double>(e, (size_t)icap * 12);
+  E.in_pot4 = dalloc<double>(e, (size_t)icap * 4);
+  E.part = dalloc<double>(e, (size_t)2 * GE_GMAX * (4 * MAXSG + 1));
+  E.touch = dalloc<unsigned>(e, (size_t)2 * MAXSG * (GE_GMAX / 32));
+  E.in_cap = icap;
+  if (!G.ns_x || !G.sl_flag || !E.sy || !E.touch || !E.in_fp4) return set_error(MPCX_E_HIP, "device allocation failed");
+  e->geG = std::min(GE_GMAX, std::max(2, 2 * e->nd));
+  HIPCHK(hipMemset(E.sy, 0, sizeof(GeSync)));
+  HIPCHK(hipMemset(E.touch, 0, (size_t)2 * MAXSG * (GE_GMAX / 32) * 4));
+  HIPCHK(hipMemset(E.in_fp4, 0, (size_t)icap * 12 * 8));
+  HIPCHK(hipMemset(E.in_pot4, 0, (size_t)icap * 4 * 8));
+  HIPCHK(hipMemset(G.in_msd, 0, (size_t)icap * 8));
+  e->has_ge = true;
+  return 0;
+}
 
 extern "C" int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg) {
   if (!out || !cfg) return set_error(MPCX_E_ARG, "null argument");
@@ -2957,29 +3370,9 @@ extern "C" int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg) {
   D.sl_x = dalloc<double>(e, D.in_cap); D.sl_y = dalloc<double>(e, D.in_cap); D.sl_z = dalloc<double>(e, D.in_cap);
   D.sl_vx = dalloc<double>(e, D.in_cap); D.sl_vy = dalloc<double>(e, D.in_cap); D.sl_vz = dalloc<double>(e, D.in_cap);
   D.sl_flag = dalloc<int>(e, D.in_cap);
-  if (e->can_cluster) {   // the global engine's arrays (entries of the whole system)
-    CoD& G = e->Dg;
-    G = D;   // particle arrays and scalars are filled in below / shared
-    const size_t nq = (size_t)D.near_cap;
-    const int icap = (int)std::min<long long>(NT, (long long)NE_CAP + 8192LL * e->nd);
-    G.ns_x = dalloc<double>(e, nq); G.ns_y = dalloc<double>(e, nq); G.ns_z = dalloc<double>(e, nq);
-    G.ns_vx = dalloc<double>(e, nq); G.ns_vy = dalloc<double>(e, nq); G.ns_vz = dalloc<double>(e, nq);
-    G.ns_flag = dalloc<int>(e, nq); G.ns_stat = dalloc<unsigned char>(e, nq);
-    G.in_cap = icap;
-    G.in_list = dalloc<int>(e, icap); G.in_plist = dalloc<int>(e, icap); G.in_msd = dalloc<double>(e, icap);
-    G.sl_x = dalloc<double>(e, icap); G.sl_y = dalloc<double>(e, icap); G.sl_z = dalloc<double>(e, icap);
-    G.sl_vx = dalloc<double>(e, icap); G.sl_vy = dalloc<double>(e, icap); G.sl_vz = dalloc<double>(e, icap);
-    G.sl_flag = dalloc<int>(e, icap);
-    GeD& E = e->E;
-    E.sy = dalloc<GeSync>(e, 1);
-    E.memb_q = dalloc<unsigned>(e, nq);
-    E.in_memb4 = dalloc<unsigned>(e, icap);
-    E.in_fp4 = dalloc<double>(e, (size_t)icap * 12);
-    E.in_pot4 = dalloc<double>(e, (size_t)icap * 4);
-    E.part = dalloc<double>(e, (size_t)2 * GE_GMAX * (4 * MAXSG + 1));
-    E.touch = dalloc<unsigned>(e, (size_t)2 * MAXSG * (GE_GMAX / 32));
-    E.in_cap = icap;
-    e->geG = std::min(GE_GMAX, std::max(2, 2 * e->nd));
+  if (e->nd >= 2) {
+    int rc = ge_alloc(e);
+    if (rc) { mpcx_co_destroy(e); return rc; }
   }
   D.cell_of = dalloc<int>(e, NT);
   D.rank_of = dalloc<unsigned short>(e, NT);
@@ -3027,16 +3420,6 @@ extern "C" int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg) {
   HIPCHK(hipMemset(D.in_pot, 0, (size_t)MAXM * D.in_cap * 8));
   HIPCHK(hipMemset(D.in_msd, 0, (size_t)D.in_cap * 8));
   if (e->can_cluster) {
-    // the global engine shares the particle arrays, scalars and cell data
-    CoD& G = e->Dg;
-    G.x = D.x; G.y = D.y; G.z = D.z; G.vx = D.vx; G.vy = D.vy; G.vz = D.vz; G.flag = D.flag; G.dim = D.dim;
-    G.sc = D.sc; G.react_box = D.react_box; G.near_list = D.near_list; G.near_bits = D.near_bits;
-    G.status = D.status; G.near_cap = D.near_cap;
-    HIPCHK(hipMemset(e->E.sy, 0, sizeof(GeSync)));
-    HIPCHK(hipMemset(e->E.touch, 0, (size_t)2 * MAXSG * (GE_GMAX / 32) * 4));
-    HIPCHK(hipMemset(e->E.in_fp4, 0, (size_t)e->E.in_cap * 12 * 8));
-    HIPCHK(hipMemset(e->E.in_pot4, 0, (size_t)e->E.in_cap * 4 * 8));
-    HIPCHK(hipMemset(G.in_msd, 0, (size_t)e->E.in_cap * 8));
     HIPCHK(hipMemset(D.clu, 0, sizeof(CluInfo)));
     HIPCHK(hipMemset(D.rec_ids, 0xff, (size_t)NT * 4));
     HIPCHK(hipMemset(D.rec_msd, 0, (size_t)NT * 8));
@@ -3048,11 +3431,42 @@ extern "C" int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg) {
 // 0: one workgroup for the whole system (up to four dimers); 1: one
 // workgroup per cluster of dimers (clusters of up to four); 2: the system's
 // near set over cooperative workgroups (any number of dimers)
+// 1: split-precision solvent storage (device Philox draws only); call before
+// mpcx_co_setup, which converts the uploaded FP64 state
+extern "C" int mpcx_co_set_precision(mpcx_co* e, int32_t split) {
+  if (!e) return set_error(MPCX_E_ARG, "null engine");
+  if (split && e->P.rng_mode != 1) return set_error(MPCX_E_UNSUPPORTED, "split precision runs with device Philox draws");
+  if (split && !e->S.cell) {
+    const size_t NT = (size_t)e->P.NT;
+    for (SpD* q : {&e->S, &e->Salt}) {
+      q->cell = dalloc<int>(e, NT);
+      q->ox = dalloc<float>(e, NT); q->oy = dalloc<float>(e, NT); q->oz = dalloc<float>(e, NT);
+      q->vx = dalloc<float>(e, NT); q->vy = dalloc<float>(e, NT); q->vz = dalloc<float>(e, NT);
+      q->flag = dalloc<signed char>(e, NT);
+      if (!q->cell || !q->ox || !q->oy || !q->oz || !q->vx || !q->vy || !q->vz || !q->flag)
+        return set_error(MPCX_E_HIP, "device allocation failed");
+    }
+  }
+  e->split = split != 0;
+  return 0;
+}
+
+// workgroups of the global near engine (mode 2): 1..128
+extern "C" int mpcx_co_set_near_workgroups(mpcx_co* e, int32_t G) {
+  if (!e || G < 1 || G > GE_GMAX) return set_error(MPCX_E_ARG, "1 to 128 workgroups");
+  e->geG = G;
+  return 0;
+}
+
 extern "C" int mpcx_co_set_near_mode(mpcx_co* e, int32_t mode) {
   if (!e) return set_error(MPCX_E_ARG, "null engine");
   if (mode < 0 || mode > 2) return set_error(MPCX_E_ARG, "near mode is 0, 1 or 2");
-  if (mode && !e->can_cluster) return set_error(MPCX_E_UNSUPPORTED, "modes 1 and 2 need two or more dimers");
+  if (mode == 1 && !e->can_cluster) return set_error(MPCX_E_UNSUPPORTED, "clusters need two or more dimers");
   if (!mode && e->nd > MAXD) return set_error(MPCX_E_UNSUPPORTED, "one workgroup handles up to four dimers");
+  if (mode == 2) {
+    int rc = ge_alloc(e);
+    if (rc) return rc;
+  }
   e->near_mode = mode;
   e->clustered = mode == 1;
   return 0;
@@ -3161,6 +3575,36 @@ static int co_sort(mpcx_co* e, hipStream_t st) {
   return 0;
 }
 
+static int sp_sort(mpcx_co* e, hipStream_t st) {
+  CoD& D = e->D;
+  CoP& P = e->P;
+  const long long NT = P.NT, vol = P.vol;
+  if (!e->has_tour) return set_error(MPCX_E_ARG, "a sort step needs the cell tour (pass it to mpcx_co_set_state)");
+  hipLaunchKernelGGL(fill_kernel<int>, dim3(grid_for(vol, 256)), dim3(256), 0, st, e->d_scnt, vol, 0);
+  hipLaunchKernelGGL(sp_sort_bin, dim3(grid_for(NT, 256)), dim3(256), 0, st, P, D, e->S, (const int*)e->d_tourpos,
+                     e->d_scnt, e->d_skey, e->d_srk);
+  launch_scan(st, e->d_scnt, vol, D.partial, e->d_soff);
+  hipLaunchKernelGGL(fill_kernel<int>, dim3(grid_for(NT, 256)), dim3(256), 0, st, e->d_sb, NT, 0);
+  hipLaunchKernelGGL(sort_scatter, dim3(grid_for(NT, 256)), dim3(256), 0, st, (int)NT, (const int*)e->d_skey,
+                     (const unsigned short*)e->d_srk, (const int*)e->d_soff, e->d_sb);
+  hipLaunchKernelGGL(sort_order, dim3((unsigned)((vol + 255) / 256)), dim3(256), 0, st, vol, (const int*)e->d_scnt,
+                     (const int*)e->d_soff, e->d_sb, BMAX);
+  hipLaunchKernelGGL(fill_kernel<unsigned>, dim3(grid_for(e->nwords + 1, 256)), dim3(256), 0, st, e->alt_bits,
+                     e->nwords + 1, 0u);
+  hipLaunchKernelGGL(sp_sort_gather, dim3(grid_for(NT, 256)), dim3(256), 0, st, P, D, e->S, e->Salt,
+                     (const int*)e->d_sb, e->alt_status, e->alt_bits);
+  std::swap(e->S, e->Salt);
+  std::swap(D.status, e->alt_status); std::swap(D.near_bits, e->alt_bits);
+  return 0;
+}
+
+// split precision: the near set (or every particle) between the split arrays and the FP64 staging
+static void sp_convert(mpcx_co* e, bool to_fp64, bool near_only) {
+  const int g = near_only ? 256 : grid_for(e->P.NT, 256, 2048);
+  if (to_fp64) hipLaunchKernelGGL(sp_load, dim3(g), dim3(256), 0, e->st, e->P, e->D, e->S, near_only ? 1 : 0);
+  else hipLaunchKernelGGL(sp_store, dim3(g), dim3(256), 0, e->st, e->P, e->D, e->S, near_only ? 1 : 0);
+}
+
 static void co_near_list(mpcx_co* e) {
   launch_bits_list(e->st, (const unsigned*)e->D.near_bits, e->nwords, e->D.partial, e->D.near_list, e->D.near_cap,
                    e->P.NT, &e->D.sc->near_num, &e->D.sc->err);
@@ -3242,6 +3686,7 @@ extern "C" int mpcx_co_setup(mpcx_co* e, int32_t do_sort) {
   hipLaunchKernelGGL(fill_kernel<unsigned char>, dim3(grid_for(NT, 256)), dim3(256), 0, st, D.status, NT,
                      (unsigned char)0);
   HIPCHK(launch_near(e, 0, M_SETUP));
+  if (e->split) sp_convert(e, false, false);
   HIPCHK(hipStreamSynchronize(st));
   return check_device_error(e);
 }
@@ -3282,25 +3727,34 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
     if (timed)
       for (int k = 0; k < mpcx_co::NKT; k++) e->tev_on[(size_t)c * mpcx_co::NKT + k] = 0;
     HIPCHK(hipEventRecord(e->n0, st));
+    if (e->split) sp_convert(e, true, true);
     HIPCHK(tmark(0, 0));
     HIPCHK(launch_near(e, P.MPCtstep, sort_due ? M_STOP : 0));
     HIPCHK(tmark(0, 1));
+    if (e->split) sp_convert(e, false, true);
     HIPCHK(hipEventRecord(e->n1, st));
     hipLaunchKernelGGL(fill_kernel<int>, dim3(grid_for(mc, 256)), dim3(256), 0, st, D.cnt, mc, 0);
     hipLaunchKernelGGL(fill_kernel<int>, dim3(grid_for(3 * vol, 256)), dim3(256), 0, st, D.tcount, 3 * vol, 0);
     hipLaunchKernelGGL(fill_kernel<unsigned>, dim3(grid_for(vol, 256)), dim3(256), 0, st, D.firstB, vol, 0xffffffffu);
+    const bool sp = e->split;
+    auto bulk = [&](int stream, int bin) {
+      if (sp) hipLaunchKernelGGL(sp_bulk, dim3(gN), dim3(256), 0, st, P, D, e->S, stream, bin);
+      else hipLaunchKernelGGL(co_bulk, dim3(gN), dim3(256), 0, st, P, D, stream, bin);
+    };
     if (sort_due) {
-      hipLaunchKernelGGL(co_bulk, dim3(gN), dim3(256), 0, st, P, D, 1, 0);
-      int r = co_sort(e, st);
+      bulk(1, 0);
+      int r = sp ? sp_sort(e, st) : co_sort(e, st);
       if (r) return r;
       co_near_list(e);
+      if (sp) sp_convert(e, true, true);
       HIPCHK(launch_near(e, 1, M_RESUME));
+      if (sp) sp_convert(e, false, true);
       HIPCHK(hipEventRecord(e->b0, st));
-      hipLaunchKernelGGL(co_bulk, dim3(gN), dim3(256), 0, st, P, D, 0, 1);
+      bulk(0, 1);
     } else {
       HIPCHK(hipEventRecord(e->b0, st));
       HIPCHK(tmark(1, 0));
-      hipLaunchKernelGGL(co_bulk, dim3(gN), dim3(256), 0, st, P, D, 1, 1);
+      bulk(1, 1);
       HIPCHK(tmark(1, 1));
     }
     launch_scan(st, D.cnt, mc, D.partial, D.off);
@@ -3308,13 +3762,15 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
     hipLaunchKernelGGL(co_scatter, dim3(gN), dim3(256), 0, st, P, D);
     HIPCHK(tmark(4, 1));
     HIPCHK(tmark(2, 0));
-    hipLaunchKernelGGL(co_collide, dim3((unsigned)((mc + CT - 1) / CT)), dim3(CT), 0, st, P, D);
+    if (sp) hipLaunchKernelGGL(sp_collide, dim3((unsigned)((mc + CT - 1) / CT)), dim3(CT), 0, st, P, D, e->S);
+    else hipLaunchKernelGGL(co_collide, dim3((unsigned)((mc + CT - 1) / CT)), dim3(CT), 0, st, P, D);
     HIPCHK(tmark(2, 1));
     hipLaunchKernelGGL(fill_kernel<unsigned>, dim3(grid_for(e->nwords + 1, 256)), dim3(256), 0, st, D.near_bits,
                        e->nwords + 1, 0u);
     hipLaunchKernelGGL(reset_far, dim3(1), dim3(1), 0, st, D.sc);
     HIPCHK(tmark(3, 0));
-    hipLaunchKernelGGL(co_rotate, dim3(gN), dim3(256), 0, st, P, D, 1);
+    if (sp) hipLaunchKernelGGL(sp_rotate, dim3(gN), dim3(256), 0, st, P, D, e->S);
+    else hipLaunchKernelGGL(co_rotate, dim3(gN), dim3(256), 0, st, P, D, 1);
     HIPCHK(tmark(3, 1));
     HIPCHK(hipEventRecord(e->b1, st));
     co_near_list(e);
@@ -3439,6 +3895,7 @@ extern "C" int mpcx_co_init_solvent(mpcx_co* e, int64_t Na0, int64_t Nb0, double
 extern "C" int mpcx_co_get_state(mpcx_co* e, double* x, double* y, double* z, double* vx, double* vy, double* vz,
                                  int32_t* flag, double* dimer12, int32_t* scalars) {
   if (!e) return set_error(MPCX_E_ARG, "null engine");
+  if (e->split) sp_convert(e, true, false);   // the split arrays hold the state
   HIPCHK(hipStreamSynchronize(e->st));
   const size_t NT = (size_t)e->P.NT;
   CoD& D = e->D;
@@ -3618,6 +4075,7 @@ extern "C" int mpcx_co_near_profile(mpcx_co* e, uint64_t* prof8) {
 namespace mpcx {
 int co_view(mpcx_co* e, DevView* v) {
   if (!e || !v) return set_error(MPCX_E_ARG, "null argument");
+  if (e->split) sp_convert(e, true, false);
   if (hipStreamSynchronize(e->st) != hipSuccess) return set_error(MPCX_E_HIP, "stream synchronize failed");
   memset(v, 0, sizeof *v);
   const CoD& D = e->D;
@@ -4003,6 +4461,7 @@ extern "C" int mpcx_cob_create(mpcx_cob** out, int32_t R, mpcx_co* const* engine
     if (e->nd != e0->nd || e->P.MPCtstep != e0->P.MPCtstep || e->P.sort_step != e0->P.sort_step)
       return set_error(MPCX_E_UNSUPPORTED, "replicas of a batch share numdims, MPCtstep and sort_step");
     if (e->nd > 2) return set_error(MPCX_E_UNSUPPORTED, "batched replicas hold one or two dimers");
+    if (e->split) return set_error(MPCX_E_UNSUPPORTED, "batched replicas use FP64 storage");
   }
   if (tile < 256 || tile % 256 != 0) return set_error(MPCX_E_ARG, "tile must be a positive multiple of 256");
   mpcx_cob* b = new mpcx_cob();
