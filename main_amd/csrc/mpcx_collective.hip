@@ -143,15 +143,10 @@ struct CoD {
 // ------------------------------------------------------------- streaming
 // One streaming update of evolve_mod.f95:259-312 (tc = 1: inside particles,
 // whose "x + v*MDt" equals "(v*MDt)*1"), or of the lazy update :474-527.
-__device__ __forceinline__ bool stream_particle(double& x, double& y, double& z, double& vx, double& vy,
-                                                double& vz, int& flag, double tc, const CoP& P,
-                                                const int* react_box, int* err) {
-  const double L0 = P.box[0], L1 = P.box[1], L2 = P.box[2], MDt = P.MDt;
-  x = x + (vx * MDt) * tc;
-  if (x >= L0) x = x - L0; else if (x < 0.0) x = x + L0;
-  y = y + (vy * MDt) * tc;
-  if (y >= L1) y = y - L1; else if (y < 0.0) y = y + L1;
-  z = z + (vz * MDt) * tc;
+// the wall part of a streaming update, after the free flight
+__device__ __forceinline__ bool stream_walls(double& x, double& y, double& z, double& vx, double& vy, double& vz,
+                                             int& flag, const CoP& P, const int* react_box, int* err) {
+  const double L0 = P.box[0], L1 = P.box[1], L2 = P.box[2];
   if (z < 0.0) {
     double t = z / vz;
     // reactive_grid_mod.f95:128-149
@@ -178,6 +173,24 @@ __device__ __forceinline__ bool stream_particle(double& x, double& y, double& z,
     return true;
   }
   return false;
+}
+// free flight of tc steps with the step displacements d = v*MDt (the
+// reference's (v*MDt)*tc) and periodic wrap in x and y
+__device__ __forceinline__ void stream_flight(double& x, double& y, double& z, double dx, double dy, double dz,
+                                              double tc, const CoP& P) {
+  const double L0 = P.box[0], L1 = P.box[1];
+  x = x + dx * tc;
+  if (x >= L0) x = x - L0; else if (x < 0.0) x = x + L0;
+  y = y + dy * tc;
+  if (y >= L1) y = y - L1; else if (y < 0.0) y = y + L1;
+  z = z + dz * tc;
+}
+__device__ __forceinline__ bool stream_particle(double& x, double& y, double& z, double& vx, double& vy,
+                                                double& vz, int& flag, double tc, const CoP& P,
+                                                const int* react_box, int* err) {
+  const double MDt = P.MDt;
+  stream_flight(x, y, z, vx * MDt, vy * MDt, vz * MDt, tc, P);
+  return stream_walls(x, y, z, vx, vy, vz, flag, P, react_box, err);
 }
 
 __device__ __forceinline__ double kick_factor(const CoP& P, int flag, bool& ok) {
@@ -2363,13 +2376,21 @@ __device__ __forceinline__ void collision_shift(const CoP& P, const CoD& D, doub
   zs = P.a0 * (u2 - 0.5);
 }
 
+// X - L*floor(X/L) of the reference without the division: for -L <= X < 2L a
+// correctly rounded X/L lies in [-1,0), [0,1) or [1,2) exactly when X does, so
+// floor(X/L) is -1, 0 or 1 by comparison and the subtraction is the same
+// IEEE operation (X - (-L) is X + L; X - 0*L keeps X, -0.0 included)
+__device__ __forceinline__ double wrap_periodic(double X, double L) {
+  return X < 0.0 ? X + L : (X >= L ? X - L : X);
+}
+
 // mpc_rotation_mod.f95:116-145: cell of the shifted grid, 0-based
 __device__ __forceinline__ long long shifted_cell(const CoP& P, double x, double y, double z, double xs, double ys,
                                                   double zs) {
   double X = x + xs;
-  X = X - P.box[0] * floor(X / P.box[0]);
+  X = wrap_periodic(X, P.box[0]);
   double Y = y + ys;
-  Y = Y - P.box[1] * floor(Y / P.box[1]);
+  Y = wrap_periodic(Y, P.box[1]);
   double Z = z + zs;
   double fz = zs < 0.0 ? floor(Z + 1.0) : floor(Z);
   double c = ffloor_i(X) + P.box[0] * ffloor_i(Y);
@@ -2436,13 +2457,15 @@ __device__ __forceinline__ void bulk_body(const CoP& P, const CoD& D, int do_str
       if (ok && r >= BMAX) raise_error(&D.sc->err, ERR_BUCKET);
       D.cell_of[i] = t;
       D.rank_of[i] = ok ? (unsigned short)min(r, 65535) : (unsigned short)0xffff;
-      // distance to the nearest sphere beyond its list radius (next near set)
+      // distance to the nearest sphere beyond its list radius (next near set);
+      // 8 or more beyond every radius gives the cap (255 at 7.97) without a sqrt
       double dmin = 1e30;
       for (int s = 0; s < 2 * P.nd; s++) {
         double a = pbc_half(x - sph[0][s], P.box[0]);
         double b = pbc_half(y - sph[1][s], P.box[1]);
         double c = z - sph[2][s];
-        dmin = fmin(dmin, sqrt(a * a + b * b + c * c) - P.full[s >= P.nd]);
+        const double d2 = a * a + b * b + c * c, far = P.full[s >= P.nd] + 8.0;
+        if (d2 < far * far) dmin = fmin(dmin, sqrt(d2) - P.full[s >= P.nd]);
       }
       double qd = floor(fmax(dmin, 0.0) * 32.0 * (1.0 - 1e-12));
       if (st & ST_INSIDE) qd = 0.0;   // list members stay near until the next rebuild
@@ -2940,7 +2963,8 @@ __global__ __launch_bounds__(256) void sp_bulk(CoP P, CoD D, SpD S, int do_strea
         double a = pbc_half(x - sph[0][s], P.box[0]);
         double b = pbc_half(y - sph[1][s], P.box[1]);
         double c = z - sph[2][s];
-        dmin = fmin(dmin, sqrt(a * a + b * b + c * c) - P.full[s >= P.nd]);
+        const double d2 = a * a + b * b + c * c, far = P.full[s >= P.nd] + 8.0;
+        if (d2 < far * far) dmin = fmin(dmin, sqrt(d2) - P.full[s >= P.nd]);
       }
       // 1e-5 below the measured distance: the FP32 position must not make a particle look farther
       double qd = floor(fmax(dmin - 1e-5, 0.0) * 32.0);
