@@ -142,6 +142,24 @@ int mpcx_co_set_state(mpcx_co* e, int32_t cstep, const double* x, const double* 
 /* Set-up of main_pattern.f95:170-188: optional sort, neighbour and in/out
  * lists, time_penetrate; prepares the near/far split of the first interval. */
 int mpcx_co_setup(mpcx_co* e, int32_t do_sort);
+/* Set-up of a restart (restart_mod.f95:416-495): the same sort, lists and
+ * time_penetrate, then the forces, pair forces and potential evaluated from
+ * the uploaded positions (an A particle inside a catalytic cutoff turns B, as
+ * in any force evaluation) instead of the zero forces of a fresh start.
+ * do_sort = (cstep % sort_step == 0) as restart_mod.f95:423.  Near modes 0
+ * and 2. */
+int mpcx_co_setup_restart(mpcx_co* e, int32_t do_sort);
+
+/* Frame of the system at a collision step: solvent positions, velocities and
+ * species, dimer positions and velocities -- what restart_mod.f95:362-406
+ * reads from the last data.h5 frame -- in a binary file (64-byte header with
+ * sizes, box and a checksum; written under a temporary name, then renamed).
+ * mpcx_co_load_frame checks the file against the engine's system and uploads
+ * it as mpcx_co_set_state does (dimer forces zero until
+ * mpcx_co_setup_restart); *cstep (may be NULL) receives the frame's step. */
+int mpcx_co_save_frame(mpcx_co* e, const char* path);
+int mpcx_co_load_frame(mpcx_co* e, const char* path, const int32_t* react_box, const int64_t* tour,
+                       int32_t* cstep);
 
 /* Advance ncoll collision intervals (MPCtstep MD steps plus one collision
  * each).  In rng_mode 0, draws holds ncoll blocks of
@@ -340,6 +358,30 @@ int mpcx_mlp_train_stats(mpcx_mlp* e, double* ms, double* gemm_flops);
 /* C = A B (row-major, M x K and K x N) with BF16 operands on the MFMA units */
 int mpcx_gemm_bf16(int32_t M, int32_t N, int32_t K, const float* A, const float* B, float* C, int32_t reps,
                    double* ms);
+
+/* ------------------------------------------------------ restart files (host)
+ * The reference's own restart files, byte for byte where the format is fixed:
+ *  timestep.dat: main_pattern.f95:130-136 (read) / :243-249 (write): cstep,
+ *    iTotA, iTotB, iTotS, one list-directed item per record;
+ *  mt_state.dat: mtmod.f90:152-229 (mtsavef/mtgetf, form 'u'): sequential
+ *    unformatted records mti(T) and mt(624,T), 4-byte record markers;
+ *  motor_positions.dat: initial_mod.f95:79,118 / restart_mod.f95:410-414:
+ *    motorCM(numdims,3) (column-major), one dimer per record.
+ * List-directed output follows gfortran's layout; the readers take any
+ * list-directed form.  mpcx_mt_state_read with mt or mti NULL only returns
+ * the stream count. */
+int mpcx_timestep_write(const char* path, int32_t cstep, int64_t n_a, int64_t n_b, int64_t n_s);
+int mpcx_timestep_read(const char* path, int32_t* cstep, int64_t* n_a, int64_t* n_b, int64_t* n_s);
+int mpcx_mt_state_write(const char* path, const uint32_t* mt, const int32_t* mti, int32_t nstreams);
+int mpcx_mt_state_read(const char* path, uint32_t* mt, int32_t* mti, int32_t max_streams, int32_t* nstreams);
+int mpcx_motor_positions_write(const char* path, const double* cm, int32_t nd);
+int mpcx_motor_positions_read(const char* path, double* cm, int32_t nd);
+/* One record of fixed-form output with I (Iw), E (Ew.d), S (ESw.d) and X (nX)
+ * edit descriptors, e.g. "I8 X1 E23.16 X1 E23.16 X1 E23.16 X1" for the
+ * selkov_code dimer records (Output_SelkovDimer.f90:26) or "S23.16 X1 ... I8"
+ * for its solvent records (:139); values in order, integers as whole doubles.
+ * Writes a NUL-terminated line (no newline) into out[cap]. */
+int mpcx_format_record(const char* spec, const double* values, int32_t nvalues, char* out, int64_t cap);
 
 #ifdef __cplusplus
 }

@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import os
 
 import numpy as np
 
@@ -137,8 +138,28 @@ class CollectiveEngine:
         check(lib().mpcx_co_init_solvent(self.h, C.c_int64(int(p["Na0"])), C.c_int64(int(p["Nb0"])),
                                          C.c_double(float(p["T"])), C.c_uint64(int(seed))), "mpcx_co_init_solvent")
 
-    def setup(self, do_sort: bool = True):
-        check(lib().mpcx_co_setup(self.h, C.c_int32(1 if do_sort else 0)), "mpcx_co_setup")
+    def setup(self, do_sort: bool = True, restart: bool = False):
+        """Set-up of main_pattern.f95:170-188 (zero forces, as after
+        initial_mod.f95:348-353) or, with restart, of restart_mod.f95:416-495
+        (forces evaluated from the uploaded positions)."""
+        if restart:
+            check(lib().mpcx_co_setup_restart(self.h, C.c_int32(1 if do_sort else 0)), "mpcx_co_setup_restart")
+        else:
+            check(lib().mpcx_co_setup(self.h, C.c_int32(1 if do_sort else 0)), "mpcx_co_setup")
+
+    def save_frame(self, path: str):
+        """Frame of the system (solvent and dimer positions, velocities, species)
+        at a collision step; see mpcx_co_save_frame."""
+        check(lib().mpcx_co_save_frame(self.h, os.fsencode(path)), "mpcx_co_save_frame")
+
+    def load_frame(self, path: str, react_box, tour=None) -> int:
+        """Upload a frame written by save_frame; returns its collision step."""
+        rb = np.ascontiguousarray(react_box, dtype=np.int32)
+        t = None if tour is None else np.ascontiguousarray(tour, dtype=np.int64)
+        cs = C.c_int32(0)
+        check(lib().mpcx_co_load_frame(self.h, os.fsencode(path), ptr(rb), ptr(t), C.byref(cs)),
+              "mpcx_co_load_frame")
+        return int(cs.value)
 
     def run(self, ncoll: int, draws: np.ndarray | None = None):
         d = None if draws is None else np.ascontiguousarray(draws, dtype=np.float64)
@@ -161,6 +182,12 @@ class CollectiveEngine:
             out[k] = np.asfortranarray(dim[f].reshape(2, nd).T)   # s = k + i*nd -> (k, i)
         out["cstep"], out["time_count"], out["time_penetrate"], out["in_num"] = (int(v) for v in sc)
         return out
+
+    def scalars(self) -> dict:
+        """cstep, time_count, time_penetrate and the inside-list length."""
+        sc = np.zeros(4, np.int32)
+        check(lib().mpcx_co_get_state(self.h, *([None] * 8), ptr(sc)), "mpcx_co_get_state")
+        return dict(cstep=int(sc[0]), time_count=int(sc[1]), time_penetrate=int(sc[2]), in_num=int(sc[3]))
 
     def get_lists(self) -> dict:
         """Inside list, neighbour lists (reference layout, 1-based) and Fpair."""
@@ -189,6 +216,12 @@ class CollectiveEngine:
                     uvy=rec[:, 4].copy(), uvz=rec[:, 5].copy(), tcs=rec[:, 6].copy(),
                     ncn=cell.astype(np.int64) + 1, tA=census[0].astype(np.int64), tB=census[1].astype(np.int64),
                     tS=census[2].astype(np.int64), tpotential=float(tp[0]))
+
+    def tpotential(self) -> float:
+        """TPotential of the last force evaluation (set-up or MD step)."""
+        tp = np.zeros(1)
+        check(lib().mpcx_co_get_cells(self.h, None, None, None, ptr(tp)), "mpcx_co_get_cells")
+        return float(tp[0])
 
     def near_profile(self) -> dict:
         a = np.zeros(8, np.uint64)

@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
+#include <stdio.h>
 #include <string.h>
 #include <string>
 #include <vector>
@@ -3189,6 +3190,8 @@ struct mpcx_co {
   // split precision: solvent in cell + FP32 offsets / FP32 velocities / 8-bit species
   bool split = false;
   SpD S, Salt;
+  double* d_rsum = nullptr;   // [4][MAXSG] per-sphere totals of the restart evaluation
+  bool cells_valid = false;   // a collision has binned the particles since the last upload
 };
 
 template <typename T>
@@ -3417,6 +3420,7 @@ extern "C" int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg) {
   D.ptab = ptab;
   e->ndraw = 3 + 2 * mc + cfg->rxnnumcell;
   e->d_U = dalloc<double>(e, cfg->rng_mode == 0 ? e->ndraw : 4);
+  e->d_rsum = dalloc<double>(e, 4 * MAXSG);
   D.U = e->d_U;
   e->d_tourpos = dalloc<int>(e, vol);
   e->d_skey = dalloc<int>(e, NT); e->d_srk = dalloc<unsigned short>(e, NT);
@@ -3572,6 +3576,7 @@ extern "C" int mpcx_co_set_state(mpcx_co* e, int32_t cstep, const double* x, con
   memset(&sc, 0, sizeof sc);
   sc.cstep = cstep;
   HIPCHK(hipMemcpy(D.sc, &sc, sizeof sc, hipMemcpyHostToDevice));
+  e->cells_valid = false;
   return 0;
 }
 
@@ -3684,7 +3689,166 @@ static hipError_t launch_near(mpcx_co* e, int nsteps, int mode) {
   return hipGetLastError();
 }
 
-extern "C" int mpcx_co_setup(mpcx_co* e, int32_t do_sort) {
+// ------------------------------------------------------ restart evaluation
+// restart_mod.f95:429-495 after the lists are built: the pair force of every
+// inside entry with each sphere whose list holds it (the forces_mod.f95:62-136
+// arithmetic; an A particle inside a catalytic cutoff turns B), the
+// per-sphere totals, then the dimer forces and the potential with the
+// motor-motor and wall terms.  The run then starts from these forces instead
+// of the zero forces of a fresh start (initial_mod.f95:348-353).
+
+// pair force of one particle with sphere s (list id 2k+i, ii = i)
+__device__ __forceinline__ void restart_pair(const CoP& P, const CoD& D, double x, double y, double z, int& fl,
+                                             int s, int ii, double& fx, double& fy, double& fz, double& pot) {
+  fx = fy = fz = pot = 0.0;
+  double d1 = pbc_half(x - D.dim[DX * MAXSG + s], P.box[0]);
+  double d2 = pbc_half(y - D.dim[DY * MAXSG + s], P.box[1]);
+  double d3 = z - D.dim[DZ * MAXSG + s];
+  double dRsq = d1 * d1 + d2 * d2 + d3 * d3;
+  if (!(dRsq <= P.cut2[ii])) return;
+  double E;
+  if (fl == 0) {
+    if (ii == 0) { fl = 1; E = P.Eb[0]; } else E = P.Ea[1];
+  } else if (fl == 1) E = P.Eb[ii];
+  else if (fl == 2) E = P.Es[ii];
+  else return;
+  double F, Ub;
+  lj_terms(dRsq, P.R6[ii], P.R12[ii], P.r12R12[ii], P.r6R6[ii], F, Ub);
+  double EF = E * F;
+  fx = EF * d1; fy = EF * d2; fz = EF * d3; pot = E * Ub;
+}
+
+// one-workgroup engine: entry k holds membership bits b = 2k'+i (sphere k' + i*nd)
+__global__ __launch_bounds__(256) void co_restart_pairs(CoP P, CoD D) {
+  const int cap = D.in_cap, n = min(D.sc->in_num, cap), nd = P.nd;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const int p = D.in_plist[k];
+    const unsigned mb = D.in_memb[k];
+    const double x = D.x[p], y = D.y[p], z = D.z[p];
+    const int f0 = D.flag[p];
+    int fl = f0;
+    for (int b = 0; b < 2 * nd; b++) {
+      double fx = 0.0, fy = 0.0, fz = 0.0, pot = 0.0;
+      if ((mb >> b) & 1u) restart_pair(P, D, x, y, z, fl, (b >> 1) + (b & 1) * nd, b & 1, fx, fy, fz, pot);
+      D.in_fp[(b * 3 + 0) * cap + k] = fx;
+      D.in_fp[(b * 3 + 1) * cap + k] = fy;
+      D.in_fp[(b * 3 + 2) * cap + k] = fz;
+      D.in_pot[b * cap + k] = pot;
+    }
+    if (fl != f0) D.flag[p] = fl;
+  }
+}
+
+// global engine: entry k holds up to four list ids (bytes, ascending, 0xff ends)
+__global__ __launch_bounds__(256) void co_restart_pairs_ge(CoP P, CoD D, GeD E) {
+  const int n = min(D.sc->in_num, E.in_cap), nd = P.nd;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const int p = D.in_plist[k];
+    const unsigned ids = E.in_memb4[k];
+    const double x = D.x[p], y = D.y[p], z = D.z[p];
+    const int f0 = D.flag[p];
+    int fl = f0;
+    for (int m = 0; m < 4; m++) {
+      const unsigned b = (ids >> (8 * m)) & 0xffu;
+      double fx = 0.0, fy = 0.0, fz = 0.0, pot = 0.0;
+      if (b != 0xffu) restart_pair(P, D, x, y, z, fl, (int)(b >> 1) + (int)(b & 1u) * nd, (int)(b & 1u), fx, fy, fz, pot);
+      E.in_fp4[(size_t)k * 12 + m * 3 + 0] = fx;
+      E.in_fp4[(size_t)k * 12 + m * 3 + 1] = fy;
+      E.in_fp4[(size_t)k * 12 + m * 3 + 2] = fz;
+      E.in_pot4[(size_t)k * 4 + m] = pot;
+    }
+    if (fl != f0) D.flag[p] = fl;
+  }
+}
+
+// per-sphere totals of the pair forces and potential (one workgroup per
+// sphere, fixed summation order)
+__global__ __launch_bounds__(256) void co_restart_sums(CoP P, CoD D, GeD E, int ge, double* out) {
+  __shared__ double red[4][4];
+  const int s = blockIdx.x, nd = P.nd, tid = threadIdx.x;
+  const unsigned b = 2u * (unsigned)(s % nd) + (unsigned)(s / nd);
+  const int cap = D.in_cap, n = min(D.sc->in_num, ge ? E.in_cap : cap);
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int k = tid; k < n; k += blockDim.x) {
+    if (!ge) {
+      if ((D.in_memb[k] >> b) & 1u) {
+        for (int c = 0; c < 3; c++) a[c] += D.in_fp[(b * 3 + c) * cap + k];
+        a[3] += D.in_pot[b * cap + k];
+      }
+    } else {
+      const unsigned ids = E.in_memb4[k];
+      for (int m = 0; m < 4; m++)
+        if (((ids >> (8 * m)) & 0xffu) == b) {
+          for (int c = 0; c < 3; c++) a[c] += E.in_fp4[(size_t)k * 12 + m * 3 + c];
+          a[3] += E.in_pot4[(size_t)k * 4 + m];
+        }
+    }
+  }
+  for (int c = 0; c < 4; c++) a[c] = wave_sum(a[c]);
+  if ((tid & 63) == 0)
+    for (int c = 0; c < 4; c++) red[tid >> 6][c] = a[c];
+  __syncthreads();
+  if (tid < 4) out[4 * s + tid] = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
+}
+
+// dimer forces and potential from the totals (the force part of
+// dimer_forces_and_rattle_v, velocities untouched)
+__global__ __launch_bounds__(256) void co_restart_dimers(CoP P, CoD D, const double* sums) {
+  __shared__ double tp[MAXSG];
+  const int nd = P.nd, s = threadIdx.x;
+  if (s < 2 * nd) {
+    const int k = s % nd, i = s / nd;
+    const double* dm = D.dim;
+    double fw, wp;
+    lj_mot_wall(P, dm[DZ * MAXSG + s], i, fw, wp);
+    double fx = 0.0 - sums[4 * s + 0], fy = 0.0 - sums[4 * s + 1], fz = 0.0 - sums[4 * s + 2];
+    double t = 0.0 + sums[4 * s + 3];
+    for (int j = 0; j < nd; j++)
+      for (int l = 0; l < 2; l++) {
+        if (j == k) continue;
+        const int b = j + l * nd;
+        const int kind = (i == 0 && l == 0) ? 0 : (i == 1 && l == 1) ? 2 : 1;
+        double mx, my, mz, mp;
+        lj_mot_mot(P, dm[DX * MAXSG + s], dm[DY * MAXSG + s], dm[DZ * MAXSG + s], dm[DX * MAXSG + b],
+                   dm[DY * MAXSG + b], dm[DZ * MAXSG + b], kind, mx, my, mz, mp);
+        fx = fx + mx; fy = fy + my; fz = fz + mz;
+        if (j < k) t = t + mp;
+      }
+    fz = fz + fw;
+    t = t + wp;
+    D.dim[DFX * MAXSG + s] = fx; D.dim[DFY * MAXSG + s] = fy; D.dim[DFZ * MAXSG + s] = fz;
+    tp[s] = t;
+  }
+  __syncthreads();
+  if (s == 0) {
+    double T = 0.0;
+    for (int k = 0; k < nd; k++)
+      for (int i = 0; i < 2; i++) T = T + tp[k + i * nd];
+    D.sc->tpotential = T;
+  }
+}
+
+static hipError_t launch_restart(mpcx_co* e) {
+  const CoP& P = e->P;
+  hipStream_t st = e->st;
+  const int ge = e->near_mode == 2 ? 1 : 0;
+  CoD D = ge ? e->Dg : e->D;
+  {   // particle arrays of the system as they are now
+    const CoD& S = e->D;
+    D.x = S.x; D.y = S.y; D.z = S.z; D.flag = S.flag; D.dim = S.dim; D.sc = S.sc;
+  }
+  const int cap = ge ? e->E.in_cap : D.in_cap;
+  const int grid = std::max(1, std::min(1024, (cap + 255) / 256));
+  if (ge)
+    hipLaunchKernelGGL(co_restart_pairs_ge, dim3(grid), dim3(256), 0, st, P, D, e->E);
+  else
+    hipLaunchKernelGGL(co_restart_pairs, dim3(grid), dim3(256), 0, st, P, D);
+  hipLaunchKernelGGL(co_restart_sums, dim3(2 * e->nd), dim3(256), 0, st, P, D, e->E, ge, e->d_rsum);
+  hipLaunchKernelGGL(co_restart_dimers, dim3(1), dim3(256), 0, st, P, D, (const double*)e->d_rsum);
+  return hipGetLastError();
+}
+
+static int co_setup(mpcx_co* e, int32_t do_sort, bool restart) {
   if (!e) return set_error(MPCX_E_ARG, "null engine");
   CoD& D = e->D;
   const long long NT = e->P.NT;
@@ -3710,9 +3874,19 @@ extern "C" int mpcx_co_setup(mpcx_co* e, int32_t do_sort) {
   hipLaunchKernelGGL(fill_kernel<unsigned char>, dim3(grid_for(NT, 256)), dim3(256), 0, st, D.status, NT,
                      (unsigned char)0);
   HIPCHK(launch_near(e, 0, M_SETUP));
+  if (restart) HIPCHK(launch_restart(e));
   if (e->split) sp_convert(e, false, false);
   HIPCHK(hipStreamSynchronize(st));
   return check_device_error(e);
+}
+
+extern "C" int mpcx_co_setup(mpcx_co* e, int32_t do_sort) { return co_setup(e, do_sort, false); }
+
+extern "C" int mpcx_co_setup_restart(mpcx_co* e, int32_t do_sort) {
+  if (!e) return set_error(MPCX_E_ARG, "null engine");
+  if (e->clustered)
+    return set_error(MPCX_E_UNSUPPORTED, "the restart evaluation runs with near mode 0 or 2");
+  return co_setup(e, do_sort, true);
 }
 
 extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
@@ -3809,6 +3983,7 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
     e->bulk_ms = bm;
   }
   HIPCHK(hipEventRecord(e->e1, st));
+  if (ncoll > 0) e->cells_valid = true;
   HIPCHK(hipEventSynchronize(e->e1));
   float ms = 0;
   (void)hipEventElapsedTime(&ms, e->e0, e->e1);
@@ -3945,6 +4120,102 @@ extern "C" int mpcx_co_get_state(mpcx_co* e, double* x, double* y, double* z, do
   return 0;
 }
 
+// ---------------------------------------------------------------- frames
+// One frame of the system, written at a collision step and read back before
+// mpcx_co_setup_restart; it carries what restart_mod.f95:362-406 reads from
+// the last data.h5 frame (solvent positions, velocities and species, dimer
+// positions and velocities).  Layout (host byte order, little-endian here):
+//   0  char[8] "MPCXFRM1"   8 int32 NT   12 int32 numdims   16 int32 cstep
+//  20  int32 0             24 double box[3]                 48 uint64 payload bytes
+//  56  uint64 FNV-1a hash of the payload
+//  64  x y z vx vy vz (NT doubles each), flag (NT int32),
+//      xd yd zd vxd vyd vzd (2*numdims doubles each, sphere s = k + i*numdims)
+// The file is written under a temporary name and renamed into place.
+namespace {
+struct FrameHeader {
+  char magic[8];
+  int32_t NT, nd, cstep, zero;
+  double box[3];
+  uint64_t bytes, hash;
+};
+static_assert(sizeof(FrameHeader) == 64, "frame header is 64 bytes");
+uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
+  const unsigned char* c = (const unsigned char*)p;
+  for (size_t i = 0; i < n; i++) { h ^= c[i]; h *= 1099511628211ull; }
+  return h;
+}
+}  // namespace
+
+extern "C" int mpcx_co_save_frame(mpcx_co* e, const char* path) {
+  if (!e || !path) return set_error(MPCX_E_ARG, "null argument");
+  const size_t NT = (size_t)e->P.NT, ns = 2 * (size_t)e->nd;
+  std::vector<double> a(6 * NT);
+  std::vector<int32_t> flag(NT);
+  std::vector<double> dim(12 * ns);
+  int32_t sc[4];
+  int r = mpcx_co_get_state(e, a.data(), a.data() + NT, a.data() + 2 * NT, a.data() + 3 * NT, a.data() + 4 * NT,
+                            a.data() + 5 * NT, flag.data(), dim.data(), sc);
+  if (r) return r;
+  if (sc[0] % e->P.MPCtstep != 0) return set_error(MPCX_E_ARG, "a frame is written at a collision step");
+  FrameHeader h;
+  memset(&h, 0, sizeof h);
+  memcpy(h.magic, "MPCXFRM1", 8);
+  h.NT = (int32_t)NT; h.nd = e->nd; h.cstep = sc[0];
+  for (int i = 0; i < 3; i++) h.box[i] = e->P.box[i];
+  h.bytes = 6 * NT * 8 + NT * 4 + 6 * ns * 8;
+  uint64_t hs = 1469598103934665603ull;
+  hs = fnv1a(hs, a.data(), a.size() * 8);
+  hs = fnv1a(hs, flag.data(), NT * 4);
+  hs = fnv1a(hs, dim.data(), 6 * ns * 8);   // positions and velocities: fields 0-5
+  h.hash = hs;
+  const std::string tmp = std::string(path) + ".tmp";
+  FILE* f = fopen(tmp.c_str(), "wb");
+  if (!f) return set_error(MPCX_E_ARG, "frame: cannot open for writing");
+  bool ok = fwrite(&h, sizeof h, 1, f) == 1 && fwrite(a.data(), 8, a.size(), f) == a.size() &&
+            fwrite(flag.data(), 4, NT, f) == NT && fwrite(dim.data(), 8, 6 * ns, f) == 6 * ns;
+  ok = (fclose(f) == 0) && ok;
+  if (!ok || rename(tmp.c_str(), path) != 0) {
+    remove(tmp.c_str());
+    return set_error(MPCX_E_ARG, "frame: write failed");
+  }
+  return 0;
+}
+
+extern "C" int mpcx_co_load_frame(mpcx_co* e, const char* path, const int32_t* react_box, const int64_t* tour,
+                                  int32_t* cstep) {
+  if (!e || !path) return set_error(MPCX_E_ARG, "null argument");
+  const size_t NT = (size_t)e->P.NT, ns = 2 * (size_t)e->nd;
+  FILE* f = fopen(path, "rb");
+  if (!f) return set_error(MPCX_E_ARG, "frame: cannot open");
+  FrameHeader h;
+  std::vector<double> a(6 * NT), dim(6 * ns);
+  std::vector<int32_t> flag(NT);
+  bool ok = fread(&h, sizeof h, 1, f) == 1;
+  if (ok && (memcmp(h.magic, "MPCXFRM1", 8) != 0 || h.NT != (int32_t)NT || h.nd != e->nd ||
+             h.bytes != 6 * NT * 8 + NT * 4 + 6 * ns * 8 || h.box[0] != e->P.box[0] || h.box[1] != e->P.box[1] ||
+             h.box[2] != e->P.box[2])) {
+    fclose(f);
+    return set_error(MPCX_E_ARG, "frame: written for another system (particle count, dimers or box)");
+  }
+  ok = ok && fread(a.data(), 8, a.size(), f) == a.size() && fread(flag.data(), 4, NT, f) == NT &&
+       fread(dim.data(), 8, 6 * ns, f) == 6 * ns;
+  fclose(f);
+  if (!ok) return set_error(MPCX_E_ARG, "frame: truncated");
+  uint64_t hs = 1469598103934665603ull;
+  hs = fnv1a(hs, a.data(), a.size() * 8);
+  hs = fnv1a(hs, flag.data(), NT * 4);
+  hs = fnv1a(hs, dim.data(), 6 * ns * 8);
+  if (hs != h.hash) return set_error(MPCX_E_ARG, "frame: checksum mismatch");
+  if (h.cstep % e->P.MPCtstep != 0) return set_error(MPCX_E_ARG, "frame: not at a collision step");
+  const double* d = dim.data();
+  int r = mpcx_co_set_state(e, h.cstep, a.data(), a.data() + NT, a.data() + 2 * NT, a.data() + 3 * NT,
+                            a.data() + 4 * NT, a.data() + 5 * NT, flag.data(), d, d + ns, d + 2 * ns, d + 3 * ns,
+                            d + 4 * ns, d + 5 * ns, nullptr, nullptr, nullptr, react_box, tour);
+  if (r) return r;
+  if (cstep) *cstep = h.cstep;
+  return 0;
+}
+
 extern "C" int mpcx_co_get_inside(mpcx_co* e, int32_t* list, uint32_t* memb, double* fp, double* pot, int64_t cap,
                                   int64_t* n) {
   if (!e || !n) return set_error(MPCX_E_ARG, "null argument");
@@ -4073,6 +4344,8 @@ extern "C" int mpcx_co_get_lists(mpcx_co* e, int64_t max_nb, int64_t* nb, int64_
 extern "C" int mpcx_co_get_cells(mpcx_co* e, double* vcm_uv, int32_t* cell_of, int32_t* census, double* tpotential) {
   if (!e) return set_error(MPCX_E_ARG, "null engine");
   HIPCHK(hipStreamSynchronize(e->st));
+  if (!e->cells_valid && (vcm_uv || cell_of || census))
+    return set_error(MPCX_E_ARG, "no collision has run since the state was uploaded: the cell arrays are empty");
   CoD& D = e->D;
   const size_t mc = (size_t)e->P.mpcnumcell;
   if (vcm_uv) {   // stored by tour position; returned by cell
@@ -4082,7 +4355,10 @@ extern "C" int mpcx_co_get_cells(mpcx_co* e, double* vcm_uv, int32_t* cell_of, i
   }
   if (cell_of) {
     HIPCHK(hipMemcpy(cell_of, D.cell_of, (size_t)e->P.NT * 4, hipMemcpyDeviceToHost));
-    for (long long i = 0; i < e->P.NT; i++) cell_of[i] = e->mpc_cell[cell_of[i]];
+    for (long long i = 0; i < e->P.NT; i++) {
+      if (cell_of[i] < 0 || (size_t)cell_of[i] >= mc) return set_error(MPCX_E_DEVICE, "cell index out of range");
+      cell_of[i] = e->mpc_cell[cell_of[i]];
+    }
   }
   if (census) HIPCHK(hipMemcpy(census, D.tcount, (size_t)e->P.vol * 12, hipMemcpyDeviceToHost));
   if (tpotential) HIPCHK(hipMemcpy(tpotential, &D.sc->tpotential, 8, hipMemcpyDeviceToHost));
@@ -4556,6 +4832,7 @@ static int cob_step(mpcx_cob* b, int* cstep, int use_graph) {
   } else {
     cob_interval(b);
   }
+  for (int r = 0; r < b->R; r++) b->eng[r]->cells_valid = true;
   *cstep = cnext;
   return rc;
 }

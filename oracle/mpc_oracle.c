@@ -426,16 +426,16 @@ void or_co_setup_lists(const or_co_params *p, or_co_state *s, int do_sort) {
   s->time_count = 0;
 }
 
-void or_co_evolve(const or_co_params *p, or_co_state *s) {
+/* forces_mod.f95 calls of evolve_mod.f95:567-602 and restart_mod.f95:467-475 (their
+ * order and arguments are the same): solvent pair forces over the lists, the
+ * motor-motor terms into Fm (FmotX,Y,Z,pot (k,i,j,l)) and the wall terms. */
+static void co_pair_terms(const or_co_params *p, or_co_state *s, double *Fm, double *fwz, double *wpot) {
   const int nd = p->numdims;
-  const double MDt = p->MDt;
-  const double hc = (0.5 * MDt) / p->Mc, hn = (0.5 * MDt) / p->Mn;
-  const double IIMcIMn = 1.0 / ((1.0 / p->Mc) + (1.0 / p->Mn));
+  const double cutC = p->Rc * p->cutoff, cutN = p->Rn * p->cutoff;
+  const double cutC2 = cutC * cutC, cutN2 = cutN * cutN;
   const double Rn6 = pow(p->Rn, 6), Rn12 = pow(p->Rn, 12);
   const double Rc6 = pow(p->Rc, 6), Rc12 = pow(p->Rc, 12);
   const double r6Rn6 = 6.0 * Rn6, r12Rn12 = 12.0 * Rn12, r6Rc6 = 6.0 * Rc6, r12Rc12 = 12.0 * Rc12;
-  const double cutC = p->Rc * p->cutoff, cutN = p->Rn * p->cutoff;
-  const double cutC2 = cutC * cutC, cutN2 = cutN * cutN;
   const double dcc = p->Rc + p->Rc + p->Rm_delta, dcn = p->Rc + p->Rn + p->Rm_delta,
                dnn = p->Rn + p->Rn + p->Rm_delta;
   const double Rcc6 = pow(dcc, 6), Rcc12 = pow(dcc, 12), Rcn6 = pow(dcn, 6), Rcn12 = pow(dcn, 12);
@@ -447,6 +447,96 @@ void or_co_evolve(const or_co_params *p, or_co_state *s) {
   const double r3Rwc3 = 3.0 * Rwc3, r9Rwc9 = 9.0 * Rwc9, r3Rwn3 = 3.0 * Rwn3, r9Rwn9 = 9.0 * Rwn9;
   const double w6 = pow(3.0, (double)(1.0f / 6.0f));
   const double wcC2 = (wc * w6) * (wc * w6), wcN2 = (wn * w6) * (wn * w6);
+  const double *box = p->box;
+  int64_t nd4 = (int64_t)nd * 2 * nd * 2;
+#define FM(c, k, i, j, l) Fm[(int64_t)(c) * nd4 + (k) + (int64_t)(i) * nd + (int64_t)(j) * 2 * nd + (int64_t)(l) * 2 * nd * nd]
+  for (int k = 0; k < nd; k++) {                                        /* :567-602 */
+    int64_t o0 = IX2(k, 0, nd), o1 = IX2(k, 1, nd);
+    or_co_lj_mot_sol(1, box, s->flag, s->x, s->y, s->z, s->xd[o0], s->yd[o0], s->zd[o0],
+                     &s->fpx[IX3(k, 0, 0, nd)], &s->fpy[IX3(k, 0, 0, nd)], &s->fpz[IX3(k, 0, 0, nd)],
+                     &s->pot[IX3(k, 0, 0, nd)], p->RPab, p->RPba, cutC2, p->EaC, p->EbC, p->EsC,
+                     Rc6, Rc12, r12Rc12, r6Rc6, s->nb_num[o0], &s->nb[IX3(k, 0, 0, nd)], 2 * nd);
+    or_co_lj_mot_sol(2, box, s->flag, s->x, s->y, s->z, s->xd[o1], s->yd[o1], s->zd[o1],
+                     &s->fpx[IX3(k, 1, 0, nd)], &s->fpy[IX3(k, 1, 0, nd)], &s->fpz[IX3(k, 1, 0, nd)],
+                     &s->pot[IX3(k, 1, 0, nd)], p->RPab, p->RPba, cutN2, p->EaN, p->EbN, p->EsN,
+                     Rn6, Rn12, r12Rn12, r6Rn6, s->nb_num[o1], &s->nb[IX3(k, 1, 0, nd)], 2 * nd);
+    for (int j = 0; j < nd; j++) {
+      if (j == k) continue;
+      int64_t j0 = IX2(j, 0, nd), j1 = IX2(j, 1, nd);
+      lj_mot_mot(box, s->xd[o0], s->yd[o0], s->zd[o0], s->xd[j0], s->yd[j0], s->zd[j0],
+                 &FM(0, k, 0, j, 0), &FM(1, k, 0, j, 0), &FM(2, k, 0, j, 0), &FM(3, k, 0, j, 0),
+                 mcc2, p->Em, Rcc6, Rcc12, 6.0 * Rcc6, 12.0 * Rcc12);
+      lj_mot_mot(box, s->xd[o0], s->yd[o0], s->zd[o0], s->xd[j1], s->yd[j1], s->zd[j1],
+                 &FM(0, k, 0, j, 1), &FM(1, k, 0, j, 1), &FM(2, k, 0, j, 1), &FM(3, k, 0, j, 1),
+                 mcn2, p->Em, Rcn6, Rcn12, 6.0 * Rcn6, 12.0 * Rcn12);
+      lj_mot_mot(box, s->xd[o1], s->yd[o1], s->zd[o1], s->xd[j0], s->yd[j0], s->zd[j0],
+                 &FM(0, k, 1, j, 0), &FM(1, k, 1, j, 0), &FM(2, k, 1, j, 0), &FM(3, k, 1, j, 0),
+                 mcn2, p->Em, Rcn6, Rcn12, 6.0 * Rcn6, 12.0 * Rcn12);
+      lj_mot_mot(box, s->xd[o1], s->yd[o1], s->zd[o1], s->xd[j1], s->yd[j1], s->zd[j1],
+                 &FM(0, k, 1, j, 1), &FM(1, k, 1, j, 1), &FM(2, k, 1, j, 1), &FM(3, k, 1, j, 1),
+                 mnn2, p->Em, Rnn6, Rnn12, 6.0 * Rnn6, 12.0 * Rnn12);
+    }
+    /* evolve_mod.f95:597-600 passes (R3,R9) then (3R^3, 9R^9) into the dummies
+     * named (r9R9, r3R3): the swap is part of the reference arithmetic. */
+    lj_mot_wall(box, s->zd[o0], &fwz[o0], p->Ew, Rwc3, Rwc9, r3Rwc3, r9Rwc9, wcC2, &wpot[o0]);
+    lj_mot_wall(box, s->zd[o1], &fwz[o1], p->Ew, Rwn3, Rwn9, r3Rwn3, r9Rwn9, wcN2, &wpot[o1]);
+  }
+#undef FM
+}
+
+void or_co_restart_forces(const or_co_params *p, or_co_state *s) {
+  /* restart_mod.f95:429-495, after the lists of or_co_setup_lists: every
+   * force, pair force and potential is zeroed and evaluated again from the
+   * restored positions (A particles inside a catalytic cutoff turn B on the
+   * way, as in any force evaluation); TPotential is summed in the restart's
+   * running order */
+  const int nd = p->numdims;
+  double fwz[2 * 64], wpot[2 * 64];
+  if (nd > 64) abort();
+  double *Fm = calloc((size_t)(4 * nd * 2 * nd * 2), sizeof(double));
+  int64_t nd4 = (int64_t)nd * 2 * nd * 2;
+#define FM(c, k, i, j, l) Fm[(int64_t)(c) * nd4 + (k) + (int64_t)(i) * nd + (int64_t)(j) * 2 * nd + (int64_t)(l) * 2 * nd * nd]
+  for (int k = 0; k < nd; k++)
+    for (int i = 0; i < 2; i++) {
+      int64_t o = IX2(k, i, nd);
+      s->fxd[o] = 0.0; s->fyd[o] = 0.0; s->fzd[o] = 0.0;
+      fwz[o] = 0.0; wpot[o] = 0.0;
+      for (int64_t j = 0; j < s->max_nb; j++) {
+        int64_t q = IX3(k, i, j, nd);
+        s->fpx[q] = 0.0; s->fpy[q] = 0.0; s->fpz[q] = 0.0; s->pot[q] = 0.0;
+      }
+    }
+  s->tpotential = 0.0;
+  co_pair_terms(p, s, Fm, fwz, wpot);
+  for (int k = 0; k < nd; k++)
+    for (int i = 0; i < 2; i++) {
+      int64_t o = IX2(k, i, nd);
+      for (int64_t j = 0; j < s->nb_num[o]; j++) {
+        int64_t q = IX3(k, i, j, nd);
+        s->tpotential = s->tpotential + s->pot[q];
+        s->fxd[o] = s->fxd[o] - s->fpx[q];
+        s->fyd[o] = s->fyd[o] - s->fpy[q];
+        s->fzd[o] = s->fzd[o] - s->fpz[q];
+      }
+      for (int j = 0; j < nd; j++)
+        for (int l = 0; l < 2; l++) {
+          s->fxd[o] = s->fxd[o] + FM(0, k, i, j, l);
+          s->fyd[o] = s->fyd[o] + FM(1, k, i, j, l);
+          s->fzd[o] = s->fzd[o] + FM(2, k, i, j, l);
+          if (j < k) s->tpotential = s->tpotential + FM(3, k, i, j, l);
+        }
+      s->tpotential = s->tpotential + wpot[o];
+      s->fzd[o] = s->fzd[o] + fwz[o];
+    }
+#undef FM
+  free(Fm);
+}
+
+void or_co_evolve(const or_co_params *p, or_co_state *s) {
+  const int nd = p->numdims;
+  const double MDt = p->MDt;
+  const double hc = (0.5 * MDt) / p->Mc, hn = (0.5 * MDt) / p->Mn;
+  const double IIMcIMn = 1.0 / ((1.0 / p->Mc) + (1.0 / p->Mn));
   const double *box = p->box;
   double QX[2 * 64], QY[2 * 64], QZ[2 * 64];
   if (nd > 64) abort();
@@ -539,37 +629,7 @@ void or_co_evolve(const or_co_params *p, or_co_state *s) {
     s->time_penetrate = penetrate_steps(p, s);                          /* :555-563 */
     s->time_count = 0;
   }
-  for (int k = 0; k < nd; k++) {                                        /* :567-602 */
-    int64_t o0 = IX2(k, 0, nd), o1 = IX2(k, 1, nd);
-    or_co_lj_mot_sol(1, box, s->flag, s->x, s->y, s->z, s->xd[o0], s->yd[o0], s->zd[o0],
-                     &s->fpx[IX3(k, 0, 0, nd)], &s->fpy[IX3(k, 0, 0, nd)], &s->fpz[IX3(k, 0, 0, nd)],
-                     &s->pot[IX3(k, 0, 0, nd)], p->RPab, p->RPba, cutC2, p->EaC, p->EbC, p->EsC,
-                     Rc6, Rc12, r12Rc12, r6Rc6, s->nb_num[o0], &s->nb[IX3(k, 0, 0, nd)], 2 * nd);
-    or_co_lj_mot_sol(2, box, s->flag, s->x, s->y, s->z, s->xd[o1], s->yd[o1], s->zd[o1],
-                     &s->fpx[IX3(k, 1, 0, nd)], &s->fpy[IX3(k, 1, 0, nd)], &s->fpz[IX3(k, 1, 0, nd)],
-                     &s->pot[IX3(k, 1, 0, nd)], p->RPab, p->RPba, cutN2, p->EaN, p->EbN, p->EsN,
-                     Rn6, Rn12, r12Rn12, r6Rn6, s->nb_num[o1], &s->nb[IX3(k, 1, 0, nd)], 2 * nd);
-    for (int j = 0; j < nd; j++) {
-      if (j == k) continue;
-      int64_t j0 = IX2(j, 0, nd), j1 = IX2(j, 1, nd);
-      lj_mot_mot(box, s->xd[o0], s->yd[o0], s->zd[o0], s->xd[j0], s->yd[j0], s->zd[j0],
-                 &FM(0, k, 0, j, 0), &FM(1, k, 0, j, 0), &FM(2, k, 0, j, 0), &FM(3, k, 0, j, 0),
-                 mcc2, p->Em, Rcc6, Rcc12, 6.0 * Rcc6, 12.0 * Rcc12);
-      lj_mot_mot(box, s->xd[o0], s->yd[o0], s->zd[o0], s->xd[j1], s->yd[j1], s->zd[j1],
-                 &FM(0, k, 0, j, 1), &FM(1, k, 0, j, 1), &FM(2, k, 0, j, 1), &FM(3, k, 0, j, 1),
-                 mcn2, p->Em, Rcn6, Rcn12, 6.0 * Rcn6, 12.0 * Rcn12);
-      lj_mot_mot(box, s->xd[o1], s->yd[o1], s->zd[o1], s->xd[j0], s->yd[j0], s->zd[j0],
-                 &FM(0, k, 1, j, 0), &FM(1, k, 1, j, 0), &FM(2, k, 1, j, 0), &FM(3, k, 1, j, 0),
-                 mcn2, p->Em, Rcn6, Rcn12, 6.0 * Rcn6, 12.0 * Rcn12);
-      lj_mot_mot(box, s->xd[o1], s->yd[o1], s->zd[o1], s->xd[j1], s->yd[j1], s->zd[j1],
-                 &FM(0, k, 1, j, 1), &FM(1, k, 1, j, 1), &FM(2, k, 1, j, 1), &FM(3, k, 1, j, 1),
-                 mnn2, p->Em, Rnn6, Rnn12, 6.0 * Rnn6, 12.0 * Rnn12);
-    }
-    /* evolve_mod.f95:597-600 passes (R3,R9) then (3R^3, 9R^9) into the dummies
-     * named (r9R9, r3R3): the swap is part of the reference arithmetic. */
-    lj_mot_wall(box, s->zd[o0], &fwz[o0], p->Ew, Rwc3, Rwc9, r3Rwc3, r9Rwc9, wcC2, &wpot[o0]);
-    lj_mot_wall(box, s->zd[o1], &fwz[o1], p->Ew, Rwn3, Rwn9, r3Rwn3, r9Rwn9, wcN2, &wpot[o1]);
-  }
+  co_pair_terms(p, s, Fm, fwz, wpot);
   for (int k = 0; k < nd; k++)                                          /* :609-636 */
     for (int i = 0; i < 2; i++) {
       int64_t o = IX2(k, i, nd);
@@ -1209,7 +1269,8 @@ int or_co_run(int64_t *ip, double *rp, int64_t nsteps, int32_t do_setup, int32_t
   memcpy(rng.mt, mt, sizeof rng.mt);
   rng.mti = *mti;
   int rc = 0;
-  if (do_setup) or_co_setup_lists(&p, &s, do_setup == 2);
+  if (do_setup) or_co_setup_lists(&p, &s, do_setup == 2 || do_setup == 4);
+  if (do_setup >= 3) or_co_restart_forces(&p, &s);
   for (int64_t it = 0; it < nsteps && rc == 0; it++) rc = or_co_time_step(&p, &s, &m, &rng);
   memcpy(mt, rng.mt, sizeof rng.mt);
   *mti = rng.mti;

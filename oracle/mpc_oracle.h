@@ -114,6 +114,9 @@ typedef struct {
 int or_co_time_step(const or_co_params *p, or_co_state *s, or_co_mpc *m, or_mtmod_t *rng);
 /* main_pattern.f95:170-188: sort, neighbour lists, in/out lists, time_penetrate */
 void or_co_setup_lists(const or_co_params *p, or_co_state *s, int do_sort);
+/* restart_mod.f95:425-495: forces, pair forces and potential evaluated again
+ * from the restored positions over the lists or_co_setup_lists built */
+void or_co_restart_forces(const or_co_params *p, or_co_state *s);
 
 /* ---- selkov_code (module globals become explicit arrays) ---- */
 typedef struct {

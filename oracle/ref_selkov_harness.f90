@@ -111,4 +111,32 @@ subroutine ref_sk_run(ip, rp, seed, cstep0, nsteps, flg, xs, ys, zs, vxs, vys, v
   DEALLOCATE(newFXsol,newFYsol,newFZsol,newpFXsol,newpFYsol,newpFZsol)
 end subroutine
 
+! The fixed-form records of Output_SelkovDimer.f90 (FORMAT 15 :26, 16 :139,
+! 18 :186, 19 :238, 17 :305) and Main_SelkovDimer.f90 (23 :158), written with
+! the reference's FORMAT statements into the file at p: one record of each.
+subroutine ref_sk_records(p, n, cstep, v) bind(C, name="ref_sk_records")
+  integer(c_int32_t), value :: n, cstep
+  character(kind=c_char) :: p(n)
+  real(c_double) :: v(10)
+  character(len=n) :: fname
+  integer :: i
+  do i = 1, n
+    fname(i:i) = p(i)
+  end do
+  OPEN(UNIT=10, FILE=fname, STATUS='replace', ACTION='WRITE')
+  WRITE(10,15) cstep, v(1), v(2), v(3)
+  WRITE(10,16) v(1), v(2), v(3), cstep
+  WRITE(10,18) cstep, v(1), v(2), v(3), v(4)
+  WRITE(10,19) cstep, v(1), v(2), v(3), v(4), v(5), v(6), v(7), v(8), v(9), v(10)
+  WRITE(10,17) cstep, v(1), v(2), v(3), v(4), v(5), v(6), v(7), v(8)
+  WRITE(10,23) cstep
+  CLOSE(10)
+15 FORMAT(I8,1X,3(E23.16,1X))
+16 FORMAT(3(ES23.16,1X),I8)
+17 FORMAT(I8,8(1X,ES23.16))
+18 FORMAT(I8,4(1X,ES23.16))
+19 FORMAT(I8,10(1X,ES23.16))
+23 FORMAT(I9)
+end subroutine
+
 end module ref_selkov_harness

@@ -49,6 +49,17 @@ COLLECTIVE_CASES = {
     "co_six_dimers_sort": dict(params=dict(Lx=28.0, Ly=28.0, Lz=16.0, numdims=6, rhoa=4.0, rhob=0.5, Rc=1.5,
                                            Rn=2.0, Dcn=4.0, MPCt=0.1, skinrad=0.3, sort_step=20, k1=0.05, EbN=0.1),
                                seed=21, mt_seed=4357787, steps=50, flags_mix=True, react_type=7, sort=True),
+    # restarts (restart_mod.f95:416-495) in the middle of the run: lists and
+    # forces evaluated again from the positions, with and without the sort
+    "co_one_dimer_restart": dict(params=dict(Lx=16.0, Ly=16.0, Lz=20.0, rhoa=5.0, Rc=1.5, Rn=2.0, Dcn=4.0, MPCt=0.1,
+                                             skinrad=0.3, sort_step=1000000, k1=0.05),
+                                 seed=3, mt_seed=4357787, steps=40, restart_at=20, flags_mix=False, react_type=7,
+                                 sort=False),
+    "co_two_dimers_restart_sort": dict(params=dict(Lx=20.0, Ly=20.0, Lz=24.0, numdims=2, rhoa=4.0, rhob=1.0, Rc=1.5,
+                                                   Rn=2.0, Dcn=4.0, MPCt=0.1, skinrad=0.5, sort_step=20, k1=0.05,
+                                                   react_box_type=2, EbN=0.1),
+                                       seed=11, mt_seed=4357787, steps=60, restart_at=40, flags_mix=True,
+                                       react_type=2, sort=True),
 }
 SELKOV_CASES = {
     "sk_reactive": dict(params=dict(L=40.0, rho=2.0, Rc=2.0, Rn=4.0, Dcn=7.0, MDt=0.005, MPCt=0.1, NG=20000,
@@ -89,7 +100,16 @@ def run_collective(case, use_ref):
             np.savetxt(os.path.join(d, "hilbert.txt"), c, fmt="%d", delimiter="\t")
             os.chdir(d)
     O.co_run(s, 0, do_setup=2 if case["sort"] else 1, use_ref=use_ref, tour=tour)
-    O.co_run(s, case["steps"], use_ref=use_ref, tour=tour)
+    ra = case.get("restart_at")
+    if ra:
+        O.co_run(s, ra, use_ref=use_ref, tour=tour)
+        for k in ("fxd", "fyd", "fzd", "fpx", "fpy", "fpz", "pot"):   # a restart frame carries no forces
+            s[k][...] = 0.0
+        O.co_run(s, 0, do_setup=4 if (case["sort"] and ra % int(p["sort_step"]) == 0) else 3, use_ref=use_ref,
+                 tour=tour)
+        O.co_run(s, case["steps"] - ra, use_ref=use_ref, tour=tour)
+    else:
+        O.co_run(s, case["steps"], use_ref=use_ref, tour=tour)
     os.chdir(ROOT)
     return s
 
