@@ -175,6 +175,20 @@ def write_timestep(directory: str, cstep: int, counts: tuple[int, int, int]):
     write_files(directory, {"timestep.dat": _lines(FMT23, [[cstep], [counts[0]], [counts[1]], [counts[2]]])})
 
 
+def output_records(engine, p: dict) -> dict[str, str]:
+    """The record files of Output_SelkovDimer.f90 for the engine's state now
+    (the reference writes each at its own cadence Freq1..Freq5): dimer
+    records, solvent dumps, DotVZ, Concentration and EnergyMomentum."""
+    s = engine.get_state()
+    cstep = int(s["cstep"])
+    files = dimer_records(s, cstep)
+    files.update(solvent_dumps(s, cstep))
+    files["DotVZ.dat"] = format_record(FMT18, [cstep, *dot_vz(s, p)]) + "\n"
+    files["Concentration.dat"] = format_record(FMT19, [cstep, *concentration(s, engine.get_cells(), p)]) + "\n"
+    files["EnergyMomentum.dat"] = format_record(FMT17, [cstep, *energy_momentum(s, p)]) + "\n"
+    return files
+
+
 def save_restart(engine, directory: str) -> int:
     """Write what Restart.f90 reads (dimer records, solvent dumps, timestep.dat)
     for the engine's state at a collision step; returns cstep."""
