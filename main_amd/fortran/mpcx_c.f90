@@ -119,6 +119,57 @@ module mpcx_c
       integer(c_int32_t) :: flag(*)
       integer(c_int) :: rc
     end function
+
+    ! restart files (main_pattern.f95:128-146,243-251; mtmod.f90:152-229;
+    ! initial_mod.f95:118; restart_mod.f95:410-414); paths NUL-terminated
+    function mpcx_timestep_write(path, cstep, n_a, n_b, n_s) bind(C, name="mpcx_timestep_write") result(rc)
+      import :: c_int, c_int32_t, c_int64_t, c_char
+      character(kind=c_char) :: path(*)
+      integer(c_int32_t), value :: cstep
+      integer(c_int64_t), value :: n_a, n_b, n_s
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_timestep_read(path, cstep, n_a, n_b, n_s) bind(C, name="mpcx_timestep_read") result(rc)
+      import :: c_int, c_int32_t, c_int64_t, c_char
+      character(kind=c_char) :: path(*)
+      integer(c_int32_t) :: cstep
+      integer(c_int64_t) :: n_a, n_b, n_s
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_mt_state_write(path, mt, mti, nstreams) bind(C, name="mpcx_mt_state_write") result(rc)
+      import :: c_int, c_int32_t, c_char
+      character(kind=c_char) :: path(*)
+      integer(c_int32_t) :: mt(*), mti(*)
+      integer(c_int32_t), value :: nstreams
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_mt_state_read(path, mt, mti, max_streams, nstreams) bind(C, name="mpcx_mt_state_read") result(rc)
+      import :: c_int, c_int32_t, c_char
+      character(kind=c_char) :: path(*)
+      integer(c_int32_t) :: mt(*), mti(*)
+      integer(c_int32_t), value :: max_streams
+      integer(c_int32_t) :: nstreams
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_motor_positions_write(path, cm, nd) bind(C, name="mpcx_motor_positions_write") result(rc)
+      import :: c_int, c_int32_t, c_double, c_char
+      character(kind=c_char) :: path(*)
+      real(c_double) :: cm(*)
+      integer(c_int32_t), value :: nd
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_motor_positions_read(path, cm, nd) bind(C, name="mpcx_motor_positions_read") result(rc)
+      import :: c_int, c_int32_t, c_double, c_char
+      character(kind=c_char) :: path(*)
+      real(c_double) :: cm(*)
+      integer(c_int32_t), value :: nd
+      integer(c_int) :: rc
+    end function
   end interface
 
 contains

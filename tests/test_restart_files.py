@@ -15,6 +15,7 @@ cases depend on it.
 """
 import ctypes as C
 import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -24,6 +25,8 @@ from main_amd import selkov_files as SF
 from main_amd._lib import MtmodStream
 from oracle import oracle as O
 
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FCHECK = os.path.join(ROOT, "tests", "fortran", "_build", "restart_files_check")
 REF_CO = O.ref_lib("collective")
 REF_SK = O.ref_lib("selkov")
 need_ref_co = pytest.mark.skipif(REF_CO is None, reason="oracle/_ref not built (needs /root/reference)")
@@ -283,3 +286,12 @@ def test_restart_cases_depend_on_the_restart_forces():
         out.append(s)
     assert np.max(np.abs(out[0]["vx"] - out[1]["vx"])) > 1e-6
     assert np.max(np.abs(out[0]["xd"] - out[1]["xd"])) > 1e-7
+
+
+@pytest.mark.skipif(not os.path.exists(FCHECK), reason="Fortran check not built (make -C main_amd/fortran)")
+def test_fortran_statements_read_and_write_the_library_files(tmp_path):
+    """A flang program (tests/fortran/restart_files_check.f90) calls the codecs
+    through the ISO_C_BINDING interfaces and reads/writes the same files with
+    the reference's own list-directed and unformatted statements."""
+    r = subprocess.run([FCHECK], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "restart files ok" in r.stdout, r.stdout + r.stderr
