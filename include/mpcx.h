@@ -213,6 +213,12 @@ int mpcx_co_set_near_workgroups(mpcx_co* e, int32_t G);
  * particles and the dimers stay FP64).  Device Philox draws only; call before
  * mpcx_co_setup.  No counterpart in the reference (FP64 throughout). */
 int mpcx_co_set_precision(mpcx_co* e, int32_t split);
+/* Workgroups of the interval's streaming/binning kernel (co_bulk, or sp_bulk
+ * in split storage), a grid-stride loop over the particles: 0 (the default)
+ * takes one per 256 particles up to 8192; a positive value is used as given
+ * (at most 65536), for tuning.  Results do not depend on it.  No counterpart
+ * in the reference. */
+int mpcx_co_set_bulk_grid(mpcx_co* e, int32_t grid);
 /* Collision-cell data of the last collision: (mpcnumcell) arrays, 0-based
  * cell of each particle, and unshifted census counts (3, vol). */
 int mpcx_co_get_cells(mpcx_co* e, double* vcm_uv /* (mpcnumcell, 8) */, int32_t* cell_of,

@@ -114,3 +114,24 @@ def test_split_many_dimer_slab_runs():
     assert g["cstep"] == 3 * p["MPCtstep"]
     ke = 0.5 * np.mean(g["vx"] ** 2 + g["vy"] ** 2 + g["vz"] ** 2)
     assert abs(ke / (1.5 * p["T"]) - 1.0) < 0.05
+
+
+def test_split_keeps_particles_below_the_top_wall():
+    """FP32 offsets cannot hold a z within half an ulp below Lz: a far particle
+    that ends its flight there (or converts from FP64 there) must stay in the
+    top cell instead of leaving the cell arrays (a regression test: such a
+    particle used to get cell z index Lz and stop the run with a range error)."""
+    p = _params(L=20)
+    s = S.collective_initial_state(p, seed=8)
+    Lz = float(p["box"][2])
+    cases = [(Lz - 2.0 ** -24, 6e-8), (Lz - 1e-12, 0.0), (Lz - 2.0 ** -24, 2.0 ** -26), (Lz - 2.0 ** -25, 1e-9)]
+    for n, (z, vz) in enumerate(cases):
+        s["x"][n], s["y"][n], s["z"][n] = 1.5 + 0.1 * n, 1.5, z
+        s["vx"][n], s["vy"][n], s["vz"][n] = 0.0, 0.0, vz
+    e = _engine(p, s, True)
+    try:
+        e.run(2)
+        g = e.get_state()
+    finally:
+        e.close()
+    assert np.all(g["z"] < Lz) and np.all(g["z"] >= 0.0)
