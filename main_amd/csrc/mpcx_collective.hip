@@ -223,6 +223,19 @@ __device__ __forceinline__ unsigned memberships(double x, double y, double z, co
   return m;
 }
 
+// X - L*floor(X/L) of the reference without the division: for -L <= X < 2L a
+// correctly rounded X/L lies in [-1,0), [0,1) or [1,2) exactly when X does, so
+// floor(X/L) is -1, 0 or 1 by comparison and the subtraction is the same
+// IEEE operation (X - (-L) is X + L; X - 0*L keeps X, -0.0 included)
+__device__ __forceinline__ double wrap_periodic(double X, double L) {
+  return X < 0.0 ? X + L : (X >= L ? X - L : X);
+}
+// the same for any X: the division only outside [-L, 2L)
+__device__ __forceinline__ double wrap_periodic_any(double X, double L) {
+  if (__builtin_expect(X < -L || X >= 2.0 * L, 0)) return X - L * floor(X / L);
+  return wrap_periodic(X, L);
+}
+
 // ------------------------------------------------------------- dimer (thread 0)
 // The dimer state of one system, held in registers while thread 0 integrates
 // it (all indices are compile-time after unrolling).  Sphere s = k + i*ND.
@@ -270,8 +283,8 @@ __device__ __forceinline__ void rattle_positions(const CoP& P, Dimer<ND>& d, int
       double x = d.v[DX][s] + MDt * d.v[DQX][s];
       double y = d.v[DY][s] + MDt * d.v[DQY][s];
       d.v[DZ][s] = d.v[DZ][s] + MDt * d.v[DQZ][s];
-      d.v[DX][s] = x - L0 * floor(x / L0);
-      d.v[DY][s] = y - L1 * floor(y / L1);
+      d.v[DX][s] = wrap_periodic_any(x, L0);
+      d.v[DY][s] = wrap_periodic_any(y, L1);
     }
     for (long long acc = 0;; acc++) {
       if (acc == RATTLE_CAP) { raise_error(err, ERR_RATTLE); break; }
@@ -288,8 +301,8 @@ __device__ __forceinline__ void rattle_positions(const CoP& P, Dimer<ND>& d, int
       double x1 = (prX[1] + d.v[DQX][s1] * MDt) - (g2 * dP0) / P.Mn;
       double y1 = (prY[1] + d.v[DQY][s1] * MDt) - (g2 * dP1) / P.Mn;
       double z1 = (prZ[1] + d.v[DQZ][s1] * MDt) - (g2 * dP2) / P.Mn;
-      d.v[DX][s0] = x0 - L0 * floor(x0 / L0); d.v[DY][s0] = y0 - L1 * floor(y0 / L1); d.v[DZ][s0] = z0;
-      d.v[DX][s1] = x1 - L0 * floor(x1 / L0); d.v[DY][s1] = y1 - L1 * floor(y1 / L1); d.v[DZ][s1] = z1;
+      d.v[DX][s0] = wrap_periodic_any(x0, L0); d.v[DY][s0] = wrap_periodic_any(y0, L1); d.v[DZ][s0] = z0;
+      d.v[DX][s1] = wrap_periodic_any(x1, L0); d.v[DY][s1] = wrap_periodic_any(y1, L1); d.v[DZ][s1] = z1;
       d.v[DQX][s0] = d.v[DQX][s0] + (g2 * dP0) / (P.Mc * MDt);
       d.v[DQY][s0] = d.v[DQY][s0] + (g2 * dP1) / (P.Mc * MDt);
       d.v[DQZ][s0] = d.v[DQZ][s0] + (g2 * dP2) / (P.Mc * MDt);
@@ -438,6 +451,10 @@ struct NearShared {
   int wsum[NE_W + 1];
   double vmax2;
   int cstep, tc, tp, in_num, nev, near_num, rebuild, clustered;
+  // scalars the MD steps read or write, kept here for the launch (no global
+  // memory round trip on the serial path; written back with the scalars)
+  double tpot, fmaxv2, x0[MAXS], y0[MAXS], z0[MAXS];
+  int ev[MAXEV];
   CoP P;   // launch parameters, read from LDS by the helpers below
   CoD D;
 };
@@ -693,7 +710,7 @@ __device__ __forceinline__ void ne_compact(const CoP& P, const CoD& D, const Mem
   }
   if (threadIdx.x == 0) {
     g_ne.in_num = total;
-    double fm = __longlong_as_double((long long)D.sc->far_maxv2);
+    double fm = g_ne.fmaxv2;
     double vm = g_ne.vmax2;
     if (g_ne.clustered) vm = clu_exchange_max(D.clu, D.cid, vm, &D.sc->err);
     double maxt = fmax(fm, vm);
@@ -813,13 +830,13 @@ __device__ __forceinline__ void ne_begin_step_regs(const CoP& P, const CoD& D, D
   const int rb = (maxv > P.hs2 || g_ne.tc == g_ne.tp || g_ne.cstep % P.MPCtstep == 0);
   g_ne.rebuild = rb;
   if (rb) {
-    if (g_ne.nev < MAXEV) D.sc->events[g_ne.nev] = g_ne.tc; else raise_error(&D.sc->err, ERR_EVENTS);
+    if (g_ne.nev < MAXEV) g_ne.ev[g_ne.nev] = g_ne.tc; else raise_error(&D.sc->err, ERR_EVENTS);
     g_ne.nev += 1;
 #pragma unroll
     for (int s = 0; s < 2 * ND; s++) {   // the near/far split needs every sphere inside its guard
-      double a = pbc_half(d.v[DX][s] - D.sc->x0[s], P.box[0]);
-      double b = pbc_half(d.v[DY][s] - D.sc->y0[s], P.box[1]);
-      double c = d.v[DZ][s] - D.sc->z0[s];
+      double a = pbc_half(d.v[DX][s] - g_ne.x0[s], P.box[0]);
+      double b = pbc_half(d.v[DY][s] - g_ne.y0[s], P.box[1]);
+      double c = d.v[DZ][s] - g_ne.z0[s];
       if (a * a + b * b + c * c > P.guard * P.guard) raise_error(&D.sc->err, ERR_NEAR_GUARD);
     }
   }
@@ -846,7 +863,7 @@ __device__ __attribute__((noinline)) void ne_end_step(bool last) {
 #pragma unroll
   for (int v = 0; v < 4 * 2 * ND + 1; v++) fin[v] = g_ne.fin[v];
   dimer_forces_and_rattle_v<ND>(P, d, fin, tpot, &D.sc->err);
-  D.sc->tpotential = tpot;
+  g_ne.tpot = tpot;
   if (!last) ne_begin_step_regs<ND>(P, D, d, fin[4 * 2 * ND]);
   dimer_store<ND>(d, g_ne.sd);
 }
@@ -989,6 +1006,8 @@ __device__ __forceinline__ bool ne_excluded(const CoP& P, const CoD& D, int k) {
 __device__ __forceinline__ void ne_save_scalars(CoScalars* sc) {
   sc->cstep = g_ne.cstep; sc->time_count = g_ne.tc; sc->time_penetrate = g_ne.tp; sc->in_num = g_ne.in_num;
   sc->n_ev = g_ne.nev; sc->near_vmax2 = g_ne.vmax2;
+  sc->tpotential = g_ne.tpot;
+  for (int e = 0; e < g_ne.nev && e < MAXEV; e++) sc->events[e] = g_ne.ev[e];
 }
 
 // One interval of the near set of one system.  P and D are the launch
@@ -1013,6 +1032,10 @@ __device__ __forceinline__ void near_body(const CoP P, const CoD D, int nsteps, 
     g_ne.vmax2 = sc->near_vmax2;
     g_ne.rebuild = 0;
     g_ne.clustered = (mode & M_CLUSTER) != 0;
+    g_ne.tpot = sc->tpotential;
+    g_ne.fmaxv2 = __longlong_as_double((long long)sc->far_maxv2);
+    for (int s = 0; s < NSPH; s++) { g_ne.x0[s] = sc->x0[s]; g_ne.y0[s] = sc->y0[s]; g_ne.z0[s] = sc->z0[s]; }
+    for (int e = 0; e < g_ne.nev && e < MAXEV; e++) g_ne.ev[e] = sc->events[e];
   }
   __syncthreads();
   if (g_ne.near_num > NE_NEAR) {   // the set does not fit the lanes' membership registers
@@ -2379,13 +2402,6 @@ __device__ __forceinline__ void collision_shift(const CoP& P, const CoD& D, doub
   zs = P.a0 * (u2 - 0.5);
 }
 
-// X - L*floor(X/L) of the reference without the division: for -L <= X < 2L a
-// correctly rounded X/L lies in [-1,0), [0,1) or [1,2) exactly when X does, so
-// floor(X/L) is -1, 0 or 1 by comparison and the subtraction is the same
-// IEEE operation (X - (-L) is X + L; X - 0*L keeps X, -0.0 included)
-__device__ __forceinline__ double wrap_periodic(double X, double L) {
-  return X < 0.0 ? X + L : (X >= L ? X - L : X);
-}
 
 // mpc_rotation_mod.f95:116-145: cell of the shifted grid, 0-based
 __device__ __forceinline__ long long shifted_cell(const CoP& P, double x, double y, double z, double xs, double ys,
