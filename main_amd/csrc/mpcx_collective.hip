@@ -2577,7 +2577,10 @@ __global__ __launch_bounds__(256) void co_scatter_b(const CoD* Ds, const Tile* t
 // (mpc_rotation_mod.f95:153-182), random axis (:184-193).  Thread t also does
 // the B->A reaction of unshifted cell t (reaction_mod.f95:41-62).
 constexpr int LB = 32;   // members sorted in LDS; larger cells sort in place in the bucket
-__device__ __forceinline__ void collide_body(const CoP& P, const CoD& D, long long t, int* sidx) {
+// stage: the workgroup's whole stretch of the bucket copied to LDS (its cells'
+// members are contiguous there, from bucket entry sbase), or null
+__device__ __forceinline__ void collide_body(const CoP& P, const CoD& D, long long t, int* sidx,
+                                             int* stage = nullptr, int sbase = 0) {
   if (t < P.mpcnumcell) {
     const int c = D.mpc_cell[t];
     int n = D.cnt[t];
@@ -2585,7 +2588,15 @@ __device__ __forceinline__ void collide_body(const CoP& P, const CoD& D, long lo
     const int* b = D.bucket + D.off[t];
     int* idx;
     int istride;
-    if (n <= LB) {
+    if (stage) {   // insertion sort in place in the staged stretch: ascending particle index
+      idx = stage + (D.off[t] - sbase);
+      istride = 1;
+      for (int j = 1; j < n; j++) {
+        int v = idx[j], k = j;
+        while (k > 0 && idx[k - 1] > v) { idx[k] = idx[k - 1]; k--; }
+        idx[k] = v;
+      }
+    } else if (n <= LB) {
       idx = sidx + threadIdx.x;
       istride = CT;
       for (int j = 0; j < n; j++) {   // insertion: ascending particle index
@@ -2635,8 +2646,10 @@ __device__ __forceinline__ void collide_body(const CoP& P, const CoD& D, long lo
     double st = sqrt(1.0 - th * th);
     double2* rec = (double2*)(D.cellrec + t * 8);
     rec[0] = make_double2(sx, sy);
-    rec[1] = make_double2(sz, st * cos(phi));
-    rec[2] = make_double2(st * sin(phi), th);
+    double sp, cp;
+    sincos(phi, &sp, &cp);   // one reduction; the same values as sin() and cos()
+    rec[1] = make_double2(sz, st * cp);
+    rec[2] = make_double2(st * sp, th);
     rec[3] = make_double2((double)n, 0.0);
   }
   if (t < P.rxnnumcell) {
@@ -2655,7 +2668,22 @@ __device__ __forceinline__ void collide_body(const CoP& P, const CoD& D, long lo
 
 __global__ __launch_bounds__(CT) void co_collide(CoP P, CoD D) {
   __shared__ int sidx[LB * CT];
-  collide_body(P, D, (long long)xcd_block(blockIdx.x, gridDim.x) * CT + threadIdx.x, sidx);
+  __shared__ int srange[2];
+  // the workgroup's cells t0 .. t0+CT-1 own one contiguous stretch of the
+  // bucket: read it once, coalesced, instead of member by member per thread
+  const long long t0 = (long long)xcd_block(blockIdx.x, gridDim.x) * CT;
+  if (threadIdx.x == 0) {
+    const long long te = min(t0 + CT, P.mpcnumcell);
+    srange[0] = t0 < te ? D.off[t0] : 0;
+    srange[1] = t0 < te ? D.off[te] : 0;
+  }
+  __syncthreads();
+  const int sb = srange[0], total = srange[1] - sb;
+  const bool staged = total >= 0 && total <= LB * CT;
+  if (staged)
+    for (int j = threadIdx.x; j < total; j += CT) sidx[j] = D.bucket[sb + j];
+  __syncthreads();
+  collide_body(P, D, t0 + threadIdx.x, sidx, staged ? sidx : nullptr, sb);
 }
 __global__ __launch_bounds__(CT) void co_collide_b(const CoP* Ps, const CoD* Ds, const Tile* tiles) {
   __shared__ int sidx[LB * CT];
@@ -3111,8 +3139,10 @@ __global__ __launch_bounds__(CT) void sp_collide(CoP P, CoD D, SpD S) {
     double st = sqrt(1.0 - th * th);
     double2* rec = (double2*)(D.cellrec + t * 8);
     rec[0] = make_double2(mx, my);
-    rec[1] = make_double2(mz, st * cos(phi));
-    rec[2] = make_double2(st * sin(phi), th);
+    double sp, cp;
+    sincos(phi, &sp, &cp);   // one reduction; the same values as sin() and cos()
+    rec[1] = make_double2(mz, st * cp);
+    rec[2] = make_double2(st * sp, th);
     rec[3] = make_double2((double)n, 0.0);
   }
   if (t < P.rxnnumcell) {
