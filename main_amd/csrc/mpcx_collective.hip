@@ -692,18 +692,51 @@ __device__ __forceinline__ void ne_compact(const CoP& P, const CoD& D, const Mem
   }
   int base = g_ne.wsum[wv];
   const unsigned long long lt = lanes_below();
-  for (int j = 0; j < E; j++) {
-    const int q = qb + j * 64;
-    const unsigned m = mk.get(j);
-    const unsigned long long b = __ballot(m != 0u);
-    const int k = base + __popcll(b & lt);
-    base += __popcll(b);
-    if (q < nn) {
+  // three entries per pass: their slot loads are issued together, so the L2
+  // latency is paid once per pass rather than once per entry
+  constexpr int G = 3;
+  for (int j0 = 0; j0 < E; j0 += G) {
+    int qg[G], kg[G];
+    unsigned mg[G];
+#pragma unroll
+    for (int u = 0; u < G; u++) {
+      const int j = j0 + u;
+      qg[u] = qb + j * 64;
+      mg[u] = j < E ? mk.get(j) : 0u;
+      const unsigned long long b = __ballot(mg[u] != 0u);
+      kg[u] = base + __popcll(b & lt);
+      base += __popcll(b);
+      if (j >= E || qg[u] >= nn) mg[u] = 0u, qg[u] = -1;
+    }
+    double sx[G], sy[G], sz[G], svx[G], svy[G], svz[G];
+    int sf[G];
+#pragma unroll
+    for (int u = 0; u < G; u++)
+      if (mg[u] != 0u && kg[u] < D.in_cap) {
+        const int q = qg[u];
+        sx[u] = D.ns_x[q]; sy[u] = D.ns_y[q]; sz[u] = D.ns_z[q];
+        svx[u] = D.ns_vx[q]; svy[u] = D.ns_vy[q]; svz[u] = D.ns_vz[q];
+        sf[u] = D.ns_flag[q];
+      }
+#pragma unroll
+    for (int u = 0; u < G; u++) {
+      const int q = qg[u], k = kg[u];
+      const unsigned m = mg[u];
+      if (q < 0) continue;
       if (m != 0u && k < D.in_cap) {
         D.in_list[k] = q;
         D.in_memb[k] = m;
-        if (k < LCAP) { g_ne.lmemb[k] = m; g_ne.lmsd[k] = 0.0; } else D.in_msd[k] = 0.0;
-        slot_from_ns(D, k, q);
+        if (k < LCAP) {
+          g_ne.lmemb[k] = m; g_ne.lmsd[k] = 0.0;
+          g_ne.lx[k] = sx[u]; g_ne.ly[k] = sy[u]; g_ne.lz[k] = sz[u];
+          g_ne.lvx[k] = svx[u]; g_ne.lvy[k] = svy[u]; g_ne.lvz[k] = svz[u];
+          g_ne.lflag[k] = sf[u];
+        } else {
+          D.in_msd[k] = 0.0;
+          D.sl_x[k] = sx[u]; D.sl_y[k] = sy[u]; D.sl_z[k] = sz[u];
+          D.sl_vx[k] = svx[u]; D.sl_vy[k] = svy[u]; D.sl_vz[k] = svz[u];
+          D.sl_flag[k] = sf[u];
+        }
       }
       D.ns_stat[q] = m != 0u ? ST_INSIDE : 0;
     }
