@@ -2811,16 +2811,25 @@ __global__ void set_coll(CoScalars* sc, unsigned long long v) { sc->coll = v; }
 
 // --------------------------------------------------------------- Hilbert sort
 // sort_hilbert_mod.f95:50-92: stable bin by tour position of the unshifted cell
+// first and step are multiples of the wave size: every wave runs the loop as a
+// whole, one counter atomic per distinct cell of the wave
 __device__ __forceinline__ void sort_bin_body(const CoP& P, const CoD& D, const int* tourpos, int* scnt, int* key,
                                               unsigned short* rk, long long first, long long end, long long step) {
-  for (long long i = first + threadIdx.x; i < end; i += step) {
-    long long u = cell_unshifted0(D.x[i], D.y[i], D.z[i], P.box[0], P.box[1]);
-    if (u < 0 || u >= P.vol || tourpos[u] < 0) { raise_error(&D.sc->err, ERR_CELL_RANGE); key[i] = 0; rk[i] = 0; continue; }
-    int t = tourpos[u];
-    int r = atomicAdd(&scnt[t], 1);
-    if (r >= BMAX) raise_error(&D.sc->err, ERR_BUCKET);
-    key[i] = t;
-    rk[i] = (unsigned short)min(r, 65535);
+  for (long long base = first; base < end; base += step) {
+    const long long i = base + threadIdx.x;
+    int t = 0;
+    bool ok = false;
+    if (i < end) {
+      const long long u = cell_unshifted0(D.x[i], D.y[i], D.z[i], P.box[0], P.box[1]);
+      if (u < 0 || u >= P.vol || tourpos[u] < 0) { raise_error(&D.sc->err, ERR_CELL_RANGE); key[i] = 0; rk[i] = 0; }
+      else { t = tourpos[u]; ok = true; }
+    }
+    const int r = wave_atomic_rank(scnt, t, ok);
+    if (ok) {
+      if (r >= BMAX) raise_error(&D.sc->err, ERR_BUCKET);
+      key[i] = t;
+      rk[i] = (unsigned short)min(r, 65535);
+    }
   }
 }
 __global__ __launch_bounds__(256) void sort_bin(CoP P, CoD D, const int* tourpos, int* scnt, int* key,
@@ -2858,12 +2867,35 @@ __device__ __forceinline__ void sort_gather_body(const CoD& D, const int* sb, do
     if (lane == 32) nbits[w0 + 1] = (unsigned)(m >> 32);
   }
 }
-__global__ __launch_bounds__(256) void sort_gather(CoP P, CoD D, const int* sb, double* nx, double* ny, double* nz,
-                                                   double* nvx, double* nvy, double* nvz, int* nf,
-                                                   unsigned char* nst, unsigned* nbits) {
+
+// one array of the permutation at a time: the working set of each pass (one
+// source array) stays in the last-level cache, where the eight interleaved
+// arrays of sort_gather did not
+template <typename T>
+__global__ __launch_bounds__(256) void sort_gather_one(long long n, const int* __restrict__ sb,
+                                                       const T* __restrict__ src, T* __restrict__ dst) {
+  long long first, end;
+  block_range(n, gridDim.x, xcd_block(blockIdx.x, gridDim.x), first, end);
+  for (long long j = first + threadIdx.x; j < end; j += 256) dst[j] = src[sb[j]];
+}
+__global__ __launch_bounds__(256) void sort_gather_bits(CoP P, CoD D, const int* sb, unsigned char* nst,
+                                                        unsigned* nbits) {
   long long first, end;
   block_range(P.NT, gridDim.x, xcd_block(blockIdx.x, gridDim.x), first, end);
-  sort_gather_body(D, sb, nx, ny, nz, nvx, nvy, nvz, nf, nst, nbits, first, end, 256);
+  const int lane = threadIdx.x & 63;
+  for (long long base = first; base < end; base += 256) {
+    const long long j = base + threadIdx.x;
+    bool bit = false;
+    if (j < end) {
+      const int i = sb[j];
+      nst[j] = D.status[i];
+      bit = (D.near_bits[i >> 5] >> (i & 31)) & 1u;
+    }
+    unsigned long long m = __ballot(bit);   // the wave covers 64 consecutive j: two words
+    const long long w0 = (base + (threadIdx.x & ~63)) >> 5;
+    if (lane == 0) nbits[w0] = (unsigned)m;
+    if (lane == 32) nbits[w0 + 1] = (unsigned)(m >> 32);
+  }
 }
 
 }  // namespace mpcx
@@ -3713,8 +3745,15 @@ static int co_sort(mpcx_co* e, hipStream_t st) {
                      (const int*)e->d_soff, e->d_sb, BMAX);
   hipLaunchKernelGGL(fill_kernel<unsigned>, dim3(grid_for(e->nwords + 1, 256)), dim3(256), 0, st, e->alt_bits,
                      e->nwords + 1, 0u);
-  hipLaunchKernelGGL(sort_gather, dim3(grid_for(NT, 256)), dim3(256), 0, st, P, D, (const int*)e->d_sb, e->alt[0],
-                     e->alt[1], e->alt[2], e->alt[3], e->alt[4], e->alt[5], e->alt_flag, e->alt_status, e->alt_bits);
+  {
+    const int g = grid_for(NT, 256);
+    const int* sb = e->d_sb;
+    const double* src[6] = {D.x, D.y, D.z, D.vx, D.vy, D.vz};
+    for (int a = 0; a < 6; a++)
+      hipLaunchKernelGGL(sort_gather_one<double>, dim3(g), dim3(256), 0, st, NT, sb, src[a], e->alt[a]);
+    hipLaunchKernelGGL(sort_gather_one<int>, dim3(g), dim3(256), 0, st, NT, sb, (const int*)D.flag, e->alt_flag);
+    hipLaunchKernelGGL(sort_gather_bits, dim3(g), dim3(256), 0, st, P, D, sb, e->alt_status, e->alt_bits);
+  }
   std::swap(D.x, e->alt[0]); std::swap(D.y, e->alt[1]); std::swap(D.z, e->alt[2]);
   std::swap(D.vx, e->alt[3]); std::swap(D.vy, e->alt[4]); std::swap(D.vz, e->alt[5]);
   std::swap(D.flag, e->alt_flag); std::swap(D.status, e->alt_status); std::swap(D.near_bits, e->alt_bits);

@@ -123,15 +123,36 @@ __global__ void fill_kernel(T* p, long long n, T v) {
 }
 // members of each bucket in ascending order (insertion sort; buckets hold at
 // most `cap` entries, longer ones are cut and reported by the caller)
+// The workgroup's buckets t0..t0+255 are one stretch of sb (soff has ncell+1
+// entries): sorted in LDS when the stretch fits, in place in sb otherwise.
+constexpr int SO_CAP = 8192;
 __global__ __launch_bounds__(256) void sort_order(long long ncell, const int* scnt, const int* soff, int* sb, int cap) {
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ncell) return;
-  int n = min(scnt[t], cap), o = soff[t];
-  for (int j = 1; j < n; j++) {
-    int v = sb[o + j], k = j;
-    while (k > 0 && sb[o + k - 1] > v) { sb[o + k] = sb[o + k - 1]; k--; }
-    sb[o + k] = v;
+  __shared__ int stage[SO_CAP];
+  __shared__ int srange[2];
+  const long long t0 = (long long)blockIdx.x * blockDim.x, t = t0 + threadIdx.x;
+  if (threadIdx.x == 0) {
+    const long long te = t0 + blockDim.x < ncell ? t0 + blockDim.x : ncell;
+    srange[0] = soff[t0];
+    srange[1] = soff[te];
   }
+  __syncthreads();
+  const int s0 = srange[0], total = srange[1] - s0;
+  const bool staged = total >= 0 && total <= SO_CAP;
+  if (staged)
+    for (int j = threadIdx.x; j < total; j += blockDim.x) stage[j] = sb[s0 + j];
+  __syncthreads();
+  if (t < ncell) {
+    const int n = min(scnt[t], cap), o = soff[t];
+    int* a = staged ? stage + (o - s0) : sb + o;
+    for (int j = 1; j < n; j++) {
+      int v = a[j], k = j;
+      while (k > 0 && a[k - 1] > v) { a[k] = a[k - 1]; k--; }
+      a[k] = v;
+    }
+  }
+  __syncthreads();
+  if (staged)
+    for (int j = threadIdx.x; j < total; j += blockDim.x) sb[s0 + j] = stage[j];
 }
 
 // ------------------------------------------------------------ host helpers
