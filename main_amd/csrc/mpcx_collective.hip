@@ -2571,8 +2571,7 @@ __device__ __forceinline__ void bulk_body(const CoP& P, const CoD& D, int do_str
         todo &= ~grp;
       }
       if (leader) {
-        int old = atomicAdd(&D.tcount[ck], gsize);
-        if (old + gsize > CENSUS_MAX) raise_error(&D.sc->err, ERR_MEMBERS);
+        atomicAdd(&D.tcount[ck], gsize);   // no return: the collision kernel checks the totals against CENSUS_MAX
         if (ck >= P.vol && ck < 2 * P.vol) atomicMin(&D.firstB[ck - P.vol], gkey);
       }
     }
@@ -2685,6 +2684,9 @@ __device__ __forceinline__ void collide_body(const CoP& P, const CoD& D, long lo
     rec[2] = make_double2(st * sp, th);
     rec[3] = make_double2((double)n, 0.0);
   }
+  if (t < P.vol)   // the census totals of the binning kernel against the reference's MaxA/MaxB/MaxS
+    if (D.tcount[t] > CENSUS_MAX || D.tcount[P.vol + t] > CENSUS_MAX || D.tcount[2 * P.vol + t] > CENSUS_MAX)
+      raise_error(&D.sc->err, ERR_MEMBERS);
   if (t < P.rxnnumcell) {
     int nb = D.tcount[1 * P.vol + t];
     double prob = D.ptab[min(nb, CENSUS_MAX)];
@@ -3148,8 +3150,7 @@ __global__ __launch_bounds__(256) void sp_bulk(CoP P, CoD D, SpD S, int do_strea
         todo &= ~grp;
       }
       if (leader) {
-        int old = atomicAdd(&D.tcount[ck], gsize);
-        if (old + gsize > CENSUS_MAX) raise_error(&D.sc->err, ERR_MEMBERS);
+        atomicAdd(&D.tcount[ck], gsize);   // no return: the collision kernel checks the totals against CENSUS_MAX
         if (ck >= P.vol && ck < 2 * P.vol) atomicMin(&D.firstB[ck - P.vol], gkey);
       }
     }
@@ -3210,6 +3211,9 @@ __global__ __launch_bounds__(CT) void sp_collide(CoP P, CoD D, SpD S) {
     rec[2] = make_double2(st * sp, th);
     rec[3] = make_double2((double)n, 0.0);
   }
+  if (t < P.vol)   // the census totals of the binning kernel against the reference's MaxA/MaxB/MaxS
+    if (D.tcount[t] > CENSUS_MAX || D.tcount[P.vol + t] > CENSUS_MAX || D.tcount[2 * P.vol + t] > CENSUS_MAX)
+      raise_error(&D.sc->err, ERR_MEMBERS);
   if (t < P.rxnnumcell) {
     int nb = D.tcount[1 * P.vol + t];
     double prob = D.ptab[min(nb, CENSUS_MAX)];
