@@ -60,6 +60,11 @@ COLLECTIVE_CASES = {
                                                    react_box_type=2, EbN=0.1),
                                        seed=11, mt_seed=4357787, steps=60, restart_at=40, flags_mix=True,
                                        react_type=2, sort=True),
+    "co_six_dimers_restart": dict(params=dict(Lx=28.0, Ly=28.0, Lz=16.0, numdims=6, rhoa=4.0, rhob=0.5, Rc=1.5,
+                                              Rn=2.0, Dcn=4.0, MPCt=0.1, skinrad=0.3, sort_step=20, k1=0.05,
+                                              EbN=0.1),
+                                  seed=21, mt_seed=4357787, steps=50, restart_at=30, flags_mix=True, react_type=7,
+                                  sort=True),
 }
 SELKOV_CASES = {
     "sk_reactive": dict(params=dict(L=40.0, rho=2.0, Rc=2.0, Rn=4.0, Dcn=7.0, MDt=0.005, MPCt=0.1, NG=20000,

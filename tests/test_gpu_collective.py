@@ -190,3 +190,28 @@ def test_reruns_are_bit_identical():
             eng.close()
     for k in ("x", "y", "z", "vx", "vy", "vz", "flag", "xd", "vxd"):
         assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
+def test_results_do_not_depend_on_the_bulk_grid():
+    """co_bulk is a grid-stride loop whose per-cell ranks come from atomics in
+    whatever order the workgroups run; the collision kernel orders the members,
+    so every grid gives bit-identical states (40^3, 12 collisions with a re-sort)."""
+    from bench import headline_params
+    from main_amd._lib import check, lib
+    p = headline_params(40)
+    p["sort_step"] = 250     # a re-sort every 5 collisions
+    s = S.collective_initial_state(p, seed=12)
+    out = []
+    for grid in (0, 64, 1000, 20000):
+        eng = CollectiveEngine(p, rng_mode=1, seed=12)
+        try:
+            check(lib().mpcx_co_set_bulk_grid(eng.h, C.c_int32(grid)), "mpcx_co_set_bulk_grid")
+            eng.set_state(s, tour=cell_tour(40, 40, 40))
+            eng.setup(do_sort=True)
+            eng.run(12)
+            out.append(eng.get_state())
+        finally:
+            eng.close()
+    for g in out[1:]:
+        for k in ("x", "y", "z", "vx", "vy", "vz", "flag", "xd", "vxd"):
+            assert np.array_equal(np.asarray(g[k]).view(np.uint8), np.asarray(out[0][k]).view(np.uint8)), k

@@ -45,7 +45,7 @@ def _inputs(case_name, golden):
 
 
 @pytest.mark.parametrize("case,near_mode", [("co_one_dimer_restart", None), ("co_two_dimers_restart_sort", None),
-                                            ("co_two_dimers_restart_sort", 2)])
+                                            ("co_two_dimers_restart_sort", 2), ("co_six_dimers_restart", None)])
 def test_restart_through_files_equals_reference(golden, tmp_path, case, near_mode):
     from tests.golden.make_golden import run_collective
     _say(f"[restart] {case} near_mode={near_mode}")
@@ -88,7 +88,7 @@ def test_restart_through_files_equals_reference(golden, tmp_path, case, near_mod
 
 
 @pytest.mark.parametrize("case,near_mode", [("co_one_dimer_restart", None), ("co_two_dimers_restart_sort", 2),
-                                            ("co_two_dimers_restart_sort", 0)])
+                                            ("co_two_dimers_restart_sort", 0), ("co_six_dimers_restart", None)])
 def test_restart_evaluation_matches_oracle(golden, case, near_mode):
     """Forces right after the restart set-up, from a mid-run state with no
     forces: dimer forces, every pair force of every list, the potential and
