@@ -2808,8 +2808,21 @@ __global__ __launch_bounds__(256) void co_classify_all(CoP P, CoD D) {
   }
 }
 
-__global__ void reset_far(CoScalars* sc) { sc->far_maxv2 = 0ull; }
-__global__ void set_coll(CoScalars* sc, unsigned long long v) { sc->coll = v; }
+// interval start: collision counter, cleared cell counts, census and first-B keys
+__global__ __launch_bounds__(256) void interval_begin(CoP P, CoD D, unsigned long long coll) {
+  const long long mc = P.mpcnumcell, vol = P.vol;
+  const long long i0 = (long long)blockIdx.x * 256 + threadIdx.x, st = (long long)gridDim.x * 256;
+  for (long long i = i0; i < mc; i += st) D.cnt[i] = 0;
+  for (long long i = i0; i < 3 * vol; i += st) D.tcount[i] = 0;
+  for (long long i = i0; i < vol; i += st) D.firstB[i] = 0xffffffffu;
+  if (i0 == 0) D.sc->coll = coll;
+}
+// before the rotation: cleared near bitmap (nwords + 1 words) and far max v^2
+__global__ __launch_bounds__(256) void rotate_begin(CoD D, long long nwords) {
+  const long long i0 = (long long)blockIdx.x * 256 + threadIdx.x, st = (long long)gridDim.x * 256;
+  for (long long i = i0; i <= nwords; i += st) D.near_bits[i] = 0u;
+  if (i0 == 0) D.sc->far_maxv2 = 0ull;
+}
 
 // --------------------------------------------------------------- Hilbert sort
 // sort_hilbert_mod.f95:50-92: stable bin by tour position of the unshifted cell
@@ -4087,7 +4100,8 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
   for (int64_t c = 0; c < ncoll; c++) {
     const int cnext = cstep + P.MPCtstep;
     const bool sort_due = (cnext % P.sort_step) == 0;
-    hipLaunchKernelGGL(set_coll, dim3(1), dim3(1), 0, st, D.sc, (unsigned long long)(cnext / P.MPCtstep));
+    hipLaunchKernelGGL(interval_begin, dim3(grid_for(3 * vol, 256, 2048)), dim3(256), 0, st, P, D,
+                       (unsigned long long)(cnext / P.MPCtstep));
     if (P.rng_mode == 0)
       HIPCHK(hipMemcpyAsync(e->d_U, draws + c * e->ndraw, e->ndraw * 8, hipMemcpyHostToDevice, st));
     const bool timed = c < mpcx_co::TMAX;
@@ -4105,9 +4119,6 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
     HIPCHK(tmark(0, 1));
     if (e->split) sp_convert(e, false, true);
     HIPCHK(hipEventRecord(e->n1, st));
-    hipLaunchKernelGGL(fill_kernel<int>, dim3(grid_for(mc, 256)), dim3(256), 0, st, D.cnt, mc, 0);
-    hipLaunchKernelGGL(fill_kernel<int>, dim3(grid_for(3 * vol, 256)), dim3(256), 0, st, D.tcount, 3 * vol, 0);
-    hipLaunchKernelGGL(fill_kernel<unsigned>, dim3(grid_for(vol, 256)), dim3(256), 0, st, D.firstB, vol, 0xffffffffu);
     const bool sp = e->split;
     auto bulk = [&](int stream, int bin) {
       if (sp) hipLaunchKernelGGL(sp_bulk, dim3(e->g_spbulk), dim3(256), 0, st, P, D, e->S, stream, bin);
@@ -4137,9 +4148,7 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
     if (sp) hipLaunchKernelGGL(sp_collide, dim3((unsigned)((mc + CT - 1) / CT)), dim3(CT), 0, st, P, D, e->S);
     else hipLaunchKernelGGL(co_collide, dim3((unsigned)((mc + CT - 1) / CT)), dim3(CT), 0, st, P, D);
     HIPCHK(tmark(2, 1));
-    hipLaunchKernelGGL(fill_kernel<unsigned>, dim3(grid_for(e->nwords + 1, 256)), dim3(256), 0, st, D.near_bits,
-                       e->nwords + 1, 0u);
-    hipLaunchKernelGGL(reset_far, dim3(1), dim3(1), 0, st, D.sc);
+    hipLaunchKernelGGL(rotate_begin, dim3(grid_for(e->nwords + 1, 256, 2048)), dim3(256), 0, st, D, e->nwords);
     HIPCHK(tmark(3, 0));
     if (sp) hipLaunchKernelGGL(sp_rotate, dim3(gN), dim3(256), 0, st, P, D, e->S);
     else hipLaunchKernelGGL(co_rotate, dim3(gN), dim3(256), 0, st, P, D, 1);

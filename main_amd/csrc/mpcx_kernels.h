@@ -173,11 +173,46 @@ static void launch_scan(hipStream_t st, const int* cnt, long long n, int* partia
 
 
 // ordered list of the set bits of a bitmap (ascending index), its length in *out_n
+// bits_emit with the scan of the block counts folded in: block b sums the
+// counts of blocks 0..b-1 itself (cheap while the blocks are few)
+__global__ __launch_bounds__(1024) void bits_emit_scan(const unsigned* bits, long long nwords, const int* counts,
+                                                       int* list, int cap, int NT, int* out_n, int* err, int nblocks) {
+  __shared__ unsigned long long s[17];
+  unsigned long long pre = 0;
+  for (int j = threadIdx.x; j < (int)blockIdx.x; j += blockDim.x) pre += (unsigned long long)counts[j];
+  unsigned long long before;
+  block_excl_scan64(pre, s, before);
+  long long w = (long long)blockIdx.x * 1024 + threadIdx.x;
+  unsigned word = w < nwords ? bits[w] : 0u;
+  unsigned long long tot;
+  unsigned long long e = block_excl_scan64(__popc(word), s, tot);
+  int pos = (int)(before + e);
+  while (word) {
+    int b = __ffs(word) - 1;
+    word &= word - 1;
+    int idx = (int)(w * 32 + b);
+    if (idx < NT) {
+      if (pos < cap) list[pos] = idx;
+      pos++;
+    }
+  }
+  if (blockIdx.x == nblocks - 1 && threadIdx.x == 0) {
+    const long long n = (long long)before + (long long)tot;
+    *out_n = n < cap ? (int)n : cap;
+    if (n > cap) raise_error(err, ERR_NEAR_CAP);
+  }
+}
+
 static inline void launch_bits_list(hipStream_t st, const unsigned* bits, long long nwords, int* partial, int* list,
                                     int cap, int NT, int* out_n, int* err) {
   int nb = (int)((nwords + 1023) / 1024);
   if (nb < 1) nb = 1;
   hipLaunchKernelGGL(bits_count, dim3(nb), dim3(1024), 0, st, bits, nwords, partial);
+  if (nb <= 8192) {   // up to 2.7e8 particles: one launch less
+    hipLaunchKernelGGL(bits_emit_scan, dim3(nb), dim3(1024), 0, st, bits, nwords, (const int*)partial, list, cap, NT,
+                       out_n, err, nb);
+    return;
+  }
   hipLaunchKernelGGL(scan_top, dim3(1), dim3(1024), 0, st, partial, nb);
   hipLaunchKernelGGL(bits_emit, dim3(nb), dim3(1024), 0, st, bits, nwords, (const int*)partial, list, cap, NT, out_n,
                      err, nb);
