@@ -163,10 +163,41 @@ static int grid_for(long long n, int threads, int cap = 4096) {
   return (int)g;
 }
 
+// scan_final with the scan of the block sums folded in: block b adds the sums
+// of blocks 0..b-1 itself (cheap while the blocks are few)
+__global__ __launch_bounds__(1024) void scan_final_sums(const int* cnt, long long n, const int* sums, int* off) {
+  __shared__ unsigned long long s[17];
+  unsigned long long pre = 0;
+  for (int j = threadIdx.x; j < (int)blockIdx.x; j += blockDim.x) pre += (unsigned long long)sums[j];
+  unsigned long long before;
+  block_excl_scan64(pre, s, before);
+  long long base = (long long)blockIdx.x * 4096;
+  int v[4];
+  unsigned long long sum = 0;
+  for (int j = 0; j < 4; j++) {
+    long long i = base + threadIdx.x * 4 + j;
+    v[j] = i < n ? cnt[i] : 0;
+    sum += v[j];
+  }
+  unsigned long long tot;
+  unsigned long long e = block_excl_scan64(sum, s, tot);
+  long long run = (long long)before + (long long)e;
+  for (int j = 0; j < 4; j++) {
+    long long i = base + threadIdx.x * 4 + j;
+    if (i < n) off[i] = (int)run;
+    run += v[j];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) off[n] = (int)((long long)before + (long long)tot);
+}
+
 static void launch_scan(hipStream_t st, const int* cnt, long long n, int* partial, int* off) {
   int nb = (int)((n + 4095) / 4096);
   if (nb < 1) nb = 1;
   hipLaunchKernelGGL(scan_partial, dim3(nb), dim3(1024), 0, st, cnt, n, partial);
+  if (nb <= 4096) {   // up to 1.7e7 entries: one launch less
+    hipLaunchKernelGGL(scan_final_sums, dim3(nb), dim3(1024), 0, st, cnt, n, (const int*)partial, off);
+    return;
+  }
   hipLaunchKernelGGL(scan_top, dim3(1), dim3(1024), 0, st, partial, nb);
   hipLaunchKernelGGL(scan_final, dim3(nb), dim3(1024), 0, st, cnt, n, (const int*)partial, off);
 }
