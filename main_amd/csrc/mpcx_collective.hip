@@ -34,7 +34,7 @@
 
 namespace mpcx {
 
-constexpr int NE_T = 512;    // near-engine threads (one workgroup, 256 VGPRs per lane)
+constexpr int NE_T = 768;    // near-engine threads (one workgroup, 256 VGPRs per lane)
 constexpr int NE_CAP = 16384;  // inside-list capacity of the near engine
 constexpr int MAXD = 4;      // dimers one near-engine workgroup handles (a cluster)
 constexpr int MAXS = 2 * MAXD;
