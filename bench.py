@@ -42,7 +42,9 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_100cube.txt")
 #            (+ the cell records, 64 B per cell, about 6.4 B per particle)  = 59.4
 #  co_collide per particle: bucket entry (4) + vx,vy,vz gathered (24); per cell 64 B written
 #  co_scatter reads cell (4) + rank (2) + offset (4), writes bucket entry (4)     = 14
-ALGO_BYTES = {"co_bulk": 84.0, "co_rotate": 59.4, "co_collide": 34.4, "co_scatter": 14.0}
+#  co_far    the bytes of co_bulk for the far particles (all but the ~4,300 of the near
+#            set at 100^3, counted as NT), run beside the near engine               = 84
+ALGO_BYTES = {"co_bulk": 84.0, "co_far": 84.0, "co_rotate": 59.4, "co_collide": 34.4, "co_scatter": 14.0}
 
 
 def headline_params(L=100):

@@ -219,6 +219,16 @@ int mpcx_co_set_precision(mpcx_co* e, int32_t split);
  * (at most 65536), for tuning.  Results do not depend on it.  No counterpart
  * in the reference. */
 int mpcx_co_set_bulk_grid(mpcx_co* e, int32_t grid);
+/* on = 1 (the default): with the one-workgroup near engine (mode 0) and FP64
+ * storage, the far particles stream in one free flight of the whole interval
+ * and bin on a second stream while the near engine runs (co_far); the near
+ * set is binned afterwards (co_bin_near).  Far positions then agree with the
+ * reference's event-by-event lazy streaming (evolve_mod.f95:474-527) to a few
+ * ulps rather than bitwise; integer state is unaffected unless a particle
+ * lies within that rounding of a cell face.  on = 0: every rebuild event is
+ * replayed after the near engine, bit for bit.  No counterpart in the
+ * reference. */
+int mpcx_co_set_overlap(mpcx_co* e, int32_t on);
 /* Collision-cell data of the last collision: (mpcnumcell) arrays, 0-based
  * cell of each particle, and unshifted census counts (3, vol). */
 int mpcx_co_get_cells(mpcx_co* e, double* vcm_uv /* (mpcnumcell, 8) */, int32_t* cell_of,

@@ -74,11 +74,14 @@ class CollectiveEngine:
     """Device-resident collective_code system (one GPU)."""
 
     def __init__(self, p: dict, rng_mode: int = 0, seed: int = 4357787, sphere_guard: float = 0.3,
-                 near_mode: int | None = None, near_workgroups: int | None = None, split: bool = False):
+                 near_mode: int | None = None, near_workgroups: int | None = None, split: bool = False,
+                 overlap: bool = True):
         """near_mode (two or more dimers): 0 one near-engine workgroup for the
         system (up to four dimers), 1 one per cluster of dimers, 2 the near
         set over cooperative workgroups (default above two dimers); see
-        mpcx_co_set_near_mode."""
+        mpcx_co_set_near_mode.  overlap: far particles in one flight beside
+        the near engine (False: the bitwise event replay after it); see
+        mpcx_co_set_overlap."""
         require_gpu()
         self.p = p
         self.cfg = make_config(p, rng_mode, seed, sphere_guard)
@@ -90,9 +93,18 @@ class CollectiveEngine:
         if split:
             check(lib().mpcx_co_set_precision(self.h, C.c_int32(1)), "mpcx_co_set_precision")
         self.split = split
+        # co_far beside the near engine: one-workgroup mode (the default up to two dimers), FP64 storage
+        self._ov_mode = near_mode if near_mode is not None else (0 if int(p["numdims"]) <= 2 else 2)
+        self.set_overlap(overlap)
         if near_workgroups is not None:
             check(lib().mpcx_co_set_near_workgroups(self.h, C.c_int32(int(near_workgroups))),
                   "mpcx_co_set_near_workgroups")
+
+    def set_overlap(self, on: bool):
+        """Far particles in one flight beside the near engine (True) or the
+        bitwise event replay after it (False), from the next run on."""
+        check(lib().mpcx_co_set_overlap(self.h, C.c_int32(1 if on else 0)), "mpcx_co_set_overlap")
+        self.overlap = bool(on) and not self.split and self._ov_mode == 0
 
     def close(self):
         if self.h:
@@ -272,7 +284,8 @@ class CollectiveEngine:
         a = np.zeros(5)
         n = C.c_int64(0)
         check(lib().mpcx_co_kernel_ms(self.h, ptr(a), C.byref(n)), "mpcx_co_kernel_ms")
-        return dict(zip(self.KERNELS, a.tolist()), intervals=int(n.value))
+        names = [("co_far" if self.overlap and k == "co_bulk" else k) for k in self.KERNELS]
+        return dict(zip(names, a.tolist()), intervals=int(n.value))
 
     def timing(self):
         r, b, n = C.c_double(), C.c_double(), C.c_double()

@@ -63,6 +63,9 @@ struct CoScalars {
   double x0[MAXSG], y0[MAXSG], z0[MAXSG];   // spheres at the interval start
   unsigned long long prof[8];   // near engine: total, rebuild, particle phase, serial (100 MHz ticks), rebuilds
   unsigned long long coll;      // collision counter of the current interval (Philox counter)
+  double xs0[MAXSG], ys0[MAXSG], zs0[MAXSG];   // copy of x0..z0 made at the interval start (the near
+                                               // engine overwrites x0..z0 while co_far still reads them)
+  int n_wall, pad_w;            // far particles co_far left to co_bin_rest (they reach a wall)
 };
 
 // Dimers whose near regions can overlap within an interval form a cluster;
@@ -139,6 +142,8 @@ struct CoD {
   double* rec_msd;
   double* rec_fp;         // [p][4][3]
   double* rec_pot;        // [p][4]
+  int* wall_list;         // [wall_cap] far particles whose flight reaches a wall (co_far -> co_bin_rest)
+  int wall_cap, pad_w;
 };
 
 // ------------------------------------------------------------- streaming
@@ -2451,141 +2456,251 @@ __device__ __forceinline__ long long shifted_cell(const CoP& P, double x, double
   return (long long)c - 1;
 }
 
-// Far particles: replay the interval's lazy-streaming events; then bin every
-// particle into the shifted grid (count + rank, by the cell's tour position)
-// and into the census; record its distance to the nearest sphere for the
-// selection of the next interval's near set.
-__device__ __forceinline__ void bulk_body(const CoP& P, const CoD& D, int do_stream, int do_bin, long long first,
-                                          long long end, long long step) {
+// Binning of one particle per lane; whole waves call it together (`valid`
+// false for idle lanes).  The shifted cell, counted by tour position with the
+// particle's rank in it; the distance to the nearest sphere of `sph` beyond
+// its list radius (selection of the next interval's near set); the census of
+// distribute_mod.f95:95-183: outside particles keyed by index, inside ones
+// after all outside ones, inside ones within a cutoff skipped.
+__device__ __forceinline__ void bin_particle(const CoP& P, const CoD& D, int i, bool valid, bool near,
+                                             unsigned char st, double x, double y, double z, int f, double xs,
+                                             double ys, double zs, const double (*sph)[MAXSG]) {
   const int NT = P.NT;
-  const int nev = D.sc->n_ev;
-  __shared__ int ev[MAXEV];
-  __shared__ double sph[3][MAXSG];
-  for (int j = threadIdx.x; j < nev && j < MAXEV; j += blockDim.x) ev[j] = D.sc->events[j];
-  for (int j = threadIdx.x; j < 2 * P.nd; j += blockDim.x) {
-    sph[0][j] = D.sc->x0[j];
-    sph[1][j] = D.sc->y0[j];
-    sph[2][j] = D.sc->z0[j];
-  }
-  __syncthreads();
-  double xs = 0, ys = 0, zs = 0;
-  if (do_bin) collision_shift(P, D, xs, ys, zs);
   const int lane = threadIdx.x & 63;
-  for (long long base = first; base < end; base += step) {
-    const int i = (int)(base + threadIdx.x);
-    const bool valid = i < end;
-    bool near = false;
-    double x = 0, y = 0, z = 0;
-    int f = -1;
-    unsigned char st = 0;
-    if (valid) {
-      near = (D.near_bits[i >> 5] >> (i & 31)) & 1u;
-      st = D.status[i];
-      if (!near && st != 0) { D.status[i] = 0; st = 0; }   // left the near set earlier
-      x = D.x[i]; y = D.y[i]; z = D.z[i];
-      f = D.flag[i];
-      if (do_stream && !near && nev > 0) {
-        double vx = D.vx[i], vy = D.vy[i], vz = D.vz[i];
-        // step displacements v*MDt, negated with v at a bounce ((-v)*MDt is -(v*MDt) exactly)
-        double dx = vx * P.MDt, dy = vy * P.MDt, dz = vz * P.MDt;
-        const int f0 = f;
-        bool bounced = false;
-        for (int e = 0; e < nev && e < MAXEV; e++) {
-          stream_flight(x, y, z, dx, dy, dz, (double)ev[e], P);
-          if (stream_walls(x, y, z, vx, vy, vz, f, P, D.react_box, &D.sc->err)) {
-            bounced = true;
-            dx = -dx; dy = -dy; dz = -dz;
-          }
-        }
-        D.x[i] = x; D.y[i] = y; D.z[i] = z;
-        if (bounced) { D.vx[i] = vx; D.vy[i] = vy; D.vz[i] = vz; }
-        if (f != f0) D.flag[i] = f;
-      }
+  bool ok = valid;
+  int t = 0;
+  if (valid) {
+    long long c = shifted_cell(P, x, y, z, xs, ys, zs);
+    if (c < 0 || c >= P.mpcnumcell) { raise_error(&D.sc->err, ERR_CELL_RANGE); ok = false; }
+    else t = D.mpc_pos[c];
+  }
+  int r = wave_atomic_rank(D.cnt, t, ok);
+  if (valid) {
+    if (ok && r >= BMAX) raise_error(&D.sc->err, ERR_BUCKET);
+    D.cell_of[i] = t;
+    D.rank_of[i] = ok ? (unsigned short)min(r, 65535) : (unsigned short)0xffff;
+    // distance beyond the list radius; 8 or more beyond every radius gives the
+    // cap (255 at 7.97) without a sqrt
+    double dmin = 1e30;
+    bool close = false;
+    for (int s = 0; s < 2 * P.nd; s++) {
+      double a = pbc_half(x - sph[0][s], P.box[0]);
+      double b = pbc_half(y - sph[1][s], P.box[1]);
+      double c = z - sph[2][s];
+      const double d2 = a * a + b * b + c * c, far = P.full[s >= P.nd] + 8.0;
+      close |= d2 < far * far;
     }
-    if (!do_bin) continue;
-    // shifted cell, counted by tour position
-    bool ok = valid;
-    int t = 0;
-    if (valid) {
-      long long c = shifted_cell(P, x, y, z, xs, ys, zs);
-      if (c < 0 || c >= P.mpcnumcell) { raise_error(&D.sc->err, ERR_CELL_RANGE); ok = false; }
-      else t = D.mpc_pos[c];
-    }
-    int r = wave_atomic_rank(D.cnt, t, ok);
-    if (valid) {
-      if (ok && r >= BMAX) raise_error(&D.sc->err, ERR_BUCKET);
-      D.cell_of[i] = t;
-      D.rank_of[i] = ok ? (unsigned short)min(r, 65535) : (unsigned short)0xffff;
-      // distance to the nearest sphere beyond its list radius (next near set);
-      // 8 or more beyond every radius gives the cap (255 at 7.97) without a sqrt
-      double dmin = 1e30;
-      bool close = false;
+    if (__ballot(close))   // most waves lie far from every sphere: no square roots at all
       for (int s = 0; s < 2 * P.nd; s++) {
         double a = pbc_half(x - sph[0][s], P.box[0]);
         double b = pbc_half(y - sph[1][s], P.box[1]);
         double c = z - sph[2][s];
         const double d2 = a * a + b * b + c * c, far = P.full[s >= P.nd] + 8.0;
-        close |= d2 < far * far;
+        if (d2 < far * far) dmin = fmin(dmin, sqrt(d2) - P.full[s >= P.nd]);
       }
-      if (__ballot(close))   // most waves lie far from every sphere: no square roots at all
-        for (int s = 0; s < 2 * P.nd; s++) {
-          double a = pbc_half(x - sph[0][s], P.box[0]);
-          double b = pbc_half(y - sph[1][s], P.box[1]);
-          double c = z - sph[2][s];
-          const double d2 = a * a + b * b + c * c, far = P.full[s >= P.nd] + 8.0;
-          if (d2 < far * far) dmin = fmin(dmin, sqrt(d2) - P.full[s >= P.nd]);
-        }
-      double qd = floor(fmax(dmin, 0.0) * 32.0 * (1.0 - 1e-12));
-      if (st & ST_INSIDE) qd = 0.0;   // list members stay near until the next rebuild
-      D.dq[i] = (unsigned char)fmin(qd, 255.0);
+    double qd = floor(fmax(dmin, 0.0) * 32.0 * (1.0 - 1e-12));
+    if (st & ST_INSIDE) qd = 0.0;   // list members stay near until the next rebuild
+    D.dq[i] = (unsigned char)fmin(qd, 255.0);
+  }
+  unsigned key = (unsigned)i;
+  bool cen = false;
+  int ck = 0;
+  if (valid && ok) {
+    bool skip = false;
+    if (near && (st & ST_INSIDE)) { key = (unsigned)NT + (unsigned)i; skip = (st & ST_EXCL) != 0; }
+    int sp = f == 0 ? 0 : f == 1 ? 1 : f == 2 ? 2 : -1;
+    if (!skip && sp >= 0) {
+      long long u = cell_unshifted0(x, y, z, P.box[0], P.box[1]);
+      if (u < 0 || u >= P.vol) raise_error(&D.sc->err, ERR_CELL_RANGE);
+      else { cen = true; ck = (int)(sp * P.vol + u); }
     }
-    // census of distribute_mod.f95:95-183: outside particles keyed by index,
-    // inside ones after all outside ones, inside ones within a cutoff skipped
-    unsigned key = (unsigned)i;
-    bool cen = false;
-    int ck = 0;
-    if (valid && ok) {
-      bool skip = false;
-      if (near && (st & ST_INSIDE)) { key = (unsigned)NT + (unsigned)i; skip = (st & ST_EXCL) != 0; }
-      int sp = f == 0 ? 0 : f == 1 ? 1 : f == 2 ? 2 : -1;
-      if (!skip && sp >= 0) {
-        long long u = cell_unshifted0(x, y, z, P.box[0], P.box[1]);
-        if (u < 0 || u >= P.vol) raise_error(&D.sc->err, ERR_CELL_RANGE);
-        else { cen = true; ck = (int)(sp * P.vol + u); }
-      }
+  }
+  {   // census groups: one count and one minimum key per (species, cell) of the wave
+    unsigned long long todo = __ballot(cen);
+    const unsigned long long outside = __ballot(cen && key < (unsigned)NT);
+    int gsize = 0;
+    unsigned gkey = 0;
+    bool leader = false;
+    while (todo) {
+      int ld = __ffsll((unsigned long long)todo) - 1;
+      int k = __shfl(ck, ld);
+      unsigned long long grp = __ballot(cen && ck == k);
+      unsigned long long go = grp & outside;
+      int src = __ffsll((unsigned long long)(go ? go : grp)) - 1;
+      unsigned mk = (unsigned)__shfl((int)key, src);
+      if (lane == ld) { leader = true; gsize = __popcll(grp); gkey = mk; }
+      todo &= ~grp;
     }
-    {   // census groups: one count and one minimum key per (species, cell) of the wave
-      unsigned long long todo = __ballot(cen);
-      const unsigned long long outside = __ballot(cen && key < (unsigned)NT);
-      int gsize = 0;
-      unsigned gkey = 0;
-      bool leader = false;
-      while (todo) {
-        int ld = __ffsll((unsigned long long)todo) - 1;
-        int k = __shfl(ck, ld);
-        unsigned long long grp = __ballot(cen && ck == k);
-        unsigned long long go = grp & outside;
-        int src = __ffsll((unsigned long long)(go ? go : grp)) - 1;
-        unsigned mk = (unsigned)__shfl((int)key, src);
-        if (lane == ld) { leader = true; gsize = __popcll(grp); gkey = mk; }
-        todo &= ~grp;
-      }
-      if (leader) {
-        atomicAdd(&D.tcount[ck], gsize);   // no return: the collision kernel checks the totals against CENSUS_MAX
-        if (ck >= P.vol && ck < 2 * P.vol) atomicMin(&D.firstB[ck - P.vol], gkey);
-      }
+    if (leader) {
+      atomicAdd(&D.tcount[ck], gsize);   // no return: the collision kernel checks the totals against CENSUS_MAX
+      if (ck >= P.vol && ck < 2 * P.vol) atomicMin(&D.firstB[ck - P.vol], gkey);
     }
   }
 }
+
+// The lazy-streaming events of the interval (evolve_mod.f95:474-527) for one
+// far particle, with the reference's rounding: one free flight per rebuild
+// event, the walls checked after each; position, velocity (at a bounce) and
+// species (at a reactive wall) written back.
+__device__ __forceinline__ void replay_events(const CoP& P, const CoD& D, int i, double& x, double& y, double& z,
+                                              int& f, const int* ev, int nev) {
+  double vx = D.vx[i], vy = D.vy[i], vz = D.vz[i];
+  // step displacements v*MDt, negated with v at a bounce ((-v)*MDt is -(v*MDt) exactly)
+  double dx = vx * P.MDt, dy = vy * P.MDt, dz = vz * P.MDt;
+  const int f0 = f;
+  bool bounced = false;
+  for (int e = 0; e < nev; e++) {
+    stream_flight(x, y, z, dx, dy, dz, (double)ev[e], P);
+    if (stream_walls(x, y, z, vx, vy, vz, f, P, D.react_box, &D.sc->err)) {
+      bounced = true;
+      dx = -dx; dy = -dy; dz = -dz;
+    }
+  }
+  D.x[i] = x; D.y[i] = y; D.z[i] = z;
+  if (bounced) { D.vx[i] = vx; D.vy[i] = vy; D.vz[i] = vz; }
+  if (f != f0) D.flag[i] = f;
+}
+
+// A far particle whose single flight ends this close to a wall, or beyond it,
+// is left to the event replay: far beyond the flights' rounding difference
+constexpr double WALL_MARGIN = 1e-9;
+
+// appends the lanes' particles with `wall` set to the deferred list, one
+// atomic per wave (whole waves call it together)
+__device__ __forceinline__ void wall_defer(const CoD& D, bool wall, int i) {
+  const unsigned long long m = __ballot(wall);
+  if (!m) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll(m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(&D.sc->n_wall, __popcll(m));
+  base = __shfl(base, leader);
+  if (wall) {
+    const int k = base + __popcll(m & ((1ull << lane) - 1ull));
+    if (k < D.wall_cap) D.wall_list[k] = i;
+    else raise_error(&D.sc->err, ERR_WALL_CAP);
+  }
+}
+
+// Streaming of the far particles, then binning.
+// FAR = false: the interval's rebuild events replayed (replay_events), every
+// particle binned, distances to the spheres at the interval's end (x0..z0).
+// FAR = true (co_far, concurrent with the near engine): far particles only,
+// one free flight of the whole interval -- the replay's sum of flights up to
+// a few ulps, since a far particle meets no list -- and distances to the
+// spheres at the interval's start (xs0..zs0).  A flight that reaches a wall
+// is not taken: the reference's bounce time (z - Lz)/vz magnifies the ulps by
+// |v|/|vz|, so such particles go to the deferred list and replay the events
+// in co_bin_rest, which also bins the near set once the near engine is done.
+template <bool FAR>
+__device__ __forceinline__ void bulk_body(const CoP& P, const CoD& D, int do_stream, int do_bin, long long first,
+                                          long long end, long long step) {
+  const int nev = FAR ? 0 : min(D.sc->n_ev, MAXEV);
+  __shared__ int ev[MAXEV];
+  __shared__ double sph[3][MAXSG];
+  if (!FAR)
+    for (int j = threadIdx.x; j < nev; j += blockDim.x) ev[j] = D.sc->events[j];
+  const double* sx = FAR ? D.sc->xs0 : D.sc->x0;
+  const double* sy = FAR ? D.sc->ys0 : D.sc->y0;
+  const double* sz = FAR ? D.sc->zs0 : D.sc->z0;
+  for (int j = threadIdx.x; j < 2 * P.nd; j += blockDim.x) {
+    sph[0][j] = sx[j];
+    sph[1][j] = sy[j];
+    sph[2][j] = sz[j];
+  }
+  __syncthreads();
+  double xs = 0, ys = 0, zs = 0;
+  if (do_bin) collision_shift(P, D, xs, ys, zs);
+  const double flight = (double)P.MPCtstep;
+  const double ztop = P.box[2] - WALL_MARGIN;
+  for (long long base = first; base < end; base += step) {
+    const int i = (int)(base + threadIdx.x);
+    const bool valid = i < end;
+    bool near = false, wall = false;
+    double x = 0, y = 0, z = 0;
+    int f = -1;
+    unsigned char st = 0;
+    if (valid) near = (D.near_bits[i >> 5] >> (i & 31)) & 1u;
+    const bool mine = valid && !(FAR && near);
+    if (mine) {
+      st = D.status[i];
+      if (!near && st != 0) { D.status[i] = 0; st = 0; }   // left the near set earlier
+      x = D.x[i]; y = D.y[i]; z = D.z[i];
+      f = D.flag[i];
+      if (FAR) {
+        const double dx = D.vx[i] * P.MDt, dy = D.vy[i] * P.MDt, dz = D.vz[i] * P.MDt;
+        const double zt = z + dz * flight;   // the flight's z, as stream_flight computes it
+        wall = !(zt > WALL_MARGIN && zt < ztop);
+        if (!wall) {
+          stream_flight(x, y, z, dx, dy, dz, flight, P);
+          D.x[i] = x; D.y[i] = y; D.z[i] = z;
+        }
+      } else if (do_stream && !near && nev > 0) {
+        replay_events(P, D, i, x, y, z, f, ev, nev);
+      }
+    }
+    if (FAR) wall_defer(D, wall, i);
+    if (!do_bin) continue;
+    bin_particle(P, D, i, mine && !wall, near, st, x, y, z, f, xs, ys, zs, sph);
+  }
+}
 __global__ __launch_bounds__(256) void co_bulk(CoP P, CoD D, int do_stream, int do_bin) {
-  bulk_body(P, D, do_stream, do_bin, (long long)blockIdx.x * blockDim.x, P.NT, (long long)gridDim.x * blockDim.x);
+  bulk_body<false>(P, D, do_stream, do_bin, (long long)blockIdx.x * blockDim.x, P.NT,
+                   (long long)gridDim.x * blockDim.x);
+}
+__global__ __launch_bounds__(256) void co_far(CoP P, CoD D, int do_bin) {
+  bulk_body<true>(P, D, 1, do_bin, (long long)blockIdx.x * blockDim.x, P.NT, (long long)gridDim.x * blockDim.x);
+}
+// co_far's single flight covers the whole interval: the rebuild events
+// recorded by the near engine must add up to it
+__device__ __forceinline__ void check_flight(const CoP& P, const CoD& D) {
+  int sum = 0;
+  const int nev = D.sc->n_ev;
+  for (int e = 0; e < nev && e < MAXEV; e++) sum += D.sc->events[e];
+  if (nev > MAXEV || sum != P.MPCtstep) raise_error(&D.sc->err, ERR_FLIGHT);
+}
+// After the near engine and co_far: the deferred far particles replay the
+// interval's events; with do_bin they and the near set are binned (distances
+// to the spheres at the interval's start, as co_far).  Then the flight check.
+__global__ __launch_bounds__(256) void co_bin_rest(CoP P, CoD D, int do_bin) {
+  __shared__ int ev[MAXEV];
+  __shared__ double sph[3][MAXSG];
+  const int nev = min(D.sc->n_ev, MAXEV);
+  for (int j = threadIdx.x; j < nev; j += blockDim.x) ev[j] = D.sc->events[j];
+  for (int j = threadIdx.x; j < 2 * P.nd; j += blockDim.x) {
+    sph[0][j] = D.sc->xs0[j];
+    sph[1][j] = D.sc->ys0[j];
+    sph[2][j] = D.sc->zs0[j];
+  }
+  __syncthreads();
+  double xs = 0, ys = 0, zs = 0;
+  if (do_bin) collision_shift(P, D, xs, ys, zs);
+  // an overflowing near list or deferred list was reported when it was built
+  const long long nn = do_bin ? min(D.sc->near_num, D.near_cap) : 0;
+  const long long total = nn + min(D.sc->n_wall, D.wall_cap);
+  for (long long base = (long long)blockIdx.x * blockDim.x; base < total; base += (long long)gridDim.x * blockDim.x) {
+    const long long q = base + threadIdx.x;
+    const bool valid = q < total, near = q < nn;
+    int i = 0, f = -1;
+    double x = 0, y = 0, z = 0;
+    unsigned char st = 0;
+    if (valid) {
+      i = near ? D.near_list[q] : D.wall_list[q - nn];
+      x = D.x[i]; y = D.y[i]; z = D.z[i];
+      f = D.flag[i];
+      st = D.status[i];
+      if (!near) replay_events(P, D, i, x, y, z, f, ev, nev);
+    }
+    if (do_bin) bin_particle(P, D, i, valid, near, st, x, y, z, f, xs, ys, zs, sph);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) check_flight(P, D);
 }
 __global__ __launch_bounds__(256) void co_bulk_b(const CoP* Ps, const CoD* Ds, const Tile* tiles, int do_stream,
                                                  int do_bin) {
   const Tile t = tiles[blockIdx.x];
   const CoP P = Ps[t.r];
   const CoD D = Ds[t.r];
-  bulk_body(P, D, do_stream, do_bin, t.begin, t.end, 256);
+  bulk_body<false>(P, D, do_stream, do_bin, t.begin, t.end, 256);
 }
 
 __device__ __forceinline__ void scatter_body(const CoD& D, long long first, long long end, long long step) {
@@ -2732,10 +2847,21 @@ __global__ __launch_bounds__(CT) void co_collide_b(const CoP* Ps, const CoD* Ds,
 // Rotation of mpc_rotation_mod.f95:205-230 and selection of the next
 // interval's near set: a particle is near when its distance beyond a list
 // radius (recorded by co_bulk, rounded down) is within |v| tau + guard.
+// classify 0: rotation only; 1: distances measured to the spheres at the
+// interval's end; 2: measured to the spheres at its start (co_far,
+// co_bin_rest), so the reach widens by the largest sphere displacement since.
 __device__ __forceinline__ void rotate_body(const CoP& P, const CoD& D, int classify, long long first, long long end,
                                             long long step) {
   double fmax2 = 0.0;
   const double tau = P.tau * (1.0 + 1e-9);
+  double shift = 0.0;
+  if (classify == 2)
+    for (int s = 0; s < 2 * P.nd; s++) {
+      const double a = pbc_half(D.sc->x0[s] - D.sc->xs0[s], P.box[0]);
+      const double b = pbc_half(D.sc->y0[s] - D.sc->ys0[s], P.box[1]);
+      const double c = D.sc->z0[s] - D.sc->zs0[s];
+      shift = fmax(shift, sqrt(a * a + b * b + c * c));
+    }
   for (long long i = first + threadIdx.x; i < end; i += step) {
     int t = D.cell_of[i];
     const double2* rec = (const double2*)(D.cellrec + (long long)t * 8);
@@ -2751,7 +2877,7 @@ __device__ __forceinline__ void rotate_body(const CoP& P, const CoD& D, int clas
     D.vx[i] = vx; D.vy[i] = vy; D.vz[i] = vz;
     if (!classify) continue;
     double v2 = (vx * vx + vy * vy) + vz * vz;
-    double reach = sqrt(v2) * tau + P.guard + 1e-6;
+    double reach = sqrt(v2) * tau + P.guard + 1e-6 + shift;
     if ((double)D.dq[i] * (1.0 / 32.0) <= reach) atomicOr(&D.near_bits[i >> 5], 1u << (i & 31));
     else fmax2 = fmax(fmax2, v2);
   }
@@ -2808,14 +2934,16 @@ __global__ __launch_bounds__(256) void co_classify_all(CoP P, CoD D) {
   }
 }
 
-// interval start: collision counter, cleared cell counts, census and first-B keys
+// interval start: collision counter, cleared cell counts, census and first-B
+// keys, the spheres' start positions kept for co_far
 __global__ __launch_bounds__(256) void interval_begin(CoP P, CoD D, unsigned long long coll) {
   const long long mc = P.mpcnumcell, vol = P.vol;
   const long long i0 = (long long)blockIdx.x * 256 + threadIdx.x, st = (long long)gridDim.x * 256;
   for (long long i = i0; i < mc; i += st) D.cnt[i] = 0;
   for (long long i = i0; i < 3 * vol; i += st) D.tcount[i] = 0;
   for (long long i = i0; i < vol; i += st) D.firstB[i] = 0xffffffffu;
-  if (i0 == 0) D.sc->coll = coll;
+  if (i0 == 0) { D.sc->coll = coll; D.sc->n_wall = 0; }
+  if (i0 < 2 * P.nd) { D.sc->xs0[i0] = D.sc->x0[i0]; D.sc->ys0[i0] = D.sc->y0[i0]; D.sc->zs0[i0] = D.sc->z0[i0]; }
 }
 // before the rotation: cleared near bitmap (nwords + 1 words) and far max v^2
 __global__ __launch_bounds__(256) void rotate_begin(CoD D, long long nwords) {
@@ -3322,6 +3450,11 @@ struct mpcx_co {
   CoD D;
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr, b0 = nullptr, b1 = nullptr, n0 = nullptr, n1 = nullptr;
+  // co_far runs on st2 beside the near engine: st2 waits for fork (cleared
+  // counters, draws), st waits for join before binning the near set
+  hipStream_t st2 = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  bool overlap = true;
   double run_ms = 0, bulk_ms = 0, near_ms = 0;
   static constexpr int NKT = 5, TMAX = 256;   // timed kernels, timed intervals
   std::vector<hipEvent_t> tev;                // [TMAX][NKT][2]
@@ -3389,8 +3522,10 @@ static int check_device_error(mpcx_co* e) {
                                 "more than four dimers lie within reach of each other (one cluster)",
                                 "the clusters of the system fell out of step",
                                 "a neighbour-list member was carried to a cluster without its dimer",
-                                "no place outside the spheres found for a solvent particle"};
-    std::string m = std::string("device check failed: ") + (err > 0 && err < 17 ? msg[err] : "unknown");
+                                "no place outside the spheres found for a solvent particle",
+                                "the rebuild events of an interval do not add up to the interval",
+                                "more far particles reach a wall in one interval than the deferred list holds"};
+    std::string m = std::string("device check failed: ") + (err > 0 && err < 19 ? msg[err] : "unknown");
     int zero = 0;
     (void)hipMemcpy(&e->D.sc->err, &zero, sizeof(int), hipMemcpyHostToDevice);
     return set_error(MPCX_E_DEVICE, m.c_str());
@@ -3523,6 +3658,10 @@ extern "C" int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg) {
   const long long nsq = std::min<long long>(NT, NE_NEAR);
   D.near_cap = (int)std::min<long long>(NT, (long long)NE_NEAR * (e->can_cluster ? e->nd : 1));
   D.near_list = dalloc<int>(e, D.near_cap);
+  // far particles reaching a wall within one interval: about <|vz|> tau / Lz of
+  // them (0.2 % at 100^3); the list holds 1/16 of the particles, at least 65536
+  D.wall_cap = (int)std::min<long long>(NT, std::max<long long>(65536, NT / 16));
+  D.wall_list = dalloc<int>(e, D.wall_cap);
   D.ns_x = dalloc<double>(e, nsq); D.ns_y = dalloc<double>(e, nsq); D.ns_z = dalloc<double>(e, nsq);
   D.ns_vx = dalloc<double>(e, nsq); D.ns_vy = dalloc<double>(e, nsq);
   D.ns_vz = dalloc<double>(e, nsq);
@@ -3602,6 +3741,9 @@ extern "C" int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg) {
     return set_error(MPCX_E_HIP, "device allocation failed");
   }
   HIPCHK(hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&e->st2, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&e->fork, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&e->join, hipEventDisableTiming));
   HIPCHK(hipEventCreate(&e->e0)); HIPCHK(hipEventCreate(&e->e1));
   HIPCHK(hipEventCreate(&e->b0)); HIPCHK(hipEventCreate(&e->b1));
   HIPCHK(hipEventCreate(&e->n0)); HIPCHK(hipEventCreate(&e->n1));
@@ -3669,10 +3811,12 @@ extern "C" int mpcx_co_set_near_mode(mpcx_co* e, int32_t mode) {
 extern "C" int mpcx_co_destroy(mpcx_co* e) {
   if (!e) return 0;
   if (e->st) (void)hipStreamSynchronize(e->st);
+  if (e->st2) (void)hipStreamSynchronize(e->st2);
   for (void* p : e->allocs)
     if (p) (void)hipFree(p);
   if (e->st) (void)hipStreamDestroy(e->st);
-  for (hipEvent_t ev : {e->e0, e->e1, e->b0, e->b1, e->n0, e->n1})
+  if (e->st2) (void)hipStreamDestroy(e->st2);
+  for (hipEvent_t ev : {e->e0, e->e1, e->b0, e->b1, e->n0, e->n1, e->fork, e->join})
     if (ev) (void)hipEventDestroy(ev);
   for (hipEvent_t ev : e->tev)
     if (ev) (void)hipEventDestroy(ev);
@@ -4075,6 +4219,12 @@ extern "C" int mpcx_co_set_bulk_grid(mpcx_co* e, int32_t grid) {
   return 0;
 }
 
+extern "C" int mpcx_co_set_overlap(mpcx_co* e, int32_t on) {
+  if (!e) return set_error(MPCX_E_ARG, "null engine");
+  e->overlap = on != 0;
+  return 0;
+}
+
 extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
   if (!e || ncoll < 0) return set_error(MPCX_E_ARG, "bad arguments");
   {
@@ -4105,27 +4255,39 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
     if (P.rng_mode == 0)
       HIPCHK(hipMemcpyAsync(e->d_U, draws + c * e->ndraw, e->ndraw * 8, hipMemcpyHostToDevice, st));
     const bool timed = c < mpcx_co::TMAX;
-    auto tmark = [&](int k, int edge) -> hipError_t {
+    auto tmark = [&](int k, int edge, hipStream_t s) -> hipError_t {
       if (!timed) return hipSuccess;
       e->tev_on[(size_t)c * mpcx_co::NKT + k] = 1;
-      return hipEventRecord(e->tev[((size_t)c * mpcx_co::NKT + k) * 2 + edge], st);
+      return hipEventRecord(e->tev[((size_t)c * mpcx_co::NKT + k) * 2 + edge], s);
     };
     if (timed)
       for (int k = 0; k < mpcx_co::NKT; k++) e->tev_on[(size_t)c * mpcx_co::NKT + k] = 0;
+    // the far particles' flight and binning beside the near engine
+    const bool ov = e->overlap && !e->split && e->near_mode == 0;
+    if (ov) {
+      HIPCHK(hipEventRecord(e->fork, st));
+      HIPCHK(hipStreamWaitEvent(e->st2, e->fork, 0));
+      if (!sort_due) HIPCHK(tmark(1, 0, e->st2));
+      hipLaunchKernelGGL(co_far, dim3(e->g_bulk), dim3(256), 0, e->st2, P, D, sort_due ? 0 : 1);
+      if (!sort_due) HIPCHK(tmark(1, 1, e->st2));
+      HIPCHK(hipEventRecord(e->join, e->st2));
+    }
     HIPCHK(hipEventRecord(e->n0, st));
     if (e->split) sp_convert(e, true, true);
-    HIPCHK(tmark(0, 0));
+    HIPCHK(tmark(0, 0, st));
     HIPCHK(launch_near(e, P.MPCtstep, sort_due ? M_STOP : 0));
-    HIPCHK(tmark(0, 1));
+    HIPCHK(tmark(0, 1, st));
     if (e->split) sp_convert(e, false, true);
     HIPCHK(hipEventRecord(e->n1, st));
+    if (ov) HIPCHK(hipStreamWaitEvent(st, e->join, 0));
     const bool sp = e->split;
     auto bulk = [&](int stream, int bin) {
       if (sp) hipLaunchKernelGGL(sp_bulk, dim3(e->g_spbulk), dim3(256), 0, st, P, D, e->S, stream, bin);
       else hipLaunchKernelGGL(co_bulk, dim3(e->g_bulk), dim3(256), 0, st, P, D, stream, bin);
     };
     if (sort_due) {
-      bulk(1, 0);
+      if (ov) hipLaunchKernelGGL(co_bin_rest, dim3(128), dim3(256), 0, st, P, D, 0);
+      else bulk(1, 0);
       int r = sp ? sp_sort(e, st) : co_sort(e, st);
       if (r) return r;
       co_near_list(e);
@@ -4134,25 +4296,28 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
       if (sp) sp_convert(e, false, true);
       HIPCHK(hipEventRecord(e->b0, st));
       bulk(0, 1);
+    } else if (ov) {
+      HIPCHK(hipEventRecord(e->b0, st));
+      hipLaunchKernelGGL(co_bin_rest, dim3(128), dim3(256), 0, st, P, D, 1);
     } else {
       HIPCHK(hipEventRecord(e->b0, st));
-      HIPCHK(tmark(1, 0));
+      HIPCHK(tmark(1, 0, st));
       bulk(1, 1);
-      HIPCHK(tmark(1, 1));
+      HIPCHK(tmark(1, 1, st));
     }
     launch_scan(st, D.cnt, mc, D.partial, D.off);
-    HIPCHK(tmark(4, 0));
+    HIPCHK(tmark(4, 0, st));
     hipLaunchKernelGGL(co_scatter, dim3(gN), dim3(256), 0, st, P, D);
-    HIPCHK(tmark(4, 1));
-    HIPCHK(tmark(2, 0));
+    HIPCHK(tmark(4, 1, st));
+    HIPCHK(tmark(2, 0, st));
     if (sp) hipLaunchKernelGGL(sp_collide, dim3((unsigned)((mc + CT - 1) / CT)), dim3(CT), 0, st, P, D, e->S);
     else hipLaunchKernelGGL(co_collide, dim3((unsigned)((mc + CT - 1) / CT)), dim3(CT), 0, st, P, D);
-    HIPCHK(tmark(2, 1));
+    HIPCHK(tmark(2, 1, st));
     hipLaunchKernelGGL(rotate_begin, dim3(grid_for(e->nwords + 1, 256, 2048)), dim3(256), 0, st, D, e->nwords);
-    HIPCHK(tmark(3, 0));
+    HIPCHK(tmark(3, 0, st));
     if (sp) hipLaunchKernelGGL(sp_rotate, dim3(gN), dim3(256), 0, st, P, D, e->S);
-    else hipLaunchKernelGGL(co_rotate, dim3(gN), dim3(256), 0, st, P, D, 1);
-    HIPCHK(tmark(3, 1));
+    else hipLaunchKernelGGL(co_rotate, dim3(gN), dim3(256), 0, st, P, D, ov && !sort_due ? 2 : 1);
+    HIPCHK(tmark(3, 1, st));
     HIPCHK(hipEventRecord(e->b1, st));
     co_near_list(e);
     cstep = cnext;

@@ -30,6 +30,8 @@ enum DevError : int {
   ERR_BARRIER = 14,        // the clusters of one system fell out of step
   ERR_CLUSTER_MEMB = 15,   // a list member carried over to a cluster without its dimer
   ERR_INIT = 16,           // no place outside the spheres found for a solvent particle
+  ERR_FLIGHT = 17,         // the rebuild events of an interval do not add up to the interval
+  ERR_WALL_CAP = 18,       // more far particles reach a wall in one interval than the deferred list holds
 };
 
 __device__ __forceinline__ void raise_error(int* err, int code) { atomicCAS(err, 0, code); }

@@ -53,7 +53,7 @@ def close(a, b, scale):
 
 
 def collective_parity_case(ncoll=1, seed=3, p=None, flags_mix=False, sort_at_setup=True, tour=None,
-                           react_box=None, near_mode=None):
+                           react_box=None, near_mode=None, overlap=True):
     """Run ncoll collision intervals on the oracle and on the GPU engine."""
     p = p or small_collective_params()
     s = build_collective(p, seed=seed, flags_mix=flags_mix)
@@ -66,7 +66,7 @@ def collective_parity_case(ncoll=1, seed=3, p=None, flags_mix=False, sort_at_set
 
     stream = MtmodStream(mt=s["mt"], mti=s["mti"])
     draws = host_draws(stream, p, ncoll)
-    eng = CollectiveEngine(p, rng_mode=0, near_mode=near_mode)
+    eng = CollectiveEngine(p, rng_mode=0, near_mode=near_mode, overlap=overlap)
     try:
         eng.set_state(s, tour=tour)
         eng.setup(do_sort=sort_at_setup)

@@ -23,7 +23,7 @@ REPLICAS = [dict(Lx=16.0, Ly=16.0, Lz=20.0, EbN=0.1, RPab=1.0),
 def _engine(i, kw):
     p = small_collective_params(sort_step=20, k1=0.05, **kw)
     s = build_collective(p, seed=40 + i, flags_mix=(i % 2 == 1))
-    e = CollectiveEngine(p, rng_mode=1, seed=1000 + i)
+    e = CollectiveEngine(p, rng_mode=1, seed=1000 + i, overlap=False)   # batches replay the events bitwise
     L = [int(v) for v in p["box"]]
     e.set_state(s, tour=cell_tour(*L))
     e.setup(do_sort=True)

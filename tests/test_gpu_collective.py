@@ -44,7 +44,10 @@ def _reactive_grid(p, kind):
 
 CASES = {
     "basic_3coll": dict(ncoll=3, seed=3),
+    "basic_3coll_event_replay": dict(ncoll=3, seed=3, overlap=False),
     "sort_and_species": dict(ncoll=4, seed=4, flags_mix=True, p=("small", dict(sort_step=20)), tour="hilbert"),
+    "sort_and_species_event_replay": dict(ncoll=4, seed=4, flags_mix=True, p=("small", dict(sort_step=20)),
+                                          tour="hilbert", overlap=False),
     "two_dimers_sort": dict(ncoll=3, seed=7, flags_mix=True,
                             p=("small", dict(numdims=2, Lx=20.0, Ly=20.0, Lz=24.0, sort_step=20, EbN=0.1)),
                             tour="hilbert"),
@@ -104,13 +107,16 @@ def _h(a):
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
 
-@pytest.mark.parametrize("case,near_mode", [("co_one_dimer", None), ("co_two_dimers_sort_wall", None),
-                                            ("co_six_dimers_sort", 1),
-                                           ("co_two_dimers_sort_wall", 1), ("co_two_dimers_sort_wall", 2), ("co_six_dimers_sort", None)])
-def test_engine_integer_state_equals_reference_fixture(golden, case, near_mode):
+@pytest.mark.parametrize("case,near_mode,overlap", [
+    ("co_one_dimer", None, True), ("co_one_dimer", None, False), ("co_two_dimers_sort_wall", None, True),
+    ("co_two_dimers_sort_wall", None, False), ("co_six_dimers_sort", 1, True), ("co_two_dimers_sort_wall", 1, True),
+    ("co_two_dimers_sort_wall", 2, True), ("co_six_dimers_sort", None, True)])
+def test_engine_integer_state_equals_reference_fixture(golden, case, near_mode, overlap):
     """Flags and cell arrays hashed from the reference's Fortran run, in
     every near-engine mode (0 one workgroup, 1 clusters, 2 cooperative
-    workgroups over the near set; None the default for the dimer count)."""
+    workgroups over the near set; None the default for the dimer count), with
+    the far particles' single flight beside the near engine (overlap, used by
+    the one-workgroup mode) and with the event replay after it."""
     from tests.golden.make_golden import collective_inputs
     rec = golden["cases"][case]
     cs = rec["case"]
@@ -120,7 +126,7 @@ def test_engine_integer_state_equals_reference_fixture(golden, case, near_mode):
     ncoll = cs["steps"] // p["MPCtstep"]
     assert ncoll * p["MPCtstep"] == cs["steps"]
     draws = host_draws(MtmodStream(mt=s["mt"], mti=s["mti"]), p, ncoll)
-    eng = CollectiveEngine(p, rng_mode=0, near_mode=near_mode)
+    eng = CollectiveEngine(p, rng_mode=0, near_mode=near_mode, overlap=overlap)
     try:
         eng.set_state(s, tour=tour)
         eng.setup(do_sort=cs["sort"])
@@ -190,6 +196,70 @@ def test_reruns_are_bit_identical():
             eng.close()
     for k in ("x", "y", "z", "vx", "vy", "vz", "flag", "xd", "vxd"):
         assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
+def test_overlap_matches_the_event_replay():
+    """The far particles' single flight beside the near engine against the
+    bitwise replay of the rebuild events after it, 40^3 headline physics,
+    Philox draws, re-sorts every 2 collisions.  Per interval, from one
+    bit-identical state (k replayed intervals, then one of each kind):
+    integer state identical, positions and velocities within 1e-12 -- the
+    flights differ by rounding only.  Over 4 intervals in a row the integer
+    state stays identical; the floating-point state is not compared there,
+    since the dimer's LJ dynamics amplify last-bit differences (one particle
+    reached 1.2e-12 after 4 collisions)."""
+    from bench import headline_params
+    from tests.parity_cases import close
+    p = headline_params(40)
+    p["sort_step"] = 100
+    s = S.collective_initial_state(p, seed=31)
+    tour = cell_tour(40, 40, 40)
+    vscale = np.sqrt(p["T"] / p["Ma"])
+
+    def engine(overlap):
+        e = CollectiveEngine(p, rng_mode=1, seed=31, overlap=overlap)
+        e.set_state(s, tour=tour)
+        e.setup(do_sort=True)
+        return e
+
+    def same_integers(g, c, r, rc):
+        assert np.array_equal(g["flag"], r["flag"])
+        for k in ("ncn", "tcs", "tA", "tB", "tS"):
+            assert np.array_equal(c[k], rc[k]), k
+        for k in ("cstep", "time_count", "time_penetrate"):
+            assert g[k] == r[k], k
+
+    differs = 0
+    ref = engine(False)
+    try:
+        for k in range(4):   # the (k+1)-th interval; the 2nd and 4th re-sort
+            ref.run(1)
+            r, rc = ref.get_state(), ref.get_cells()
+            a = engine(False)
+            try:
+                if k:
+                    a.run(k)
+                a.set_overlap(True)
+                a.run(1)
+                g, c = a.get_state(), a.get_cells()
+            finally:
+                a.close()
+            _say(f"[overlap] interval {k + 1}")
+            same_integers(g, c, r, rc)
+            for key, sc in (("x", 1.0), ("y", 1.0), ("z", 1.0), ("vx", vscale), ("vy", vscale), ("vz", vscale),
+                            ("xd", 1.0), ("yd", 1.0), ("zd", 1.0), ("vxd", 1e-3), ("vyd", 1e-3), ("vzd", 1e-3)):
+                n, err = close(g[key], r[key], sc)
+                assert n == 0, f"interval {k + 1}, {key}: {n} values beyond 1e-12 (worst {err:.2e})"
+            differs += int(not np.array_equal(g["x"], r["x"]))
+        a = engine(True)
+        try:
+            a.run(4)
+            same_integers(a.get_state(), a.get_cells(), r, rc)
+        finally:
+            a.close()
+    finally:
+        ref.close()
+    assert differs > 0   # the single flight did run (its last bits differ from the replay's)
 
 
 def test_results_do_not_depend_on_the_bulk_grid():
