@@ -2625,17 +2625,28 @@ __device__ __forceinline__ void bulk_body(const CoP& P, const CoD& D, int do_str
     if (mine) {
       st = D.status[i];
       if (!near && st != 0) { D.status[i] = 0; st = 0; }   // left the near set earlier
-      x = D.x[i]; y = D.y[i]; z = D.z[i];
-      f = D.flag[i];
-      if (FAR) {
-        const double dx = D.vx[i] * P.MDt, dy = D.vy[i] * P.MDt, dz = D.vz[i] * P.MDt;
+      if (FAR) {   // streamed past the caches: the near engine's slots stay in L2 and the
+                   // binning's outputs in the infinity cache for co_scatter (1.48 -> 1.45 ms per interval)
+        x = __builtin_nontemporal_load(&D.x[i]);
+        y = __builtin_nontemporal_load(&D.y[i]);
+        z = __builtin_nontemporal_load(&D.z[i]);
+        f = D.flag[i];
+        const double dx = __builtin_nontemporal_load(&D.vx[i]) * P.MDt;
+        const double dy = __builtin_nontemporal_load(&D.vy[i]) * P.MDt;
+        const double dz = __builtin_nontemporal_load(&D.vz[i]) * P.MDt;
         const double zt = z + dz * flight;   // the flight's z, as stream_flight computes it
         wall = !(zt > WALL_MARGIN && zt < ztop);
         if (!wall) {
           stream_flight(x, y, z, dx, dy, dz, flight, P);
-          D.x[i] = x; D.y[i] = y; D.z[i] = z;
+          __builtin_nontemporal_store(x, &D.x[i]);
+          __builtin_nontemporal_store(y, &D.y[i]);
+          __builtin_nontemporal_store(z, &D.z[i]);
         }
-      } else if (do_stream && !near && nev > 0) {
+      } else {
+        x = D.x[i]; y = D.y[i]; z = D.z[i];
+        f = D.flag[i];
+      }
+      if (!FAR && do_stream && !near && nev > 0) {
         replay_events(P, D, i, x, y, z, f, ev, nev);
       }
     }
