@@ -242,9 +242,10 @@ int mpcx_co_timing(mpcx_co* e, double* run_ms, double* bulk_ms, double* near_ms)
  * inside-list size; [7] launches. */
 int mpcx_co_near_profile(mpcx_co* e, uint64_t* prof8);
 /* Mean duration (ms) of each kernel of the collision interval over the last
- * mpcx_co_run, from HIP events on the engine's stream (at most 256 intervals
- * are timed): [0] near engine, [1] co_bulk (streaming replay + binning; at a
- * re-sort these run as two launches around the sort, which are not timed),
+ * mpcx_co_run, from HIP events on the stream each runs on; every 8th interval
+ * is timed (at most 256 of them), since each event costs the stream a few
+ * microseconds: [0] near engine, [1] co_far (or co_bulk without overlap;
+ * at a re-sort these run as launches around the sort, which are not timed),
  * [2] co_collide, [3] co_rotate, [4] co_scatter; count = timed intervals. */
 int mpcx_co_kernel_ms(mpcx_co* e, double* ms5, int64_t* count);
 
@@ -315,8 +316,9 @@ int mpcx_sk_get_state(mpcx_sk* e, double* x, double* y, double* z, double* vx, d
  * collision cell of each particle (-1: inactive); census counts (3, vol) F, G,
  * I; unaffected mask (vol). */
 int mpcx_sk_get_cells(mpcx_sk* e, double* rec8, int32_t* cell_of, int32_t* census, uint8_t* unaffected);
-/* Mean kernel durations (ms) over the last run: near engine, sk_bulk,
- * sk_collide, sk_rotate; and the run's wall time. */
+/* Mean kernel durations (ms) over the last run (every 8th interval timed, at
+ * most 256 of them): near engine, sk_bulk, sk_collide, sk_rotate; and the
+ * run's wall time. */
 int mpcx_sk_kernel_ms(mpcx_sk* e, double* ms4, double* run_ms);
 
 /* ------------------------------------------------------------------------

@@ -3467,7 +3467,9 @@ struct mpcx_co {
   hipEvent_t fork = nullptr, join = nullptr;
   bool overlap = true;
   double run_ms = 0, bulk_ms = 0, near_ms = 0;
-  static constexpr int NKT = 5, TMAX = 256;   // timed kernels, timed intervals
+  // timed kernels, timed intervals, and the sampling: every TSTRIDE-th interval
+  // carries the kernel-timing events (each event costs the stream a few us)
+  static constexpr int NKT = 5, TMAX = 256, TSTRIDE = 8;
   std::vector<hipEvent_t> tev;                // [TMAX][NKT][2]
   std::vector<char> tev_on;                   // [TMAX][NKT]: pair recorded
   double kt_ms[NKT] = {0};
@@ -4265,14 +4267,16 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
                        (unsigned long long)(cnext / P.MPCtstep));
     if (P.rng_mode == 0)
       HIPCHK(hipMemcpyAsync(e->d_U, draws + c * e->ndraw, e->ndraw * 8, hipMemcpyHostToDevice, st));
-    const bool timed = c < mpcx_co::TMAX;
+    const long long slot = c / mpcx_co::TSTRIDE;
+    const bool timed = c % mpcx_co::TSTRIDE == 0 && slot < mpcx_co::TMAX;
+    const bool last = c == ncoll - 1;   // the group events (mpcx_co_timing) of the last interval
     auto tmark = [&](int k, int edge, hipStream_t s) -> hipError_t {
       if (!timed) return hipSuccess;
-      e->tev_on[(size_t)c * mpcx_co::NKT + k] = 1;
-      return hipEventRecord(e->tev[((size_t)c * mpcx_co::NKT + k) * 2 + edge], s);
+      e->tev_on[(size_t)slot * mpcx_co::NKT + k] = 1;
+      return hipEventRecord(e->tev[((size_t)slot * mpcx_co::NKT + k) * 2 + edge], s);
     };
     if (timed)
-      for (int k = 0; k < mpcx_co::NKT; k++) e->tev_on[(size_t)c * mpcx_co::NKT + k] = 0;
+      for (int k = 0; k < mpcx_co::NKT; k++) e->tev_on[(size_t)slot * mpcx_co::NKT + k] = 0;
     // the far particles' flight and binning beside the near engine
     const bool ov = e->overlap && !e->split && e->near_mode == 0;
     if (ov) {
@@ -4283,13 +4287,13 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
       if (!sort_due) HIPCHK(tmark(1, 1, e->st2));
       HIPCHK(hipEventRecord(e->join, e->st2));
     }
-    HIPCHK(hipEventRecord(e->n0, st));
+    if (last) HIPCHK(hipEventRecord(e->n0, st));
     if (e->split) sp_convert(e, true, true);
     HIPCHK(tmark(0, 0, st));
     HIPCHK(launch_near(e, P.MPCtstep, sort_due ? M_STOP : 0));
     HIPCHK(tmark(0, 1, st));
     if (e->split) sp_convert(e, false, true);
-    HIPCHK(hipEventRecord(e->n1, st));
+    if (last) HIPCHK(hipEventRecord(e->n1, st));
     if (ov) HIPCHK(hipStreamWaitEvent(st, e->join, 0));
     const bool sp = e->split;
     auto bulk = [&](int stream, int bin) {
@@ -4305,13 +4309,13 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
       if (sp) sp_convert(e, true, true);
       HIPCHK(launch_near(e, 1, M_RESUME));
       if (sp) sp_convert(e, false, true);
-      HIPCHK(hipEventRecord(e->b0, st));
+      if (last) HIPCHK(hipEventRecord(e->b0, st));
       bulk(0, 1);
     } else if (ov) {
-      HIPCHK(hipEventRecord(e->b0, st));
+      if (last) HIPCHK(hipEventRecord(e->b0, st));
       hipLaunchKernelGGL(co_bin_rest, dim3(128), dim3(256), 0, st, P, D, 1);
     } else {
-      HIPCHK(hipEventRecord(e->b0, st));
+      if (last) HIPCHK(hipEventRecord(e->b0, st));
       HIPCHK(tmark(1, 0, st));
       bulk(1, 1);
       HIPCHK(tmark(1, 1, st));
@@ -4329,11 +4333,11 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
     if (sp) hipLaunchKernelGGL(sp_rotate, dim3(gN), dim3(256), 0, st, P, D, e->S);
     else hipLaunchKernelGGL(co_rotate, dim3(gN), dim3(256), 0, st, P, D, ov && !sort_due ? 2 : 1);
     HIPCHK(tmark(3, 1, st));
-    HIPCHK(hipEventRecord(e->b1, st));
+    if (last) HIPCHK(hipEventRecord(e->b1, st));
     co_near_list(e);
     cstep = cnext;
   }
-  {   // kernel-group times of the last interval
+  if (ncoll > 0) {   // kernel-group times of the last interval
     HIPCHK(hipEventSynchronize(e->b1));
     float nm = 0, bm = 0;
     (void)hipEventElapsedTime(&nm, e->n0, e->n1);
@@ -4347,7 +4351,7 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
   float ms = 0;
   (void)hipEventElapsedTime(&ms, e->e0, e->e1);
   e->run_ms = ms;
-  e->kt_n = std::min<long long>(ncoll, mpcx_co::TMAX);
+  e->kt_n = std::min<long long>((ncoll + mpcx_co::TSTRIDE - 1) / mpcx_co::TSTRIDE, mpcx_co::TMAX);
   for (int k = 0; k < mpcx_co::NKT; k++) {
     double sum = 0;
     long long n = 0;

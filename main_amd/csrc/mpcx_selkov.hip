@@ -814,12 +814,23 @@ __global__ __launch_bounds__(CT) void sk_collide(SkP P, SkD D) {
 }
 
 // draws of the collision: 3 shift + 2 per cell + 2 per unaffected cell
-__global__ void sk_advance(SkP P, SkD D) {
-  if (P.rng_mode == 0) {
+// before the rotation: the draw cursor advanced past this collision's draws,
+// and the near bitmap (nwords words) cleared for the next near set
+__global__ __launch_bounds__(256) void sk_rotate_begin(SkP P, SkD D, long long nwords) {
+  const long long i0 = (long long)blockIdx.x * 256 + threadIdx.x, st = (long long)gridDim.x * 256;
+  for (long long i = i0; i < nwords; i += st) D.near_bits[i] = 0u;
+  if (i0 == 0 && P.rng_mode == 0) {
     const long long used = 3 + 2 * P.vol + 2 * (long long)D.uoff[P.vol];
     if (D.sc->cursor + used > D.nU) raise_error(&D.sc->err, ERR_DRAWS);
     D.sc->cursor += used;
   }
+}
+// interval start: cleared cell counts, census and first keys, every cell
+// unaffected until sk_affected marks the ones near a sphere
+__global__ __launch_bounds__(256) void sk_interval_begin(SkD D, long long V) {
+  const long long i0 = (long long)blockIdx.x * 256 + threadIdx.x, st = (long long)gridDim.x * 256;
+  for (long long i = i0; i < V; i += st) { D.cnt[i] = 0; D.unaff[i] = 1; }
+  for (long long i = i0; i < 3 * V; i += st) { D.tcount[i] = 0; D.first[i] = 0xffffffffu; }
 }
 
 // Rotation of RMPC_SelkovDimer.f90:79-121 (projection, cos-rotated
@@ -944,7 +955,9 @@ struct mpcx_sk {
   SkD D;
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  static constexpr int NKT = 4, TMAX = 256;   // near engine, sk_bulk, sk_collide, sk_rotate
+  // near engine, sk_bulk, sk_collide, sk_rotate: every TSTRIDE-th interval is
+  // timed (each event costs the stream a few us), at most TMAX of them
+  static constexpr int NKT = 4, TMAX = 256, TSTRIDE = 8;
   std::vector<hipEvent_t> tev;
   std::vector<char> tev_on;
   double kt_ms[NKT] = {0};
@@ -1218,21 +1231,19 @@ extern "C" int mpcx_sk_run(mpcx_sk* e, int64_t ncoll, const double* draws, int64
     const long long cnext = cstep + P.S;
     const bool sort_due = cnext % P.sort_step == 0;
     D.coll = (unsigned long long)(cnext / P.S);
-    const bool timed = c < mpcx_sk::TMAX;
+    const long long slot = c / mpcx_sk::TSTRIDE;
+    const bool timed = c % mpcx_sk::TSTRIDE == 0 && slot < mpcx_sk::TMAX;
     auto tmark = [&](int k, int edge) -> hipError_t {
       if (!timed) return hipSuccess;
-      e->tev_on[(size_t)c * mpcx_sk::NKT + k] = 1;
-      return hipEventRecord(e->tev[((size_t)c * mpcx_sk::NKT + k) * 2 + edge], st);
+      e->tev_on[(size_t)slot * mpcx_sk::NKT + k] = 1;
+      return hipEventRecord(e->tev[((size_t)slot * mpcx_sk::NKT + k) * 2 + edge], st);
     };
     if (timed)
-      for (int k = 0; k < mpcx_sk::NKT; k++) e->tev_on[(size_t)c * mpcx_sk::NKT + k] = 0;
+      for (int k = 0; k < mpcx_sk::NKT; k++) e->tev_on[(size_t)slot * mpcx_sk::NKT + k] = 0;
+    hipLaunchKernelGGL(sk_interval_begin, dim3(grid_for(3 * V, 256, 2048)), dim3(256), 0, st, D, (long long)V);
     SKCHK(tmark(0, 0));
     hipLaunchKernelGGL(sk_near_engine, dim3(1), dim3(SK_T), 0, st, P, D, sort_due ? SM_STOP : 0);
     SKCHK(tmark(0, 1));
-    hipLaunchKernelGGL(fill_kernel<int>, dim3(grid_for(V, 256)), dim3(256), 0, st, D.cnt, V, 0);
-    hipLaunchKernelGGL(fill_kernel<int>, dim3(grid_for(3 * V, 256)), dim3(256), 0, st, D.tcount, 3 * V, 0);
-    hipLaunchKernelGGL(fill_kernel<unsigned>, dim3(grid_for(3 * V, 256)), dim3(256), 0, st, D.first, 3 * V,
-                       0xffffffffu);
     if (sort_due) {
       hipLaunchKernelGGL(sk_bulk, dim3(gN), dim3(256), 0, st, P, D, P.S - 1, 0);
       sk_sort(e);
@@ -1244,7 +1255,6 @@ extern "C" int mpcx_sk_run(mpcx_sk* e, int64_t ncoll, const double* draws, int64
       hipLaunchKernelGGL(sk_bulk, dim3(gN), dim3(256), 0, st, P, D, P.S, 1);
       SKCHK(tmark(1, 1));
     }
-    hipLaunchKernelGGL(fill_kernel<int>, dim3(grid_for(V, 256)), dim3(256), 0, st, D.unaff, V, 1);
     hipLaunchKernelGGL(sk_affected, dim3(1), dim3(256), 0, st, P, D);
     launch_scan(st, D.unaff, V, D.partial, D.uoff);
     launch_scan(st, D.cnt, V, D.partial, D.off);
@@ -1252,9 +1262,8 @@ extern "C" int mpcx_sk_run(mpcx_sk* e, int64_t ncoll, const double* draws, int64
     SKCHK(tmark(2, 0));
     hipLaunchKernelGGL(sk_collide, dim3((unsigned)((V + CT - 1) / CT)), dim3(CT), 0, st, P, D);
     SKCHK(tmark(2, 1));
-    hipLaunchKernelGGL(sk_advance, dim3(1), dim3(1), 0, st, P, D);
-    hipLaunchKernelGGL(fill_kernel<unsigned>, dim3(grid_for(e->nwords + 2, 256)), dim3(256), 0, st, D.near_bits,
-                       e->nwords + 2, 0u);
+    hipLaunchKernelGGL(sk_rotate_begin, dim3(grid_for(e->nwords + 2, 256, 2048)), dim3(256), 0, st, P, D,
+                       (long long)(e->nwords + 2));
     SKCHK(tmark(3, 0));
     hipLaunchKernelGGL(sk_rotate, dim3(gN), dim3(256), 0, st, P, D);
     SKCHK(tmark(3, 1));
@@ -1266,7 +1275,7 @@ extern "C" int mpcx_sk_run(mpcx_sk* e, int64_t ncoll, const double* draws, int64
   float ms = 0;
   (void)hipEventElapsedTime(&ms, e->e0, e->e1);
   e->run_ms = ms;
-  const long long kt_n = std::min<long long>(ncoll, mpcx_sk::TMAX);
+  const long long kt_n = std::min<long long>((ncoll + mpcx_sk::TSTRIDE - 1) / mpcx_sk::TSTRIDE, mpcx_sk::TMAX);
   for (int k = 0; k < mpcx_sk::NKT; k++) {
     double sum = 0;
     long long n = 0;
