@@ -200,6 +200,7 @@ def selkov(steps=20, warmup=5, seed=8154543):
 ALGO_BYTES_SPLIT = {
     "co_bulk": 53.0,     # sp_bulk: cell r/w 8, offsets r/w 24, species 1, velocities read 12 (far replay),
                          # collision cell 4 + rank 2, distance byte 1, status 1
+    "co_far": 53.0,      # sp_far: the bytes of sp_bulk for the far particles, beside the near engine
     "co_rotate": 35.0,   # sp_rotate: velocities r/w 24, collision cell 4, cell record share 6, distance byte 1
     "co_collide": 23.0,  # sp_collide: velocities 12, bucket 4, cell record + counts share 7
     "co_scatter": 14.0,

@@ -219,15 +219,18 @@ int mpcx_co_set_precision(mpcx_co* e, int32_t split);
  * (at most 65536), for tuning.  Results do not depend on it.  No counterpart
  * in the reference. */
 int mpcx_co_set_bulk_grid(mpcx_co* e, int32_t grid);
-/* on = 1 (the default): with the one-workgroup near engine (mode 0) and FP64
- * storage, the far particles stream in one free flight of the whole interval
- * and bin on a second stream while the near engine runs (co_far); the near
- * set is binned afterwards (co_bin_near).  Far positions then agree with the
- * reference's event-by-event lazy streaming (evolve_mod.f95:474-527) to a few
- * ulps rather than bitwise; integer state is unaffected unless a particle
- * lies within that rounding of a cell face.  on = 0: every rebuild event is
- * replayed after the near engine, bit for bit.  No counterpart in the
- * reference. */
+/* on = 1 (the default): with the one-workgroup near engine (mode 0), the far
+ * particles stream in one free flight of the whole interval and bin on a
+ * second stream while the near engine runs (co_far; sp_far in split
+ * storage); the near set is binned afterwards (co_bin_rest, sp_bin_near).
+ * In FP64 storage far positions then agree with the reference's
+ * event-by-event lazy streaming (evolve_mod.f95:474-527) to a few ulps rather
+ * than bitwise, and a particle whose flight reaches a z wall -- where the
+ * bounce time (z - Lz)/vz would magnify those ulps -- replays the events
+ * exactly after the near engine; integer state is unaffected unless a
+ * particle lies within that rounding of a cell face.  on = 0: every rebuild
+ * event is replayed after the near engine, bit for bit.  No counterpart in
+ * the reference. */
 int mpcx_co_set_overlap(mpcx_co* e, int32_t on);
 /* Collision-cell data of the last collision: (mpcnumcell) arrays, 0-based
  * cell of each particle, and unshifted census counts (3, vol). */

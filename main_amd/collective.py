@@ -104,7 +104,7 @@ class CollectiveEngine:
         """Far particles in one flight beside the near engine (True) or the
         bitwise event replay after it (False), from the next run on."""
         check(lib().mpcx_co_set_overlap(self.h, C.c_int32(1 if on else 0)), "mpcx_co_set_overlap")
-        self.overlap = bool(on) and not self.split and self._ov_mode == 0
+        self.overlap = bool(on) and self._ov_mode == 0
 
     def close(self):
         if self.h:

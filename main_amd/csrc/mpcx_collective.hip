@@ -2858,6 +2858,17 @@ __global__ __launch_bounds__(CT) void co_collide_b(const CoP* Ps, const CoD* Ds,
 // Rotation of mpc_rotation_mod.f95:205-230 and selection of the next
 // interval's near set: a particle is near when its distance beyond a list
 // radius (recorded by co_bulk, rounded down) is within |v| tau + guard.
+// the largest sphere displacement since the interval's start (xs0 -> x0)
+__device__ __forceinline__ double start_shift(const CoP& P, const CoD& D) {
+  double shift = 0.0;
+  for (int s = 0; s < 2 * P.nd; s++) {
+    const double a = pbc_half(D.sc->x0[s] - D.sc->xs0[s], P.box[0]);
+    const double b = pbc_half(D.sc->y0[s] - D.sc->ys0[s], P.box[1]);
+    const double c = D.sc->z0[s] - D.sc->zs0[s];
+    shift = fmax(shift, sqrt(a * a + b * b + c * c));
+  }
+  return shift;
+}
 // classify 0: rotation only; 1: distances measured to the spheres at the
 // interval's end; 2: measured to the spheres at its start (co_far,
 // co_bin_rest), so the reach widens by the largest sphere displacement since.
@@ -2865,14 +2876,7 @@ __device__ __forceinline__ void rotate_body(const CoP& P, const CoD& D, int clas
                                             long long step) {
   double fmax2 = 0.0;
   const double tau = P.tau * (1.0 + 1e-9);
-  double shift = 0.0;
-  if (classify == 2)
-    for (int s = 0; s < 2 * P.nd; s++) {
-      const double a = pbc_half(D.sc->x0[s] - D.sc->xs0[s], P.box[0]);
-      const double b = pbc_half(D.sc->y0[s] - D.sc->ys0[s], P.box[1]);
-      const double c = D.sc->z0[s] - D.sc->zs0[s];
-      shift = fmax(shift, sqrt(a * a + b * b + c * c));
-    }
+  const double shift = classify == 2 ? start_shift(P, D) : 0.0;
   for (long long i = first + threadIdx.x; i < end; i += step) {
     int t = D.cell_of[i];
     const double2* rec = (const double2*)(D.cellrec + (long long)t * 8);
@@ -3186,24 +3190,116 @@ __device__ __forceinline__ bool sp_stream(int& ix, int& iy, int& iz, float& ox, 
   return false;
 }
 
-// co_bulk in split storage
-__global__ __launch_bounds__(256) void sp_bulk(CoP P, CoD D, SpD S, int do_stream, int do_bin) {
+// Binning of one split-storage particle per lane; whole waves call it
+// together (`valid` false for idle lanes): shifted cell (mpc_rotation_mod.f95:
+// 116-145) from the cell and offsets with the particle's rank in it, the
+// distance byte to the spheres `sph`, and the census, as bin_particle.
+__device__ __forceinline__ void sp_bin_particle(const CoP& P, const CoD& D, int i, bool valid, bool near,
+                                                unsigned char st, int ix, int iy, int iz, float ox, float oy,
+                                                float oz, int f, float fxs, float fys, float fzs, double zs,
+                                                const double (*sph)[MAXSG]) {
   const int NT = P.NT;
-  const int nev = D.sc->n_ev;
-  __shared__ int ev[MAXEV];
+  const int lane = threadIdx.x & 63;
+  bool ok = valid;
+  int t = 0;
+  if (valid) {
+    const int cx = sp_wrap(ix + (int)floorf(ox + fxs), P.Lx);
+    const int cy = sp_wrap(iy + (int)floorf(oy + fys), P.Ly);
+    const int cz = iz + (int)floorf(oz + fzs) + (zs < 0.0 ? 1 : 0);
+    const long long c = (long long)cx + (long long)P.Lx * cy + (long long)P.Lx * P.Ly * cz;
+    if (cz < 0 || c >= P.mpcnumcell) { raise_error(&D.sc->err, ERR_CELL_RANGE); ok = false; }
+    else t = D.mpc_pos[c];
+  }
+  int r = wave_atomic_rank(D.cnt, t, ok);
+  if (valid) {
+    if (ok && r >= BMAX) raise_error(&D.sc->err, ERR_BUCKET);
+    D.cell_of[i] = t;
+    D.rank_of[i] = ok ? (unsigned short)min(r, 65535) : (unsigned short)0xffff;
+    const double x = (double)ix + (double)ox, y = (double)iy + (double)oy, z = (double)iz + (double)oz;
+    double dmin = 1e30;
+    bool close = false;
+    for (int s = 0; s < 2 * P.nd; s++) {
+      double a = pbc_half(x - sph[0][s], P.box[0]);
+      double b = pbc_half(y - sph[1][s], P.box[1]);
+      double c = z - sph[2][s];
+      const double d2 = a * a + b * b + c * c, far = P.full[s >= P.nd] + 8.0;
+      close |= d2 < far * far;
+    }
+    if (__ballot(close))   // most waves lie far from every sphere: no square roots at all
+      for (int s = 0; s < 2 * P.nd; s++) {
+        double a = pbc_half(x - sph[0][s], P.box[0]);
+        double b = pbc_half(y - sph[1][s], P.box[1]);
+        double c = z - sph[2][s];
+        const double d2 = a * a + b * b + c * c, far = P.full[s >= P.nd] + 8.0;
+        if (d2 < far * far) dmin = fmin(dmin, sqrt(d2) - P.full[s >= P.nd]);
+      }
+    // 1e-5 below the measured distance: the FP32 position must not make a particle look farther
+    double qd = floor(fmax(dmin - 1e-5, 0.0) * 32.0);
+    if (st & ST_INSIDE) qd = 0.0;
+    D.dq[i] = (unsigned char)fmin(qd, 255.0);
+  }
+  unsigned key = (unsigned)i;
+  bool cen = false;
+  int ck = 0;
+  if (valid && ok) {
+    bool skip = false;
+    if (near && (st & ST_INSIDE)) { key = (unsigned)NT + (unsigned)i; skip = (st & ST_EXCL) != 0; }
+    int sp = f == 0 ? 0 : f == 1 ? 1 : f == 2 ? 2 : -1;
+    if (!skip && sp >= 0) {
+      const long long u = (long long)ix + (long long)P.Lx * iy + (long long)P.Lx * P.Ly * iz;
+      if (u < 0 || u >= P.vol) raise_error(&D.sc->err, ERR_CELL_RANGE);
+      else { cen = true; ck = (int)(sp * P.vol + u); }
+    }
+  }
+  {
+    unsigned long long todo = __ballot(cen);
+    const unsigned long long outside = __ballot(cen && key < (unsigned)NT);
+    int gsize = 0;
+    unsigned gkey = 0;
+    bool leader = false;
+    while (todo) {
+      int ld = __ffsll((unsigned long long)todo) - 1;
+      int k = __shfl(ck, ld);
+      unsigned long long grp = __ballot(cen && ck == k);
+      unsigned long long go = grp & outside;
+      int src = __ffsll((unsigned long long)(go ? go : grp)) - 1;
+      unsigned mk = (unsigned)__shfl((int)key, src);
+      if (lane == ld) { leader = true; gsize = __popcll(grp); gkey = mk; }
+      todo &= ~grp;
+    }
+    if (leader) {
+      atomicAdd(&D.tcount[ck], gsize);   // no return: the collision kernel checks the totals against CENSUS_MAX
+      if (ck >= P.vol && ck < 2 * P.vol) atomicMin(&D.firstB[ck - P.vol], gkey);
+    }
+  }
+}
+
+// co_bulk in split storage.  Far particles take one free flight over all the
+// interval's steps: the reference streams such a particle lazily at each
+// rebuild, the same path (a far particle moves at most |v| tau, about a cell,
+// so it meets at most one wall).  FAR = true (sp_far, beside the near engine,
+// as co_far): far particles only, the flight of MPCtstep steps, distances to
+// the spheres at the interval's start; sp_bin_near bins the near set after.
+template <bool FAR>
+__device__ __forceinline__ void sp_bulk_body(const CoP& P, const CoD& D, const SpD& S, int do_stream, int do_bin) {
+  const int NT = P.NT;
+  const int nev = FAR ? 0 : D.sc->n_ev;
   __shared__ double sph[3][MAXSG];
-  for (int j = threadIdx.x; j < nev && j < MAXEV; j += blockDim.x) ev[j] = D.sc->events[j];
+  const double* sx = FAR ? D.sc->xs0 : D.sc->x0;
+  const double* sy = FAR ? D.sc->ys0 : D.sc->y0;
+  const double* sz = FAR ? D.sc->zs0 : D.sc->z0;
   for (int j = threadIdx.x; j < 2 * P.nd; j += blockDim.x) {
-    sph[0][j] = D.sc->x0[j]; sph[1][j] = D.sc->y0[j]; sph[2][j] = D.sc->z0[j];
+    sph[0][j] = sx[j]; sph[1][j] = sy[j]; sph[2][j] = sz[j];
   }
   __syncthreads();
   int ev_total = 0;
-  for (int e = 0; e < nev && e < MAXEV; e++) ev_total += ev[e];
+  if (FAR) ev_total = P.MPCtstep;
+  else
+    for (int e = 0; e < nev && e < MAXEV; e++) ev_total += D.sc->events[e];
   double xs = 0, ys = 0, zs = 0;
   if (do_bin) collision_shift(P, D, xs, ys, zs);
   const float fxs = (float)xs, fys = (float)ys, fzs = (float)zs;
   const float mdt = (float)P.MDt;
-  const int lane = threadIdx.x & 63;
   const long long step = (long long)gridDim.x * blockDim.x;
   for (long long base = (long long)blockIdx.x * blockDim.x; base < NT; base += step) {
     const int i = (int)(base + threadIdx.x);
@@ -3212,17 +3308,15 @@ __global__ __launch_bounds__(256) void sp_bulk(CoP P, CoD D, SpD S, int do_strea
     int ix = 0, iy = 0, iz = 0, f = -1;
     float ox = 0, oy = 0, oz = 0;
     unsigned char st = 0;
-    if (valid) {
-      near = (D.near_bits[i >> 5] >> (i & 31)) & 1u;
+    if (valid) near = (D.near_bits[i >> 5] >> (i & 31)) & 1u;
+    const bool mine = valid && !(FAR && near);
+    if (mine) {
       st = D.status[i];
       if (!near && st != 0) { D.status[i] = 0; st = 0; }
       sp_cell_xyz(P, S.cell[i], ix, iy, iz);
       ox = S.ox[i]; oy = S.oy[i]; oz = S.oz[i];
       f = S.flag[i];
-      if (do_stream && !near && nev > 0) {
-        // one free flight over all the interval's steps: the reference streams
-        // such a particle lazily at each rebuild, the same path (a far particle
-        // moves at most |v| tau, about a cell, so it meets at most one wall)
+      if (do_stream && !near && (FAR || nev > 0)) {
         float vx = S.vx[i], vy = S.vy[i], vz = S.vz[i];
         const int f0 = f;
         const bool bounced = sp_stream(ix, iy, iz, ox, oy, oz, vx, vy, vz, f, mdt * (float)ev_total, P, D.react_box,
@@ -3234,79 +3328,44 @@ __global__ __launch_bounds__(256) void sp_bulk(CoP P, CoD D, SpD S, int do_strea
       }
     }
     if (!do_bin) continue;
-    bool ok = valid;
-    int t = 0;
-    if (valid) {   // shifted cell (mpc_rotation_mod.f95:116-145) from the cell and offsets
-      const int cx = sp_wrap(ix + (int)floorf(ox + fxs), P.Lx);
-      const int cy = sp_wrap(iy + (int)floorf(oy + fys), P.Ly);
-      const int cz = iz + (int)floorf(oz + fzs) + (zs < 0.0 ? 1 : 0);
-      const long long c = (long long)cx + (long long)P.Lx * cy + (long long)P.Lx * P.Ly * cz;
-      if (cz < 0 || c >= P.mpcnumcell) { raise_error(&D.sc->err, ERR_CELL_RANGE); ok = false; }
-      else t = D.mpc_pos[c];
-    }
-    int r = wave_atomic_rank(D.cnt, t, ok);
-    if (valid) {
-      if (ok && r >= BMAX) raise_error(&D.sc->err, ERR_BUCKET);
-      D.cell_of[i] = t;
-      D.rank_of[i] = ok ? (unsigned short)min(r, 65535) : (unsigned short)0xffff;
-      const double x = (double)ix + (double)ox, y = (double)iy + (double)oy, z = (double)iz + (double)oz;
-      double dmin = 1e30;
-      bool close = false;
-      for (int s = 0; s < 2 * P.nd; s++) {
-        double a = pbc_half(x - sph[0][s], P.box[0]);
-        double b = pbc_half(y - sph[1][s], P.box[1]);
-        double c = z - sph[2][s];
-        const double d2 = a * a + b * b + c * c, far = P.full[s >= P.nd] + 8.0;
-        close |= d2 < far * far;
-      }
-      if (__ballot(close))   // most waves lie far from every sphere: no square roots at all
-        for (int s = 0; s < 2 * P.nd; s++) {
-          double a = pbc_half(x - sph[0][s], P.box[0]);
-          double b = pbc_half(y - sph[1][s], P.box[1]);
-          double c = z - sph[2][s];
-          const double d2 = a * a + b * b + c * c, far = P.full[s >= P.nd] + 8.0;
-          if (d2 < far * far) dmin = fmin(dmin, sqrt(d2) - P.full[s >= P.nd]);
-        }
-      // 1e-5 below the measured distance: the FP32 position must not make a particle look farther
-      double qd = floor(fmax(dmin - 1e-5, 0.0) * 32.0);
-      if (st & ST_INSIDE) qd = 0.0;
-      D.dq[i] = (unsigned char)fmin(qd, 255.0);
-    }
-    unsigned key = (unsigned)i;
-    bool cen = false;
-    int ck = 0;
-    if (valid && ok) {
-      bool skip = false;
-      if (near && (st & ST_INSIDE)) { key = (unsigned)NT + (unsigned)i; skip = (st & ST_EXCL) != 0; }
-      int sp = f == 0 ? 0 : f == 1 ? 1 : f == 2 ? 2 : -1;
-      if (!skip && sp >= 0) {
-        const long long u = (long long)ix + (long long)P.Lx * iy + (long long)P.Lx * P.Ly * iz;
-        if (u < 0 || u >= P.vol) raise_error(&D.sc->err, ERR_CELL_RANGE);
-        else { cen = true; ck = (int)(sp * P.vol + u); }
-      }
-    }
-    {
-      unsigned long long todo = __ballot(cen);
-      const unsigned long long outside = __ballot(cen && key < (unsigned)NT);
-      int gsize = 0;
-      unsigned gkey = 0;
-      bool leader = false;
-      while (todo) {
-        int ld = __ffsll((unsigned long long)todo) - 1;
-        int k = __shfl(ck, ld);
-        unsigned long long grp = __ballot(cen && ck == k);
-        unsigned long long go = grp & outside;
-        int src = __ffsll((unsigned long long)(go ? go : grp)) - 1;
-        unsigned mk = (unsigned)__shfl((int)key, src);
-        if (lane == ld) { leader = true; gsize = __popcll(grp); gkey = mk; }
-        todo &= ~grp;
-      }
-      if (leader) {
-        atomicAdd(&D.tcount[ck], gsize);   // no return: the collision kernel checks the totals against CENSUS_MAX
-        if (ck >= P.vol && ck < 2 * P.vol) atomicMin(&D.firstB[ck - P.vol], gkey);
-      }
-    }
+    sp_bin_particle(P, D, i, mine, near, st, ix, iy, iz, ox, oy, oz, f, fxs, fys, fzs, zs, sph);
   }
+}
+__global__ __launch_bounds__(256) void sp_bulk(CoP P, CoD D, SpD S, int do_stream, int do_bin) {
+  sp_bulk_body<false>(P, D, S, do_stream, do_bin);
+}
+__global__ __launch_bounds__(256) void sp_far(CoP P, CoD D, SpD S, int do_bin) {
+  sp_bulk_body<true>(P, D, S, 1, do_bin);
+}
+// The near set after the near engine (stored back to split storage): binning
+// with distances to the spheres at the interval's start, as sp_far; and the
+// flight check.
+__global__ __launch_bounds__(256) void sp_bin_near(CoP P, CoD D, SpD S, int do_bin) {
+  __shared__ double sph[3][MAXSG];
+  for (int j = threadIdx.x; j < 2 * P.nd; j += blockDim.x) {
+    sph[0][j] = D.sc->xs0[j]; sph[1][j] = D.sc->ys0[j]; sph[2][j] = D.sc->zs0[j];
+  }
+  __syncthreads();
+  double xs = 0, ys = 0, zs = 0;
+  if (do_bin) collision_shift(P, D, xs, ys, zs);
+  const float fxs = (float)xs, fys = (float)ys, fzs = (float)zs;
+  const long long nn = do_bin ? min(D.sc->near_num, D.near_cap) : 0;
+  for (long long base = (long long)blockIdx.x * blockDim.x; base < nn; base += (long long)gridDim.x * blockDim.x) {
+    const long long q = base + threadIdx.x;
+    const bool valid = q < nn;
+    int i = 0, ix = 0, iy = 0, iz = 0, f = -1;
+    float ox = 0, oy = 0, oz = 0;
+    unsigned char st = 0;
+    if (valid) {
+      i = D.near_list[q];
+      st = D.status[i];
+      sp_cell_xyz(P, S.cell[i], ix, iy, iz);
+      ox = S.ox[i]; oy = S.oy[i]; oz = S.oz[i];
+      f = S.flag[i];
+    }
+    sp_bin_particle(P, D, i, valid, true, st, ix, iy, iz, ox, oy, oz, f, fxs, fys, fzs, zs, sph);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) check_flight(P, D);
 }
 
 // co_collide in split storage: Kahan-compensated FP32 sums in ascending order
@@ -3380,9 +3439,11 @@ __global__ __launch_bounds__(CT) void sp_collide(CoP P, CoD D, SpD S) {
 }
 
 // co_rotate in split storage (FP32 rotation) and the next near set
-__global__ __launch_bounds__(256) void sp_rotate(CoP P, CoD D, SpD S) {
+// classify as rotate_body (1 or 2)
+__global__ __launch_bounds__(256) void sp_rotate(CoP P, CoD D, SpD S, int classify) {
   float fmax2 = 0.0f;
   const double tau = P.tau * (1.0 + 1e-9);
+  const double shift = classify == 2 ? start_shift(P, D) : 0.0;
   const float cr = (float)P.cosrot, sr = (float)P.sinrot, omc = (float)P.omc;
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < P.NT; i += (long long)gridDim.x * 256) {
     const int t = D.cell_of[i];
@@ -3398,7 +3459,7 @@ __global__ __launch_bounds__(256) void sp_rotate(CoP P, CoD D, SpD S) {
     vx = cx + rx; vy = cy + ry; vz = cz + rz;
     S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz;
     const float v2 = (vx * vx + vy * vy) + vz * vz;
-    const double reach = sqrt((double)v2) * tau + P.guard + 1e-6;
+    const double reach = sqrt((double)v2) * tau + P.guard + 1e-6 + shift;
     if ((double)D.dq[i] * (1.0 / 32.0) <= reach) atomicOr(&D.near_bits[i >> 5], 1u << (i & 31));
     else fmax2 = fmaxf(fmax2, v2);
   }
@@ -4278,12 +4339,13 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
     if (timed)
       for (int k = 0; k < mpcx_co::NKT; k++) e->tev_on[(size_t)slot * mpcx_co::NKT + k] = 0;
     // the far particles' flight and binning beside the near engine
-    const bool ov = e->overlap && !e->split && e->near_mode == 0;
+    const bool ov = e->overlap && e->near_mode == 0;
     if (ov) {
       HIPCHK(hipEventRecord(e->fork, st));
       HIPCHK(hipStreamWaitEvent(e->st2, e->fork, 0));
       if (!sort_due) HIPCHK(tmark(1, 0, e->st2));
-      hipLaunchKernelGGL(co_far, dim3(e->g_bulk), dim3(256), 0, e->st2, P, D, sort_due ? 0 : 1);
+      if (e->split) hipLaunchKernelGGL(sp_far, dim3(e->g_spbulk), dim3(256), 0, e->st2, P, D, e->S, sort_due ? 0 : 1);
+      else hipLaunchKernelGGL(co_far, dim3(e->g_bulk), dim3(256), 0, e->st2, P, D, sort_due ? 0 : 1);
       if (!sort_due) HIPCHK(tmark(1, 1, e->st2));
       HIPCHK(hipEventRecord(e->join, e->st2));
     }
@@ -4301,7 +4363,8 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
       else hipLaunchKernelGGL(co_bulk, dim3(e->g_bulk), dim3(256), 0, st, P, D, stream, bin);
     };
     if (sort_due) {
-      if (ov) hipLaunchKernelGGL(co_bin_rest, dim3(128), dim3(256), 0, st, P, D, 0);
+      if (ov && sp) hipLaunchKernelGGL(sp_bin_near, dim3(1), dim3(256), 0, st, P, D, e->S, 0);   // the flight check
+      else if (ov) hipLaunchKernelGGL(co_bin_rest, dim3(128), dim3(256), 0, st, P, D, 0);
       else bulk(1, 0);
       int r = sp ? sp_sort(e, st) : co_sort(e, st);
       if (r) return r;
@@ -4313,7 +4376,8 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
       bulk(0, 1);
     } else if (ov) {
       if (last) HIPCHK(hipEventRecord(e->b0, st));
-      hipLaunchKernelGGL(co_bin_rest, dim3(128), dim3(256), 0, st, P, D, 1);
+      if (sp) hipLaunchKernelGGL(sp_bin_near, dim3(64), dim3(256), 0, st, P, D, e->S, 1);
+      else hipLaunchKernelGGL(co_bin_rest, dim3(128), dim3(256), 0, st, P, D, 1);
     } else {
       if (last) HIPCHK(hipEventRecord(e->b0, st));
       HIPCHK(tmark(1, 0, st));
@@ -4330,7 +4394,7 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
     HIPCHK(tmark(2, 1, st));
     hipLaunchKernelGGL(rotate_begin, dim3(grid_for(e->nwords + 1, 256, 2048)), dim3(256), 0, st, D, e->nwords);
     HIPCHK(tmark(3, 0, st));
-    if (sp) hipLaunchKernelGGL(sp_rotate, dim3(gN), dim3(256), 0, st, P, D, e->S);
+    if (sp) hipLaunchKernelGGL(sp_rotate, dim3(gN), dim3(256), 0, st, P, D, e->S, ov && !sort_due ? 2 : 1);
     else hipLaunchKernelGGL(co_rotate, dim3(gN), dim3(256), 0, st, P, D, ov && !sort_due ? 2 : 1);
     HIPCHK(tmark(3, 1, st));
     if (last) HIPCHK(hipEventRecord(e->b1, st));
