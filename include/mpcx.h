@@ -239,11 +239,13 @@ int mpcx_co_get_cells(mpcx_co* e, double* vcm_uv /* (mpcnumcell, 8) */, int32_t*
 /* Wall time of the HIP events bracketing the last mpcx_co_run (ms), and of
  * the last interval's bulk kernels (co_bulk .. co_rotate) and near engine (ms). */
 int mpcx_co_timing(mpcx_co* e, double* run_ms, double* bulk_ms, double* near_ms);
-/* Near-engine counters accumulated since creation: real-time ticks (100 MHz)
- * of [0] whole launches, [1] list rebuilds, [2] particle phases, [3] dimer
- * (serial) sections; [4] rebuild count; [5] last near-set size; [6] last
- * inside-list size; [7] launches. */
-int mpcx_co_near_profile(mpcx_co* e, uint64_t* prof8);
+/* Near-engine counters accumulated since creation (10 values): real-time
+ * ticks (100 MHz) of [0] launches from the first MD step to the end, [1] list
+ * rebuilds, [2] particle phases, [3] dimer (serial) sections; [4] rebuild
+ * count; [5] last near-set size; [6] last inside-list size; [7] launches;
+ * ticks of [8] set-up and gather before the first step, [9] the end phase
+ * (census exclusion, write-back).  The one-workgroup engine fills [8], [9]. */
+int mpcx_co_near_profile(mpcx_co* e, uint64_t* prof10);
 /* Mean duration (ms) of each kernel of the collision interval over the last
  * mpcx_co_run, from HIP events on the stream each runs on; every 8th interval
  * is timed (at most 256 of them), since each event costs the stream a few

@@ -236,13 +236,13 @@ class CollectiveEngine:
         return float(tp[0])
 
     def near_profile(self) -> dict:
-        a = np.zeros(8, np.uint64)
+        a = np.zeros(10, np.uint64)
         check(lib().mpcx_co_near_profile(self.h, ptr(a)), "mpcx_co_near_profile")
         us = lambda v: float(v) / 100.0   # 100 MHz ticks -> microseconds
         n = max(int(a[7]), 1)
         return dict(launches=int(a[7]), total_us=us(a[0]) / n, rebuild_us=us(a[1]) / n,
                     particle_us=us(a[2]) / n, serial_us=us(a[3]) / n, rebuilds=int(a[4]) / n,
-                    near_num=int(a[5]), inside_num=int(a[6]))
+                    near_num=int(a[5]), inside_num=int(a[6]), gather_us=us(a[8]) / n, end_us=us(a[9]) / n)
 
     OBSERVABLES = ("n", "nA", "nB", "nS", "psol_x", "psol_y", "psol_z", "p_x", "p_y", "p_z", "ke_sol", "ke_dim",
                    "T_sol", "T_dim", "E_tot", "v_prop")

@@ -61,11 +61,14 @@ struct CoScalars {
   double near_vmax2;
   double tpotential;
   double x0[MAXSG], y0[MAXSG], z0[MAXSG];   // spheres at the interval start
-  unsigned long long prof[8];   // near engine: total, rebuild, particle phase, serial (100 MHz ticks), rebuilds
+  unsigned long long prof[10];  // near engine: total, rebuild, particle phase, serial (100 MHz ticks), rebuilds,
+                                // near set, inside list, launches, set-up + gather, end phase (ticks)
   unsigned long long coll;      // collision counter of the current interval (Philox counter)
   double xs0[MAXSG], ys0[MAXSG], zs0[MAXSG];   // copy of x0..z0 made at the interval start (the near
                                                // engine overwrites x0..z0 while co_far still reads them)
   int n_wall, pad_w;            // far particles co_far left to co_bin_rest (they reach a wall)
+  int n_late, pad_l;            // late collision cells listed (zone around the spheres, deferred particles)
+  unsigned long long next_fmax2;   // far max |v|^2 of the next interval while far_maxv2 is still in use
 };
 
 // Dimers whose near regions can overlap within an interval form a cluster;
@@ -144,6 +147,12 @@ struct CoD {
   double* rec_pot;        // [p][4]
   int* wall_list;         // [wall_cap] far particles whose flight reaches a wall (co_far -> co_bin_rest)
   int wall_cap, pad_w;
+  // cells collided in two passes (one beside the near engine, one after it):
+  int* slab;              // [mpcnumcell][BMAX] members of each collision cell (by tour position), fixed slots
+  int* stamp;             // [mpcnumcell] epoch at which the cell was listed late (collided after the near engine)
+  int* late;              // [late_cap] late cells of the interval (tour positions)
+  unsigned* next_bits;    // the next interval's near bitmap while near_bits is still in use
+  int late_cap, epoch;
 };
 
 // ------------------------------------------------------------- streaming
@@ -1057,6 +1066,7 @@ __device__ __forceinline__ void near_body(const CoP P, const CoD D, int nsteps, 
   constexpr int NSPH = 2 * ND;
   constexpr int NV = 4 * NSPH + 1;
   CoScalars* sc = D.sc;
+  const unsigned long long Tentry = __builtin_amdgcn_s_memrealtime();
 
   for (int j = tid; j < 12 * MAXS; j += NE_T) g_ne.sd[j / MAXS][j % MAXS] = D.dim[(j / MAXS) * MAXSG + j % MAXS];
   if (tid == 0) {
@@ -1188,6 +1198,7 @@ __device__ __forceinline__ void near_body(const CoP P, const CoD D, int nsteps, 
   }
 
   // ---- end of the interval: census exclusion, write back through the near slots
+  const unsigned long long Tend = __builtin_amdgcn_s_memrealtime();
   const int n = g_ne.in_num;
   for (int k = tid; k < n; k += NE_T) {
     const bool ex = k < LCAP ? ne_excluded<ND, true>(P, D, k) : ne_excluded<ND, false>(P, D, k);
@@ -1209,6 +1220,7 @@ __device__ __forceinline__ void near_body(const CoP P, const CoD D, int nsteps, 
     sc->prof[0] += __builtin_amdgcn_s_memrealtime() - T0;
     sc->prof[1] += Tr; sc->prof[2] += Tp; sc->prof[3] += Ts; sc->prof[4] += Nr;
     sc->prof[5] = g_ne.near_num; sc->prof[6] = g_ne.in_num; sc->prof[7] += 1;
+    sc->prof[8] += T0 - Tentry; sc->prof[9] += __builtin_amdgcn_s_memrealtime() - Tend;
     ne_save_scalars(sc);
     for (int s = 0; s < NSPH; s++) { sc->x0[s] = g_ne.sd[DX][s]; sc->y0[s] = g_ne.sd[DY][s]; sc->z0[s] = g_ne.sd[DZ][s]; }
   }
@@ -2456,15 +2468,64 @@ __device__ __forceinline__ long long shifted_cell(const CoP& P, double x, double
   return (long long)c - 1;
 }
 
+// how far beyond the larger list radius a near particle may end the interval:
+// its reach (|v| tau + guard, within 1.5 for |v| below 2.4) plus its own
+// travel, with margin, plus half a cell diagonal for the test on cell centres
+constexpr double ZONE_PAD = 5.9;
+// a deferred particle's cell is taken from its single flight; faces within
+// this distance of it add the neighbouring cells (the replay differs by rounding)
+constexpr double LATE_MARGIN = 1e-6;
+
+// lists tour position t as late, once per interval
+__device__ __forceinline__ void late_cell(const CoD& D, int t) {
+  if (atomicMax(&D.stamp[t], D.epoch) < D.epoch) {
+    const int k = atomicAdd(&D.sc->n_late, 1);
+    if (k < D.late_cap) D.late[k] = t;
+    else raise_error(&D.sc->err, ERR_ZONE_CAP);
+  }
+}
+// A deferred (wall-reaching) far particle: its single flight with the bounce
+// locates its end cell, listed late with the neighbours across faces within
+// LATE_MARGIN; returns that cell's tour position (or -1 out of range)
+__device__ __forceinline__ int late_cells_of(const CoP& P, const CoD& D, double x, double y, double z, double vx,
+                                             double vy, double vz, double xs, double ys, double zs) {
+  int f = 0;
+  stream_flight(x, y, z, vx * P.MDt, vy * P.MDt, vz * P.MDt, (double)P.MPCtstep, P);
+  (void)stream_walls(x, y, z, vx, vy, vz, f, P, D.react_box, &D.sc->err);
+  const double X = wrap_periodic_any(x + xs, P.box[0]), Y = wrap_periodic_any(y + ys, P.box[1]);
+  const double Zc = z + zs + (zs < 0.0 ? 1.0 : 0.0);
+  const double fx = floor(X), fy = floor(Y), fz = floor(Zc);
+  int lo[3], hi[3];
+  const double fr[3] = {X - fx, Y - fy, Zc - fz};
+  const int base[3] = {(int)fx, (int)fy, (int)fz};
+  for (int d = 0; d < 3; d++) {
+    lo[d] = base[d] - (fr[d] < LATE_MARGIN ? 1 : 0);
+    hi[d] = base[d] + (fr[d] > 1.0 - LATE_MARGIN ? 1 : 0);
+  }
+  int first = -1;
+  for (int kz = lo[2]; kz <= hi[2]; kz++)
+    for (int ky = lo[1]; ky <= hi[1]; ky++)
+      for (int kx = lo[0]; kx <= hi[0]; kx++) {
+        if (kz < 0 || kz > P.Lz) continue;
+        const int ix = ((kx % P.Lx) + P.Lx) % P.Lx, iy = ((ky % P.Ly) + P.Ly) % P.Ly;
+        const int t = D.mpc_pos[(long long)ix + (long long)P.Lx * iy + (long long)P.Lx * P.Ly * kz];
+        late_cell(D, t);
+        if (kx == base[0] && ky == base[1] && kz == base[2]) first = t;
+      }
+  return first;
+}
 // Binning of one particle per lane; whole waves call it together (`valid`
 // false for idle lanes).  The shifted cell, counted by tour position with the
 // particle's rank in it; the distance to the nearest sphere of `sph` beyond
 // its list radius (selection of the next interval's near set); the census of
 // distribute_mod.f95:95-183: outside particles keyed by index, inside ones
 // after all outside ones, inside ones within a cutoff skipped.
+// slab 0: cell and rank for co_scatter; 1: the particle's slot in its cell's
+// slab written directly; 2: the same, and the cell must be listed late (the
+// particle was binned after the first collision pass).
 __device__ __forceinline__ void bin_particle(const CoP& P, const CoD& D, int i, bool valid, bool near,
                                              unsigned char st, double x, double y, double z, int f, double xs,
-                                             double ys, double zs, const double (*sph)[MAXSG]) {
+                                             double ys, double zs, const double (*sph)[MAXSG], int slab = 0) {
   const int NT = P.NT;
   const int lane = threadIdx.x & 63;
   bool ok = valid;
@@ -2478,7 +2539,11 @@ __device__ __forceinline__ void bin_particle(const CoP& P, const CoD& D, int i, 
   if (valid) {
     if (ok && r >= BMAX) raise_error(&D.sc->err, ERR_BUCKET);
     D.cell_of[i] = t;
-    D.rank_of[i] = ok ? (unsigned short)min(r, 65535) : (unsigned short)0xffff;
+    if (!slab) D.rank_of[i] = ok ? (unsigned short)min(r, 65535) : (unsigned short)0xffff;
+    else if (ok && r < BMAX) {
+      D.slab[(long long)t * BMAX + r] = i;
+      if (slab == 2 && D.stamp[t] != D.epoch) raise_error(&D.sc->err, ERR_ZONE);
+    }
     // distance beyond the list radius; 8 or more beyond every radius gives the
     // cap (255 at 7.97) without a sqrt
     double dmin = 1e30;
@@ -2592,9 +2657,11 @@ __device__ __forceinline__ void wall_defer(const CoD& D, bool wall, int i) {
 // is not taken: the reference's bounce time (z - Lz)/vz magnifies the ulps by
 // |v|/|vz|, so such particles go to the deferred list and replay the events
 // in co_bin_rest, which also bins the near set once the near engine is done.
+// slab (FAR only): bin into the cells' slabs (first collision pass beside the
+// near engine); deferred particles then list their possible end cells late.
 template <bool FAR>
 __device__ __forceinline__ void bulk_body(const CoP& P, const CoD& D, int do_stream, int do_bin, long long first,
-                                          long long end, long long step) {
+                                          long long end, long long step, int slab = 0) {
   const int nev = FAR ? 0 : min(D.sc->n_ev, MAXEV);
   __shared__ int ev[MAXEV];
   __shared__ double sph[3][MAXSG];
@@ -2641,6 +2708,10 @@ __device__ __forceinline__ void bulk_body(const CoP& P, const CoD& D, int do_str
           __builtin_nontemporal_store(x, &D.x[i]);
           __builtin_nontemporal_store(y, &D.y[i]);
           __builtin_nontemporal_store(z, &D.z[i]);
+        } else if (slab && do_bin) {   // the first pass must leave its possible end cells alone
+          const int t = late_cells_of(P, D, x, y, z, D.vx[i], D.vy[i], D.vz[i], xs, ys, zs);
+          D.cell_of[i] = t >= 0 ? t : 0;   // co_rotate_far skips it there (a late cell)
+          if (t < 0) raise_error(&D.sc->err, ERR_CELL_RANGE);
         }
       } else {
         x = D.x[i]; y = D.y[i]; z = D.z[i];
@@ -2652,15 +2723,16 @@ __device__ __forceinline__ void bulk_body(const CoP& P, const CoD& D, int do_str
     }
     if (FAR) wall_defer(D, wall, i);
     if (!do_bin) continue;
-    bin_particle(P, D, i, mine && !wall, near, st, x, y, z, f, xs, ys, zs, sph);
+    bin_particle(P, D, i, mine && !wall, near, st, x, y, z, f, xs, ys, zs, sph, slab);
   }
 }
 __global__ __launch_bounds__(256) void co_bulk(CoP P, CoD D, int do_stream, int do_bin) {
   bulk_body<false>(P, D, do_stream, do_bin, (long long)blockIdx.x * blockDim.x, P.NT,
                    (long long)gridDim.x * blockDim.x);
 }
-__global__ __launch_bounds__(256) void co_far(CoP P, CoD D, int do_bin) {
-  bulk_body<true>(P, D, 1, do_bin, (long long)blockIdx.x * blockDim.x, P.NT, (long long)gridDim.x * blockDim.x);
+__global__ __launch_bounds__(256) void co_far(CoP P, CoD D, int do_bin, int slab) {
+  bulk_body<true>(P, D, 1, do_bin, (long long)blockIdx.x * blockDim.x, P.NT, (long long)gridDim.x * blockDim.x,
+                  slab);
 }
 // co_far's single flight covers the whole interval: the rebuild events
 // recorded by the near engine must add up to it
@@ -2673,7 +2745,8 @@ __device__ __forceinline__ void check_flight(const CoP& P, const CoD& D) {
 // After the near engine and co_far: the deferred far particles replay the
 // interval's events; with do_bin they and the near set are binned (distances
 // to the spheres at the interval's start, as co_far).  Then the flight check.
-__global__ __launch_bounds__(256) void co_bin_rest(CoP P, CoD D, int do_bin) {
+// slab: bin into the cells' slabs, each cell listed late (second pass)
+__global__ __launch_bounds__(256) void co_bin_rest(CoP P, CoD D, int do_bin, int slab) {
   __shared__ int ev[MAXEV];
   __shared__ double sph[3][MAXSG];
   const int nev = min(D.sc->n_ev, MAXEV);
@@ -2702,7 +2775,7 @@ __global__ __launch_bounds__(256) void co_bin_rest(CoP P, CoD D, int do_bin) {
       st = D.status[i];
       if (!near) replay_events(P, D, i, x, y, z, f, ev, nev);
     }
-    if (do_bin) bin_particle(P, D, i, valid, near, st, x, y, z, f, xs, ys, zs, sph);
+    if (do_bin) bin_particle(P, D, i, valid, near, st, x, y, z, f, xs, ys, zs, sph, slab ? 2 : 0);
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) check_flight(P, D);
 }
@@ -2727,6 +2800,107 @@ __global__ __launch_bounds__(256) void co_scatter_b(const CoD* Ds, const Tile* t
   const Tile t = tiles[blockIdx.x];
   const CoD D = Ds[t.r];
   scatter_body(D, t.begin, t.end, 256);
+}
+
+// Rotation of mpc_rotation_mod.f95:205-230: v <- c + R(axis u) (v - c)
+__device__ __forceinline__ void rotate_velocity(const CoP& P, double cx, double cy, double cz, double ux, double uy,
+                                                double uz, double& vx, double& vy, double& vz) {
+  double dx = vx - cx, dy = vy - cy, dz = vz - cz;
+  const double cr = P.cosrot, sr = P.sinrot, omc = P.omc;
+  double rx = (dx * (cr + (ux * ux) * omc) + dy * ((ux * uy) * omc - uz * sr)) + dz * ((ux * uz) * omc + uy * sr);
+  double ry = (dx * ((ux * uy) * omc + uz * sr) + dy * (cr + (uy * uy) * omc)) + dz * ((uy * uz) * omc - ux * sr);
+  double rz = (dx * ((ux * uz) * omc - uy * sr) + dy * ((uy * uz) * omc + ux * sr)) + dz * (cr + (uz * uz) * omc);
+  vx = cx + rx; vy = cy + ry; vz = cz + rz;
+}
+// the next interval's near set: near when the distance beyond a list radius
+// (recorded by the binning, rounded down) is within |v| tau + guard, widened
+// by `shift` (sphere motion since the distance was measured); far ones feed
+// the far max |v|^2
+__device__ __forceinline__ void classify_next(const CoP& P, unsigned* bits, const unsigned char* dq, int i,
+                                              double vx, double vy, double vz, double tau, double shift,
+                                              double& fmax2) {
+  double v2 = (vx * vx + vy * vy) + vz * vz;
+  double reach = sqrt(v2) * tau + P.guard + 1e-6 + shift;
+  if ((double)dq[i] * (1.0 / 32.0) <= reach) atomicOr(&bits[i >> 5], 1u << (i & 31));
+  else fmax2 = fmax(fmax2, v2);
+}
+// a block's far max |v|^2 into *dst (every thread calls it)
+__device__ __forceinline__ void far_max_publish(unsigned long long* dst, double fmax2) {
+  fmax2 = wave_max(fmax2);
+  __shared__ double red[16];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = fmax2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmax(m, red[w]);
+    atomicMax(dst, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// The cell's centre-of-mass velocity from its n members idx[0], idx[istride],
+// ... (ascending particle index) summed in that order (mpc_rotation_mod.f95:
+// 153-182), its random axis (:184-193), and its record at tour position t;
+// r6 receives the velocity and the axis.
+__device__ __forceinline__ void cell_record(const CoP& P, const CoD& D, long long t, int c, const int* idx,
+                                            int istride, int n, double* r6) {
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  int j = 0;
+  for (; j + 4 <= n; j += 4) {   // loads of four members in flight, sums in order
+    int i0 = idx[j * istride], i1 = idx[(j + 1) * istride], i2 = idx[(j + 2) * istride], i3 = idx[(j + 3) * istride];
+    double ax = D.vx[i0], bx = D.vx[i1], cx = D.vx[i2], dx = D.vx[i3];
+    double ay = D.vy[i0], by = D.vy[i1], cy = D.vy[i2], dy = D.vy[i3];
+    double az = D.vz[i0], bz = D.vz[i1], cz = D.vz[i2], dz = D.vz[i3];
+    sx = sx + ax; sx = sx + bx; sx = sx + cx; sx = sx + dx;
+    sy = sy + ay; sy = sy + by; sy = sy + cy; sy = sy + dy;
+    sz = sz + az; sz = sz + bz; sz = sz + cz; sz = sz + dz;
+  }
+  for (; j < n; j++) {
+    int i = idx[j * istride];
+    sx = sx + D.vx[i];
+    sy = sy + D.vy[i];
+    sz = sz + D.vz[i];
+  }
+  if (n != 0) {
+    double tn = (double)n;
+    sx = sx / tn; sy = sy / tn; sz = sz / tn;
+  }
+  double u1, u2;
+  if (P.rng_mode == 0) {
+    u1 = D.U[3 + 2 * (long long)c];
+    u2 = D.U[4 + 2 * (long long)c];
+  } else {
+    philox_uniform2(P.seed, 1, D.sc->coll, (unsigned long long)c, u1, u2);
+  }
+  double phi = P.twopi * u1;
+  double th = 2.0 * u2 - 1.0;
+  double st = sqrt(1.0 - th * th);
+  double2* rec = (double2*)(D.cellrec + t * 8);
+  rec[0] = make_double2(sx, sy);
+  double sp, cp;
+  sincos(phi, &sp, &cp);   // one reduction; the same values as sin() and cos()
+  rec[1] = make_double2(sz, st * cp);
+  rec[2] = make_double2(st * sp, th);
+  rec[3] = make_double2((double)n, 0.0);
+  r6[0] = sx; r6[1] = sy; r6[2] = sz; r6[3] = st * cp; r6[4] = st * sp; r6[5] = th;
+}
+// the census limit and the bulk B->A reaction of unshifted cell t
+// (reaction_mod.f95:41-62)
+__device__ __forceinline__ void cell_react(const CoP& P, const CoD& D, long long t) {
+  if (t < P.vol)   // the census totals of the binning kernel against the reference's MaxA/MaxB/MaxS
+    if (D.tcount[t] > CENSUS_MAX || D.tcount[P.vol + t] > CENSUS_MAX || D.tcount[2 * P.vol + t] > CENSUS_MAX)
+      raise_error(&D.sc->err, ERR_MEMBERS);
+  if (t < P.rxnnumcell) {
+    int nb = D.tcount[1 * P.vol + t];
+    double prob = D.ptab[min(nb, CENSUS_MAX)];
+    double r;
+    if (P.rng_mode == 0) r = D.U[3 + 2 * P.mpcnumcell + t];
+    else { double q; philox_uniform2(P.seed, 2, D.sc->coll, (unsigned long long)t, r, q); }
+    if (0.0 < r && r <= prob) {
+      unsigned key = D.firstB[t];
+      int i = key >= (unsigned)P.NT ? (int)(key - (unsigned)P.NT) : (int)key;
+      D.flag[i] = 0;
+    }
+  }
 }
 
 // Per collision cell, in tour order (thread t <-> t-th cell of the tour, so a
@@ -2771,60 +2945,10 @@ __device__ __forceinline__ void collide_body(const CoP& P, const CoD& D, long lo
         idx[k] = v;
       }
     }
-    double sx = 0.0, sy = 0.0, sz = 0.0;
-    int j = 0;
-    for (; j + 4 <= n; j += 4) {   // loads of four members in flight, sums in order
-      int i0 = idx[j * istride], i1 = idx[(j + 1) * istride], i2 = idx[(j + 2) * istride], i3 = idx[(j + 3) * istride];
-      double ax = D.vx[i0], bx = D.vx[i1], cx = D.vx[i2], dx = D.vx[i3];
-      double ay = D.vy[i0], by = D.vy[i1], cy = D.vy[i2], dy = D.vy[i3];
-      double az = D.vz[i0], bz = D.vz[i1], cz = D.vz[i2], dz = D.vz[i3];
-      sx = sx + ax; sx = sx + bx; sx = sx + cx; sx = sx + dx;
-      sy = sy + ay; sy = sy + by; sy = sy + cy; sy = sy + dy;
-      sz = sz + az; sz = sz + bz; sz = sz + cz; sz = sz + dz;
-    }
-    for (; j < n; j++) {
-      int i = idx[j * istride];
-      sx = sx + D.vx[i];
-      sy = sy + D.vy[i];
-      sz = sz + D.vz[i];
-    }
-    if (n != 0) {
-      double tn = (double)n;
-      sx = sx / tn; sy = sy / tn; sz = sz / tn;
-    }
-    double u1, u2;
-    if (P.rng_mode == 0) {
-      u1 = D.U[3 + 2 * (long long)c];
-      u2 = D.U[4 + 2 * (long long)c];
-    } else {
-      philox_uniform2(P.seed, 1, D.sc->coll, (unsigned long long)c, u1, u2);
-    }
-    double phi = P.twopi * u1;
-    double th = 2.0 * u2 - 1.0;
-    double st = sqrt(1.0 - th * th);
-    double2* rec = (double2*)(D.cellrec + t * 8);
-    rec[0] = make_double2(sx, sy);
-    double sp, cp;
-    sincos(phi, &sp, &cp);   // one reduction; the same values as sin() and cos()
-    rec[1] = make_double2(sz, st * cp);
-    rec[2] = make_double2(st * sp, th);
-    rec[3] = make_double2((double)n, 0.0);
+    double r6[6];
+    cell_record(P, D, t, c, idx, istride, n, r6);
   }
-  if (t < P.vol)   // the census totals of the binning kernel against the reference's MaxA/MaxB/MaxS
-    if (D.tcount[t] > CENSUS_MAX || D.tcount[P.vol + t] > CENSUS_MAX || D.tcount[2 * P.vol + t] > CENSUS_MAX)
-      raise_error(&D.sc->err, ERR_MEMBERS);
-  if (t < P.rxnnumcell) {
-    int nb = D.tcount[1 * P.vol + t];
-    double prob = D.ptab[min(nb, CENSUS_MAX)];
-    double r;
-    if (P.rng_mode == 0) r = D.U[3 + 2 * P.mpcnumcell + t];
-    else { double q; philox_uniform2(P.seed, 2, D.sc->coll, (unsigned long long)t, r, q); }
-    if (0.0 < r && r <= prob) {
-      unsigned key = D.firstB[t];
-      int i = key >= (unsigned)P.NT ? (int)(key - (unsigned)P.NT) : (int)key;
-      D.flag[i] = 0;
-    }
-  }
+  cell_react(P, D, t);
 }
 
 __global__ __launch_bounds__(CT) void co_collide(CoP P, CoD D) {
@@ -2881,30 +3005,12 @@ __device__ __forceinline__ void rotate_body(const CoP& P, const CoD& D, int clas
     int t = D.cell_of[i];
     const double2* rec = (const double2*)(D.cellrec + (long long)t * 8);
     double2 r0 = rec[0], r1 = rec[1], r2 = rec[2];
-    double cx = r0.x, cy = r0.y, cz = r1.x, ux = r1.y, uy = r2.x, uz = r2.y;
     double vx = D.vx[i], vy = D.vy[i], vz = D.vz[i];
-    double dx = vx - cx, dy = vy - cy, dz = vz - cz;
-    const double cr = P.cosrot, sr = P.sinrot, omc = P.omc;
-    double rx = (dx * (cr + (ux * ux) * omc) + dy * ((ux * uy) * omc - uz * sr)) + dz * ((ux * uz) * omc + uy * sr);
-    double ry = (dx * ((ux * uy) * omc + uz * sr) + dy * (cr + (uy * uy) * omc)) + dz * ((uy * uz) * omc - ux * sr);
-    double rz = (dx * ((ux * uz) * omc - uy * sr) + dy * ((uy * uz) * omc + ux * sr)) + dz * (cr + (uz * uz) * omc);
-    vx = cx + rx; vy = cy + ry; vz = cz + rz;
+    rotate_velocity(P, r0.x, r0.y, r1.x, r1.y, r2.x, r2.y, vx, vy, vz);
     D.vx[i] = vx; D.vy[i] = vy; D.vz[i] = vz;
-    if (!classify) continue;
-    double v2 = (vx * vx + vy * vy) + vz * vz;
-    double reach = sqrt(v2) * tau + P.guard + 1e-6 + shift;
-    if ((double)D.dq[i] * (1.0 / 32.0) <= reach) atomicOr(&D.near_bits[i >> 5], 1u << (i & 31));
-    else fmax2 = fmax(fmax2, v2);
+    if (classify) classify_next(P, D.near_bits, D.dq, (int)i, vx, vy, vz, tau, shift, fmax2);
   }
-  if (!classify) return;
-  fmax2 = wave_max(fmax2);
-  __shared__ double red[4];
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = fmax2;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-    atomicMax(&D.sc->far_maxv2, (unsigned long long)__double_as_longlong(m));
-  }
+  if (classify) far_max_publish(&D.sc->far_maxv2, fmax2);
 }
 __global__ __launch_bounds__(256) void co_rotate(CoP P, CoD D, int classify) {
   rotate_body(P, D, classify, (long long)blockIdx.x * blockDim.x, P.NT, (long long)gridDim.x * blockDim.x);
@@ -2914,6 +3020,131 @@ __global__ __launch_bounds__(256) void co_rotate_b(const CoP* Ps, const CoD* Ds,
   const CoP P = Ps[t.r];
   const CoD D = Ds[t.r];
   rotate_body(P, D, 1, t.begin, t.end, 256);
+}
+
+// ------------------------------------------------- collisions in two passes
+// In an interval without a re-sort the cells are collided in two passes.
+// Beside the near engine: every cell no near particle can end in (co_far has
+// binned all its members by then), and the rotation of their particles.
+// After it: the late cells -- those within reach of the near set (a ball
+// around each sphere's start position) and those a wall-reaching far particle
+// may end in -- collided and rotated together.  Every cell keeps one slab of
+// BMAX slots (no prefix scan), and a cell is late when its stamp holds the
+// interval's epoch.  A near or deferred particle outside the late cells is
+// reported (ERR_ZONE) rather than collided twice.
+
+// the late cells within ZONE_PAD beyond the larger list radius of sphere
+// blockIdx.x (its start position), by their centres in particle coordinates
+__global__ __launch_bounds__(256) void co_zone_mark(CoP P, CoD D) {
+  const int s = blockIdx.x;
+  if (s >= 2 * P.nd) return;
+  double xs, ys, zs;
+  collision_shift(P, D, xs, ys, zs);
+  const double R = fmax(P.full[0], P.full[1]) + ZONE_PAD;
+  const int r = (int)ceil(R) + 1, w = 2 * r + 1;
+  const double sx = D.sc->xs0[s], sy = D.sc->ys0[s], sz = D.sc->zs0[s];
+  const double zlow = zs < 0.0 ? 1.0 : 0.0;   // cell iz holds Z + zlow in [iz, iz + 1)
+  const int cx0 = (int)floor(sx + xs), cy0 = (int)floor(sy + ys), cz0 = (int)floor(sz + zs + zlow);
+  for (int k = threadIdx.x; k < w * w * w; k += blockDim.x) {
+    const int cx = cx0 + k % w - r, cy = cy0 + (k / w) % w - r, cz = cz0 + k / (w * w) - r;
+    if (cz < 0 || cz > P.Lz) continue;
+    const double a = pbc_half((cx + 0.5 - xs) - sx, P.box[0]);
+    const double b = pbc_half((cy + 0.5 - ys) - sy, P.box[1]);
+    const double c = (cz + 0.5 - zlow - zs) - sz;
+    if (a * a + b * b + c * c > R * R) continue;
+    const int ix = ((cx % P.Lx) + P.Lx) % P.Lx, iy = ((cy % P.Ly) + P.Ly) % P.Ly;
+    late_cell(D, D.mpc_pos[(long long)ix + (long long)P.Lx * iy + (long long)P.Lx * P.Ly * cz]);
+  }
+}
+// one cell of the slab layout: members sorted by particle index (in LDS,
+// strided per thread, or in place in the slab), then cell_record
+__device__ __forceinline__ const int* slab_cell(const CoP& P, const CoD& D, int t, int* sidx, int& istride, int& n,
+                                                double* r6) {
+  n = min(D.cnt[t], BMAX);   // ERR_BUCKET already raised beyond
+  int* b = D.slab + (long long)t * BMAX;
+  int* idx;
+  if (n <= LB) {
+    idx = sidx + threadIdx.x;
+    istride = blockDim.x;
+    for (int j = 0; j < n; j++) {
+      int v = b[j], k = j;
+      while (k > 0 && idx[(k - 1) * istride] > v) { idx[k * istride] = idx[(k - 1) * istride]; k--; }
+      idx[k * istride] = v;
+    }
+  } else {
+    idx = b;
+    istride = 1;
+    for (int j = 1; j < n; j++) {
+      int v = idx[j], k = j;
+      while (k > 0 && idx[k - 1] > v) { idx[k] = idx[k - 1]; k--; }
+      idx[k] = v;
+    }
+  }
+  cell_record(P, D, t, D.mpc_cell[t], idx, istride, n, r6);
+  return idx;
+}
+// first pass, beside the near engine: the cells that are not late
+__global__ __launch_bounds__(CT) void co_collide_far(CoP P, CoD D) {
+  __shared__ int sidx[LB * CT];
+  const long long t = (long long)xcd_block(blockIdx.x, gridDim.x) * CT + threadIdx.x;
+  if (t < P.mpcnumcell && D.stamp[t] != D.epoch) {
+    int istride, n;
+    double r6[6];
+    (void)slab_cell(P, D, (int)t, sidx, istride, n, r6);
+  }
+}
+// first pass: the particles of those cells rotated (coalesced), classified
+// into the next bitmap with the largest displacement the guard allows
+__global__ __launch_bounds__(256) void co_rotate_far(CoP P, CoD D) {
+  double fmax2 = 0.0;
+  const double tau = P.tau * (1.0 + 1e-9);
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < P.NT; i += (long long)gridDim.x * 256) {
+    if ((D.near_bits[i >> 5] >> (i & 31)) & 1u) continue;   // the near set is binned after the near engine
+    const int t = D.cell_of[i];
+    if (D.stamp[t] == D.epoch) continue;   // late cell: the second pass
+    const double2* rec = (const double2*)(D.cellrec + (long long)t * 8);
+    double2 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+    double vx = D.vx[i], vy = D.vy[i], vz = D.vz[i];
+    rotate_velocity(P, r0.x, r0.y, r1.x, r1.y, r2.x, r2.y, vx, vy, vz);
+    D.vx[i] = vx; D.vy[i] = vy; D.vz[i] = vz;
+    classify_next(P, D.next_bits, D.dq, (int)i, vx, vy, vz, tau, P.guard, fmax2);
+  }
+  far_max_publish(&D.sc->next_fmax2, fmax2);
+}
+// second pass, after the near engine: the late cells collided and their
+// members rotated and classified (the spheres' actual displacement)
+__global__ __launch_bounds__(CT) void co_collide_late(CoP P, CoD D) {
+  __shared__ int sidx[LB * CT];
+  const double tau = P.tau * (1.0 + 1e-9), shift = start_shift(P, D);
+  const long long nl = min(D.sc->n_late, D.late_cap);
+  double fmax2 = 0.0;
+  for (long long base = (long long)blockIdx.x * CT; base < nl; base += (long long)gridDim.x * CT) {
+    const long long k = base + threadIdx.x;
+    if (k >= nl) continue;
+    const int t = D.late[k];
+    int istride, n;
+    double r6[6];
+    const int* idx = slab_cell(P, D, t, sidx, istride, n, r6);
+    for (int j = 0; j < n; j++) {
+      const int i = idx[j * istride];
+      double vx = D.vx[i], vy = D.vy[i], vz = D.vz[i];
+      rotate_velocity(P, r6[0], r6[1], r6[2], r6[3], r6[4], r6[5], vx, vy, vz);
+      D.vx[i] = vx; D.vy[i] = vy; D.vz[i] = vz;
+      classify_next(P, D.next_bits, D.dq, i, vx, vy, vz, tau, shift, fmax2);
+    }
+  }
+  far_max_publish(&D.sc->next_fmax2, fmax2);
+}
+// after both passes: census limits and bulk reactions of every unshifted
+// cell (their census is complete), and the next far max handed over
+__global__ __launch_bounds__(256) void co_react(CoP P, CoD D) {
+  const long long nt = max(P.vol, P.rxnnumcell);
+  for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < nt; t += (long long)gridDim.x * 256)
+    cell_react(P, D, t);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    D.sc->far_maxv2 = D.sc->next_fmax2;
+    D.sc->next_fmax2 = 0ull;
+  }
 }
 
 // Near set from particle positions (set-up, before any collision).
@@ -2951,13 +3182,17 @@ __global__ __launch_bounds__(256) void co_classify_all(CoP P, CoD D) {
 
 // interval start: collision counter, cleared cell counts, census and first-B
 // keys, the spheres' start positions kept for co_far
-__global__ __launch_bounds__(256) void interval_begin(CoP P, CoD D, unsigned long long coll) {
+// (clear, nclear: the next interval's bitmap of the two-pass collisions)
+__global__ __launch_bounds__(256) void interval_begin(CoP P, CoD D, unsigned long long coll, unsigned* clear,
+                                                      long long nclear) {
   const long long mc = P.mpcnumcell, vol = P.vol;
   const long long i0 = (long long)blockIdx.x * 256 + threadIdx.x, st = (long long)gridDim.x * 256;
   for (long long i = i0; i < mc; i += st) D.cnt[i] = 0;
   for (long long i = i0; i < 3 * vol; i += st) D.tcount[i] = 0;
   for (long long i = i0; i < vol; i += st) D.firstB[i] = 0xffffffffu;
-  if (i0 == 0) { D.sc->coll = coll; D.sc->n_wall = 0; }
+  if (clear)
+    for (long long i = i0; i < nclear; i += st) clear[i] = 0u;
+  if (i0 == 0) { D.sc->coll = coll; D.sc->n_wall = 0; D.sc->n_late = 0; }
   if (i0 < 2 * P.nd) { D.sc->xs0[i0] = D.sc->x0[i0]; D.sc->ys0[i0] = D.sc->y0[i0]; D.sc->zs0[i0] = D.sc->z0[i0]; }
 }
 // before the rotation: cleared near bitmap (nwords + 1 words) and far max v^2
@@ -3527,6 +3762,7 @@ struct mpcx_co {
   hipStream_t st2 = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
   bool overlap = true;
+  int epoch = 0;   // intervals collided in two passes (the cells' late stamps)
   double run_ms = 0, bulk_ms = 0, near_ms = 0;
   // timed kernels, timed intervals, and the sampling: every TSTRIDE-th interval
   // carries the kernel-timing events (each event costs the stream a few us)
@@ -3598,8 +3834,10 @@ static int check_device_error(mpcx_co* e) {
                                 "a neighbour-list member was carried to a cluster without its dimer",
                                 "no place outside the spheres found for a solvent particle",
                                 "the rebuild events of an interval do not add up to the interval",
-                                "more far particles reach a wall in one interval than the deferred list holds"};
-    std::string m = std::string("device check failed: ") + (err > 0 && err < 19 ? msg[err] : "unknown");
+                                "more far particles reach a wall in one interval than the deferred list holds",
+                                "a near or deferred particle ended in a cell collided before the near engine finished",
+                                "more late collision cells than their list holds"};
+    std::string m = std::string("device check failed: ") + (err > 0 && err < 21 ? msg[err] : "unknown");
     int zero = 0;
     (void)hipMemcpy(&e->D.sc->err, &zero, sizeof(int), hipMemcpyHostToDevice);
     return set_error(MPCX_E_DEVICE, m.c_str());
@@ -3766,6 +4004,13 @@ extern "C" int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg) {
   }
   e->nwords = (NT + 31) / 32;
   D.near_bits = dalloc<unsigned>(e, e->nwords + 1);
+  D.next_bits = dalloc<unsigned>(e, e->nwords + 1);
+  // two-pass collisions: a slab of BMAX slots per collision cell, the cells'
+  // late stamps and the list of late cells (each cell at most once)
+  D.slab = dalloc<int>(e, (size_t)P.mpcnumcell * BMAX);
+  D.stamp = dalloc<int>(e, (size_t)P.mpcnumcell);
+  D.late = dalloc<int>(e, (size_t)P.mpcnumcell);
+  D.late_cap = (int)P.mpcnumcell;
   D.status = dalloc<unsigned char>(e, NT);
   D.in_list = dalloc<int>(e, D.in_cap);
   D.in_plist = dalloc<int>(e, D.in_cap);
@@ -3826,6 +4071,8 @@ extern "C" int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg) {
   HIPCHK(hipMemset(D.sc, 0, sizeof(CoScalars)));
   HIPCHK(hipMemset(D.status, 0, NT));
   HIPCHK(hipMemset(D.near_bits, 0, (e->nwords + 1) * 4));
+  HIPCHK(hipMemset(D.next_bits, 0, (e->nwords + 1) * 4));
+  HIPCHK(hipMemset(D.stamp, 0, (size_t)P.mpcnumcell * 4));
   HIPCHK(hipMemset(D.in_fp, 0, (size_t)MAXM * 3 * D.in_cap * 8));
   HIPCHK(hipMemset(D.in_pot, 0, (size_t)MAXM * D.in_cap * 8));
   HIPCHK(hipMemset(D.in_msd, 0, (size_t)D.in_cap * 8));
@@ -4324,8 +4571,13 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
   for (int64_t c = 0; c < ncoll; c++) {
     const int cnext = cstep + P.MPCtstep;
     const bool sort_due = (cnext % P.sort_step) == 0;
+    // FP64 with the far flight beside the one-workgroup near engine: cells
+    // collided in two passes (a re-sort interval keeps one pass after the sort)
+    const bool two = e->overlap && !e->split && e->near_mode == 0 && !sort_due;
+    if (two) D.epoch = ++e->epoch;
     hipLaunchKernelGGL(interval_begin, dim3(grid_for(3 * vol, 256, 2048)), dim3(256), 0, st, P, D,
-                       (unsigned long long)(cnext / P.MPCtstep));
+                       (unsigned long long)(cnext / P.MPCtstep), two ? D.next_bits : nullptr,
+                       (long long)e->nwords + 1);
     if (P.rng_mode == 0)
       HIPCHK(hipMemcpyAsync(e->d_U, draws + c * e->ndraw, e->ndraw * 8, hipMemcpyHostToDevice, st));
     const long long slot = c / mpcx_co::TSTRIDE;
@@ -4341,13 +4593,23 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
     // the far particles' flight and binning beside the near engine
     const bool ov = e->overlap && e->near_mode == 0;
     if (ov) {
+      hipStream_t s2 = e->st2;
       HIPCHK(hipEventRecord(e->fork, st));
-      HIPCHK(hipStreamWaitEvent(e->st2, e->fork, 0));
-      if (!sort_due) HIPCHK(tmark(1, 0, e->st2));
-      if (e->split) hipLaunchKernelGGL(sp_far, dim3(e->g_spbulk), dim3(256), 0, e->st2, P, D, e->S, sort_due ? 0 : 1);
-      else hipLaunchKernelGGL(co_far, dim3(e->g_bulk), dim3(256), 0, e->st2, P, D, sort_due ? 0 : 1);
-      if (!sort_due) HIPCHK(tmark(1, 1, e->st2));
-      HIPCHK(hipEventRecord(e->join, e->st2));
+      HIPCHK(hipStreamWaitEvent(s2, e->fork, 0));
+      if (two) hipLaunchKernelGGL(co_zone_mark, dim3(2 * e->nd), dim3(256), 0, s2, P, D);
+      if (!sort_due) HIPCHK(tmark(1, 0, s2));
+      if (e->split) hipLaunchKernelGGL(sp_far, dim3(e->g_spbulk), dim3(256), 0, s2, P, D, e->S, sort_due ? 0 : 1);
+      else hipLaunchKernelGGL(co_far, dim3(e->g_bulk), dim3(256), 0, s2, P, D, sort_due ? 0 : 1, two ? 1 : 0);
+      if (!sort_due) HIPCHK(tmark(1, 1, s2));
+      if (two) {   // the first collision pass, beside the near engine too
+        HIPCHK(tmark(2, 0, s2));
+        hipLaunchKernelGGL(co_collide_far, dim3((unsigned)((mc + CT - 1) / CT)), dim3(CT), 0, s2, P, D);
+        HIPCHK(tmark(2, 1, s2));
+        HIPCHK(tmark(3, 0, s2));
+        hipLaunchKernelGGL(co_rotate_far, dim3(gN), dim3(256), 0, s2, P, D);
+        HIPCHK(tmark(3, 1, s2));
+      }
+      HIPCHK(hipEventRecord(e->join, s2));
     }
     if (last) HIPCHK(hipEventRecord(e->n0, st));
     if (e->split) sp_convert(e, true, true);
@@ -4364,7 +4626,7 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
     };
     if (sort_due) {
       if (ov && sp) hipLaunchKernelGGL(sp_bin_near, dim3(1), dim3(256), 0, st, P, D, e->S, 0);   // the flight check
-      else if (ov) hipLaunchKernelGGL(co_bin_rest, dim3(128), dim3(256), 0, st, P, D, 0);
+      else if (ov) hipLaunchKernelGGL(co_bin_rest, dim3(128), dim3(256), 0, st, P, D, 0, 0);
       else bulk(1, 0);
       int r = sp ? sp_sort(e, st) : co_sort(e, st);
       if (r) return r;
@@ -4374,29 +4636,44 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
       if (sp) sp_convert(e, false, true);
       if (last) HIPCHK(hipEventRecord(e->b0, st));
       bulk(0, 1);
+    } else if (two) {   // the second collision pass: late cells, then reactions
+      if (last) HIPCHK(hipEventRecord(e->b0, st));
+      hipLaunchKernelGGL(co_bin_rest, dim3(128), dim3(256), 0, st, P, D, 1, 1);
+      hipLaunchKernelGGL(co_collide_late, dim3(64), dim3(CT), 0, st, P, D);
+      hipLaunchKernelGGL(co_react, dim3(grid_for(vol, 256, 2048)), dim3(256), 0, st, P, D);
+      if (last) HIPCHK(hipEventRecord(e->b1, st));
+      std::swap(D.near_bits, D.next_bits);   // the next interval's near set
+      co_near_list(e);
+      cstep = cnext;
+      continue;
     } else if (ov) {
       if (last) HIPCHK(hipEventRecord(e->b0, st));
       if (sp) hipLaunchKernelGGL(sp_bin_near, dim3(64), dim3(256), 0, st, P, D, e->S, 1);
-      else hipLaunchKernelGGL(co_bin_rest, dim3(128), dim3(256), 0, st, P, D, 1);
+      else hipLaunchKernelGGL(co_bin_rest, dim3(128), dim3(256), 0, st, P, D, 1, 0);
     } else {
       if (last) HIPCHK(hipEventRecord(e->b0, st));
       HIPCHK(tmark(1, 0, st));
       bulk(1, 1);
       HIPCHK(tmark(1, 1, st));
     }
+    // with two-pass intervals the timed collide and rotate are their first
+    // passes; the re-sort intervals' single pass is left out of the means
+    auto tmark1 = [&](int k, int edge) -> hipError_t {
+      return (e->overlap && !e->split && e->near_mode == 0) ? hipSuccess : tmark(k, edge, st);
+    };
     launch_scan(st, D.cnt, mc, D.partial, D.off);
-    HIPCHK(tmark(4, 0, st));
+    HIPCHK(tmark1(4, 0));
     hipLaunchKernelGGL(co_scatter, dim3(gN), dim3(256), 0, st, P, D);
-    HIPCHK(tmark(4, 1, st));
-    HIPCHK(tmark(2, 0, st));
+    HIPCHK(tmark1(4, 1));
+    HIPCHK(tmark1(2, 0));
     if (sp) hipLaunchKernelGGL(sp_collide, dim3((unsigned)((mc + CT - 1) / CT)), dim3(CT), 0, st, P, D, e->S);
     else hipLaunchKernelGGL(co_collide, dim3((unsigned)((mc + CT - 1) / CT)), dim3(CT), 0, st, P, D);
-    HIPCHK(tmark(2, 1, st));
+    HIPCHK(tmark1(2, 1));
     hipLaunchKernelGGL(rotate_begin, dim3(grid_for(e->nwords + 1, 256, 2048)), dim3(256), 0, st, D, e->nwords);
-    HIPCHK(tmark(3, 0, st));
+    HIPCHK(tmark1(3, 0));
     if (sp) hipLaunchKernelGGL(sp_rotate, dim3(gN), dim3(256), 0, st, P, D, e->S, ov && !sort_due ? 2 : 1);
     else hipLaunchKernelGGL(co_rotate, dim3(gN), dim3(256), 0, st, P, D, ov && !sort_due ? 2 : 1);
-    HIPCHK(tmark(3, 1, st));
+    HIPCHK(tmark1(3, 1));
     if (last) HIPCHK(hipEventRecord(e->b1, st));
     co_near_list(e);
     cstep = cnext;
@@ -4795,7 +5072,7 @@ extern "C" int mpcx_co_get_cells(mpcx_co* e, double* vcm_uv, int32_t* cell_of, i
 extern "C" int mpcx_co_near_profile(mpcx_co* e, uint64_t* prof8) {
   if (!e || !prof8) return set_error(MPCX_E_ARG, "null argument");
   HIPCHK(hipStreamSynchronize(e->st));
-  HIPCHK(hipMemcpy(prof8, e->D.sc->prof, 64, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(prof8, e->D.sc->prof, 10 * sizeof(uint64_t), hipMemcpyDeviceToHost));
   return 0;
 }
 

@@ -32,6 +32,8 @@ enum DevError : int {
   ERR_INIT = 16,           // no place outside the spheres found for a solvent particle
   ERR_FLIGHT = 17,         // the rebuild events of an interval do not add up to the interval
   ERR_WALL_CAP = 18,       // more far particles reach a wall in one interval than the deferred list holds
+  ERR_ZONE = 19,           // a near or deferred particle ended in a cell collided before the near engine finished
+  ERR_ZONE_CAP = 20,       // more late collision cells than their list holds
 };
 
 __device__ __forceinline__ void raise_error(int* err, int code) { atomicCAS(err, 0, code); }
