@@ -2698,18 +2698,33 @@ __device__ __forceinline__ void bulk_body(const CoP& P, const CoD& D, int do_str
         y = __builtin_nontemporal_load(&D.y[i]);
         z = __builtin_nontemporal_load(&D.z[i]);
         f = D.flag[i];
-        const double dx = __builtin_nontemporal_load(&D.vx[i]) * P.MDt;
-        const double dy = __builtin_nontemporal_load(&D.vy[i]) * P.MDt;
-        const double dz = __builtin_nontemporal_load(&D.vz[i]) * P.MDt;
+        double vx = __builtin_nontemporal_load(&D.vx[i]), vy = __builtin_nontemporal_load(&D.vy[i]);
+        double vz = __builtin_nontemporal_load(&D.vz[i]);
+        const double dx = vx * P.MDt, dy = vy * P.MDt, dz = vz * P.MDt;
         const double zt = z + dz * flight;   // the flight's z, as stream_flight computes it
-        wall = !(zt > WALL_MARGIN && zt < ztop);
+        // The bounce time z/vz (or (z - Lz)/vz) turns the flights' rounding
+        // difference dz in z into 2 |v_xy| dz / |vz| in x and y.  At z = 0 the
+        // ulps are below 1e-16, so a bounce is taken here unless it is
+        // shallower than |vz| = |v_xy| / 100 (then below 1e-13); at the top
+        // wall the ulps of Lz are not, so every crossing there is deferred.
+        // A flight ending within WALL_MARGIN of z = 0 is deferred as well: the
+        // replay might end on the other side.
+        const bool top = !(zt < ztop), clear = zt > WALL_MARGIN;
+        const bool bottom = zt <= -WALL_MARGIN && fabs(vz) >= 0.01 * fmax(fabs(vx), fabs(vy));
+        wall = top || !(clear || bottom);
         if (!wall) {
           stream_flight(x, y, z, dx, dy, dz, flight, P);
+          if (bottom) {
+            const int f0 = f;
+            (void)stream_walls(x, y, z, vx, vy, vz, f, P, D.react_box, &D.sc->err);
+            D.vx[i] = vx; D.vy[i] = vy; D.vz[i] = vz;
+            if (f != f0) D.flag[i] = f;
+          }
           __builtin_nontemporal_store(x, &D.x[i]);
           __builtin_nontemporal_store(y, &D.y[i]);
           __builtin_nontemporal_store(z, &D.z[i]);
         } else if (slab && do_bin) {   // the first pass must leave its possible end cells alone
-          const int t = late_cells_of(P, D, x, y, z, D.vx[i], D.vy[i], D.vz[i], xs, ys, zs);
+          const int t = late_cells_of(P, D, x, y, z, vx, vy, vz, xs, ys, zs);
           D.cell_of[i] = t >= 0 ? t : 0;   // co_rotate_far skips it there (a late cell)
           if (t < 0) raise_error(&D.sc->err, ERR_CELL_RANGE);
         }
@@ -3034,9 +3049,10 @@ __global__ __launch_bounds__(256) void co_rotate_b(const CoP* Ps, const CoD* Ds,
 // reported (ERR_ZONE) rather than collided twice.
 
 // the late cells within ZONE_PAD beyond the larger list radius of sphere
-// blockIdx.x (its start position), by their centres in particle coordinates
+// blockIdx.y (its start position), by their centres in particle coordinates;
+// the candidate cube is dealt over the blockIdx.x workgroups
 __global__ __launch_bounds__(256) void co_zone_mark(CoP P, CoD D) {
-  const int s = blockIdx.x;
+  const int s = blockIdx.y;
   if (s >= 2 * P.nd) return;
   double xs, ys, zs;
   collision_shift(P, D, xs, ys, zs);
@@ -3045,7 +3061,7 @@ __global__ __launch_bounds__(256) void co_zone_mark(CoP P, CoD D) {
   const double sx = D.sc->xs0[s], sy = D.sc->ys0[s], sz = D.sc->zs0[s];
   const double zlow = zs < 0.0 ? 1.0 : 0.0;   // cell iz holds Z + zlow in [iz, iz + 1)
   const int cx0 = (int)floor(sx + xs), cy0 = (int)floor(sy + ys), cz0 = (int)floor(sz + zs + zlow);
-  for (int k = threadIdx.x; k < w * w * w; k += blockDim.x) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < w * w * w; k += gridDim.x * blockDim.x) {
     const int cx = cx0 + k % w - r, cy = cy0 + (k / w) % w - r, cz = cz0 + k / (w * w) - r;
     if (cz < 0 || cz > P.Lz) continue;
     const double a = pbc_half((cx + 0.5 - xs) - sx, P.box[0]);
@@ -4596,7 +4612,7 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
       hipStream_t s2 = e->st2;
       HIPCHK(hipEventRecord(e->fork, st));
       HIPCHK(hipStreamWaitEvent(s2, e->fork, 0));
-      if (two) hipLaunchKernelGGL(co_zone_mark, dim3(2 * e->nd), dim3(256), 0, s2, P, D);
+      if (two) hipLaunchKernelGGL(co_zone_mark, dim3(64, 2 * e->nd), dim3(256), 0, s2, P, D);
       if (!sort_due) HIPCHK(tmark(1, 0, s2));
       if (e->split) hipLaunchKernelGGL(sp_far, dim3(e->g_spbulk), dim3(256), 0, s2, P, D, e->S, sort_due ? 0 : 1);
       else hipLaunchKernelGGL(co_far, dim3(e->g_bulk), dim3(256), 0, s2, P, D, sort_due ? 0 : 1, two ? 1 : 0);
@@ -4639,7 +4655,7 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
     } else if (two) {   // the second collision pass: late cells, then reactions
       if (last) HIPCHK(hipEventRecord(e->b0, st));
       hipLaunchKernelGGL(co_bin_rest, dim3(128), dim3(256), 0, st, P, D, 1, 1);
-      hipLaunchKernelGGL(co_collide_late, dim3(64), dim3(CT), 0, st, P, D);
+      hipLaunchKernelGGL(co_collide_late, dim3(512), dim3(CT), 0, st, P, D);
       hipLaunchKernelGGL(co_react, dim3(grid_for(vol, 256, 2048)), dim3(256), 0, st, P, D);
       if (last) HIPCHK(hipEventRecord(e->b1, st));
       std::swap(D.near_bits, D.next_bits);   // the next interval's near set
