@@ -3776,7 +3776,7 @@ struct mpcx_co {
   // co_far runs on st2 beside the near engine: st2 waits for fork (cleared
   // counters, draws), st waits for join before binning the near set
   hipStream_t st2 = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr, far_done = nullptr;   // far_done: co_far finished (two passes)
   bool overlap = true;
   int epoch = 0;   // intervals collided in two passes (the cells' late stamps)
   double run_ms = 0, bulk_ms = 0, near_ms = 0;
@@ -4079,6 +4079,7 @@ extern "C" int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg) {
   HIPCHK(hipStreamCreateWithFlags(&e->st2, hipStreamNonBlocking));
   HIPCHK(hipEventCreateWithFlags(&e->fork, hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&e->join, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&e->far_done, hipEventDisableTiming));
   HIPCHK(hipEventCreate(&e->e0)); HIPCHK(hipEventCreate(&e->e1));
   HIPCHK(hipEventCreate(&e->b0)); HIPCHK(hipEventCreate(&e->b1));
   HIPCHK(hipEventCreate(&e->n0)); HIPCHK(hipEventCreate(&e->n1));
@@ -4153,7 +4154,7 @@ extern "C" int mpcx_co_destroy(mpcx_co* e) {
     if (p) (void)hipFree(p);
   if (e->st) (void)hipStreamDestroy(e->st);
   if (e->st2) (void)hipStreamDestroy(e->st2);
-  for (hipEvent_t ev : {e->e0, e->e1, e->b0, e->b1, e->n0, e->n1, e->fork, e->join})
+  for (hipEvent_t ev : {e->e0, e->e1, e->b0, e->b1, e->n0, e->n1, e->fork, e->join, e->far_done})
     if (ev) (void)hipEventDestroy(ev);
   for (hipEvent_t ev : e->tev)
     if (ev) (void)hipEventDestroy(ev);
@@ -4618,6 +4619,7 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
       else hipLaunchKernelGGL(co_far, dim3(e->g_bulk), dim3(256), 0, s2, P, D, sort_due ? 0 : 1, two ? 1 : 0);
       if (!sort_due) HIPCHK(tmark(1, 1, s2));
       if (two) {   // the first collision pass, beside the near engine too
+        HIPCHK(hipEventRecord(e->far_done, s2));
         HIPCHK(tmark(2, 0, s2));
         hipLaunchKernelGGL(co_collide_far, dim3((unsigned)((mc + CT - 1) / CT)), dim3(CT), 0, s2, P, D);
         HIPCHK(tmark(2, 1, s2));
@@ -4634,6 +4636,12 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
     HIPCHK(tmark(0, 1, st));
     if (e->split) sp_convert(e, false, true);
     if (last) HIPCHK(hipEventRecord(e->n1, st));
+    if (two) {   // the second pass needs co_far's binning only, not the first pass
+      if (last) HIPCHK(hipEventRecord(e->b0, st));
+      HIPCHK(hipStreamWaitEvent(st, e->far_done, 0));
+      hipLaunchKernelGGL(co_bin_rest, dim3(128), dim3(256), 0, st, P, D, 1, 1);
+      hipLaunchKernelGGL(co_collide_late, dim3(512), dim3(CT), 0, st, P, D);
+    }
     if (ov) HIPCHK(hipStreamWaitEvent(st, e->join, 0));
     const bool sp = e->split;
     auto bulk = [&](int stream, int bin) {
@@ -4652,10 +4660,7 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
       if (sp) sp_convert(e, false, true);
       if (last) HIPCHK(hipEventRecord(e->b0, st));
       bulk(0, 1);
-    } else if (two) {   // the second collision pass: late cells, then reactions
-      if (last) HIPCHK(hipEventRecord(e->b0, st));
-      hipLaunchKernelGGL(co_bin_rest, dim3(128), dim3(256), 0, st, P, D, 1, 1);
-      hipLaunchKernelGGL(co_collide_late, dim3(512), dim3(CT), 0, st, P, D);
+    } else if (two) {   // after both passes: reactions, the next far max and near set
       hipLaunchKernelGGL(co_react, dim3(grid_for(vol, 256, 2048)), dim3(256), 0, st, P, D);
       if (last) HIPCHK(hipEventRecord(e->b1, st));
       std::swap(D.near_bits, D.next_bits);   // the next interval's near set
