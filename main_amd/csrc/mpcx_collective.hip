@@ -3151,13 +3151,16 @@ __global__ __launch_bounds__(CT) void co_collide_late(CoP P, CoD D) {
   }
   far_max_publish(&D.sc->next_fmax2, fmax2);
 }
-// after both passes: census limits and bulk reactions of every unshifted
-// cell (their census is complete), and the next far max handed over
+// once the near set is binned: census limits and bulk reactions of every
+// unshifted cell (the census is complete; the collision passes read no species)
 __global__ __launch_bounds__(256) void co_react(CoP P, CoD D) {
   const long long nt = max(P.vol, P.rxnnumcell);
   for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < nt; t += (long long)gridDim.x * 256)
     cell_react(P, D, t);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+}
+// the far max of a two-pass interval handed over (after both passes)
+__global__ void co_far_handoff(CoD D) {
+  if (threadIdx.x == 0) {
     D.sc->far_maxv2 = D.sc->next_fmax2;
     D.sc->next_fmax2 = 0ull;
   }
@@ -3198,9 +3201,10 @@ __global__ __launch_bounds__(256) void co_classify_all(CoP P, CoD D) {
 
 // interval start: collision counter, cleared cell counts, census and first-B
 // keys, the spheres' start positions kept for co_far
-// (clear, nclear: the next interval's bitmap of the two-pass collisions)
+// (clear, nclear: the next interval's bitmap of the two-pass collisions;
+// handoff: the previous interval's far max, held in next_fmax2)
 __global__ __launch_bounds__(256) void interval_begin(CoP P, CoD D, unsigned long long coll, unsigned* clear,
-                                                      long long nclear) {
+                                                      long long nclear, int handoff) {
   const long long mc = P.mpcnumcell, vol = P.vol;
   const long long i0 = (long long)blockIdx.x * 256 + threadIdx.x, st = (long long)gridDim.x * 256;
   for (long long i = i0; i < mc; i += st) D.cnt[i] = 0;
@@ -3208,7 +3212,10 @@ __global__ __launch_bounds__(256) void interval_begin(CoP P, CoD D, unsigned lon
   for (long long i = i0; i < vol; i += st) D.firstB[i] = 0xffffffffu;
   if (clear)
     for (long long i = i0; i < nclear; i += st) clear[i] = 0u;
-  if (i0 == 0) { D.sc->coll = coll; D.sc->n_wall = 0; D.sc->n_late = 0; }
+  if (i0 == 0) {
+    D.sc->coll = coll; D.sc->n_wall = 0; D.sc->n_late = 0;
+    if (handoff) { D.sc->far_maxv2 = D.sc->next_fmax2; D.sc->next_fmax2 = 0ull; }
+  }
   if (i0 < 2 * P.nd) { D.sc->xs0[i0] = D.sc->x0[i0]; D.sc->ys0[i0] = D.sc->y0[i0]; D.sc->zs0[i0] = D.sc->z0[i0]; }
 }
 // before the rotation: cleared near bitmap (nwords + 1 words) and far max v^2
@@ -3779,6 +3786,9 @@ struct mpcx_co {
   hipEvent_t fork = nullptr, join = nullptr, far_done = nullptr;   // far_done: co_far finished (two passes)
   bool overlap = true;
   int epoch = 0;   // intervals collided in two passes (the cells' late stamps)
+  // after a two-pass interval: the far max waits in next_fmax2 and the near
+  // list is built at the start of the next interval (beside its co_far)
+  bool handoff = false, list_due = false;
   double run_ms = 0, bulk_ms = 0, near_ms = 0;
   // timed kernels, timed intervals, and the sampling: every TSTRIDE-th interval
   // carries the kernel-timing events (each event costs the stream a few us)
@@ -4514,8 +4524,9 @@ static int co_setup(mpcx_co* e, int32_t do_sort, bool restart) {
   CoScalars sc;
   HIPCHK(hipMemcpy(&sc, D.sc, sizeof sc, hipMemcpyDeviceToHost));
   for (int s = 0; s < 2 * e->nd; s++) { sc.x0[s] = dim[DX * MAXSG + s]; sc.y0[s] = dim[DY * MAXSG + s]; sc.z0[s] = dim[DZ * MAXSG + s]; }
-  sc.far_maxv2 = 0; sc.near_vmax2 = 0.0; sc.n_ev = 0; sc.err = 0;
+  sc.far_maxv2 = 0; sc.near_vmax2 = 0.0; sc.n_ev = 0; sc.err = 0; sc.next_fmax2 = 0;
   HIPCHK(hipMemcpy(D.sc, &sc, sizeof sc, hipMemcpyHostToDevice));
+  e->handoff = e->list_due = false;
   hipLaunchKernelGGL(fill_kernel<unsigned>, dim3(grid_for(e->nwords + 1, 256)), dim3(256), 0, st, D.near_bits,
                      e->nwords + 1, 0u);
   if (do_sort) {
@@ -4594,7 +4605,8 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
     if (two) D.epoch = ++e->epoch;
     hipLaunchKernelGGL(interval_begin, dim3(grid_for(3 * vol, 256, 2048)), dim3(256), 0, st, P, D,
                        (unsigned long long)(cnext / P.MPCtstep), two ? D.next_bits : nullptr,
-                       (long long)e->nwords + 1);
+                       (long long)e->nwords + 1, e->handoff ? 1 : 0);
+    e->handoff = false;
     if (P.rng_mode == 0)
       HIPCHK(hipMemcpyAsync(e->d_U, draws + c * e->ndraw, e->ndraw * 8, hipMemcpyHostToDevice, st));
     const long long slot = c / mpcx_co::TSTRIDE;
@@ -4607,18 +4619,25 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
     };
     if (timed)
       for (int k = 0; k < mpcx_co::NKT; k++) e->tev_on[(size_t)slot * mpcx_co::NKT + k] = 0;
+    // The near list (the set classified by the last two-pass interval) comes
+    // before the fork: the near engine must reach a CU before co_far's
+    // workgroups fill them all, since it needs nearly a whole CU's LDS.
+    if (e->list_due) {
+      co_near_list(e);
+      e->list_due = false;
+    }
     // the far particles' flight and binning beside the near engine
     const bool ov = e->overlap && e->near_mode == 0;
     if (ov) {
       hipStream_t s2 = e->st2;
       HIPCHK(hipEventRecord(e->fork, st));
       HIPCHK(hipStreamWaitEvent(s2, e->fork, 0));
-      if (two) hipLaunchKernelGGL(co_zone_mark, dim3(64, 2 * e->nd), dim3(256), 0, s2, P, D);
       if (!sort_due) HIPCHK(tmark(1, 0, s2));
       if (e->split) hipLaunchKernelGGL(sp_far, dim3(e->g_spbulk), dim3(256), 0, s2, P, D, e->S, sort_due ? 0 : 1);
       else hipLaunchKernelGGL(co_far, dim3(e->g_bulk), dim3(256), 0, s2, P, D, sort_due ? 0 : 1, two ? 1 : 0);
       if (!sort_due) HIPCHK(tmark(1, 1, s2));
-      if (two) {   // the first collision pass, beside the near engine too
+      if (two) {   // the late cells around the spheres; the first collision pass
+        hipLaunchKernelGGL(co_zone_mark, dim3(64, 2 * e->nd), dim3(256), 0, s2, P, D);
         HIPCHK(hipEventRecord(e->far_done, s2));
         HIPCHK(tmark(2, 0, s2));
         hipLaunchKernelGGL(co_collide_far, dim3((unsigned)((mc + CT - 1) / CT)), dim3(CT), 0, s2, P, D);
@@ -4640,6 +4659,7 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
       if (last) HIPCHK(hipEventRecord(e->b0, st));
       HIPCHK(hipStreamWaitEvent(st, e->far_done, 0));
       hipLaunchKernelGGL(co_bin_rest, dim3(128), dim3(256), 0, st, P, D, 1, 1);
+      hipLaunchKernelGGL(co_react, dim3(grid_for(vol, 256, 2048)), dim3(256), 0, st, P, D);
       hipLaunchKernelGGL(co_collide_late, dim3(512), dim3(CT), 0, st, P, D);
     }
     if (ov) HIPCHK(hipStreamWaitEvent(st, e->join, 0));
@@ -4660,11 +4680,11 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
       if (sp) sp_convert(e, false, true);
       if (last) HIPCHK(hipEventRecord(e->b0, st));
       bulk(0, 1);
-    } else if (two) {   // after both passes: reactions, the next far max and near set
-      hipLaunchKernelGGL(co_react, dim3(grid_for(vol, 256, 2048)), dim3(256), 0, st, P, D);
+    } else if (two) {   // after both passes: the next interval's near set (its list and
+                        // far max follow at the next interval's start)
       if (last) HIPCHK(hipEventRecord(e->b1, st));
-      std::swap(D.near_bits, D.next_bits);   // the next interval's near set
-      co_near_list(e);
+      std::swap(D.near_bits, D.next_bits);
+      e->list_due = e->handoff = true;
       cstep = cnext;
       continue;
     } else if (ov) {
@@ -4698,6 +4718,14 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
     if (last) HIPCHK(hipEventRecord(e->b1, st));
     co_near_list(e);
     cstep = cnext;
+  }
+  if (e->list_due) {   // leave the state complete for the other entry points
+    co_near_list(e);
+    e->list_due = false;
+  }
+  if (e->handoff) {
+    hipLaunchKernelGGL(co_far_handoff, dim3(1), dim3(64), 0, st, D);
+    e->handoff = false;
   }
   if (ncoll > 0) {   // kernel-group times of the last interval
     HIPCHK(hipEventSynchronize(e->b1));
