@@ -3072,10 +3072,9 @@ __global__ __launch_bounds__(256) void co_zone_mark(CoP P, CoD D) {
     late_cell(D, D.mpc_pos[(long long)ix + (long long)P.Lx * iy + (long long)P.Lx * P.Ly * cz]);
   }
 }
-// one cell of the slab layout: members sorted by particle index (in LDS,
-// strided per thread, or in place in the slab), then cell_record
-__device__ __forceinline__ const int* slab_cell(const CoP& P, const CoD& D, int t, int* sidx, int& istride, int& n,
-                                                double* r6) {
+// the members of slab cell t sorted by particle index (in LDS, strided per
+// thread, or in place in the slab)
+__device__ __forceinline__ const int* slab_members(const CoD& D, int t, int* sidx, int& istride, int& n) {
   n = min(D.cnt[t], BMAX);   // ERR_BUCKET already raised beyond
   int* b = D.slab + (long long)t * BMAX;
   int* idx;
@@ -3096,6 +3095,12 @@ __device__ __forceinline__ const int* slab_cell(const CoP& P, const CoD& D, int 
       idx[k] = v;
     }
   }
+  return idx;
+}
+// one cell of the slab layout: sorted members, then cell_record
+__device__ __forceinline__ const int* slab_cell(const CoP& P, const CoD& D, int t, int* sidx, int& istride, int& n,
+                                                double* r6) {
+  const int* idx = slab_members(D, t, sidx, istride, n);
   cell_record(P, D, t, D.mpc_cell[t], idx, istride, n, r6);
   return idx;
 }
@@ -3455,7 +3460,7 @@ __device__ __forceinline__ bool sp_stream(int& ix, int& iy, int& iz, float& ox, 
 __device__ __forceinline__ void sp_bin_particle(const CoP& P, const CoD& D, int i, bool valid, bool near,
                                                 unsigned char st, int ix, int iy, int iz, float ox, float oy,
                                                 float oz, int f, float fxs, float fys, float fzs, double zs,
-                                                const double (*sph)[MAXSG]) {
+                                                const double (*sph)[MAXSG], int slab = 0) {
   const int NT = P.NT;
   const int lane = threadIdx.x & 63;
   bool ok = valid;
@@ -3472,7 +3477,11 @@ __device__ __forceinline__ void sp_bin_particle(const CoP& P, const CoD& D, int 
   if (valid) {
     if (ok && r >= BMAX) raise_error(&D.sc->err, ERR_BUCKET);
     D.cell_of[i] = t;
-    D.rank_of[i] = ok ? (unsigned short)min(r, 65535) : (unsigned short)0xffff;
+    if (!slab) D.rank_of[i] = ok ? (unsigned short)min(r, 65535) : (unsigned short)0xffff;
+    else if (ok && r < BMAX) {   // as bin_particle
+      D.slab[(long long)t * BMAX + r] = i;
+      if (slab == 2 && D.stamp[t] != D.epoch) raise_error(&D.sc->err, ERR_ZONE);
+    }
     const double x = (double)ix + (double)ox, y = (double)iy + (double)oy, z = (double)iz + (double)oz;
     double dmin = 1e30;
     bool close = false;
@@ -3539,7 +3548,8 @@ __device__ __forceinline__ void sp_bin_particle(const CoP& P, const CoD& D, int 
 // as co_far): far particles only, the flight of MPCtstep steps, distances to
 // the spheres at the interval's start; sp_bin_near bins the near set after.
 template <bool FAR>
-__device__ __forceinline__ void sp_bulk_body(const CoP& P, const CoD& D, const SpD& S, int do_stream, int do_bin) {
+__device__ __forceinline__ void sp_bulk_body(const CoP& P, const CoD& D, const SpD& S, int do_stream, int do_bin,
+                                             int slab = 0) {
   const int NT = P.NT;
   const int nev = FAR ? 0 : D.sc->n_ev;
   __shared__ double sph[3][MAXSG];
@@ -3586,19 +3596,19 @@ __device__ __forceinline__ void sp_bulk_body(const CoP& P, const CoD& D, const S
       }
     }
     if (!do_bin) continue;
-    sp_bin_particle(P, D, i, mine, near, st, ix, iy, iz, ox, oy, oz, f, fxs, fys, fzs, zs, sph);
+    sp_bin_particle(P, D, i, mine, near, st, ix, iy, iz, ox, oy, oz, f, fxs, fys, fzs, zs, sph, slab);
   }
 }
 __global__ __launch_bounds__(256) void sp_bulk(CoP P, CoD D, SpD S, int do_stream, int do_bin) {
   sp_bulk_body<false>(P, D, S, do_stream, do_bin);
 }
-__global__ __launch_bounds__(256) void sp_far(CoP P, CoD D, SpD S, int do_bin) {
-  sp_bulk_body<true>(P, D, S, 1, do_bin);
+__global__ __launch_bounds__(256) void sp_far(CoP P, CoD D, SpD S, int do_bin, int slab) {
+  sp_bulk_body<true>(P, D, S, 1, do_bin, slab);
 }
 // The near set after the near engine (stored back to split storage): binning
 // with distances to the spheres at the interval's start, as sp_far; and the
 // flight check.
-__global__ __launch_bounds__(256) void sp_bin_near(CoP P, CoD D, SpD S, int do_bin) {
+__global__ __launch_bounds__(256) void sp_bin_near(CoP P, CoD D, SpD S, int do_bin, int slab) {
   __shared__ double sph[3][MAXSG];
   for (int j = threadIdx.x; j < 2 * P.nd; j += blockDim.x) {
     sph[0][j] = D.sc->xs0[j]; sph[1][j] = D.sc->ys0[j]; sph[2][j] = D.sc->zs0[j];
@@ -3621,12 +3631,61 @@ __global__ __launch_bounds__(256) void sp_bin_near(CoP P, CoD D, SpD S, int do_b
       ox = S.ox[i]; oy = S.oy[i]; oz = S.oz[i];
       f = S.flag[i];
     }
-    sp_bin_particle(P, D, i, valid, true, st, ix, iy, iz, ox, oy, oz, f, fxs, fys, fzs, zs, sph);
+    sp_bin_particle(P, D, i, valid, true, st, ix, iy, iz, ox, oy, oz, f, fxs, fys, fzs, zs, sph, slab ? 2 : 0);
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) check_flight(P, D);
 }
 
 // co_collide in split storage: Kahan-compensated FP32 sums in ascending order
+// split storage: the cell's FP32 Kahan-compensated velocity sum over its
+// members in ascending particle order, the axis, and the record (as
+// cell_record); r6 receives the mean velocity and the axis
+__device__ __forceinline__ void sp_cell_record(const CoP& P, const CoD& D, const SpD& S, long long t, int c,
+                                               const int* idx, int istride, int n, double* r6) {
+  float sx = 0.0f, sy = 0.0f, sz = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
+  for (int j = 0; j < n; j++) {
+    const int i = idx[j * istride];
+    const float yx = S.vx[i] - cx, yy = S.vy[i] - cy, yz = S.vz[i] - cz;
+    const float tx = sx + yx, ty = sy + yy, tz = sz + yz;
+    cx = (tx - sx) - yx; cy = (ty - sy) - yy; cz = (tz - sz) - yz;
+    sx = tx; sy = ty; sz = tz;
+  }
+  double mx = 0.0, my = 0.0, mz = 0.0;
+  if (n != 0) {
+    const float inv = 1.0f / (float)n;
+    mx = (double)(sx * inv); my = (double)(sy * inv); mz = (double)(sz * inv);
+  }
+  double u1, u2;
+  philox_uniform2(P.seed, 1, D.sc->coll, (unsigned long long)c, u1, u2);
+  double phi = P.twopi * u1;
+  double th = 2.0 * u2 - 1.0;
+  double st = sqrt(1.0 - th * th);
+  double2* rec = (double2*)(D.cellrec + t * 8);
+  rec[0] = make_double2(mx, my);
+  double sp, cp;
+  sincos(phi, &sp, &cp);   // one reduction; the same values as sin() and cos()
+  rec[1] = make_double2(mz, st * cp);
+  rec[2] = make_double2(st * sp, th);
+  rec[3] = make_double2((double)n, 0.0);
+  r6[0] = mx; r6[1] = my; r6[2] = mz; r6[3] = st * cp; r6[4] = st * sp; r6[5] = th;
+}
+// split storage: census limits and the bulk B->A reaction of unshifted cell t
+__device__ __forceinline__ void sp_cell_react(const CoP& P, const CoD& D, const SpD& S, long long t) {
+  if (t < P.vol)   // the census totals of the binning kernel against the reference's MaxA/MaxB/MaxS
+    if (D.tcount[t] > CENSUS_MAX || D.tcount[P.vol + t] > CENSUS_MAX || D.tcount[2 * P.vol + t] > CENSUS_MAX)
+      raise_error(&D.sc->err, ERR_MEMBERS);
+  if (t < P.rxnnumcell) {
+    int nb = D.tcount[1 * P.vol + t];
+    double prob = D.ptab[min(nb, CENSUS_MAX)];
+    double r, q;
+    philox_uniform2(P.seed, 2, D.sc->coll, (unsigned long long)t, r, q);
+    if (0.0 < r && r <= prob) {
+      unsigned key = D.firstB[t];
+      int i = key >= (unsigned)P.NT ? (int)(key - (unsigned)P.NT) : (int)key;
+      S.flag[i] = 0;
+    }
+  }
+}
 __global__ __launch_bounds__(CT) void sp_collide(CoP P, CoD D, SpD S) {
   __shared__ int sidx[LB * CT];
   const long long t = (long long)xcd_block(blockIdx.x, gridDim.x) * CT + threadIdx.x;
@@ -3654,82 +3713,109 @@ __global__ __launch_bounds__(CT) void sp_collide(CoP P, CoD D, SpD S) {
         idx[k] = v;
       }
     }
-    float sx = 0.0f, sy = 0.0f, sz = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
-    for (int j = 0; j < n; j++) {
-      const int i = idx[j * istride];
-      const float yx = S.vx[i] - cx, yy = S.vy[i] - cy, yz = S.vz[i] - cz;
-      const float tx = sx + yx, ty = sy + yy, tz = sz + yz;
-      cx = (tx - sx) - yx; cy = (ty - sy) - yy; cz = (tz - sz) - yz;
-      sx = tx; sy = ty; sz = tz;
-    }
-    double mx = 0.0, my = 0.0, mz = 0.0;
-    if (n != 0) {
-      const float inv = 1.0f / (float)n;
-      mx = (double)(sx * inv); my = (double)(sy * inv); mz = (double)(sz * inv);
-    }
-    double u1, u2;
-    philox_uniform2(P.seed, 1, D.sc->coll, (unsigned long long)c, u1, u2);
-    double phi = P.twopi * u1;
-    double th = 2.0 * u2 - 1.0;
-    double st = sqrt(1.0 - th * th);
-    double2* rec = (double2*)(D.cellrec + t * 8);
-    rec[0] = make_double2(mx, my);
-    double sp, cp;
-    sincos(phi, &sp, &cp);   // one reduction; the same values as sin() and cos()
-    rec[1] = make_double2(mz, st * cp);
-    rec[2] = make_double2(st * sp, th);
-    rec[3] = make_double2((double)n, 0.0);
+    double r6[6];
+    sp_cell_record(P, D, S, t, c, idx, istride, n, r6);
   }
-  if (t < P.vol)   // the census totals of the binning kernel against the reference's MaxA/MaxB/MaxS
-    if (D.tcount[t] > CENSUS_MAX || D.tcount[P.vol + t] > CENSUS_MAX || D.tcount[2 * P.vol + t] > CENSUS_MAX)
-      raise_error(&D.sc->err, ERR_MEMBERS);
-  if (t < P.rxnnumcell) {
-    int nb = D.tcount[1 * P.vol + t];
-    double prob = D.ptab[min(nb, CENSUS_MAX)];
-    double r, q;
-    philox_uniform2(P.seed, 2, D.sc->coll, (unsigned long long)t, r, q);
-    if (0.0 < r && r <= prob) {
-      unsigned key = D.firstB[t];
-      int i = key >= (unsigned)P.NT ? (int)(key - (unsigned)P.NT) : (int)key;
-      S.flag[i] = 0;
-    }
-  }
+  sp_cell_react(P, D, S, t);
 }
 
 // co_rotate in split storage (FP32 rotation) and the next near set
+// split storage: the rotation of co_rotate in FP32 with the record's mean and axis
+__device__ __forceinline__ void sp_rotate_velocity(const CoP& P, double rcx, double rcy, double rcz, double rux,
+                                                   double ruy, double ruz, float& vx, float& vy, float& vz) {
+  const float cr = (float)P.cosrot, sr = (float)P.sinrot, omc = (float)P.omc;
+  const float cx = (float)rcx, cy = (float)rcy, cz = (float)rcz;
+  const float ux = (float)rux, uy = (float)ruy, uz = (float)ruz;
+  const float dx = vx - cx, dy = vy - cy, dz = vz - cz;
+  const float rx = (dx * (cr + (ux * ux) * omc) + dy * ((ux * uy) * omc - uz * sr)) + dz * ((ux * uz) * omc + uy * sr);
+  const float ry = (dx * ((ux * uy) * omc + uz * sr) + dy * (cr + (uy * uy) * omc)) + dz * ((uy * uz) * omc - ux * sr);
+  const float rz = (dx * ((ux * uz) * omc - uy * sr) + dy * ((uy * uz) * omc + ux * sr)) + dz * (cr + (uz * uz) * omc);
+  vx = cx + rx; vy = cy + ry; vz = cz + rz;
+}
+__device__ __forceinline__ void sp_classify_next(const CoP& P, unsigned* bits, const unsigned char* dq, int i,
+                                                 float vx, float vy, float vz, double tau, double shift,
+                                                 float& fmax2) {
+  const float v2 = (vx * vx + vy * vy) + vz * vz;
+  const double reach = sqrt((double)v2) * tau + P.guard + 1e-6 + shift;
+  if ((double)dq[i] * (1.0 / 32.0) <= reach) atomicOr(&bits[i >> 5], 1u << (i & 31));
+  else fmax2 = fmaxf(fmax2, v2);
+}
+// FP32 velocities: the far maximum is raised by a relative 1e-6 so time_penetrate stays conservative
+__device__ __forceinline__ void sp_far_max_publish(unsigned long long* dst, float fmax2) {
+  far_max_publish(dst, (double)fmax2 * (1.0 + 1e-6));
+}
 // classify as rotate_body (1 or 2)
 __global__ __launch_bounds__(256) void sp_rotate(CoP P, CoD D, SpD S, int classify) {
   float fmax2 = 0.0f;
   const double tau = P.tau * (1.0 + 1e-9);
   const double shift = classify == 2 ? start_shift(P, D) : 0.0;
-  const float cr = (float)P.cosrot, sr = (float)P.sinrot, omc = (float)P.omc;
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < P.NT; i += (long long)gridDim.x * 256) {
     const int t = D.cell_of[i];
     const double2* rec = (const double2*)(D.cellrec + (long long)t * 8);
     const double2 r0 = rec[0], r1 = rec[1], r2 = rec[2];
-    const float cx = (float)r0.x, cy = (float)r0.y, cz = (float)r1.x;
-    const float ux = (float)r1.y, uy = (float)r2.x, uz = (float)r2.y;
     float vx = S.vx[i], vy = S.vy[i], vz = S.vz[i];
-    const float dx = vx - cx, dy = vy - cy, dz = vz - cz;
-    const float rx = (dx * (cr + (ux * ux) * omc) + dy * ((ux * uy) * omc - uz * sr)) + dz * ((ux * uz) * omc + uy * sr);
-    const float ry = (dx * ((ux * uy) * omc + uz * sr) + dy * (cr + (uy * uy) * omc)) + dz * ((uy * uz) * omc - ux * sr);
-    const float rz = (dx * ((ux * uz) * omc - uy * sr) + dy * ((uy * uz) * omc + ux * sr)) + dz * (cr + (uz * uz) * omc);
-    vx = cx + rx; vy = cy + ry; vz = cz + rz;
+    sp_rotate_velocity(P, r0.x, r0.y, r1.x, r1.y, r2.x, r2.y, vx, vy, vz);
     S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz;
-    const float v2 = (vx * vx + vy * vy) + vz * vz;
-    const double reach = sqrt((double)v2) * tau + P.guard + 1e-6 + shift;
-    if ((double)D.dq[i] * (1.0 / 32.0) <= reach) atomicOr(&D.near_bits[i >> 5], 1u << (i & 31));
-    else fmax2 = fmaxf(fmax2, v2);
+    sp_classify_next(P, D.near_bits, D.dq, (int)i, vx, vy, vz, tau, shift, fmax2);
   }
-  double m = wave_max((double)fmax2);
-  __shared__ double red[4];
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double mm = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-    // FP32 velocities: the far maximum is raised by a relative 1e-6 so time_penetrate stays conservative
-    atomicMax(&D.sc->far_maxv2, (unsigned long long)__double_as_longlong(mm * (1.0 + 1e-6)));
+  sp_far_max_publish(&D.sc->far_maxv2, fmax2);
+}
+// split storage, first collision pass beside the near engine (as
+// co_collide_far / co_rotate_far)
+__global__ __launch_bounds__(CT) void sp_collide_far(CoP P, CoD D, SpD S) {
+  __shared__ int sidx[LB * CT];
+  const long long t = (long long)xcd_block(blockIdx.x, gridDim.x) * CT + threadIdx.x;
+  if (t < P.mpcnumcell && D.stamp[t] != D.epoch) {
+    int istride, n;
+    double r6[6];
+    const int* idx = slab_members(D, (int)t, sidx, istride, n);
+    sp_cell_record(P, D, S, t, D.mpc_cell[t], idx, istride, n, r6);
   }
+}
+__global__ __launch_bounds__(256) void sp_rotate_far(CoP P, CoD D, SpD S) {
+  float fmax2 = 0.0f;
+  const double tau = P.tau * (1.0 + 1e-9);
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < P.NT; i += (long long)gridDim.x * 256) {
+    if ((D.near_bits[i >> 5] >> (i & 31)) & 1u) continue;
+    const int t = D.cell_of[i];
+    if (D.stamp[t] == D.epoch) continue;
+    const double2* rec = (const double2*)(D.cellrec + (long long)t * 8);
+    const double2 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+    float vx = S.vx[i], vy = S.vy[i], vz = S.vz[i];
+    sp_rotate_velocity(P, r0.x, r0.y, r1.x, r1.y, r2.x, r2.y, vx, vy, vz);
+    S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz;
+    sp_classify_next(P, D.next_bits, D.dq, (int)i, vx, vy, vz, tau, P.guard, fmax2);
+  }
+  sp_far_max_publish(&D.sc->next_fmax2, fmax2);
+}
+// split storage, second pass: the late cells (as co_collide_late)
+__global__ __launch_bounds__(CT) void sp_collide_late(CoP P, CoD D, SpD S) {
+  __shared__ int sidx[LB * CT];
+  const double tau = P.tau * (1.0 + 1e-9), shift = start_shift(P, D);
+  const long long nl = min(D.sc->n_late, D.late_cap);
+  float fmax2 = 0.0f;
+  for (long long base = (long long)blockIdx.x * CT; base < nl; base += (long long)gridDim.x * CT) {
+    const long long k = base + threadIdx.x;
+    if (k >= nl) continue;
+    const int t = D.late[k];
+    int istride, n;
+    double r6[6];
+    const int* idx = slab_members(D, t, sidx, istride, n);
+    sp_cell_record(P, D, S, t, D.mpc_cell[t], idx, istride, n, r6);
+    for (int j = 0; j < n; j++) {
+      const int i = idx[j * istride];
+      float vx = S.vx[i], vy = S.vy[i], vz = S.vz[i];
+      sp_rotate_velocity(P, r6[0], r6[1], r6[2], r6[3], r6[4], r6[5], vx, vy, vz);
+      S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz;
+      sp_classify_next(P, D.next_bits, D.dq, i, vx, vy, vz, tau, shift, fmax2);
+    }
+  }
+  sp_far_max_publish(&D.sc->next_fmax2, fmax2);
+}
+__global__ __launch_bounds__(256) void sp_react(CoP P, CoD D, SpD S) {
+  const long long nt = max(P.vol, P.rxnnumcell);
+  for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < nt; t += (long long)gridDim.x * 256)
+    sp_cell_react(P, D, S, t);
 }
 
 // Hilbert re-sort of the split arrays (sort_hilbert_mod.f95:50-92: stable bin
@@ -4601,7 +4687,7 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
     const bool sort_due = (cnext % P.sort_step) == 0;
     // FP64 with the far flight beside the one-workgroup near engine: cells
     // collided in two passes (a re-sort interval keeps one pass after the sort)
-    const bool two = e->overlap && !e->split && e->near_mode == 0 && !sort_due;
+    const bool two = e->overlap && e->near_mode == 0 && !sort_due;
     if (two) D.epoch = ++e->epoch;
     hipLaunchKernelGGL(interval_begin, dim3(grid_for(3 * vol, 256, 2048)), dim3(256), 0, st, P, D,
                        (unsigned long long)(cnext / P.MPCtstep), two ? D.next_bits : nullptr,
@@ -4626,6 +4712,7 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
       co_near_list(e);
       e->list_due = false;
     }
+    if (e->split) sp_convert(e, true, true);   // the near set in FP64 for the near engine
     // the far particles' flight and binning beside the near engine
     const bool ov = e->overlap && e->near_mode == 0;
     if (ov) {
@@ -4633,23 +4720,25 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
       HIPCHK(hipEventRecord(e->fork, st));
       HIPCHK(hipStreamWaitEvent(s2, e->fork, 0));
       if (!sort_due) HIPCHK(tmark(1, 0, s2));
-      if (e->split) hipLaunchKernelGGL(sp_far, dim3(e->g_spbulk), dim3(256), 0, s2, P, D, e->S, sort_due ? 0 : 1);
+      if (e->split)
+        hipLaunchKernelGGL(sp_far, dim3(e->g_spbulk), dim3(256), 0, s2, P, D, e->S, sort_due ? 0 : 1, two ? 1 : 0);
       else hipLaunchKernelGGL(co_far, dim3(e->g_bulk), dim3(256), 0, s2, P, D, sort_due ? 0 : 1, two ? 1 : 0);
       if (!sort_due) HIPCHK(tmark(1, 1, s2));
       if (two) {   // the late cells around the spheres; the first collision pass
         hipLaunchKernelGGL(co_zone_mark, dim3(64, 2 * e->nd), dim3(256), 0, s2, P, D);
         HIPCHK(hipEventRecord(e->far_done, s2));
         HIPCHK(tmark(2, 0, s2));
-        hipLaunchKernelGGL(co_collide_far, dim3((unsigned)((mc + CT - 1) / CT)), dim3(CT), 0, s2, P, D);
+        if (e->split) hipLaunchKernelGGL(sp_collide_far, dim3((unsigned)((mc + CT - 1) / CT)), dim3(CT), 0, s2, P, D, e->S);
+        else hipLaunchKernelGGL(co_collide_far, dim3((unsigned)((mc + CT - 1) / CT)), dim3(CT), 0, s2, P, D);
         HIPCHK(tmark(2, 1, s2));
         HIPCHK(tmark(3, 0, s2));
-        hipLaunchKernelGGL(co_rotate_far, dim3(gN), dim3(256), 0, s2, P, D);
+        if (e->split) hipLaunchKernelGGL(sp_rotate_far, dim3(gN), dim3(256), 0, s2, P, D, e->S);
+        else hipLaunchKernelGGL(co_rotate_far, dim3(gN), dim3(256), 0, s2, P, D);
         HIPCHK(tmark(3, 1, s2));
       }
       HIPCHK(hipEventRecord(e->join, s2));
     }
     if (last) HIPCHK(hipEventRecord(e->n0, st));
-    if (e->split) sp_convert(e, true, true);
     HIPCHK(tmark(0, 0, st));
     HIPCHK(launch_near(e, P.MPCtstep, sort_due ? M_STOP : 0));
     HIPCHK(tmark(0, 1, st));
@@ -4658,9 +4747,15 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
     if (two) {   // the second pass needs co_far's binning only, not the first pass
       if (last) HIPCHK(hipEventRecord(e->b0, st));
       HIPCHK(hipStreamWaitEvent(st, e->far_done, 0));
-      hipLaunchKernelGGL(co_bin_rest, dim3(128), dim3(256), 0, st, P, D, 1, 1);
-      hipLaunchKernelGGL(co_react, dim3(grid_for(vol, 256, 2048)), dim3(256), 0, st, P, D);
-      hipLaunchKernelGGL(co_collide_late, dim3(512), dim3(CT), 0, st, P, D);
+      if (e->split) {
+        hipLaunchKernelGGL(sp_bin_near, dim3(64), dim3(256), 0, st, P, D, e->S, 1, 1);
+        hipLaunchKernelGGL(sp_react, dim3(grid_for(vol, 256, 2048)), dim3(256), 0, st, P, D, e->S);
+        hipLaunchKernelGGL(sp_collide_late, dim3(512), dim3(CT), 0, st, P, D, e->S);
+      } else {
+        hipLaunchKernelGGL(co_bin_rest, dim3(128), dim3(256), 0, st, P, D, 1, 1);
+        hipLaunchKernelGGL(co_react, dim3(grid_for(vol, 256, 2048)), dim3(256), 0, st, P, D);
+        hipLaunchKernelGGL(co_collide_late, dim3(512), dim3(CT), 0, st, P, D);
+      }
     }
     if (ov) HIPCHK(hipStreamWaitEvent(st, e->join, 0));
     const bool sp = e->split;
@@ -4669,7 +4764,7 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
       else hipLaunchKernelGGL(co_bulk, dim3(e->g_bulk), dim3(256), 0, st, P, D, stream, bin);
     };
     if (sort_due) {
-      if (ov && sp) hipLaunchKernelGGL(sp_bin_near, dim3(1), dim3(256), 0, st, P, D, e->S, 0);   // the flight check
+      if (ov && sp) hipLaunchKernelGGL(sp_bin_near, dim3(1), dim3(256), 0, st, P, D, e->S, 0, 0);   // the flight check
       else if (ov) hipLaunchKernelGGL(co_bin_rest, dim3(128), dim3(256), 0, st, P, D, 0, 0);
       else bulk(1, 0);
       int r = sp ? sp_sort(e, st) : co_sort(e, st);
@@ -4689,7 +4784,7 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
       continue;
     } else if (ov) {
       if (last) HIPCHK(hipEventRecord(e->b0, st));
-      if (sp) hipLaunchKernelGGL(sp_bin_near, dim3(64), dim3(256), 0, st, P, D, e->S, 1);
+      if (sp) hipLaunchKernelGGL(sp_bin_near, dim3(64), dim3(256), 0, st, P, D, e->S, 1, 0);
       else hipLaunchKernelGGL(co_bin_rest, dim3(128), dim3(256), 0, st, P, D, 1, 0);
     } else {
       if (last) HIPCHK(hipEventRecord(e->b0, st));
