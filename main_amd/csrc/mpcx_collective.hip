@@ -3292,6 +3292,21 @@ __device__ __forceinline__ void sort_gather_body(const CoD& D, const int* sb, do
 // one array of the permutation at a time: the working set of each pass (one
 // source array) stays in the last-level cache, where the eight interleaved
 // arrays of sort_gather did not
+// the six FP64 state arrays in one pass: the source index read once, six
+// loads in flight per element
+struct Six { const double* s[6]; double* d[6]; };
+__global__ __launch_bounds__(256) void sort_gather_six(long long n, const int* __restrict__ sb, Six a) {
+  long long first, end;
+  block_range(n, gridDim.x, xcd_block(blockIdx.x, gridDim.x), first, end);
+  for (long long j = first + threadIdx.x; j < end; j += 256) {
+    const int i = sb[j];
+    double v[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) v[k] = a.s[k][i];
+#pragma unroll
+    for (int k = 0; k < 6; k++) a.d[k][j] = v[k];
+  }
+}
 template <typename T>
 __global__ __launch_bounds__(256) void sort_gather_one(long long n, const int* __restrict__ sb,
                                                        const T* __restrict__ src, T* __restrict__ dst) {
@@ -4343,9 +4358,8 @@ static int co_sort(mpcx_co* e, hipStream_t st) {
   {
     const int g = grid_for(NT, 256);
     const int* sb = e->d_sb;
-    const double* src[6] = {D.x, D.y, D.z, D.vx, D.vy, D.vz};
-    for (int a = 0; a < 6; a++)
-      hipLaunchKernelGGL(sort_gather_one<double>, dim3(g), dim3(256), 0, st, NT, sb, src[a], e->alt[a]);
+    Six six = {{D.x, D.y, D.z, D.vx, D.vy, D.vz}, {e->alt[0], e->alt[1], e->alt[2], e->alt[3], e->alt[4], e->alt[5]}};
+    hipLaunchKernelGGL(sort_gather_six, dim3(g), dim3(256), 0, st, NT, sb, six);
     hipLaunchKernelGGL(sort_gather_one<int>, dim3(g), dim3(256), 0, st, NT, sb, (const int*)D.flag, e->alt_flag);
     hipLaunchKernelGGL(sort_gather_bits, dim3(g), dim3(256), 0, st, P, D, sb, e->alt_status, e->alt_bits);
   }
