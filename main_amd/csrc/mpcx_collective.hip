@@ -900,7 +900,19 @@ __device__ __attribute__((noinline)) void ne_begin_step(double maxv) {
 // thread 0, end of step t: dimer forces, velocity RATTLE and, unless t is the
 // last step of the launch, the start of step t+1
 template <int ND>
-__device__ __attribute__((noinline)) void ne_end_step(bool last) {
+__device__ __forceinline__ void ne_end_step_body(bool last);
+// One dimer: inlined into the kernel (a call saves and restores some 33
+// callee-saved registers through scratch, a memory round trip per MD step on
+// the serial path); more dimers need more registers than the kernel has left.
+template <int ND>
+__device__ __attribute__((noinline)) void ne_end_step_call(bool last) { ne_end_step_body<ND>(last); }
+template <int ND>
+__device__ __forceinline__ void ne_end_step(bool last) {
+  if constexpr (ND == 1) ne_end_step_body<ND>(last);
+  else ne_end_step_call<ND>(last);
+}
+template <int ND>
+__device__ __forceinline__ void ne_end_step_body(bool last) {
   const CoP& P = g_ne.P;
   const CoD& D = g_ne.D;
   Dimer<ND> d;
