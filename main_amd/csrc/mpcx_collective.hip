@@ -2600,11 +2600,11 @@ __device__ __forceinline__ void bin_particle(const CoP& P, const CoD& D, int i, 
     bool leader = false;
     while (todo) {
       int ld = __ffsll((unsigned long long)todo) - 1;
-      int k = __shfl(ck, ld);
+      int k = wave_bcast(ck, ld);
       unsigned long long grp = __ballot(cen && ck == k);
       unsigned long long go = grp & outside;
       int src = __ffsll((unsigned long long)(go ? go : grp)) - 1;
-      unsigned mk = (unsigned)__shfl((int)key, src);
+      unsigned mk = (unsigned)wave_bcast((int)key, src);
       if (lane == ld) { leader = true; gsize = __popcll(grp); gkey = mk; }
       todo &= ~grp;
     }
@@ -2651,7 +2651,7 @@ __device__ __forceinline__ void wall_defer(const CoD& D, bool wall, int i) {
   const int leader = __ffsll(m) - 1;
   int base = 0;
   if (lane == leader) base = atomicAdd(&D.sc->n_wall, __popcll(m));
-  base = __shfl(base, leader);
+  base = wave_bcast(base, leader);
   if (wall) {
     const int k = base + __popcll(m & ((1ull << lane) - 1ull));
     if (k < D.wall_cap) D.wall_list[k] = i;
@@ -3553,11 +3553,11 @@ __device__ __forceinline__ void sp_bin_particle(const CoP& P, const CoD& D, int 
     bool leader = false;
     while (todo) {
       int ld = __ffsll((unsigned long long)todo) - 1;
-      int k = __shfl(ck, ld);
+      int k = wave_bcast(ck, ld);
       unsigned long long grp = __ballot(cen && ck == k);
       unsigned long long go = grp & outside;
       int src = __ffsll((unsigned long long)(go ? go : grp)) - 1;
-      unsigned mk = (unsigned)__shfl((int)key, src);
+      unsigned mk = (unsigned)wave_bcast((int)key, src);
       if (lane == ld) { leader = true; gsize = __popcll(grp); gkey = mk; }
       todo &= ~grp;
     }

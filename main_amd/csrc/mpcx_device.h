@@ -38,6 +38,11 @@ enum DevError : int {
 
 __device__ __forceinline__ void raise_error(int* err, int code) { atomicCAS(err, 0, code); }
 
+// value of v in lane `lane` of the wave, for a wave-uniform lane: one
+// v_readlane instead of a shuffle through the LDS crossbar (__shfl), which the
+// grouping loops would otherwise wait on once per distinct key
+__device__ __forceinline__ int wave_bcast(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+
 // --------------------------------------------------------- geometry helpers
 // "if (d > 0.5d0*L) d = d - L; else if (d < -0.5d0*L) d = d + L"
 __device__ __forceinline__ double pbc_half(double d, double L) {

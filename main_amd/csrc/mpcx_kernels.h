@@ -19,7 +19,7 @@ __device__ __forceinline__ int wave_atomic_rank(int* counter, int key, bool acti
   int leader_of = 0, rank = 0, gsize = 0;
   while (todo) {
     int leader = __ffsll((unsigned long long)todo) - 1;
-    int k = __shfl(key, leader);
+    int k = wave_bcast(key, leader);
     bool mine = active && key == k;
     unsigned long long grp = __ballot(mine);
     if (mine) { leader_of = leader; rank = __popcll(grp & lt); }

@@ -599,7 +599,7 @@ __global__ __launch_bounds__(256) void sk_bulk(SkP P, SkD D, int nmove, int do_b
       bool leader = false;
       while (todo) {
         const int ld = __ffsll((unsigned long long)todo) - 1;
-        const int k = __shfl(ck, ld);
+        const int k = wave_bcast(ck, ld);
         const unsigned long long grp = __ballot(cen && ck == k);
         if (lane == ld) { leader = true; gsize = __popcll(grp); gmin = (unsigned)i; }
         todo &= ~grp;
