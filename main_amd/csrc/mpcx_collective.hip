@@ -2535,9 +2535,13 @@ __device__ __forceinline__ int late_cells_of(const CoP& P, const CoD& D, double 
 // slab 0: cell and rank for co_scatter; 1: the particle's slot in its cell's
 // slab written directly; 2: the same, and the cell must be listed late (the
 // particle was binned after the first collision pass).
+// ascending: the lanes' particle indices ascend (a grid-stride pass, or the
+// near list), so a group's lowest lane holds its minimum key; otherwise (the
+// deferred particles, in append order) each lane offers its own key.
 __device__ __forceinline__ void bin_particle(const CoP& P, const CoD& D, int i, bool valid, bool near,
                                              unsigned char st, double x, double y, double z, int f, double xs,
-                                             double ys, double zs, const double (*sph)[MAXSG], int slab = 0) {
+                                             double ys, double zs, const double (*sph)[MAXSG], int slab = 0,
+                                             bool ascending = true) {
   const int NT = P.NT;
   const int lane = threadIdx.x & 63;
   bool ok = valid;
@@ -2608,10 +2612,12 @@ __device__ __forceinline__ void bin_particle(const CoP& P, const CoD& D, int i, 
       if (lane == ld) { leader = true; gsize = __popcll(grp); gkey = mk; }
       todo &= ~grp;
     }
+    const bool bcell = ck >= P.vol && ck < 2 * P.vol;
     if (leader) {
       atomicAdd(&D.tcount[ck], gsize);   // no return: the collision kernel checks the totals against CENSUS_MAX
-      if (ck >= P.vol && ck < 2 * P.vol) atomicMin(&D.firstB[ck - P.vol], gkey);
+      if (bcell && ascending) atomicMin(&D.firstB[ck - P.vol], gkey);
     }
+    if (cen && bcell && !ascending) atomicMin(&D.firstB[ck - P.vol], key);
   }
 }
 
@@ -2802,7 +2808,7 @@ __global__ __launch_bounds__(256) void co_bin_rest(CoP P, CoD D, int do_bin, int
       st = D.status[i];
       if (!near) replay_events(P, D, i, x, y, z, f, ev, nev);
     }
-    if (do_bin) bin_particle(P, D, i, valid, near, st, x, y, z, f, xs, ys, zs, sph, slab ? 2 : 0);
+    if (do_bin) bin_particle(P, D, i, valid, near, st, x, y, z, f, xs, ys, zs, sph, slab ? 2 : 0, false);
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) check_flight(P, D);
 }
