@@ -2526,24 +2526,66 @@ __device__ __forceinline__ int late_cells_of(const CoP& P, const CoD& D, double 
       }
   return first;
 }
-// Binning of one particle per lane; whole waves call it together (`valid`
-// false for idle lanes).  The shifted cell, counted by tour position with the
-// particle's rank in it; the distance to the nearest sphere of `sph` beyond
-// its list radius (selection of the next interval's near set); the census of
-// distribute_mod.f95:95-183: outside particles keyed by index, inside ones
-// after all outside ones, inside ones within a cutoff skipped.
+
+// Grouping of a block's binning (every thread of the block calls it
+// together): each lane's rank among the block's lanes with the same cell t,
+// and the census -- count and minimum key per (species, cell) ck.  The keys go
+// into hash tables in LDS with one LDS atomic per lane, then one global atomic
+// per distinct key of the block: no loop over a wave's distinct keys.
+constexpr int GT = 512;   // hash slots per table, twice the block
+struct GroupTables {
+  int rk[GT], rc[GT], rb[GT];   // cell key, lanes, global base
+  int ck[GT], cc[GT];           // census key, lanes
+  unsigned cm[GT];              // census minimum key
+};
+__device__ __forceinline__ int gt_slot(int* keys, int key) {
+  int j = (int)(((unsigned)key * 2654435761u) >> 23);   // 9 bits
+  for (;;) {   // fewer keys than slots: an empty or equal slot is always found
+    const int old = atomicCAS(&keys[j], -1, key);
+    if (old == -1 || old == key) return j;
+    j = (j + 1) & (GT - 1);
+  }
+}
+__device__ __forceinline__ int block_groups(const CoP& P, const CoD& D, GroupTables& G, int t, bool ok, int ck,
+                                            bool cen, unsigned key) {
+  for (int j = threadIdx.x; j < GT; j += blockDim.x) {
+    G.rk[j] = -1; G.rc[j] = 0; G.ck[j] = -1; G.cc[j] = 0; G.cm[j] = 0xffffffffu;
+  }
+  __syncthreads();
+  int rs = 0, local = 0;
+  if (ok) { rs = gt_slot(G.rk, t); local = atomicAdd(&G.rc[rs], 1); }
+  if (cen) {
+    const int cs = gt_slot(G.ck, ck);
+    atomicAdd(&G.cc[cs], 1);
+    atomicMin(&G.cm[cs], key);
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < GT; j += blockDim.x) {
+    if (G.rc[j]) G.rb[j] = atomicAdd(&D.cnt[G.rk[j]], G.rc[j]);
+    if (G.cc[j]) {
+      const int k = G.ck[j];
+      atomicAdd(&D.tcount[k], G.cc[j]);   // no return: the collision kernel checks the totals against CENSUS_MAX
+      if (k >= P.vol && k < 2 * P.vol) atomicMin(&D.firstB[k - P.vol], G.cm[j]);
+    }
+  }
+  __syncthreads();
+  return ok ? G.rb[rs] + local : 0;
+}
+
+// Binning of one particle per lane; every thread of the block calls it
+// together (`valid` false for idle lanes).  The shifted cell, counted by tour
+// position with the particle's rank in it; the distance to the nearest sphere
+// of `sph` beyond its list radius (selection of the next interval's near set);
+// the census of distribute_mod.f95:95-183: outside particles keyed by index,
+// inside ones after all outside ones, inside ones within a cutoff skipped.
 // slab 0: cell and rank for co_scatter; 1: the particle's slot in its cell's
 // slab written directly; 2: the same, and the cell must be listed late (the
 // particle was binned after the first collision pass).
-// ascending: the lanes' particle indices ascend (a grid-stride pass, or the
-// near list), so a group's lowest lane holds its minimum key; otherwise (the
-// deferred particles, in append order) each lane offers its own key.
 __device__ __forceinline__ void bin_particle(const CoP& P, const CoD& D, int i, bool valid, bool near,
                                              unsigned char st, double x, double y, double z, int f, double xs,
-                                             double ys, double zs, const double (*sph)[MAXSG], int slab = 0,
-                                             bool ascending = true) {
+                                             double ys, double zs, const double (*sph)[MAXSG], int slab = 0) {
+  __shared__ GroupTables G;
   const int NT = P.NT;
-  const int lane = threadIdx.x & 63;
   bool ok = valid;
   int t = 0;
   if (valid) {
@@ -2551,7 +2593,20 @@ __device__ __forceinline__ void bin_particle(const CoP& P, const CoD& D, int i, 
     if (c < 0 || c >= P.mpcnumcell) { raise_error(&D.sc->err, ERR_CELL_RANGE); ok = false; }
     else t = D.mpc_pos[c];
   }
-  int r = wave_atomic_rank(D.cnt, t, ok);
+  unsigned key = (unsigned)i;
+  bool cen = false;
+  int ck = 0;
+  if (valid && ok) {
+    bool skip = false;
+    if (near && (st & ST_INSIDE)) { key = (unsigned)NT + (unsigned)i; skip = (st & ST_EXCL) != 0; }
+    int sp = f == 0 ? 0 : f == 1 ? 1 : f == 2 ? 2 : -1;
+    if (!skip && sp >= 0) {
+      long long u = cell_unshifted0(x, y, z, P.box[0], P.box[1]);
+      if (u < 0 || u >= P.vol) raise_error(&D.sc->err, ERR_CELL_RANGE);
+      else { cen = true; ck = (int)(sp * P.vol + u); }
+    }
+  }
+  const int r = block_groups(P, D, G, t, ok, ck, cen, key);
   if (valid) {
     if (ok && r >= BMAX) raise_error(&D.sc->err, ERR_BUCKET);
     D.cell_of[i] = t;
@@ -2582,42 +2637,6 @@ __device__ __forceinline__ void bin_particle(const CoP& P, const CoD& D, int i, 
     double qd = floor(fmax(dmin, 0.0) * 32.0 * (1.0 - 1e-12));
     if (st & ST_INSIDE) qd = 0.0;   // list members stay near until the next rebuild
     D.dq[i] = (unsigned char)fmin(qd, 255.0);
-  }
-  unsigned key = (unsigned)i;
-  bool cen = false;
-  int ck = 0;
-  if (valid && ok) {
-    bool skip = false;
-    if (near && (st & ST_INSIDE)) { key = (unsigned)NT + (unsigned)i; skip = (st & ST_EXCL) != 0; }
-    int sp = f == 0 ? 0 : f == 1 ? 1 : f == 2 ? 2 : -1;
-    if (!skip && sp >= 0) {
-      long long u = cell_unshifted0(x, y, z, P.box[0], P.box[1]);
-      if (u < 0 || u >= P.vol) raise_error(&D.sc->err, ERR_CELL_RANGE);
-      else { cen = true; ck = (int)(sp * P.vol + u); }
-    }
-  }
-  {   // census groups: one count and one minimum key per (species, cell) of the wave
-    unsigned long long todo = __ballot(cen);
-    const unsigned long long outside = __ballot(cen && key < (unsigned)NT);
-    int gsize = 0;
-    unsigned gkey = 0;
-    bool leader = false;
-    while (todo) {
-      int ld = __ffsll((unsigned long long)todo) - 1;
-      int k = wave_bcast(ck, ld);
-      unsigned long long grp = __ballot(cen && ck == k);
-      unsigned long long go = grp & outside;
-      int src = __ffsll((unsigned long long)(go ? go : grp)) - 1;
-      unsigned mk = (unsigned)wave_bcast((int)key, src);
-      if (lane == ld) { leader = true; gsize = __popcll(grp); gkey = mk; }
-      todo &= ~grp;
-    }
-    const bool bcell = ck >= P.vol && ck < 2 * P.vol;
-    if (leader) {
-      atomicAdd(&D.tcount[ck], gsize);   // no return: the collision kernel checks the totals against CENSUS_MAX
-      if (bcell && ascending) atomicMin(&D.firstB[ck - P.vol], gkey);
-    }
-    if (cen && bcell && !ascending) atomicMin(&D.firstB[ck - P.vol], key);
   }
 }
 
@@ -2808,7 +2827,7 @@ __global__ __launch_bounds__(256) void co_bin_rest(CoP P, CoD D, int do_bin, int
       st = D.status[i];
       if (!near) replay_events(P, D, i, x, y, z, f, ev, nev);
     }
-    if (do_bin) bin_particle(P, D, i, valid, near, st, x, y, z, f, xs, ys, zs, sph, slab ? 2 : 0, false);
+    if (do_bin) bin_particle(P, D, i, valid, near, st, x, y, z, f, xs, ys, zs, sph, slab ? 2 : 0);
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) check_flight(P, D);
 }
@@ -3486,16 +3505,16 @@ __device__ __forceinline__ bool sp_stream(int& ix, int& iy, int& iz, float& ox, 
   return false;
 }
 
-// Binning of one split-storage particle per lane; whole waves call it
-// together (`valid` false for idle lanes): shifted cell (mpc_rotation_mod.f95:
+// Binning of one split-storage particle per lane; every thread of the block
+// calls it together (`valid` false for idle lanes): shifted cell (mpc_rotation_mod.f95:
 // 116-145) from the cell and offsets with the particle's rank in it, the
 // distance byte to the spheres `sph`, and the census, as bin_particle.
 __device__ __forceinline__ void sp_bin_particle(const CoP& P, const CoD& D, int i, bool valid, bool near,
                                                 unsigned char st, int ix, int iy, int iz, float ox, float oy,
                                                 float oz, int f, float fxs, float fys, float fzs, double zs,
                                                 const double (*sph)[MAXSG], int slab = 0) {
+  __shared__ GroupTables G;
   const int NT = P.NT;
-  const int lane = threadIdx.x & 63;
   bool ok = valid;
   int t = 0;
   if (valid) {
@@ -3506,7 +3525,20 @@ __device__ __forceinline__ void sp_bin_particle(const CoP& P, const CoD& D, int 
     if (cz < 0 || c >= P.mpcnumcell) { raise_error(&D.sc->err, ERR_CELL_RANGE); ok = false; }
     else t = D.mpc_pos[c];
   }
-  int r = wave_atomic_rank(D.cnt, t, ok);
+  unsigned key = (unsigned)i;
+  bool cen = false;
+  int ck = 0;
+  if (valid && ok) {
+    bool skip = false;
+    if (near && (st & ST_INSIDE)) { key = (unsigned)NT + (unsigned)i; skip = (st & ST_EXCL) != 0; }
+    int sp = f == 0 ? 0 : f == 1 ? 1 : f == 2 ? 2 : -1;
+    if (!skip && sp >= 0) {
+      const long long u = (long long)ix + (long long)P.Lx * iy + (long long)P.Lx * P.Ly * iz;
+      if (u < 0 || u >= P.vol) raise_error(&D.sc->err, ERR_CELL_RANGE);
+      else { cen = true; ck = (int)(sp * P.vol + u); }
+    }
+  }
+  const int r = block_groups(P, D, G, t, ok, ck, cen, key);
   if (valid) {
     if (ok && r >= BMAX) raise_error(&D.sc->err, ERR_BUCKET);
     D.cell_of[i] = t;
@@ -3537,40 +3569,6 @@ __device__ __forceinline__ void sp_bin_particle(const CoP& P, const CoD& D, int 
     double qd = floor(fmax(dmin - 1e-5, 0.0) * 32.0);
     if (st & ST_INSIDE) qd = 0.0;
     D.dq[i] = (unsigned char)fmin(qd, 255.0);
-  }
-  unsigned key = (unsigned)i;
-  bool cen = false;
-  int ck = 0;
-  if (valid && ok) {
-    bool skip = false;
-    if (near && (st & ST_INSIDE)) { key = (unsigned)NT + (unsigned)i; skip = (st & ST_EXCL) != 0; }
-    int sp = f == 0 ? 0 : f == 1 ? 1 : f == 2 ? 2 : -1;
-    if (!skip && sp >= 0) {
-      const long long u = (long long)ix + (long long)P.Lx * iy + (long long)P.Lx * P.Ly * iz;
-      if (u < 0 || u >= P.vol) raise_error(&D.sc->err, ERR_CELL_RANGE);
-      else { cen = true; ck = (int)(sp * P.vol + u); }
-    }
-  }
-  {
-    unsigned long long todo = __ballot(cen);
-    const unsigned long long outside = __ballot(cen && key < (unsigned)NT);
-    int gsize = 0;
-    unsigned gkey = 0;
-    bool leader = false;
-    while (todo) {
-      int ld = __ffsll((unsigned long long)todo) - 1;
-      int k = wave_bcast(ck, ld);
-      unsigned long long grp = __ballot(cen && ck == k);
-      unsigned long long go = grp & outside;
-      int src = __ffsll((unsigned long long)(go ? go : grp)) - 1;
-      unsigned mk = (unsigned)wave_bcast((int)key, src);
-      if (lane == ld) { leader = true; gsize = __popcll(grp); gkey = mk; }
-      todo &= ~grp;
-    }
-    if (leader) {
-      atomicAdd(&D.tcount[ck], gsize);   // no return: the collision kernel checks the totals against CENSUS_MAX
-      if (ck >= P.vol && ck < 2 * P.vol) atomicMin(&D.firstB[ck - P.vol], gkey);
-    }
   }
 }
 
