@@ -3170,35 +3170,46 @@ __global__ __launch_bounds__(256) void co_rotate_far(CoP P, CoD D) {
   far_max_publish(&D.sc->next_fmax2, fmax2);
 }
 // second pass, after the near engine: the late cells collided and their
-// members rotated and classified (the spheres' actual displacement)
-__global__ __launch_bounds__(CT) void co_collide_late(CoP P, CoD D) {
+// members rotated and classified (the spheres' actual displacement); the
+// workgroups from `nlate_wg` on run the census limits and bulk reactions of
+// every unshifted cell instead (the census is complete once the near set is
+// binned, and the collisions read no species, so the two are independent)
+__global__ __launch_bounds__(CT) void co_collide_late(CoP P, CoD D, int nlate_wg) {
   __shared__ int sidx[LB * CT];
+  if ((int)blockIdx.x >= nlate_wg) {
+    const long long nt = max(P.vol, P.rxnnumcell);
+    const long long g = (long long)(gridDim.x - nlate_wg) * CT;
+    for (long long t = (long long)(blockIdx.x - nlate_wg) * CT + threadIdx.x; t < nt; t += g) cell_react(P, D, t);
+    return;
+  }
   const double tau = P.tau * (1.0 + 1e-9), shift = start_shift(P, D);
   const long long nl = min(D.sc->n_late, D.late_cap);
   double fmax2 = 0.0;
-  for (long long base = (long long)blockIdx.x * CT; base < nl; base += (long long)gridDim.x * CT) {
+  for (long long base = (long long)blockIdx.x * CT; base < nl; base += (long long)nlate_wg * CT) {
     const long long k = base + threadIdx.x;
     if (k >= nl) continue;
     const int t = D.late[k];
     int istride, n;
     double r6[6];
     const int* idx = slab_cell(P, D, t, sidx, istride, n, r6);
-    for (int j = 0; j < n; j++) {
-      const int i = idx[j * istride];
-      double vx = D.vx[i], vy = D.vy[i], vz = D.vz[i];
-      rotate_velocity(P, r6[0], r6[1], r6[2], r6[3], r6[4], r6[5], vx, vy, vz);
-      D.vx[i] = vx; D.vy[i] = vy; D.vz[i] = vz;
-      classify_next(P, D.next_bits, D.dq, i, vx, vy, vz, tau, shift, fmax2);
+    for (int j = 0; j < n; j += 4) {   // four members' loads in flight at a time
+      int ii[4];
+      double vx[4], vy[4], vz[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) ii[u] = j + u < n ? idx[(j + u) * istride] : -1;
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (ii[u] >= 0) { vx[u] = D.vx[ii[u]]; vy[u] = D.vy[ii[u]]; vz[u] = D.vz[ii[u]]; }
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (ii[u] >= 0) {
+          rotate_velocity(P, r6[0], r6[1], r6[2], r6[3], r6[4], r6[5], vx[u], vy[u], vz[u]);
+          D.vx[ii[u]] = vx[u]; D.vy[ii[u]] = vy[u]; D.vz[ii[u]] = vz[u];
+          classify_next(P, D.next_bits, D.dq, ii[u], vx[u], vy[u], vz[u], tau, shift, fmax2);
+        }
     }
   }
   far_max_publish(&D.sc->next_fmax2, fmax2);
-}
-// once the near set is binned: census limits and bulk reactions of every
-// unshifted cell (the census is complete; the collision passes read no species)
-__global__ __launch_bounds__(256) void co_react(CoP P, CoD D) {
-  const long long nt = max(P.vol, P.rxnnumcell);
-  for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < nt; t += (long long)gridDim.x * 256)
-    cell_react(P, D, t);
 }
 // the far max of a two-pass interval handed over (after both passes)
 __global__ void co_far_handoff(CoD D) {
@@ -3833,12 +3844,21 @@ __global__ __launch_bounds__(CT) void sp_collide_late(CoP P, CoD D, SpD S) {
     double r6[6];
     const int* idx = slab_members(D, t, sidx, istride, n);
     sp_cell_record(P, D, S, t, D.mpc_cell[t], idx, istride, n, r6);
-    for (int j = 0; j < n; j++) {
-      const int i = idx[j * istride];
-      float vx = S.vx[i], vy = S.vy[i], vz = S.vz[i];
-      sp_rotate_velocity(P, r6[0], r6[1], r6[2], r6[3], r6[4], r6[5], vx, vy, vz);
-      S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz;
-      sp_classify_next(P, D.next_bits, D.dq, i, vx, vy, vz, tau, shift, fmax2);
+    for (int j = 0; j < n; j += 4) {   // four members' loads in flight at a time
+      int ii[4];
+      float vx[4], vy[4], vz[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) ii[u] = j + u < n ? idx[(j + u) * istride] : -1;
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (ii[u] >= 0) { vx[u] = S.vx[ii[u]]; vy[u] = S.vy[ii[u]]; vz[u] = S.vz[ii[u]]; }
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (ii[u] >= 0) {
+          sp_rotate_velocity(P, r6[0], r6[1], r6[2], r6[3], r6[4], r6[5], vx[u], vy[u], vz[u]);
+          S.vx[ii[u]] = vx[u]; S.vy[ii[u]] = vy[u]; S.vz[ii[u]] = vz[u];
+          sp_classify_next(P, D.next_bits, D.dq, ii[u], vx[u], vy[u], vz[u], tau, shift, fmax2);
+        }
     }
   }
   sp_far_max_publish(&D.sc->next_fmax2, fmax2);
@@ -4783,8 +4803,7 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
         hipLaunchKernelGGL(sp_collide_late, dim3(512), dim3(CT), 0, st, P, D, e->S);
       } else {
         hipLaunchKernelGGL(co_bin_rest, dim3(128), dim3(256), 0, st, P, D, 1, 1);
-        hipLaunchKernelGGL(co_react, dim3(grid_for(vol, 256, 2048)), dim3(256), 0, st, P, D);
-        hipLaunchKernelGGL(co_collide_late, dim3(512), dim3(CT), 0, st, P, D);
+        hipLaunchKernelGGL(co_collide_late, dim3(512 + 1024), dim3(CT), 0, st, P, D, 512);   // + reactions
       }
     }
     if (ov) HIPCHK(hipStreamWaitEvent(st, e->join, 0));
