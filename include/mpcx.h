@@ -325,6 +325,13 @@ int mpcx_sk_get_cells(mpcx_sk* e, double* rec8, int32_t* cell_of, int32_t* censu
  * most 256 of them): near engine, sk_bulk, sk_collide, sk_rotate; and the
  * run's wall time. */
 int mpcx_sk_kernel_ms(mpcx_sk* e, double* ms4, double* run_ms);
+/* on = 1 (the default): with device Philox draws, the far particles stream
+ * and bin on a second stream while the near engine runs, and the near set is
+ * binned after it; the results are bitwise those of on = 0 (the far
+ * particles' per-step streaming does not depend on the near engine).  Host
+ * draws keep the order of the reference's draw stream and never overlap.  No
+ * counterpart in the reference. */
+int mpcx_sk_set_overlap(mpcx_sk* e, int32_t on);
 
 /* ------------------------------------------------------------------------
  * Diagnostics on the device-resident state (either engine)

@@ -57,6 +57,7 @@ struct SkScalars {
   double bond[3];       // dX, dY, dZ left by the dimer motion for the velocity RATTLE
   double x0[2], y0[2], z0[2];
   unsigned long long prof[8];
+  double xs0[2], ys0[2], zs0[2];   // spheres at the interval start (kept while the near engine moves them)
 };
 
 struct SkP {
@@ -520,7 +521,12 @@ __global__ __launch_bounds__(SK_T) void sk_near_engine(SkP P, SkD D, int mode) {
 // had before the step's conversions), the shifted cell of
 // RMPC_SelkovDimer.f90:22-46 counted by cell and rank, and the distance byte
 // for the next near set.
-__global__ __launch_bounds__(256) void sk_bulk(SkP P, SkD D, int nmove, int do_bin) {
+__device__ __forceinline__ void sk_bin_one(const SkP& P, const SkD& D, int i, bool valid, bool near, int f,
+                                           double x, double y, double z, double xs, double ys, double zs,
+                                           const double (*sph)[2], bool asc);
+// far: the far particles only, beside the near engine (sk_bin_near bins the
+// near set afterwards), distances to the spheres at the interval's start
+__global__ __launch_bounds__(256) void sk_bulk(SkP P, SkD D, int nmove, int do_bin, int far) {
   const int NT = P.space;
   double xs = 0, ys = 0, zs = 0;
   if (do_bin) {
@@ -537,9 +543,9 @@ __global__ __launch_bounds__(256) void sk_bulk(SkP P, SkD D, int nmove, int do_b
   }
   __shared__ double sph[3][2];
   if (threadIdx.x < 2) {
-    sph[0][threadIdx.x] = D.dim[SX * 2 + threadIdx.x];
-    sph[1][threadIdx.x] = D.dim[SY * 2 + threadIdx.x];
-    sph[2][threadIdx.x] = D.dim[SZ * 2 + threadIdx.x];
+    sph[0][threadIdx.x] = far ? D.sc->xs0[threadIdx.x] : D.dim[SX * 2 + threadIdx.x];
+    sph[1][threadIdx.x] = far ? D.sc->ys0[threadIdx.x] : D.dim[SY * 2 + threadIdx.x];
+    sph[2][threadIdx.x] = far ? D.sc->zs0[threadIdx.x] : D.dim[SZ * 2 + threadIdx.x];
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
@@ -566,6 +572,31 @@ __global__ __launch_bounds__(256) void sk_bulk(SkP P, SkD D, int nmove, int do_b
       }
     }
     if (!do_bin) continue;
+    sk_bin_one(P, D, i, valid && !(far && near), near, f, x, y, z, xs, ys, zs, sph, true);
+  }
+}
+// distance beyond the nearest cutoff in 1/32 units, rounded down, capped at 255
+__device__ __forceinline__ unsigned char sk_qdist(const SkP& P, double x, double y, double z,
+                                                  const double (*sph)[2]) {
+  double dmin = 1e30;
+  for (int s = 0; s < 2; s++) {
+    double a = pbc_h(x - sph[0][s], P.hbox[0], P.box[0]);
+    double b = pbc_h(y - sph[1][s], P.hbox[1], P.box[1]);
+    double cz = pbc_h(z - sph[2][s], P.hbox[2], P.box[2]);
+    dmin = fmin(dmin, sqrt(a * a + b * b + cz * cz) - (s == 0 ? P.cutC : P.cutN));
+  }
+  const double qd = floor(fmax(dmin, 0.0) * 32.0 * (1.0 - 1e-12));
+  return (unsigned char)fmin(qd, 255.0);
+}
+// Binning of one particle per lane (whole waves call it together; `valid`
+// false for idle lanes): shifted cell and rank, census with the flag before
+// the collision step's conversions, distance byte.  asc: the wave's lanes
+// hold ascending indices, so a group's leader holds its lowest index.
+__device__ __forceinline__ void sk_bin_one(const SkP& P, const SkD& D, int i, bool valid, bool near, int f,
+                                           double x, double y, double z, double xs, double ys, double zs,
+                                           const double (*sph)[2], bool asc) {
+  const int lane = threadIdx.x & 63;
+  {
     const bool act = valid && active(f);
     // shifted cell (all three directions periodic)
     bool ok = act;
@@ -604,25 +635,43 @@ __global__ __launch_bounds__(256) void sk_bulk(SkP P, SkD D, int nmove, int do_b
         if (lane == ld) { leader = true; gsize = __popcll(grp); gmin = (unsigned)i; }
         todo &= ~grp;
       }
-      if (leader) {   // the leader holds the group's lowest index (lanes ascend with i)
+      if (leader) {
         int old = atomicAdd(&D.tcount[ck], gsize);
         if (old + gsize > SK_MAXS) raise_error(&D.sc->err, ERR_MEMBERS);
-        atomicMin(&D.first[ck], gmin);
+        if (asc) atomicMin(&D.first[ck], gmin);
       }
+      if (!asc && cen) atomicMin(&D.first[ck], (unsigned)i);
     }
     // distance beyond the nearest cutoff, 1/32 units (0 for particles carrying a force)
-    if (valid && act) {
-      double dmin = 1e30;
-      for (int s = 0; s < 2; s++) {
-        double a = pbc_h(x - sph[0][s], P.hbox[0], P.box[0]);
-        double b = pbc_h(y - sph[1][s], P.hbox[1], P.box[1]);
-        double cz = pbc_h(z - sph[2][s], P.hbox[2], P.box[2]);
-        dmin = fmin(dmin, sqrt(a * a + b * b + cz * cz) - (s == 0 ? P.cutC : P.cutN));
-      }
-      double qd = floor(fmax(dmin, 0.0) * 32.0 * (1.0 - 1e-12));
-      if (near && (D.cflag[i] & CF_FORCE)) qd = 0.0;
-      D.dq[i] = (unsigned char)fmin(qd, 255.0);
+    if (valid && act) D.dq[i] = (near && (D.cflag[i] & CF_FORCE)) ? (unsigned char)0 : sk_qdist(P, x, y, z, sph);
+  }
+}
+// the near set once the near engine is done (beside it, sk_bulk binned the
+// far particles): binning with distances to the spheres where they ended
+__global__ __launch_bounds__(256) void sk_bin_near(SkP P, SkD D) {
+  double u0, u1, u2, t;
+  philox_uniform2(P.seed, 4, D.coll, 0, u0, u1);   // the shift of sk_bulk (device draws)
+  philox_uniform2(P.seed, 4, D.coll, 1, u2, t);
+  const double xs = P.A0 * (u0 - 0.5), ys = P.A0 * (u1 - 0.5), zs = P.A0 * (u2 - 0.5);
+  __shared__ double sph[3][2];
+  if (threadIdx.x < 2) {
+    sph[0][threadIdx.x] = D.dim[SX * 2 + threadIdx.x];
+    sph[1][threadIdx.x] = D.dim[SY * 2 + threadIdx.x];
+    sph[2][threadIdx.x] = D.dim[SZ * 2 + threadIdx.x];
+  }
+  __syncthreads();
+  const long long nn = min(D.sc->near_num, D.near_cap);
+  for (long long b0 = (long long)blockIdx.x * blockDim.x; b0 < nn; b0 += (long long)gridDim.x * blockDim.x) {
+    const long long q = b0 + threadIdx.x;
+    const bool valid = q < nn;
+    int i = 0, f = 2;
+    double x = 0, y = 0, z = 0;
+    if (valid) {
+      i = D.near_list[q];
+      f = D.flag[i];
+      x = D.x[i]; y = D.y[i]; z = D.z[i];
     }
+    sk_bin_one(P, D, i, valid, true, f, x, y, z, xs, ys, zs, sph, false);
   }
 }
 
@@ -829,31 +878,76 @@ __global__ __launch_bounds__(256) void sk_rotate_begin(SkP P, SkD D, long long n
 // unaffected until sk_affected marks the ones near a sphere
 __global__ __launch_bounds__(256) void sk_interval_begin(SkD D, long long V) {
   const long long i0 = (long long)blockIdx.x * 256 + threadIdx.x, st = (long long)gridDim.x * 256;
+  if (i0 < 2) {   // the spheres' start positions, for the far particles' distances
+    D.sc->xs0[i0] = D.dim[SX * 2 + i0]; D.sc->ys0[i0] = D.dim[SY * 2 + i0]; D.sc->zs0[i0] = D.dim[SZ * 2 + i0];
+  }
   for (long long i = i0; i < V; i += st) { D.cnt[i] = 0; D.unaff[i] = 1; }
   for (long long i = i0; i < 3 * V; i += st) { D.tcount[i] = 0; D.first[i] = 0xffffffffu; }
 }
 
 // Rotation of RMPC_SelkovDimer.f90:79-121 (projection, cos-rotated
-// orthogonal part, sin-weighted cross product) and the next near set.
-__global__ __launch_bounds__(256) void sk_rotate(SkP P, SkD D) {
+// orthogonal part, sin-weighted cross product) and the next near set: within
+// |v| tau + guard of a cutoff by the distance byte, or carrying a force.
+// overlap: the far particles' distance bytes were measured to the spheres at
+// the interval's start (the near set's to where they ended); a far particle
+// within reach + the spheres' largest displacement + 1/16 of that measure is
+// measured again to where the spheres ended, so the near set is the one of
+// the one-stream order.  The near bitmap still holds this interval's set; each
+// wave reads its 64 bits and writes them whole.
+__global__ __launch_bounds__(256) void sk_rotate(SkP P, SkD D, int overlap) {
   const double tau = P.tau * (1.0 + 1e-9);
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.space; i += gridDim.x * blockDim.x) {
-    const int c = D.cell_of[i];
-    if (c < 0) continue;
-    const double2* rec = (const double2*)(D.cellrec + (long long)c * 8);
-    const double2 r0 = rec[0], r1 = rec[1], r2 = rec[2];
-    const double cx = r0.x, cy = r0.y, cz = r1.x, ux = r1.y, uy = r2.x, uz = r2.y;
-    const double dx = D.vx[i] - cx, dy = D.vy[i] - cy, dz = D.vz[i] - cz;
-    const double pr = (dx * ux + dy * uy) + dz * uz;
-    const double px = pr * ux, py = pr * uy, pz = pr * uz;
-    const double ox = dx - px, oy = dy - py, oz = dz - pz;
-    const double o2x = (oy * uz) - (oz * uy), o2y = (oz * ux) - (ox * uz), o2z = (ox * uy) - (oy * ux);
-    const double vx = cx + ((px + ox * P.cosrot) + o2x * P.sinrot);
-    const double vy = cy + ((py + oy * P.cosrot) + o2y * P.sinrot);
-    const double vz = cz + ((pz + oz * P.cosrot) + o2z * P.sinrot);
-    D.vx[i] = vx; D.vy[i] = vy; D.vz[i] = vz;
-    const double reach = sqrt((vx * vx + vy * vy) + vz * vz) * tau + P.guard + 1e-6;
-    if ((double)D.dq[i] * (1.0 / 32.0) <= reach) atomicOr(&D.near_bits[i >> 5], 1u << (i & 31));
+  __shared__ double sph[3][2];
+  __shared__ double shift_s;
+  if (overlap) {
+    if (threadIdx.x < 2) {
+      sph[0][threadIdx.x] = D.dim[SX * 2 + threadIdx.x];
+      sph[1][threadIdx.x] = D.dim[SY * 2 + threadIdx.x];
+      sph[2][threadIdx.x] = D.dim[SZ * 2 + threadIdx.x];
+    }
+    if (threadIdx.x == 0) {
+      double sh = 0.0;
+      for (int s = 0; s < 2; s++) {
+        const double a = pbc_h(D.dim[SX * 2 + s] - D.sc->xs0[s], P.hbox[0], P.box[0]);
+        const double b = pbc_h(D.dim[SY * 2 + s] - D.sc->ys0[s], P.hbox[1], P.box[1]);
+        const double c = pbc_h(D.dim[SZ * 2 + s] - D.sc->zs0[s], P.hbox[2], P.box[2]);
+        sh = fmax(sh, sqrt(a * a + b * b + c * c));
+      }
+      shift_s = sh;
+    }
+    __syncthreads();
+  }
+  const int lane = threadIdx.x & 63;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long b0 = (long long)blockIdx.x * blockDim.x; b0 < P.space; b0 += stride) {
+    const int i = (int)(b0 + threadIdx.x);
+    const int c = i < P.space ? D.cell_of[i] : -1;
+    bool nb = false;
+    if (c >= 0) {
+      const double2* rec = (const double2*)(D.cellrec + (long long)c * 8);
+      const double2 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+      const double cx = r0.x, cy = r0.y, cz = r1.x, ux = r1.y, uy = r2.x, uz = r2.y;
+      const double dx = D.vx[i] - cx, dy = D.vy[i] - cy, dz = D.vz[i] - cz;
+      const double pr = (dx * ux + dy * uy) + dz * uz;
+      const double px = pr * ux, py = pr * uy, pz = pr * uz;
+      const double ox = dx - px, oy = dy - py, oz = dz - pz;
+      const double o2x = (oy * uz) - (oz * uy), o2y = (oz * ux) - (ox * uz), o2z = (ox * uy) - (oy * ux);
+      const double vx = cx + ((px + ox * P.cosrot) + o2x * P.sinrot);
+      const double vy = cy + ((py + oy * P.cosrot) + o2y * P.sinrot);
+      const double vz = cz + ((pz + oz * P.cosrot) + o2z * P.sinrot);
+      D.vx[i] = vx; D.vy[i] = vy; D.vz[i] = vz;
+      const double reach = sqrt((vx * vx + vy * vy) + vz * vz) * tau + P.guard + 1e-6;
+      unsigned char q = D.dq[i];
+      if (overlap) {
+        const bool was = (D.near_bits[i >> 5] >> (i & 31)) & 1u;
+        if (!was && (double)q * (1.0 / 32.0) <= reach + shift_s + 0.0625) q = sk_qdist(P, D.x[i], D.y[i], D.z[i], sph);
+      }
+      nb = (double)q * (1.0 / 32.0) <= reach;
+      if (!overlap && nb) atomicOr(&D.near_bits[i >> 5], 1u << (i & 31));
+    }
+    if (overlap) {   // lanes 0 and 32 hold words of indices below space + 32 (< nwords + 2 words)
+      const unsigned long long m = __ballot(nb);
+      if (lane == 0 || lane == 32) D.near_bits[i >> 5] = (unsigned)(m >> lane);
+    }
   }
 }
 
@@ -955,6 +1049,11 @@ struct mpcx_sk {
   SkD D;
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  // with device draws, sk_bulk streams and bins the far particles on st2
+  // beside the near engine (st2 waits for fork, st for join)
+  hipStream_t st2 = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  bool overlap = true;
   // near engine, sk_bulk, sk_collide, sk_rotate: every TSTRIDE-th interval is
   // timed (each event costs the stream a few us), at most TMAX of them
   static constexpr int NKT = 4, TMAX = 256, TSTRIDE = 8;
@@ -1008,9 +1107,13 @@ static int sk_check_error(mpcx_sk* e) {
 extern "C" int mpcx_sk_destroy(mpcx_sk* e) {
   if (!e) return 0;
   if (e->st) (void)hipStreamSynchronize(e->st);
+  if (e->st2) (void)hipStreamSynchronize(e->st2);
   for (void* p : e->allocs)
     if (p) (void)hipFree(p);
   if (e->st) (void)hipStreamDestroy(e->st);
+  if (e->st2) (void)hipStreamDestroy(e->st2);
+  if (e->fork) (void)hipEventDestroy(e->fork);
+  if (e->join) (void)hipEventDestroy(e->join);
   if (e->e0) (void)hipEventDestroy(e->e0);
   if (e->e1) (void)hipEventDestroy(e->e1);
   for (hipEvent_t ev : e->tev)
@@ -1103,6 +1206,9 @@ extern "C" int mpcx_sk_create(mpcx_sk** out, const mpcx_sk_config* cfg) {
     return set_error(MPCX_E_HIP, "device allocation failed");
   }
   SKCHK(hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking));
+  SKCHK(hipStreamCreateWithFlags(&e->st2, hipStreamNonBlocking));
+  SKCHK(hipEventCreateWithFlags(&e->fork, hipEventDisableTiming));
+  SKCHK(hipEventCreateWithFlags(&e->join, hipEventDisableTiming));
   SKCHK(hipEventCreate(&e->e0)); SKCHK(hipEventCreate(&e->e1));
   e->tev.assign((size_t)mpcx_sk::TMAX * mpcx_sk::NKT * 2, nullptr);
   e->tev_on.assign((size_t)mpcx_sk::TMAX * mpcx_sk::NKT, 0);
@@ -1241,18 +1347,35 @@ extern "C" int mpcx_sk_run(mpcx_sk* e, int64_t ncoll, const double* draws, int64
     if (timed)
       for (int k = 0; k < mpcx_sk::NKT; k++) e->tev_on[(size_t)slot * mpcx_sk::NKT + k] = 0;
     hipLaunchKernelGGL(sk_interval_begin, dim3(grid_for(3 * V, 256, 2048)), dim3(256), 0, st, D, (long long)V);
+    // device draws: the far particles stream and bin beside the near engine
+    // (host draws: the shift is read at the draw cursor the near engine moves)
+    const bool ov = e->overlap && P.rng_mode == 1 && !sort_due;
+    if (ov) SKCHK(hipEventRecord(e->fork, st));
     SKCHK(tmark(0, 0));
     hipLaunchKernelGGL(sk_near_engine, dim3(1), dim3(SK_T), 0, st, P, D, sort_due ? SM_STOP : 0);
     SKCHK(tmark(0, 1));
-    if (sort_due) {
-      hipLaunchKernelGGL(sk_bulk, dim3(gN), dim3(256), 0, st, P, D, P.S - 1, 0);
+    if (ov) {   // enqueued after the near engine, so that it is dispatched first
+      SKCHK(hipStreamWaitEvent(e->st2, e->fork, 0));
+      if (timed) {
+        e->tev_on[(size_t)slot * mpcx_sk::NKT + 1] = 1;
+        SKCHK(hipEventRecord(e->tev[((size_t)slot * mpcx_sk::NKT + 1) * 2], e->st2));
+      }
+      hipLaunchKernelGGL(sk_bulk, dim3(gN), dim3(256), 0, e->st2, P, D, P.S, 1, 1);
+      if (timed) SKCHK(hipEventRecord(e->tev[((size_t)slot * mpcx_sk::NKT + 1) * 2 + 1], e->st2));
+      SKCHK(hipEventRecord(e->join, e->st2));
+    }
+    if (ov) {
+      SKCHK(hipStreamWaitEvent(st, e->join, 0));
+      hipLaunchKernelGGL(sk_bin_near, dim3(64), dim3(256), 0, st, P, D);
+    } else if (sort_due) {
+      hipLaunchKernelGGL(sk_bulk, dim3(gN), dim3(256), 0, st, P, D, P.S - 1, 0, 0);
       sk_sort(e);
       sk_near_list(e);
       hipLaunchKernelGGL(sk_near_engine, dim3(1), dim3(SK_T), 0, st, P, D, SM_RESUME);
-      hipLaunchKernelGGL(sk_bulk, dim3(gN), dim3(256), 0, st, P, D, 1, 1);
+      hipLaunchKernelGGL(sk_bulk, dim3(gN), dim3(256), 0, st, P, D, 1, 1, 0);
     } else {
       SKCHK(tmark(1, 0));
-      hipLaunchKernelGGL(sk_bulk, dim3(gN), dim3(256), 0, st, P, D, P.S, 1);
+      hipLaunchKernelGGL(sk_bulk, dim3(gN), dim3(256), 0, st, P, D, P.S, 1, 0);
       SKCHK(tmark(1, 1));
     }
     hipLaunchKernelGGL(sk_affected, dim3(1), dim3(256), 0, st, P, D);
@@ -1263,9 +1386,9 @@ extern "C" int mpcx_sk_run(mpcx_sk* e, int64_t ncoll, const double* draws, int64
     hipLaunchKernelGGL(sk_collide, dim3((unsigned)((V + CT - 1) / CT)), dim3(CT), 0, st, P, D);
     SKCHK(tmark(2, 1));
     hipLaunchKernelGGL(sk_rotate_begin, dim3(grid_for(e->nwords + 2, 256, 2048)), dim3(256), 0, st, P, D,
-                       (long long)(e->nwords + 2));
+                       ov ? 0LL : (long long)(e->nwords + 2));
     SKCHK(tmark(3, 0));
-    hipLaunchKernelGGL(sk_rotate, dim3(gN), dim3(256), 0, st, P, D);
+    hipLaunchKernelGGL(sk_rotate, dim3(gN), dim3(256), 0, st, P, D, ov ? 1 : 0);
     SKCHK(tmark(3, 1));
     sk_near_list(e);
     cstep = cnext;
@@ -1365,6 +1488,12 @@ int sk_view(mpcx_sk* e, DevView* v) {
   return 0;
 }
 }  // namespace mpcx
+
+extern "C" int mpcx_sk_set_overlap(mpcx_sk* e, int32_t on) {
+  if (!e) return set_error(MPCX_E_ARG, "null engine");
+  e->overlap = on != 0;
+  return 0;
+}
 
 extern "C" int mpcx_sk_kernel_ms(mpcx_sk* e, double* ms4, double* run_ms) {
   if (!e || !ms4) return set_error(MPCX_E_ARG, "null argument");

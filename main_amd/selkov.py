@@ -147,6 +147,12 @@ class SelkovEngine:
                                          C.c_double(zmax), ptr(a)), "mpcx_sk_body_density")
         return a
 
+    def set_overlap(self, on: bool):
+        """With device draws (rng_mode 1), stream and bin the far particles
+        on a second stream beside the near engine (the default); the states
+        are bitwise those of the one-stream order."""
+        check(lib().mpcx_sk_set_overlap(self.h, C.c_int32(1 if on else 0)), "mpcx_sk_set_overlap")
+
     KERNELS = ("sk_near_engine", "sk_bulk", "sk_collide", "sk_rotate")
 
     def kernel_ms(self) -> dict:

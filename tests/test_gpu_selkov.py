@@ -97,14 +97,18 @@ def test_selkov_engine_matches_oracle_dense_fast_reactions():
 
 
 def test_selkov_throughput_mode_properties():
-    """50^3 cells at 10 per cell (1,250,000 particles), device Philox draws:
-    particle number, flags, confinement, equipartition, bit-identical reruns."""
+    """50^3 cells at 10 per cell (1,250,000 particles), device Philox draws,
+    a sort every third interval: particle number, flags, confinement,
+    equipartition, bit-identical reruns, and bit-identical states with the far
+    particles streamed beside the near engine (overlap) and after it."""
     p = S.selkov_params(L=50.0, rho=10.0, NG=100000)
     outs = []
-    for rep in range(2):
+    for overlap in (True, True, False):
+        _say(f"[selkov] throughput mode, overlap={overlap}")
         s = S.selkov_initial_state(p, seed=17)
-        eng = SelkovEngine(p, rng_mode=1, seed=99)
+        eng = SelkovEngine(p, rng_mode=1, seed=99, sort_step=3 * p["MPCtstep"])
         try:
+            eng.set_overlap(overlap)
             eng.set_state(s, cstep=0)
             eng.run(6)
             outs.append(eng.get_state())
@@ -118,5 +122,6 @@ def test_selkov_throughput_mode_properties():
         assert np.all((g[k] >= 0) & (g[k] < L))
     T = (p["Mf"] * (g["vx"] ** 2 + g["vy"] ** 2 + g["vz"] ** 2)).sum() / (3.0 * p["NT"])
     assert abs(T / p["T"] - 1.0) < 5e-3, T
-    for k in ("x", "vx", "flag", "fx", "dm"):
-        assert np.array_equal(outs[0][k], outs[1][k]), k
+    for o in outs[1:]:
+        for k in ("x", "y", "z", "vx", "vy", "vz", "flag", "fx", "dm"):
+            assert np.array_equal(outs[0][k], o[k]), k
