@@ -2889,6 +2889,35 @@ __device__ __forceinline__ void far_max_publish(unsigned long long* dst, double 
   }
 }
 
+// the centre velocity from the ordered sums, the random axis and the record
+// (written when `write`) of the cell at tour position t
+__device__ __forceinline__ void cell_finish(const CoP& P, const CoD& D, long long t, int c, int n, double sx,
+                                            double sy, double sz, double* r6, bool write) {
+  if (n != 0) {
+    double tn = (double)n;
+    sx = sx / tn; sy = sy / tn; sz = sz / tn;
+  }
+  double u1, u2;
+  if (P.rng_mode == 0) {
+    u1 = D.U[3 + 2 * (long long)c];
+    u2 = D.U[4 + 2 * (long long)c];
+  } else {
+    philox_uniform2(P.seed, 1, D.sc->coll, (unsigned long long)c, u1, u2);
+  }
+  double phi = P.twopi * u1;
+  double th = 2.0 * u2 - 1.0;
+  double st = sqrt(1.0 - th * th);
+  double sp, cp;
+  sincos(phi, &sp, &cp);   // one reduction; the same values as sin() and cos()
+  if (write) {
+    double2* rec = (double2*)(D.cellrec + t * 8);
+    rec[0] = make_double2(sx, sy);
+    rec[1] = make_double2(sz, st * cp);
+    rec[2] = make_double2(st * sp, th);
+    rec[3] = make_double2((double)n, 0.0);
+  }
+  r6[0] = sx; r6[1] = sy; r6[2] = sz; r6[3] = st * cp; r6[4] = st * sp; r6[5] = th;
+}
 // The cell's centre-of-mass velocity from its n members idx[0], idx[istride],
 // ... (ascending particle index) summed in that order (mpc_rotation_mod.f95:
 // 153-182), its random axis (:184-193), and its record at tour position t;
@@ -2912,28 +2941,7 @@ __device__ __forceinline__ void cell_record(const CoP& P, const CoD& D, long lon
     sy = sy + D.vy[i];
     sz = sz + D.vz[i];
   }
-  if (n != 0) {
-    double tn = (double)n;
-    sx = sx / tn; sy = sy / tn; sz = sz / tn;
-  }
-  double u1, u2;
-  if (P.rng_mode == 0) {
-    u1 = D.U[3 + 2 * (long long)c];
-    u2 = D.U[4 + 2 * (long long)c];
-  } else {
-    philox_uniform2(P.seed, 1, D.sc->coll, (unsigned long long)c, u1, u2);
-  }
-  double phi = P.twopi * u1;
-  double th = 2.0 * u2 - 1.0;
-  double st = sqrt(1.0 - th * th);
-  double2* rec = (double2*)(D.cellrec + t * 8);
-  rec[0] = make_double2(sx, sy);
-  double sp, cp;
-  sincos(phi, &sp, &cp);   // one reduction; the same values as sin() and cos()
-  rec[1] = make_double2(sz, st * cp);
-  rec[2] = make_double2(st * sp, th);
-  rec[3] = make_double2((double)n, 0.0);
-  r6[0] = sx; r6[1] = sy; r6[2] = sz; r6[3] = st * cp; r6[4] = st * sp; r6[5] = th;
+  cell_finish(P, D, t, c, n, sx, sy, sz, r6, true);
 }
 // the census limit and the bulk B->A reaction of unshifted cell t
 // (reaction_mod.f95:41-62)
@@ -3175,7 +3183,6 @@ __global__ __launch_bounds__(256) void co_rotate_far(CoP P, CoD D) {
 // every unshifted cell instead (the census is complete once the near set is
 // binned, and the collisions read no species, so the two are independent)
 __global__ __launch_bounds__(CT) void co_collide_late(CoP P, CoD D, int nlate_wg) {
-  __shared__ int sidx[LB * CT];
   if ((int)blockIdx.x >= nlate_wg) {
     const long long nt = max(P.vol, P.rxnnumcell);
     const long long g = (long long)(gridDim.x - nlate_wg) * CT;
@@ -3185,29 +3192,37 @@ __global__ __launch_bounds__(CT) void co_collide_late(CoP P, CoD D, int nlate_wg
   const double tau = P.tau * (1.0 + 1e-9), shift = start_shift(P, D);
   const long long nl = min(D.sc->n_late, D.late_cap);
   double fmax2 = 0.0;
-  for (long long base = (long long)blockIdx.x * CT; base < nl; base += (long long)nlate_wg * CT) {
-    const long long k = base + threadIdx.x;
-    if (k >= nl) continue;
+  // one wave per late cell: lane j loads the cell's slab slot j and its
+  // velocity; the members' velocities go to LDS in ascending particle order,
+  // where every lane sums them in that order (the sums of cell_record); each
+  // lane rotates and classifies its own member
+  __shared__ double sv[CT / 64][3][BMAX];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (long long k = ((long long)blockIdx.x * CT + threadIdx.x) >> 6; k < nl; k += (long long)nlate_wg * (CT / 64)) {
     const int t = D.late[k];
-    int istride, n;
-    double r6[6];
-    const int* idx = slab_cell(P, D, t, sidx, istride, n, r6);
-    for (int j = 0; j < n; j += 4) {   // four members' loads in flight at a time
-      int ii[4];
-      double vx[4], vy[4], vz[4];
-#pragma unroll
-      for (int u = 0; u < 4; u++) ii[u] = j + u < n ? idx[(j + u) * istride] : -1;
-#pragma unroll
-      for (int u = 0; u < 4; u++)
-        if (ii[u] >= 0) { vx[u] = D.vx[ii[u]]; vy[u] = D.vy[ii[u]]; vz[u] = D.vz[ii[u]]; }
-#pragma unroll
-      for (int u = 0; u < 4; u++)
-        if (ii[u] >= 0) {
-          rotate_velocity(P, r6[0], r6[1], r6[2], r6[3], r6[4], r6[5], vx[u], vy[u], vz[u]);
-          D.vx[ii[u]] = vx[u]; D.vy[ii[u]] = vy[u]; D.vz[ii[u]] = vz[u];
-          classify_next(P, D.next_bits, D.dq, ii[u], vx[u], vy[u], vz[u], tau, shift, fmax2);
-        }
+    const int n = min(D.cnt[t], BMAX);   // ERR_BUCKET already raised beyond
+    const bool mine = lane < n;
+    const int v = mine ? D.slab[(long long)t * BMAX + lane] : 0x7fffffff;
+    int rank = 0;
+    for (int q = 0; q < n; q++) rank += wave_bcast(v, q) < v;
+    double vx = 0.0, vy = 0.0, vz = 0.0;
+    if (mine) {
+      vx = D.vx[v]; vy = D.vy[v]; vz = D.vz[v];
+      sv[w][0][rank] = vx; sv[w][1][rank] = vy; sv[w][2][rank] = vz;
     }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int j = 0; j < n; j++) { sx = sx + sv[w][0][j]; sy = sy + sv[w][1][j]; sz = sz + sv[w][2][j]; }
+    double r6[6];
+    cell_finish(P, D, t, D.mpc_cell[t], n, sx, sy, sz, r6, lane == 0);
+    if (mine) {
+      rotate_velocity(P, r6[0], r6[1], r6[2], r6[3], r6[4], r6[5], vx, vy, vz);
+      D.vx[v] = vx; D.vy[v] = vy; D.vz[v] = vz;
+      classify_next(P, D.next_bits, D.dq, v, vx, vy, vz, tau, shift, fmax2);
+    }
+    __builtin_amdgcn_wave_barrier();   // the sums are read before the next cell's stores
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   }
   far_max_publish(&D.sc->next_fmax2, fmax2);
 }
