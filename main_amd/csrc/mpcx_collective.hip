@@ -3694,6 +3694,31 @@ __global__ __launch_bounds__(256) void sp_bin_near(CoP P, CoD D, SpD S, int do_b
 }
 
 // co_collide in split storage: Kahan-compensated FP32 sums in ascending order
+// split storage: the mean velocity from the compensated sums, the axis and the
+// record (written when `write`) of the cell at tour position t
+__device__ __forceinline__ void sp_cell_finish(const CoP& P, const CoD& D, long long t, int c, int n, float sx,
+                                               float sy, float sz, double* r6, bool write) {
+  double mx = 0.0, my = 0.0, mz = 0.0;
+  if (n != 0) {
+    const float inv = 1.0f / (float)n;
+    mx = (double)(sx * inv); my = (double)(sy * inv); mz = (double)(sz * inv);
+  }
+  double u1, u2;
+  philox_uniform2(P.seed, 1, D.sc->coll, (unsigned long long)c, u1, u2);
+  double phi = P.twopi * u1;
+  double th = 2.0 * u2 - 1.0;
+  double st = sqrt(1.0 - th * th);
+  double sp, cp;
+  sincos(phi, &sp, &cp);   // one reduction; the same values as sin() and cos()
+  if (write) {
+    double2* rec = (double2*)(D.cellrec + t * 8);
+    rec[0] = make_double2(mx, my);
+    rec[1] = make_double2(mz, st * cp);
+    rec[2] = make_double2(st * sp, th);
+    rec[3] = make_double2((double)n, 0.0);
+  }
+  r6[0] = mx; r6[1] = my; r6[2] = mz; r6[3] = st * cp; r6[4] = st * sp; r6[5] = th;
+}
 // split storage: the cell's FP32 Kahan-compensated velocity sum over its
 // members in ascending particle order, the axis, and the record (as
 // cell_record); r6 receives the mean velocity and the axis
@@ -3707,24 +3732,7 @@ __device__ __forceinline__ void sp_cell_record(const CoP& P, const CoD& D, const
     cx = (tx - sx) - yx; cy = (ty - sy) - yy; cz = (tz - sz) - yz;
     sx = tx; sy = ty; sz = tz;
   }
-  double mx = 0.0, my = 0.0, mz = 0.0;
-  if (n != 0) {
-    const float inv = 1.0f / (float)n;
-    mx = (double)(sx * inv); my = (double)(sy * inv); mz = (double)(sz * inv);
-  }
-  double u1, u2;
-  philox_uniform2(P.seed, 1, D.sc->coll, (unsigned long long)c, u1, u2);
-  double phi = P.twopi * u1;
-  double th = 2.0 * u2 - 1.0;
-  double st = sqrt(1.0 - th * th);
-  double2* rec = (double2*)(D.cellrec + t * 8);
-  rec[0] = make_double2(mx, my);
-  double sp, cp;
-  sincos(phi, &sp, &cp);   // one reduction; the same values as sin() and cos()
-  rec[1] = make_double2(mz, st * cp);
-  rec[2] = make_double2(st * sp, th);
-  rec[3] = make_double2((double)n, 0.0);
-  r6[0] = mx; r6[1] = my; r6[2] = mz; r6[3] = st * cp; r6[4] = st * sp; r6[5] = th;
+  sp_cell_finish(P, D, t, c, n, sx, sy, sz, r6, true);
 }
 // split storage: census limits and the bulk B->A reaction of unshifted cell t
 __device__ __forceinline__ void sp_cell_react(const CoP& P, const CoD& D, const SpD& S, long long t) {
@@ -3847,34 +3855,43 @@ __global__ __launch_bounds__(256) void sp_rotate_far(CoP P, CoD D, SpD S) {
 }
 // split storage, second pass: the late cells (as co_collide_late)
 __global__ __launch_bounds__(CT) void sp_collide_late(CoP P, CoD D, SpD S) {
-  __shared__ int sidx[LB * CT];
   const double tau = P.tau * (1.0 + 1e-9), shift = start_shift(P, D);
   const long long nl = min(D.sc->n_late, D.late_cap);
   float fmax2 = 0.0f;
-  for (long long base = (long long)blockIdx.x * CT; base < nl; base += (long long)gridDim.x * CT) {
-    const long long k = base + threadIdx.x;
-    if (k >= nl) continue;
+  // one wave per late cell, as in co_collide_late: the compensated sums of
+  // sp_cell_record run over the members in ascending order from LDS
+  __shared__ float sv[CT / 64][3][BMAX];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (long long k = ((long long)blockIdx.x * CT + threadIdx.x) >> 6; k < nl; k += (long long)gridDim.x * (CT / 64)) {
     const int t = D.late[k];
-    int istride, n;
-    double r6[6];
-    const int* idx = slab_members(D, t, sidx, istride, n);
-    sp_cell_record(P, D, S, t, D.mpc_cell[t], idx, istride, n, r6);
-    for (int j = 0; j < n; j += 4) {   // four members' loads in flight at a time
-      int ii[4];
-      float vx[4], vy[4], vz[4];
-#pragma unroll
-      for (int u = 0; u < 4; u++) ii[u] = j + u < n ? idx[(j + u) * istride] : -1;
-#pragma unroll
-      for (int u = 0; u < 4; u++)
-        if (ii[u] >= 0) { vx[u] = S.vx[ii[u]]; vy[u] = S.vy[ii[u]]; vz[u] = S.vz[ii[u]]; }
-#pragma unroll
-      for (int u = 0; u < 4; u++)
-        if (ii[u] >= 0) {
-          sp_rotate_velocity(P, r6[0], r6[1], r6[2], r6[3], r6[4], r6[5], vx[u], vy[u], vz[u]);
-          S.vx[ii[u]] = vx[u]; S.vy[ii[u]] = vy[u]; S.vz[ii[u]] = vz[u];
-          sp_classify_next(P, D.next_bits, D.dq, ii[u], vx[u], vy[u], vz[u], tau, shift, fmax2);
-        }
+    const int n = min(D.cnt[t], BMAX);   // ERR_BUCKET already raised beyond
+    const bool mine = lane < n;
+    const int v = mine ? D.slab[(long long)t * BMAX + lane] : 0x7fffffff;
+    int rank = 0;
+    for (int q = 0; q < n; q++) rank += wave_bcast(v, q) < v;
+    float vx = 0.0f, vy = 0.0f, vz = 0.0f;
+    if (mine) {
+      vx = S.vx[v]; vy = S.vy[v]; vz = S.vz[v];
+      sv[w][0][rank] = vx; sv[w][1][rank] = vy; sv[w][2][rank] = vz;
     }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
+    for (int j = 0; j < n; j++) {
+      const float yx = sv[w][0][j] - cx, yy = sv[w][1][j] - cy, yz = sv[w][2][j] - cz;
+      const float tx = sx + yx, ty = sy + yy, tz = sz + yz;
+      cx = (tx - sx) - yx; cy = (ty - sy) - yy; cz = (tz - sz) - yz;
+      sx = tx; sy = ty; sz = tz;
+    }
+    double r6[6];
+    sp_cell_finish(P, D, t, D.mpc_cell[t], n, sx, sy, sz, r6, lane == 0);
+    if (mine) {
+      sp_rotate_velocity(P, r6[0], r6[1], r6[2], r6[3], r6[4], r6[5], vx, vy, vz);
+      S.vx[v] = vx; S.vy[v] = vy; S.vz[v] = vz;
+      sp_classify_next(P, D.next_bits, D.dq, v, vx, vy, vz, tau, shift, fmax2);
+    }
+    __builtin_amdgcn_wave_barrier();   // the sums are read before the next cell's stores
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   }
   sp_far_max_publish(&D.sc->next_fmax2, fmax2);
 }
