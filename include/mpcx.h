@@ -208,10 +208,14 @@ int mpcx_co_set_near_mode(mpcx_co* e, int32_t mode);
  * runs one-dimer systems (its near set split over the workgroups). */
 int mpcx_co_set_near_workgroups(mpcx_co* e, int32_t G);
 /* split = 1: split-precision solvent (the 0-based unshifted cell and FP32
- * offsets in it, FP32 velocities, 8-bit species; far particles stream, bin,
- * collide with Kahan-compensated FP32 cell sums and rotate in FP32; near
- * particles and the dimers stay FP64).  Device Philox draws only; call before
- * mpcx_co_setup.  No counterpart in the reference (FP64 throughout). */
+ * offsets in it, FP32 velocities, 8-bit species; far particles stream in one
+ * flight beside the near engine and bin, every cell's velocity sum is exact
+ * (FP32 velocities in 64-bit fixed point, 2^-40 units, integer addition: no
+ * dependence on the order of the members), particles rotate in FP32; near
+ * particles and the dimers stay FP64 in the near engine).  Device Philox
+ * draws only; call before mpcx_co_setup.  mpcx_co_run then runs the system
+ * as a batch of one (mpcx_cob_*), in any near mode.  No counterpart in the
+ * reference (FP64 throughout). */
 int mpcx_co_set_precision(mpcx_co* e, int32_t split);
 /* Workgroups of the interval's streaming/binning kernel (co_bulk, or sp_bulk
  * in split storage), a grid-stride loop over the particles: 0 (the default)
@@ -251,26 +255,40 @@ int mpcx_co_near_profile(mpcx_co* e, uint64_t* prof10);
  * is timed (at most 256 of them), since each event costs the stream a few
  * microseconds: [0] near engine, [1] co_far (or co_bulk without overlap;
  * at a re-sort these run as launches around the sort, which are not timed),
- * [2] co_collide, [3] co_rotate, [4] co_scatter; count = timed intervals. */
+ * [2] co_collide, [3] co_rotate, [4] co_scatter; count = timed intervals.
+ * Split storage: the kernels of mpcx_cob_kernel_ms. */
 int mpcx_co_kernel_ms(mpcx_co* e, double* ms5, int64_t* count);
 
 /* ------------------------------------------------------------------------
  * Batches of collective systems (no reference counterpart: the reference
- * runs one system per process).  A batch advances R engines created and set
- * up as above (rng_mode 1, same numdims, MPCtstep and sort_step; boxes,
- * parameters and keys may differ) with one launch per kernel of an interval
- * for all of them: the near engine with one workgroup per replica, the
- * particle and cell kernels over tiles of `tile` particles / 256 cells.
- * Each replica ends bit-identical to running it alone; its state is read
- * with mpcx_co_get_state.  With use_graph, the launches of an interval
- * without a re-sort are captured once into a HIP graph and replayed.
+ * runs one system per process; collective_code/main_pattern.f95:202-225 is
+ * the loop each replica runs).  A batch advances R engines created and set
+ * up as above (rng_mode 1, same numdims (1 or 2, near mode 0), MPCtstep,
+ * sort_step and storage precision; boxes, parameters and keys may differ)
+ * with one launch per kernel of an interval for all of them: the near engine
+ * with one workgroup per replica, the particle and cell kernels over tiles
+ * of `tile` particles / 256 cells.  Split-precision replicas stream and bin
+ * their far particles on a second stream beside the near engines.  Each
+ * replica ends bit-identical to running it alone (a split engine's
+ * mpcx_co_run is a batch of one); its state is read with mpcx_co_get_state.
+ * With use_graph, the launches of an interval without a re-sort are
+ * captured once into a HIP graph and replayed.
  * ---------------------------------------------------------------------- */
 typedef struct mpcx_cob mpcx_cob;
 int mpcx_cob_create(mpcx_cob** out, int32_t R, mpcx_co* const* engines, int32_t tile);
 int mpcx_cob_destroy(mpcx_cob* b);
 int mpcx_cob_run(mpcx_cob* b, int64_t ncoll, int32_t use_graph);
 int mpcx_cob_set_tile(mpcx_cob* b, int32_t tile);
+/* Collision cells per workgroup of the split-precision cell kernel (a
+ * multiple of 256; default 256).  Results do not depend on it. */
+int mpcx_cob_set_cell_tile(mpcx_cob* b, int32_t cells);
 int mpcx_cob_info(mpcx_cob* b, int64_t* particles, int64_t* cells, double* run_ms);
+/* Split batches run without the graph: mean kernel durations (ms) of the last
+ * run from HIP events on the stream each runs on (every 8th interval, at most
+ * 256 of them): [0] near engines, [1] sb_bulk (far flight and binning), [2]
+ * sb_cells (cell records, reactions), [3] sb_rotate, [4] sb_near (near set
+ * binning); *count = timed intervals. */
+int mpcx_cob_kernel_ms(mpcx_cob* b, double* ms5, int64_t* count);
 /* Advance n intervals and, after each, add every replica's body-frame B
  * counts (R, nz, nr) (frame of its dimer 0, as mpcx_co_body_density) to
  * `counts` and its propulsion velocity to vsum[r]: the time averages that
@@ -338,10 +356,15 @@ int mpcx_sk_set_overlap(mpcx_sk* e, int32_t on);
  * ---------------------------------------------------------------------- */
 /* out16: [0] particles counted, [1..3] per species (A B S / F G I), [4..6]
  * solvent momentum, [7..9] total momentum with the dimers, [10] solvent and
- * [11] dimer kinetic energy, [12] solvent temperature (2/3)KE/N, [13] dimer
- * (2/3)KE, [14] total energy with TPotential (collective_code/output_mod.f95:
- * 681-805), [15] propulsion velocity of dimer 0: centre-of-mass velocity
- * along the N->C unit vector (selkov_code/Output_SelkovDimer.f90:147-173). */
+ * [11] dimer kinetic energy, [12] solvent temperature (KE/N)*(2./3.), [13]
+ * dimer KE*(2./3.), [14] total energy with TPotential, [15] propulsion
+ * velocity of dimer 0: centre-of-mass velocity along the N->C unit vector
+ * (selkov_code/Output_SelkovDimer.f90:147-173).  2./3. is the reference's
+ * default-real literal (0.6666666865...).  Collective: collective_code/
+ * output_mod.f95:681-756, the dimers sphere by sphere.  Selkov:
+ * Output_SelkovDimer.f90:243-300, the dimer as its centre of mass (momentum
+ * (Mc+Mn)*Vdimav, kinetic energy of the centre of mass).  Solvent sums are
+ * tree sums (the reference adds in particle order). */
 int mpcx_co_observables(mpcx_co* e, double* out16);
 int mpcx_sk_observables(mpcx_sk* e, double* out16);
 /* A/B counts in the concentration slabs of collective_code/output_mod.f95:

@@ -1,8 +1,10 @@
 """Batches of collective systems advanced together on one MI355X (see
 mpcx_cob_* in include/mpcx.h): independent replicas with their own boxes,
 parameters and Philox keys, one launch per kernel of an interval for all of
-them, the normal intervals replayed from a HIP graph.  The tile size of the
-particle kernels is chosen by `autotune`, which times short runs.
+them, the normal intervals replayed from a HIP graph; FP64 or split-precision
+replicas.  The tile size of the particle kernels (and, for split replicas,
+the cells per workgroup of the cell kernel) is chosen by `autotune`, which
+times short runs.
 """
 from __future__ import annotations
 
@@ -20,6 +22,7 @@ class CollectiveBatch:
         check(lib().mpcx_cob_create(C.byref(self.h), C.c_int32(len(engines)), arr, C.c_int32(tile)),
               "mpcx_cob_create")
         self.tile = tile
+        self.cell_tile = 256
 
     def close(self):
         if self.h:
@@ -44,6 +47,19 @@ class CollectiveBatch:
         check(lib().mpcx_cob_info(self.h, C.byref(p), C.byref(c), C.byref(ms)), "mpcx_cob_info")
         return dict(particles=p.value, cells=c.value, run_ms=ms.value)
 
+    KERNELS = ("near_engine", "sb_bulk", "sb_cells", "sb_rotate", "sb_near")
+
+    def kernel_ms(self) -> dict:
+        """Mean kernel durations (ms) of the last split run without the graph."""
+        import numpy as np
+        from ._lib import ptr
+        a = np.zeros(5)
+        n = C.c_int64(0)
+        check(lib().mpcx_cob_kernel_ms(self.h, ptr(a), C.byref(n)), "mpcx_cob_kernel_ms")
+        d = dict(zip(self.KERNELS, a.tolist()))
+        d["timed_intervals"] = n.value
+        return d
+
     def body_average(self, n: int, nr=32, nz=64, rmax=12.0, zmin=-20.0, zmax=12.0):
         """Run n intervals; per replica the mean body-frame B counts (nz, nr)
         and the mean propulsion velocity over the n samples."""
@@ -57,15 +73,30 @@ class CollectiveBatch:
               "mpcx_cob_body_average")
         return counts.astype(np.float64) / max(n, 1), vsum / max(n, 1)
 
-    def autotune(self, tiles=(512, 1024, 2048, 4096, 8192), ncoll: int = 4, use_graph: bool = True) -> dict:
-        """Time `ncoll` intervals at each tile size (advancing the systems) and
-        keep the fastest; returns {tile: ms per interval}."""
+    def set_cell_tile(self, cells: int):
+        """Collision cells per workgroup of the split-precision cell kernel."""
+        check(lib().mpcx_cob_set_cell_tile(self.h, C.c_int32(cells)), "mpcx_cob_set_cell_tile")
+        self.cell_tile = cells
+
+    def autotune(self, tiles=(512, 1024, 2048, 4096, 8192), ncoll: int = 4, use_graph: bool = True,
+                 cell_tiles=None) -> dict:
+        """Time `ncoll` intervals at each tile size -- and, given cell_tiles,
+        at each number of cells per workgroup of the cell kernel -- advancing
+        the systems, and keep the fastest; returns {tile: ms per interval}, or
+        {(tile, cells): ms per interval} with cell_tiles."""
         times = {}
         for t in tiles:
             self.set_tile(t)
-            self.run(1, use_graph)          # capture / warm
-            self.run(ncoll, use_graph)
-            times[t] = self.info()["run_ms"] / ncoll
+            for c in (cell_tiles or (None,)):
+                if c is not None:
+                    self.set_cell_tile(c)
+                self.run(1, use_graph)          # capture / warm
+                self.run(ncoll, use_graph)
+                times[t if c is None else (t, c)] = self.info()["run_ms"] / ncoll
         best = min(times, key=times.get)
-        self.set_tile(best)
+        if cell_tiles:
+            self.set_tile(best[0])
+            self.set_cell_tile(best[1])
+        else:
+            self.set_tile(best)
         return times

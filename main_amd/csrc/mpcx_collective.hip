@@ -153,6 +153,11 @@ struct CoD {
   int* late;              // [late_cap] late cells of the interval (tour positions)
   unsigned* next_bits;    // the next interval's near bitmap while near_bits is still in use
   int late_cap, epoch;
+  // split storage: per collision cell (by tour position) the velocity sums in
+  // 64-bit fixed point, three words, the member count in the low 12 bits of
+  // the z word; and the cell records written from them (vcm x y z, count; axis)
+  unsigned long long* sacc;
+  float4* srec;
 };
 
 // ------------------------------------------------------------- streaming
@@ -3346,9 +3351,9 @@ __device__ __forceinline__ void sort_gather_body(const CoD& D, const int* sb, do
       bit = (D.near_bits[i >> 5] >> (i & 31)) & 1u;
     }
     unsigned long long m = __ballot(bit);   // the wave covers 64 consecutive j: two words
-    const long long w0 = (base + (threadIdx.x & ~63)) >> 5;
-    if (lane == 0) nbits[w0] = (unsigned)m;
-    if (lane == 32) nbits[w0 + 1] = (unsigned)(m >> 32);
+    const long long w0 = (base + (threadIdx.x & ~63)) >> 5, nw = (end + 31) >> 5;   // words below end
+    if (lane == 0 && w0 < nw) nbits[w0] = (unsigned)m;
+    if (lane == 32 && w0 + 1 < nw) nbits[w0 + 1] = (unsigned)(m >> 32);
   }
 }
 
@@ -3391,9 +3396,9 @@ __global__ __launch_bounds__(256) void sort_gather_bits(CoP P, CoD D, const int*
       bit = (D.near_bits[i >> 5] >> (i & 31)) & 1u;
     }
     unsigned long long m = __ballot(bit);   // the wave covers 64 consecutive j: two words
-    const long long w0 = (base + (threadIdx.x & ~63)) >> 5;
-    if (lane == 0) nbits[w0] = (unsigned)m;
-    if (lane == 32) nbits[w0 + 1] = (unsigned)(m >> 32);
+    const long long w0 = (base + (threadIdx.x & ~63)) >> 5, nw = (end + 31) >> 5;   // words below end
+    if (lane == 0 && w0 < nw) nbits[w0] = (unsigned)m;
+    if (lane == 32 && w0 + 1 < nw) nbits[w0 + 1] = (unsigned)(m >> 32);
   }
 }
 
@@ -3531,259 +3536,6 @@ __device__ __forceinline__ bool sp_stream(int& ix, int& iy, int& iz, float& ox, 
   return false;
 }
 
-// Binning of one split-storage particle per lane; every thread of the block
-// calls it together (`valid` false for idle lanes): shifted cell (mpc_rotation_mod.f95:
-// 116-145) from the cell and offsets with the particle's rank in it, the
-// distance byte to the spheres `sph`, and the census, as bin_particle.
-__device__ __forceinline__ void sp_bin_particle(const CoP& P, const CoD& D, int i, bool valid, bool near,
-                                                unsigned char st, int ix, int iy, int iz, float ox, float oy,
-                                                float oz, int f, float fxs, float fys, float fzs, double zs,
-                                                const double (*sph)[MAXSG], int slab = 0) {
-  __shared__ GroupTables G;
-  const int NT = P.NT;
-  bool ok = valid;
-  int t = 0;
-  if (valid) {
-    const int cx = sp_wrap(ix + (int)floorf(ox + fxs), P.Lx);
-    const int cy = sp_wrap(iy + (int)floorf(oy + fys), P.Ly);
-    const int cz = iz + (int)floorf(oz + fzs) + (zs < 0.0 ? 1 : 0);
-    const long long c = (long long)cx + (long long)P.Lx * cy + (long long)P.Lx * P.Ly * cz;
-    if (cz < 0 || c >= P.mpcnumcell) { raise_error(&D.sc->err, ERR_CELL_RANGE); ok = false; }
-    else t = D.mpc_pos[c];
-  }
-  unsigned key = (unsigned)i;
-  bool cen = false;
-  int ck = 0;
-  if (valid && ok) {
-    bool skip = false;
-    if (near && (st & ST_INSIDE)) { key = (unsigned)NT + (unsigned)i; skip = (st & ST_EXCL) != 0; }
-    int sp = f == 0 ? 0 : f == 1 ? 1 : f == 2 ? 2 : -1;
-    if (!skip && sp >= 0) {
-      const long long u = (long long)ix + (long long)P.Lx * iy + (long long)P.Lx * P.Ly * iz;
-      if (u < 0 || u >= P.vol) raise_error(&D.sc->err, ERR_CELL_RANGE);
-      else { cen = true; ck = (int)(sp * P.vol + u); }
-    }
-  }
-  const int r = block_groups(P, D, G, t, ok, ck, cen, key);
-  if (valid) {
-    if (ok && r >= BMAX) raise_error(&D.sc->err, ERR_BUCKET);
-    D.cell_of[i] = t;
-    if (!slab) D.rank_of[i] = ok ? (unsigned short)min(r, 65535) : (unsigned short)0xffff;
-    else if (ok && r < BMAX) {   // as bin_particle
-      D.slab[(long long)t * BMAX + r] = i;
-      if (slab == 2 && D.stamp[t] != D.epoch) raise_error(&D.sc->err, ERR_ZONE);
-    }
-    const double x = (double)ix + (double)ox, y = (double)iy + (double)oy, z = (double)iz + (double)oz;
-    double dmin = 1e30;
-    bool close = false;
-    for (int s = 0; s < 2 * P.nd; s++) {
-      double a = pbc_half(x - sph[0][s], P.box[0]);
-      double b = pbc_half(y - sph[1][s], P.box[1]);
-      double c = z - sph[2][s];
-      const double d2 = a * a + b * b + c * c, far = P.full[s >= P.nd] + 8.0;
-      close |= d2 < far * far;
-    }
-    if (__ballot(close))   // most waves lie far from every sphere: no square roots at all
-      for (int s = 0; s < 2 * P.nd; s++) {
-        double a = pbc_half(x - sph[0][s], P.box[0]);
-        double b = pbc_half(y - sph[1][s], P.box[1]);
-        double c = z - sph[2][s];
-        const double d2 = a * a + b * b + c * c, far = P.full[s >= P.nd] + 8.0;
-        if (d2 < far * far) dmin = fmin(dmin, sqrt(d2) - P.full[s >= P.nd]);
-      }
-    // 1e-5 below the measured distance: the FP32 position must not make a particle look farther
-    double qd = floor(fmax(dmin - 1e-5, 0.0) * 32.0);
-    if (st & ST_INSIDE) qd = 0.0;
-    D.dq[i] = (unsigned char)fmin(qd, 255.0);
-  }
-}
-
-// co_bulk in split storage.  Far particles take one free flight over all the
-// interval's steps: the reference streams such a particle lazily at each
-// rebuild, the same path (a far particle moves at most |v| tau, about a cell,
-// so it meets at most one wall).  FAR = true (sp_far, beside the near engine,
-// as co_far): far particles only, the flight of MPCtstep steps, distances to
-// the spheres at the interval's start; sp_bin_near bins the near set after.
-template <bool FAR>
-__device__ __forceinline__ void sp_bulk_body(const CoP& P, const CoD& D, const SpD& S, int do_stream, int do_bin,
-                                             int slab = 0) {
-  const int NT = P.NT;
-  const int nev = FAR ? 0 : D.sc->n_ev;
-  __shared__ double sph[3][MAXSG];
-  const double* sx = FAR ? D.sc->xs0 : D.sc->x0;
-  const double* sy = FAR ? D.sc->ys0 : D.sc->y0;
-  const double* sz = FAR ? D.sc->zs0 : D.sc->z0;
-  for (int j = threadIdx.x; j < 2 * P.nd; j += blockDim.x) {
-    sph[0][j] = sx[j]; sph[1][j] = sy[j]; sph[2][j] = sz[j];
-  }
-  __syncthreads();
-  int ev_total = 0;
-  if (FAR) ev_total = P.MPCtstep;
-  else
-    for (int e = 0; e < nev && e < MAXEV; e++) ev_total += D.sc->events[e];
-  double xs = 0, ys = 0, zs = 0;
-  if (do_bin) collision_shift(P, D, xs, ys, zs);
-  const float fxs = (float)xs, fys = (float)ys, fzs = (float)zs;
-  const float mdt = (float)P.MDt;
-  const long long step = (long long)gridDim.x * blockDim.x;
-  for (long long base = (long long)blockIdx.x * blockDim.x; base < NT; base += step) {
-    const int i = (int)(base + threadIdx.x);
-    const bool valid = i < NT;
-    bool near = false;
-    int ix = 0, iy = 0, iz = 0, f = -1;
-    float ox = 0, oy = 0, oz = 0;
-    unsigned char st = 0;
-    if (valid) near = (D.near_bits[i >> 5] >> (i & 31)) & 1u;
-    const bool mine = valid && !(FAR && near);
-    if (mine) {
-      st = D.status[i];
-      if (!near && st != 0) { D.status[i] = 0; st = 0; }
-      sp_cell_xyz(P, S.cell[i], ix, iy, iz);
-      ox = S.ox[i]; oy = S.oy[i]; oz = S.oz[i];
-      f = S.flag[i];
-      if (do_stream && !near && (FAR || nev > 0)) {
-        float vx = S.vx[i], vy = S.vy[i], vz = S.vz[i];
-        const int f0 = f;
-        const bool bounced = sp_stream(ix, iy, iz, ox, oy, oz, vx, vy, vz, f, mdt * (float)ev_total, P, D.react_box,
-                                       &D.sc->err);
-        S.cell[i] = ix + P.Lx * (iy + P.Ly * iz);
-        S.ox[i] = ox; S.oy[i] = oy; S.oz[i] = oz;
-        if (bounced) { S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz; }
-        if (f != f0) S.flag[i] = (signed char)f;
-      }
-    }
-    if (!do_bin) continue;
-    sp_bin_particle(P, D, i, mine, near, st, ix, iy, iz, ox, oy, oz, f, fxs, fys, fzs, zs, sph, slab);
-  }
-}
-__global__ __launch_bounds__(256) void sp_bulk(CoP P, CoD D, SpD S, int do_stream, int do_bin) {
-  sp_bulk_body<false>(P, D, S, do_stream, do_bin);
-}
-__global__ __launch_bounds__(256) void sp_far(CoP P, CoD D, SpD S, int do_bin, int slab) {
-  sp_bulk_body<true>(P, D, S, 1, do_bin, slab);
-}
-// The near set after the near engine (stored back to split storage): binning
-// with distances to the spheres at the interval's start, as sp_far; and the
-// flight check.
-__global__ __launch_bounds__(256) void sp_bin_near(CoP P, CoD D, SpD S, int do_bin, int slab) {
-  __shared__ double sph[3][MAXSG];
-  for (int j = threadIdx.x; j < 2 * P.nd; j += blockDim.x) {
-    sph[0][j] = D.sc->xs0[j]; sph[1][j] = D.sc->ys0[j]; sph[2][j] = D.sc->zs0[j];
-  }
-  __syncthreads();
-  double xs = 0, ys = 0, zs = 0;
-  if (do_bin) collision_shift(P, D, xs, ys, zs);
-  const float fxs = (float)xs, fys = (float)ys, fzs = (float)zs;
-  const long long nn = do_bin ? min(D.sc->near_num, D.near_cap) : 0;
-  for (long long base = (long long)blockIdx.x * blockDim.x; base < nn; base += (long long)gridDim.x * blockDim.x) {
-    const long long q = base + threadIdx.x;
-    const bool valid = q < nn;
-    int i = 0, ix = 0, iy = 0, iz = 0, f = -1;
-    float ox = 0, oy = 0, oz = 0;
-    unsigned char st = 0;
-    if (valid) {
-      i = D.near_list[q];
-      st = D.status[i];
-      sp_cell_xyz(P, S.cell[i], ix, iy, iz);
-      ox = S.ox[i]; oy = S.oy[i]; oz = S.oz[i];
-      f = S.flag[i];
-    }
-    sp_bin_particle(P, D, i, valid, true, st, ix, iy, iz, ox, oy, oz, f, fxs, fys, fzs, zs, sph, slab ? 2 : 0);
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) check_flight(P, D);
-}
-
-// co_collide in split storage: Kahan-compensated FP32 sums in ascending order
-// split storage: the mean velocity from the compensated sums, the axis and the
-// record (written when `write`) of the cell at tour position t
-__device__ __forceinline__ void sp_cell_finish(const CoP& P, const CoD& D, long long t, int c, int n, float sx,
-                                               float sy, float sz, double* r6, bool write) {
-  double mx = 0.0, my = 0.0, mz = 0.0;
-  if (n != 0) {
-    const float inv = 1.0f / (float)n;
-    mx = (double)(sx * inv); my = (double)(sy * inv); mz = (double)(sz * inv);
-  }
-  double u1, u2;
-  philox_uniform2(P.seed, 1, D.sc->coll, (unsigned long long)c, u1, u2);
-  double phi = P.twopi * u1;
-  double th = 2.0 * u2 - 1.0;
-  double st = sqrt(1.0 - th * th);
-  double sp, cp;
-  sincos(phi, &sp, &cp);   // one reduction; the same values as sin() and cos()
-  if (write) {
-    double2* rec = (double2*)(D.cellrec + t * 8);
-    rec[0] = make_double2(mx, my);
-    rec[1] = make_double2(mz, st * cp);
-    rec[2] = make_double2(st * sp, th);
-    rec[3] = make_double2((double)n, 0.0);
-  }
-  r6[0] = mx; r6[1] = my; r6[2] = mz; r6[3] = st * cp; r6[4] = st * sp; r6[5] = th;
-}
-// split storage: the cell's FP32 Kahan-compensated velocity sum over its
-// members in ascending particle order, the axis, and the record (as
-// cell_record); r6 receives the mean velocity and the axis
-__device__ __forceinline__ void sp_cell_record(const CoP& P, const CoD& D, const SpD& S, long long t, int c,
-                                               const int* idx, int istride, int n, double* r6) {
-  float sx = 0.0f, sy = 0.0f, sz = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
-  for (int j = 0; j < n; j++) {
-    const int i = idx[j * istride];
-    const float yx = S.vx[i] - cx, yy = S.vy[i] - cy, yz = S.vz[i] - cz;
-    const float tx = sx + yx, ty = sy + yy, tz = sz + yz;
-    cx = (tx - sx) - yx; cy = (ty - sy) - yy; cz = (tz - sz) - yz;
-    sx = tx; sy = ty; sz = tz;
-  }
-  sp_cell_finish(P, D, t, c, n, sx, sy, sz, r6, true);
-}
-// split storage: census limits and the bulk B->A reaction of unshifted cell t
-__device__ __forceinline__ void sp_cell_react(const CoP& P, const CoD& D, const SpD& S, long long t) {
-  if (t < P.vol)   // the census totals of the binning kernel against the reference's MaxA/MaxB/MaxS
-    if (D.tcount[t] > CENSUS_MAX || D.tcount[P.vol + t] > CENSUS_MAX || D.tcount[2 * P.vol + t] > CENSUS_MAX)
-      raise_error(&D.sc->err, ERR_MEMBERS);
-  if (t < P.rxnnumcell) {
-    int nb = D.tcount[1 * P.vol + t];
-    double prob = D.ptab[min(nb, CENSUS_MAX)];
-    double r, q;
-    philox_uniform2(P.seed, 2, D.sc->coll, (unsigned long long)t, r, q);
-    if (0.0 < r && r <= prob) {
-      unsigned key = D.firstB[t];
-      int i = key >= (unsigned)P.NT ? (int)(key - (unsigned)P.NT) : (int)key;
-      S.flag[i] = 0;
-    }
-  }
-}
-__global__ __launch_bounds__(CT) void sp_collide(CoP P, CoD D, SpD S) {
-  __shared__ int sidx[LB * CT];
-  const long long t = (long long)xcd_block(blockIdx.x, gridDim.x) * CT + threadIdx.x;
-  if (t < P.mpcnumcell) {
-    const int c = D.mpc_cell[t];
-    int n = D.cnt[t];
-    if (n > BMAX) n = BMAX;
-    const int* b = D.bucket + D.off[t];
-    int* idx;
-    int istride;
-    if (n <= LB) {
-      idx = sidx + threadIdx.x;
-      istride = CT;
-      for (int j = 0; j < n; j++) {
-        int v = b[j], k = j;
-        while (k > 0 && idx[(k - 1) * CT] > v) { idx[k * CT] = idx[(k - 1) * CT]; k--; }
-        idx[k * CT] = v;
-      }
-    } else {
-      idx = (int*)b;
-      istride = 1;
-      for (int j = 1; j < n; j++) {
-        int v = idx[j], k = j;
-        while (k > 0 && idx[k - 1] > v) { idx[k] = idx[k - 1]; k--; }
-        idx[k] = v;
-      }
-    }
-    double r6[6];
-    sp_cell_record(P, D, S, t, c, idx, istride, n, r6);
-  }
-  sp_cell_react(P, D, S, t);
-}
-
 // co_rotate in split storage (FP32 rotation) and the next near set
 // split storage: the rotation of co_rotate in FP32 with the record's mean and axis
 __device__ __forceinline__ void sp_rotate_velocity(const CoP& P, double rcx, double rcy, double rcz, double rux,
@@ -3797,144 +3549,9 @@ __device__ __forceinline__ void sp_rotate_velocity(const CoP& P, double rcx, dou
   const float rz = (dx * ((ux * uz) * omc - uy * sr) + dy * ((uy * uz) * omc + ux * sr)) + dz * (cr + (uz * uz) * omc);
   vx = cx + rx; vy = cy + ry; vz = cz + rz;
 }
-__device__ __forceinline__ void sp_classify_next(const CoP& P, unsigned* bits, const unsigned char* dq, int i,
-                                                 float vx, float vy, float vz, double tau, double shift,
-                                                 float& fmax2) {
-  const float v2 = (vx * vx + vy * vy) + vz * vz;
-  const double reach = sqrt((double)v2) * tau + P.guard + 1e-6 + shift;
-  if ((double)dq[i] * (1.0 / 32.0) <= reach) atomicOr(&bits[i >> 5], 1u << (i & 31));
-  else fmax2 = fmaxf(fmax2, v2);
-}
 // FP32 velocities: the far maximum is raised by a relative 1e-6 so time_penetrate stays conservative
 __device__ __forceinline__ void sp_far_max_publish(unsigned long long* dst, float fmax2) {
   far_max_publish(dst, (double)fmax2 * (1.0 + 1e-6));
-}
-// classify as rotate_body (1 or 2)
-__global__ __launch_bounds__(256) void sp_rotate(CoP P, CoD D, SpD S, int classify) {
-  float fmax2 = 0.0f;
-  const double tau = P.tau * (1.0 + 1e-9);
-  const double shift = classify == 2 ? start_shift(P, D) : 0.0;
-  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < P.NT; i += (long long)gridDim.x * 256) {
-    const int t = D.cell_of[i];
-    const double2* rec = (const double2*)(D.cellrec + (long long)t * 8);
-    const double2 r0 = rec[0], r1 = rec[1], r2 = rec[2];
-    float vx = S.vx[i], vy = S.vy[i], vz = S.vz[i];
-    sp_rotate_velocity(P, r0.x, r0.y, r1.x, r1.y, r2.x, r2.y, vx, vy, vz);
-    S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz;
-    sp_classify_next(P, D.near_bits, D.dq, (int)i, vx, vy, vz, tau, shift, fmax2);
-  }
-  sp_far_max_publish(&D.sc->far_maxv2, fmax2);
-}
-// split storage, first collision pass beside the near engine (as
-// co_collide_far / co_rotate_far)
-__global__ __launch_bounds__(CT) void sp_collide_far(CoP P, CoD D, SpD S) {
-  __shared__ int sidx[LB * CT];
-  const long long t = (long long)xcd_block(blockIdx.x, gridDim.x) * CT + threadIdx.x;
-  if (t < P.mpcnumcell && D.stamp[t] != D.epoch) {
-    int istride, n;
-    double r6[6];
-    const int* idx = slab_members(D, (int)t, sidx, istride, n);
-    sp_cell_record(P, D, S, t, D.mpc_cell[t], idx, istride, n, r6);
-  }
-}
-__global__ __launch_bounds__(256) void sp_rotate_far(CoP P, CoD D, SpD S) {
-  float fmax2 = 0.0f;
-  const double tau = P.tau * (1.0 + 1e-9);
-  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < P.NT; i += (long long)gridDim.x * 256) {
-    if ((D.near_bits[i >> 5] >> (i & 31)) & 1u) continue;
-    const int t = D.cell_of[i];
-    if (D.stamp[t] == D.epoch) continue;
-    const double2* rec = (const double2*)(D.cellrec + (long long)t * 8);
-    const double2 r0 = rec[0], r1 = rec[1], r2 = rec[2];
-    float vx = S.vx[i], vy = S.vy[i], vz = S.vz[i];
-    sp_rotate_velocity(P, r0.x, r0.y, r1.x, r1.y, r2.x, r2.y, vx, vy, vz);
-    S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz;
-    sp_classify_next(P, D.next_bits, D.dq, (int)i, vx, vy, vz, tau, P.guard, fmax2);
-  }
-  sp_far_max_publish(&D.sc->next_fmax2, fmax2);
-}
-// split storage, second pass: the late cells (as co_collide_late)
-__global__ __launch_bounds__(CT) void sp_collide_late(CoP P, CoD D, SpD S) {
-  const double tau = P.tau * (1.0 + 1e-9), shift = start_shift(P, D);
-  const long long nl = min(D.sc->n_late, D.late_cap);
-  float fmax2 = 0.0f;
-  // one wave per late cell, as in co_collide_late: the compensated sums of
-  // sp_cell_record run over the members in ascending order from LDS
-  __shared__ float sv[CT / 64][3][BMAX];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (long long k = ((long long)blockIdx.x * CT + threadIdx.x) >> 6; k < nl; k += (long long)gridDim.x * (CT / 64)) {
-    const int t = D.late[k];
-    const int n = min(D.cnt[t], BMAX);   // ERR_BUCKET already raised beyond
-    const bool mine = lane < n;
-    const int v = mine ? D.slab[(long long)t * BMAX + lane] : 0x7fffffff;
-    int rank = 0;
-    for (int q = 0; q < n; q++) rank += wave_bcast(v, q) < v;
-    float vx = 0.0f, vy = 0.0f, vz = 0.0f;
-    if (mine) {
-      vx = S.vx[v]; vy = S.vy[v]; vz = S.vz[v];
-      sv[w][0][rank] = vx; sv[w][1][rank] = vy; sv[w][2][rank] = vz;
-    }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    float sx = 0.0f, sy = 0.0f, sz = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
-    for (int j = 0; j < n; j++) {
-      const float yx = sv[w][0][j] - cx, yy = sv[w][1][j] - cy, yz = sv[w][2][j] - cz;
-      const float tx = sx + yx, ty = sy + yy, tz = sz + yz;
-      cx = (tx - sx) - yx; cy = (ty - sy) - yy; cz = (tz - sz) - yz;
-      sx = tx; sy = ty; sz = tz;
-    }
-    double r6[6];
-    sp_cell_finish(P, D, t, D.mpc_cell[t], n, sx, sy, sz, r6, lane == 0);
-    if (mine) {
-      sp_rotate_velocity(P, r6[0], r6[1], r6[2], r6[3], r6[4], r6[5], vx, vy, vz);
-      S.vx[v] = vx; S.vy[v] = vy; S.vz[v] = vz;
-      sp_classify_next(P, D.next_bits, D.dq, v, vx, vy, vz, tau, shift, fmax2);
-    }
-    __builtin_amdgcn_wave_barrier();   // the sums are read before the next cell's stores
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-  }
-  sp_far_max_publish(&D.sc->next_fmax2, fmax2);
-}
-__global__ __launch_bounds__(256) void sp_react(CoP P, CoD D, SpD S) {
-  const long long nt = max(P.vol, P.rxnnumcell);
-  for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < nt; t += (long long)gridDim.x * 256)
-    sp_cell_react(P, D, S, t);
-}
-
-// Hilbert re-sort of the split arrays (sort_hilbert_mod.f95:50-92: stable bin
-// by the tour position of the unshifted cell, which split storage holds)
-__global__ __launch_bounds__(256) void sp_sort_bin(CoP P, CoD D, SpD S, const int* tourpos, int* scnt, int* key,
-                                                   unsigned short* rk) {
-  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < P.NT; i += (long long)gridDim.x * 256) {
-    const long long u = S.cell[i];
-    if (u < 0 || u >= P.vol || tourpos[u] < 0) { raise_error(&D.sc->err, ERR_CELL_RANGE); key[i] = 0; rk[i] = 0; continue; }
-    const int t = tourpos[u];
-    const int r = atomicAdd(&scnt[t], 1);
-    if (r >= BMAX) raise_error(&D.sc->err, ERR_BUCKET);
-    key[i] = t;
-    rk[i] = (unsigned short)min(r, 65535);
-  }
-}
-__global__ __launch_bounds__(256) void sp_sort_gather(CoP P, CoD D, SpD S, SpD N, const int* sb, unsigned char* nst,
-                                                      unsigned* nbits) {
-  long long first, end;
-  block_range(P.NT, gridDim.x, xcd_block(blockIdx.x, gridDim.x), first, end);
-  const int lane = threadIdx.x & 63;
-  for (long long base = first; base < end; base += 256) {
-    const long long j = base + threadIdx.x;
-    bool bit = false;
-    if (j < end) {
-      const int i = sb[j];
-      N.cell[j] = S.cell[i]; N.ox[j] = S.ox[i]; N.oy[j] = S.oy[i]; N.oz[j] = S.oz[i];
-      N.vx[j] = S.vx[i]; N.vy[j] = S.vy[i]; N.vz[j] = S.vz[i]; N.flag[j] = S.flag[i];
-      nst[j] = D.status[i];
-      bit = (D.near_bits[i >> 5] >> (i & 31)) & 1u;
-    }
-    unsigned long long m = __ballot(bit);
-    const long long w0 = (base + (threadIdx.x & ~63)) >> 5;
-    if (lane == 0) nbits[w0] = (unsigned)m;
-    if (lane == 32) nbits[w0 + 1] = (unsigned)(m >> 32);
-  }
 }
 }  // namespace mpcx
 
@@ -3993,6 +3610,7 @@ struct mpcx_co {
   // split precision: solvent in cell + FP32 offsets / FP32 velocities / 8-bit species
   bool split = false;
   SpD S, Salt;
+  mpcx_cob* solo = nullptr;   // split runs: this engine as a batch of one (made on the first run)
   double* d_rsum = nullptr;   // [4][MAXSG] per-sphere totals of the restart evaluation
   int g_bulk = 0, g_spbulk = 0;   // co_bulk / sp_bulk workgroups (0: not chosen yet)
   bool cells_valid = false;   // a collision has binned the particles since the last upload
@@ -4299,6 +3917,12 @@ extern "C" int mpcx_co_set_precision(mpcx_co* e, int32_t split) {
       if (!q->cell || !q->ox || !q->oy || !q->oz || !q->vx || !q->vy || !q->vz || !q->flag)
         return set_error(MPCX_E_HIP, "device allocation failed");
     }
+    const size_t mc = (size_t)e->P.mpcnumcell;
+    e->D.sacc = dalloc<unsigned long long>(e, 3 * mc);
+    e->D.srec = dalloc<float4>(e, 2 * mc);
+    if (!e->D.sacc || !e->D.srec) return set_error(MPCX_E_HIP, "device allocation failed");
+    HIPCHK(hipMemset(e->D.sacc, 0, 3 * mc * 8));
+    HIPCHK(hipMemset(e->D.srec, 0, 2 * mc * 16));
   }
   e->split = split != 0;
   return 0;
@@ -4327,6 +3951,7 @@ extern "C" int mpcx_co_set_near_mode(mpcx_co* e, int32_t mode) {
 
 extern "C" int mpcx_co_destroy(mpcx_co* e) {
   if (!e) return 0;
+  if (e->solo) { (void)mpcx_cob_destroy(e->solo); e->solo = nullptr; }
   if (e->st) (void)hipStreamSynchronize(e->st);
   if (e->st2) (void)hipStreamSynchronize(e->st2);
   for (void* p : e->allocs)
@@ -4434,29 +4059,6 @@ static int co_sort(mpcx_co* e, hipStream_t st) {
   std::swap(D.x, e->alt[0]); std::swap(D.y, e->alt[1]); std::swap(D.z, e->alt[2]);
   std::swap(D.vx, e->alt[3]); std::swap(D.vy, e->alt[4]); std::swap(D.vz, e->alt[5]);
   std::swap(D.flag, e->alt_flag); std::swap(D.status, e->alt_status); std::swap(D.near_bits, e->alt_bits);
-  return 0;
-}
-
-static int sp_sort(mpcx_co* e, hipStream_t st) {
-  CoD& D = e->D;
-  CoP& P = e->P;
-  const long long NT = P.NT, vol = P.vol;
-  if (!e->has_tour) return set_error(MPCX_E_ARG, "a sort step needs the cell tour (pass it to mpcx_co_set_state)");
-  hipLaunchKernelGGL(fill_kernel<int>, dim3(grid_for(vol, 256)), dim3(256), 0, st, e->d_scnt, vol, 0);
-  hipLaunchKernelGGL(sp_sort_bin, dim3(grid_for(NT, 256)), dim3(256), 0, st, P, D, e->S, (const int*)e->d_tourpos,
-                     e->d_scnt, e->d_skey, e->d_srk);
-  launch_scan(st, e->d_scnt, vol, D.partial, e->d_soff);
-  hipLaunchKernelGGL(fill_kernel<int>, dim3(grid_for(NT, 256)), dim3(256), 0, st, e->d_sb, NT, 0);
-  hipLaunchKernelGGL(sort_scatter, dim3(grid_for(NT, 256)), dim3(256), 0, st, (int)NT, (const int*)e->d_skey,
-                     (const unsigned short*)e->d_srk, (const int*)e->d_soff, e->d_sb);
-  hipLaunchKernelGGL(sort_order, dim3((unsigned)((vol + 255) / 256)), dim3(256), 0, st, vol, (const int*)e->d_scnt,
-                     (const int*)e->d_soff, e->d_sb, BMAX);
-  hipLaunchKernelGGL(fill_kernel<unsigned>, dim3(grid_for(e->nwords + 1, 256)), dim3(256), 0, st, e->alt_bits,
-                     e->nwords + 1, 0u);
-  hipLaunchKernelGGL(sp_sort_gather, dim3(grid_for(NT, 256)), dim3(256), 0, st, P, D, e->S, e->Salt,
-                     (const int*)e->d_sb, e->alt_status, e->alt_bits);
-  std::swap(e->S, e->Salt);
-  std::swap(D.status, e->alt_status); std::swap(D.near_bits, e->alt_bits);
   return 0;
 }
 
@@ -4742,8 +4344,11 @@ extern "C" int mpcx_co_set_overlap(mpcx_co* e, int32_t on) {
   return 0;
 }
 
+static int co_run_split(mpcx_co* e, int64_t ncoll);
+
 extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
   if (!e || ncoll < 0) return set_error(MPCX_E_ARG, "bad arguments");
+  if (e->split) return co_run_split(e, ncoll);   // the split-precision interval (one-system batch)
   {
     int cs = 0;
     HIPCHK(hipMemcpy(&cs, &e->D.sc->cstep, sizeof(int), hipMemcpyDeviceToHost));
@@ -4794,7 +4399,6 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
       co_near_list(e);
       e->list_due = false;
     }
-    if (e->split) sp_convert(e, true, true);   // the near set in FP64 for the near engine
     // the far particles' flight and binning beside the near engine
     const bool ov = e->overlap && e->near_mode == 0;
     if (ov) {
@@ -4802,20 +4406,16 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
       HIPCHK(hipEventRecord(e->fork, st));
       HIPCHK(hipStreamWaitEvent(s2, e->fork, 0));
       if (!sort_due) HIPCHK(tmark(1, 0, s2));
-      if (e->split)
-        hipLaunchKernelGGL(sp_far, dim3(e->g_spbulk), dim3(256), 0, s2, P, D, e->S, sort_due ? 0 : 1, two ? 1 : 0);
-      else hipLaunchKernelGGL(co_far, dim3(e->g_bulk), dim3(256), 0, s2, P, D, sort_due ? 0 : 1, two ? 1 : 0);
+      hipLaunchKernelGGL(co_far, dim3(e->g_bulk), dim3(256), 0, s2, P, D, sort_due ? 0 : 1, two ? 1 : 0);
       if (!sort_due) HIPCHK(tmark(1, 1, s2));
       if (two) {   // the late cells around the spheres; the first collision pass
         hipLaunchKernelGGL(co_zone_mark, dim3(64, 2 * e->nd), dim3(256), 0, s2, P, D);
         HIPCHK(hipEventRecord(e->far_done, s2));
         HIPCHK(tmark(2, 0, s2));
-        if (e->split) hipLaunchKernelGGL(sp_collide_far, dim3((unsigned)((mc + CT - 1) / CT)), dim3(CT), 0, s2, P, D, e->S);
-        else hipLaunchKernelGGL(co_collide_far, dim3((unsigned)((mc + CT - 1) / CT)), dim3(CT), 0, s2, P, D);
+        hipLaunchKernelGGL(co_collide_far, dim3((unsigned)((mc + CT - 1) / CT)), dim3(CT), 0, s2, P, D);
         HIPCHK(tmark(2, 1, s2));
         HIPCHK(tmark(3, 0, s2));
-        if (e->split) hipLaunchKernelGGL(sp_rotate_far, dim3(gN), dim3(256), 0, s2, P, D, e->S);
-        else hipLaunchKernelGGL(co_rotate_far, dim3(gN), dim3(256), 0, s2, P, D);
+        hipLaunchKernelGGL(co_rotate_far, dim3(gN), dim3(256), 0, s2, P, D);
         HIPCHK(tmark(3, 1, s2));
       }
       HIPCHK(hipEventRecord(e->join, s2));
@@ -4824,36 +4424,24 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
     HIPCHK(tmark(0, 0, st));
     HIPCHK(launch_near(e, P.MPCtstep, sort_due ? M_STOP : 0));
     HIPCHK(tmark(0, 1, st));
-    if (e->split) sp_convert(e, false, true);
     if (last) HIPCHK(hipEventRecord(e->n1, st));
     if (two) {   // the second pass needs co_far's binning only, not the first pass
       if (last) HIPCHK(hipEventRecord(e->b0, st));
       HIPCHK(hipStreamWaitEvent(st, e->far_done, 0));
-      if (e->split) {
-        hipLaunchKernelGGL(sp_bin_near, dim3(64), dim3(256), 0, st, P, D, e->S, 1, 1);
-        hipLaunchKernelGGL(sp_react, dim3(grid_for(vol, 256, 2048)), dim3(256), 0, st, P, D, e->S);
-        hipLaunchKernelGGL(sp_collide_late, dim3(512), dim3(CT), 0, st, P, D, e->S);
-      } else {
-        hipLaunchKernelGGL(co_bin_rest, dim3(128), dim3(256), 0, st, P, D, 1, 1);
-        hipLaunchKernelGGL(co_collide_late, dim3(512 + 1024), dim3(CT), 0, st, P, D, 512);   // + reactions
-      }
+      hipLaunchKernelGGL(co_bin_rest, dim3(128), dim3(256), 0, st, P, D, 1, 1);
+      hipLaunchKernelGGL(co_collide_late, dim3(512 + 1024), dim3(CT), 0, st, P, D, 512);   // + reactions
     }
     if (ov) HIPCHK(hipStreamWaitEvent(st, e->join, 0));
-    const bool sp = e->split;
     auto bulk = [&](int stream, int bin) {
-      if (sp) hipLaunchKernelGGL(sp_bulk, dim3(e->g_spbulk), dim3(256), 0, st, P, D, e->S, stream, bin);
-      else hipLaunchKernelGGL(co_bulk, dim3(e->g_bulk), dim3(256), 0, st, P, D, stream, bin);
+      hipLaunchKernelGGL(co_bulk, dim3(e->g_bulk), dim3(256), 0, st, P, D, stream, bin);
     };
     if (sort_due) {
-      if (ov && sp) hipLaunchKernelGGL(sp_bin_near, dim3(1), dim3(256), 0, st, P, D, e->S, 0, 0);   // the flight check
-      else if (ov) hipLaunchKernelGGL(co_bin_rest, dim3(128), dim3(256), 0, st, P, D, 0, 0);
+      if (ov) hipLaunchKernelGGL(co_bin_rest, dim3(128), dim3(256), 0, st, P, D, 0, 0);   // the flight check
       else bulk(1, 0);
-      int r = sp ? sp_sort(e, st) : co_sort(e, st);
+      int r = co_sort(e, st);
       if (r) return r;
       co_near_list(e);
-      if (sp) sp_convert(e, true, true);
       HIPCHK(launch_near(e, 1, M_RESUME));
-      if (sp) sp_convert(e, false, true);
       if (last) HIPCHK(hipEventRecord(e->b0, st));
       bulk(0, 1);
     } else if (two) {   // after both passes: the next interval's near set (its list and
@@ -4865,8 +4453,7 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
       continue;
     } else if (ov) {
       if (last) HIPCHK(hipEventRecord(e->b0, st));
-      if (sp) hipLaunchKernelGGL(sp_bin_near, dim3(64), dim3(256), 0, st, P, D, e->S, 1, 0);
-      else hipLaunchKernelGGL(co_bin_rest, dim3(128), dim3(256), 0, st, P, D, 1, 0);
+      hipLaunchKernelGGL(co_bin_rest, dim3(128), dim3(256), 0, st, P, D, 1, 0);
     } else {
       if (last) HIPCHK(hipEventRecord(e->b0, st));
       HIPCHK(tmark(1, 0, st));
@@ -4876,20 +4463,18 @@ extern "C" int mpcx_co_run(mpcx_co* e, int64_t ncoll, const double* draws) {
     // with two-pass intervals the timed collide and rotate are their first
     // passes; the re-sort intervals' single pass is left out of the means
     auto tmark1 = [&](int k, int edge) -> hipError_t {
-      return (e->overlap && !e->split && e->near_mode == 0) ? hipSuccess : tmark(k, edge, st);
+      return (e->overlap && e->near_mode == 0) ? hipSuccess : tmark(k, edge, st);
     };
     launch_scan(st, D.cnt, mc, D.partial, D.off);
     HIPCHK(tmark1(4, 0));
     hipLaunchKernelGGL(co_scatter, dim3(gN), dim3(256), 0, st, P, D);
     HIPCHK(tmark1(4, 1));
     HIPCHK(tmark1(2, 0));
-    if (sp) hipLaunchKernelGGL(sp_collide, dim3((unsigned)((mc + CT - 1) / CT)), dim3(CT), 0, st, P, D, e->S);
-    else hipLaunchKernelGGL(co_collide, dim3((unsigned)((mc + CT - 1) / CT)), dim3(CT), 0, st, P, D);
+    hipLaunchKernelGGL(co_collide, dim3((unsigned)((mc + CT - 1) / CT)), dim3(CT), 0, st, P, D);
     HIPCHK(tmark1(2, 1));
     hipLaunchKernelGGL(rotate_begin, dim3(grid_for(e->nwords + 1, 256, 2048)), dim3(256), 0, st, D, e->nwords);
     HIPCHK(tmark1(3, 0));
-    if (sp) hipLaunchKernelGGL(sp_rotate, dim3(gN), dim3(256), 0, st, P, D, e->S, ov && !sort_due ? 2 : 1);
-    else hipLaunchKernelGGL(co_rotate, dim3(gN), dim3(256), 0, st, P, D, ov && !sort_due ? 2 : 1);
+    hipLaunchKernelGGL(co_rotate, dim3(gN), dim3(256), 0, st, P, D, ov && !sort_due ? 2 : 1);
     HIPCHK(tmark1(3, 1));
     if (last) HIPCHK(hipEventRecord(e->b1, st));
     co_near_list(e);
@@ -5277,7 +4862,15 @@ extern "C" int mpcx_co_get_cells(mpcx_co* e, double* vcm_uv, int32_t* cell_of, i
     return set_error(MPCX_E_ARG, "no collision has run since the state was uploaded: the cell arrays are empty");
   CoD& D = e->D;
   const size_t mc = (size_t)e->P.mpcnumcell;
-  if (vcm_uv) {   // stored by tour position; returned by cell
+  if (vcm_uv && e->split) {   // FP32 records (vcm, count; axis) by tour position
+    std::vector<float> rec(mc * 8);
+    HIPCHK(hipMemcpy(rec.data(), D.srec, mc * 32, hipMemcpyDeviceToHost));
+    for (size_t t = 0; t < mc; t++) {
+      double* o = vcm_uv + (size_t)e->mpc_cell[t] * 8;
+      const float* r = &rec[t * 8];
+      o[0] = r[0]; o[1] = r[1]; o[2] = r[2]; o[3] = r[4]; o[4] = r[5]; o[5] = r[6]; o[6] = r[3]; o[7] = 0.0;
+    }
+  } else if (vcm_uv) {   // stored by tour position; returned by cell
     std::vector<double> rec(mc * 8);
     HIPCHK(hipMemcpy(rec.data(), D.cellrec, mc * 64, hipMemcpyDeviceToHost));
     for (size_t t = 0; t < mc; t++) memcpy(vcm_uv + (size_t)e->mpc_cell[t] * 8, &rec[t * 8], 64);
@@ -5289,6 +4882,7 @@ extern "C" int mpcx_co_get_cells(mpcx_co* e, double* vcm_uv, int32_t* cell_of, i
       cell_of[i] = e->mpc_cell[cell_of[i]];
     }
   }
+  if (census && e->split) return set_error(MPCX_E_UNSUPPORTED, "split storage clears its census after each collision");
   if (census) HIPCHK(hipMemcpy(census, D.tcount, (size_t)e->P.vol * 12, hipMemcpyDeviceToHost));
   if (tpotential) HIPCHK(hipMemcpy(tpotential, &D.sc->tpotential, 8, hipMemcpyDeviceToHost));
   return 0;
@@ -5520,6 +5114,422 @@ __global__ __launch_bounds__(256) void sort_gather_b(const CoD* Ds, const CoSort
                    t.begin, t.end, 256);
 }
 
+
+// =====================================================================================
+// Split-precision intervals (one system, or a batch of systems)
+//
+// The far particles take one free flight of the interval and are binned on a
+// second stream while the near engines run; the near set is binned when they
+// are done; every collision cell is then collided and every particle rotated.
+// The per-cell velocity sums are exact: each FP32 velocity is scaled by 2^40
+// (an integer for |v| >= 2^-17; below that the rounding is under 2^-41) and
+// added in 64-bit integer arithmetic, grouped per workgroup in LDS and then
+// added with one global atomic per word and distinct cell of the workgroup.
+// Integer sums do not depend on the order the members arrive in, so the far
+// particles need not wait for the near set, no cell is collided twice, and a
+// system's results do not depend on which workgroups ran it.  The census
+// (unshifted cells, distribute_mod.f95:95-183) is grouped the same way, and
+// the bulk B->A reaction (reaction_mod.f95:28-66) runs per unshifted cell from
+// it.  Philox streams: 0 the grid shift, 1 the axes (counter: collision cell),
+// 2 the reactions (counter: unshifted cell), keyed by the system's seed.
+// =====================================================================================
+constexpr double SB_SCALE = 0x1p40;   // fixed point of the cell sums
+constexpr int SB_NBITS = 12;          // count bits below the z sum (at most 4095 members)
+
+__device__ __forceinline__ long long sb_fix(float v) { return __double2ll_rn((double)v * SB_SCALE); }
+
+// per-workgroup hash tables of the binning (as GroupTables): cell sums, census counts and minimum keys
+struct SbTables {
+  int rk[GT], rc[GT];
+  unsigned long long rs[3][GT];
+  int ck[GT], cc[GT];
+  unsigned cm[GT];
+};
+__device__ __forceinline__ void sb_tables_init(SbTables& G) {
+  for (int j = threadIdx.x; j < GT; j += blockDim.x) {
+    G.rk[j] = -1; G.rc[j] = 0; G.rs[0][j] = 0ull; G.rs[1][j] = 0ull; G.rs[2][j] = 0ull;
+    G.ck[j] = -1; G.cc[j] = 0; G.cm[j] = 0xffffffffu;
+  }
+  __syncthreads();
+}
+// Every thread of the block together: each lane's velocity into its cell t
+// (ok), its census entry ck with key (cen); the block's groups are added to
+// the global sums and the tables left cleared for the next round.
+__device__ __forceinline__ void sb_groups(const CoP& P, const CoD& D, SbTables& G, bool ok, int t, float vx, float vy,
+                                          float vz, bool cen, int ck, unsigned key) {
+  if (ok) {
+    const int j = gt_slot(G.rk, t);
+    atomicAdd(&G.rc[j], 1);
+    atomicAdd(&G.rs[0][j], (unsigned long long)sb_fix(vx));
+    atomicAdd(&G.rs[1][j], (unsigned long long)sb_fix(vy));
+    atomicAdd(&G.rs[2][j], ((unsigned long long)sb_fix(vz) << SB_NBITS) + 1ull);
+  }
+  if (cen) {
+    const int j = gt_slot(G.ck, ck);
+    atomicAdd(&G.cc[j], 1);
+    atomicMin(&G.cm[j], key);
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < GT; j += blockDim.x) {
+    if (G.rc[j]) {
+      unsigned long long* a = D.sacc + 3ll * G.rk[j];
+      atomicAdd(&a[0], G.rs[0][j]);
+      atomicAdd(&a[1], G.rs[1][j]);
+      atomicAdd(&a[2], G.rs[2][j]);
+      G.rk[j] = -1; G.rc[j] = 0; G.rs[0][j] = 0ull; G.rs[1][j] = 0ull; G.rs[2][j] = 0ull;
+    }
+    if (G.cc[j]) {
+      const int k = G.ck[j];
+      atomicAdd(&D.tcount[k], G.cc[j]);   // checked against CENSUS_MAX by sb_cells
+      if (k >= P.vol && k < 2 * P.vol) atomicMin(&D.firstB[k - P.vol], G.cm[j]);
+      G.ck[j] = -1; G.cc[j] = 0; G.cm[j] = 0xffffffffu;
+    }
+  }
+  __syncthreads();
+}
+
+// Binning of one split-storage particle per lane (every thread of the block
+// together, `valid` false for idle lanes): its shifted cell (mpc_rotation_mod.f95:
+// 116-145 in cell coordinates), its velocity into that cell's sums, its census
+// entry (outside-list particles keyed by index, inside ones after them, inside
+// ones within an LJ cutoff left out), and the distance byte to the spheres sph
+// (the selection of the next near set).
+__device__ __forceinline__ void sb_bin(const CoP& P, const CoD& D, SbTables& G, int i, bool valid, bool inside,
+                                       bool excl, int ix, int iy, int iz, float ox, float oy, float oz, float vx,
+                                       float vy, float vz, int f, float fxs, float fys, float fzs, double zs,
+                                       const double (*sph)[MAXSG]) {
+  bool ok = valid;
+  int t = 0;
+  if (valid) {
+    const int cx = sp_wrap(ix + (int)floorf(ox + fxs), P.Lx);
+    const int cy = sp_wrap(iy + (int)floorf(oy + fys), P.Ly);
+    const int cz = iz + (int)floorf(oz + fzs) + (zs < 0.0 ? 1 : 0);
+    const long long c = (long long)cx + (long long)P.Lx * cy + (long long)P.Lx * P.Ly * cz;
+    if (cz < 0 || c >= P.mpcnumcell) { raise_error(&D.sc->err, ERR_CELL_RANGE); ok = false; }
+    else t = D.mpc_pos[c];
+  }
+  unsigned key = (unsigned)i;
+  bool cen = false;
+  int ck = 0;
+  if (ok) {
+    if (inside) key = (unsigned)P.NT + (unsigned)i;
+    const int sp = f == 0 ? 0 : f == 1 ? 1 : f == 2 ? 2 : -1;
+    if (!(inside && excl) && sp >= 0) {
+      const long long u = (long long)ix + (long long)P.Lx * iy + (long long)P.Lx * P.Ly * iz;
+      if (u < 0 || u >= P.vol) raise_error(&D.sc->err, ERR_CELL_RANGE);
+      else { cen = true; ck = (int)(sp * P.vol + u); }
+    }
+  }
+  sb_groups(P, D, G, ok, t, vx, vy, vz, cen, ck, key);
+  if (valid) {
+    D.cell_of[i] = t;
+    const double x = (double)ix + (double)ox, y = (double)iy + (double)oy, z = (double)iz + (double)oz;
+    double dmin = 1e30;
+    bool close = false;
+    for (int s = 0; s < 2 * P.nd; s++) {
+      const double a = pbc_half(x - sph[0][s], P.box[0]);
+      const double b = pbc_half(y - sph[1][s], P.box[1]);
+      const double c = z - sph[2][s];
+      const double d2 = a * a + b * b + c * c, far = P.full[s >= P.nd] + 8.0;
+      close |= d2 < far * far;
+    }
+    if (__ballot(close))   // most waves lie far from every sphere: no square roots at all
+      for (int s = 0; s < 2 * P.nd; s++) {
+        const double a = pbc_half(x - sph[0][s], P.box[0]);
+        const double b = pbc_half(y - sph[1][s], P.box[1]);
+        const double c = z - sph[2][s];
+        const double d2 = a * a + b * b + c * c, far = P.full[s >= P.nd] + 8.0;
+        if (d2 < far * far) dmin = fmin(dmin, sqrt(d2) - P.full[s >= P.nd]);
+      }
+    // 1e-5 below the measured distance: the FP32 position must not make a particle look farther
+    double qd = floor(fmax(dmin - 1e-5, 0.0) * 32.0);
+    if (inside) qd = 0.0;   // list members stay near until the next rebuild
+    D.dq[i] = (unsigned char)fmin(qd, 255.0);
+  }
+}
+
+__device__ __forceinline__ void sb_load_spheres(const CoP& P, const double* x, const double* y, const double* z,
+                                                double (*sph)[MAXSG]) {
+  for (int j = threadIdx.x; j < 2 * P.nd; j += blockDim.x) { sph[0][j] = x[j]; sph[1][j] = y[j]; sph[2][j] = z[j]; }
+}
+
+// Tile of particles of one system.  ALL = false (beside the near engine): the
+// particles outside the interval's near set take one free flight of MPCtstep
+// steps (sp_stream: bounce-back at the z walls, the reactive patch of z = 0)
+// and, with do_bin, are binned with distances to the spheres' start
+// positions.  ALL = true (a re-sort interval, after the sort and the near
+// engine): every particle binned where it is, distances to the spheres' end
+// positions.
+template <bool ALL>
+__global__ __launch_bounds__(256) void sb_bulk(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds,
+                                               const SpD* __restrict__ Ss, const Tile* __restrict__ tiles, int do_bin) {
+  __shared__ SbTables G;
+  __shared__ double sph[3][MAXSG];
+  const Tile tl = tiles[blockIdx.x];
+  const CoP P = Ps[tl.r];
+  const CoD D = Ds[tl.r];
+  const SpD S = Ss[tl.r];
+  if (ALL) sb_load_spheres(P, D.sc->x0, D.sc->y0, D.sc->z0, sph);
+  else sb_load_spheres(P, D.sc->xs0, D.sc->ys0, D.sc->zs0, sph);
+  if (do_bin) sb_tables_init(G);
+  else __syncthreads();
+  double xs = 0, ys = 0, zs = 0;
+  if (do_bin) collision_shift(P, D, xs, ys, zs);
+  const float fxs = (float)xs, fys = (float)ys, fzs = (float)zs;
+  const float dtc = (float)P.MDt * (float)P.MPCtstep;
+  for (long long base = tl.begin; base < tl.end; base += 256) {
+    const int i = (int)(base + threadIdx.x);
+    const bool valid = i < tl.end;
+    const bool near = valid && ((D.near_bits[i >> 5] >> (i & 31)) & 1u);
+    const bool mine = valid && (ALL || !near);
+    int ix = 0, iy = 0, iz = 0, f = -1;
+    float ox = 0.0f, oy = 0.0f, oz = 0.0f, vx = 0.0f, vy = 0.0f, vz = 0.0f;
+    bool inside = false, excl = false;
+    if (mine) {
+      const unsigned char st = D.status[i];
+      if (near) { inside = (st & ST_INSIDE) != 0; excl = (st & ST_EXCL) != 0; }
+      else if (st) D.status[i] = 0;
+      sp_cell_xyz(P, S.cell[i], ix, iy, iz);
+      ox = S.ox[i]; oy = S.oy[i]; oz = S.oz[i];
+      vx = S.vx[i]; vy = S.vy[i]; vz = S.vz[i];
+      f = S.flag[i];
+      if (!ALL) {
+        const int f0 = f;
+        const bool bounced = sp_stream(ix, iy, iz, ox, oy, oz, vx, vy, vz, f, dtc, P, D.react_box, &D.sc->err);
+        S.cell[i] = ix + P.Lx * (iy + P.Ly * iz);
+        S.ox[i] = ox; S.oy[i] = oy; S.oz[i] = oz;
+        if (bounced) { S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz; }
+        if (f != f0) S.flag[i] = (signed char)f;
+      }
+    }
+    if (do_bin) sb_bin(P, D, G, i, mine, inside, excl, ix, iy, iz, ox, oy, oz, vx, vy, vz, f, fxs, fys, fzs, zs, sph);
+  }
+}
+
+// Per system (blockIdx.y): the interval's start -- the collision counter and
+// the spheres' start positions (start = 1) -- and the near set from split
+// storage into the FP64 arrays the near engine works on.
+__global__ __launch_bounds__(256) void sb_load_near(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds,
+                                                    const SpD* __restrict__ Ss, int start) {
+  const int r = blockIdx.y;
+  const CoP P = Ps[r];
+  const CoD D = Ds[r];
+  const SpD S = Ss[r];
+  if (start && blockIdx.x == 0) {
+    if (threadIdx.x == 0) D.sc->coll += 1;
+    for (int j = threadIdx.x; j < 2 * P.nd; j += blockDim.x) {
+      D.sc->xs0[j] = D.sc->x0[j]; D.sc->ys0[j] = D.sc->y0[j]; D.sc->zs0[j] = D.sc->z0[j];
+    }
+  }
+  const long long nn = min(D.sc->near_num, D.near_cap);
+  for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < nn; q += (long long)gridDim.x * 256) {
+    const int i = D.near_list[q];
+    int ix, iy, iz;
+    sp_cell_xyz(P, S.cell[i], ix, iy, iz);
+    D.x[i] = (double)ix + (double)S.ox[i]; D.y[i] = (double)iy + (double)S.oy[i]; D.z[i] = (double)iz + (double)S.oz[i];
+    D.vx[i] = (double)S.vx[i]; D.vy[i] = (double)S.vy[i]; D.vz[i] = (double)S.vz[i];
+    D.flag[i] = (int)S.flag[i];
+  }
+}
+
+// Per system (blockIdx.y), after its near engine: the near set back into
+// split storage and, with do_bin, binned (distances to the spheres' start
+// positions, as sb_bulk<false>); with check, the flight check (the near
+// engine's rebuild events must add up to the interval sb_bulk flew).
+__global__ __launch_bounds__(256) void sb_near(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds,
+                                               const SpD* __restrict__ Ss, int do_bin, int check) {
+  __shared__ SbTables G;
+  __shared__ double sph[3][MAXSG];
+  const int r = blockIdx.y;
+  const CoP P = Ps[r];
+  const CoD D = Ds[r];
+  const SpD S = Ss[r];
+  sb_load_spheres(P, D.sc->xs0, D.sc->ys0, D.sc->zs0, sph);
+  if (do_bin) sb_tables_init(G);
+  else __syncthreads();
+  double xs = 0, ys = 0, zs = 0;
+  if (do_bin) collision_shift(P, D, xs, ys, zs);
+  const float fxs = (float)xs, fys = (float)ys, fzs = (float)zs;
+  const long long nn = min(D.sc->near_num, D.near_cap);
+  for (long long base = (long long)blockIdx.x * 256; base < nn; base += (long long)gridDim.x * 256) {
+    const long long q = base + threadIdx.x;
+    const bool valid = q < nn;
+    int i = 0, ix = 0, iy = 0, iz = 0, f = -1;
+    float ox = 0.0f, oy = 0.0f, oz = 0.0f, vx = 0.0f, vy = 0.0f, vz = 0.0f;
+    bool inside = false, excl = false;
+    if (valid) {
+      i = D.near_list[q];
+      int c;
+      sp_split(D.x[i], D.y[i], D.z[i], P, c, ox, oy, oz);
+      vx = (float)D.vx[i]; vy = (float)D.vy[i]; vz = (float)D.vz[i];
+      f = D.flag[i];
+      S.cell[i] = c; S.ox[i] = ox; S.oy[i] = oy; S.oz[i] = oz;
+      S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz; S.flag[i] = (signed char)f;
+      sp_cell_xyz(P, c, ix, iy, iz);
+      const unsigned char st = D.status[i];
+      inside = (st & ST_INSIDE) != 0; excl = (st & ST_EXCL) != 0;
+    }
+    if (do_bin) sb_bin(P, D, G, i, valid, inside, excl, ix, iy, iz, ox, oy, oz, vx, vy, vz, f, fxs, fys, fzs, zs, sph);
+  }
+  if (check && blockIdx.x == 0 && threadIdx.x == 0) check_flight(P, D);
+}
+
+// Tile of 256 cells of one system, once every particle is binned: the record
+// of collision cell t (mean velocity from the exact sums, random axis of
+// mpc_rotation_mod.f95:184-194) with its sums cleared; the census limits and
+// the bulk B->A reaction of unshifted cell t (reaction_mod.f95:28-66: p from
+// the host table of 1 - exp(-k1 nB tau), one draw per cell, the first B
+// member turned A) with its census cleared.
+__device__ __forceinline__ void sb_cell(const CoP& P, const CoD& D, const SpD& S, long long t);
+__global__ __launch_bounds__(256) void sb_cells(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds,
+                                                const SpD* __restrict__ Ss, const Tile* __restrict__ tiles) {
+  const Tile tl = tiles[blockIdx.x];
+  const CoP P = Ps[tl.r];
+  const CoD D = Ds[tl.r];
+  const SpD S = Ss[tl.r];
+  for (long long t = tl.begin + threadIdx.x; t < tl.end; t += 256) sb_cell(P, D, S, t);
+}
+__device__ __forceinline__ void sb_cell(const CoP& P, const CoD& D, const SpD& S, long long t) {
+  if (t == 0) D.sc->far_maxv2 = 0ull;   // sb_rotate publishes the next interval's far maximum
+  if (t < P.mpcnumcell) {
+    unsigned long long* a = D.sacc + 3 * t;
+    const long long sx = (long long)a[0], sy = (long long)a[1], sz = (long long)a[2];
+    a[0] = 0ull; a[1] = 0ull; a[2] = 0ull;
+    const int n = (int)(sz & ((1ll << SB_NBITS) - 1));
+    float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+    if (n) {
+      const double dn = (double)n;
+      cx = (float)(((double)sx / SB_SCALE) / dn);
+      cy = (float)(((double)sy / SB_SCALE) / dn);
+      cz = (float)(((double)((sz - n) >> SB_NBITS) / SB_SCALE) / dn);
+    }
+    double u1, u2;
+    philox_uniform2(P.seed, 1, D.sc->coll, (unsigned long long)D.mpc_cell[t], u1, u2);
+    const double phi = P.twopi * u1, th = 2.0 * u2 - 1.0, st = sqrt(1.0 - th * th);
+    double sp, cp;
+    sincos(phi, &sp, &cp);
+    D.srec[2 * t] = make_float4(cx, cy, cz, (float)n);
+    D.srec[2 * t + 1] = make_float4((float)(st * cp), (float)(st * sp), (float)th, 0.0f);
+  }
+  if (t < P.vol) {
+    int* tc = D.tcount;
+    if (tc[t] > CENSUS_MAX || tc[P.vol + t] > CENSUS_MAX || tc[2 * P.vol + t] > CENSUS_MAX)
+      raise_error(&D.sc->err, ERR_MEMBERS);
+    if (t < P.rxnnumcell) {
+      const int nb = tc[P.vol + t];
+      const double prob = D.ptab[min(nb, CENSUS_MAX)];
+      double u, w;
+      philox_uniform2(P.seed, 2, D.sc->coll, (unsigned long long)t, u, w);
+      if (0.0 < u && u <= prob) {
+        const unsigned key = D.firstB[t];
+        const int i = key >= (unsigned)P.NT ? (int)(key - (unsigned)P.NT) : (int)key;
+        S.flag[i] = 0;
+      }
+    }
+    tc[t] = 0; tc[P.vol + t] = 0; tc[2 * P.vol + t] = 0;
+    D.firstB[t] = 0xffffffffu;
+  }
+}
+
+// Tile of particles of one system: the rotation in FP32 with its cell's
+// record (mpc_rotation_mod.f95:198-230) and the next interval's near set: a
+// particle is near when its distance byte (beyond a list radius) is within
+// |v| tau + guard, widened by the spheres' largest displacement since the
+// interval's start when the distances were measured there (classify 2).
+// Whole words of the near bitmap are written (tiles are multiples of 256).
+__global__ __launch_bounds__(256) void sb_rotate(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds,
+                                                 const SpD* __restrict__ Ss, const Tile* __restrict__ tiles,
+                                                 int classify) {
+  const Tile tl = tiles[blockIdx.x];
+  const CoP P = Ps[tl.r];
+  const CoD D = Ds[tl.r];
+  const SpD S = Ss[tl.r];
+  const double tau = P.tau * (1.0 + 1e-9);
+  const double reach0 = P.guard + 1e-6 + (classify == 2 ? start_shift(P, D) : 0.0);
+  const int lane = threadIdx.x & 63;
+  float fmax2 = 0.0f;
+  for (long long base = tl.begin; base < tl.end; base += 256) {
+    const long long i = base + threadIdx.x;
+    bool nb = false;
+    if (i < tl.end) {
+      const int t = D.cell_of[i];
+      const float4 r0 = D.srec[2 * (long long)t], r1 = D.srec[2 * (long long)t + 1];
+      float vx = S.vx[i], vy = S.vy[i], vz = S.vz[i];
+      sp_rotate_velocity(P, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, vx, vy, vz);
+      S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz;
+      const float v2 = (vx * vx + vy * vy) + vz * vz;
+      nb = (double)D.dq[i] * (1.0 / 32.0) <= sqrt((double)v2) * tau + reach0;
+      if (!nb) fmax2 = fmaxf(fmax2, v2);
+    }
+    const unsigned long long m = __ballot(nb);
+    const long long w0 = (base + (threadIdx.x & ~63)) >> 5, nw = (tl.end + 31) >> 5;   // words below the tile's end
+    if (lane == 0 && w0 < nw) D.near_bits[w0] = (unsigned)m;
+    if (lane == 32 && w0 + 1 < nw) D.near_bits[w0 + 1] = (unsigned)(m >> 32);
+  }
+  sp_far_max_publish(&D.sc->far_maxv2, fmax2);
+}
+
+// Per system (blockIdx.y): cleared cell sums, census and first-B keys (every
+// split run starts from them; each interval's sb_cells leaves them cleared)
+__global__ __launch_bounds__(256) void sb_clear(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds) {
+  const int r = blockIdx.y;
+  const CoP P = Ps[r];
+  const CoD D = Ds[r];
+  const long long i0 = (long long)blockIdx.x * 256 + threadIdx.x, st = (long long)gridDim.x * 256;
+  for (long long i = i0; i < 3 * P.mpcnumcell; i += st) D.sacc[i] = 0ull;
+  for (long long i = i0; i < 3 * P.vol; i += st) D.tcount[i] = 0;
+  for (long long i = i0; i < P.vol; i += st) D.firstB[i] = 0xffffffffu;
+}
+
+// Hilbert re-sort of split storage (sort_hilbert_mod.f95:50-92), per tile:
+// bin by the tour position of the unshifted cell; gather into the second buffers
+__global__ __launch_bounds__(256) void sb_sort_bin(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds,
+                                                   const SpD* __restrict__ Ss, const CoSortD* __restrict__ Xs,
+                                                   const Tile* __restrict__ tiles) {
+  const Tile tl = tiles[blockIdx.x];
+  const CoP P = Ps[tl.r];
+  const CoD D = Ds[tl.r];
+  const SpD S = Ss[tl.r];
+  const CoSortD X = Xs[tl.r];
+  for (long long i = tl.begin + threadIdx.x; i < tl.end; i += 256) {
+    const long long u = S.cell[i];
+    if (u < 0 || u >= P.vol || X.tourpos[u] < 0) {
+      raise_error(&D.sc->err, ERR_CELL_RANGE);
+      X.key[i] = 0; X.rk[i] = 0;
+      continue;
+    }
+    const int t = X.tourpos[u];
+    const int r = atomicAdd(&X.scnt[t], 1);
+    if (r >= BMAX) raise_error(&D.sc->err, ERR_BUCKET);
+    X.key[i] = t;
+    X.rk[i] = (unsigned short)min(r, 65535);
+  }
+}
+__global__ __launch_bounds__(256) void sb_sort_gather(const CoD* __restrict__ Ds, const SpD* __restrict__ Ss,
+                                                      const SpD* __restrict__ Ns, const CoSortD* __restrict__ Xs,
+                                                      const Tile* __restrict__ tiles) {
+  const Tile tl = tiles[blockIdx.x];
+  const CoD D = Ds[tl.r];
+  const SpD S = Ss[tl.r], N = Ns[tl.r];
+  const CoSortD X = Xs[tl.r];
+  const int lane = threadIdx.x & 63;
+  for (long long base = tl.begin; base < tl.end; base += 256) {
+    const long long j = base + threadIdx.x;
+    bool bit = false;
+    if (j < tl.end) {
+      const int i = X.sb[j];
+      N.cell[j] = S.cell[i]; N.ox[j] = S.ox[i]; N.oy[j] = S.oy[i]; N.oz[j] = S.oz[i];
+      N.vx[j] = S.vx[i]; N.vy[j] = S.vy[i]; N.vz[j] = S.vz[i]; N.flag[j] = S.flag[i];
+      X.astat[j] = D.status[i];
+      bit = (D.near_bits[i >> 5] >> (i & 31)) & 1u;
+    }
+    const unsigned long long m = __ballot(bit);
+    const long long w0 = (base + (threadIdx.x & ~63)) >> 5, nw = (tl.end + 31) >> 5;
+    if (lane == 0 && w0 < nw) X.abits[w0] = (unsigned)m;
+    if (lane == 32 && w0 + 1 < nw) X.abits[w0 + 1] = (unsigned)(m >> 32);
+  }
+}
+
 }  // namespace mpcx
 
 struct mpcx_cob {
@@ -5528,27 +5538,50 @@ struct mpcx_cob {
   CoP* d_P = nullptr;
   CoD* d_D = nullptr;
   std::vector<CoD> h_D;
-  Tile *d_pt = nullptr, *d_ct = nullptr, *d_vt = nullptr, *d_gt = nullptr;
-  int n_pt = 0, n_ct = 0, n_vt = 0, tile = 2048;
-  CoSortD* d_S = nullptr;
-  std::vector<CoSortD> h_S;
-  hipStream_t st = nullptr;
+  Tile *d_pt = nullptr, *d_ct = nullptr, *d_vt = nullptr, *d_gt = nullptr, *d_st = nullptr;
+  int n_pt = 0, n_ct = 0, n_vt = 0, n_st = 0, tile = 2048;
+  int cell_tile = 256;   // cells per workgroup of sb_cells (a multiple of 256)
+  CoSortD* d_X = nullptr;   // sort buffers of each replica
+  std::vector<CoSortD> h_X;
+  // split storage: the replicas' split arrays and their second (sort) buffers
+  bool split = false;
+  SpD *d_S = nullptr, *d_N = nullptr;
+  std::vector<SpD> h_S, h_N;
+  // solo: one engine run through the split interval on its own streams (its
+  // near engine in whatever mode the engine uses); otherwise one
+  // workgroup per replica.  overlap: the far particles stream beside the near
+  // engines (one-workgroup engines only: a cooperative near engine needs
+  // every workgroup resident)
+  bool solo = false, overlap = true;
+  hipStream_t st = nullptr, st2 = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   double run_ms = 0;
   long long particles = 0, cells = 0;
   int fill_blocks = 64;
+  // kernel timing of split runs without a graph: every TSTRIDE-th interval,
+  // [0] near engines, [1] sb_bulk (far flight and binning), [2] sb_cells,
+  // [3] sb_rotate, [4] sb_near (near set binning)
+  static constexpr int NKT = 5, TMAX = 256, TSTRIDE = 8;
+  std::vector<hipEvent_t> tev;
+  std::vector<char> tev_on;
+  double kt_ms[NKT] = {0};
+  long long kt_n = 0;
 };
 
 static int cob_tiles(mpcx_cob* b) {
-  std::vector<Tile> pt, ct, vt;
+  std::vector<Tile> pt, ct, vt, sc;
   for (int r = 0; r < b->R; r++) {
     const CoP& P = b->eng[r]->P;
     for (long long i = 0; i < P.NT; i += b->tile) pt.push_back(Tile{r, 0, i, std::min<long long>(i + b->tile, P.NT)});
     for (long long t = 0; t < P.mpcnumcell; t += CT)
       ct.push_back(Tile{r, 0, t, std::min<long long>(t + CT, P.mpcnumcell)});
     for (long long t = 0; t < P.vol; t += 256) vt.push_back(Tile{r, 0, t, std::min<long long>(t + 256, P.vol)});
+    const long long nc = std::max(P.mpcnumcell, P.vol);
+    for (long long t = 0; t < nc; t += b->cell_tile)
+      sc.push_back(Tile{r, 0, t, std::min<long long>(t + b->cell_tile, nc)});
   }
   // workgroup b runs on XCD b % 8: lay the tiles out so that each XCD's
   // workgroups take one contiguous run of tiles
@@ -5564,19 +5597,22 @@ static int cob_tiles(mpcx_cob* b) {
   std::vector<Tile> gt = pt;   // gather kernels (sort gather) take the XCD-contiguous order
   xcd_order(gt);
   xcd_order(ct);
-  for (Tile** d : {&b->d_pt, &b->d_ct, &b->d_vt, &b->d_gt})
+  for (Tile** d : {&b->d_pt, &b->d_ct, &b->d_vt, &b->d_gt, &b->d_st})
     if (*d) { (void)hipFree(*d); *d = nullptr; }
   HIPCHK(hipMalloc(&b->d_pt, pt.size() * sizeof(Tile)));
   HIPCHK(hipMalloc(&b->d_gt, gt.size() * sizeof(Tile)));
   HIPCHK(hipMalloc(&b->d_ct, ct.size() * sizeof(Tile)));
   HIPCHK(hipMalloc(&b->d_vt, vt.size() * sizeof(Tile)));
+  HIPCHK(hipMalloc(&b->d_st, sc.size() * sizeof(Tile)));
   HIPCHK(hipMemcpy(b->d_gt, gt.data(), gt.size() * sizeof(Tile), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(b->d_pt, pt.data(), pt.size() * sizeof(Tile), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(b->d_ct, ct.data(), ct.size() * sizeof(Tile), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(b->d_vt, vt.data(), vt.size() * sizeof(Tile), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b->d_st, sc.data(), sc.size() * sizeof(Tile), hipMemcpyHostToDevice));
   b->n_pt = (int)pt.size();
   b->n_ct = (int)ct.size();
   b->n_vt = (int)vt.size();
+  b->n_st = (int)sc.size();
   return 0;
 }
 
@@ -5584,14 +5620,22 @@ static int cob_upload_D(mpcx_cob* b) {
   for (int r = 0; r < b->R; r++) {
     mpcx_co* e = b->eng[r];
     b->h_D[r] = e->D;
-    CoSortD& S = b->h_S[r];
-    S.tourpos = e->d_tourpos; S.scnt = e->d_scnt; S.soff = e->d_soff; S.sb = e->d_sb; S.key = e->d_skey;
-    S.rk = e->d_srk;
-    for (int a = 0; a < 6; a++) S.alt[a] = e->alt[a];
-    S.aflag = e->alt_flag; S.astat = e->alt_status; S.abits = e->alt_bits;
+    CoSortD& X = b->h_X[r];
+    X.tourpos = e->d_tourpos; X.scnt = e->d_scnt; X.soff = e->d_soff; X.sb = e->d_sb; X.key = e->d_skey;
+    X.rk = e->d_srk;
+    for (int a = 0; a < 6; a++) X.alt[a] = e->alt[a];
+    X.aflag = e->alt_flag; X.astat = e->alt_status; X.abits = e->alt_bits;
+    if (b->split) {
+      b->h_S[r] = e->S;
+      b->h_N[r] = e->Salt;
+    }
   }
   HIPCHK(hipMemcpyAsync(b->d_D, b->h_D.data(), b->R * sizeof(CoD), hipMemcpyHostToDevice, b->st));
-  HIPCHK(hipMemcpyAsync(b->d_S, b->h_S.data(), b->R * sizeof(CoSortD), hipMemcpyHostToDevice, b->st));
+  HIPCHK(hipMemcpyAsync(b->d_X, b->h_X.data(), b->R * sizeof(CoSortD), hipMemcpyHostToDevice, b->st));
+  if (b->split) {
+    HIPCHK(hipMemcpyAsync(b->d_S, b->h_S.data(), b->R * sizeof(SpD), hipMemcpyHostToDevice, b->st));
+    HIPCHK(hipMemcpyAsync(b->d_N, b->h_N.data(), b->R * sizeof(SpD), hipMemcpyHostToDevice, b->st));
+  }
   HIPCHK(hipStreamSynchronize(b->st));
   return 0;
 }
@@ -5602,33 +5646,50 @@ static int cob_sort_all(mpcx_cob* b) {
   for (int r = 0; r < b->R; r++)
     if (!b->eng[r]->has_tour) return set_error(MPCX_E_ARG, "a sort step needs the cell tour of every replica");
   const dim3 fg(b->fill_blocks, b->R);
-  hipLaunchKernelGGL(sort_fills_b, fg, dim3(256), 0, b->st, b->d_P, b->d_S);
-  hipLaunchKernelGGL(sort_bin_b, dim3(b->n_pt), dim3(256), 0, b->st, b->d_P, b->d_D, b->d_S, b->d_pt);
-  hipLaunchKernelGGL(sort_scan_b, dim3(b->R), dim3(1024), 0, b->st, b->d_P, b->d_S);
-  hipLaunchKernelGGL(sort_scatter_b, dim3(b->n_pt), dim3(256), 0, b->st, b->d_S, b->d_pt);
-  hipLaunchKernelGGL(sort_order_b, dim3(b->n_vt), dim3(256), 0, b->st, b->d_S, b->d_vt);
-  hipLaunchKernelGGL(sort_gather_b, dim3(b->n_pt), dim3(256), 0, b->st, b->d_D, b->d_S, b->d_gt);
+  hipLaunchKernelGGL(sort_fills_b, fg, dim3(256), 0, b->st, b->d_P, b->d_X);
+  if (b->split)
+    hipLaunchKernelGGL(sb_sort_bin, dim3(b->n_pt), dim3(256), 0, b->st, b->d_P, b->d_D, b->d_S, b->d_X, b->d_pt);
+  else hipLaunchKernelGGL(sort_bin_b, dim3(b->n_pt), dim3(256), 0, b->st, b->d_P, b->d_D, b->d_X, b->d_pt);
+  hipLaunchKernelGGL(sort_scan_b, dim3(b->R), dim3(1024), 0, b->st, b->d_P, b->d_X);
+  hipLaunchKernelGGL(sort_scatter_b, dim3(b->n_pt), dim3(256), 0, b->st, b->d_X, b->d_pt);
+  hipLaunchKernelGGL(sort_order_b, dim3(b->n_vt), dim3(256), 0, b->st, b->d_X, b->d_vt);
+  if (b->split)
+    hipLaunchKernelGGL(sb_sort_gather, dim3(b->n_pt), dim3(256), 0, b->st, b->d_D, b->d_S, b->d_N, b->d_X, b->d_gt);
+  else hipLaunchKernelGGL(sort_gather_b, dim3(b->n_pt), dim3(256), 0, b->st, b->d_D, b->d_X, b->d_gt);
   for (int r = 0; r < b->R; r++) {
     mpcx_co* e = b->eng[r];
     CoD& D = e->D;
-    std::swap(D.x, e->alt[0]); std::swap(D.y, e->alt[1]); std::swap(D.z, e->alt[2]);
-    std::swap(D.vx, e->alt[3]); std::swap(D.vy, e->alt[4]); std::swap(D.vz, e->alt[5]);
-    std::swap(D.flag, e->alt_flag); std::swap(D.status, e->alt_status); std::swap(D.near_bits, e->alt_bits);
+    if (b->split) {
+      std::swap(e->S, e->Salt);
+    } else {
+      std::swap(D.x, e->alt[0]); std::swap(D.y, e->alt[1]); std::swap(D.z, e->alt[2]);
+      std::swap(D.vx, e->alt[3]); std::swap(D.vy, e->alt[4]); std::swap(D.vz, e->alt[5]);
+      std::swap(D.flag, e->alt_flag);
+    }
+    std::swap(D.status, e->alt_status); std::swap(D.near_bits, e->alt_bits);
   }
   return cob_upload_D(b);
 }
 
-static void cob_launch_near(mpcx_cob* b, int nsteps, int mode) {
+static hipError_t cob_launch_near(mpcx_cob* b, int nsteps, int mode) {
+  if (b->solo) return launch_near(b->eng[0], nsteps, mode);   // the engine's own stream is b->st
   if (b->nd == 1)
     hipLaunchKernelGGL(co_near_engine_b<1>, dim3(b->R), dim3(NE_T), 0, b->st, b->d_P, b->d_D, nsteps, mode);
   else
     hipLaunchKernelGGL(co_near_engine_b<2>, dim3(b->R), dim3(NE_T), 0, b->st, b->d_P, b->d_D, nsteps, mode);
+  return hipGetLastError();
+}
+
+// the near list of every replica from its bitmap
+static void cob_near_list(mpcx_cob* b) {
+  if (b->solo) co_near_list(b->eng[0]);   // many workgroups for one large system
+  else hipLaunchKernelGGL(near_list_b, dim3(b->R), dim3(1024), 0, b->st, b->d_P, b->d_D);
 }
 
 // the launches of one interval without a re-sort (captured into the graph)
 static void cob_interval(mpcx_cob* b) {
   const dim3 fg(b->fill_blocks, b->R);
-  cob_launch_near(b, b->S, 0);
+  (void)cob_launch_near(b, b->S, 0);
   hipLaunchKernelGGL(fills_begin_b, fg, dim3(256), 0, b->st, b->d_P, b->d_D);
   hipLaunchKernelGGL(co_bulk_b, dim3(b->n_pt), dim3(256), 0, b->st, b->d_P, b->d_D, b->d_pt, 1, 1);
   hipLaunchKernelGGL(scan_cells_b, dim3(b->R), dim3(1024), 0, b->st, b->d_P, b->d_D);
@@ -5643,13 +5704,13 @@ static void cob_interval(mpcx_cob* b) {
 // step's list rebuild, every replica is sorted, then they resume
 static int cob_sort_interval(mpcx_cob* b) {
   const dim3 fg(b->fill_blocks, b->R);
-  cob_launch_near(b, b->S, M_STOP);
+  HIPCHK(cob_launch_near(b, b->S, M_STOP));
   hipLaunchKernelGGL(fills_begin_b, fg, dim3(256), 0, b->st, b->d_P, b->d_D);
   hipLaunchKernelGGL(co_bulk_b, dim3(b->n_pt), dim3(256), 0, b->st, b->d_P, b->d_D, b->d_pt, 1, 0);
   int rc = cob_sort_all(b);
   if (rc) return rc;
   hipLaunchKernelGGL(near_list_b, dim3(b->R), dim3(1024), 0, b->st, b->d_P, b->d_D);
-  cob_launch_near(b, 1, M_RESUME);
+  HIPCHK(cob_launch_near(b, 1, M_RESUME));
   hipLaunchKernelGGL(co_bulk_b, dim3(b->n_pt), dim3(256), 0, b->st, b->d_P, b->d_D, b->d_pt, 0, 1);
   hipLaunchKernelGGL(scan_cells_b, dim3(b->R), dim3(1024), 0, b->st, b->d_P, b->d_D);
   hipLaunchKernelGGL(co_scatter_b, dim3(b->n_pt), dim3(256), 0, b->st, b->d_D, b->d_pt);
@@ -5660,26 +5721,93 @@ static int cob_sort_interval(mpcx_cob* b) {
   return 0;
 }
 
+// One split-precision interval of every replica (the kernels above).  With
+// overlap the far particles' flight and binning (st2) run beside the near
+// engines (st); the near set is binned on st when they are done; every cell
+// is collided and every particle rotated once both are.  A re-sort interval
+// stops the near engines before the last step's list rebuild (the far flight
+// is not binned), re-sorts, lets them resume, then bins every particle.
+// slot >= 0: the interval's kernels are timed (events into that slot).
+static int sb_interval(mpcx_cob* b, bool sort_due, long long slot) {
+  hipStream_t st = b->st, s2 = b->st2;
+  // near-set kernels: blockIdx.y = replica (a solo system's near set may hold many dimers' regions)
+  const dim3 ng(b->solo ? grid_for(b->eng[0]->D.near_cap, 256, 1024) : 32, b->R);
+  const bool ov = b->overlap;
+  auto tmark = [&](int k, int edge, hipStream_t s) -> hipError_t {
+    if (slot < 0) return hipSuccess;
+    b->tev_on[(size_t)slot * mpcx_cob::NKT + k] = 1;
+    return hipEventRecord(b->tev[((size_t)slot * mpcx_cob::NKT + k) * 2 + edge], s);
+  };
+  if (slot >= 0)
+    for (int k = 0; k < mpcx_cob::NKT; k++) b->tev_on[(size_t)slot * mpcx_cob::NKT + k] = 0;
+  hipLaunchKernelGGL(sb_load_near, ng, dim3(256), 0, st, b->d_P, b->d_D, b->d_S, 1);
+  const int bin = sort_due ? 0 : 1;
+  if (ov) {
+    HIPCHK(hipEventRecord(b->fork, st));
+    HIPCHK(hipStreamWaitEvent(s2, b->fork, 0));
+    HIPCHK(tmark(1, 0, s2));
+    hipLaunchKernelGGL(sb_bulk<false>, dim3(b->n_pt), dim3(256), 0, s2, b->d_P, b->d_D, b->d_S, b->d_pt, bin);
+    HIPCHK(tmark(1, 1, s2));
+    HIPCHK(hipEventRecord(b->join, s2));
+  }
+  HIPCHK(tmark(0, 0, st));
+  HIPCHK(cob_launch_near(b, b->S, sort_due ? M_STOP : 0));
+  HIPCHK(tmark(0, 1, st));
+  if (!ov) {
+    HIPCHK(tmark(1, 0, st));
+    hipLaunchKernelGGL(sb_bulk<false>, dim3(b->n_pt), dim3(256), 0, st, b->d_P, b->d_D, b->d_S, b->d_pt, bin);
+    HIPCHK(tmark(1, 1, st));
+  }
+  HIPCHK(tmark(4, 0, st));
+  hipLaunchKernelGGL(sb_near, ng, dim3(256), 0, st, b->d_P, b->d_D, b->d_S, bin, 1);
+  HIPCHK(tmark(4, 1, st));
+  if (ov) HIPCHK(hipStreamWaitEvent(st, b->join, 0));
+  if (sort_due) {
+    int rc = cob_sort_all(b);
+    if (rc) return rc;
+    cob_near_list(b);
+    hipLaunchKernelGGL(sb_load_near, ng, dim3(256), 0, st, b->d_P, b->d_D, b->d_S, 0);
+    HIPCHK(cob_launch_near(b, 1, M_RESUME));
+    hipLaunchKernelGGL(sb_near, ng, dim3(256), 0, st, b->d_P, b->d_D, b->d_S, 0, 0);
+    hipLaunchKernelGGL(sb_bulk<true>, dim3(b->n_pt), dim3(256), 0, st, b->d_P, b->d_D, b->d_S, b->d_pt, 1);
+  }
+  HIPCHK(tmark(2, 0, st));
+  hipLaunchKernelGGL(sb_cells, dim3(b->n_st), dim3(256), 0, st, b->d_P, b->d_D, b->d_S, b->d_st);
+  HIPCHK(tmark(2, 1, st));
+  HIPCHK(tmark(3, 0, st));
+  hipLaunchKernelGGL(sb_rotate, dim3(b->n_pt), dim3(256), 0, st, b->d_P, b->d_D, b->d_S, b->d_pt, sort_due ? 1 : 2);
+  HIPCHK(tmark(3, 1, st));
+  cob_near_list(b);
+  return 0;
+}
+
+static void cob_free_graph(mpcx_cob* b) {
+  if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+  if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+}
+
 extern "C" int mpcx_cob_destroy(mpcx_cob* b) {
   if (!b) return 0;
   if (b->st) (void)hipStreamSynchronize(b->st);
-  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
-  if (b->d_P) (void)hipFree(b->d_P);
-  if (b->d_D) (void)hipFree(b->d_D);
-  if (b->d_pt) (void)hipFree(b->d_pt);
-  if (b->d_ct) (void)hipFree(b->d_ct);
-  if (b->d_vt) (void)hipFree(b->d_vt);
-  if (b->d_gt) (void)hipFree(b->d_gt);
-  if (b->d_S) (void)hipFree(b->d_S);
-  if (b->e0) (void)hipEventDestroy(b->e0);
-  if (b->e1) (void)hipEventDestroy(b->e1);
-  if (b->st) (void)hipStreamDestroy(b->st);
+  if (b->st2) (void)hipStreamSynchronize(b->st2);
+  cob_free_graph(b);
+  for (void* p : {(void*)b->d_P, (void*)b->d_D, (void*)b->d_pt, (void*)b->d_ct, (void*)b->d_vt, (void*)b->d_gt,
+                  (void*)b->d_st, (void*)b->d_X, (void*)b->d_S, (void*)b->d_N})
+    if (p) (void)hipFree(p);
+  for (hipEvent_t ev : {b->e0, b->e1, b->fork, b->join})
+    if (ev) (void)hipEventDestroy(ev);
+  for (hipEvent_t ev : b->tev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (!b->solo) {   // a solo batch runs on its engine's streams
+    if (b->st) (void)hipStreamDestroy(b->st);
+    if (b->st2) (void)hipStreamDestroy(b->st2);
+  }
   delete b;
   return 0;
 }
 
-extern "C" int mpcx_cob_create(mpcx_cob** out, int32_t R, mpcx_co* const* engines, int32_t tile) {
+// solo: one split engine on its own streams (any near mode)
+static int cob_make(mpcx_cob** out, int32_t R, mpcx_co* const* engines, int32_t tile, bool solo) {
   if (!out || R < 1 || !engines) return set_error(MPCX_E_ARG, "bad arguments");
   *out = nullptr;
   const mpcx_co* e0 = engines[0];
@@ -5689,8 +5817,9 @@ extern "C" int mpcx_cob_create(mpcx_cob** out, int32_t R, mpcx_co* const* engine
     if (e->P.rng_mode != 1) return set_error(MPCX_E_UNSUPPORTED, "a batch runs in rng_mode 1 (device Philox)");
     if (e->nd != e0->nd || e->P.MPCtstep != e0->P.MPCtstep || e->P.sort_step != e0->P.sort_step)
       return set_error(MPCX_E_UNSUPPORTED, "replicas of a batch share numdims, MPCtstep and sort_step");
-    if (e->nd > 2) return set_error(MPCX_E_UNSUPPORTED, "batched replicas hold one or two dimers");
-    if (e->split) return set_error(MPCX_E_UNSUPPORTED, "batched replicas use FP64 storage");
+    if (e->split != e0->split) return set_error(MPCX_E_UNSUPPORTED, "replicas of a batch share their storage precision");
+    if (!solo && (e->nd > 2 || e->near_mode != 0))
+      return set_error(MPCX_E_UNSUPPORTED, "batched replicas hold one or two dimers (one-workgroup near engines)");
   }
   if (tile < 256 || tile % 256 != 0) return set_error(MPCX_E_ARG, "tile must be a positive multiple of 256");
   mpcx_cob* b = new mpcx_cob();
@@ -5699,9 +5828,14 @@ extern "C" int mpcx_cob_create(mpcx_cob** out, int32_t R, mpcx_co* const* engine
   b->S = e0->P.MPCtstep;
   b->sort_step = e0->P.sort_step;
   b->tile = tile;
+  b->split = e0->split;
+  b->solo = solo;
+  b->overlap = !solo || (e0->near_mode == 0 && e0->overlap);
   b->eng.assign(engines, engines + R);
   b->h_D.resize(R);
+  b->h_X.resize(R);
   b->h_S.resize(R);
+  b->h_N.resize(R);
   std::vector<CoP> hp(R);
   for (int r = 0; r < R; r++) {
     hp[r] = engines[r]->P;
@@ -5709,10 +5843,19 @@ extern "C" int mpcx_cob_create(mpcx_cob** out, int32_t R, mpcx_co* const* engine
     b->cells += engines[r]->P.mpcnumcell;
   }
   int rc = 0;
-  if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess || hipMalloc(&b->d_P, R * sizeof(CoP)) != hipSuccess ||
-      hipMalloc(&b->d_D, R * sizeof(CoD)) != hipSuccess || hipMalloc(&b->d_S, R * sizeof(CoSortD)) != hipSuccess ||
-      hipEventCreate(&b->e0) != hipSuccess ||
-      hipEventCreate(&b->e1) != hipSuccess)
+  if (solo) {
+    b->st = e0->st;
+    b->st2 = e0->st2;
+  } else if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess ||
+             hipStreamCreateWithFlags(&b->st2, hipStreamNonBlocking) != hipSuccess) {
+    rc = set_error(MPCX_E_HIP, "batch stream creation failed");
+  }
+  if (!rc && (hipMalloc(&b->d_P, R * sizeof(CoP)) != hipSuccess || hipMalloc(&b->d_D, R * sizeof(CoD)) != hipSuccess ||
+              hipMalloc(&b->d_X, R * sizeof(CoSortD)) != hipSuccess || hipMalloc(&b->d_S, R * sizeof(SpD)) != hipSuccess ||
+              hipMalloc(&b->d_N, R * sizeof(SpD)) != hipSuccess || hipEventCreate(&b->e0) != hipSuccess ||
+              hipEventCreate(&b->e1) != hipSuccess ||
+              hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+              hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess))
     rc = set_error(MPCX_E_HIP, "batch allocation failed");
   if (!rc && hipMemcpy(b->d_P, hp.data(), R * sizeof(CoP), hipMemcpyHostToDevice) != hipSuccess)
     rc = set_error(MPCX_E_HIP, "hipMemcpy failed");
@@ -5723,6 +5866,10 @@ extern "C" int mpcx_cob_create(mpcx_cob** out, int32_t R, mpcx_co* const* engine
   }
   *out = b;
   return 0;
+}
+
+extern "C" int mpcx_cob_create(mpcx_cob** out, int32_t R, mpcx_co* const* engines, int32_t tile) {
+  return cob_make(out, R, engines, tile, false);
 }
 
 // checks that every replica sits at the same collision step, sets the
@@ -5741,14 +5888,38 @@ static int cob_begin(mpcx_cob* b, int* cstep_out) {
     HIPCHK(hipMemcpy(&e->D.sc->coll, &coll, sizeof coll, hipMemcpyHostToDevice));
   }
   *cstep_out = cstep;
-  return cob_upload_D(b);
+  int rc = cob_upload_D(b);
+  if (rc) return rc;
+  if (b->split) {
+    long long big = 0;
+    for (int r = 0; r < b->R; r++) big = std::max(big, 3 * std::max(b->eng[r]->P.mpcnumcell, b->eng[r]->P.vol));
+    hipLaunchKernelGGL(sb_clear, dim3(grid_for(big, 256, 1024), b->R), dim3(256), 0, b->st, b->d_P, b->d_D);
+  }
+  return 0;
 }
 
-// launches one interval (graph replay when possible)
-static int cob_step(mpcx_cob* b, int* cstep, int use_graph) {
+// launches one interval (graph replay when possible); slot >= 0 times it
+static int cob_step(mpcx_cob* b, int* cstep, int use_graph, long long slot = -1) {
   const int cnext = *cstep + b->S;
+  const bool sort_due = cnext % b->sort_step == 0;
   int rc = 0;
-  if (cnext % b->sort_step == 0) {
+  if (b->split) {
+    if (sort_due || !use_graph || slot >= 0) {
+      rc = sb_interval(b, sort_due, slot);
+    } else {
+      if (!b->gexec) {
+        HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+        rc = sb_interval(b, false, -1);
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(b->st, &g);
+        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+        HIPCHK(ce);
+        b->graph = g;
+        HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+      }
+      HIPCHK(hipGraphLaunch(b->gexec, b->st));
+    }
+  } else if (sort_due) {
     rc = cob_sort_interval(b);
   } else if (use_graph) {
     if (!b->gexec) {
@@ -5775,20 +5946,68 @@ static int cob_end(mpcx_cob* b) {
   return 0;
 }
 
-extern "C" int mpcx_cob_run(mpcx_cob* b, int64_t ncoll, int32_t use_graph) {
-  if (!b || ncoll < 0) return set_error(MPCX_E_ARG, "bad arguments");
+static int cob_run(mpcx_cob* b, int64_t ncoll, int32_t use_graph) {
   int cstep = 0;
   int rc = cob_begin(b, &cstep);
   if (rc) return rc;
+  const bool timed_run = b->split && !use_graph;
+  if (timed_run && b->tev.empty()) {
+    b->tev.assign((size_t)mpcx_cob::TMAX * mpcx_cob::NKT * 2, nullptr);
+    b->tev_on.assign((size_t)mpcx_cob::TMAX * mpcx_cob::NKT, 0);
+    for (auto& ev : b->tev) HIPCHK(hipEventCreate(&ev));
+  }
   HIPCHK(hipEventRecord(b->e0, b->st));
-  for (int64_t c = 0; c < ncoll && !rc; c++) rc = cob_step(b, &cstep, use_graph);
+  for (int64_t c = 0; c < ncoll && !rc; c++) {
+    const long long slot = c / mpcx_cob::TSTRIDE;
+    const bool timed = timed_run && c % mpcx_cob::TSTRIDE == 0 && slot < mpcx_cob::TMAX;
+    rc = cob_step(b, &cstep, use_graph, timed ? slot : -1);
+  }
   if (rc) return rc;
   HIPCHK(hipEventRecord(b->e1, b->st));
   HIPCHK(hipEventSynchronize(b->e1));
   float ms = 0;
   (void)hipEventElapsedTime(&ms, b->e0, b->e1);
   b->run_ms = ms;
+  if (timed_run) {
+    b->kt_n = std::min<long long>((ncoll + mpcx_cob::TSTRIDE - 1) / mpcx_cob::TSTRIDE, mpcx_cob::TMAX);
+    for (int k = 0; k < mpcx_cob::NKT; k++) {
+      double sum = 0;
+      long long n = 0;
+      for (long long c = 0; c < b->kt_n; c++) {
+        if (!b->tev_on[(size_t)c * mpcx_cob::NKT + k]) continue;
+        float t = 0;
+        (void)hipEventElapsedTime(&t, b->tev[((size_t)c * mpcx_cob::NKT + k) * 2],
+                                  b->tev[((size_t)c * mpcx_cob::NKT + k) * 2 + 1]);
+        sum += t;
+        n++;
+      }
+      b->kt_ms[k] = n ? sum / (double)n : 0.0;
+    }
+  }
   return cob_end(b);
+}
+
+extern "C" int mpcx_cob_run(mpcx_cob* b, int64_t ncoll, int32_t use_graph) {
+  if (!b || ncoll < 0) return set_error(MPCX_E_ARG, "bad arguments");
+  return cob_run(b, ncoll, use_graph);
+}
+
+// cells per workgroup of the split cell kernel (the autotuner's second knob); drops a captured graph
+extern "C" int mpcx_cob_set_cell_tile(mpcx_cob* b, int32_t cells) {
+  if (!b || cells < 256 || cells % 256 != 0 || cells > 65536)
+    return set_error(MPCX_E_ARG, "cells per workgroup must be a multiple of 256, at most 65536");
+  HIPCHK(hipStreamSynchronize(b->st));
+  cob_free_graph(b);
+  b->cell_tile = cells;
+  return cob_tiles(b);
+}
+
+// mean kernel durations (ms) of the last split run without a graph
+extern "C" int mpcx_cob_kernel_ms(mpcx_cob* b, double* ms5, int64_t* count) {
+  if (!b || !ms5) return set_error(MPCX_E_ARG, "bad arguments");
+  for (int k = 0; k < mpcx_cob::NKT; k++) ms5[k] = b->kt_ms[k];
+  if (count) *count = b->kt_n;
+  return 0;
 }
 
 // particles and collision cells of the batch, and the wall time of the last run (ms)
@@ -5804,10 +6023,28 @@ extern "C" int mpcx_cob_info(mpcx_cob* b, int64_t* particles, int64_t* cells, do
 extern "C" int mpcx_cob_set_tile(mpcx_cob* b, int32_t tile) {
   if (!b || tile < 256 || tile % 256 != 0) return set_error(MPCX_E_ARG, "tile must be a positive multiple of 256");
   HIPCHK(hipStreamSynchronize(b->st));
-  if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
-  if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+  cob_free_graph(b);
   b->tile = tile;
   return cob_tiles(b);
+}
+
+// A split-precision engine's run: a batch of one on the engine's streams
+static int co_run_split(mpcx_co* e, int64_t ncoll) {
+  if (!e->solo) {
+    mpcx_cob* b = nullptr;
+    int rc = cob_make(&b, 1, &e, 2048, true);
+    if (rc) return rc;
+    e->solo = b;
+  }
+  mpcx_cob* b = e->solo;
+  b->overlap = e->near_mode == 0 && !e->clustered && e->overlap;
+  int rc = cob_run(b, ncoll, 0);
+  e->run_ms = b->run_ms;
+  e->kt_n = b->kt_n;
+  for (int k = 0; k < mpcx_co::NKT && k < mpcx_cob::NKT; k++) e->kt_ms[k] = b->kt_ms[k];
+  e->near_ms = b->kt_ms[0];
+  e->bulk_ms = b->kt_ms[1] + b->kt_ms[2] + b->kt_ms[3];
+  return rc;
 }
 
 // ---------------------------------------------------------------- batch diagnostics
@@ -5820,8 +6057,10 @@ struct BodyGrid {
   int nr, nz;
   double rmax, zmin, zmax;
 };
+// Ss (split storage) or null (FP64 storage)
 __global__ __launch_bounds__(256) void body_acc_b(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds,
-                                                  BodyGrid G, unsigned* acc, double* vacc) {
+                                                  const SpD* __restrict__ Ss, BodyGrid G, unsigned* acc,
+                                                  double* vacc) {
   const int r = blockIdx.y;
   const CoP P = Ps[r];
   const CoD D = Ds[r];
@@ -5842,8 +6081,18 @@ __global__ __launch_bounds__(256) void body_acc_b(const CoP* __restrict__ Ps, co
   const double dr = G.rmax / G.nr, dz = (G.zmax - G.zmin) / G.nz;
   unsigned* a = acc + (size_t)r * G.nz * G.nr;
   for (long long i = blockIdx.x * 256ll + threadIdx.x; i < P.NT; i += (long long)gridDim.x * 256) {
-    if (D.flag[i] != 1) continue;
-    const double ax = pbc_half(D.x[i] - cx, P.box[0]), ay = pbc_half(D.y[i] - cy, P.box[1]), az = D.z[i] - cz;
+    double x, y, z;
+    if (Ss) {
+      const SpD S = Ss[r];
+      if (S.flag[i] != 1) continue;
+      int ix, iy, iz;
+      sp_cell_xyz(P, S.cell[i], ix, iy, iz);
+      x = (double)ix + (double)S.ox[i]; y = (double)iy + (double)S.oy[i]; z = (double)iz + (double)S.oz[i];
+    } else {
+      if (D.flag[i] != 1) continue;
+      x = D.x[i]; y = D.y[i]; z = D.z[i];
+    }
+    const double ax = pbc_half(x - cx, P.box[0]), ay = pbc_half(y - cy, P.box[1]), az = z - cz;
     const double zr = (ax * ux + ay * uy) + az * uz;
     const double px = ax - zr * ux, py = ay - zr * uy, pz = az - zr * uz;
     const double rr = sqrt((px * px + py * py) + pz * pz);
@@ -5874,7 +6123,9 @@ extern "C" int mpcx_cob_body_average(mpcx_cob* b, int64_t n, int32_t nr, int32_t
   if (!rc) rc = cob_begin(b, &cstep);
   for (int64_t i = 0; i < n && !rc; i++) {
     rc = cob_step(b, &cstep, 1);
-    if (!rc) hipLaunchKernelGGL(body_acc_b, dim3(64, b->R), dim3(256), 0, b->st, b->d_P, b->d_D, G, acc, vacc);
+    if (!rc)
+      hipLaunchKernelGGL(body_acc_b, dim3(64, b->R), dim3(256), 0, b->st, b->d_P, b->d_D,
+                         (const SpD*)(b->split ? b->d_S : nullptr), G, acc, vacc);
   }
   if (!rc) rc = cob_end(b);
   if (!rc) {

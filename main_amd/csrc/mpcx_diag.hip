@@ -168,7 +168,15 @@ static int grid_n(long long n) {
     if (_r != hipSuccess) return set_error(MPCX_E_HIP, hipGetErrorString(_r)); \
   } while (0)
 
-static int observables(const DevView& v, double* out) {
+// The observables of collective_code/output_mod.f95:681-756 (selkov = false)
+// or selkov_code/Output_SelkovDimer.f90:147-300 (selkov = true).  The solvent
+// sums are tree sums (the reference adds in particle order); the dimer terms
+// and the temperatures follow the reference's expressions and constants: the
+// temperatures multiply by the default-real literal 2./3. (0.6666666865...,
+// the source files declare no -fdefault-real-8); in the Selkov form the dimer
+// enters through its centre-of-mass velocity (momentum (Mc+Mn)*Vdimav,
+// kinetic energy of the centre of mass), in the collective form sphere by sphere.
+static int observables(const DevView& v, double* out, bool selkov) {
   hipStream_t st = (hipStream_t)v.stream;
   const int g = grid_n(v.n);
   double* part = nullptr;
@@ -183,23 +191,10 @@ static int observables(const DevView& v, double* out) {
   for (int b = 0; b < g; b++)
     for (int k = 0; k < 7; k++) s[k] = s[k] + h[(size_t)b * 7 + k];
   for (int i = 0; i < 16; i++) out[i] = 0.0;
+  const double two_thirds = (double)(2.0f / 3.0f);
   const double n = s[0] + s[1] + s[2];
   out[0] = n; out[1] = s[0]; out[2] = s[1]; out[3] = s[2];
   out[4] = s[3]; out[5] = s[4]; out[6] = s[5];
-  double pt[3] = {s[3], s[4], s[5]}, ked = 0.0;
-  for (int k = 0; k < v.nd; k++) {
-    const int c = k, nn = k + v.nd;
-    const double pc[3] = {v.msph[0] * v.dim[3][c], v.msph[0] * v.dim[4][c], v.msph[0] * v.dim[5][c]};
-    const double pn[3] = {v.msph[1] * v.dim[3][nn], v.msph[1] * v.dim[4][nn], v.msph[1] * v.dim[5][nn]};
-    for (int d = 0; d < 3; d++) pt[d] = (pt[d] + pc[d]) + pn[d];
-    ked = (ked + 0.5 * (1.0 / v.msph[0]) * (pc[0] * pc[0] + pc[1] * pc[1] + pc[2] * pc[2])) +
-          0.5 * (1.0 / v.msph[1]) * (pn[0] * pn[0] + pn[1] * pn[1] + pn[2] * pn[2]);
-  }
-  out[7] = pt[0]; out[8] = pt[1]; out[9] = pt[2];
-  out[10] = s[6]; out[11] = ked;
-  out[12] = n > 0 ? (s[6] / n) * (2.0 / 3.0) : 0.0;
-  out[13] = ked * (2.0 / 3.0);
-  out[14] = (s[6] + ked) + v.tpotential;
   // DotVZ of dimer 0 (Output_SelkovDimer.f90:147-173): centre-of-mass velocity along the N -> C unit vector
   const int c = 0, nn = v.nd;
   double d[3];
@@ -210,10 +205,33 @@ static int observables(const DevView& v, double* out) {
     }
   }
   const double len = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-  const double M = v.msph[1] + v.msph[0];
-  double dot = 0.0;
-  for (int a = 0; a < 3; a++) dot = dot + ((v.msph[0] * v.dim[3 + a][c] + v.msph[1] * v.dim[3 + a][nn]) / M) * (d[a] / len);
-  out[15] = dot;
+  const double mc = v.msph[0], mn = v.msph[1];
+  double vav[3];
+  for (int a = 0; a < 3; a++) vav[a] = (mc * v.dim[3 + a][c] + mn * v.dim[3 + a][nn]) / (mn + mc);
+  out[15] = (vav[0] * (d[0] / len) + vav[1] * (d[1] / len)) + vav[2] * (d[2] / len);
+  double pt[3] = {s[3], s[4], s[5]}, ked = 0.0;
+  if (selkov) {   // Output_SelkovDimer.f90:270-285
+    double pd[3];
+    for (int a = 0; a < 3; a++) {
+      pd[a] = (mc + mn) * vav[a];
+      pt[a] = pt[a] + pd[a];
+    }
+    ked = 0.0 + (0.5 * (1.0 / (mc + mn))) * ((pd[0] * pd[0] + pd[1] * pd[1]) + pd[2] * pd[2]);
+  } else {        // output_mod.f95:719-732
+    for (int k = 0; k < v.nd; k++) {
+      const int sc = k, sn = k + v.nd;
+      const double pc[3] = {mc * v.dim[3][sc], mc * v.dim[4][sc], mc * v.dim[5][sc]};
+      const double pn[3] = {mn * v.dim[3][sn], mn * v.dim[4][sn], mn * v.dim[5][sn]};
+      for (int a = 0; a < 3; a++) pt[a] = (pt[a] + pc[a]) + pn[a];
+      ked = (ked + (0.5 * (1.0 / mc)) * ((pc[0] * pc[0] + pc[1] * pc[1]) + pc[2] * pc[2])) +
+            (0.5 * (1.0 / mn)) * ((pn[0] * pn[0] + pn[1] * pn[1]) + pn[2] * pn[2]);
+    }
+  }
+  out[7] = pt[0]; out[8] = pt[1]; out[9] = pt[2];
+  out[10] = s[6]; out[11] = ked;
+  out[12] = n > 0 ? (s[6] / n) * two_thirds : 0.0;
+  out[13] = ked * two_thirds;
+  out[14] = (ked + s[6]) + v.tpotential;
   return 0;
 }
 
@@ -259,14 +277,14 @@ extern "C" int mpcx_co_observables(mpcx_co* e, double* out16) {
   if (!out16) return set_error(MPCX_E_ARG, "null argument");
   DevView v;
   int rc = co_view(e, &v);
-  return rc ? rc : observables(v, out16);
+  return rc ? rc : observables(v, out16, false);
 }
 
 extern "C" int mpcx_sk_observables(mpcx_sk* e, double* out16) {
   if (!out16) return set_error(MPCX_E_ARG, "null argument");
   DevView v;
   int rc = sk_view(e, &v);
-  return rc ? rc : observables(v, out16);
+  return rc ? rc : observables(v, out16, true);
 }
 
 extern "C" int mpcx_co_conc_slabs(mpcx_co* e, double x_slab, double y_slab, double z_slab, double half_width,

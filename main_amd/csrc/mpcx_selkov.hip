@@ -944,9 +944,11 @@ __global__ __launch_bounds__(256) void sk_rotate(SkP P, SkD D, int overlap) {
       nb = (double)q * (1.0 / 32.0) <= reach;
       if (!overlap && nb) atomicOr(&D.near_bits[i >> 5], 1u << (i & 31));
     }
-    if (overlap) {   // lanes 0 and 32 hold words of indices below space + 32 (< nwords + 2 words)
+    if (overlap) {   // lanes 0 and 32 write the wave's two words; words past the last particle
+                     // ((space + 31) / 32 of them hold indices below space) are left alone
       const unsigned long long m = __ballot(nb);
-      if (lane == 0 || lane == 32) D.near_bits[i >> 5] = (unsigned)(m >> lane);
+      if ((lane == 0 || lane == 32) && (long long)(i >> 5) < (P.space + 31) / 32)
+        D.near_bits[i >> 5] = (unsigned)(m >> lane);
     }
   }
 }
@@ -1027,9 +1029,9 @@ __global__ __launch_bounds__(256) void sk_sort_gather(SkP P, SkD D, const int* s
       }
     }
     const unsigned long long m = __ballot(bit);
-    const long long w0 = (b0 + (threadIdx.x & ~63)) >> 5;
-    if (lane == 0) abits[w0] = (unsigned)m;
-    if (lane == 32) abits[w0 + 1] = (unsigned)(m >> 32);
+    const long long w0 = (b0 + (threadIdx.x & ~63)) >> 5, nw = (P.space + 31) / 32;   // words holding slots
+    if (lane == 0 && w0 < nw) abits[w0] = (unsigned)m;
+    if (lane == 32 && w0 + 1 < nw) abits[w0 + 1] = (unsigned)(m >> 32);
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) D.sc->NT = NTn;
 }
@@ -1229,6 +1231,7 @@ extern "C" int mpcx_sk_create(mpcx_sk** out, const mpcx_sk_config* cfg) {
   SKCHK(hipMemcpy(e->d_alt, e->alt, sizeof(e->alt), hipMemcpyHostToDevice));
   SKCHK(hipMemset(D.sc, 0, sizeof(SkScalars)));
   SKCHK(hipMemset(D.near_bits, 0, (e->nwords + 2) * 4));
+  SKCHK(hipMemset(e->alt_bits, 0, (e->nwords + 2) * 4));   // words past the last particle stay zero in both
   SKCHK(hipMemset(D.cflag, 0, NS));
   *out = e;
   return 0;

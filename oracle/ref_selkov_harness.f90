@@ -111,6 +111,60 @@ subroutine ref_sk_run(ip, rp, seed, cstep0, nsteps, flg, xs, ys, zs, vxs, vys, v
   DEALLOCATE(newFXsol,newFYsol,newFZsol,newpFXsol,newpFYsol,newpFZsol)
 end subroutine
 
+! The reference's own diagnostics on a given state: fills the globals that
+! Output_SelkovDimer (Output_SelkovDimer.f90:1-305) and
+! Output_Concentration_Catalytic (Output_Concentration_Catalytic.f90:1-191)
+! read, sets cstep and Freq1..Freq6 from freq(7) = (cstep, Freq1..Freq6), and
+! calls both in the current working directory, where they write their .dat
+! records (DotVZ, Concentration, EnergyMomentum, ShellConcentration_Catalytic
+! when cstep is a multiple of Freq3..Freq5; choose Freq6 not dividing cstep so
+! the flow-field section stays off).  The census tF/tG/tI and the affected
+! mask (Vol = 64000 entries) are taken as given.
+!   rp(double): 1-3 box, 4 Mf, 5 Mg, 6 Mi, 7 Mc, 8 Mn, 9 Dcn, 10 TPotential
+subroutine ref_sk_output(ip, rp, freq, flg, xs, ys, zs, vxs, vys, vzs, dm, tf, tg, ti, aff) &
+    bind(C, name="ref_sk_output")
+  integer(c_int64_t) :: ip(2)
+  real(c_double) :: rp(10)
+  integer(c_int64_t) :: freq(7)
+  integer(c_int64_t) :: flg(ip(2))
+  real(c_double), dimension(ip(2)) :: xs, ys, zs, vxs, vys, vzs
+  real(c_double) :: dm(2,12)
+  integer(c_int64_t), dimension(64000) :: tf, tg, ti, aff
+  integer(c_int64_t) :: q
+
+  NT = ip(1); space = ip(2)
+  Box = rp(1:3); Mf = rp(4); Mg = rp(5); Mi = rp(6); Mc = rp(7); Mn = rp(8); Dcn = rp(9)
+  TPotential = rp(10)
+  do q = 1, 3
+    HBox(q) = 0.5d0*Box(q)
+    iBox(q) = INT(Box(q))
+  end do
+  cstep = freq(1)
+  Freq1 = freq(2); Freq2 = freq(3); Freq3 = freq(4); Freq4 = freq(5); Freq5 = freq(6); Freq6 = freq(7)
+
+  ALLOCATE(Xdim(1:2),Ydim(1:2),Zdim(1:2),VXdim(1:2),VYdim(1:2),VZdim(1:2),FXdim(1:2),FYdim(1:2),FZdim(1:2))
+  ALLOCATE(Xsol(1:space),Ysol(1:space),Zsol(1:space),FLAG(1:space))
+  ALLOCATE(VXsol(1:space),VYsol(1:space),VZsol(1:space),FXsol(1:space),FYsol(1:space),FZsol(1:space))
+  ALLOCATE(PXsol(1:space),PYsol(1:space),PZsol(1:space))
+  ALLOCATE(tFCellN(1:Vol),tGCellN(1:Vol),tICellN(1:Vol),affectedCellN(1:Vol))
+  ALLOCATE(SolCenterX(1:space),SolCenterY(1:space),SolCenterZ(1:space))
+  ALLOCATE(Rx(1:space),Ry(1:space),Rz(1:space),R2(1:space))
+
+  FLAG = flg; Xsol = xs; Ysol = ys; Zsol = zs; VXsol = vxs; VYsol = vys; VZsol = vzs
+  FXsol = 0.d0; FYsol = 0.d0; FZsol = 0.d0
+  PXsol = 0.d0; PYsol = 0.d0; PZsol = 0.d0
+  Xdim = dm(:,1); Ydim = dm(:,2); Zdim = dm(:,3); VXdim = dm(:,4); VYdim = dm(:,5); VZdim = dm(:,6)
+  FXdim = dm(:,7); FYdim = dm(:,8); FZdim = dm(:,9)
+  tFCellN = tf; tGCellN = tg; tICellN = ti; affectedCellN = aff
+
+  call Output_SelkovDimer
+  call Output_Concentration_Catalytic
+
+  DEALLOCATE(Xdim,Ydim,Zdim,VXdim,VYdim,VZdim,FXdim,FYdim,FZdim,Xsol,Ysol,Zsol,FLAG)
+  DEALLOCATE(VXsol,VYsol,VZsol,FXsol,FYsol,FZsol,PXsol,PYsol,PZsol)
+  DEALLOCATE(tFCellN,tGCellN,tICellN,affectedCellN,SolCenterX,SolCenterY,SolCenterZ,Rx,Ry,Rz,R2)
+end subroutine
+
 ! The fixed-form records of Output_SelkovDimer.f90 (FORMAT 15 :26, 16 :139,
 ! 18 :186, 19 :238, 17 :305) and Main_SelkovDimer.f90 (23 :158), written with
 ! the reference's FORMAT statements into the file at p: one record of each.
