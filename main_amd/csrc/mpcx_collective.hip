@@ -45,6 +45,7 @@ constexpr int MAXEV = 64;    // rebuilds recorded per interval
 constexpr int BMAX = 64;     // particles per collision cell in the ordered sum
 constexpr int CENSUS_MAX = 100;  // MaxA/MaxB/MaxS, global_mod.f95:114-116
 constexpr int CT = 256;      // threads of the per-cell kernel
+constexpr int NXCD = 8;      // XCDs (accelerator complex dies) of an MI355X, one L2 each
 // The reference caps its RATTLE iterations at 2e8 (evolve_mod.f95:104,115); on
 // the GPU a non-converging constraint stops at this cap and is reported.
 constexpr long long RATTLE_CAP = 100000;
@@ -69,6 +70,7 @@ struct CoScalars {
   int n_wall, pad_w;            // far particles co_far left to co_bin_rest (they reach a wall)
   int n_late, pad_l;            // late collision cells listed (zone around the spheres, deferred particles)
   unsigned long long next_fmax2;   // far max |v|^2 of the next interval while far_maxv2 is still in use
+  unsigned xmask, pad_xm;          // split storage: XCDs whose accumulator copies hold this interval's sums
 };
 
 // Dimers whose near regions can overlap within an interval form a cluster;
@@ -158,6 +160,8 @@ struct CoD {
   // the z word; and the cell records written from them (vcm x y z, count; axis)
   unsigned long long* sacc;
   float4* srec;
+  int* xtc;               // split storage B census, one copy per XCD: [8][vol]
+  unsigned* xfb;          // split storage first-B keys, one copy per XCD: [8][vol]
 };
 
 // ------------------------------------------------------------- streaming
@@ -3421,6 +3425,10 @@ struct SpD {
   signed char* flag;
 };
 
+// The cell of a split-storage particle is held as its packed coordinates
+// ix | iy << 11 | iz << 22 (boxes up to 2048 x 2048 x 1024 cells), so no
+// kernel divides by the box edges to find them.
+__device__ __forceinline__ int sp_pack(int ix, int iy, int iz) { return ix | (iy << 11) | (iz << 22); }
 __device__ __forceinline__ void sp_split(double x, double y, double z, const CoP& P, int& cell, float& ox, float& oy,
                                          float& oz) {
   int ix = (int)floor(x), iy = (int)floor(y), iz = (int)floor(z);
@@ -3431,14 +3439,29 @@ __device__ __forceinline__ void sp_split(double x, double y, double z, const CoP
   if (iz >= P.Lz) { iz = P.Lz - 1; fz = 0x1.fffffep-1f; }   // within half an FP32 ulp of the top wall
   if (ix >= P.Lx) ix -= P.Lx; else if (ix < 0) ix += P.Lx;
   if (iy >= P.Ly) iy -= P.Ly; else if (iy < 0) iy += P.Ly;
-  cell = ix + P.Lx * (iy + P.Ly * iz);
+  cell = sp_pack(ix, iy, iz);
   ox = fx; oy = fy; oz = fz;
 }
 __device__ __forceinline__ void sp_cell_xyz(const CoP& P, int cell, int& ix, int& iy, int& iz) {
-  ix = cell % P.Lx;
-  const int r = cell / P.Lx;
-  iy = r % P.Ly;
-  iz = r / P.Ly;
+  (void)P;
+  ix = cell & 2047;
+  iy = (cell >> 11) & 2047;
+  iz = (int)((unsigned)cell >> 22);
+}
+
+// a split particle whose cell lies in the box and whose offsets and
+// velocity are finite (anything else -- say a velocity so large that its
+// flight overflows the cell coordinates -- is reported, never binned)
+__device__ __forceinline__ bool sp_sane(const CoP& P, int ix, int iy, int iz, float ox, float oy, float oz, float vx,
+                                        float vy, float vz) {
+  return ix >= 0 && ix < P.Lx && iy >= 0 && iy < P.Ly && iz >= 0 && iz < P.Lz && ox >= 0.0f && ox < 1.0f &&
+         oy >= 0.0f && oy < 1.0f && oz >= 0.0f && oz < 1.0f && __builtin_isfinite(vx) && __builtin_isfinite(vy) &&
+         __builtin_isfinite(vz);
+}
+// an FP64 position inside the box (z walls: 0 <= z <= Lz) and a finite velocity
+__device__ __forceinline__ bool fp64_sane(const CoP& P, double x, double y, double z, double vx, double vy, double vz) {
+  return x >= 0.0 && x < P.box[0] && y >= 0.0 && y < P.box[1] && z >= 0.0 && z <= P.box[2] &&
+         __builtin_isfinite(vx) && __builtin_isfinite(vy) && __builtin_isfinite(vz);
 }
 
 // FP64 staging <-> split storage, for all particles or the near list
@@ -3446,6 +3469,10 @@ __global__ __launch_bounds__(256) void sp_store(CoP P, CoD D, SpD S, int near_on
   const long long n = near_only ? (long long)min(D.sc->near_num, D.near_cap) : (long long)P.NT;
   for (long long j = (long long)blockIdx.x * 256 + threadIdx.x; j < n; j += (long long)gridDim.x * 256) {
     const int i = near_only ? D.near_list[j] : (int)j;
+    if (!fp64_sane(P, D.x[i], D.y[i], D.z[i], D.vx[i], D.vy[i], D.vz[i])) {
+      raise_error(&D.sc->err, ERR_CELL_RANGE);
+      continue;
+    }
     int c;
     float ox, oy, oz;
     sp_split(D.x[i], D.y[i], D.z[i], P, c, ox, oy, oz);
@@ -3884,6 +3911,7 @@ extern "C" int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg) {
   HIPCHK(hipMemcpy(ptab, cfg->reaction_p, 101 * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemset(D.sc, 0, sizeof(CoScalars)));
   HIPCHK(hipMemset(D.status, 0, NT));
+  HIPCHK(hipMemset(D.cell_of, 0, NT * 4));
   HIPCHK(hipMemset(D.near_bits, 0, (e->nwords + 1) * 4));
   HIPCHK(hipMemset(D.next_bits, 0, (e->nwords + 1) * 4));
   HIPCHK(hipMemset(D.stamp, 0, (size_t)P.mpcnumcell * 4));
@@ -3907,6 +3935,8 @@ extern "C" int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg) {
 extern "C" int mpcx_co_set_precision(mpcx_co* e, int32_t split) {
   if (!e) return set_error(MPCX_E_ARG, "null engine");
   if (split && e->P.rng_mode != 1) return set_error(MPCX_E_UNSUPPORTED, "split precision runs with device Philox draws");
+  if (split && (e->P.Lx > 2048 || e->P.Ly > 2048 || e->P.Lz > 1024))
+    return set_error(MPCX_E_UNSUPPORTED, "split precision holds boxes up to 2048 x 2048 x 1024 cells");
   if (split && !e->S.cell) {
     const size_t NT = (size_t)e->P.NT;
     for (SpD* q : {&e->S, &e->Salt}) {
@@ -3917,12 +3947,16 @@ extern "C" int mpcx_co_set_precision(mpcx_co* e, int32_t split) {
       if (!q->cell || !q->ox || !q->oy || !q->oz || !q->vx || !q->vy || !q->vz || !q->flag)
         return set_error(MPCX_E_HIP, "device allocation failed");
     }
-    const size_t mc = (size_t)e->P.mpcnumcell;
-    e->D.sacc = dalloc<unsigned long long>(e, 3 * mc);
+    const size_t mc = (size_t)e->P.mpcnumcell, vol = (size_t)e->P.vol;
+    e->D.sacc = dalloc<unsigned long long>(e, NXCD * 3 * mc);
     e->D.srec = dalloc<float4>(e, 2 * mc);
-    if (!e->D.sacc || !e->D.srec) return set_error(MPCX_E_HIP, "device allocation failed");
-    HIPCHK(hipMemset(e->D.sacc, 0, 3 * mc * 8));
+    e->D.xtc = dalloc<int>(e, NXCD * vol);
+    e->D.xfb = dalloc<unsigned>(e, NXCD * vol);
+    if (!e->D.sacc || !e->D.srec || !e->D.xtc || !e->D.xfb) return set_error(MPCX_E_HIP, "device allocation failed");
+    HIPCHK(hipMemset(e->D.sacc, 0, NXCD * 3 * mc * 8));
     HIPCHK(hipMemset(e->D.srec, 0, 2 * mc * 16));
+    HIPCHK(hipMemset(e->D.xtc, 0, NXCD * vol * 4));
+    HIPCHK(hipMemset(e->D.xfb, 0xff, NXCD * vol * 4));
   }
   e->split = split != 0;
   return 0;
@@ -4865,8 +4899,8 @@ extern "C" int mpcx_co_get_cells(mpcx_co* e, double* vcm_uv, int32_t* cell_of, i
   if (vcm_uv && e->split) {   // FP32 records (vcm, count; axis) by tour position
     std::vector<float> rec(mc * 8);
     HIPCHK(hipMemcpy(rec.data(), D.srec, mc * 32, hipMemcpyDeviceToHost));
-    for (size_t t = 0; t < mc; t++) {
-      double* o = vcm_uv + (size_t)e->mpc_cell[t] * 8;
+    for (size_t t = 0; t < mc; t++) {   // split records are by collision cell
+      double* o = vcm_uv + t * 8;
       const float* r = &rec[t * 8];
       o[0] = r[0]; o[1] = r[1]; o[2] = r[2]; o[3] = r[4]; o[4] = r[5]; o[5] = r[6]; o[6] = r[3]; o[7] = 0.0;
     }
@@ -4879,7 +4913,7 @@ extern "C" int mpcx_co_get_cells(mpcx_co* e, double* vcm_uv, int32_t* cell_of, i
     HIPCHK(hipMemcpy(cell_of, D.cell_of, (size_t)e->P.NT * 4, hipMemcpyDeviceToHost));
     for (long long i = 0; i < e->P.NT; i++) {
       if (cell_of[i] < 0 || (size_t)cell_of[i] >= mc) return set_error(MPCX_E_DEVICE, "cell index out of range");
-      cell_of[i] = e->mpc_cell[cell_of[i]];
+      if (!e->split) cell_of[i] = e->mpc_cell[cell_of[i]];   // FP64 storage binned by tour position
     }
   }
   if (census && e->split) return set_error(MPCX_E_UNSUPPORTED, "split storage clears its census after each collision");
@@ -5134,58 +5168,149 @@ __global__ __launch_bounds__(256) void sort_gather_b(const CoD* Ds, const CoSort
 // 2 the reactions (counter: unshifted cell), keyed by the system's seed.
 // =====================================================================================
 constexpr double SB_SCALE = 0x1p40;   // fixed point of the cell sums
-constexpr int SB_NBITS = 12;          // count bits below the z sum (at most 4095 members)
+// The sums and the census are added with atomics that execute in the L2 of
+// the XCD the workgroup runs on (workgroup scope: no bypass to memory, where
+// device-scope atomics serialise at about 3e10/s); each of the eight XCDs
+// adds into its own copy of the accumulators, named by the hardware XCC_ID
+// register, so no two L2s ever hold the same accumulator line; sb_cells adds
+// the copies (integer sums: the result does not depend on how the work was
+// spread) after the kernels that fill them have ended.
+__device__ __forceinline__ int hw_xcc() {
+  unsigned v;
+  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+  return (int)(v & (unsigned)(NXCD - 1));
+}
+// Global-memory views of one system's arrays for the split kernels.  The
+// descriptors are read from device memory, so the compiler cannot tell their
+// pointers are global and would emit FLAT accesses, which also count against
+// the LDS wait counter: every LDS wait of the binning would then wait for all
+// outstanding memory operations.  Address-space 1 pointers give GLOBAL ones.
+template <typename T>
+using gp = __attribute__((address_space(1))) T*;
+typedef float f4v __attribute__((ext_vector_type(4)));   // a cell record half (builtin vector: no operator= member)
+template <typename T>
+__device__ __forceinline__ gp<T> g1(T* p) { return (gp<T>)p; }
+struct SbV {
+  gp<int> cell;
+  gp<float> ox, oy, oz, vx, vy, vz;
+  gp<signed char> flag;
+  gp<unsigned> near_bits;
+  gp<unsigned char> status, dq;
+  gp<int> cell_of;
+  gp<unsigned long long> acc;   // this workgroup's XCD copy: [3 * mpcnumcell], by collision cell
+  gp<int> tc;                   // B census: [vol]
+  gp<unsigned> fb;              // first-B keys: [vol]
+};
+__device__ __forceinline__ SbV sb_view(const CoP& P, const CoD& D, const SpD& S, int x) {
+  SbV v;
+  v.cell = g1(S.cell); v.ox = g1(S.ox); v.oy = g1(S.oy); v.oz = g1(S.oz);
+  v.vx = g1(S.vx); v.vy = g1(S.vy); v.vz = g1(S.vz); v.flag = g1(S.flag);
+  v.near_bits = g1(D.near_bits); v.status = g1(D.status); v.dq = g1(D.dq); v.cell_of = g1(D.cell_of);
+  v.acc = g1(D.sacc + (long long)x * 3 * P.mpcnumcell);
+  v.tc = g1(D.xtc + (long long)x * P.vol);
+  v.fb = g1(D.xfb + (long long)x * P.vol);
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ void l2_add(gp<T> p, T v) {
+  (void)__hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void l2_min(gp<unsigned> p, unsigned v) {
+  (void)__hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 
+constexpr int SB_NBITS = 12;          // count bits below the z sum (at most 4095 members)
 __device__ __forceinline__ long long sb_fix(float v) { return __double2ll_rn((double)v * SB_SCALE); }
 
-// per-workgroup hash tables of the binning (as GroupTables): cell sums, census counts and minimum keys
+// Per-workgroup hash tables of the binning, filled over the workgroup's whole
+// tile and added to the global sums once at its end (one atomic per word and
+// distinct cell or census key of the tile): collision-cell sums (the count in
+// the low bits of the z word) and the B census (count and minimum key per
+// unshifted cell).  A key
+// that finds no free slot within SB_PROBE probes goes to the global sums
+// directly.
+constexpr int SB_GT = 1024;    // slots of each table
+constexpr int SB_PROBE = 64;
 struct SbTables {
-  int rk[GT], rc[GT];
-  unsigned long long rs[3][GT];
-  int ck[GT], cc[GT];
-  unsigned cm[GT];
+  int rk[SB_GT];
+  unsigned long long rs[3][SB_GT];
+  int ck[SB_GT], cc[SB_GT];
+  unsigned cm[SB_GT];
 };
 __device__ __forceinline__ void sb_tables_init(SbTables& G) {
-  for (int j = threadIdx.x; j < GT; j += blockDim.x) {
-    G.rk[j] = -1; G.rc[j] = 0; G.rs[0][j] = 0ull; G.rs[1][j] = 0ull; G.rs[2][j] = 0ull;
+  for (int j = threadIdx.x; j < SB_GT; j += blockDim.x) {
+    G.rk[j] = -1; G.rs[0][j] = 0ull; G.rs[1][j] = 0ull; G.rs[2][j] = 0ull;
     G.ck[j] = -1; G.cc[j] = 0; G.cm[j] = 0xffffffffu;
   }
   __syncthreads();
 }
-// Every thread of the block together: each lane's velocity into its cell t
-// (ok), its census entry ck with key (cen); the block's groups are added to
-// the global sums and the tables left cleared for the next round.
-__device__ __forceinline__ void sb_groups(const CoP& P, const CoD& D, SbTables& G, bool ok, int t, float vx, float vy,
+// slot of key in keys (inserted if absent), or -1 after SB_PROBE full probes
+__device__ __forceinline__ int sb_slot(int* keys, int key) {
+  int j = (int)(((unsigned)key * 2654435761u) >> 22);   // 10 bits
+#pragma unroll 1
+  for (int p = 0; p < SB_PROBE; p++) {
+    const int old = atomicCAS(&keys[j], -1, key);
+    if (old == -1 || old == key) return j;
+    j = (j + 1) & (SB_GT - 1);
+  }
+  return -1;
+}
+// one lane's velocity into cell t (ok) and census entry ck with key (cen); no barrier
+__device__ __forceinline__ void sb_insert(const CoP& P, const SbV& A, SbTables& G, bool ok, int t, float vx, float vy,
                                           float vz, bool cen, int ck, unsigned key) {
   if (ok) {
-    const int j = gt_slot(G.rk, t);
-    atomicAdd(&G.rc[j], 1);
-    atomicAdd(&G.rs[0][j], (unsigned long long)sb_fix(vx));
-    atomicAdd(&G.rs[1][j], (unsigned long long)sb_fix(vy));
-    atomicAdd(&G.rs[2][j], ((unsigned long long)sb_fix(vz) << SB_NBITS) + 1ull);
+    const unsigned long long w0 = (unsigned long long)sb_fix(vx), w1 = (unsigned long long)sb_fix(vy);
+    const unsigned long long w2 = ((unsigned long long)sb_fix(vz) << SB_NBITS) + 1ull;
+    const int j = sb_slot(G.rk, t);
+    if (j >= 0) {
+      atomicAdd(&G.rs[0][j], w0);
+      atomicAdd(&G.rs[1][j], w1);
+      atomicAdd(&G.rs[2][j], w2);
+    } else {
+      const gp<unsigned long long> a = A.acc + 3ll * t;
+      l2_add<unsigned long long>(a, w0);
+      l2_add<unsigned long long>(a + 1, w1);
+      l2_add<unsigned long long>(a + 2, w2);
+    }
   }
   if (cen) {
-    const int j = gt_slot(G.ck, ck);
-    atomicAdd(&G.cc[j], 1);
-    atomicMin(&G.cm[j], key);
-  }
-  __syncthreads();
-  for (int j = threadIdx.x; j < GT; j += blockDim.x) {
-    if (G.rc[j]) {
-      unsigned long long* a = D.sacc + 3ll * G.rk[j];
-      atomicAdd(&a[0], G.rs[0][j]);
-      atomicAdd(&a[1], G.rs[1][j]);
-      atomicAdd(&a[2], G.rs[2][j]);
-      G.rk[j] = -1; G.rc[j] = 0; G.rs[0][j] = 0ull; G.rs[1][j] = 0ull; G.rs[2][j] = 0ull;
-    }
-    if (G.cc[j]) {
-      const int k = G.ck[j];
-      atomicAdd(&D.tcount[k], G.cc[j]);   // checked against CENSUS_MAX by sb_cells
-      if (k >= P.vol && k < 2 * P.vol) atomicMin(&D.firstB[k - P.vol], G.cm[j]);
-      G.ck[j] = -1; G.cc[j] = 0; G.cm[j] = 0xffffffffu;
+    const int j = sb_slot(G.ck, ck);
+    if (j >= 0) {
+      atomicAdd(&G.cc[j], 1);
+      atomicMin(&G.cm[j], key);
+    } else {
+      l2_add<int>(A.tc + ck, 1);
+      l2_min(A.fb + ck, key);
     }
   }
+}
+// every thread of the block together, after the tile: the tables into the global sums
+__device__ __forceinline__ void sb_flush(const CoP& P, const SbV& A, SbTables& G) {
   __syncthreads();
+  for (int j = threadIdx.x; j < SB_GT; j += blockDim.x) {
+    const int t = G.rk[j];
+    if (t >= 0) {
+      const gp<unsigned long long> a = A.acc + 3ll * t;
+      l2_add<unsigned long long>(a, G.rs[0][j]);
+      l2_add<unsigned long long>(a + 1, G.rs[1][j]);
+      l2_add<unsigned long long>(a + 2, G.rs[2][j]);
+    }
+    const int k = G.ck[j];
+    if (k >= 0) {
+      l2_add<int>(A.tc + k, G.cc[j]);   // checked against CENSUS_MAX by sb_cells
+      l2_min(A.fb + k, G.cm[j]);
+    }
+  }
+}
+
+// The spheres of the distance bytes: each coordinate as its cell (integer)
+// and the offset in it (FP32)
+struct SbSpheres {
+  int ic[3][MAXSG];
+  float fr[3][MAXSG];
+};
+__device__ __forceinline__ float sp_pbc(float d, float L) {
+  return d > 0.5f * L ? d - L : (d < -0.5f * L ? d + L : d);
 }
 
 // Binning of one split-storage particle per lane (every thread of the block
@@ -5194,10 +5319,10 @@ __device__ __forceinline__ void sb_groups(const CoP& P, const CoD& D, SbTables& 
 // entry (outside-list particles keyed by index, inside ones after them, inside
 // ones within an LJ cutoff left out), and the distance byte to the spheres sph
 // (the selection of the next near set).
-__device__ __forceinline__ void sb_bin(const CoP& P, const CoD& D, SbTables& G, int i, bool valid, bool inside,
+__device__ __forceinline__ void sb_bin(const CoP& P, const CoD& D, const SbV& A, SbTables& G, int i, bool valid, bool inside,
                                        bool excl, int ix, int iy, int iz, float ox, float oy, float oz, float vx,
                                        float vy, float vz, int f, float fxs, float fys, float fzs, double zs,
-                                       const double (*sph)[MAXSG]) {
+                                       const SbSpheres& sph) {
   bool ok = valid;
   int t = 0;
   if (valid) {
@@ -5205,52 +5330,61 @@ __device__ __forceinline__ void sb_bin(const CoP& P, const CoD& D, SbTables& G, 
     const int cy = sp_wrap(iy + (int)floorf(oy + fys), P.Ly);
     const int cz = iz + (int)floorf(oz + fzs) + (zs < 0.0 ? 1 : 0);
     const long long c = (long long)cx + (long long)P.Lx * cy + (long long)P.Lx * P.Ly * cz;
-    if (cz < 0 || c >= P.mpcnumcell) { raise_error(&D.sc->err, ERR_CELL_RANGE); ok = false; }
-    else t = D.mpc_pos[c];
+    if (cx < 0 || cx >= P.Lx || cy < 0 || cy >= P.Ly || cz < 0 || c >= P.mpcnumcell) {
+      raise_error(&D.sc->err, ERR_CELL_RANGE);
+      ok = false;
+    } else {
+      t = (int)c;
+    }
   }
   unsigned key = (unsigned)i;
   bool cen = false;
   int ck = 0;
-  if (ok) {
+  if (ok && f == 1 && !(inside && excl)) {   // the B census (all the bulk reaction reads)
     if (inside) key = (unsigned)P.NT + (unsigned)i;
-    const int sp = f == 0 ? 0 : f == 1 ? 1 : f == 2 ? 2 : -1;
-    if (!(inside && excl) && sp >= 0) {
-      const long long u = (long long)ix + (long long)P.Lx * iy + (long long)P.Lx * P.Ly * iz;
-      if (u < 0 || u >= P.vol) raise_error(&D.sc->err, ERR_CELL_RANGE);
-      else { cen = true; ck = (int)(sp * P.vol + u); }
-    }
+    const long long u = (long long)ix + (long long)P.Lx * iy + (long long)P.Lx * P.Ly * iz;
+    if (u < 0 || u >= P.vol) raise_error(&D.sc->err, ERR_CELL_RANGE);
+    else { cen = true; ck = (int)u; }
   }
-  sb_groups(P, D, G, ok, t, vx, vy, vz, cen, ck, key);
+  sb_insert(P, A, G, ok, t, vx, vy, vz, cen, ck, key);
   if (valid) {
-    D.cell_of[i] = t;
-    const double x = (double)ix + (double)ox, y = (double)iy + (double)oy, z = (double)iz + (double)oz;
-    double dmin = 1e30;
+    A.cell_of[i] = t;
+    // distance beyond each list radius in FP32, from differences of cells
+    // (exact integers) and of offsets, so no large coordinate is rounded; the
+    // rounding (under 1e-5 at these distances) is covered by the 1e-4 taken off
+    float dmin = 1e30f;
     bool close = false;
+    auto d2_of = [&](int s) {
+      const float a = sp_pbc((float)(ix - sph.ic[0][s]) + (ox - sph.fr[0][s]), (float)P.Lx);
+      const float b = sp_pbc((float)(iy - sph.ic[1][s]) + (oy - sph.fr[1][s]), (float)P.Ly);
+      const float c = (float)(iz - sph.ic[2][s]) + (oz - sph.fr[2][s]);
+      return (a * a + b * b) + c * c;
+    };
     for (int s = 0; s < 2 * P.nd; s++) {
-      const double a = pbc_half(x - sph[0][s], P.box[0]);
-      const double b = pbc_half(y - sph[1][s], P.box[1]);
-      const double c = z - sph[2][s];
-      const double d2 = a * a + b * b + c * c, far = P.full[s >= P.nd] + 8.0;
-      close |= d2 < far * far;
+      const float far = (float)P.full[s >= P.nd] + 8.0f;
+      close |= d2_of(s) < far * far;
     }
     if (__ballot(close))   // most waves lie far from every sphere: no square roots at all
       for (int s = 0; s < 2 * P.nd; s++) {
-        const double a = pbc_half(x - sph[0][s], P.box[0]);
-        const double b = pbc_half(y - sph[1][s], P.box[1]);
-        const double c = z - sph[2][s];
-        const double d2 = a * a + b * b + c * c, far = P.full[s >= P.nd] + 8.0;
-        if (d2 < far * far) dmin = fmin(dmin, sqrt(d2) - P.full[s >= P.nd]);
+        const float far = (float)P.full[s >= P.nd] + 8.0f, d2 = d2_of(s);
+        if (d2 < far * far) dmin = fminf(dmin, sqrtf(d2) - (float)P.full[s >= P.nd]);
       }
-    // 1e-5 below the measured distance: the FP32 position must not make a particle look farther
-    double qd = floor(fmax(dmin - 1e-5, 0.0) * 32.0);
-    if (inside) qd = 0.0;   // list members stay near until the next rebuild
-    D.dq[i] = (unsigned char)fmin(qd, 255.0);
+    float qd = floorf(fmaxf(dmin - 1e-4f, 0.0f) * 32.0f);
+    if (inside) qd = 0.0f;   // list members stay near until the next rebuild
+    A.dq[i] = (unsigned char)fminf(qd, 255.0f);
   }
 }
 
 __device__ __forceinline__ void sb_load_spheres(const CoP& P, const double* x, const double* y, const double* z,
-                                                double (*sph)[MAXSG]) {
-  for (int j = threadIdx.x; j < 2 * P.nd; j += blockDim.x) { sph[0][j] = x[j]; sph[1][j] = y[j]; sph[2][j] = z[j]; }
+                                                SbSpheres& sph) {
+  for (int j = threadIdx.x; j < 2 * P.nd; j += blockDim.x) {
+    const double c[3] = {x[j], y[j], z[j]};
+    for (int a = 0; a < 3; a++) {
+      const double f = floor(c[a]);
+      sph.ic[a][j] = (int)f;
+      sph.fr[a][j] = (float)(c[a] - f);
+    }
+  }
 }
 
 // Tile of particles of one system.  ALL = false (beside the near engine): the
@@ -5260,15 +5394,48 @@ __device__ __forceinline__ void sb_load_spheres(const CoP& P, const double* x, c
 // positions.  ALL = true (a re-sort interval, after the sort and the near
 // engine): every particle binned where it is, distances to the spheres' end
 // positions.
+// A split particle's state, loaded one round ahead of its use
+struct SbPart {
+  int cell, f;
+  float ox, oy, oz, vx, vy, vz;
+  unsigned char st;
+  bool near;
+};
+__device__ __forceinline__ void sb_fetch(const SbV& V, long long i, long long end, SbPart& p) {
+  p.near = false; p.cell = 0; p.f = -1; p.st = 0;
+  p.ox = p.oy = p.oz = p.vx = p.vy = p.vz = 0.0f;
+  if (i < end) {
+    p.near = (V.near_bits[i >> 5] >> (i & 31)) & 1u;
+    p.st = V.status[i];
+    p.cell = V.cell[i];
+    p.ox = V.ox[i]; p.oy = V.oy[i]; p.oz = V.oz[i];
+    p.vx = V.vx[i]; p.vy = V.vy[i]; p.vz = V.vz[i];
+    p.f = V.flag[i];
+  }
+}
+
+// Tile of particles of one system (an empty tile, begin == end, pads the
+// placement of tiles on XCDs).  ALL = false (beside the near engine): the
+// particles outside the interval's near set take one free flight of MPCtstep
+// steps (sp_stream: bounce-back at the z walls, the reactive patch of z = 0)
+// and, with do_bin, are binned with distances to the spheres' start
+// positions.  ALL = true (a re-sort interval, after the sort and the near
+// engine): every particle binned where it is, distances to the spheres' end
+// positions.  Each round's loads are issued before the previous round is
+// binned, so their latency overlaps the binning's LDS work and barriers.
 template <bool ALL>
 __global__ __launch_bounds__(256) void sb_bulk(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds,
                                                const SpD* __restrict__ Ss, const Tile* __restrict__ tiles, int do_bin) {
   __shared__ SbTables G;
-  __shared__ double sph[3][MAXSG];
+  __shared__ SbSpheres sph;
   const Tile tl = tiles[blockIdx.x];
+  if (tl.begin >= tl.end) return;
   const CoP P = Ps[tl.r];
   const CoD D = Ds[tl.r];
   const SpD S = Ss[tl.r];
+  const int xcc = hw_xcc();
+  const SbV V = sb_view(P, D, S, xcc);
+  if (do_bin && threadIdx.x == 0) atomicOr(&D.sc->xmask, 1u << xcc);
   if (ALL) sb_load_spheres(P, D.sc->x0, D.sc->y0, D.sc->z0, sph);
   else sb_load_spheres(P, D.sc->xs0, D.sc->ys0, D.sc->zs0, sph);
   if (do_bin) sb_tables_init(G);
@@ -5277,33 +5444,42 @@ __global__ __launch_bounds__(256) void sb_bulk(const CoP* __restrict__ Ps, const
   if (do_bin) collision_shift(P, D, xs, ys, zs);
   const float fxs = (float)xs, fys = (float)ys, fzs = (float)zs;
   const float dtc = (float)P.MDt * (float)P.MPCtstep;
+  SbPart nxt;
+  sb_fetch(V, tl.begin + threadIdx.x, tl.end, nxt);
   for (long long base = tl.begin; base < tl.end; base += 256) {
+    const SbPart cur = nxt;
+    sb_fetch(V, base + 256 + threadIdx.x, tl.end, nxt);
     const int i = (int)(base + threadIdx.x);
     const bool valid = i < tl.end;
-    const bool near = valid && ((D.near_bits[i >> 5] >> (i & 31)) & 1u);
-    const bool mine = valid && (ALL || !near);
-    int ix = 0, iy = 0, iz = 0, f = -1;
-    float ox = 0.0f, oy = 0.0f, oz = 0.0f, vx = 0.0f, vy = 0.0f, vz = 0.0f;
+    bool mine = valid && (ALL || !cur.near);
+    int ix = 0, iy = 0, iz = 0, f = cur.f;
+    float ox = cur.ox, oy = cur.oy, oz = cur.oz, vx = cur.vx, vy = cur.vy, vz = cur.vz;
     bool inside = false, excl = false;
     if (mine) {
-      const unsigned char st = D.status[i];
-      if (near) { inside = (st & ST_INSIDE) != 0; excl = (st & ST_EXCL) != 0; }
-      else if (st) D.status[i] = 0;
-      sp_cell_xyz(P, S.cell[i], ix, iy, iz);
-      ox = S.ox[i]; oy = S.oy[i]; oz = S.oz[i];
-      vx = S.vx[i]; vy = S.vy[i]; vz = S.vz[i];
-      f = S.flag[i];
+      if (cur.near) { inside = (cur.st & ST_INSIDE) != 0; excl = (cur.st & ST_EXCL) != 0; }
+      else if (cur.st) V.status[i] = 0;
+      sp_cell_xyz(P, cur.cell, ix, iy, iz);
       if (!ALL) {
         const int f0 = f;
         const bool bounced = sp_stream(ix, iy, iz, ox, oy, oz, vx, vy, vz, f, dtc, P, D.react_box, &D.sc->err);
-        S.cell[i] = ix + P.Lx * (iy + P.Ly * iz);
-        S.ox[i] = ox; S.oy[i] = oy; S.oz[i] = oz;
-        if (bounced) { S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz; }
-        if (f != f0) S.flag[i] = (signed char)f;
+        if (sp_sane(P, ix, iy, iz, ox, oy, oz, vx, vy, vz)) {
+          V.cell[i] = sp_pack(ix, iy, iz);
+          V.ox[i] = ox; V.oy[i] = oy; V.oz[i] = oz;
+          if (bounced) { V.vx[i] = vx; V.vy[i] = vy; V.vz[i] = vz; }
+          if (f != f0) V.flag[i] = (signed char)f;
+        } else {
+          raise_error(&D.sc->err, ERR_CELL_RANGE);
+          mine = false;
+        }
+      } else if (!sp_sane(P, ix, iy, iz, ox, oy, oz, vx, vy, vz)) {
+        raise_error(&D.sc->err, ERR_CELL_RANGE);
+        mine = false;
       }
     }
-    if (do_bin) sb_bin(P, D, G, i, mine, inside, excl, ix, iy, iz, ox, oy, oz, vx, vy, vz, f, fxs, fys, fzs, zs, sph);
+    if (do_bin)
+      sb_bin(P, D, V, G, i, mine, inside, excl, ix, iy, iz, ox, oy, oz, vx, vy, vz, f, fxs, fys, fzs, zs, sph);
   }
+  if (do_bin) sb_flush(P, V, G);
 }
 
 // Per system (blockIdx.y): the interval's start -- the collision counter and
@@ -5332,18 +5508,24 @@ __global__ __launch_bounds__(256) void sb_load_near(const CoP* __restrict__ Ps, 
   }
 }
 
-// Per system (blockIdx.y), after its near engine: the near set back into
+// After the near engines: near-set tile (replica r, chunk begin of end
+// chunks; begin == end pads the placement on XCDs) of the near list back into
 // split storage and, with do_bin, binned (distances to the spheres' start
 // positions, as sb_bulk<false>); with check, the flight check (the near
 // engine's rebuild events must add up to the interval sb_bulk flew).
 __global__ __launch_bounds__(256) void sb_near(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds,
-                                               const SpD* __restrict__ Ss, int do_bin, int check) {
+                                               const SpD* __restrict__ Ss, const Tile* __restrict__ tiles, int do_bin,
+                                               int check) {
   __shared__ SbTables G;
-  __shared__ double sph[3][MAXSG];
-  const int r = blockIdx.y;
-  const CoP P = Ps[r];
-  const CoD D = Ds[r];
-  const SpD S = Ss[r];
+  __shared__ SbSpheres sph;
+  const Tile tl = tiles[blockIdx.x];
+  if (tl.begin >= tl.end) return;
+  const CoP P = Ps[tl.r];
+  const CoD D = Ds[tl.r];
+  const SpD S = Ss[tl.r];
+  const int xcc = hw_xcc();
+  const SbV V = sb_view(P, D, S, xcc);
+  if (do_bin && threadIdx.x == 0) atomicOr(&D.sc->xmask, 1u << xcc);
   sb_load_spheres(P, D.sc->xs0, D.sc->ys0, D.sc->zs0, sph);
   if (do_bin) sb_tables_init(G);
   else __syncthreads();
@@ -5351,14 +5533,21 @@ __global__ __launch_bounds__(256) void sb_near(const CoP* __restrict__ Ps, const
   if (do_bin) collision_shift(P, D, xs, ys, zs);
   const float fxs = (float)xs, fys = (float)ys, fzs = (float)zs;
   const long long nn = min(D.sc->near_num, D.near_cap);
-  for (long long base = (long long)blockIdx.x * 256; base < nn; base += (long long)gridDim.x * 256) {
+  for (long long base = tl.begin * 256; base < nn; base += tl.end * 256) {
     const long long q = base + threadIdx.x;
-    const bool valid = q < nn;
+    bool valid = q < nn;
     int i = 0, ix = 0, iy = 0, iz = 0, f = -1;
     float ox = 0.0f, oy = 0.0f, oz = 0.0f, vx = 0.0f, vy = 0.0f, vz = 0.0f;
     bool inside = false, excl = false;
     if (valid) {
       i = D.near_list[q];
+      const double x = D.x[i], y = D.y[i], z = D.z[i], dvx = D.vx[i], dvy = D.vy[i], dvz = D.vz[i];
+      if (!fp64_sane(P, x, y, z, dvx, dvy, dvz)) {   // reported; the particle keeps its split state
+        raise_error(&D.sc->err, ERR_CELL_RANGE);
+        valid = false;
+      }
+    }
+    if (valid) {
       int c;
       sp_split(D.x[i], D.y[i], D.z[i], P, c, ox, oy, oz);
       vx = (float)D.vx[i]; vy = (float)D.vy[i]; vz = (float)D.vz[i];
@@ -5369,17 +5558,20 @@ __global__ __launch_bounds__(256) void sb_near(const CoP* __restrict__ Ps, const
       const unsigned char st = D.status[i];
       inside = (st & ST_INSIDE) != 0; excl = (st & ST_EXCL) != 0;
     }
-    if (do_bin) sb_bin(P, D, G, i, valid, inside, excl, ix, iy, iz, ox, oy, oz, vx, vy, vz, f, fxs, fys, fzs, zs, sph);
+    if (do_bin)
+      sb_bin(P, D, V, G, i, valid, inside, excl, ix, iy, iz, ox, oy, oz, vx, vy, vz, f, fxs, fys, fzs, zs, sph);
   }
-  if (check && blockIdx.x == 0 && threadIdx.x == 0) check_flight(P, D);
+  if (do_bin) sb_flush(P, V, G);
+  if (check && tl.begin == 0 && threadIdx.x == 0) check_flight(P, D);
 }
 
-// Tile of 256 cells of one system, once every particle is binned: the record
-// of collision cell t (mean velocity from the exact sums, random axis of
-// mpc_rotation_mod.f95:184-194) with its sums cleared; the census limits and
-// the bulk B->A reaction of unshifted cell t (reaction_mod.f95:28-66: p from
-// the host table of 1 - exp(-k1 nB tau), one draw per cell, the first B
-// member turned A) with its census cleared.
+// Tile of cells of one system, once every particle is binned: the record of
+// collision cell t (linear index; mean velocity from the exact sums of the
+// XCD copies, random axis of mpc_rotation_mod.f95:184-194) with its sums
+// cleared; the bulk B->A reaction of unshifted cell t (reaction_mod.f95:28-66:
+// p from the host table of 1 - exp(-k1 nB tau), one draw per cell, the first
+// B member in census order turned A) with its B census cleared.  Split
+// storage keeps the census of B only: nothing else reads the A and S counts.
 __device__ __forceinline__ void sb_cell(const CoP& P, const CoD& D, const SpD& S, long long t);
 __global__ __launch_bounds__(256) void sb_cells(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds,
                                                 const SpD* __restrict__ Ss, const Tile* __restrict__ tiles) {
@@ -5390,11 +5582,15 @@ __global__ __launch_bounds__(256) void sb_cells(const CoP* __restrict__ Ps, cons
   for (long long t = tl.begin + threadIdx.x; t < tl.end; t += 256) sb_cell(P, D, S, t);
 }
 __device__ __forceinline__ void sb_cell(const CoP& P, const CoD& D, const SpD& S, long long t) {
+  const unsigned xm = D.sc->xmask;   // copies filled this interval
   if (t == 0) D.sc->far_maxv2 = 0ull;   // sb_rotate publishes the next interval's far maximum
   if (t < P.mpcnumcell) {
-    unsigned long long* a = D.sacc + 3 * t;
-    const long long sx = (long long)a[0], sy = (long long)a[1], sz = (long long)a[2];
-    a[0] = 0ull; a[1] = 0ull; a[2] = 0ull;
+    long long sx = 0, sy = 0, sz = 0;
+    for (unsigned m = xm; m; m &= m - 1) {
+      const gp<unsigned long long> a = g1(D.sacc + (long long)(__ffs(m) - 1) * 3 * P.mpcnumcell + 3 * t);
+      sx += (long long)a[0]; sy += (long long)a[1]; sz += (long long)a[2];
+      a[0] = 0ull; a[1] = 0ull; a[2] = 0ull;
+    }
     const int n = (int)(sz & ((1ll << SB_NBITS) - 1));
     float cx = 0.0f, cy = 0.0f, cz = 0.0f;
     if (n) {
@@ -5404,30 +5600,36 @@ __device__ __forceinline__ void sb_cell(const CoP& P, const CoD& D, const SpD& S
       cz = (float)(((double)((sz - n) >> SB_NBITS) / SB_SCALE) / dn);
     }
     double u1, u2;
-    philox_uniform2(P.seed, 1, D.sc->coll, (unsigned long long)D.mpc_cell[t], u1, u2);
+    philox_uniform2(P.seed, 1, D.sc->coll, (unsigned long long)t, u1, u2);
     const double phi = P.twopi * u1, th = 2.0 * u2 - 1.0, st = sqrt(1.0 - th * th);
     double sp, cp;
     sincos(phi, &sp, &cp);
-    D.srec[2 * t] = make_float4(cx, cy, cz, (float)n);
-    D.srec[2 * t + 1] = make_float4((float)(st * cp), (float)(st * sp), (float)th, 0.0f);
+    const gp<f4v> rec = g1((f4v*)D.srec);
+    rec[2 * t] = f4v{cx, cy, cz, (float)n};
+    rec[2 * t + 1] = f4v{(float)(st * cp), (float)(st * sp), (float)th, 0.0f};
   }
   if (t < P.vol) {
-    int* tc = D.tcount;
-    if (tc[t] > CENSUS_MAX || tc[P.vol + t] > CENSUS_MAX || tc[2 * P.vol + t] > CENSUS_MAX)
-      raise_error(&D.sc->err, ERR_MEMBERS);
+    int nb = 0;
+    unsigned first = 0xffffffffu;
+    for (unsigned m = xm; m; m &= m - 1) {
+      const int x = __ffs(m) - 1;
+      const gp<int> tc = g1(D.xtc + (long long)x * P.vol);
+      const gp<unsigned> fb = g1(D.xfb + (long long)x * P.vol);
+      nb += tc[t];
+      first = min(first, fb[t]);
+      tc[t] = 0;
+      fb[t] = 0xffffffffu;
+    }
+    if (nb > CENSUS_MAX) raise_error(&D.sc->err, ERR_MEMBERS);   // the reference's MaxB
     if (t < P.rxnnumcell) {
-      const int nb = tc[P.vol + t];
-      const double prob = D.ptab[min(nb, CENSUS_MAX)];
+      const double prob = g1(D.ptab)[min(nb, CENSUS_MAX)];
       double u, w;
       philox_uniform2(P.seed, 2, D.sc->coll, (unsigned long long)t, u, w);
       if (0.0 < u && u <= prob) {
-        const unsigned key = D.firstB[t];
-        const int i = key >= (unsigned)P.NT ? (int)(key - (unsigned)P.NT) : (int)key;
-        S.flag[i] = 0;
+        const int i = first >= (unsigned)P.NT ? (int)(first - (unsigned)P.NT) : (int)first;
+        g1(S.flag)[i] = 0;
       }
     }
-    tc[t] = 0; tc[P.vol + t] = 0; tc[2 * P.vol + t] = 0;
-    D.firstB[t] = 0xffffffffu;
   }
 }
 
@@ -5447,38 +5649,50 @@ __global__ __launch_bounds__(256) void sb_rotate(const CoP* __restrict__ Ps, con
   const double tau = P.tau * (1.0 + 1e-9);
   const double reach0 = P.guard + 1e-6 + (classify == 2 ? start_shift(P, D) : 0.0);
   const int lane = threadIdx.x & 63;
+  if (tl.begin == 0 && threadIdx.x == 0) D.sc->xmask = 0u;   // sb_cells has merged the copies
+  const gp<const int> cell_of = g1((const int*)D.cell_of);
+  const gp<const f4v> rec = g1((const f4v*)D.srec);
+  const gp<const unsigned char> dq = g1((const unsigned char*)D.dq);
+  const gp<unsigned> bits = g1(D.near_bits);
+  const gp<float> svx = g1(S.vx), svy = g1(S.vy), svz = g1(S.vz);
   float fmax2 = 0.0f;
   for (long long base = tl.begin; base < tl.end; base += 256) {
     const long long i = base + threadIdx.x;
     bool nb = false;
     if (i < tl.end) {
-      const int t = D.cell_of[i];
-      const float4 r0 = D.srec[2 * (long long)t], r1 = D.srec[2 * (long long)t + 1];
-      float vx = S.vx[i], vy = S.vy[i], vz = S.vz[i];
-      sp_rotate_velocity(P, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, vx, vy, vz);
-      S.vx[i] = vx; S.vy[i] = vy; S.vz[i] = vz;
+      const int t = cell_of[i];
+      float vx = svx[i], vy = svy[i], vz = svz[i];
+      if (t >= 0 && t < P.mpcnumcell) {
+        const f4v r0 = rec[2 * (long long)t], r1 = rec[2 * (long long)t + 1];
+        sp_rotate_velocity(P, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, vx, vy, vz);
+        svx[i] = vx; svy[i] = vy; svz[i] = vz;
+      } else {
+        raise_error(&D.sc->err, ERR_CELL_RANGE);
+      }
       const float v2 = (vx * vx + vy * vy) + vz * vz;
-      nb = (double)D.dq[i] * (1.0 / 32.0) <= sqrt((double)v2) * tau + reach0;
+      nb = (double)dq[i] * (1.0 / 32.0) <= sqrt((double)v2) * tau + reach0;
       if (!nb) fmax2 = fmaxf(fmax2, v2);
     }
     const unsigned long long m = __ballot(nb);
     const long long w0 = (base + (threadIdx.x & ~63)) >> 5, nw = (tl.end + 31) >> 5;   // words below the tile's end
-    if (lane == 0 && w0 < nw) D.near_bits[w0] = (unsigned)m;
-    if (lane == 32 && w0 + 1 < nw) D.near_bits[w0 + 1] = (unsigned)(m >> 32);
+    if (lane == 0 && w0 < nw) bits[w0] = (unsigned)m;
+    if (lane == 32 && w0 + 1 < nw) bits[w0 + 1] = (unsigned)(m >> 32);
   }
   sp_far_max_publish(&D.sc->far_maxv2, fmax2);
 }
 
-// Per system (blockIdx.y): cleared cell sums, census and first-B keys (every
-// split run starts from them; each interval's sb_cells leaves them cleared)
+// Per system (blockIdx.y): cleared cell sums, census and first-B keys of
+// every XCD copy (every split run starts from them; each interval's sb_cells
+// leaves the copies it merged cleared)
 __global__ __launch_bounds__(256) void sb_clear(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds) {
   const int r = blockIdx.y;
   const CoP P = Ps[r];
   const CoD D = Ds[r];
   const long long i0 = (long long)blockIdx.x * 256 + threadIdx.x, st = (long long)gridDim.x * 256;
-  for (long long i = i0; i < 3 * P.mpcnumcell; i += st) D.sacc[i] = 0ull;
-  for (long long i = i0; i < 3 * P.vol; i += st) D.tcount[i] = 0;
-  for (long long i = i0; i < P.vol; i += st) D.firstB[i] = 0xffffffffu;
+  for (long long i = i0; i < NXCD * 3 * P.mpcnumcell; i += st) D.sacc[i] = 0ull;
+  for (long long i = i0; i < NXCD * P.vol; i += st) D.xtc[i] = 0;
+  for (long long i = i0; i < NXCD * P.vol; i += st) D.xfb[i] = 0xffffffffu;
+  if (i0 == 0) D.sc->xmask = 0u;
 }
 
 // Hilbert re-sort of split storage (sort_hilbert_mod.f95:50-92), per tile:
@@ -5492,8 +5706,10 @@ __global__ __launch_bounds__(256) void sb_sort_bin(const CoP* __restrict__ Ps, c
   const SpD S = Ss[tl.r];
   const CoSortD X = Xs[tl.r];
   for (long long i = tl.begin + threadIdx.x; i < tl.end; i += 256) {
-    const long long u = S.cell[i];
-    if (u < 0 || u >= P.vol || X.tourpos[u] < 0) {
+    int ix, iy, iz;
+    sp_cell_xyz(P, S.cell[i], ix, iy, iz);
+    const long long u = (long long)ix + (long long)P.Lx * (iy + (long long)P.Ly * iz);
+    if (ix >= P.Lx || iy >= P.Ly || iz >= P.Lz || X.tourpos[u] < 0) {
       raise_error(&D.sc->err, ERR_CELL_RANGE);
       X.key[i] = 0; X.rk[i] = 0;
       continue;
@@ -5541,6 +5757,13 @@ struct mpcx_cob {
   Tile *d_pt = nullptr, *d_ct = nullptr, *d_vt = nullptr, *d_gt = nullptr, *d_st = nullptr;
   int n_pt = 0, n_ct = 0, n_vt = 0, n_st = 0, tile = 2048;
   int cell_tile = 256;   // cells per workgroup of sb_cells (a multiple of 256)
+  // split replicas: particle tiles of sb_bulk and near-set tiles of sb_near
+  // placed on the XCDs (workgroup b runs on XCD b % 8): each replica on one
+  // XCD when there are enough replicas to balance them, so its sums fill one
+  // accumulator copy; a solo system spreads over all eight
+  Tile *d_xt = nullptr, *d_nt = nullptr;
+  int n_xt = 0, n_nt = 0;
+  std::vector<int> xcd_of;   // XCD of each replica (-1: spread)
   CoSortD* d_X = nullptr;   // sort buffers of each replica
   std::vector<CoSortD> h_X;
   // split storage: the replicas' split arrays and their second (sort) buffers
@@ -5597,8 +5820,53 @@ static int cob_tiles(mpcx_cob* b) {
   std::vector<Tile> gt = pt;   // gather kernels (sort gather) take the XCD-contiguous order
   xcd_order(gt);
   xcd_order(ct);
-  for (Tile** d : {&b->d_pt, &b->d_ct, &b->d_vt, &b->d_gt, &b->d_st})
+  // split: replicas onto XCDs, largest first onto the least loaded (when each
+  // XCD gets at least two and no replica exceeds 1/16 of the particles)
+  std::vector<Tile> xt, nt;
+  {
+    long long total = 0, big = 0;
+    for (int r = 0; r < b->R; r++) { total += b->eng[r]->P.NT; big = std::max<long long>(big, b->eng[r]->P.NT); }
+    const bool place = b->R >= 2 * NXCD && big * 16 <= total;
+    b->xcd_of.assign(b->R, -1);
+    std::vector<std::vector<Tile>> xq(NXCD), nq(NXCD);
+    const int K = b->solo ? grid_for(b->eng[0]->D.near_cap, 256, 1024) : NXCD;   // near-set chunks per replica
+    if (place) {
+      std::vector<int> order(b->R);
+      for (int r = 0; r < b->R; r++) order[r] = r;
+      std::stable_sort(order.begin(), order.end(),
+                       [&](int a, int c) { return b->eng[a]->P.NT > b->eng[c]->P.NT; });
+      long long load[NXCD] = {0};
+      for (int r : order) {
+        const int x = (int)(std::min_element(load, load + NXCD) - load);
+        b->xcd_of[r] = x;
+        load[x] += b->eng[r]->P.NT;
+      }
+      for (const Tile& t : pt) xq[b->xcd_of[t.r]].push_back(t);
+      for (int r = 0; r < b->R; r++)
+        for (int k = 0; k < K; k++) nq[b->xcd_of[r]].push_back(Tile{r, 0, k, K});
+      auto interleave = [](std::vector<std::vector<Tile>>& q, std::vector<Tile>& out) {
+        size_t m = 0;
+        for (auto& v : q) m = std::max(m, v.size());
+        out.assign(m * NXCD, Tile{0, 0, 0, 0});   // begin == end: an empty tile
+        for (int x = 0; x < NXCD; x++)
+          for (size_t k = 0; k < q[x].size(); k++) out[k * NXCD + x] = q[x][k];
+      };
+      interleave(xq, xt);
+      interleave(nq, nt);
+    } else {
+      xt = pt;
+      for (int r = 0; r < b->R; r++)
+        for (int k = 0; k < K; k++) nt.push_back(Tile{r, 0, k, K});
+    }
+  }
+  for (Tile** d : {&b->d_pt, &b->d_ct, &b->d_vt, &b->d_gt, &b->d_st, &b->d_xt, &b->d_nt})
     if (*d) { (void)hipFree(*d); *d = nullptr; }
+  HIPCHK(hipMalloc(&b->d_xt, xt.size() * sizeof(Tile)));
+  HIPCHK(hipMalloc(&b->d_nt, nt.size() * sizeof(Tile)));
+  HIPCHK(hipMemcpy(b->d_xt, xt.data(), xt.size() * sizeof(Tile), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b->d_nt, nt.data(), nt.size() * sizeof(Tile), hipMemcpyHostToDevice));
+  b->n_xt = (int)xt.size();
+  b->n_nt = (int)nt.size();
   HIPCHK(hipMalloc(&b->d_pt, pt.size() * sizeof(Tile)));
   HIPCHK(hipMalloc(&b->d_gt, gt.size() * sizeof(Tile)));
   HIPCHK(hipMalloc(&b->d_ct, ct.size() * sizeof(Tile)));
@@ -5730,8 +5998,7 @@ static int cob_sort_interval(mpcx_cob* b) {
 // slot >= 0: the interval's kernels are timed (events into that slot).
 static int sb_interval(mpcx_cob* b, bool sort_due, long long slot) {
   hipStream_t st = b->st, s2 = b->st2;
-  // near-set kernels: blockIdx.y = replica (a solo system's near set may hold many dimers' regions)
-  const dim3 ng(b->solo ? grid_for(b->eng[0]->D.near_cap, 256, 1024) : 32, b->R);
+  const dim3 ng(32, b->R);   // sb_load_near: blockIdx.y = replica
   const bool ov = b->overlap;
   auto tmark = [&](int k, int edge, hipStream_t s) -> hipError_t {
     if (slot < 0) return hipSuccess;
@@ -5746,7 +6013,7 @@ static int sb_interval(mpcx_cob* b, bool sort_due, long long slot) {
     HIPCHK(hipEventRecord(b->fork, st));
     HIPCHK(hipStreamWaitEvent(s2, b->fork, 0));
     HIPCHK(tmark(1, 0, s2));
-    hipLaunchKernelGGL(sb_bulk<false>, dim3(b->n_pt), dim3(256), 0, s2, b->d_P, b->d_D, b->d_S, b->d_pt, bin);
+    hipLaunchKernelGGL(sb_bulk<false>, dim3(b->n_xt), dim3(256), 0, s2, b->d_P, b->d_D, b->d_S, b->d_xt, bin);
     HIPCHK(tmark(1, 1, s2));
     HIPCHK(hipEventRecord(b->join, s2));
   }
@@ -5755,11 +6022,11 @@ static int sb_interval(mpcx_cob* b, bool sort_due, long long slot) {
   HIPCHK(tmark(0, 1, st));
   if (!ov) {
     HIPCHK(tmark(1, 0, st));
-    hipLaunchKernelGGL(sb_bulk<false>, dim3(b->n_pt), dim3(256), 0, st, b->d_P, b->d_D, b->d_S, b->d_pt, bin);
+    hipLaunchKernelGGL(sb_bulk<false>, dim3(b->n_xt), dim3(256), 0, st, b->d_P, b->d_D, b->d_S, b->d_xt, bin);
     HIPCHK(tmark(1, 1, st));
   }
   HIPCHK(tmark(4, 0, st));
-  hipLaunchKernelGGL(sb_near, ng, dim3(256), 0, st, b->d_P, b->d_D, b->d_S, bin, 1);
+  hipLaunchKernelGGL(sb_near, dim3(b->n_nt), dim3(256), 0, st, b->d_P, b->d_D, b->d_S, b->d_nt, bin, 1);
   HIPCHK(tmark(4, 1, st));
   if (ov) HIPCHK(hipStreamWaitEvent(st, b->join, 0));
   if (sort_due) {
@@ -5768,8 +6035,8 @@ static int sb_interval(mpcx_cob* b, bool sort_due, long long slot) {
     cob_near_list(b);
     hipLaunchKernelGGL(sb_load_near, ng, dim3(256), 0, st, b->d_P, b->d_D, b->d_S, 0);
     HIPCHK(cob_launch_near(b, 1, M_RESUME));
-    hipLaunchKernelGGL(sb_near, ng, dim3(256), 0, st, b->d_P, b->d_D, b->d_S, 0, 0);
-    hipLaunchKernelGGL(sb_bulk<true>, dim3(b->n_pt), dim3(256), 0, st, b->d_P, b->d_D, b->d_S, b->d_pt, 1);
+    hipLaunchKernelGGL(sb_near, dim3(b->n_nt), dim3(256), 0, st, b->d_P, b->d_D, b->d_S, b->d_nt, 0, 0);
+    hipLaunchKernelGGL(sb_bulk<true>, dim3(b->n_xt), dim3(256), 0, st, b->d_P, b->d_D, b->d_S, b->d_xt, 1);
   }
   HIPCHK(tmark(2, 0, st));
   hipLaunchKernelGGL(sb_cells, dim3(b->n_st), dim3(256), 0, st, b->d_P, b->d_D, b->d_S, b->d_st);
@@ -5792,7 +6059,7 @@ extern "C" int mpcx_cob_destroy(mpcx_cob* b) {
   if (b->st2) (void)hipStreamSynchronize(b->st2);
   cob_free_graph(b);
   for (void* p : {(void*)b->d_P, (void*)b->d_D, (void*)b->d_pt, (void*)b->d_ct, (void*)b->d_vt, (void*)b->d_gt,
-                  (void*)b->d_st, (void*)b->d_X, (void*)b->d_S, (void*)b->d_N})
+                  (void*)b->d_st, (void*)b->d_xt, (void*)b->d_nt, (void*)b->d_X, (void*)b->d_S, (void*)b->d_N})
     if (p) (void)hipFree(p);
   for (hipEvent_t ev : {b->e0, b->e1, b->fork, b->join})
     if (ev) (void)hipEventDestroy(ev);
@@ -5892,7 +6159,8 @@ static int cob_begin(mpcx_cob* b, int* cstep_out) {
   if (rc) return rc;
   if (b->split) {
     long long big = 0;
-    for (int r = 0; r < b->R; r++) big = std::max(big, 3 * std::max(b->eng[r]->P.mpcnumcell, b->eng[r]->P.vol));
+    for (int r = 0; r < b->R; r++)
+      big = std::max(big, NXCD * 3 * std::max(b->eng[r]->P.mpcnumcell, b->eng[r]->P.vol));
     hipLaunchKernelGGL(sb_clear, dim3(grid_for(big, 256, 1024), b->R), dim3(256), 0, b->st, b->d_P, b->d_D);
   }
   return 0;

@@ -79,12 +79,11 @@ def test_selkov_records_agree_with_device_observables(tmp_path):
     assert em[0] == cstep
     for i, k in enumerate(("p_x", "p_y", "p_z")):          # particle-order sums vs device tree sums
         assert abs(em[1 + i] - o[k]) <= 1e-12 * max(abs(o[k]), 1.0), k
-    # the record's dimer term is the centre-of-mass energy (Output_SelkovDimer.f90:279-283)
-    Mc, Mn = float(p["Mc"]), float(p["Mn"])
-    P = [(Mc * g["dm"][0, 3 + a] + Mn * g["dm"][1, 3 + a]) for a in range(3)]
-    ke = o["ke_sol"] + 0.5 / (Mc + Mn) * (P[0] ** 2 + P[1] ** 2 + P[2] ** 2)
-    assert abs(em[4] - ke) <= 1e-12 * ke
-    assert abs(em[7] - o["T_sol"] * SF.TWO_THIRDS / (2.0 / 3.0)) <= 1e-12 * o["T_sol"]
+    # the device observables follow Output_SelkovDimer.f90:243-300 as the record does: the
+    # dimer as its centre of mass, temperatures with the single-precision 2./3.
+    assert abs(em[4] - (o["ke_sol"] + o["ke_dim"])) <= 1e-12 * em[4]
+    assert abs(em[7] - o["T_sol"]) <= 1e-12 * o["T_sol"]
+    assert em[8] == o["T_dim"]
     conc = SF._numbers(files["Concentration.dat"])
     assert conc[1] + conc[2] + conc[3] == int(g["NT"])
     assert 0 < conc[10] <= p["vol"]

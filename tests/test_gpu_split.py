@@ -158,3 +158,24 @@ def test_split_keeps_particles_below_the_top_wall():
     finally:
         e.close()
     assert np.all(g["z"] < Lz) and np.all(g["z"] >= 0.0)
+
+
+def test_split_reports_a_particle_thrown_out_of_the_box():
+    """A near particle that the FP64 near engine carries far outside the box
+    (planted here with vx = -1e25) is reported when the near set goes back to
+    split storage, and is never binned (a regression test: its cell
+    coordinates used to wrap into a negative collision-cell index)."""
+    from main_amd._lib import MpcxError
+    p = _params(L=20)
+    s = S.collective_initial_state(p, seed=8)
+    pos = np.stack([s["x"], s["y"], s["z"]], axis=1)
+    sph = np.concatenate([np.stack([s["xd"][:, i], s["yd"][:, i], s["zd"][:, i]], axis=1) for i in (0, 1)])
+    d = np.min(np.linalg.norm(pos[:, None, :] - sph[None, :, :], axis=2), axis=1)
+    pick = int(np.argmax(d))   # the particle farthest from both spheres
+    s["vx"][pick] = -1e25
+    e = _engine(p, s, True)
+    try:
+        with pytest.raises(MpcxError, match="outside the cell arrays"):
+            e.run(1)
+    finally:
+        e.close()
