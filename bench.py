@@ -81,29 +81,12 @@ def cpu_baseline(L=100, budget_s=15.0):
 
 
 def batched(R=256, steps=20, warmup=4, seed=4357787, tiles=(1024, 2048, 4096, 8192)):
-    """Replica ensemble: R single-dimer systems with cubic boxes of 20-40 cells
-    (edges uniform), 10 solvent per cell, RPab uniform in [0.2, 1], EbN uniform
-    in [0.1, 1], one Philox key per replica, the other physics as the headline
-    system; advanced together by one batch (HIP-graph replay, autotuned tile)."""
-    from main_amd import systems as S
+    """The replica ensemble in FP64 storage, advanced together by one batch
+    (HIP-graph replay, autotuned tile)."""
     from main_amd.batch import CollectiveBatch
-    from main_amd.collective import CollectiveEngine
     from main_amd._lib import device_synchronize
-    from main_amd.tour import cell_tour
-    rng = np.random.default_rng(seed)
-    engines, tours, t0 = [], {}, time.time()
-    for r in range(R):
-        L = int(rng.integers(20, 41))
-        p = S.collective_params(Lx=float(L), Ly=float(L), Lz=float(L), numdims=1, rhoa=10.0, Rc=2.0, Rn=4.0,
-                                Dcn=6.5, EbN=float(rng.uniform(0.1, 1.0)), RPab=float(rng.uniform(0.2, 1.0)),
-                                T=1.0 / 6.0, skinrad=0.3, MDt=0.01, MPCt=0.5, sort_step=500, k1=1e-3)
-        s = S.collective_initial_state(p, seed=seed + r)
-        if L not in tours:
-            tours[L] = cell_tour(L, L, L)
-        e = CollectiveEngine(p, rng_mode=1, seed=seed + r)
-        e.set_state(s, tour=tours[L])
-        e.setup(do_sort=True)
-        engines.append(e)
+    t0 = time.time()
+    engines = ensemble_engines(R, seed, False)
     print(f"# batched: {R} replicas built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     batch = CollectiveBatch(engines, tile=2048)
     info = batch.info()
@@ -119,7 +102,7 @@ def batched(R=256, steps=20, warmup=4, seed=4357787, tiles=(1024, 2048, 4096, 81
            "ms_per_interval": ms, "particle_updates_per_s": info["particles"] * steps / dt,
            "replicas_per_hour_2000_collisions": R * 3600.0 / (2000 * ms * 1e-3),
            "tile": batch.tile, "autotune_ms_per_interval": times, "rng": "device Philox4x32-10",
-           "graph": True, "dtype": "f64"}
+           "graph": True, "dtype": "f64", "roofline_step": step_roofline(FP64_STEP_BYTES, info["particles"], ms)}
     batch.close()
     for e in engines:
         e.close()
@@ -195,20 +178,23 @@ def selkov(steps=20, warmup=5, seed=8154543):
             "rng": "device Philox4x32-10"}
 
 
-# algorithmic bytes per particle of the split-precision kernels (29-byte state: int32 cell, 3 x FP32
-# offsets, 3 x FP32 velocities, int8 species)
-ALGO_BYTES_SPLIT = {
-    "co_bulk": 53.0,     # sp_bulk: cell r/w 8, offsets r/w 24, species 1, velocities read 12 (far replay),
-                         # collision cell 4 + rank 2, distance byte 1, status 1
-    "co_far": 53.0,      # sp_far: the bytes of sp_bulk for the far particles, beside the near engine
-    "co_rotate": 35.0,   # sp_rotate: velocities r/w 24, collision cell 4, cell record share 6, distance byte 1
-    "co_collide": 23.0,  # sp_collide: velocities 12, bucket 4, cell record + counts share 7
-    "co_scatter": 14.0,
-}
+# Algorithmic bytes per particle of the split-precision interval kernels (29-byte state: packed
+# int32 cell, 3 x FP32 offsets, 3 x FP32 velocities, int8 species):
+#  sb_bulk   far particles: reads cell, offsets, velocities, species (29), status (1); writes cell and
+#            offsets (16), collision cell (4), distance byte (1)                               = 51
+#  sb_rotate reads collision cell (4), velocities (12), distance byte (1), the cell record (32 B per
+#            cell, 3.2 per particle); writes velocities (12)                                  = 32.2
+#  sb_cells  per collision cell: one accumulator copy read and cleared (48), record written (32); per
+#            unshifted cell: B census and first key read and cleared (16); per particle       = 9.6
+# The interval's figure is the north-star's 57 B per particle-update (state read and written once).
+ALGO_BYTES_SPLIT = {"sb_bulk": 51.0, "sb_rotate": 32.2, "sb_cells": 9.6}
+SPLIT_STEP_BYTES = 57.0
+FP64_STEP_BYTES = 100.0
 
 
 def split_headline(L=100, steps=60, warmup=20, seed=4357787):
-    """The headline system in split-precision storage (same physics, device Philox)."""
+    """The headline system in split-precision storage (same physics, device
+    Philox): a one-system batch of the split interval."""
     from main_amd import systems as S
     from main_amd._lib import device_synchronize
     from main_amd.collective import CollectiveEngine
@@ -228,14 +214,121 @@ def split_headline(L=100, steps=60, warmup=20, seed=4357787):
     kms = eng.kernel_ms()
     eng.close()
     NT = p["NT"]
-    hbm = {k: v for k, v in kms.items() if k in ALGO_BYTES_SPLIT}
-    dom = max(hbm, key=hbm.get)
-    ach = ALGO_BYTES_SPLIT[dom] * NT / (hbm[dom] * 1e-3) / 1e9
-    print(f"# split {L}^3: {dt * 1000.0 / steps:.3f} ms/interval {kms}", file=sys.stderr, flush=True)
-    return {"particles": NT, "ms_per_interval": dt * 1000.0 / steps, "particle_updates_per_s": NT * steps / dt,
+    ms = dt * 1000.0 / steps
+    print(f"# split {L}^3: {ms:.3f} ms/interval {kms}", file=sys.stderr, flush=True)
+    return {"particles": NT, "ms_per_interval": ms, "particle_updates_per_s": NT * steps / dt,
             "kernel_ms": kms, "dtype": "f32 solvent (cell + offsets), f64 near set and dimer",
-            "roofline": {"kernel": dom.replace("co_", "sp_"), "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": ach / HBM_PEAK_GBS, "algorithmic_bytes_per_particle": ALGO_BYTES_SPLIT[dom]}}
+            "roofline_step": step_roofline(SPLIT_STEP_BYTES, NT, ms),
+            "kernels": kernel_rooflines(kms, ALGO_BYTES_SPLIT, NT)}
+
+
+def step_roofline(bytes_per_particle, particles, ms_per_interval):
+    """The whole interval against the HBM roofline: algorithmic bytes of one
+    particle-update times the particles, over the interval's wall time."""
+    ach = bytes_per_particle * particles / (ms_per_interval * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "algorithmic_bytes_per_particle": bytes_per_particle}
+
+
+def kernel_rooflines(kms, algo, particles):
+    out = {}
+    for k, b in algo.items():
+        t = kms.get(k)
+        if t:
+            ach = b * particles / (t * 1e-3) / 1e9
+            out[k] = {"launch_ms": t, "achieved": ach, "frac": ach / HBM_PEAK_GBS, "algorithmic_bytes_per_particle": b}
+    return out
+
+
+def ensemble_engines(R, seed, split):
+    """The replica ensemble: R single-dimer systems with cubic boxes of 20-40
+    cells (edges uniform), 10 solvent per cell, RPab uniform in [0.2, 1], EbN
+    uniform in [0.1, 1], one Philox key per replica, the other physics as the
+    headline system (re-sort every 10 collisions); device initial solvent."""
+    from main_amd import systems as S
+    from main_amd.collective import CollectiveEngine
+    from main_amd.tour import cell_tour
+    rng = np.random.default_rng(seed)
+    engines, tours = [], {}
+    for r in range(R):
+        L = int(rng.integers(20, 41))
+        p = S.collective_params(Lx=float(L), Ly=float(L), Lz=float(L), numdims=1, rhoa=10.0, Rc=2.0, Rn=4.0,
+                                Dcn=6.5, EbN=float(rng.uniform(0.1, 1.0)), RPab=float(rng.uniform(0.2, 1.0)),
+                                T=1.0 / 6.0, skinrad=0.3, MDt=0.01, MPCt=0.5, sort_step=500, k1=1e-3)
+        if L not in tours:
+            tours[L] = cell_tour(L, L, L)
+        e = CollectiveEngine(p, rng_mode=1, seed=seed + r, split=split)
+        e.set_dimers(S.collective_dimers(p, seed=seed + r), tour=tours[L])
+        e.init_solvent(seed + r)
+        e.setup(do_sort=True)
+        engines.append(e)
+    return engines
+
+
+def split_batched(R=256, steps=20, warmup=4, seed=4357787, fp64_value=None):
+    """The north star's mode: the replica ensemble in split-precision storage,
+    advanced by one batch (HIP-graph replay, autotuned tile and cells per
+    workgroup).  Timed over `steps` intervals (two re-sorts in 20); the kernel
+    means come from a further run without the graph (HIP events)."""
+    from main_amd.batch import CollectiveBatch
+    from main_amd._lib import device_synchronize
+    t0 = time.time()
+    engines = ensemble_engines(R, seed, True)
+    print(f"# split batched: {R} replicas built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    batch = CollectiveBatch(engines, tile=4096)
+    info = batch.info()
+    times = batch.autotune(tiles=(2048, 4096, 8192), cell_tiles=(256, 1024), ncoll=2)
+    batch.run(warmup)
+    device_synchronize()
+    t0 = time.perf_counter()
+    batch.run(steps)
+    device_synchronize()
+    dt = time.perf_counter() - t0
+    batch.run(16, use_graph=False)
+    kms = batch.kernel_ms()
+    batch.close()
+    for e in engines:
+        e.close()
+    n = info["particles"]
+    ms = dt * 1000.0 / steps
+    value = n * steps / dt
+    print(f"# split batched: {ms:.3f} ms/interval, {value:.3e} particle-updates/s, kernels {kms}", file=sys.stderr,
+          flush=True)
+    out = {"replicas": R, "particles": n, "collision_cells": info["cells"], "ms_per_interval": ms,
+           "particle_updates_per_s": value, "replicas_per_hour_2000_collisions": R * 3600.0 / (2000 * ms * 1e-3),
+           "tile": batch.tile, "cells_per_workgroup": batch.cell_tile,
+           "autotune_ms_per_interval": {f"{k[0]}x{k[1]}": v for k, v in times.items()},
+           "roofline_step": step_roofline(SPLIT_STEP_BYTES, n, ms), "kernel_ms": kms,
+           "kernels": kernel_rooflines(kms, ALGO_BYTES_SPLIT, n), "rng": "device Philox4x32-10", "graph": True,
+           "dtype": "f32 solvent (cell + offsets), f64 near set and dimers"}
+    if fp64_value:
+        out["vs_fp64_headline"] = value / fp64_value
+    return out
+
+
+# kernels of an interval without a re-sort (the two-pass collisions) and of a re-sort
+STEP_KERNELS = ("co_near_engine", "co_far", "co_zone_mark", "co_collide_far", "co_rotate_far", "co_bin_rest",
+                "co_collide_late", "interval_begin", "bits_count", "bits_emit_scan", "co_far_handoff")
+SORT_KERNELS = ("sort_bin", "sort_scatter", "sort_order", "sort_gather_six", "sort_gather_one", "sort_gather_bits",
+                "co_bulk", "co_scatter", "co_collide", "co_rotate", "rotate_begin", "scan_partial",
+                "scan_final_sums")
+
+
+def step_traffic(sort_every=10):
+    """Counter bytes of one interval: every interval's kernels, plus the
+    re-sort interval's own launches once per sort_every intervals; None
+    unless the committed summary covers the main ones."""
+    per = {}
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            m = re.match(r"traffic_bytes_per_launch\s+(\S+)\s+(\S+)", line)
+            if m:
+                per[m.group(1)] = float(m.group(2))
+    if not all(k in per for k in ("co_near_engine", "co_far", "co_collide_far", "co_rotate_far")):
+        return None
+    return sum(per.get(k, 0.0) for k in STEP_KERNELS) + sum(per.get(k, 0.0) for k in SORT_KERNELS) / sort_every
 
 
 def read_traffic(kernel):
@@ -303,6 +396,7 @@ def main():
     ap.add_argument("--many-dimers-only", action="store_true")
     ap.add_argument("--split", type=int, default=1, help="measure the split-precision headline beside the FP64 one")
     ap.add_argument("--split-only", action="store_true")
+    ap.add_argument("--split-batched-only", action="store_true", help="measure only the split-precision ensemble")
     args = ap.parse_args()
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc)
@@ -320,6 +414,10 @@ def main():
     if args.batched_only:
         print(json.dumps({"metric": "replicas/hour (2000-collision runs)", **batched(args.batched, args.steps,
                                                                                    args.warmup)}), flush=True)
+        return
+    if args.split_batched_only:
+        print(json.dumps({"metric": "particle-updates/s (MD+MPC), split-precision batched",
+                          **split_batched(args.batched, args.steps, args.warmup)}), flush=True)
         return
     if args.split_only:
         print(json.dumps({"metric": "particle-updates/s (MD+MPC), split precision",
@@ -359,12 +457,16 @@ def main():
     NT = p["NT"]
     value = NT * args.steps / dt
     ms_step = dt * 1000.0 / args.steps
-    # roofline of the dominant HBM-bound kernel of the interval
-    hbm = {k: v for k, v in kms.items() if k in ALGO_BYTES}
-    dom = max(hbm, key=hbm.get)
-    launch_s = hbm[dom] * 1e-3
-    achieved = ALGO_BYTES[dom] * NT / launch_s / 1e9
-    traffic = read_traffic(dom)
+    # roofline of the whole interval (the unit one "launch" of the step is): 100 B per
+    # particle-update over the interval's wall time; the HBM-bound kernels beside it
+    step = step_roofline(FP64_STEP_BYTES, NT, ms_step)
+    kernels = kernel_rooflines(kms, ALGO_BYTES, NT)
+    for k, v in kernels.items():
+        v["traffic"] = read_traffic(k)
+    step.update(kernel="collision interval (all its kernels, both streams)", traffic=step_traffic(),
+                traffic_note="counter bytes between L2 and the memory fabric per interval (FETCH_SIZE, "
+                             "WRITE_SIZE scaled by the copy probe): Infinity-Cache hits included, so an upper "
+                             "bound on HBM bytes", kernels=kernels, kernel_ms=kms)
     out = {
         "metric": "particle-updates/s (MD+MPC)",
         "value": value,
@@ -382,13 +484,13 @@ def main():
                                f"({NT} solvent), 1 dimer, skin 0.3, tau 0.5 = 50 x 0.01, sort every 10 collisions",
                    "rng": "device Philox4x32-10", "md_steps_per_collision": p["MPCtstep"],
                    "particle_md_updates_per_s": value * p["MPCtstep"], "cstep": st["cstep"]},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": dom, "algorithmic_bytes_per_particle": ALGO_BYTES[dom],
-                     "launch_ms": hbm[dom], "kernel_ms": kms},
+        "roofline": step,
     }
     if args.batched > 0:
         out["batched"] = batched(args.batched, steps=10, warmup=2)
+        out["split_batched"] = split_batched(args.batched, steps=20, warmup=4, fp64_value=value)
+        out["split_batched"]["vs_fp64_batched"] = (out["split_batched"]["particle_updates_per_s"] /
+                                                   out["batched"]["particle_updates_per_s"])
     if args.split:
         out["split_precision"] = split_headline(args.size)
     if args.many_dimers:

@@ -5423,9 +5423,9 @@ __device__ __forceinline__ void sb_fetch(const SbV& V, long long i, long long en
 // engine): every particle binned where it is, distances to the spheres' end
 // positions.  Each round's loads are issued before the previous round is
 // binned, so their latency overlaps the binning's LDS work and barriers.
-template <bool ALL>
-__global__ __launch_bounds__(256) void sb_bulk(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds,
-                                               const SpD* __restrict__ Ss, const Tile* __restrict__ tiles, int do_bin) {
+template <bool ALL, int T>
+__global__ __launch_bounds__(T) void sb_bulk(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds,
+                                             const SpD* __restrict__ Ss, const Tile* __restrict__ tiles, int do_bin) {
   __shared__ SbTables G;
   __shared__ SbSpheres sph;
   const Tile tl = tiles[blockIdx.x];
@@ -5446,9 +5446,9 @@ __global__ __launch_bounds__(256) void sb_bulk(const CoP* __restrict__ Ps, const
   const float dtc = (float)P.MDt * (float)P.MPCtstep;
   SbPart nxt;
   sb_fetch(V, tl.begin + threadIdx.x, tl.end, nxt);
-  for (long long base = tl.begin; base < tl.end; base += 256) {
+  for (long long base = tl.begin; base < tl.end; base += T) {
     const SbPart cur = nxt;
-    sb_fetch(V, base + 256 + threadIdx.x, tl.end, nxt);
+    sb_fetch(V, base + T + threadIdx.x, tl.end, nxt);
     const int i = (int)(base + threadIdx.x);
     const bool valid = i < tl.end;
     bool mine = valid && (ALL || !cur.near);
@@ -5705,20 +5705,30 @@ __global__ __launch_bounds__(256) void sb_sort_bin(const CoP* __restrict__ Ps, c
   const CoD D = Ds[tl.r];
   const SpD S = Ss[tl.r];
   const CoSortD X = Xs[tl.r];
-  for (long long i = tl.begin + threadIdx.x; i < tl.end; i += 256) {
-    int ix, iy, iz;
-    sp_cell_xyz(P, S.cell[i], ix, iy, iz);
-    const long long u = (long long)ix + (long long)P.Lx * (iy + (long long)P.Ly * iz);
-    if (ix >= P.Lx || iy >= P.Ly || iz >= P.Lz || X.tourpos[u] < 0) {
-      raise_error(&D.sc->err, ERR_CELL_RANGE);
-      X.key[i] = 0; X.rk[i] = 0;
-      continue;
+  // whole waves run the loop (tiles are multiples of 256): one counter atomic
+  // per distinct cell of the wave (the particles are in tour order)
+  for (long long base = tl.begin; base < tl.end; base += 256) {
+    const long long i = base + threadIdx.x;
+    int t = 0;
+    bool ok = false;
+    if (i < tl.end) {
+      int ix, iy, iz;
+      sp_cell_xyz(P, S.cell[i], ix, iy, iz);
+      const long long u = (long long)ix + (long long)P.Lx * (iy + (long long)P.Ly * iz);
+      if (ix >= P.Lx || iy >= P.Ly || iz >= P.Lz || X.tourpos[u] < 0) {
+        raise_error(&D.sc->err, ERR_CELL_RANGE);
+        X.key[i] = 0; X.rk[i] = 0;
+      } else {
+        t = X.tourpos[u];
+        ok = true;
+      }
     }
-    const int t = X.tourpos[u];
-    const int r = atomicAdd(&X.scnt[t], 1);
-    if (r >= BMAX) raise_error(&D.sc->err, ERR_BUCKET);
-    X.key[i] = t;
-    X.rk[i] = (unsigned short)min(r, 65535);
+    const int r = wave_atomic_rank(X.scnt, t, ok);
+    if (ok) {
+      if (r >= BMAX) raise_error(&D.sc->err, ERR_BUCKET);
+      X.key[i] = t;
+      X.rk[i] = (unsigned short)min(r, 65535);
+    }
   }
 }
 __global__ __launch_bounds__(256) void sb_sort_gather(const CoD* __restrict__ Ds, const SpD* __restrict__ Ss,
@@ -5757,6 +5767,7 @@ struct mpcx_cob {
   Tile *d_pt = nullptr, *d_ct = nullptr, *d_vt = nullptr, *d_gt = nullptr, *d_st = nullptr;
   int n_pt = 0, n_ct = 0, n_vt = 0, n_st = 0, tile = 2048;
   int cell_tile = 256;   // cells per workgroup of sb_cells (a multiple of 256)
+  int bulk_threads = 512;   // workgroup size of sb_bulk (256 or 512; 512 measured faster)
   // split replicas: particle tiles of sb_bulk and near-set tiles of sb_near
   // placed on the XCDs (workgroup b runs on XCD b % 8): each replica on one
   // XCD when there are enough replicas to balance them, so its sums fill one
@@ -5829,7 +5840,7 @@ static int cob_tiles(mpcx_cob* b) {
     const bool place = b->R >= 2 * NXCD && big * 16 <= total;
     b->xcd_of.assign(b->R, -1);
     std::vector<std::vector<Tile>> xq(NXCD), nq(NXCD);
-    const int K = b->solo ? grid_for(b->eng[0]->D.near_cap, 256, 1024) : NXCD;   // near-set chunks per replica
+    const int K = b->solo ? grid_for(b->eng[0]->D.near_cap, 256, 1024) : 4;   // near-set chunks per replica
     if (place) {
       std::vector<int> order(b->R);
       for (int r = 0; r < b->R; r++) order[r] = r;
@@ -5989,6 +6000,18 @@ static int cob_sort_interval(mpcx_cob* b) {
   return 0;
 }
 
+// sb_bulk over the XCD-placed particle tiles with b->bulk_threads per workgroup
+static void sb_launch_bulk(mpcx_cob* b, bool all, hipStream_t s, int bin) {
+  const dim3 g(b->n_xt);
+  if (b->bulk_threads == 512) {
+    if (all) hipLaunchKernelGGL((sb_bulk<true, 512>), g, dim3(512), 0, s, b->d_P, b->d_D, b->d_S, b->d_xt, bin);
+    else hipLaunchKernelGGL((sb_bulk<false, 512>), g, dim3(512), 0, s, b->d_P, b->d_D, b->d_S, b->d_xt, bin);
+  } else {
+    if (all) hipLaunchKernelGGL((sb_bulk<true, 256>), g, dim3(256), 0, s, b->d_P, b->d_D, b->d_S, b->d_xt, bin);
+    else hipLaunchKernelGGL((sb_bulk<false, 256>), g, dim3(256), 0, s, b->d_P, b->d_D, b->d_S, b->d_xt, bin);
+  }
+}
+
 // One split-precision interval of every replica (the kernels above).  With
 // overlap the far particles' flight and binning (st2) run beside the near
 // engines (st); the near set is binned on st when they are done; every cell
@@ -6012,17 +6035,21 @@ static int sb_interval(mpcx_cob* b, bool sort_due, long long slot) {
   if (ov) {
     HIPCHK(hipEventRecord(b->fork, st));
     HIPCHK(hipStreamWaitEvent(s2, b->fork, 0));
-    HIPCHK(tmark(1, 0, s2));
-    hipLaunchKernelGGL(sb_bulk<false>, dim3(b->n_xt), dim3(256), 0, s2, b->d_P, b->d_D, b->d_S, b->d_xt, bin);
-    HIPCHK(tmark(1, 1, s2));
-    HIPCHK(hipEventRecord(b->join, s2));
   }
+  // the near engines are submitted before the far flight: each needs most of
+  // a CU's LDS, which the far flight's workgroups would otherwise take first
   HIPCHK(tmark(0, 0, st));
   HIPCHK(cob_launch_near(b, b->S, sort_due ? M_STOP : 0));
   HIPCHK(tmark(0, 1, st));
+  if (ov) {
+    HIPCHK(tmark(1, 0, s2));
+    sb_launch_bulk(b, false, s2, bin);
+    HIPCHK(tmark(1, 1, s2));
+    HIPCHK(hipEventRecord(b->join, s2));
+  }
   if (!ov) {
     HIPCHK(tmark(1, 0, st));
-    hipLaunchKernelGGL(sb_bulk<false>, dim3(b->n_xt), dim3(256), 0, st, b->d_P, b->d_D, b->d_S, b->d_xt, bin);
+    sb_launch_bulk(b, false, st, bin);
     HIPCHK(tmark(1, 1, st));
   }
   HIPCHK(tmark(4, 0, st));
@@ -6036,7 +6063,7 @@ static int sb_interval(mpcx_cob* b, bool sort_due, long long slot) {
     hipLaunchKernelGGL(sb_load_near, ng, dim3(256), 0, st, b->d_P, b->d_D, b->d_S, 0);
     HIPCHK(cob_launch_near(b, 1, M_RESUME));
     hipLaunchKernelGGL(sb_near, dim3(b->n_nt), dim3(256), 0, st, b->d_P, b->d_D, b->d_S, b->d_nt, 0, 0);
-    hipLaunchKernelGGL(sb_bulk<true>, dim3(b->n_xt), dim3(256), 0, st, b->d_P, b->d_D, b->d_S, b->d_xt, 1);
+    sb_launch_bulk(b, true, st, 1);
   }
   HIPCHK(tmark(2, 0, st));
   hipLaunchKernelGGL(sb_cells, dim3(b->n_st), dim3(256), 0, st, b->d_P, b->d_D, b->d_S, b->d_st);
