@@ -460,28 +460,9 @@ __device__ __forceinline__ void dimer_forces_and_rattle_v(const CoP& P, Dimer<ND
 // [w*64*E, (w+1)*64*E), lane-interleaved (q = w*64*E + j*64 + lane), so the
 // order of a compaction follows ballots within a wave and one scan over the
 // waves.
-constexpr int LCAP = 2304;     // inside entries staged in LDS (the rest stay in HBM)
 constexpr int NE_EMAX = 64;    // near entries per lane: near set <= NE_T * NE_EMAX
 constexpr int NE_NEAR = NE_T * NE_EMAX;
 constexpr int NE_W = NE_T / 64;
-struct NearShared {
-  double lx[LCAP], ly[LCAP], lz[LCAP], lvx[LCAP], lvy[LCAP], lvz[LCAP], lmsd[LCAP];
-  int lflag[LCAP];
-  unsigned lmemb[LCAP];
-  double sd[12][MAXS];
-  double red[NE_W][4 * MAXS + 1];
-  double fin[4 * MAXS + 1];
-  int wsum[NE_W + 1];
-  double vmax2;
-  int cstep, tc, tp, in_num, nev, near_num, rebuild, clustered;
-  // scalars the MD steps read or write, kept here for the launch (no global
-  // memory round trip on the serial path; written back with the scalars)
-  double tpot, fmaxv2, x0[MAXS], y0[MAXS], z0[MAXS];
-  int ev[MAXEV];
-  CoP P;   // launch parameters, read from LDS by the helpers below
-  CoD D;
-};
-__shared__ NearShared g_ne;
 
 // memberships of a lane's near entries, B bits each (B = 4 for up to two
 // dimers, 8 for up to four): entry j in field j
@@ -554,726 +535,25 @@ __device__ __attribute__((noinline)) double clu_exchange_max(CluInfo* ci, int ci
   return m;
 }
 
-// exclusive scan of the per-wave counts in g_ne.wsum (thread 0), total in wsum[NE_W]
-__device__ __forceinline__ void ne_wave_scan() {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int run = 0;
-    for (int w = 0; w < NE_W; w++) { int t = g_ne.wsum[w]; g_ne.wsum[w] = run; run += t; }
-    g_ne.wsum[NE_W] = run;
-  }
-  __syncthreads();
-}
 
-// particle arrays -> near slots (interval start) and back (interval end)
-__device__ __forceinline__ void ne_gather(const CoD& D, int nn) {
-  for (int q0 = threadIdx.x; q0 < nn; q0 += 4 * NE_T) {
-    int p[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) { int q = q0 + u * NE_T; p[u] = q < nn ? D.near_list[q] : -1; }
-    double x[4], y[4], z[4], vx[4], vy[4], vz[4];
-    int f[4];
-    unsigned char st[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++)
-      if (p[u] >= 0) {
-        x[u] = D.x[p[u]]; y[u] = D.y[p[u]]; z[u] = D.z[p[u]];
-        vx[u] = D.vx[p[u]]; vy[u] = D.vy[p[u]]; vz[u] = D.vz[p[u]];
-        f[u] = D.flag[p[u]]; st[u] = D.status[p[u]];
-      }
-#pragma unroll
-    for (int u = 0; u < 4; u++)
-      if (p[u] >= 0) {
-        int q = q0 + u * NE_T;
-        D.ns_x[q] = x[u]; D.ns_y[q] = y[u]; D.ns_z[q] = z[u];
-        D.ns_vx[q] = vx[u]; D.ns_vy[q] = vy[u]; D.ns_vz[q] = vz[u];
-        D.ns_flag[q] = f[u]; D.ns_stat[q] = st[u];
-      }
-  }
-}
-__device__ __forceinline__ void ne_scatter(const CoD& D, int nn) {
-  for (int q0 = threadIdx.x; q0 < nn; q0 += 4 * NE_T) {
-    int p[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) { int q = q0 + u * NE_T; p[u] = q < nn ? D.near_list[q] : -1; }
-#pragma unroll
-    for (int u = 0; u < 4; u++)
-      if (p[u] >= 0) {
-        int q = q0 + u * NE_T;
-        D.x[p[u]] = D.ns_x[q]; D.y[p[u]] = D.ns_y[q]; D.z[p[u]] = D.ns_z[q];
-        D.vx[p[u]] = D.ns_vx[q]; D.vy[p[u]] = D.ns_vy[q]; D.vz[p[u]] = D.ns_vz[q];
-        D.flag[p[u]] = D.ns_flag[q]; D.status[p[u]] = D.ns_stat[q];
-      }
-  }
-}
+// The near engine in two variants.  ne_solo stages up to 2304 inside entries
+// in LDS (150 KB: one engine per CU, for a lone system whose engine runs
+// beside light work).  ne_batch stages 1664 (110 KB), so that one 46 KB
+// workgroup of the split far flight fits on the CU beside each of a batch's
+// engines instead of waiting for them; inside entries beyond the LDS share
+// live in the HBM slot arrays (the same arithmetic, so the same results).
+namespace ne_solo {
+constexpr int LCAP = 2304;
+#include "mpcx_near_engine.inc"
+}  // namespace ne_solo
+namespace ne_batch {
+constexpr int LCAP = 1664;
+#include "mpcx_near_engine.inc"
+}  // namespace ne_batch
+using ne_solo::co_near_engine;
+using ne_solo::co_near_engine_c;
+using ne_batch::co_near_engine_b;
 
-// inside entry k <-> slot storage (LDS below LCAP, HBM above)
-__device__ __forceinline__ void slot_from_ns(const CoD& D, int k, int q) {
-  if (k < LCAP) {
-    g_ne.lx[k] = D.ns_x[q]; g_ne.ly[k] = D.ns_y[q]; g_ne.lz[k] = D.ns_z[q];
-    g_ne.lvx[k] = D.ns_vx[q]; g_ne.lvy[k] = D.ns_vy[q]; g_ne.lvz[k] = D.ns_vz[q];
-    g_ne.lflag[k] = D.ns_flag[q];
-  } else {
-    D.sl_x[k] = D.ns_x[q]; D.sl_y[k] = D.ns_y[q]; D.sl_z[k] = D.ns_z[q];
-    D.sl_vx[k] = D.ns_vx[q]; D.sl_vy[k] = D.ns_vy[q]; D.sl_vz[k] = D.ns_vz[q];
-    D.sl_flag[k] = D.ns_flag[q];
-  }
-}
-__device__ __forceinline__ void ne_store_slots(const CoD& D, int n) {
-  for (int k = threadIdx.x; k < n; k += NE_T) {
-    const int q = D.in_list[k];
-    if (k < LCAP) {
-      D.ns_x[q] = g_ne.lx[k]; D.ns_y[q] = g_ne.ly[k]; D.ns_z[q] = g_ne.lz[k];
-      D.ns_vx[q] = g_ne.lvx[k]; D.ns_vy[q] = g_ne.lvy[k]; D.ns_vz[q] = g_ne.lvz[k];
-      D.ns_flag[q] = g_ne.lflag[k];
-    } else {
-      D.ns_x[q] = D.sl_x[k]; D.ns_y[q] = D.sl_y[k]; D.ns_z[q] = D.sl_z[k];
-      D.ns_vx[q] = D.sl_vx[k]; D.ns_vy[q] = D.sl_vy[k]; D.ns_vz[q] = D.sl_vz[k];
-      D.ns_flag[q] = D.sl_flag[k];
-    }
-  }
-}
-
-// Rebuild, first half: the lazy streaming of the near particles outside the
-// lists (evolve_mod.f95:467-529) when do_stream, the neighbour-list
-// memberships of every near particle at its position (neighbour_list_mod.f95:
-// 38-57, 129-167) when do_memb, and max v^2 over the near set
-// (evolve_mod.f95:555-562).  Returns the wave's count of list members.
-template <int ND>
-__device__ __forceinline__ int ne_memb_pass(const CoP& P, const CoD& D, bool do_stream, bool do_memb, MembN<ND>& mk) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int nn = g_ne.near_num;
-  const int E = (nn + NE_T - 1) / NE_T;
-  const int qb = wv * 64 * E + lane;
-  const double tc = (double)g_ne.tc;
-  double vm = 0.0;
-  int cnt = 0;
-  mk.clear();
-  for (int j0 = 0; j0 < E; j0 += 2) {
-    int qq[2];
-    double x[2], y[2], z[2], vx[2], vy[2], vz[2];
-    int f[2];
-    unsigned char st[2];
-#pragma unroll
-    for (int u = 0; u < 2; u++) {
-      qq[u] = (j0 + u < E) ? qb + (j0 + u) * 64 : nn;
-      if (qq[u] < nn) {
-        const int q = qq[u];
-        x[u] = D.ns_x[q]; y[u] = D.ns_y[q]; z[u] = D.ns_z[q];
-        vx[u] = D.ns_vx[q]; vy[u] = D.ns_vy[q]; vz[u] = D.ns_vz[q];
-        f[u] = D.ns_flag[q]; st[u] = D.ns_stat[q];
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < 2; u++) {
-      const bool valid = qq[u] < nn;
-      unsigned m = 0;
-      if (valid) {
-        const int q = qq[u];
-        if (do_stream && !(st[u] & ST_INSIDE)) {
-          const int f0 = f[u];
-          bool b = stream_particle(x[u], y[u], z[u], vx[u], vy[u], vz[u], f[u], tc, P, D.react_box, &D.sc->err);
-          D.ns_x[q] = x[u]; D.ns_y[q] = y[u]; D.ns_z[q] = z[u];
-          if (b) { D.ns_vx[q] = vx[u]; D.ns_vy[q] = vy[u]; D.ns_vz[q] = vz[u]; }
-          if (f[u] != f0) D.ns_flag[q] = f[u];
-        }
-        vm = fmax(vm, (vx[u] * vx[u] + vy[u] * vy[u]) + vz[u] * vz[u]);
-        if (do_memb) m = memberships<ND>(x[u], y[u], z[u], P, g_ne.sd);
-      }
-      if (j0 + u < E) {
-        mk.set(j0 + u, m);
-        cnt += __popcll(__ballot(m != 0u));
-      }
-    }
-  }
-  vm = wave_max(vm);
-  if (lane == 0) g_ne.red[wv][0] = vm;
-  __syncthreads();
-  if (threadIdx.x == 0 && do_stream) {
-    double m = 0.0;
-    for (int w = 0; w < NE_W; w++) m = fmax(m, g_ne.red[w][0]);
-    g_ne.vmax2 = m;
-  }
-  return cnt;
-}
-
-// Rebuild, second half: the inside list in ascending particle order (as
-// inout_particle_list builds it at one thread), its memberships, its slot
-// state, and time_penetrate (evolve_mod.f95:563).
-template <int ND>
-__device__ __forceinline__ void ne_compact(const CoP& P, const CoD& D, const MembN<ND>& mk, int wcount) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int nn = g_ne.near_num;
-  const int E = (nn + NE_T - 1) / NE_T;
-  const int qb = wv * 64 * E + lane;
-  if (lane == 0) g_ne.wsum[wv] = wcount;
-  ne_wave_scan();
-  int total = g_ne.wsum[NE_W];
-  if (total > D.in_cap) {
-    if (threadIdx.x == 0) raise_error(&D.sc->err, ERR_INSIDE_CAP);
-    total = D.in_cap;
-  }
-  int base = g_ne.wsum[wv];
-  const unsigned long long lt = lanes_below();
-  // three entries per pass: their slot loads are issued together, so the L2
-  // latency is paid once per pass rather than once per entry
-  constexpr int G = 3;
-  for (int j0 = 0; j0 < E; j0 += G) {
-    int qg[G], kg[G];
-    unsigned mg[G];
-#pragma unroll
-    for (int u = 0; u < G; u++) {
-      const int j = j0 + u;
-      qg[u] = qb + j * 64;
-      mg[u] = j < E ? mk.get(j) : 0u;
-      const unsigned long long b = __ballot(mg[u] != 0u);
-      kg[u] = base + __popcll(b & lt);
-      base += __popcll(b);
-      if (j >= E || qg[u] >= nn) mg[u] = 0u, qg[u] = -1;
-    }
-    double sx[G], sy[G], sz[G], svx[G], svy[G], svz[G];
-    int sf[G];
-#pragma unroll
-    for (int u = 0; u < G; u++)
-      if (mg[u] != 0u && kg[u] < D.in_cap) {
-        const int q = qg[u];
-        sx[u] = D.ns_x[q]; sy[u] = D.ns_y[q]; sz[u] = D.ns_z[q];
-        svx[u] = D.ns_vx[q]; svy[u] = D.ns_vy[q]; svz[u] = D.ns_vz[q];
-        sf[u] = D.ns_flag[q];
-      }
-#pragma unroll
-    for (int u = 0; u < G; u++) {
-      const int q = qg[u], k = kg[u];
-      const unsigned m = mg[u];
-      if (q < 0) continue;
-      if (m != 0u && k < D.in_cap) {
-        D.in_list[k] = q;
-        D.in_memb[k] = m;
-        if (k < LCAP) {
-          g_ne.lmemb[k] = m; g_ne.lmsd[k] = 0.0;
-          g_ne.lx[k] = sx[u]; g_ne.ly[k] = sy[u]; g_ne.lz[k] = sz[u];
-          g_ne.lvx[k] = svx[u]; g_ne.lvy[k] = svy[u]; g_ne.lvz[k] = svz[u];
-          g_ne.lflag[k] = sf[u];
-        } else {
-          D.in_msd[k] = 0.0;
-          D.sl_x[k] = sx[u]; D.sl_y[k] = sy[u]; D.sl_z[k] = sz[u];
-          D.sl_vx[k] = svx[u]; D.sl_vy[k] = svy[u]; D.sl_vz[k] = svz[u];
-          D.sl_flag[k] = sf[u];
-        }
-      }
-      D.ns_stat[q] = m != 0u ? ST_INSIDE : 0;
-    }
-  }
-  if (threadIdx.x == 0) {
-    g_ne.in_num = total;
-    double fm = g_ne.fmaxv2;
-    double vm = g_ne.vmax2;
-    if (g_ne.clustered) vm = clu_exchange_max(D.clu, D.cid, vm, &D.sc->err);
-    double maxt = fmax(fm, vm);
-    g_ne.tp = (int)floor(((0.5 * P.skinrad) / sqrt(maxt)) / P.MDt);
-    g_ne.tc = 0;
-  }
-  __syncthreads();
-}
-
-// One list rebuild over the near set: optionally the inside entries written
-// back to their slots first, then the membership pass and, unless the launch
-// stops here for a sort, the compaction.
-template <int ND>
-__device__ __attribute__((noinline)) void ne_rebuild(bool store_first, bool do_stream, bool compact) {
-  const CoP& P = g_ne.P;
-  const CoD& D = g_ne.D;
-  if (store_first) {
-    ne_store_slots(D, g_ne.in_num);
-    __threadfence_block();
-    __syncthreads();
-  }
-  MembN<ND> mk;
-  int c = ne_memb_pass<ND>(P, D, do_stream, compact, mk);
-  if (compact) ne_compact<ND>(P, D, mk, c);
-}
-
-// Cluster mode: the list memberships, stored pair forces and
-// max_solvent_dist of an inside particle live in per-particle records between
-// intervals (the clusters may be regrouped), keyed by global list 2k+i.
-__device__ __forceinline__ void ne_load_record(const CoD& D, int k, int p) {
-  const int cap = D.in_cap;
-  const unsigned ids = D.rec_ids[p];
-  unsigned mb = 0;
-  for (int m = 0; m < 4; m++) {
-    const unsigned gid = (ids >> (8 * m)) & 0xffu;
-    if (gid == 0xffu) break;
-    const int kg = (int)(gid >> 1), ii = (int)(gid & 1u);
-    if (kg >= MAXDG || D.clu->clu_of[kg] != D.cid || D.clu->loc[kg] >= MAXD) {
-      raise_error(&D.sc->err, ERR_CLUSTER_MEMB);
-      continue;
-    }
-    const int b = 2 * D.clu->loc[kg] + ii;
-    mb |= 1u << b;
-    for (int c = 0; c < 3; c++) D.in_fp[(b * 3 + c) * cap + k] = D.rec_fp[(size_t)p * 12 + m * 3 + c];
-    D.in_pot[b * cap + k] = D.rec_pot[(size_t)p * 4 + m];
-  }
-  D.in_memb[k] = mb;
-  D.in_msd[k] = D.rec_msd[p];
-}
-__device__ __forceinline__ void ne_write_records(const CoD& D, int n) {
-  const int cap = D.in_cap, nb = 2 * min(D.clu->cnt[D.cid], MAXD);
-  for (int k = threadIdx.x; k < n; k += NE_T) {
-    const int p = D.in_plist[k];
-    const unsigned mb = D.in_memb[k];
-    unsigned ids = 0xffffffffu;
-    int m = 0;
-    for (int b = 0; b < nb; b++) {
-      if (!((mb >> b) & 1u)) continue;
-      if (m == 4) { raise_error(&D.sc->err, ERR_MEMBERSHIPS); break; }
-      const unsigned gid = 2u * (unsigned)D.clu->dims[D.cid][b >> 1] + (unsigned)(b & 1);
-      ids = (ids & ~(0xffu << (8 * m))) | (gid << (8 * m));
-      for (int c = 0; c < 3; c++) D.rec_fp[(size_t)p * 12 + m * 3 + c] = D.in_fp[(b * 3 + c) * cap + k];
-      D.rec_pot[(size_t)p * 4 + m] = D.in_pot[b * cap + k];
-      m++;
-    }
-    D.rec_ids[p] = ids;
-    D.rec_msd[p] = D.in_msd[k];
-  }
-}
-
-// Interval start: the inside list (unchanged since the last rebuild, same
-// ascending order, so entry k keeps its memberships, stored forces and
-// max_solvent_dist) re-expressed as near indices of this interval's set.
-__device__ __attribute__((noinline)) void ne_map_inside() {
-  const CoD& D = g_ne.D;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int nn = g_ne.near_num;
-  const int E = (nn + NE_T - 1) / NE_T;
-  const int qb = wv * 64 * E + lane;
-  int cnt = 0;
-  for (int j = 0; j < E; j++) {
-    const int q = qb + j * 64;
-    cnt += __popcll(__ballot(q < nn && (D.ns_stat[q] & ST_INSIDE)));
-  }
-  if (lane == 0) g_ne.wsum[wv] = cnt;
-  ne_wave_scan();
-  if (g_ne.wsum[NE_W] != g_ne.in_num) {   // every list member must be in the near set
-    if (threadIdx.x == 0) raise_error(&D.sc->err, ERR_INSIDE_LOST);
-  }
-  const int n = min(g_ne.in_num, D.in_cap);
-  int base = g_ne.wsum[wv];
-  const unsigned long long lt = lanes_below();
-  for (int j = 0; j < E; j++) {
-    const int q = qb + j * 64;
-    const bool in = q < nn && (D.ns_stat[q] & ST_INSIDE);
-    const unsigned long long b = __ballot(in);
-    const int k = base + __popcll(b & lt);
-    base += __popcll(b);
-    if (in && k < n) {
-      D.in_list[k] = q;
-      slot_from_ns(D, k, q);
-      if (g_ne.clustered) ne_load_record(D, k, D.near_list[q]);
-      if (k < LCAP) { g_ne.lmemb[k] = D.in_memb[k]; g_ne.lmsd[k] = D.in_msd[k]; }
-    }
-  }
-  __syncthreads();
-}
-
-// thread 0: bookkeeping of the next MD step, RATTLE positions and the
-// rebuild test of evolve_mod.f95:457-465
-template <int ND>
-__device__ __forceinline__ void ne_begin_step_regs(const CoP& P, const CoD& D, Dimer<ND>& d, double maxv) {
-  g_ne.cstep += 1;
-  g_ne.tc += 1;
-  rattle_positions<ND>(P, d, &D.sc->err);
-  if (g_ne.clustered) maxv = clu_exchange_max(D.clu, D.cid, maxv, &D.sc->err);
-  const int rb = (maxv > P.hs2 || g_ne.tc == g_ne.tp || g_ne.cstep % P.MPCtstep == 0);
-  g_ne.rebuild = rb;
-  if (rb) {
-    if (g_ne.nev < MAXEV) g_ne.ev[g_ne.nev] = g_ne.tc; else raise_error(&D.sc->err, ERR_EVENTS);
-    g_ne.nev += 1;
-#pragma unroll
-    for (int s = 0; s < 2 * ND; s++) {   // the near/far split needs every sphere inside its guard
-      double a = pbc_half(d.v[DX][s] - g_ne.x0[s], P.box[0]);
-      double b = pbc_half(d.v[DY][s] - g_ne.y0[s], P.box[1]);
-      double c = d.v[DZ][s] - g_ne.z0[s];
-      if (a * a + b * b + c * c > P.guard * P.guard) raise_error(&D.sc->err, ERR_NEAR_GUARD);
-    }
-  }
-}
-template <int ND>
-__device__ __attribute__((noinline)) void ne_begin_step(double maxv) {
-  const CoP& P = g_ne.P;
-  const CoD& D = g_ne.D;
-  Dimer<ND> d;
-  dimer_load<ND>(d, g_ne.sd);
-  ne_begin_step_regs<ND>(P, D, d, maxv);
-  dimer_store<ND>(d, g_ne.sd);
-}
-// thread 0, end of step t: dimer forces, velocity RATTLE and, unless t is the
-// last step of the launch, the start of step t+1
-template <int ND>
-__device__ __forceinline__ void ne_end_step_body(bool last);
-// One dimer: inlined into the kernel (a call saves and restores some 33
-// callee-saved registers through scratch, a memory round trip per MD step on
-// the serial path); more dimers need more registers than the kernel has left.
-template <int ND>
-__device__ __attribute__((noinline)) void ne_end_step_call(bool last) { ne_end_step_body<ND>(last); }
-template <int ND>
-__device__ __forceinline__ void ne_end_step(bool last) {
-  if constexpr (ND == 1) ne_end_step_body<ND>(last);
-  else ne_end_step_call<ND>(last);
-}
-template <int ND>
-__device__ __forceinline__ void ne_end_step_body(bool last) {
-  const CoP& P = g_ne.P;
-  const CoD& D = g_ne.D;
-  Dimer<ND> d;
-  dimer_load<ND>(d, g_ne.sd);
-  double tpot;
-  double fin[4 * 2 * ND + 1];
-#pragma unroll
-  for (int v = 0; v < 4 * 2 * ND + 1; v++) fin[v] = g_ne.fin[v];
-  dimer_forces_and_rattle_v<ND>(P, d, fin, tpot, &D.sc->err);
-  g_ne.tpot = tpot;
-  if (!last) ne_begin_step_regs<ND>(P, D, d, fin[4 * 2 * ND]);
-  dimer_store<ND>(d, g_ne.sd);
-}
-
-// kick (evolve_mod.f95:225-250 and :645-670) in list order, then inside
-// streaming (:252-316) of one entry
-#define NE_KICK(vx, vy, vz, fl, mb, fx, fy, fz)                  \
-  {                                                              \
-    bool ok_;                                                    \
-    double h_ = kick_factor(P, fl, ok_);                         \
-    if (ok_) {                                                   \
-      _Pragma("unroll") for (int b_ = 0; b_ < 2 * ND; b_++)      \
-        if (((mb) >> b_) & 1u) {                                 \
-          vx = vx + h_ * fx[b_];                                 \
-          vy = vy + h_ * fy[b_];                                 \
-          vz = vz + h_ * fz[b_];                                 \
-        }                                                        \
-    }                                                            \
-  }
-#define NE_STREAM(x, y, z, vx, vy, vz, fl, ms)                                              \
-  {                                                                                        \
-    stream_particle(x, y, z, vx, vy, vz, fl, 1.0, P, D.react_box, &D.sc->err);             \
-    double v2_ = (vx * vx + vy * vy) + vz * vz;                                            \
-    ms = ms + v2_ * (P.MDt * P.MDt);                                                       \
-  }
-
-// slot field access: LDS (L = true) or the HBM slot arrays
-#define SLOT(L, f, k) (*((L) ? &g_ne.l##f[k] : &D.sl_##f[k]))
-
-// Particle phase of one inside entry: forces (forces_mod.f95:62-136), kick,
-// and, unless this is the launch's last step, the next step's kick and
-// streaming (they do not depend on the dimer); at the last step the pair
-// forces are stored for the first kick of the next interval.
-template <int ND, bool L>
-__device__ __forceinline__ void ne_particle(const CoP& P, const CoD& D, int k, bool last, double (&acc)[8 * ND + 1]) {
-  const int cap = D.in_cap;
-  double x, y, z, vx, vy, vz, ms;
-  int fl;
-  unsigned mb;
-  if constexpr (L) {
-    x = g_ne.lx[k]; y = g_ne.ly[k]; z = g_ne.lz[k]; vx = g_ne.lvx[k]; vy = g_ne.lvy[k]; vz = g_ne.lvz[k];
-    fl = g_ne.lflag[k]; mb = g_ne.lmemb[k]; ms = g_ne.lmsd[k];
-  } else {
-    x = D.sl_x[k]; y = D.sl_y[k]; z = D.sl_z[k]; vx = D.sl_vx[k]; vy = D.sl_vy[k]; vz = D.sl_vz[k];
-    fl = D.sl_flag[k]; mb = D.in_memb[k]; ms = D.in_msd[k];
-  }
-  double fx[2 * ND], fy[2 * ND], fz[2 * ND], pot[2 * ND];
-  // lists in the reference's order: (dimer kk, sphere ii) -> b = 2kk + ii
-#pragma unroll
-  for (int b = 0; b < 2 * ND; b++) {
-    const int kk = b >> 1, ii = b & 1, s = kk + ii * ND;
-    fx[b] = fy[b] = fz[b] = pot[b] = 0.0;
-    if (!((mb >> b) & 1u)) continue;
-    double d1 = pbc_half(x - g_ne.sd[DX][s], P.box[0]);
-    double d2 = pbc_half(y - g_ne.sd[DY][s], P.box[1]);
-    double d3 = z - g_ne.sd[DZ][s];
-    double dRsq = d1 * d1 + d2 * d2 + d3 * d3;
-    if (dRsq <= P.cut2[ii]) {
-      double E = 0.0;
-      bool on = false;
-      if (fl == 0) {
-        if (ii == 0) { fl = 1; E = P.Eb[0]; } else E = P.Ea[1];
-        on = true;
-      } else if (fl == 1) { E = P.Eb[ii]; on = true; }
-      else if (fl == 2) { E = P.Es[ii]; on = true; }
-      if (on) {
-        double F, Ub;
-        lj_terms(dRsq, P.R6[ii], P.R12[ii], P.r12R12[ii], P.r6R6[ii], F, Ub);
-        double EF = E * F;
-        fx[b] = EF * d1; fy[b] = EF * d2; fz[b] = EF * d3; pot[b] = E * Ub;
-      }
-    }
-    acc[4 * s + 0] += fx[b]; acc[4 * s + 1] += fy[b];
-    acc[4 * s + 2] += fz[b]; acc[4 * s + 3] += pot[b];
-  }
-  NE_KICK(vx, vy, vz, fl, mb, fx, fy, fz);
-  if (!last) {
-    NE_KICK(vx, vy, vz, fl, mb, fx, fy, fz);
-    NE_STREAM(x, y, z, vx, vy, vz, fl, ms);
-    acc[8 * ND] = fmax(acc[8 * ND], ms);
-  } else {
-#pragma unroll
-    for (int b = 0; b < 2 * ND; b++) {
-      D.in_fp[(b * 3 + 0) * cap + k] = fx[b];
-      D.in_fp[(b * 3 + 1) * cap + k] = fy[b];
-      D.in_fp[(b * 3 + 2) * cap + k] = fz[b];
-      D.in_pot[b * cap + k] = pot[b];
-    }
-  }
-  SLOT(L, x, k) = x; SLOT(L, y, k) = y; SLOT(L, z, k) = z;
-  SLOT(L, vx, k) = vx; SLOT(L, vy, k) = vy; SLOT(L, vz, k) = vz;
-  SLOT(L, flag, k) = fl;
-  if constexpr (L) g_ne.lmsd[k] = ms; else D.in_msd[k] = ms;
-}
-
-// first step of an interval: kick with the forces stored at the end of the
-// previous interval, then stream
-template <int ND, bool L>
-__device__ __forceinline__ double ne_first_kick(const CoP& P, const CoD& D, int k) {
-  const int cap = D.in_cap;
-  double x = SLOT(L, x, k), y = SLOT(L, y, k), z = SLOT(L, z, k);
-  double vx = SLOT(L, vx, k), vy = SLOT(L, vy, k), vz = SLOT(L, vz, k);
-  int fl = SLOT(L, flag, k);
-  double ms = L ? g_ne.lmsd[k] : D.in_msd[k];
-  const unsigned mb = D.in_memb[k];
-  double fx[2 * ND], fy[2 * ND], fz[2 * ND];
-#pragma unroll
-  for (int b = 0; b < 2 * ND; b++) {
-    fx[b] = D.in_fp[(b * 3 + 0) * cap + k];
-    fy[b] = D.in_fp[(b * 3 + 1) * cap + k];
-    fz[b] = D.in_fp[(b * 3 + 2) * cap + k];
-  }
-  NE_KICK(vx, vy, vz, fl, mb, fx, fy, fz);
-  NE_STREAM(x, y, z, vx, vy, vz, fl, ms);
-  SLOT(L, x, k) = x; SLOT(L, y, k) = y; SLOT(L, z, k) = z;
-  SLOT(L, vx, k) = vx; SLOT(L, vy, k) = vy; SLOT(L, vz, k) = vz;
-  SLOT(L, flag, k) = fl;
-  if constexpr (L) g_ne.lmsd[k] = ms; else D.in_msd[k] = ms;
-  return ms;
-}
-
-// census exclusion of an inside entry (distribute_mod.f95:149-158)
-template <int ND, bool L>
-__device__ __forceinline__ bool ne_excluded(const CoP& P, const CoD& D, int k) {
-  const double x = SLOT(L, x, k), y = SLOT(L, y, k), z = SLOT(L, z, k);
-  bool excl = false;
-#pragma unroll
-  for (int kk = 0; kk < ND; kk++)
-#pragma unroll
-    for (int ii = 0; ii < 2; ii++) {
-      const int s = kk + ii * ND;
-      double d1 = pbc_half(x - g_ne.sd[DX][s], P.box[0]);
-      double d2 = pbc_half(y - g_ne.sd[DY][s], P.box[1]);
-      double d3 = z - g_ne.sd[DZ][s];
-      if (d1 * d1 + d2 * d2 + d3 * d3 <= P.cut2[ii]) excl = true;
-    }
-  return excl;
-}
-
-__device__ __forceinline__ void ne_save_scalars(CoScalars* sc) {
-  sc->cstep = g_ne.cstep; sc->time_count = g_ne.tc; sc->time_penetrate = g_ne.tp; sc->in_num = g_ne.in_num;
-  sc->n_ev = g_ne.nev; sc->near_vmax2 = g_ne.vmax2;
-  sc->tpotential = g_ne.tpot;
-  for (int e = 0; e < g_ne.nev && e < MAXEV; e++) sc->events[e] = g_ne.ev[e];
-}
-
-// One interval of the near set of one system.  P and D are the launch
-// parameters (kernel arguments, or the LDS copies g_ne.P/g_ne.D when
-// `in_lds`, which then need no copy).
-template <int ND>
-__device__ __forceinline__ void near_body(const CoP P, const CoD D, int nsteps, int mode, bool in_lds) {
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  constexpr int NSPH = 2 * ND;
-  constexpr int NV = 4 * NSPH + 1;
-  CoScalars* sc = D.sc;
-  const unsigned long long Tentry = __builtin_amdgcn_s_memrealtime();
-
-  for (int j = tid; j < 12 * MAXS; j += NE_T) g_ne.sd[j / MAXS][j % MAXS] = D.dim[(j / MAXS) * MAXSG + j % MAXS];
-  if (tid == 0) {
-    if (!in_lds) {
-      g_ne.P = P;
-      g_ne.D = D;
-    }
-    g_ne.cstep = sc->cstep; g_ne.tc = sc->time_count; g_ne.tp = sc->time_penetrate; g_ne.in_num = sc->in_num;
-    g_ne.nev = (mode & M_RESUME) ? sc->n_ev : 0;
-    g_ne.near_num = sc->near_num;
-    g_ne.vmax2 = sc->near_vmax2;
-    g_ne.rebuild = 0;
-    g_ne.clustered = (mode & M_CLUSTER) != 0;
-    g_ne.tpot = sc->tpotential;
-    g_ne.fmaxv2 = __longlong_as_double((long long)sc->far_maxv2);
-    for (int s = 0; s < NSPH; s++) { g_ne.x0[s] = sc->x0[s]; g_ne.y0[s] = sc->y0[s]; g_ne.z0[s] = sc->z0[s]; }
-    for (int e = 0; e < g_ne.nev && e < MAXEV; e++) g_ne.ev[e] = sc->events[e];
-  }
-  __syncthreads();
-  if (g_ne.near_num > NE_NEAR) {   // the set does not fit the lanes' membership registers
-    if (tid == 0) {
-      raise_error(&sc->err, ERR_NEAR_CAP);
-      if (mode & M_CLUSTER) atomicExch(&D.clu->abort, 1);
-    }
-    return;
-  }
-  ne_gather(D, g_ne.near_num);
-  __threadfence_block();
-  __syncthreads();
-
-  if (mode & M_SETUP) {
-    // main_pattern.f95:173-188 over the near set; near_vmax2 was set by the classifier kernel
-    ne_rebuild<ND>(false, false, true);
-    const int cap = D.in_cap;
-    for (int k = tid; k < g_ne.in_num; k += NE_T) {
-      D.in_plist[k] = D.near_list[D.in_list[k]];
-      for (int j = 0; j < MAXM * 3; j++) D.in_fp[j * cap + k] = 0.0;
-      for (int j = 0; j < MAXM; j++) D.in_pot[j * cap + k] = 0.0;
-      D.in_msd[k] = 0.0;
-    }
-    __threadfence_block();
-    __syncthreads();
-    if (mode & M_CLUSTER) ne_write_records(D, g_ne.in_num);
-    ne_scatter(D, g_ne.near_num);
-    if (tid == 0) {
-      ne_save_scalars(sc);
-      sc->n_ev = 0;
-      for (int s = 0; s < NSPH; s++) { sc->x0[s] = g_ne.sd[DX][s]; sc->y0[s] = g_ne.sd[DY][s]; sc->z0[s] = g_ne.sd[DZ][s]; }
-    }
-    return;
-  }
-
-  const bool resume = (mode & M_RESUME) != 0;
-  unsigned long long T0 = __builtin_amdgcn_s_memrealtime(), Tr = 0, Tp = 0, Ts = 0, Nr = 0, ta, tb;
-  if (!resume) {
-    ne_map_inside();
-    double mx = 0.0;
-    for (int k = tid; k < g_ne.in_num; k += NE_T)
-      mx = fmax(mx, k < LCAP ? ne_first_kick<ND, true>(P, D, k) : ne_first_kick<ND, false>(P, D, k));
-    mx = wave_max(mx);
-    if (lane == 0) g_ne.red[wv][0] = mx;
-    __syncthreads();
-    if (tid == 0) {
-      double m = 0.0;
-      for (int w = 0; w < NE_W; w++) m = fmax(m, g_ne.red[w][0]);
-      ne_begin_step<ND>(m);
-    }
-    __syncthreads();
-  } else {
-    g_ne.rebuild = 1;   // resuming inside the rebuild of the final step
-    __syncthreads();
-  }
-
-  for (int t = 1; t <= nsteps; t++) {
-    const bool last = (t == nsteps);
-    // ---- neighbour-list rebuild of step t (evolve_mod.f95:465-564) over the near set
-    ta = __builtin_amdgcn_s_memrealtime();
-    if (g_ne.rebuild) {
-      Nr++;
-      if (!(resume && last)) {
-        const bool stop = (mode & M_STOP) && last;
-        ne_rebuild<ND>(true, true, !stop);
-        if (stop) {   // the host sorts, then resumes with M_RESUME
-          __threadfence_block();
-          __syncthreads();
-          ne_scatter(D, g_ne.near_num);
-          if (tid == 0) ne_save_scalars(sc);
-          for (int j = tid; j < 12 * NSPH; j += NE_T) D.dim[(j / NSPH) * MAXSG + j % NSPH] = g_ne.sd[j / NSPH][j % NSPH];
-          return;
-        }
-      } else {
-        ne_rebuild<ND>(false, false, true);
-      }
-    }
-    tb = __builtin_amdgcn_s_memrealtime();
-    Tr += tb - ta;
-
-    // ---- particle phase
-    double acc[NV];
-#pragma unroll
-    for (int v = 0; v < NV; v++) acc[v] = 0.0;
-    const int n = g_ne.in_num;
-    for (int k = tid; k < n; k += NE_T) {
-      if (k < LCAP) ne_particle<ND, true>(P, D, k, last, acc);
-      else ne_particle<ND, false>(P, D, k, last, acc);
-    }
-    // ---- block reduction in a fixed order: sphere sums and max_solvent_dist
-#pragma unroll
-    for (int v = 0; v < NV - 1; v++) {
-      double r = wave_sum(acc[v]);
-      if (lane == 0) g_ne.red[wv][v] = r;
-    }
-    {
-      double r = wave_max(acc[NV - 1]);
-      if (lane == 0) g_ne.red[wv][NV - 1] = r;
-    }
-    __syncthreads();
-    if (tid < NV) {
-      double a = g_ne.red[0][tid];
-      for (int w = 1; w < NE_W; w++) a = (tid == NV - 1) ? fmax(a, g_ne.red[w][tid]) : a + g_ne.red[w][tid];
-      g_ne.fin[tid] = a;
-    }
-    __syncthreads();
-    ta = __builtin_amdgcn_s_memrealtime();
-    Tp += ta - tb;
-    if (tid == 0) ne_end_step<ND>(last);
-    __syncthreads();
-    Ts += __builtin_amdgcn_s_memrealtime() - ta;
-  }
-
-  // ---- end of the interval: census exclusion, write back through the near slots
-  const unsigned long long Tend = __builtin_amdgcn_s_memrealtime();
-  const int n = g_ne.in_num;
-  for (int k = tid; k < n; k += NE_T) {
-    const bool ex = k < LCAP ? ne_excluded<ND, true>(P, D, k) : ne_excluded<ND, false>(P, D, k);
-    const int q = D.in_list[k];
-    D.ns_stat[q] = ST_INSIDE | (ex ? ST_EXCL : 0);
-    D.in_plist[k] = D.near_list[q];
-    if (k < LCAP) D.in_msd[k] = g_ne.lmsd[k];
-  }
-  if (mode & M_CLUSTER) {
-    __threadfence_block();
-    __syncthreads();
-    ne_write_records(D, n);
-  }
-  ne_store_slots(D, n);
-  __threadfence_block();
-  __syncthreads();
-  ne_scatter(D, g_ne.near_num);
-  if (tid == 0) {
-    sc->prof[0] += __builtin_amdgcn_s_memrealtime() - T0;
-    sc->prof[1] += Tr; sc->prof[2] += Tp; sc->prof[3] += Ts; sc->prof[4] += Nr;
-    sc->prof[5] = g_ne.near_num; sc->prof[6] = g_ne.in_num; sc->prof[7] += 1;
-    sc->prof[8] += T0 - Tentry; sc->prof[9] += __builtin_amdgcn_s_memrealtime() - Tend;
-    ne_save_scalars(sc);
-    for (int s = 0; s < NSPH; s++) { sc->x0[s] = g_ne.sd[DX][s]; sc->y0[s] = g_ne.sd[DY][s]; sc->z0[s] = g_ne.sd[DZ][s]; }
-  }
-  for (int j = tid; j < 12 * NSPH; j += NE_T) D.dim[(j / NSPH) * MAXSG + j % NSPH] = g_ne.sd[j / NSPH][j % NSPH];
-}
-
-template <int ND>
-__global__ __launch_bounds__(NE_T) void co_near_engine(CoP P, CoD D, int nsteps, int mode) {
-  near_body<ND>(P, D, nsteps, mode, false);
-}
-// one workgroup per system of a batch
-template <int ND>
-__global__ __launch_bounds__(NE_T) void co_near_engine_b(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds,
-                                                         int nsteps, int mode) {
-  near_body<ND>(Ps[blockIdx.x], Ds[blockIdx.x], nsteps, mode, false);
-}
-// one workgroup per cluster of a many-dimer system (a cooperative launch:
-// every workgroup is resident, so the clusters' grid barriers complete)
-__global__ __launch_bounds__(NE_T) void co_near_engine_c(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds,
-                                                         const CluInfo* __restrict__ ci, int nsteps, int mode) {
-  const int c = blockIdx.x;
-  if (c >= ci->nclu) return;
-  switch (Ps[c].nd) {
-    case 1: near_body<1>(Ps[c], Ds[c], nsteps, mode | M_CLUSTER, false); break;
-    case 2: near_body<2>(Ps[c], Ds[c], nsteps, mode | M_CLUSTER, false); break;
-    case 3: near_body<3>(Ps[c], Ds[c], nsteps, mode | M_CLUSTER, false); break;
-    default: near_body<4>(Ps[c], Ds[c], nsteps, mode | M_CLUSTER, false); break;
-  }
-}
-#undef NE_KICK
-#undef NE_STREAM
-#undef SLOT
 
 // ------------------------------------------------------------- clusters
 // Segment bases of the per-cluster near engines (cluster c owns entries
@@ -5127,17 +4407,37 @@ __global__ __launch_bounds__(256) void sort_scatter_b(const CoSortD* Ss, const T
   const CoSortD S = Ss[t.r];
   sort_scatter_body(S.key, S.rk, S.soff, S.sb, t.begin, t.end, 256);
 }
-// per tour cell of a tile: its members in ascending original index
+// per tour cell of a tile (256 cells): its members in ascending original
+// index.  The tile's members are one contiguous run of sb; a run of at most
+// ORD_CAP entries (10 per cell on average: 2,560) is sorted in LDS, read and
+// written back coalesced, a longer one in place in global memory.
+constexpr int ORD_CAP = 6144;
 __global__ __launch_bounds__(256) void sort_order_b(const CoSortD* Ss, const Tile* tiles) {
+  __shared__ int s[ORD_CAP];
   const Tile t = tiles[blockIdx.x];
   const CoSortD S = Ss[t.r];
   const long long c = t.begin + threadIdx.x;
-  if (c >= t.end) return;
-  const int n = min(S.scnt[c], BMAX), o = S.soff[c];
+  const int o0 = S.soff[t.begin], o1 = S.soff[t.end];   // soff holds vol + 1 entries
+  const int m = o1 - o0;
+  const bool lds = m <= ORD_CAP;   // the same for the whole block
+  int* seg = nullptr;
+  int n = 0;
+  if (c < t.end) n = min(S.scnt[c], BMAX);
+  if (lds) {
+    for (int j = threadIdx.x; j < m; j += 256) s[j] = S.sb[o0 + j];
+    __syncthreads();
+    if (c < t.end) seg = s + (S.soff[c] - o0);
+  } else if (c < t.end) {
+    seg = S.sb + S.soff[c];
+  }
   for (int j = 1; j < n; j++) {
-    int v = S.sb[o + j], k = j;
-    while (k > 0 && S.sb[o + k - 1] > v) { S.sb[o + k] = S.sb[o + k - 1]; k--; }
-    S.sb[o + k] = v;
+    int v = seg[j], k = j;
+    while (k > 0 && seg[k - 1] > v) { seg[k] = seg[k - 1]; k--; }
+    seg[k] = v;
+  }
+  if (lds) {
+    __syncthreads();
+    for (int j = threadIdx.x; j < m; j += 256) S.sb[o0 + j] = s[j];
   }
 }
 __global__ __launch_bounds__(256) void sort_gather_b(const CoD* Ds, const CoSortD* Ss, const Tile* tiles) {
@@ -5803,7 +5103,27 @@ struct mpcx_cob {
   std::vector<char> tev_on;
   double kt_ms[NKT] = {0};
   long long kt_n = 0;
+  // each replica's step counter (run start) or error flag (run end), read in one copy
+  int* d_io = nullptr;
+  std::vector<int> h_io;
 };
+
+// every replica's step counter into io[r], its collision counter set from it
+// (the host checks the steps before any interval runs)
+__global__ void cob_begin_k(const CoD* __restrict__ D, int R, int S, int* __restrict__ io) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  CoScalars* sc = D[r].sc;
+  const int cs = sc->cstep;
+  io[r] = cs;
+  sc->coll = (unsigned long long)(cs / S);
+}
+
+// every replica's device error flag into io[r]
+__global__ void cob_errs_k(const CoD* __restrict__ D, int R, int* __restrict__ io) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < R) io[r] = D[r].sc->err;
+}
 
 static int cob_tiles(mpcx_cob* b) {
   std::vector<Tile> pt, ct, vt, sc;
@@ -6086,7 +5406,8 @@ extern "C" int mpcx_cob_destroy(mpcx_cob* b) {
   if (b->st2) (void)hipStreamSynchronize(b->st2);
   cob_free_graph(b);
   for (void* p : {(void*)b->d_P, (void*)b->d_D, (void*)b->d_pt, (void*)b->d_ct, (void*)b->d_vt, (void*)b->d_gt,
-                  (void*)b->d_st, (void*)b->d_xt, (void*)b->d_nt, (void*)b->d_X, (void*)b->d_S, (void*)b->d_N})
+                  (void*)b->d_st, (void*)b->d_xt, (void*)b->d_nt, (void*)b->d_X, (void*)b->d_S, (void*)b->d_N,
+                  (void*)b->d_io})
     if (p) (void)hipFree(p);
   for (hipEvent_t ev : {b->e0, b->e1, b->fork, b->join})
     if (ev) (void)hipEventDestroy(ev);
@@ -6130,6 +5451,7 @@ static int cob_make(mpcx_cob** out, int32_t R, mpcx_co* const* engines, int32_t 
   b->h_X.resize(R);
   b->h_S.resize(R);
   b->h_N.resize(R);
+  b->h_io.resize(R);
   std::vector<CoP> hp(R);
   for (int r = 0; r < R; r++) {
     hp[r] = engines[r]->P;
@@ -6146,7 +5468,8 @@ static int cob_make(mpcx_cob** out, int32_t R, mpcx_co* const* engines, int32_t 
   }
   if (!rc && (hipMalloc(&b->d_P, R * sizeof(CoP)) != hipSuccess || hipMalloc(&b->d_D, R * sizeof(CoD)) != hipSuccess ||
               hipMalloc(&b->d_X, R * sizeof(CoSortD)) != hipSuccess || hipMalloc(&b->d_S, R * sizeof(SpD)) != hipSuccess ||
-              hipMalloc(&b->d_N, R * sizeof(SpD)) != hipSuccess || hipEventCreate(&b->e0) != hipSuccess ||
+              hipMalloc(&b->d_N, R * sizeof(SpD)) != hipSuccess || hipMalloc(&b->d_io, R * sizeof(int)) != hipSuccess ||
+              hipEventCreate(&b->e0) != hipSuccess ||
               hipEventCreate(&b->e1) != hipSuccess ||
               hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
               hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess))
@@ -6169,21 +5492,19 @@ extern "C" int mpcx_cob_create(mpcx_cob** out, int32_t R, mpcx_co* const* engine
 // checks that every replica sits at the same collision step, sets the
 // collision counters and uploads the replica descriptors
 static int cob_begin(mpcx_cob* b, int* cstep_out) {
-  int cstep = -1;
-  for (int r = 0; r < b->R; r++) {
-    mpcx_co* e = b->eng[r];
-    HIPCHK(hipStreamSynchronize(e->st));
-    int cs = 0;
-    HIPCHK(hipMemcpy(&cs, &e->D.sc->cstep, sizeof(int), hipMemcpyDeviceToHost));
-    if (cs % b->S != 0) return set_error(MPCX_E_ARG, "a batch runs from a collision step");
-    if (cstep >= 0 && cs != cstep) return set_error(MPCX_E_ARG, "replicas of a batch must be at the same step");
-    cstep = cs;
-    const unsigned long long coll = (unsigned long long)(cs / b->S);
-    HIPCHK(hipMemcpy(&e->D.sc->coll, &coll, sizeof coll, hipMemcpyHostToDevice));
-  }
-  *cstep_out = cstep;
+  if (!b->solo)
+    for (int r = 0; r < b->R; r++) HIPCHK(hipStreamSynchronize(b->eng[r]->st));
   int rc = cob_upload_D(b);
   if (rc) return rc;
+  hipLaunchKernelGGL(cob_begin_k, dim3((b->R + 255) / 256), dim3(256), 0, b->st, b->d_D, b->R, b->S, b->d_io);
+  HIPCHK(hipMemcpyAsync(b->h_io.data(), b->d_io, b->R * sizeof(int), hipMemcpyDeviceToHost, b->st));
+  HIPCHK(hipStreamSynchronize(b->st));
+  const int cstep = b->h_io[0];
+  for (int r = 0; r < b->R; r++) {
+    if (b->h_io[r] % b->S != 0) return set_error(MPCX_E_ARG, "a batch runs from a collision step");
+    if (b->h_io[r] != cstep) return set_error(MPCX_E_ARG, "replicas of a batch must be at the same step");
+  }
+  *cstep_out = cstep;
   if (b->split) {
     long long big = 0;
     for (int r = 0; r < b->R; r++)
@@ -6233,9 +5554,12 @@ static int cob_step(mpcx_cob* b, int* cstep, int use_graph, long long slot = -1)
 }
 
 static int cob_end(mpcx_cob* b) {
+  hipLaunchKernelGGL(cob_errs_k, dim3((b->R + 255) / 256), dim3(256), 0, b->st, b->d_D, b->R, b->d_io);
+  HIPCHK(hipMemcpyAsync(b->h_io.data(), b->d_io, b->R * sizeof(int), hipMemcpyDeviceToHost, b->st));
   HIPCHK(hipStreamSynchronize(b->st));
   for (int r = 0; r < b->R; r++) {
-    int rc = check_device_error(b->eng[r]);
+    if (!b->h_io[r]) continue;
+    int rc = check_device_error(b->eng[r]);   // reports and clears it
     if (rc) return rc;
   }
   return 0;

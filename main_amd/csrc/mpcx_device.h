@@ -90,6 +90,41 @@ __device__ __forceinline__ double wave_max(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, MPCX_WAVE));
   return v;
 }
+// The same reductions on the VALU through DPP lane moves (no LDS round trip
+// per step): sums over pairs, quads, half-rows and rows of 16 by quad
+// permutes and row mirrors, then the four row totals, read from lanes 15, 31,
+// 47 and 63, added in row order.  Each step adds a lane's value to its
+// partner's and the partner does the same (commutative), so every lane of a
+// row holds the same bits and the result is one wave-uniform value.  All 64
+// lanes must be active.
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov_d(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ double lane_read_d(double v, int lane) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(unsigned)b, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+constexpr int DPP_QUAD_1032 = 0xb1, DPP_QUAD_2301 = 0x4e, DPP_ROW_MIRROR = 0x140, DPP_ROW_HALF_MIRROR = 0x141;
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+  v = v + dpp_mov_d<DPP_QUAD_1032>(v);
+  v = v + dpp_mov_d<DPP_QUAD_2301>(v);
+  v = v + dpp_mov_d<DPP_ROW_HALF_MIRROR>(v);
+  v = v + dpp_mov_d<DPP_ROW_MIRROR>(v);
+  return ((lane_read_d(v, 15) + lane_read_d(v, 31)) + lane_read_d(v, 47)) + lane_read_d(v, 63);
+}
+__device__ __forceinline__ double wave_max_dpp(double v) {
+  v = fmax(v, dpp_mov_d<DPP_QUAD_1032>(v));
+  v = fmax(v, dpp_mov_d<DPP_QUAD_2301>(v));
+  v = fmax(v, dpp_mov_d<DPP_ROW_HALF_MIRROR>(v));
+  v = fmax(v, dpp_mov_d<DPP_ROW_MIRROR>(v));
+  return fmax(fmax(fmax(lane_read_d(v, 15), lane_read_d(v, 31)), lane_read_d(v, 47)), lane_read_d(v, 63));
+}
 __device__ __forceinline__ int wave_sum_i(int v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, MPCX_WAVE);
