@@ -88,6 +88,23 @@ int mpcx_inout_particle_list(int32_t NT, const double* box, double cutoffC, doub
                              int64_t* inside_particle_num, int64_t* inside_particle,
                              int64_t inside_capacity, int32_t numdims);
 
+/* No counterpart in the reference: mpcx_inout_particle_list for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The same lists, appended the same
+ * way, but xsol/ysol/zsol and the two lists are DEVICE arrays the caller owns
+ * (outside_capacity/inside_capacity entries); box, the sphere arrays and the
+ * counts stay on the host.  It calls no hipMalloc or hipFree: its temporaries
+ * live in a fixed 527,616-byte array that each GPU holds until the process
+ * ends.  Limits: NT <= 67,108,864 and numdims <= 64 (rejected with an error
+ * beyond them).  Runs on the null stream and returns when the lists are
+ * written.  Calls that overlap on one GPU share that array and race. */
+int mpcx_inout_particle_list_noalloc(int32_t NT, const double* box, double cutoffC, double cutoffN, double skinrad,
+                                     const double* xsol, const double* ysol, const double* zsol,
+                                     const double* xdim, const double* ydim, const double* zdim,
+                                     int64_t* outside_particle_num, int64_t* outside_particle,
+                                     int64_t outside_capacity, int64_t* inside_particle_num,
+                                     int64_t* inside_particle, int64_t inside_capacity, int32_t numdims);
+
 /* Replaces distribute (collective_code/distribute_mod.f95:6-40); the member
  * lists s?CellN are (vol, maxs) column-major. */
 int mpcx_distribute(int32_t NT, int32_t numdims, const int32_t* flag,

@@ -7,6 +7,7 @@
 #include <stdint.h>
 #include <string.h>
 #include <algorithm>
+#include <mutex>
 #include <vector>
 #include "mpcx_device.h"
 #include "mpcx_host.h"
@@ -185,6 +186,25 @@ __global__ __launch_bounds__(CB) void k_compact_write(int mode, int NT, const do
   unsigned long long tot;
   unsigned long long e = block_excl_scan64(p ? 1 : 0, s, tot);
   if (p) out[boff[blockIdx.x] + (long long)e] = i + 1;
+}
+
+// exclusive scan of the nb block counts into off[0..nb] (one block, in block order)
+__global__ __launch_bounds__(CB) void k_scan_blocks(int nb, const int* cnt, int* off) {
+  __shared__ unsigned long long s[CB / 64 + 1];
+  __shared__ long long run;
+  if (threadIdx.x == 0) run = 0;
+  __syncthreads();
+  for (int b0 = 0; b0 < nb; b0 += CB) {
+    const int b = b0 + threadIdx.x;
+    const unsigned long long v = b < nb ? (unsigned long long)cnt[b] : 0ull;
+    unsigned long long tot;
+    const unsigned long long e = block_excl_scan64(v, s, tot);
+    if (b < nb) off[b] = (int)(run + (long long)e);
+    __syncthreads();
+    if (threadIdx.x == 0) run += (long long)tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) off[nb] = (int)run;
 }
 
 static int compact(int mode, int NT, Buf& x, Buf& y, Buf& z, const double* box, Buf& sph, int nd, double r2a,
@@ -414,6 +434,109 @@ extern "C" int mpcx_inout_particle_list(int32_t NT, const double* box, double cu
   memcpy(outside_particle + *outside_particle_num, out.data(), out.size() * 8);
   *inside_particle_num += (int64_t)in.size();
   *outside_particle_num += (int64_t)out.size();
+  return 0;
+}
+
+// ------------------------------------------ inout_particle_list, no allocation
+// The scratch of mpcx_inout_particle_list_noalloc: the compaction's block
+// counts and offsets and the sphere coordinates.  Every GPU the library runs
+// on holds one copy until the process ends; host code reaches it only through
+// the address hipGetSymbolAddress returns on that device (the array's name in
+// host code is a stand-in, not a device address).
+namespace mpcx {
+namespace dropin {
+constexpr long long NA_MAX_NT = 1ll << 26;              // particles per call
+constexpr int NA_MAX_BLOCKS = (int)(NA_MAX_NT / CB);    // compaction blocks
+constexpr int NA_MAX_DIMERS = 64;
+constexpr size_t NA_OFF_COUNT = 0;                                          // int[NA_MAX_BLOCKS]
+constexpr size_t NA_OFF_OFFS = NA_OFF_COUNT + 4 * (size_t)NA_MAX_BLOCKS;    // int[NA_MAX_BLOCKS + 1]
+constexpr size_t NA_OFF_SPH = (NA_OFF_OFFS + 4 * ((size_t)NA_MAX_BLOCKS + 1) + 255) / 256 * 256;   // double[6 * 64]
+constexpr size_t NA_SCRATCH_BYTES = NA_OFF_SPH + 8 * 6 * (size_t)NA_MAX_DIMERS;   // 527,616 bytes
+static __device__ __align__(256) unsigned char scratch[NA_SCRATCH_BYTES];
+
+constexpr int NA_MAX_DEVICES = 64;
+static void* na_scratch_ptr[NA_MAX_DEVICES];
+static std::mutex na_mutex;
+
+// the current device's copy of scratch, resolved on the first call there
+static int na_scratch(unsigned char** out) {
+  int dev = -1;
+  DCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NA_MAX_DEVICES) return set_error(MPCX_E_ARG, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(na_mutex);
+  if (!na_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return set_error(MPCX_E_HIP, "hipGetSymbolAddress of the scratch array failed");
+    na_scratch_ptr[dev] = p;
+  }
+  *out = (unsigned char*)na_scratch_ptr[dev];
+  return 0;
+}
+
+// one list (mode 1 inside, 2 outside) of the particles in device memory,
+// appended at out (room for cap entries); *added = its length
+static int na_compact(int mode, int NT, const double* x, const double* y, const double* z, const double* box,
+                      unsigned char* sc, int nd, double r2a, double r2b, long long* out, long long cap,
+                      long long* added) {
+  const int nb = (NT + CB - 1) / CB;
+  int* cnt = (int*)(sc + NA_OFF_COUNT);
+  int* off = (int*)(sc + NA_OFF_OFFS);
+  const double* s = (const double*)(sc + NA_OFF_SPH);
+  hipLaunchKernelGGL(k_compact_count, dim3(nb), dim3(CB), 0, 0, mode, NT, x, y, z, box[0], box[1], s, s + 2 * nd,
+                     s + 4 * nd, nd, r2a, r2b, cnt);
+  DCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(CB), 0, 0, nb, (const int*)cnt, off);
+  DCHK(hipGetLastError());
+  int total = 0;
+  DCHK(hipMemcpy(&total, off + nb, sizeof(int), hipMemcpyDeviceToHost));
+  if (total > cap) return set_error(MPCX_E_ARG, "particle list longer than its array");
+  hipLaunchKernelGGL(k_compact_write, dim3(nb), dim3(CB), 0, 0, mode, NT, x, y, z, box[0], box[1], s, s + 2 * nd,
+                     s + 4 * nd, nd, r2a, r2b, (const int*)off, out);
+  DCHK(hipGetLastError());
+  *added = total;
+  return 0;
+}
+}  // namespace dropin
+}  // namespace mpcx
+
+extern "C" int mpcx_inout_particle_list_noalloc(int32_t NT, const double* box, double cutoffC, double cutoffN,
+                                                double skinrad, const double* xsol, const double* ysol,
+                                                const double* zsol, const double* xdim, const double* ydim,
+                                                const double* zdim, int64_t* outside_particle_num,
+                                                int64_t* outside_particle, int64_t outside_capacity,
+                                                int64_t* inside_particle_num, int64_t* inside_particle,
+                                                int64_t inside_capacity, int32_t numdims) {
+  if (NT < 0 || (long long)NT > NA_MAX_NT)
+    return set_error(MPCX_E_ARG, "NT beyond the no-allocation limit (67,108,864 particles)");
+  if (numdims < 1 || numdims > NA_MAX_DIMERS)
+    return set_error(MPCX_E_ARG, "numdims beyond the no-allocation limit (64 dimers)");
+  if (!box || !xdim || !ydim || !zdim || !outside_particle_num || !inside_particle_num ||
+      (NT > 0 && (!xsol || !ysol || !zsol || !outside_particle || !inside_particle)))
+    return set_error(MPCX_E_ARG, "bad arguments");
+  const long long on = *outside_particle_num, inn = *inside_particle_num;
+  if (on < 0 || inn < 0 || on > outside_capacity || inn > inside_capacity)
+    return set_error(MPCX_E_ARG, "list counts outside their arrays");
+  if (NT == 0) return 0;
+  unsigned char* sc = nullptr;
+  int r = na_scratch(&sc);
+  if (r) return r;
+  const int nd = numdims;
+  double* sph = (double*)(sc + NA_OFF_SPH);   // (x[2nd], y[2nd], z[2nd]), sphere s = k + i*nd
+  DCHK(hipMemcpy(sph, xdim, 2 * nd * sizeof(double), hipMemcpyHostToDevice));
+  DCHK(hipMemcpy(sph + 2 * nd, ydim, 2 * nd * sizeof(double), hipMemcpyHostToDevice));
+  DCHK(hipMemcpy(sph + 4 * nd, zdim, 2 * nd * sizeof(double), hipMemcpyHostToDevice));
+  const double fc = cutoffC + skinrad, fn = cutoffN + skinrad;
+  long long n_in = 0, n_out = 0;
+  if ((r = na_compact(1, NT, xsol, ysol, zsol, box, sc, nd, fc * fc, fn * fn, (long long*)inside_particle + inn,
+                      inside_capacity - inn, &n_in)))
+    return r;
+  if ((r = na_compact(2, NT, xsol, ysol, zsol, box, sc, nd, fc * fc, fn * fn, (long long*)outside_particle + on,
+                      outside_capacity - on, &n_out)))
+    return r;
+  DCHK(hipStreamSynchronize(nullptr));
+  *inside_particle_num = inn + n_in;
+  *outside_particle_num = on + n_out;
   return 0;
 }
 

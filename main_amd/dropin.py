@@ -101,6 +101,28 @@ def neighbour_list(NT, box, cutoff, skinrad, Xsol, Ysol, Zsol, Xdim, Ydim, Zdim,
           "neighbour_list")
 
 
+def inout_particle_list_noalloc(NT, box, cutoffC, cutoffN, skinrad, xsol_dev, ysol_dev, zsol_dev, xdim, ydim, zdim,
+                                outside_particle_num, outside_particle_dev, outside_capacity, inside_particle_num,
+                                inside_particle_dev, inside_capacity, numdims):
+    """inout_particle_list on device arrays, with no device allocation: the
+    solvent coordinates and both lists are device addresses (ints) the caller
+    owns, with room for outside_capacity / inside_capacity entries; the counts
+    (one-element int64 arrays), box and sphere arrays stay on the host.  At
+    most 67,108,864 particles and 64 dimers; calls that overlap on one GPU
+    race (they share the library's fixed scratch array)."""
+    require_gpu()
+    on = _i(outside_particle_num, np.int64, "outside_particle_num")
+    inn = _i(inside_particle_num, np.int64, "inside_particle_num")
+    dims = [np.asfortranarray(a, dtype=np.float64) for a in (xdim, ydim, zdim)]
+    b = np.ascontiguousarray(box, dtype=np.float64)
+    V = C.c_void_p
+    check(lib().mpcx_inout_particle_list_noalloc(I32(NT), ptr(b), F64(cutoffC), F64(cutoffN), F64(skinrad),
+                                                 V(xsol_dev), V(ysol_dev), V(zsol_dev), *[ptr(a) for a in dims],
+                                                 ptr(on), V(outside_particle_dev), I64(outside_capacity), ptr(inn),
+                                                 V(inside_particle_dev), I64(inside_capacity), I32(numdims)),
+          "inout_particle_list_noalloc")
+
+
 def inout_particle_list(NT, box, cutoffC, cutoffN, skinrad, xsol, ysol, zsol, xdim, ydim, zdim,
                         outside_particle_num, outside_particle, inside_particle_num, inside_particle, numdims):
     """Splits the solvent into particles within cutoff+skinrad of some sphere
