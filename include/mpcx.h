@@ -44,6 +44,15 @@ int mpcx_device_synchronize(void);
  * HBM traffic figures of the engine's kernels. */
 int mpcx_probe_copy(int64_t nbytes, int32_t reps, double* ms_per_copy);
 
+/* The throughput mode's device draws, with no counterpart in the reference
+ * (which draws from Mersenne Twister on the host): Philox4x32-10 (Salmon,
+ * Moraes, Dror, Shaw, SC'11) with key = seed, counter = (idx, step, stream),
+ * returned as out[2i], out[2i+1] = the two 53-bit uniforms in [0,1) of index
+ * idx0 + i; stream 0 is the grid shift, 1 the collision axes, 2 the bulk
+ * reaction.  For tests and for callers that want the engine's own stream.
+ * n <= 2^28, stream < 65536. */
+int mpcx_philox_uniform2(uint64_t seed, uint32_t stream, uint64_t step, uint64_t idx0, int64_t n, double* out);
+
 /* Host Mersenne Twister streams (parity mode draws).
  * mpcx_mtmod_*: sgrnd/grnd of collective_code/mtmod.f90:72-144 (state mt[624], mti).
  * mpcx_mt95_*:  init_by_scalar/genrand_real1 of selkov_code/mt95.f90:378-404,828-841

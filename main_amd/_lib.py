@@ -62,6 +62,15 @@ def require_gpu():
         raise MpcxError("no HIP device visible: the mpcx engine has no CPU path")
 
 
+def philox_uniform2(seed: int, stream: int, step: int, idx0: int, n: int) -> np.ndarray:
+    """(n, 2) uniforms of the device Philox stream the engines draw from in
+    throughput mode: row i is the pair of counter index idx0 + i."""
+    out = np.empty((n, 2), np.float64)
+    check(lib().mpcx_philox_uniform2(C.c_uint64(seed), C.c_uint32(stream), C.c_uint64(step), C.c_uint64(idx0),
+                                     C.c_int64(n), ptr(out)), "philox_uniform2")
+    return out
+
+
 # ---------------------------------------------------------------- host streams
 class MtmodStream:
     """grnd() stream of collective_code/mtmod.f90 (state owned by the host)."""

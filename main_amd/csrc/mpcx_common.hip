@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 #include <string>
 #include <stdint.h>
+#include "mpcx_device.h"
 #include "mpcx_host.h"
 #include "../../include/mpcx.h"
 
@@ -33,6 +34,31 @@ __global__ __launch_bounds__(256) void probe_copy_kernel(const double2* __restri
                                                          long long n) {
   for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
     dst[i] = src[i];
+}
+
+// the throughput mode's collision draws, philox_uniform2(seed, stream, step, idx0 + i), i < n
+__global__ void philox_fill_kernel(uint64_t seed, uint32_t stream, uint64_t step, uint64_t idx0, long long n,
+                                   double* out) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    double a, b;
+    mpcx::philox_uniform2(seed, stream, step, idx0 + (uint64_t)i, a, b);
+    out[2 * i] = a;
+    out[2 * i + 1] = b;
+  }
+}
+
+extern "C" int mpcx_philox_uniform2(uint64_t seed, uint32_t stream, uint64_t step, uint64_t idx0, int64_t n,
+                                    double* out) {
+  if (n < 0 || n > (1ll << 28) || (n > 0 && !out) || stream > 0xffffu)
+    return mpcx::set_error(MPCX_E_ARG, "bad arguments (n <= 2^28, stream < 65536)");
+  if (n == 0) return 0;
+  double* d = nullptr;
+  if (hipMalloc(&d, (size_t)n * 16) != hipSuccess) return mpcx::set_error(MPCX_E_HIP, "hipMalloc failed");
+  hipLaunchKernelGGL(philox_fill_kernel, dim3(1024), dim3(256), 0, 0, seed, stream, step, idx0, (long long)n, d);
+  hipError_t r = hipGetLastError();
+  if (r == hipSuccess) r = hipMemcpy(out, d, (size_t)n * 16, hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  return r == hipSuccess ? 0 : mpcx::set_error(MPCX_E_HIP, hipGetErrorString(r));
 }
 
 extern "C" int mpcx_probe_copy(int64_t nbytes, int32_t reps, double* ms_per_copy) {

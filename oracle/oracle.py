@@ -31,6 +31,51 @@ def _load(path):
     return C.CDLL(path)
 
 
+# ------------------------------------------------------------------ Philox
+# Philox4x32-10 of Salmon, Moraes, Dror and Shaw ("Parallel random numbers: as
+# easy as 1, 2, 3", SC'11), restated from the paper in numpy: ten rounds of
+# two 32x32->64-bit products on counter words 0 and 2, the key bumped by the
+# Weyl constants after each round.  The engines draw from it in throughput
+# mode (main_amd/csrc/mpcx_device.h: philox4x32_10, philox_uniform2); the
+# reference itself uses Mersenne Twister and has no counterpart.
+PHILOX_M = (0xD2511F53, 0xCD9E8D57)
+PHILOX_W = (0x9E3779B9, 0xBB67AE85)
+
+
+def philox4x32_10(ctr, key):
+    """ctr: (n, 4) uint32 counters; key: two uint32 words -> (n, 4) uint32."""
+    c = np.array(ctr, dtype=np.uint64).reshape(-1, 4)
+    k0, k1 = np.uint64(key[0]), np.uint64(key[1])
+    m0, m1, mask = np.uint64(PHILOX_M[0]), np.uint64(PHILOX_M[1]), np.uint64(0xFFFFFFFF)
+    for _ in range(10):
+        p0 = m0 * c[:, 0]
+        p1 = m1 * c[:, 2]
+        n = np.empty_like(c)
+        n[:, 0] = (p1 >> np.uint64(32)) ^ c[:, 1] ^ k0
+        n[:, 1] = p1 & mask
+        n[:, 2] = (p0 >> np.uint64(32)) ^ c[:, 3] ^ k1
+        n[:, 3] = p0 & mask
+        c = n
+        k0 = (k0 + np.uint64(PHILOX_W[0])) & mask
+        k1 = (k1 + np.uint64(PHILOX_W[1])) & mask
+    return c.astype(np.uint32)
+
+
+def philox_uniform2(seed: int, stream: int, step: int, idx):
+    """The pair of 53-bit uniforms of counter (idx, step, stream) under key
+    seed, as the device draws them: (n, 2) float64."""
+    idx = np.asarray(idx, dtype=np.uint64).reshape(-1)
+    ctr = np.empty((idx.size, 4), np.uint64)
+    ctr[:, 0] = idx & np.uint64(0xFFFFFFFF)
+    ctr[:, 1] = idx >> np.uint64(32)
+    ctr[:, 2] = np.uint64(step & 0xFFFFFFFF)
+    ctr[:, 3] = np.uint64(((step >> 32) & 0xFFFF) | (stream << 16))
+    r = philox4x32_10(ctr, (seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF)).astype(np.uint64)
+    a = (r[:, 0] << np.uint64(21)) ^ (r[:, 1] >> np.uint64(11))
+    b = (r[:, 2] << np.uint64(21)) ^ (r[:, 3] >> np.uint64(11))
+    return np.stack([a, b], axis=1).astype(np.float64) * 2.0 ** -53
+
+
 def oracle_lib():
     lib = _load(os.path.join(HERE, "liboracle.so"))
     if lib is None:
