@@ -38,6 +38,11 @@ const char* mpcx_last_error(void);
 int mpcx_device_count(void);
 /* hipDeviceSynchronize on the current device */
 int mpcx_device_synchronize(void);
+/* No counterpart in the reference: destroys every engine, batch and network
+ * still live (batches first) and drains the device, so no stream, event,
+ * graph or buffer outlives the HIP runtime at process exit.  Handles are
+ * invalid afterwards.  The Python front end calls it at interpreter exit. */
+int mpcx_shutdown(void);
 /* Measurement aid with no counterpart in the reference: copies nbytes between
  * two device buffers reps times (after one warm-up copy) and returns the mean
  * time of one copy.  Its hardware fetch/write counter readings calibrate the

@@ -6,14 +6,36 @@ __graft_entry__ as g; g.build()"`` (or ``make -C main_amd``).
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmpcx.so")
 _lib = None
+# objects holding library handles: batches close before the engines they run
+_live = (weakref.WeakSet(), weakref.WeakSet())
+
+
+def track(obj, first: bool = False):
+    """Registers an object with a close() method to be closed at interpreter
+    exit, before the HIP runtime tears down (first: before the others)."""
+    _live[0 if first else 1].add(obj)
+
+
+@atexit.register
+def _close_all():
+    for group in _live:
+        for obj in list(group):
+            try:
+                obj.close()
+            except Exception:
+                pass
+    if _lib is not None:
+        _lib.mpcx_shutdown()
 
 P = C.c_void_p
 

@@ -10,7 +10,7 @@ from __future__ import annotations
 
 import ctypes as C
 
-from ._lib import check, lib
+from ._lib import check, lib, track
 from .collective import CollectiveEngine
 
 
@@ -21,6 +21,7 @@ class CollectiveBatch:
         self.h = C.c_void_p()
         check(lib().mpcx_cob_create(C.byref(self.h), C.c_int32(len(engines)), arr, C.c_int32(tile)),
               "mpcx_cob_create")
+        track(self, first=True)
         self.tile = tile
         self.cell_tile = 256
 

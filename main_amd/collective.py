@@ -15,7 +15,7 @@ import os
 
 import numpy as np
 
-from ._lib import MtmodStream, check, lib, ptr, require_gpu
+from ._lib import MtmodStream, check, lib, ptr, require_gpu, track
 
 MAXM = 8            # force slots stored per inside entry by the engine (one per list of up to four dimers)
 MAX_INSIDE = 16384  # inside-list capacity of the near engine
@@ -87,6 +87,7 @@ class CollectiveEngine:
         self.cfg = make_config(p, rng_mode, seed, sphere_guard)
         self.h = C.c_void_p()
         check(lib().mpcx_co_create(C.byref(self.h), C.byref(self.cfg)), "mpcx_co_create")
+        track(self)
         self.rng_mode = rng_mode
         if near_mode is not None:
             check(lib().mpcx_co_set_near_mode(self.h, C.c_int32(int(near_mode))), "mpcx_co_set_near_mode")

@@ -3204,6 +3204,7 @@ extern "C" int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg) {
     HIPCHK(hipMemset(D.rec_msd, 0, (size_t)NT * 8));
   }
   *out = e;
+  handle_add(H_CO, e);
   return 0;
 }
 
@@ -3265,6 +3266,7 @@ extern "C" int mpcx_co_set_near_mode(mpcx_co* e, int32_t mode) {
 
 extern "C" int mpcx_co_destroy(mpcx_co* e) {
   if (!e) return 0;
+  handle_del(H_CO, e);
   if (e->solo) { (void)mpcx_cob_destroy(e->solo); e->solo = nullptr; }
   if (e->st) (void)hipStreamSynchronize(e->st);
   if (e->st2) (void)hipStreamSynchronize(e->st2);
@@ -5402,6 +5404,7 @@ static void cob_free_graph(mpcx_cob* b) {
 
 extern "C" int mpcx_cob_destroy(mpcx_cob* b) {
   if (!b) return 0;
+  handle_del(H_COB, b);
   if (b->st) (void)hipStreamSynchronize(b->st);
   if (b->st2) (void)hipStreamSynchronize(b->st2);
   cob_free_graph(b);
@@ -5486,7 +5489,9 @@ static int cob_make(mpcx_cob** out, int32_t R, mpcx_co* const* engines, int32_t 
 }
 
 extern "C" int mpcx_cob_create(mpcx_cob** out, int32_t R, mpcx_co* const* engines, int32_t tile) {
-  return cob_make(out, R, engines, tile, false);
+  const int rc = cob_make(out, R, engines, tile, false);
+  if (!rc) handle_add(H_COB, *out);
+  return rc;
 }
 
 // checks that every replica sits at the same collision step, sets the

@@ -1,6 +1,9 @@
 // Error reporting and device query of the C ABI.
 #include <hip/hip_runtime.h>
+#include <mutex>
+#include <set>
 #include <string>
+#include <vector>
 #include <stdint.h>
 #include "mpcx_device.h"
 #include "mpcx_host.h"
@@ -12,7 +15,36 @@ int set_error(int code, const char* msg) {
   g_last_error = msg ? msg : "";
   return code;
 }
+static std::mutex g_handles_mu;
+static std::set<void*> g_handles[H_KINDS];
+void handle_add(int kind, void* p) {
+  std::lock_guard<std::mutex> lock(g_handles_mu);
+  g_handles[kind].insert(p);
+}
+void handle_del(int kind, void* p) {
+  std::lock_guard<std::mutex> lock(g_handles_mu);
+  g_handles[kind].erase(p);
+}
 }  // namespace mpcx
+
+// Every handle still live, destroyed in dependency order (batches before the
+// engines they run), then the device drained: the streams, events, graphs and
+// buffers are gone before the HIP runtime (or a profiler attached to it) tears
+// down at process exit.
+extern "C" int mpcx_shutdown(void) {
+  std::vector<void*> live[mpcx::H_KINDS];
+  {
+    std::lock_guard<std::mutex> lock(mpcx::g_handles_mu);
+    for (int k = 0; k < mpcx::H_KINDS; k++) live[k].assign(mpcx::g_handles[k].begin(), mpcx::g_handles[k].end());
+  }
+  for (void* p : live[mpcx::H_COB]) (void)mpcx_cob_destroy((mpcx_cob*)p);
+  for (void* p : live[mpcx::H_CO]) (void)mpcx_co_destroy((mpcx_co*)p);
+  for (void* p : live[mpcx::H_SK]) (void)mpcx_sk_destroy((mpcx_sk*)p);
+  for (void* p : live[mpcx::H_MLP]) (void)mpcx_mlp_destroy((mpcx_mlp*)p);
+  int n = 0;
+  if (hipGetDeviceCount(&n) == hipSuccess && n > 0) (void)hipDeviceSynchronize();
+  return 0;
+}
 
 extern "C" const char* mpcx_last_error(void) { return mpcx::g_last_error.c_str(); }
 

@@ -8,6 +8,10 @@ struct mpcx_sk;
 namespace mpcx {
 // records the message of the last failure (per calling thread) and returns code
 int set_error(int code, const char* msg);
+// live handles of the C ABI, released together by mpcx_shutdown
+enum HandleKind { H_COB = 0, H_CO = 1, H_SK = 2, H_MLP = 3, H_KINDS = 4 };
+void handle_add(int kind, void* p);
+void handle_del(int kind, void* p);
 // x**n with an integer exponent, evaluated by repeated multiplication; exact
 // for the integer and dyadic radii the simulators use
 inline double ipow(double x, int n) {

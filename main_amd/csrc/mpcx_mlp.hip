@@ -270,6 +270,7 @@ static void refresh_bf16(mpcx_mlp* e) {
 
 extern "C" int mpcx_mlp_destroy(mpcx_mlp* e) {
   if (!e) return 0;
+  mpcx::handle_del(mpcx::H_MLP, e);
   if (e->st) (void)hipStreamSynchronize(e->st);
   auto fr = [](void* q) { if (q) (void)hipFree(q); };
   fr(e->p); fr(e->g); fr(e->m); fr(e->v); fr(e->pb);
@@ -323,6 +324,7 @@ extern "C" int mpcx_mlp_create(mpcx_mlp** out, int32_t n_in, int32_t n_hidden, i
   refresh_bf16(e);
   MLCHK(hipStreamSynchronize(e->st));
   *out = e;
+  mpcx::handle_add(mpcx::H_MLP, e);
   return 0;
 }
 

@@ -1108,6 +1108,7 @@ static int sk_check_error(mpcx_sk* e) {
 
 extern "C" int mpcx_sk_destroy(mpcx_sk* e) {
   if (!e) return 0;
+  handle_del(H_SK, e);
   if (e->st) (void)hipStreamSynchronize(e->st);
   if (e->st2) (void)hipStreamSynchronize(e->st2);
   for (void* p : e->allocs)
@@ -1234,6 +1235,7 @@ extern "C" int mpcx_sk_create(mpcx_sk** out, const mpcx_sk_config* cfg) {
   SKCHK(hipMemset(e->alt_bits, 0, (e->nwords + 2) * 4));   // words past the last particle stay zero in both
   SKCHK(hipMemset(D.cflag, 0, NS));
   *out = e;
+  handle_add(H_SK, e);
   return 0;
 }
 

@@ -13,7 +13,7 @@ import math
 
 import numpy as np
 
-from ._lib import Mt95Stream, check, lib, ptr, require_gpu
+from ._lib import Mt95Stream, check, lib, ptr, require_gpu, track
 
 
 class SkConfig(C.Structure):
@@ -52,6 +52,7 @@ class SelkovEngine:
         self.cfg = make_config(p, rng_mode, seed, sort_step, sphere_guard)
         self.h = C.c_void_p()
         check(lib().mpcx_sk_create(C.byref(self.h), C.byref(self.cfg)), "mpcx_sk_create")
+        track(self)
         self.rng_mode = rng_mode
         self.space = int(self.cfg.space)
 

@@ -13,7 +13,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import check, lib, ptr, require_gpu
+from ._lib import check, lib, ptr, require_gpu, track
 
 
 def gemm_bf16(A: np.ndarray, B: np.ndarray, reps: int = 1):
@@ -37,6 +37,7 @@ class Surrogate:
         self.h = C.c_void_p()
         check(lib().mpcx_mlp_create(C.byref(self.h), C.c_int32(n_in), C.c_int32(n_hidden), C.c_int32(n_layers),
                                     C.c_int32(n_out), C.c_uint64(seed)), "mpcx_mlp_create")
+        track(self)
         lib().mpcx_mlp_num_params.restype = C.c_int64
         self.n_in, self.n_hidden, self.n_layers, self.n_out = n_in, n_hidden, n_layers, n_out
         self.nparam = int(lib().mpcx_mlp_num_params(self.h))

@@ -81,3 +81,25 @@ def test_reaction_table_matches_reference_formula():
     t = reaction_table(1e-3, 0.5)
     assert t[0] == 0.0
     assert t[10] == 1.0 - np.exp(-(1e-3 * 10.0 * 0.5))
+
+
+def test_exit_hook_closes_batches_before_engines(monkeypatch):
+    """At interpreter exit every object holding a library handle is closed,
+    batches before the engines they run, before the library's own
+    mpcx_shutdown and the HIP runtime's teardown."""
+    import main_amd._lib as L
+    order = []
+
+    class Obj:
+        def __init__(self, name):
+            self.name = name
+
+        def close(self):
+            order.append(self.name)
+
+    objs = [Obj("engine"), Obj("batch")]
+    L.track(objs[0])
+    L.track(objs[1], first=True)
+    monkeypatch.setattr(L, "_lib", None)
+    L._close_all()
+    assert order == ["batch", "engine"]
