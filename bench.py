@@ -54,30 +54,69 @@ def headline_params(L=100):
                                skinrad=0.3, MDt=0.01, MPCt=0.5, sort_step=500, k1=1e-3, react_box_type=7)
 
 
-def cpu_baseline(L=100, budget_s=15.0):
-    """The reference's own Fortran modules (oracle/_ref, flang, one thread) on a
-    bounded sample of the same workload: whole collision intervals of the same
-    100^3 system, as many as fit the time budget.  Falls back to the oracle's
-    C restatement when the reference build is absent."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_leg(L, budget_s, threads):
+    """Whole collision intervals of the L^3 system on the CPU, as many as fit
+    the budget: the flang-built reference modules (threads > 1: their OpenMP
+    build, one Mersenne-Twister stream per thread as the reference keeps
+    them), else the oracle's C restatement."""
     from oracle import oracle as O
     from main_amd import systems as S
-    use_ref = O.ref_lib("collective") is not None
+    kind = "omp" if threads > 1 else True
+    use_ref = O.ref_lib("collective_omp" if threads > 1 else "collective") is not None
+    if not use_ref and threads > 1:
+        return None
     ps = headline_params(L)
     ps["sort_step"] = 1000000   # no re-sort inside the sample
     ss = S.collective_initial_state(ps, seed=11)
-    ss["mt"], ss["mti"] = O.mtmod_seed(4357787)
-    O.co_run(ss, 0, do_setup=1, use_ref=use_ref)
+    if threads > 1:
+        seeds = [O.mtmod_seed(4357787 + t) for t in range(threads)]
+        ss["mt_threads"] = np.stack([m for m, _ in seeds]).astype(np.uint32)
+        ss["mti_threads"] = np.array([i for _, i in seeds], np.int32)
+    else:
+        ss["mt"], ss["mti"] = O.mtmod_seed(4357787)
+    mode = kind if use_ref else False
+    O.co_run(ss, 0, do_setup=1, use_ref=mode)
     n, dt = 0, 0.0
     while dt < budget_s and n < 20:
         t0 = time.perf_counter()
-        O.co_run(ss, ps["MPCtstep"], use_ref=use_ref)
+        O.co_run(ss, ps["MPCtstep"], use_ref=mode)
         dt += time.perf_counter() - t0
         n += 1
-    kind = "reference" if use_ref else "port"
-    what = "flang-built reference modules" if use_ref else "C restatement of the reference"
-    return {"value": ps["NT"] * n / dt, "unit": "particle-updates/s", "cores": 1, "kind": kind,
-            "sample": f"{what}, {n} collision intervals of the {L}^3 system ({ps['NT']} particles), "
-                      f"{dt:.1f} s on one host core"}
+    what = ("flang-built reference modules" + (f", OpenMP build, {threads} threads" if threads > 1 else ", one thread")
+            if use_ref else "C restatement of the reference, one thread")
+    return {"value": ps["NT"] * n / dt, "unit": "particle-updates/s", "cores": threads,
+            "kind": "reference" if use_ref else "port",
+            "sample": f"{what}: {n} collision intervals of the {L}^3 system ({ps['NT']} particles), {dt:.1f} s"}
+
+
+def cpu_baseline(L=100, budget_s=15.0):
+    """The reference's own Fortran modules on a bounded sample of the same
+    workload, on this host's cores: at the core share (OMP_NUM_THREADS, else
+    every core; the reference's jobs run OpenMP, collective_code/Run.sh:10)
+    and at one thread.  Falls back to the oracle's C restatement on one thread
+    when the reference build is absent."""
+    ncpu = os.cpu_count() or 1
+    threads = max(1, min(ncpu, int(os.environ.get("OMP_NUM_THREADS", ncpu) or ncpu)))
+    one = _cpu_leg(L, budget_s, 1)
+    many = _cpu_leg(L, budget_s, threads) if threads > 1 else None
+    out = dict(many or one)
+    out["cpu"] = cpu_model()
+    out["host_cpus"] = ncpu
+    if many:
+        out["single_thread"] = one
+        out["thread_speedup"] = many["value"] / one["value"]
+    return out
 
 
 def batched(R=256, steps=20, warmup=4, seed=4357787, tiles=(1024, 2048, 4096, 8192)):

@@ -87,7 +87,8 @@ def oracle_lib():
 
 
 def ref_lib(kind: str):
-    """The flang-built reference ('collective' or 'selkov'), or None if absent."""
+    """The flang-built reference ('collective', 'selkov', or 'collective_omp', the
+    collective modules built with their OpenMP directives on), or None if absent."""
     return _load(os.path.join(HERE, "_ref", f"libref_{kind}.so"))
 
 
@@ -184,15 +185,22 @@ def co_run(sysd: dict, nsteps: int, do_setup: int = 0, use_ref: bool = False, to
     hilbert.txt from the working directory); otherwise the C restatement runs
     with the tour given as 1-based cell numbers.
     """
-    lib = ref_lib("collective") if use_ref else oracle_lib()
+    threaded = use_ref == "omp"
+    lib = ref_lib("collective_omp" if threaded else "collective") if use_ref else oracle_lib()
     if lib is None:
         raise RuntimeError("reference library not built")
     for k, v in co_alloc_work(sysd).items():
         if k not in sysd or sysd[k] is None or np.shape(sysd[k]) != np.shape(v):
             sysd[k] = v
     ip, rp = co_blocks(sysd)
-    mt = np.ascontiguousarray(sysd["mt"], dtype=np.uint32)
-    mti = np.array([sysd["mti"]], dtype=np.int32)
+    if threaded:   # the OpenMP build: one stream per thread, mt(624, nthreads) column-major
+        mt = np.ascontiguousarray(sysd["mt_threads"], dtype=np.uint32)
+        mti = np.ascontiguousarray(sysd["mti_threads"], dtype=np.int32)
+        lib.ref_set_mt_threads(C.c_int32(mti.size))
+        lib.omp_set_num_threads(C.c_int(mti.size))
+    else:
+        mt = np.ascontiguousarray(sysd["mt"], dtype=np.uint32)
+        mti = np.array([sysd["mti"]], dtype=np.int32)
     rb = np.ascontiguousarray(sysd["react_box"], dtype=np.int32)
     args = [_ptr(ip), _ptr(rp), C.c_int64(nsteps), C.c_int32(do_setup)]
     args += [_ptr(sysd[k]) for k in CO_ARRAYS]
@@ -206,7 +214,10 @@ def co_run(sysd: dict, nsteps: int, do_setup: int = 0, use_ref: bool = False, to
         t = np.ascontiguousarray(tour if tour is not None else np.arange(1, sysd["vol"] + 1), dtype=np.int64)
         lib.or_co_run.restype = C.c_int
         rc = lib.or_co_run(*args, _ptr(t))
-    sysd["mt"], sysd["mti"] = mt, int(mti[0])
+    if threaded:
+        sysd["mt_threads"], sysd["mti_threads"] = mt, mti
+    else:
+        sysd["mt"], sysd["mti"] = mt, int(mti[0])
     (sysd["cstep"], sysd["time_count"], sysd["time_penetrate"]) = (int(ip[5]), int(ip[6]), int(ip[7]))
     sysd["out_num"], sysd["in_num"] = int(ip[10]), int(ip[11])
     sysd["tpotential"] = float(rp[30])

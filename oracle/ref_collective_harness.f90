@@ -27,7 +27,16 @@ use sort_hilbert_mod
 use forces_mod
 implicit none
 integer, parameter :: MAXS = 100
+! Mersenne-Twister streams the modules draw from: one per OpenMP thread
+! (mt(:,thread), mpc_rotation_mod.f95:186-189); 1 unless the library was
+! compiled with -fopenmp and ref_set_mt_threads named the thread count
+integer :: nthr_mt = 1
 contains
+
+subroutine ref_set_mt_threads(n) bind(C, name="ref_set_mt_threads")
+  integer(c_int32_t), value :: n
+  nthr_mt = n
+end subroutine
 
 subroutine ref_mtmod_seed(seed, mt, mti) bind(C, name="ref_mtmod_seed")
   integer(c_int32_t), value :: seed
@@ -138,7 +147,7 @@ subroutine ref_co_run(ip, rp, nsteps, do_setup, flag, x, y, z, vx, vy, vz, xd, y
   real(c_double) :: vxcm(ip(13)), vycm(ip(13)), vzcm(ip(13)), tcs(ip(13))
   real(c_double) :: uvx(ip(13)), uvy(ip(13)), uvz(ip(13))
   integer(c_int64_t) :: ncn(ip(1))
-  integer(c_int32_t) :: mt(624,1), mti(1)
+  integer(c_int32_t) :: mt(624,nthr_mt), mti(nthr_mt)
 
   integer(c_int32_t) :: NT, cstep, MPCtstep
   integer :: numdims, sort_step, react_dir, time_count, time_penetrate, k
