@@ -34,6 +34,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 PROBE_BYTES = 256 << 20          # copy probe: reads and writes this many bytes per launch
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_100cube.txt")
+TRAFFIC_SPLIT_FILE = os.path.join(ROOT, "profiles", "traffic_split_batched_256.txt")
 
 # Algorithmic HBM bytes per particle of each interval kernel (FP64 state):
 #  co_bulk   reads x,y,z,vx,vy,vz (48), flag (4), status (1); writes x,y,z (24),
@@ -331,6 +332,9 @@ def split_batched(R=256, steps=20, warmup=4, seed=4357787, fp64_value=None):
     n = info["particles"]
     ms = dt * 1000.0 / steps
     value = n * steps / dt
+    kernels = kernel_rooflines(kms, ALGO_BYTES_SPLIT, n)
+    for k, v in kernels.items():
+        v["traffic"] = read_traffic(k, TRAFFIC_SPLIT_FILE)
     print(f"# split batched: {ms:.3f} ms/interval, {value:.3e} particle-updates/s, kernels {kms}", file=sys.stderr,
           flush=True)
     out = {"replicas": R, "particles": n, "collision_cells": info["cells"], "ms_per_interval": ms,
@@ -338,7 +342,7 @@ def split_batched(R=256, steps=20, warmup=4, seed=4357787, fp64_value=None):
            "tile": batch.tile, "cells_per_workgroup": batch.cell_tile,
            "autotune_ms_per_interval": {f"{k[0]}x{k[1]}": v for k, v in times.items()},
            "roofline_step": step_roofline(SPLIT_STEP_BYTES, n, ms), "kernel_ms": kms,
-           "kernels": kernel_rooflines(kms, ALGO_BYTES_SPLIT, n), "rng": "device Philox4x32-10", "graph": True,
+           "kernels": kernels, "rng": "device Philox4x32-10", "graph": True,
            "dtype": "f32 solvent (cell + offsets), f64 near set and dimers"}
     if fp64_value:
         out["vs_fp64_headline"] = value / fp64_value
@@ -370,11 +374,12 @@ def step_traffic(sort_every=10):
     return sum(per.get(k, 0.0) for k in STEP_KERNELS) + sum(per.get(k, 0.0) for k in SORT_KERNELS) / sort_every
 
 
-def read_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed counter summary."""
-    if not os.path.exists(TRAFFIC_FILE):
+def read_traffic(kernel, path=TRAFFIC_FILE):
+    """Bytes per launch of `kernel` between L2 and memory from a committed
+    counter summary (an upper bound on its HBM bytes)."""
+    if not os.path.exists(path):
         return None
-    with open(TRAFFIC_FILE) as f:
+    with open(path) as f:
         for line in f:
             m = re.match(r"traffic_bytes_per_launch\s+(\S+)\s+(\S+)", line)
             if m and m.group(1) == kernel:
@@ -382,10 +387,22 @@ def read_traffic(kernel):
     return None
 
 
-def summarize_pmc(fetch_dir, write_dir):
-    """Mean FETCH_SIZE / WRITE_SIZE (KiB) per dispatch of each kernel, scaled by
-    the readings of the copy probe (PROBE_BYTES read and written per launch)."""
-    def load(d, counter):
+# Counter passes of the traffic figures (one rocprofv3 --pmc run each; at most
+# four TCC counters fit one pass): L2 read requests to the memory fabric by
+# size, then write requests.  Bytes = 32 R32 + 64 R64 + 128 R128 and
+# 64 W64 + 32 (W - W64).  They count every request that leaves L2 for memory,
+# including those the Infinity Cache (MALL) then serves: no counter exposed
+# here tells those from HBM accesses, so a figure is an upper bound on the
+# kernel's HBM bytes (it equals them once the data exceeds the 256 MB MALL,
+# as the copy probe's 256 MiB buffers do).
+PMC_READ = ("TCC_EA0_RDREQ_32B_sum", "TCC_EA0_RDREQ_64B_sum", "TCC_EA0_RDREQ_128B_sum")
+PMC_WRITE = ("TCC_EA0_WRREQ_sum", "TCC_EA0_WRREQ_64B_sum")
+
+
+def summarize_pmc(read_dir, write_dir):
+    """Bytes per dispatch of each kernel between L2 and the memory fabric from
+    the two counter passes, checked against the copy probe's known bytes."""
+    def load(d, counters):
         files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
         if not files:
             raise SystemExit(f"no counter_collection.csv under {d}")
@@ -393,30 +410,37 @@ def summarize_pmc(fetch_dir, write_dir):
         for fn in files:
             with open(fn) as f:
                 for row in csv.DictReader(f):
-                    if row.get("Counter_Name") != counter:
+                    c = row.get("Counter_Name")
+                    if c not in counters:
                         continue
-                    name = row["Kernel_Name"].split("(")[0].strip()
-                    name = re.sub(r"^void\s+", "", name)
-                    per.setdefault(name, []).append(float(row["Counter_Value"]))
-        return {k: float(np.mean(v)) for k, v in per.items()}, {k: len(v) for k, v in per.items()}
-    fetch, nf = load(fetch_dir, "FETCH_SIZE")
-    write, nw = load(write_dir, "WRITE_SIZE")
-    probe = [k for k in fetch if "probe_copy_kernel" in k]
-    if not probe:
-        raise SystemExit("the copy probe kernel is missing from the counter runs (run bench.py --probe-copy)")
-    pk = probe[0]
-    kib = PROBE_BYTES / 1024.0
-    f_scale = kib / fetch[pk]
-    w_scale = kib / write[pk]
-    print(f"# copy probe {PROBE_BYTES} B: FETCH_SIZE {fetch[pk]:.0f} KiB, WRITE_SIZE {write.get(pk, float('nan')):.0f} KiB")
-    print(f"# calibration: fetch factor {f_scale:.4f}, write factor {w_scale:.4f}")
-    print(f"# {'kernel':40s} {'dispatches':>10s} {'FETCH KiB':>12s} {'WRITE KiB':>12s} {'HBM bytes/launch':>16s}")
-    for k in sorted(fetch, key=lambda k: -fetch[k] - write.get(k, 0.0)):
-        fb = fetch[k] * f_scale * 1024.0
-        wb = write.get(k, 0.0) * w_scale * 1024.0
+                    name = re.sub(r"^void\s+", "", row["Kernel_Name"].split("(")[0].strip())
+                    per.setdefault(name, {}).setdefault(c, []).append(float(row["Counter_Value"]))
+        return {k: {c: float(np.mean(v)) for c, v in d.items()} for k, d in per.items()}, \
+               {k: max(len(v) for v in d.values()) for k, d in per.items()}
+    rd, nr = load(read_dir, PMC_READ)
+    wr, _ = load(write_dir, PMC_WRITE)
+
+    def rbytes(k):
+        c = rd.get(k, {})
+        return 32 * c.get(PMC_READ[0], 0.0) + 64 * c.get(PMC_READ[1], 0.0) + 128 * c.get(PMC_READ[2], 0.0)
+
+    def wbytes(k):
+        c = wr.get(k, {})
+        w, w64 = c.get(PMC_WRITE[0], 0.0), c.get(PMC_WRITE[1], 0.0)
+        return 64 * w64 + 32 * (w - w64)
+    probe = [k for k in rd if "probe_copy_kernel" in k]
+    if probe:
+        pk = probe[0]
+        print(f"# copy probe, {PROBE_BYTES} B read and written per launch: counters give "
+              f"{rbytes(pk):.4g} B read ({rbytes(pk) / PROBE_BYTES:.3f}x), "
+              f"{wbytes(pk):.4g} B written ({wbytes(pk) / PROBE_BYTES:.3f}x)")
+    print(f"# {'kernel':44s} {'dispatches':>10s} {'read B':>12s} {'written B':>12s} {'bytes/launch':>13s}")
+    for k in sorted(rd, key=lambda k: -(rbytes(k) + wbytes(k))):
         short = re.sub(r"^mpcx::", "", k)
-        print(f"# {short[:40]:40s} {nf[k]:10d} {fetch[k]:12.0f} {write.get(k, 0.0):12.0f} {fb + wb:16.4g}")
-        print(f"traffic_bytes_per_launch {short.split('<')[0]} {fb + wb:.6g}")
+        short = re.sub(r"^(ne_solo|ne_batch)::", "", short)
+        rb, wb = rbytes(k), wbytes(k)
+        print(f"# {short[:44]:44s} {nr[k]:10d} {rb:12.4g} {wb:12.4g} {rb + wb:13.4g}")
+        print(f"traffic_bytes_per_launch {short.split('<')[0]} {rb + wb:.6g}")
 
 
 def main():
@@ -427,7 +451,8 @@ def main():
     ap.add_argument("--size", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--probe-copy", action="store_true", help="also run the copy probe (counter calibration)")
-    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("READ_DIR", "WRITE_DIR"),
+                    help="counter CSVs of the two passes (PMC_READ, PMC_WRITE) -> bytes per launch")
     ap.add_argument("--batched", type=int, default=256,
                     help="replicas of the batched ensemble measured beside the headline (0: skip)")
     ap.add_argument("--batched-only", action="store_true", help="measure only the replica ensemble")
@@ -449,6 +474,10 @@ def main():
     from main_amd._lib import check, device_synchronize, lib
     from main_amd.collective import CollectiveEngine
     from main_amd.tour import cell_tour
+    if args.probe_copy:   # the counter runs' reference kernel of known bytes
+        ms = C.c_double(0)
+        check(lib().mpcx_probe_copy(C.c_int64(PROBE_BYTES), C.c_int32(3), C.byref(ms)), "mpcx_probe_copy")
+        print(f"# copy probe: {2 * PROBE_BYTES / (ms.value * 1e-3) / 1e9:.0f} GB/s", file=sys.stderr, flush=True)
 
     if args.batched_only:
         print(json.dumps({"metric": "replicas/hour (2000-collision runs)", **batched(args.batched, args.steps,
@@ -467,11 +496,6 @@ def main():
         out["selkov_50cube"] = selkov(args.steps, args.warmup)
         print(json.dumps({"metric": "particle-updates/s (MD+MPC), many dimers", **out}), flush=True)
         return
-    if args.probe_copy:
-        ms = C.c_double(0)
-        check(lib().mpcx_probe_copy(C.c_int64(PROBE_BYTES), C.c_int32(3), C.byref(ms)), "mpcx_probe_copy")
-        print(f"# copy probe: {2 * PROBE_BYTES / (ms.value * 1e-3) / 1e9:.0f} GB/s", file=sys.stderr, flush=True)
-
     p = headline_params(args.size)
     t0 = time.time()
     s = S.collective_initial_state(p, seed=4357787)

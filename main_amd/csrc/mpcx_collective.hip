@@ -536,23 +536,11 @@ __device__ __attribute__((noinline)) double clu_exchange_max(CluInfo* ci, int ci
 }
 
 
-// The near engine in two variants.  ne_solo stages up to 2304 inside entries
-// in LDS (150 KB: one engine per CU, for a lone system whose engine runs
-// beside light work).  ne_batch stages 1664 (110 KB), so that one 46 KB
-// workgroup of the split far flight fits on the CU beside each of a batch's
-// engines instead of waiting for them; inside entries beyond the LDS share
-// live in the HBM slot arrays (the same arithmetic, so the same results).
-namespace ne_solo {
+// The near engine (its LDS state and kernels): up to LCAP inside entries are
+// staged in LDS (150 KB with the rest of its state: one engine per CU); the
+// entries beyond them live in the HBM slot arrays (the same arithmetic).
 constexpr int LCAP = 2304;
 #include "mpcx_near_engine.inc"
-}  // namespace ne_solo
-namespace ne_batch {
-constexpr int LCAP = 1664;
-#include "mpcx_near_engine.inc"
-}  // namespace ne_batch
-using ne_solo::co_near_engine;
-using ne_solo::co_near_engine_c;
-using ne_batch::co_near_engine_b;
 
 
 // ------------------------------------------------------------- clusters
