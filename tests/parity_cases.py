@@ -5,7 +5,10 @@ restatement of the reference, pinned bit-exact to the reference's own Fortran
 modules) and through the HIP engine with the same Mersenne-Twister draws, and
 compares the results:
   * integer state (flags, shifted-cell index of every particle, census
-    counts, collision-cell counts, inside list): bit-exact;
+    counts, collision-cell counts, inside list, neighbour-list counts and
+    members): bit-exact;
+  * pair forces and potentials of the list members: within 1e-12 of the
+    largest member's;
   * positions and velocities: |gpu - ref| <= 1e-12 * max(|ref|, scale)
     (scale 1 for positions, the thermal speed for velocities) -- the only
     deliberate reordering is the tree sum of the dimer forces.
@@ -99,6 +102,23 @@ def collective_parity_case(ncoll=1, seed=3, p=None, flags_mix=False, sort_at_set
     inside_ref = np.sort(ref["il"][: ref["in_num"]])
     res["inside"] = (0 if np.array_equal(lists["in_list"], inside_ref) else 1, 0.0)
     res["nbnum"] = (int(np.sum(lists["nbnum"] != ref["nbnum"])), 0.0)
+    # the neighbour lists themselves (ascending particle numbers) and the pair
+    # forces and potentials of their members at the last force evaluation:
+    # the valid prefix of each list (the reference leaves earlier builds'
+    # entries beyond it)
+    nb_bad, fp_res = 0, {k: (0, 0.0) for k in ("fpx", "fpy", "fpz", "pot")}
+    for k in range(p["numdims"]):
+        for i in range(2):
+            n = int(ref["nbnum"][k, i])
+            if n != int(lists["nbnum"][k, i]) or not np.array_equal(lists["nb"][k, i, :n], ref["nb"][k, i, :n]):
+                nb_bad += 1
+                continue
+            for f in fp_res:
+                scale = float(np.max(np.abs(ref[f][k, i, :n]))) if n else 1.0
+                b, e = close(lists[f][k, i, :n], ref[f][k, i, :n], scale)
+                fp_res[f] = (fp_res[f][0] + b, max(fp_res[f][1], e))
+    res["nb"] = (nb_bad, 0.0)
+    res.update(fp_res)
     sc = [(g["cstep"], ref["cstep"]), (g["time_count"], ref["time_count"]),
           (g["time_penetrate"], ref["time_penetrate"])]
     res["scalars"] = (int(sum(a != b for a, b in sc)), 0.0)

@@ -133,15 +133,20 @@ def test_engine_integer_state_equals_reference_fixture(golden, case, near_mode, 
         eng.run(ncoll, draws)
         g = eng.get_state()
         cells = eng.get_cells()
+        lists = eng.get_lists()
     finally:
         eng.close()
     got = {"flag": _h(g["flag"].astype(np.int32)), "ncn": _h(cells["ncn"].astype(np.int64)),
            "tcs": _h(cells["tcs"]), "tA": _h(cells["tA"][: p["vol"]]), "tB": _h(cells["tB"][: p["vol"]]),
-           "tS": _h(cells["tS"][: p["vol"]])}
+           "tS": _h(cells["tS"][: p["vol"]]), "nbnum": _h(lists["nbnum"].astype(np.int64))}
     bad = [k for k, v in got.items() if v != rec["arrays"][k]]
     assert not bad, f"{case}: differs from the reference run in {bad}"
     for k in ("cstep", "time_count", "time_penetrate"):
         assert g[k] == rec["scalars"][k], k
+    # the inside/outside split (the fixture's il/ol arrays hold earlier
+    # builds' entries past the counts, so the counts are what compare)
+    assert lists["in_list"].size == rec["scalars"]["in_num"]
+    assert lists["out_list"].size == rec["scalars"]["out_num"]
 
 
 def _full_size_engine(L, seed):
