@@ -8,7 +8,223 @@ module mpcx_c
   implicit none
   private :: c_strlen
 
+  ! bind(C) mirrors of the engines' configuration structs, component for
+  ! component in the order and kinds of include/mpcx.h (neither has padding)
+  type, bind(C) :: mpcx_co_config
+    integer(c_int64_t) :: NT, numdims, vol, mpcnumcell, rxnnumcell, max_neighbour
+    integer(c_int32_t) :: MPCtstep, sort_step, react_dir, rng_mode
+    integer(c_int64_t) :: seed                  ! uint64 Philox key
+    real(c_double) :: box(3)
+    real(c_double) :: MDt, Ma, Mb, Ms, Mc, Mn, Dcn, cutoff, Rc, Rn
+    real(c_double) :: EaC, EbC, EsC, EaN, EbN, EsN, Rw, Ew, Rm_delta, Em, RPab, RPba
+    real(c_double) :: skinrad, rotang, a0, MPCt, k1
+    real(c_double) :: sinrot, cosrot, wall_w6
+    real(c_double) :: reaction_p(0:100)
+    real(c_double) :: sphere_guard
+  end type
+
+  type, bind(C) :: mpcx_sk_config
+    integer(c_int64_t) :: NT, space
+    integer(c_int32_t) :: MPCtstep, sort_step, rng_mode, pad
+    integer(c_int64_t) :: seed
+    real(c_double) :: box(3)
+    real(c_double) :: MDt, MPCt, Mf, Mg, Mi, Mc, Mn, Dcn, Rc, Rn
+    real(c_double) :: EfC, EgC, EiC, EfN, EgN, EiN, RPfg, RPgf, A0
+    real(c_double) :: k(6)
+    real(c_double) :: sinrot, cosrot, sphere_guard
+  end type
+
   interface
+    ! ---------------------------------------------------------------- engines
+    function mpcx_co_create(eng, cfg) bind(C, name="mpcx_co_create") result(rc)
+      import :: c_int, c_ptr, mpcx_co_config
+      type(c_ptr) :: eng
+      type(mpcx_co_config) :: cfg
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_co_destroy(eng) bind(C, name="mpcx_co_destroy") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: eng
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_co_set_state(eng, cstep, x, y, z, vx, vy, vz, flag, xd, yd, zd, vxd, vyd, vzd, fxd, fyd, fzd, &
+                               react_box, tour) bind(C, name="mpcx_co_set_state") result(rc)
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
+      type(c_ptr), value :: eng
+      integer(c_int32_t), value :: cstep
+      real(c_double) :: x(*), y(*), z(*), vx(*), vy(*), vz(*)
+      integer(c_int32_t) :: flag(*)
+      real(c_double) :: xd(*), yd(*), zd(*), vxd(*), vyd(*), vzd(*), fxd(*), fyd(*), fzd(*)
+      integer(c_int32_t) :: react_box(*)
+      integer(c_int64_t) :: tour(*)
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_co_setup(eng, do_sort) bind(C, name="mpcx_co_setup") result(rc)
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: eng
+      integer(c_int32_t), value :: do_sort
+      integer(c_int) :: rc
+    end function
+
+    ! draws: (3 + 2*mpcnumcell + rxnnumcell) per collision from the caller's
+    ! grnd() stream in rng_mode 0; ignored (any one-element array) in rng_mode 1
+    function mpcx_co_run(eng, ncoll, draws) bind(C, name="mpcx_co_run") result(rc)
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: eng
+      integer(c_int64_t), value :: ncoll
+      real(c_double) :: draws(*)
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_co_init_solvent(eng, Na0, Nb0, T, seed) bind(C, name="mpcx_co_init_solvent") result(rc)
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: eng
+      integer(c_int64_t), value :: Na0, Nb0
+      real(c_double), value :: T
+      integer(c_int64_t), value :: seed
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_co_get_state(eng, x, y, z, vx, vy, vz, flag, dimer12, scalars) &
+        bind(C, name="mpcx_co_get_state") result(rc)
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: eng
+      real(c_double) :: x(*), y(*), z(*), vx(*), vy(*), vz(*)
+      integer(c_int32_t) :: flag(*)
+      real(c_double) :: dimer12(*)        ! (2*numdims, 12): x y z vx vy vz fx fy fz qx qy qz
+      integer(c_int32_t) :: scalars(4)    ! cstep, time_count, time_penetrate, inside_num
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_co_set_near_mode(eng, mode) bind(C, name="mpcx_co_set_near_mode") result(rc)
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: eng
+      integer(c_int32_t), value :: mode
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_co_set_overlap(eng, on) bind(C, name="mpcx_co_set_overlap") result(rc)
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: eng
+      integer(c_int32_t), value :: on
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_co_set_precision(eng, split) bind(C, name="mpcx_co_set_precision") result(rc)
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: eng
+      integer(c_int32_t), value :: split
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_co_observables(eng, out16) bind(C, name="mpcx_co_observables") result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: eng
+      real(c_double) :: out16(16)
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_sk_create(eng, cfg) bind(C, name="mpcx_sk_create") result(rc)
+      import :: c_int, c_ptr, mpcx_sk_config
+      type(c_ptr) :: eng
+      type(mpcx_sk_config) :: cfg
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_sk_destroy(eng) bind(C, name="mpcx_sk_destroy") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: eng
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_sk_set_state(eng, cstep, NT, x, y, z, vx, vy, vz, fx, fy, fz, flag, dm) &
+        bind(C, name="mpcx_sk_set_state") result(rc)
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: eng
+      integer(c_int64_t), value :: cstep, NT
+      real(c_double) :: x(*), y(*), z(*), vx(*), vy(*), vz(*), fx(*), fy(*), fz(*)
+      integer(c_int64_t) :: flag(*)
+      real(c_double) :: dm(2, 12)
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_sk_run(eng, ncoll, draws, ndraws, used) bind(C, name="mpcx_sk_run") result(rc)
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: eng
+      integer(c_int64_t), value :: ncoll
+      real(c_double) :: draws(*)
+      integer(c_int64_t), value :: ndraws
+      integer(c_int64_t) :: used
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_sk_get_state(eng, x, y, z, vx, vy, vz, fx, fy, fz, flag, dm, scalars, tpotential) &
+        bind(C, name="mpcx_sk_get_state") result(rc)
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: eng
+      real(c_double) :: x(*), y(*), z(*), vx(*), vy(*), vz(*), fx(*), fy(*), fz(*)
+      integer(c_int64_t) :: flag(*)
+      real(c_double) :: dm(2, 12)
+      integer(c_int64_t) :: scalars(4)   ! cstep, NT, draws used, near-set size
+      real(c_double) :: tpotential
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_sk_observables(eng, out16) bind(C, name="mpcx_sk_observables") result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: eng
+      real(c_double) :: out16(16)
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_mt95_seed(seed, val, cnt) bind(C, name="mpcx_mt95_seed") result(rc)
+      import :: c_int, c_int32_t
+      integer(c_int32_t), value :: seed
+      integer(c_int32_t) :: val(624), cnt
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_mt95_fill(val, cnt, out, n) bind(C, name="mpcx_mt95_fill") result(rc)
+      import :: c_int, c_int32_t, c_int64_t, c_double
+      integer(c_int32_t) :: val(624), cnt
+      real(c_double) :: out(*)
+      integer(c_int64_t), value :: n
+      integer(c_int) :: rc
+    end function
+
+    ! batches of replicas (engines made by mpcx_co_create)
+    function mpcx_cob_create(b, R, engines, tile) bind(C, name="mpcx_cob_create") result(rc)
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr) :: b
+      integer(c_int32_t), value :: R
+      type(c_ptr) :: engines(*)
+      integer(c_int32_t), value :: tile
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_cob_destroy(b) bind(C, name="mpcx_cob_destroy") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_cob_run(b, ncoll, use_graph) bind(C, name="mpcx_cob_run") result(rc)
+      import :: c_int, c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: b
+      integer(c_int64_t), value :: ncoll
+      integer(c_int32_t), value :: use_graph
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_shutdown() bind(C, name="mpcx_shutdown") result(rc)
+      import :: c_int
+      integer(c_int) :: rc
+    end function
+
+    ! ------------------------------------------------------ drop-ins and I/O
     function mpcx_last_error() bind(C, name="mpcx_last_error") result(p)
       import :: c_ptr
       type(c_ptr) :: p

@@ -155,3 +155,131 @@ def test_fortran_dropin_modules_match_oracle(tmp_path, numdims):
         assert close(g[k], r[k], 1.0)[0] == 0, k
     assert np.array_equal(g["mt"], r["mt"]) and int(g["mti"][0]) == r["mti"]
     assert int((g["flag"] != s["flag"]).sum()) > 0   # sort and reaction changed flags
+
+
+ECHECK = os.path.join(ROOT, "tests", "fortran", "_build", "engine_check")
+
+
+def _run_engine_check(tmp_path, mode):
+    if not os.path.exists(ECHECK):
+        pytest.fail(f"{ECHECK} is not built: run __graft_entry__.build()")
+    run = subprocess.run([ECHECK, mode], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert run.returncode == 0, run.stderr
+
+
+def test_fortran_collective_engine_loop_matches_oracle(tmp_path):
+    """The engine loop of the integration guide in Fortran (bind(C) config
+    mirror, create, set_state, setup with the sort, two chunks of two
+    collisions with host grnd() draws, get_state) against the oracle's run of
+    the same system and stream: integer state and the caller's Mersenne-Twister
+    state exact, positions and velocities within 1e-12."""
+    import copy
+    from main_amd.collective import make_config
+    p = small_collective_params(numdims=2, Lx=20.0, Ly=20.0, Lz=24.0, k1=0.3)
+    s = build_collective(p, seed=29, flags_mix=True)
+    L = [int(v) for v in p["box"]]
+    tour = cell_tour(*L)
+    nchunk, nc = 2, 2
+    cfg = make_config(p, rng_mode=0)
+    rb = np.ascontiguousarray(s["react_box"], np.int32)
+    with open(tmp_path / "co_in.bin", "wb") as f:
+        f.write(bytes(cfg))
+        f.write(np.array([rb.size, nchunk, nc], np.int64).tobytes())
+        for k in ("x", "y", "z", "vx", "vy", "vz"):
+            f.write(np.ascontiguousarray(s[k], np.float64).tobytes())
+        f.write(np.ascontiguousarray(s["flag"], np.int32).tobytes())
+        for k in ("xd", "yd", "zd", "vxd", "vyd", "vzd", "fxd", "fyd", "fzd"):
+            f.write(np.asfortranarray(s[k], np.float64).tobytes(order="F"))
+        f.write(rb.tobytes())
+        f.write(np.ascontiguousarray(tour, np.int64).tobytes())
+        f.write(np.ascontiguousarray(s["mt"]).astype(np.uint32).tobytes())
+        f.write(np.array([s["mti"]], np.int32).tobytes())
+    print(f"[fortran] collective engine loop NT={p['NT']}", file=sys.stderr, flush=True)
+    _run_engine_check(tmp_path, "co")
+    b = open(tmp_path / "co_out.bin", "rb").read()
+    NT, nd = p["NT"], p["numdims"]
+    pos = 0
+
+    def take(dtype, n):
+        nonlocal pos
+        a = np.frombuffer(b, dtype=dtype, count=n, offset=pos)
+        pos += n * np.dtype(dtype).itemsize
+        return a
+    g = {k: take(np.float64, NT) for k in ("x", "y", "z", "vx", "vy", "vz")}
+    g["flag"] = take(np.int32, NT)
+    dim = take(np.float64, 24 * nd).reshape(12, 2 * nd)
+    scal = take(np.int32, 4)
+    mt, mti = take(np.int32, 624), int(take(np.int32, 1)[0])
+    assert pos == len(b)
+    ref = copy.deepcopy(s)
+    O.co_run(ref, 0, do_setup=2, tour=tour)
+    O.co_run(ref, nchunk * nc * p["MPCtstep"], tour=tour)
+    assert np.array_equal(g["flag"], ref["flag"])
+    vs = np.sqrt(p["T"] / p["Ma"])
+    for k in ("x", "y", "z"):
+        assert close(g[k], ref[k], 1.0)[0] == 0, k
+    for k in ("vx", "vy", "vz"):
+        assert close(g[k], ref[k], vs)[0] == 0, k
+    for f, k in enumerate(("xd", "yd", "zd")):
+        assert close(dim[f].reshape(2, nd).T, ref[k], 1.0)[0] == 0, k
+    assert (int(scal[0]), int(scal[1]), int(scal[2])) == (ref["cstep"], ref["time_count"], ref["time_penetrate"])
+    assert np.array_equal(mt.view(np.uint32), np.asarray(ref["mt"], np.uint32)) and mti == ref["mti"]
+
+
+def test_fortran_selkov_engine_loop_matches_oracle(tmp_path):
+    """The selkov_code loop in Fortran: bind(C) config mirror, set_state, two
+    chunks of two collisions, each taking a buffer of mt95 draws and advancing
+    the stream by the count the run used, against the oracle: flags and the
+    stream exact, positions, velocities, forces and the dimer within 1e-12."""
+    import copy
+    from main_amd import systems as S
+    from main_amd.selkov import make_config as sk_config
+    SEED = 8154543
+    p = S.selkov_params(L=20.0, rho=4.0, Rc=2.0, Rn=4.0, Dcn=7.0, MDt=0.005, MPCt=0.1, NG=3200,
+                        k=(0.05, 0.01, 0.0004, 0.0004, 0.01, 0.01))
+    s = S.selkov_initial_state(p, seed=9)
+    s["flag"][:] = np.random.default_rng(15).choice([0, 1, 3], p["NT"]).astype(np.int64)
+    nchunk, nc = 2, 2
+    cfg = sk_config(p, rng_mode=0)
+    space = int(cfg.space)
+    with open(tmp_path / "sk_in.bin", "wb") as f:
+        f.write(bytes(cfg))
+        f.write(np.array([nchunk, nc], np.int64).tobytes())
+        f.write(np.array([SEED], np.int32).tobytes())
+        for k in ("x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz"):
+            a = np.zeros(space)
+            a[: len(s[k])] = s[k]
+            f.write(a.tobytes())
+        fl = np.full(space, 2, np.int64)
+        fl[: len(s["flag"])] = s["flag"]
+        f.write(fl.tobytes())
+        f.write(np.asfortranarray(s["dm"], np.float64).tobytes(order="F"))
+    print(f"[fortran] selkov engine loop NT={p['NT']}", file=sys.stderr, flush=True)
+    _run_engine_check(tmp_path, "sk")
+    b = open(tmp_path / "sk_out.bin", "rb").read()
+    pos = 0
+
+    def take(dtype, n):
+        nonlocal pos
+        a = np.frombuffer(b, dtype=dtype, count=n, offset=pos)
+        pos += n * np.dtype(dtype).itemsize
+        return a
+    g = {k: take(np.float64, space) for k in ("x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz")}
+    g["flag"] = take(np.int64, space)
+    dm = take(np.float64, 24).reshape(2, 12, order="F")
+    scal = take(np.int64, 4)
+    take(np.float64, 1)
+    take(np.int32, 625)
+    assert pos == len(b)
+    ref = copy.deepcopy(s)
+    ref["sort_step"] = 10000
+    O.sk_run(ref, 1, nchunk * nc * p["MPCtstep"], seed=SEED)
+    n = int(ref["NT"])
+    assert int(scal[1]) == n and int(scal[0]) == nchunk * nc * p["MPCtstep"]
+    assert np.array_equal(g["flag"][:n], np.asarray(ref["flag"])[:n])
+    vs = np.sqrt(p["T"] / p["Mf"])
+    for k in ("x", "y", "z", "fx", "fy", "fz"):
+        assert close(g[k][:n], np.asarray(ref[k])[:n], 1.0)[0] == 0, k
+    for k in ("vx", "vy", "vz"):
+        assert close(g[k][:n], np.asarray(ref[k])[:n], vs)[0] == 0, k
+    assert close(dm[:, 0:3], np.asarray(ref["dm"])[:, 0:3], 1.0)[0] == 0
