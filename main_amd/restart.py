@@ -120,3 +120,26 @@ def restart(engine, directory: str, react_box, tour=None) -> dict:
     mp = os.path.join(directory, MOTOR_POSITIONS)
     return dict(cstep=cstep, streams=read_mt_state(mt) if os.path.exists(mt) else [],
                 motor_cm=read_motor_positions(mp, int(engine.p["numdims"])) if os.path.exists(mp) else None)
+
+
+def restart_from_h5(engine, directory: str, react_box, freqs, tour=None, h5name: str = "data.h5") -> dict:
+    """Restore `engine` as restart_mod.f95:359-495 does from the reference's
+    own files: the step and species counts of timestep.dat, the solvent of
+    data.h5's frame cstep/Freq1 and the dimers of frame cstep/Freq4 (forces
+    re-evaluated by the restart set-up), the streams of mt_state.dat and
+    motorCM of motor_positions.dat.  freqs = (Freq1, Freq2, Freq3, Freq4)."""
+    from . import h5data
+    cstep, n_a, n_b, n_s = read_timestep(os.path.join(directory, TIMESTEP))
+    p = engine.p
+    if n_a + n_b + n_s != int(p["NT"]):
+        raise ValueError(f"timestep.dat counts {n_a}+{n_b}+{n_s} particles, the system has {int(p['NT'])}")
+    st = h5data.read_restart(os.path.join(directory, h5name), cstep, p, freqs)
+    for k in ("fxd", "fyd", "fzd"):
+        st[k] = np.zeros((int(p["numdims"]), 2), order="F")
+    st["cstep"], st["react_box"] = cstep, react_box
+    engine.set_state(st, tour=tour)
+    engine.setup(do_sort=(cstep % int(p["sort_step"]) == 0), restart=True)
+    mt = os.path.join(directory, MT_STATE)
+    mp = os.path.join(directory, MOTOR_POSITIONS)
+    return dict(cstep=cstep, streams=read_mt_state(mt) if os.path.exists(mt) else [],
+                motor_cm=read_motor_positions(mp, int(p["numdims"])) if os.path.exists(mp) else None)

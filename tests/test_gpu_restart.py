@@ -87,6 +87,52 @@ def test_restart_through_files_equals_reference(golden, tmp_path, case, near_mod
         assert n == 0, f"{k}: {n} values beyond 1e-12 (worst {err:.2e})"
 
 
+@pytest.mark.parametrize("case", ["co_one_dimer_restart", "co_two_dimers_restart_sort"])
+def test_restart_through_data_h5_equals_reference(golden, tmp_path, case):
+    """The reference's own restart files: data.h5 (its solvent and dimer
+    frames written at steps 0 and restart_at by main_amd/libmpcx_h5.so),
+    timestep.dat and mt_state.dat; the restarted run's integer state equals
+    the reference's restarted run (the fixture's hashes)."""
+    from main_amd import h5data
+    cs, p, s, tour = _inputs(case, golden)
+    ra, steps, S = cs["restart_at"], cs["steps"], p["MPCtstep"]
+    freqs = (ra, ra, S, S)
+    slabs = tuple(float(b) / 2.0 for b in p["box"])
+    stream = MtmodStream(mt=s["mt"], mti=s["mti"])
+    a = CollectiveEngine(p, rng_mode=0)
+    path = str(tmp_path / "data.h5")
+    try:
+        a.set_state(s, tour=tour)
+        a.setup(do_sort=cs["sort"])
+        h5data.initiate(path, p, freqs)
+        h5data.write_engine(path, a, freqs, slabs)
+        for _ in range(ra // S):
+            a.run(1, host_draws(stream, p, 1))
+            h5data.write_engine(path, a, freqs, slabs)
+        o = a.observables()
+        RS.write_timestep(str(tmp_path / RS.TIMESTEP), ra, int(o["nA"]), int(o["nB"]), int(o["nS"]))
+        RS.write_mt_state(str(tmp_path / RS.MT_STATE), [stream])
+    finally:
+        a.close()
+    b = CollectiveEngine(p, rng_mode=0)
+    try:
+        r = RS.restart_from_h5(b, str(tmp_path), s["react_box"], freqs, tour=tour)
+        assert r["cstep"] == ra
+        b.run((steps - ra) // S, host_draws(r["streams"][0], p, (steps - ra) // S))
+        g = b.get_state()
+        cells = b.get_cells()
+    finally:
+        b.close()
+    rec = golden["cases"][case]
+    got = {"flag": _h(g["flag"].astype(np.int32)), "ncn": _h(cells["ncn"].astype(np.int64)),
+           "tcs": _h(cells["tcs"]), "tA": _h(cells["tA"][: p["vol"]]), "tB": _h(cells["tB"][: p["vol"]]),
+           "tS": _h(cells["tS"][: p["vol"]])}
+    bad = [k for k, v in got.items() if v != rec["arrays"][k]]
+    assert not bad, f"{case}: differs from the reference's restarted run in {bad}"
+    for k in ("cstep", "time_count", "time_penetrate"):
+        assert g[k] == rec["scalars"][k], k
+
+
 @pytest.mark.parametrize("case,near_mode", [("co_one_dimer_restart", None), ("co_two_dimers_restart_sort", 2),
                                             ("co_two_dimers_restart_sort", 0), ("co_six_dimers_restart", None)])
 def test_restart_evaluation_matches_oracle(golden, case, near_mode):
