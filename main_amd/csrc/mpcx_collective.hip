@@ -5021,28 +5021,45 @@ __global__ __launch_bounds__(256) void sb_sort_bin(const CoP* __restrict__ Ps, c
     }
   }
 }
+// The gather of the re-sort, one field per pass (pass 0 the cell with the
+// status byte and the near bit, 1 the species, 2-7 the offsets and
+// velocities): each pass's random reads fall in one array of the replicas a
+// XCD is working through, a few MB that its L2 holds, where the fields all
+// at once overflowed it and fetched a line per element read.
+template <int PASS>
 __global__ __launch_bounds__(256) void sb_sort_gather(const CoD* __restrict__ Ds, const SpD* __restrict__ Ss,
                                                       const SpD* __restrict__ Ns, const CoSortD* __restrict__ Xs,
                                                       const Tile* __restrict__ tiles) {
   const Tile tl = tiles[blockIdx.x];
-  const CoD D = Ds[tl.r];
-  const SpD S = Ss[tl.r], N = Ns[tl.r];
   const CoSortD X = Xs[tl.r];
-  const int lane = threadIdx.x & 63;
-  for (long long base = tl.begin; base < tl.end; base += 256) {
-    const long long j = base + threadIdx.x;
-    bool bit = false;
-    if (j < tl.end) {
-      const int i = X.sb[j];
-      N.cell[j] = S.cell[i]; N.ox[j] = S.ox[i]; N.oy[j] = S.oy[i]; N.oz[j] = S.oz[i];
-      N.vx[j] = S.vx[i]; N.vy[j] = S.vy[i]; N.vz[j] = S.vz[i]; N.flag[j] = S.flag[i];
-      X.astat[j] = D.status[i];
-      bit = (D.near_bits[i >> 5] >> (i & 31)) & 1u;
+  const SpD S = Ss[tl.r], N = Ns[tl.r];
+  const gp<const int> sb = g1((const int*)X.sb);
+  if constexpr (PASS == 0) {
+    const CoD D = Ds[tl.r];
+    const int lane = threadIdx.x & 63;
+    for (long long base = tl.begin; base < tl.end; base += 256) {
+      const long long j = base + threadIdx.x;
+      bool bit = false;
+      if (j < tl.end) {
+        const int i = sb[j];
+        g1(N.cell)[j] = g1((const int*)S.cell)[i];
+        g1(X.astat)[j] = g1((const unsigned char*)D.status)[i];
+        bit = (g1((const unsigned*)D.near_bits)[i >> 5] >> (i & 31)) & 1u;
+      }
+      const unsigned long long m = __ballot(bit);
+      const long long w0 = (base + (threadIdx.x & ~63)) >> 5, nw = (tl.end + 31) >> 5;
+      if (lane == 0 && w0 < nw) g1(X.abits)[w0] = (unsigned)m;
+      if (lane == 32 && w0 + 1 < nw) g1(X.abits)[w0 + 1] = (unsigned)(m >> 32);
     }
-    const unsigned long long m = __ballot(bit);
-    const long long w0 = (base + (threadIdx.x & ~63)) >> 5, nw = (tl.end + 31) >> 5;
-    if (lane == 0 && w0 < nw) X.abits[w0] = (unsigned)m;
-    if (lane == 32 && w0 + 1 < nw) X.abits[w0 + 1] = (unsigned)(m >> 32);
+  } else if constexpr (PASS == 1) {
+    for (long long j = tl.begin + threadIdx.x; j < tl.end; j += 256)
+      g1(N.flag)[j] = g1((const signed char*)S.flag)[sb[j]];
+  } else {
+    const float* src = PASS == 2 ? S.ox : PASS == 3 ? S.oy : PASS == 4 ? S.oz : PASS == 5 ? S.vx : PASS == 6 ? S.vy : S.vz;
+    float* dst = PASS == 2 ? N.ox : PASS == 3 ? N.oy : PASS == 4 ? N.oz : PASS == 5 ? N.vx : PASS == 6 ? N.vy : N.vz;
+    const gp<const float> s1 = g1(src);
+    const gp<float> d1 = g1(dst);
+    for (long long j = tl.begin + threadIdx.x; j < tl.end; j += 256) d1[j] = s1[sb[j]];
   }
 }
 
@@ -5242,8 +5259,17 @@ static int cob_sort_all(mpcx_cob* b) {
   hipLaunchKernelGGL(sort_scan_b, dim3(b->R), dim3(1024), 0, b->st, b->d_P, b->d_X);
   hipLaunchKernelGGL(sort_scatter_b, dim3(b->n_pt), dim3(256), 0, b->st, b->d_X, b->d_pt);
   hipLaunchKernelGGL(sort_order_b, dim3(b->n_vt), dim3(256), 0, b->st, b->d_X, b->d_vt);
-  if (b->split)
-    hipLaunchKernelGGL(sb_sort_gather, dim3(b->n_pt), dim3(256), 0, b->st, b->d_D, b->d_S, b->d_N, b->d_X, b->d_gt);
+  if (b->split) {   // one field per pass
+    const dim3 g(b->n_pt), t(256);
+    hipLaunchKernelGGL(sb_sort_gather<0>, g, t, 0, b->st, b->d_D, b->d_S, b->d_N, b->d_X, b->d_gt);
+    hipLaunchKernelGGL(sb_sort_gather<1>, g, t, 0, b->st, b->d_D, b->d_S, b->d_N, b->d_X, b->d_gt);
+    hipLaunchKernelGGL(sb_sort_gather<2>, g, t, 0, b->st, b->d_D, b->d_S, b->d_N, b->d_X, b->d_gt);
+    hipLaunchKernelGGL(sb_sort_gather<3>, g, t, 0, b->st, b->d_D, b->d_S, b->d_N, b->d_X, b->d_gt);
+    hipLaunchKernelGGL(sb_sort_gather<4>, g, t, 0, b->st, b->d_D, b->d_S, b->d_N, b->d_X, b->d_gt);
+    hipLaunchKernelGGL(sb_sort_gather<5>, g, t, 0, b->st, b->d_D, b->d_S, b->d_N, b->d_X, b->d_gt);
+    hipLaunchKernelGGL(sb_sort_gather<6>, g, t, 0, b->st, b->d_D, b->d_S, b->d_N, b->d_X, b->d_gt);
+    hipLaunchKernelGGL(sb_sort_gather<7>, g, t, 0, b->st, b->d_D, b->d_S, b->d_N, b->d_X, b->d_gt);
+  }
   else hipLaunchKernelGGL(sort_gather_b, dim3(b->n_pt), dim3(256), 0, b->st, b->d_D, b->d_X, b->d_gt);
   for (int r = 0; r < b->R; r++) {
     mpcx_co* e = b->eng[r];
