@@ -130,7 +130,7 @@ def batched(R=256, steps=20, warmup=4, seed=4357787, tiles=(1024, 2048, 4096, 81
     print(f"# batched: {R} replicas built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     batch = CollectiveBatch(engines, tile=2048)
     info = batch.info()
-    times = batch.autotune(tiles=tiles, ncoll=2)
+    times = batch.autotune(tiles=tiles, ncoll=10)
     batch.run(warmup)
     device_synchronize()
     t0 = time.perf_counter()
@@ -226,7 +226,8 @@ def selkov(steps=20, warmup=5, seed=8154543):
 #            cell, 3.2 per particle); writes velocities (12)                                  = 32.2
 #  sb_cells  per collision cell: one accumulator copy read and cleared (48), record written (32); per
 #            unshifted cell: B census and first key read and cleared (16); per particle       = 9.6
-# The interval's figure is the north-star's 57 B per particle-update (state read and written once).
+# The interval's figure is 57 B per particle-update: the split state read once (cell 4, offsets 12,
+# velocities 12, species 1) and written once (the same without the species, written only on a change).
 ALGO_BYTES_SPLIT = {"sb_bulk": 51.0, "sb_rotate": 32.2, "sb_cells": 9.6}
 SPLIT_STEP_BYTES = 57.0
 FP64_STEP_BYTES = 100.0
@@ -306,7 +307,7 @@ def ensemble_engines(R, seed, split):
 
 
 def split_batched(R=256, steps=20, warmup=4, seed=4357787, fp64_value=None):
-    """The north star's mode: the replica ensemble in split-precision storage,
+    """The headline ensemble mode: the replicas in split-precision storage,
     advanced by one batch (HIP-graph replay, autotuned tile and cells per
     workgroup).  Timed over `steps` intervals (two re-sorts in 20); the kernel
     means come from a further run without the graph (HIP events)."""
@@ -317,7 +318,7 @@ def split_batched(R=256, steps=20, warmup=4, seed=4357787, fp64_value=None):
     print(f"# split batched: {R} replicas built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     batch = CollectiveBatch(engines, tile=4096)
     info = batch.info()
-    times = batch.autotune(tiles=(2048, 4096, 8192), cell_tiles=(256, 1024), ncoll=2)
+    times = batch.autotune(tiles=(2048, 4096, 8192), cell_tiles=(256, 1024), slots=(1024, 2048), ncoll=10)
     batch.run(warmup)
     device_synchronize()
     t0 = time.perf_counter()
@@ -339,8 +340,8 @@ def split_batched(R=256, steps=20, warmup=4, seed=4357787, fp64_value=None):
           flush=True)
     out = {"replicas": R, "particles": n, "collision_cells": info["cells"], "ms_per_interval": ms,
            "particle_updates_per_s": value, "replicas_per_hour_2000_collisions": R * 3600.0 / (2000 * ms * 1e-3),
-           "tile": batch.tile, "cells_per_workgroup": batch.cell_tile,
-           "autotune_ms_per_interval": {f"{k[0]}x{k[1]}": v for k, v in times.items()},
+           "tile": batch.tile, "cells_per_workgroup": batch.cell_tile, "bulk_slots": batch.bulk_slots,
+           "autotune_ms_per_interval": {"x".join(map(str, k)): v for k, v in times.items()},
            "roofline_step": step_roofline(SPLIT_STEP_BYTES, n, ms), "kernel_ms": kms,
            "kernels": kernels, "rng": "device Philox4x32-10", "graph": True,
            "dtype": "f32 solvent (cell + offsets), f64 near set and dimers"}
@@ -527,9 +528,10 @@ def main():
     for k, v in kernels.items():
         v["traffic"] = read_traffic(k)
     step.update(kernel="collision interval (all its kernels, both streams)", traffic=step_traffic(),
-                traffic_note="counter bytes between L2 and the memory fabric per interval (FETCH_SIZE, "
-                             "WRITE_SIZE scaled by the copy probe): Infinity-Cache hits included, so an upper "
-                             "bound on HBM bytes", kernels=kernels, kernel_ms=kms)
+                traffic_note="bytes between L2 and the memory fabric per interval (read and write requests by "
+                             "size; the copy probe reads back at 1.000x), the re-sort's launches counted once per "
+                             "sort_step: Infinity-Cache hits included, so an upper bound on HBM bytes",
+                kernels=kernels, kernel_ms=kms)
     out = {
         "metric": "particle-updates/s (MD+MPC)",
         "value": value,

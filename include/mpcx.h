@@ -313,6 +313,10 @@ int mpcx_cob_set_tile(mpcx_cob* b, int32_t tile);
 /* Collision cells per workgroup of the split-precision cell kernel (a
  * multiple of 256; default 256).  Results do not depend on it. */
 int mpcx_cob_set_cell_tile(mpcx_cob* b, int32_t cells);
+/* Collision-cell slots of the per-workgroup hash table of the split-precision
+ * binning: 1024, 2048, or 0 (default: 2048 for tiles of 4096 particles and
+ * more, else 1024).  Results do not depend on it. */
+int mpcx_cob_set_bulk_slots(mpcx_cob* b, int32_t slots);
 int mpcx_cob_info(mpcx_cob* b, int64_t* particles, int64_t* cells, double* run_ms);
 /* Split batches run without the graph: mean kernel durations (ms) of the last
  * run from HIP events on the stream each runs on (every 8th interval, at most

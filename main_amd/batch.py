@@ -3,8 +3,8 @@ mpcx_cob_* in include/mpcx.h): independent replicas with their own boxes,
 parameters and Philox keys, one launch per kernel of an interval for all of
 them, the normal intervals replayed from a HIP graph; FP64 or split-precision
 replicas.  The tile size of the particle kernels (and, for split replicas,
-the cells per workgroup of the cell kernel) is chosen by `autotune`, which
-times short runs.
+the cells per workgroup of the cell kernel and the slots of the binning's
+hash table) is chosen by `autotune`, which times short runs.
 """
 from __future__ import annotations
 
@@ -24,6 +24,7 @@ class CollectiveBatch:
         track(self, first=True)
         self.tile = tile
         self.cell_tile = 256
+        self.bulk_slots = 0
 
     def close(self):
         if self.h:
@@ -79,25 +80,42 @@ class CollectiveBatch:
         check(lib().mpcx_cob_set_cell_tile(self.h, C.c_int32(cells)), "mpcx_cob_set_cell_tile")
         self.cell_tile = cells
 
+    def set_bulk_slots(self, slots: int):
+        """Collision-cell slots of the split binning's hash table (1024, 2048,
+        or 0 for the default by tile size)."""
+        check(lib().mpcx_cob_set_bulk_slots(self.h, C.c_int32(slots)), "mpcx_cob_set_bulk_slots")
+        self.bulk_slots = slots
+
     def autotune(self, tiles=(512, 1024, 2048, 4096, 8192), ncoll: int = 4, use_graph: bool = True,
-                 cell_tiles=None) -> dict:
-        """Time `ncoll` intervals at each tile size -- and, given cell_tiles,
-        at each number of cells per workgroup of the cell kernel -- advancing
-        the systems, and keep the fastest; returns {tile: ms per interval}, or
-        {(tile, cells): ms per interval} with cell_tiles."""
+                 cell_tiles=None, slots=None) -> dict:
+        """Time `ncoll` intervals at each tile size -- and, given cell_tiles
+        and slots, at each number of cells per workgroup of the cell kernel
+        and each table size of the binning -- advancing the systems, and keep
+        the fastest; returns {tile: ms per interval}, or {(tile, cells[,
+        slots]): ms per interval}.  With re-sorts every k intervals, ncoll = k
+        gives every setting one re-sort and the same spread of particle
+        disorder."""
         times = {}
         for t in tiles:
             self.set_tile(t)
             for c in (cell_tiles or (None,)):
                 if c is not None:
                     self.set_cell_tile(c)
-                self.run(1, use_graph)          # capture / warm
-                self.run(ncoll, use_graph)
-                times[t if c is None else (t, c)] = self.info()["run_ms"] / ncoll
+                for g in (slots or (None,)):
+                    if g is not None:
+                        self.set_bulk_slots(g)
+                    self.run(1, use_graph)          # capture / warm
+                    self.run(ncoll, use_graph)
+                    key = (t,) + ((c,) if c is not None else ()) + ((g,) if g is not None else ())
+                    times[key if len(key) > 1 else t] = self.info()["run_ms"] / ncoll
         best = min(times, key=times.get)
-        if cell_tiles:
+        if isinstance(best, tuple):
             self.set_tile(best[0])
-            self.set_cell_tile(best[1])
+            rest = list(best[1:])
+            if cell_tiles:
+                self.set_cell_tile(rest.pop(0))
+            if slots:
+                self.set_bulk_slots(rest.pop(0))
         else:
             self.set_tile(best)
         return times

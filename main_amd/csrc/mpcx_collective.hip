@@ -155,10 +155,16 @@ struct CoD {
   int* late;              // [late_cap] late cells of the interval (tour positions)
   unsigned* next_bits;    // the next interval's near bitmap while near_bits is still in use
   int late_cap, epoch;
-  // split storage: per collision cell (by tour position) the velocity sums in
-  // 64-bit fixed point, three words, the member count in the low 12 bits of
-  // the z word; and the cell records written from them (vcm x y z, count; axis)
+  // split storage: per collision cell, by tour position (mpc_pos), the
+  // velocity sums in 64-bit fixed point, three words, the member count in the
+  // low 12 bits of the z word; and the cell records written from them (vcm x y
+  // z, count; axis); cell_of holds a particle's collision cell the same way
   unsigned long long* sacc;
+  // the part of each tour cell's sums binned by the particle tile that owns
+  // it (the tile holding the cell's first particle at the last re-sort):
+  // written whole, by plain stores, by that tile in every interval
+  unsigned long long* sown;
+  const int* soff;        // [vol + 1] first particle of each tour position at the last re-sort
   float4* srec;
   int* xtc;               // split storage B census, one copy per XCD: [8][vol]
   unsigned* xfb;          // split storage first-B keys, one copy per XCD: [8][vol]
@@ -3183,6 +3189,15 @@ extern "C" int mpcx_co_create(mpcx_co** out, const mpcx_co_config* cfg) {
   HIPCHK(hipMemset(D.near_bits, 0, (e->nwords + 1) * 4));
   HIPCHK(hipMemset(D.next_bits, 0, (e->nwords + 1) * 4));
   HIPCHK(hipMemset(D.stamp, 0, (size_t)P.mpcnumcell * 4));
+  HIPCHK(hipMemset(e->d_soff, 0, ((size_t)vol + 1) * 4));   // all zero: a valid cell ownership before any sort
+  D.soff = e->d_soff;
+  {   // collision-cell tour maps: the identity until a state with its tour is uploaded
+    e->mpc_pos.resize((size_t)mc);
+    e->mpc_cell.resize((size_t)mc);
+    for (long long c = 0; c < mc; c++) { e->mpc_pos[c] = (int)c; e->mpc_cell[c] = (int)c; }
+    HIPCHK(hipMemcpy((void*)D.mpc_pos, e->mpc_pos.data(), mc * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy((void*)D.mpc_cell, e->mpc_cell.data(), mc * 4, hipMemcpyHostToDevice));
+  }
   HIPCHK(hipMemset(D.in_fp, 0, (size_t)MAXM * 3 * D.in_cap * 8));
   HIPCHK(hipMemset(D.in_pot, 0, (size_t)MAXM * D.in_cap * 8));
   HIPCHK(hipMemset(D.in_msd, 0, (size_t)D.in_cap * 8));
@@ -3218,11 +3233,14 @@ extern "C" int mpcx_co_set_precision(mpcx_co* e, int32_t split) {
     }
     const size_t mc = (size_t)e->P.mpcnumcell, vol = (size_t)e->P.vol;
     e->D.sacc = dalloc<unsigned long long>(e, NXCD * 3 * mc);
+    e->D.sown = dalloc<unsigned long long>(e, 3 * mc);
     e->D.srec = dalloc<float4>(e, 2 * mc);
     e->D.xtc = dalloc<int>(e, NXCD * vol);
     e->D.xfb = dalloc<unsigned>(e, NXCD * vol);
-    if (!e->D.sacc || !e->D.srec || !e->D.xtc || !e->D.xfb) return set_error(MPCX_E_HIP, "device allocation failed");
+    if (!e->D.sacc || !e->D.sown || !e->D.srec || !e->D.xtc || !e->D.xfb)
+      return set_error(MPCX_E_HIP, "device allocation failed");
     HIPCHK(hipMemset(e->D.sacc, 0, NXCD * 3 * mc * 8));
+    HIPCHK(hipMemset(e->D.sown, 0, 3 * mc * 8));
     HIPCHK(hipMemset(e->D.srec, 0, 2 * mc * 16));
     HIPCHK(hipMemset(e->D.xtc, 0, NXCD * vol * 4));
     HIPCHK(hipMemset(e->D.xfb, 0xff, NXCD * vol * 4));
@@ -4166,11 +4184,11 @@ extern "C" int mpcx_co_get_cells(mpcx_co* e, double* vcm_uv, int32_t* cell_of, i
     return set_error(MPCX_E_ARG, "no collision has run since the state was uploaded: the cell arrays are empty");
   CoD& D = e->D;
   const size_t mc = (size_t)e->P.mpcnumcell;
-  if (vcm_uv && e->split) {   // FP32 records (vcm, count; axis) by tour position
+  if (vcm_uv && e->split) {   // FP32 records (vcm, count; axis) by tour position; returned by cell
     std::vector<float> rec(mc * 8);
     HIPCHK(hipMemcpy(rec.data(), D.srec, mc * 32, hipMemcpyDeviceToHost));
-    for (size_t t = 0; t < mc; t++) {   // split records are by collision cell
-      double* o = vcm_uv + t * 8;
+    for (size_t t = 0; t < mc; t++) {
+      double* o = vcm_uv + (size_t)e->mpc_cell[t] * 8;
       const float* r = &rec[t * 8];
       o[0] = r[0]; o[1] = r[1]; o[2] = r[2]; o[3] = r[4]; o[4] = r[5]; o[5] = r[6]; o[6] = r[3]; o[7] = 0.0;
     }
@@ -4183,7 +4201,7 @@ extern "C" int mpcx_co_get_cells(mpcx_co* e, double* vcm_uv, int32_t* cell_of, i
     HIPCHK(hipMemcpy(cell_of, D.cell_of, (size_t)e->P.NT * 4, hipMemcpyDeviceToHost));
     for (long long i = 0; i < e->P.NT; i++) {
       if (cell_of[i] < 0 || (size_t)cell_of[i] >= mc) return set_error(MPCX_E_DEVICE, "cell index out of range");
-      if (!e->split) cell_of[i] = e->mpc_cell[cell_of[i]];   // FP64 storage binned by tour position
+      cell_of[i] = e->mpc_cell[cell_of[i]];   // both storages bin by tour position
     }
   }
   if (census && e->split) return set_error(MPCX_E_UNSUPPORTED, "split storage clears its census after each collision");
@@ -4514,44 +4532,53 @@ __device__ __forceinline__ long long sb_fix(float v) { return __double2ll_rn((do
 
 // Per-workgroup hash tables of the binning, filled over the workgroup's whole
 // tile and added to the global sums once at its end (one atomic per word and
-// distinct cell or census key of the tile): collision-cell sums (the count in
-// the low bits of the z word) and the B census (count and minimum key per
-// unshifted cell).  A key
-// that finds no free slot within SB_PROBE probes goes to the global sums
-// directly.
-constexpr int SB_GT = 1024;    // slots of each table
+// distinct cell or census key of the tile): collision-cell sums (GT slots; the
+// count in the low bits of the z word) and the B census (SB_CT slots; count
+// and minimum key per unshifted cell).  A key that finds no free slot within
+// SB_PROBE probes goes to the global sums directly.  The distinct collision
+// cells of a tile grow as the particles drift from their sorted order (a
+// 4096-particle tile of 10 per cell: about 580 just after a re-sort, about
+// 1020 nine collisions later), so the tile size picks GT to keep the table
+// at most about half full.
+constexpr int SB_CT = 1024;    // census slots
 constexpr int SB_PROBE = 64;
+template <int GT>
 struct SbTables {
-  int rk[SB_GT];
-  unsigned long long rs[3][SB_GT];
-  int ck[SB_GT], cc[SB_GT];
-  unsigned cm[SB_GT];
+  static_assert((GT & (GT - 1)) == 0 && GT >= SB_CT, "slots: a power of two, at least the census slots");
+  static constexpr int SHIFT = 32 - __builtin_ctz(GT);
+  int rk[GT];
+  unsigned long long rs[3][GT];
+  int ck[SB_CT], cc[SB_CT];
+  unsigned cm[SB_CT];
 };
-__device__ __forceinline__ void sb_tables_init(SbTables& G) {
-  for (int j = threadIdx.x; j < SB_GT; j += blockDim.x) {
+template <int GT>
+__device__ __forceinline__ void sb_tables_init(SbTables<GT>& G) {
+  for (int j = threadIdx.x; j < GT; j += blockDim.x) {
     G.rk[j] = -1; G.rs[0][j] = 0ull; G.rs[1][j] = 0ull; G.rs[2][j] = 0ull;
-    G.ck[j] = -1; G.cc[j] = 0; G.cm[j] = 0xffffffffu;
+    if (j < SB_CT) { G.ck[j] = -1; G.cc[j] = 0; G.cm[j] = 0xffffffffu; }
   }
   __syncthreads();
 }
-// slot of key in keys (inserted if absent), or -1 after SB_PROBE full probes
+// slot of key in keys[N] (inserted if absent), or -1 after SB_PROBE full probes
+template <int N>
 __device__ __forceinline__ int sb_slot(int* keys, int key) {
-  int j = (int)(((unsigned)key * 2654435761u) >> 22);   // 10 bits
+  int j = (int)(((unsigned)key * 2654435761u) >> (32 - __builtin_ctz(N)));
 #pragma unroll 1
   for (int p = 0; p < SB_PROBE; p++) {
     const int old = atomicCAS(&keys[j], -1, key);
     if (old == -1 || old == key) return j;
-    j = (j + 1) & (SB_GT - 1);
+    j = (j + 1) & (N - 1);
   }
   return -1;
 }
-// one lane's velocity into cell t (ok) and census entry ck with key (cen); no barrier
-__device__ __forceinline__ void sb_insert(const CoP& P, const SbV& A, SbTables& G, bool ok, int t, float vx, float vy,
+// one lane's velocity into cell t (tour position; ok) and census entry ck with key (cen); no barrier
+template <int GT>
+__device__ __forceinline__ void sb_insert(const CoP& P, const SbV& A, SbTables<GT>& G, bool ok, int t, float vx, float vy,
                                           float vz, bool cen, int ck, unsigned key) {
   if (ok) {
     const unsigned long long w0 = (unsigned long long)sb_fix(vx), w1 = (unsigned long long)sb_fix(vy);
     const unsigned long long w2 = ((unsigned long long)sb_fix(vz) << SB_NBITS) + 1ull;
-    const int j = sb_slot(G.rk, t);
+    const int j = sb_slot<GT>(G.rk, t);
     if (j >= 0) {
       atomicAdd(&G.rs[0][j], w0);
       atomicAdd(&G.rs[1][j], w1);
@@ -4564,7 +4591,7 @@ __device__ __forceinline__ void sb_insert(const CoP& P, const SbV& A, SbTables& 
     }
   }
   if (cen) {
-    const int j = sb_slot(G.ck, ck);
+    const int j = sb_slot<SB_CT>(G.ck, ck);
     if (j >= 0) {
       atomicAdd(&G.cc[j], 1);
       atomicMin(&G.cm[j], key);
@@ -4574,10 +4601,72 @@ __device__ __forceinline__ void sb_insert(const CoP& P, const SbV& A, SbTables& 
     }
   }
 }
-// every thread of the block together, after the tile: the tables into the global sums
-__device__ __forceinline__ void sb_flush(const CoP& P, const SbV& A, SbTables& G) {
+// slot of key in keys[N] (the table complete), or -1
+template <int N>
+__device__ __forceinline__ int sb_find(const int* keys, int key) {
+  int j = (int)(((unsigned)key * 2654435761u) >> (32 - __builtin_ctz(N)));
+#pragma unroll 1
+  for (int p = 0; p < SB_PROBE; p++) {
+    const int k = keys[j];
+    if (k == key) return j;
+    if (k == -1) return -1;
+    j = (j + 1) & (N - 1);
+  }
+  return -1;
+}
+// Cell ownership of a particle tile [b, e) of a system of NT particles: the
+// tour positions [own_bound(b), own_bound(e)) whose first particle at the last
+// re-sort (soff, non-decreasing) lies in the tile, the first tile's range from
+// 0 and the last one's to vol.  Consecutive tiles get adjacent ranges, so each
+// tour cell has one owner whatever soff holds.
+__device__ __forceinline__ int own_bound(const int* soff, long long vol, long long i, long long NT) {
+  if (i <= 0) return 0;
+  if (i >= NT) return (int)vol;
+  long long lo = 0, hi = vol;   // first p < vol with soff[p] >= i, else vol
+  while (lo < hi) {
+    const long long mid = (lo + hi) >> 1;
+    if (soff[mid] < i) lo = mid + 1;
+    else hi = mid;
+  }
+  return (int)lo;
+}
+// every thread of the block together, after the tile: the sums of the cells
+// the tile owns (tour positions [plo, phi), one contiguous run of sown)
+// stored whole -- zero for an owned cell it binned nothing into -- and the
+// other table entries added to the workgroup's XCD copy
+template <int GT>
+__device__ __forceinline__ void sb_flush_owned(const CoP& P, const CoD& D, const SbV& A, SbTables<GT>& G, int plo,
+                                               int phi) {
   __syncthreads();
-  for (int j = threadIdx.x; j < SB_GT; j += blockDim.x) {
+  const gp<unsigned long long> own = g1(D.sown);
+  for (int p = plo + (int)threadIdx.x; p < phi; p += blockDim.x) {
+    const int j = sb_find<GT>(G.rk, p);
+    unsigned long long w0 = 0ull, w1 = 0ull, w2 = 0ull;
+    if (j >= 0) { w0 = G.rs[0][j]; w1 = G.rs[1][j]; w2 = G.rs[2][j]; }
+    own[3ll * p] = w0; own[3ll * p + 1] = w1; own[3ll * p + 2] = w2;
+  }
+  for (int j = threadIdx.x; j < GT; j += blockDim.x) {
+    const int t = G.rk[j];
+    if (t >= 0) {
+      if (t < plo || t >= phi) {
+        const gp<unsigned long long> a = A.acc + 3ll * t;
+        l2_add<unsigned long long>(a, G.rs[0][j]);
+        l2_add<unsigned long long>(a + 1, G.rs[1][j]);
+        l2_add<unsigned long long>(a + 2, G.rs[2][j]);
+      }
+    }
+    const int k = j < SB_CT ? G.ck[j] : -1;
+    if (k >= 0) {
+      l2_add<int>(A.tc + k, G.cc[j]);   // checked against CENSUS_MAX by sb_cells
+      l2_min(A.fb + k, G.cm[j]);
+    }
+  }
+}
+// every thread of the block together, after the tile: the tables into the global sums
+template <int GT>
+__device__ __forceinline__ void sb_flush(const CoP& P, const SbV& A, SbTables<GT>& G) {
+  __syncthreads();
+  for (int j = threadIdx.x; j < GT; j += blockDim.x) {
     const int t = G.rk[j];
     if (t >= 0) {
       const gp<unsigned long long> a = A.acc + 3ll * t;
@@ -4585,7 +4674,7 @@ __device__ __forceinline__ void sb_flush(const CoP& P, const SbV& A, SbTables& G
       l2_add<unsigned long long>(a + 1, G.rs[1][j]);
       l2_add<unsigned long long>(a + 2, G.rs[2][j]);
     }
-    const int k = G.ck[j];
+    const int k = j < SB_CT ? G.ck[j] : -1;
     if (k >= 0) {
       l2_add<int>(A.tc + k, G.cc[j]);   // checked against CENSUS_MAX by sb_cells
       l2_min(A.fb + k, G.cm[j]);
@@ -4609,7 +4698,8 @@ __device__ __forceinline__ float sp_pbc(float d, float L) {
 // entry (outside-list particles keyed by index, inside ones after them, inside
 // ones within an LJ cutoff left out), and the distance byte to the spheres sph
 // (the selection of the next near set).
-__device__ __forceinline__ void sb_bin(const CoP& P, const CoD& D, const SbV& A, SbTables& G, int i, bool valid, bool inside,
+template <int GT>
+__device__ __forceinline__ void sb_bin(const CoP& P, const CoD& D, const SbV& A, SbTables<GT>& G, int i, bool valid, bool inside,
                                        bool excl, int ix, int iy, int iz, float ox, float oy, float oz, float vx,
                                        float vy, float vz, int f, float fxs, float fys, float fzs, double zs,
                                        const SbSpheres& sph) {
@@ -4624,7 +4714,7 @@ __device__ __forceinline__ void sb_bin(const CoP& P, const CoD& D, const SbV& A,
       raise_error(&D.sc->err, ERR_CELL_RANGE);
       ok = false;
     } else {
-      t = (int)c;
+      t = g1(D.mpc_pos)[c];   // cells are kept by tour position: a tile's cells lie in a few runs
     }
   }
   unsigned key = (unsigned)i;
@@ -4713,10 +4803,10 @@ __device__ __forceinline__ void sb_fetch(const SbV& V, long long i, long long en
 // engine): every particle binned where it is, distances to the spheres' end
 // positions.  Each round's loads are issued before the previous round is
 // binned, so their latency overlaps the binning's LDS work and barriers.
-template <bool ALL, int T>
+template <bool ALL, int T, int GT>
 __global__ __launch_bounds__(T) void sb_bulk(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds,
                                              const SpD* __restrict__ Ss, const Tile* __restrict__ tiles, int do_bin) {
-  __shared__ SbTables G;
+  __shared__ SbTables<GT> G;
   __shared__ SbSpheres sph;
   const Tile tl = tiles[blockIdx.x];
   if (tl.begin >= tl.end) return;
@@ -4726,8 +4816,13 @@ __global__ __launch_bounds__(T) void sb_bulk(const CoP* __restrict__ Ps, const C
   const int xcc = hw_xcc();
   const SbV V = sb_view(P, D, S, xcc);
   if (do_bin && threadIdx.x == 0) atomicOr(&D.sc->xmask, 1u << xcc);
+  __shared__ int own_lo, own_hi;
   if (ALL) sb_load_spheres(P, D.sc->x0, D.sc->y0, D.sc->z0, sph);
   else sb_load_spheres(P, D.sc->xs0, D.sc->ys0, D.sc->zs0, sph);
+  if (do_bin && threadIdx.x == 0) {
+    own_lo = own_bound(D.soff, P.vol, tl.begin, P.NT);
+    own_hi = own_bound(D.soff, P.vol, tl.end, P.NT);
+  }
   if (do_bin) sb_tables_init(G);
   else __syncthreads();
   double xs = 0, ys = 0, zs = 0;
@@ -4769,7 +4864,7 @@ __global__ __launch_bounds__(T) void sb_bulk(const CoP* __restrict__ Ps, const C
     if (do_bin)
       sb_bin(P, D, V, G, i, mine, inside, excl, ix, iy, iz, ox, oy, oz, vx, vy, vz, f, fxs, fys, fzs, zs, sph);
   }
-  if (do_bin) sb_flush(P, V, G);
+  if (do_bin) sb_flush_owned(P, D, V, G, own_lo, own_hi);
 }
 
 // Per system (blockIdx.y): the interval's start -- the collision counter and
@@ -4806,7 +4901,7 @@ __global__ __launch_bounds__(256) void sb_load_near(const CoP* __restrict__ Ps, 
 __global__ __launch_bounds__(256) void sb_near(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds,
                                                const SpD* __restrict__ Ss, const Tile* __restrict__ tiles, int do_bin,
                                                int check) {
-  __shared__ SbTables G;
+  __shared__ SbTables<SB_CT> G;
   __shared__ SbSpheres sph;
   const Tile tl = tiles[blockIdx.x];
   if (tl.begin >= tl.end) return;
@@ -4856,9 +4951,10 @@ __global__ __launch_bounds__(256) void sb_near(const CoP* __restrict__ Ps, const
 }
 
 // Tile of cells of one system, once every particle is binned: the record of
-// collision cell t (linear index; mean velocity from the exact sums of the
-// XCD copies, random axis of mpc_rotation_mod.f95:184-194) with its sums
-// cleared; the bulk B->A reaction of unshifted cell t (reaction_mod.f95:28-66:
+// the collision cell at tour position t (mean velocity from the exact sums of
+// its owner tile and of the XCD copies, random axis of
+// mpc_rotation_mod.f95:184-194) with the copies cleared; the bulk B->A
+// reaction of unshifted cell t (linear index; reaction_mod.f95:28-66:
 // p from the host table of 1 - exp(-k1 nB tau), one draw per cell, the first
 // B member in census order turned A) with its B census cleared.  Split
 // storage keeps the census of B only: nothing else reads the A and S counts.
@@ -4875,7 +4971,8 @@ __device__ __forceinline__ void sb_cell(const CoP& P, const CoD& D, const SpD& S
   const unsigned xm = D.sc->xmask;   // copies filled this interval
   if (t == 0) D.sc->far_maxv2 = 0ull;   // sb_rotate publishes the next interval's far maximum
   if (t < P.mpcnumcell) {
-    long long sx = 0, sy = 0, sz = 0;
+    const gp<const unsigned long long> own = g1((const unsigned long long*)D.sown) + 3 * t;
+    long long sx = (long long)own[0], sy = (long long)own[1], sz = (long long)own[2];
     for (unsigned m = xm; m; m &= m - 1) {
       const gp<unsigned long long> a = g1(D.sacc + (long long)(__ffs(m) - 1) * 3 * P.mpcnumcell + 3 * t);
       sx += (long long)a[0]; sy += (long long)a[1]; sz += (long long)a[2];
@@ -4889,8 +4986,8 @@ __device__ __forceinline__ void sb_cell(const CoP& P, const CoD& D, const SpD& S
       cy = (float)(((double)sy / SB_SCALE) / dn);
       cz = (float)(((double)((sz - n) >> SB_NBITS) / SB_SCALE) / dn);
     }
-    double u1, u2;
-    philox_uniform2(P.seed, 1, D.sc->coll, (unsigned long long)t, u1, u2);
+    double u1, u2;   // the draw is keyed by the collision cell itself
+    philox_uniform2(P.seed, 1, D.sc->coll, (unsigned long long)g1(D.mpc_cell)[t], u1, u2);
     const double phi = P.twopi * u1, th = 2.0 * u2 - 1.0, st = sqrt(1.0 - th * th);
     double sp, cp;
     sincos(phi, &sp, &cp);
@@ -5074,7 +5171,7 @@ struct mpcx_cob {
   Tile *d_pt = nullptr, *d_ct = nullptr, *d_vt = nullptr, *d_gt = nullptr, *d_st = nullptr;
   int n_pt = 0, n_ct = 0, n_vt = 0, n_st = 0, tile = 2048;
   int cell_tile = 256;   // cells per workgroup of sb_cells (a multiple of 256)
-  int bulk_threads = 512;   // workgroup size of sb_bulk (256 or 512; 512 measured faster)
+  int bulk_slots = 0;   // cell-table slots of sb_bulk: 1024, 2048, or 0 (2048 for tiles of 4096 and more)
   // split replicas: particle tiles of sb_bulk and near-set tiles of sb_near
   // placed on the XCDs (workgroup b runs on XCD b % 8): each replica on one
   // XCD when there are enough replicas to balance them, so its sums fill one
@@ -5336,16 +5433,17 @@ static int cob_sort_interval(mpcx_cob* b) {
   return 0;
 }
 
-// sb_bulk over the XCD-placed particle tiles with b->bulk_threads per workgroup
-static void sb_launch_bulk(mpcx_cob* b, bool all, hipStream_t s, int bin) {
+// sb_bulk over the XCD-placed particle tiles, 512 threads per workgroup
+template <int GT>
+static void sb_launch_bulk_t(mpcx_cob* b, bool all, hipStream_t s, int bin) {
   const dim3 g(b->n_xt);
-  if (b->bulk_threads == 512) {
-    if (all) hipLaunchKernelGGL((sb_bulk<true, 512>), g, dim3(512), 0, s, b->d_P, b->d_D, b->d_S, b->d_xt, bin);
-    else hipLaunchKernelGGL((sb_bulk<false, 512>), g, dim3(512), 0, s, b->d_P, b->d_D, b->d_S, b->d_xt, bin);
-  } else {
-    if (all) hipLaunchKernelGGL((sb_bulk<true, 256>), g, dim3(256), 0, s, b->d_P, b->d_D, b->d_S, b->d_xt, bin);
-    else hipLaunchKernelGGL((sb_bulk<false, 256>), g, dim3(256), 0, s, b->d_P, b->d_D, b->d_S, b->d_xt, bin);
-  }
+  if (all) hipLaunchKernelGGL((sb_bulk<true, 512, GT>), g, dim3(512), 0, s, b->d_P, b->d_D, b->d_S, b->d_xt, bin);
+  else hipLaunchKernelGGL((sb_bulk<false, 512, GT>), g, dim3(512), 0, s, b->d_P, b->d_D, b->d_S, b->d_xt, bin);
+}
+static void sb_launch_bulk(mpcx_cob* b, bool all, hipStream_t s, int bin) {
+  const int slots = b->bulk_slots ? b->bulk_slots : (b->tile >= 4096 ? 2048 : 1024);
+  if (slots == 2048) sb_launch_bulk_t<2048>(b, all, s, bin);
+  else sb_launch_bulk_t<1024>(b, all, s, bin);
 }
 
 // One split-precision interval of every replica (the kernels above).  With
@@ -5654,6 +5752,16 @@ extern "C" int mpcx_cob_info(mpcx_cob* b, int64_t* particles, int64_t* cells, do
   if (particles) *particles = b->particles;
   if (cells) *cells = b->cells;
   if (run_ms) *run_ms = b->run_ms;
+  return 0;
+}
+
+// cell-table slots of the split binning (1024, 2048, or 0: by the tile); drops a captured graph
+extern "C" int mpcx_cob_set_bulk_slots(mpcx_cob* b, int32_t slots) {
+  if (!b || (slots != 0 && slots != 1024 && slots != 2048))
+    return set_error(MPCX_E_ARG, "bulk slots must be 0, 1024 or 2048");
+  HIPCHK(hipStreamSynchronize(b->st));
+  cob_free_graph(b);
+  b->bulk_slots = slots;
   return 0;
 }
 

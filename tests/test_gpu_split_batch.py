@@ -54,8 +54,11 @@ REPLICAS = [(20, 0.1, 1.0), (26, 0.5, 0.6), (23, 1.0, 0.3), (30, 0.2, 0.9)]
 KEYS = ("x", "y", "z", "vx", "vy", "vz", "flag", "xd", "yd", "zd", "vxd", "vyd", "vzd", "fxd")
 
 
-@pytest.mark.parametrize("use_graph", [True, False])
-def test_split_batch_replicas_equal_single_runs(use_graph):
+# (graph, tile, binning table slots): 1024 slots by default below 4096-particle
+# tiles, 2048 from there; the single runs use 2048-particle tiles (1024 slots)
+@pytest.mark.parametrize("use_graph,tile,slots", [(True, 512, 0), (False, 512, 0), (True, 4096, 0),
+                                                  (False, 1024, 2048)])
+def test_split_batch_replicas_equal_single_runs(use_graph, tile, slots):
     ncoll = 5   # re-sorts at the 2nd and 4th collision
     ps = [_params(L, EbN, RPab, sort_step=100) for L, EbN, RPab in REPLICAS]
     alone = []
@@ -65,8 +68,10 @@ def test_split_batch_replicas_equal_single_runs(use_graph):
         alone.append(e.get_state())
         e.close()
     engines = [_engine(p, 70 + i) for i, p in enumerate(ps)]
-    batch = CollectiveBatch(engines, tile=512)
+    batch = CollectiveBatch(engines, tile=tile)
     try:
+        if slots:
+            batch.set_bulk_slots(slots)
         batch.run(ncoll, use_graph=use_graph)
         got = [e.get_state() for e in engines]
     finally:

@@ -37,7 +37,7 @@ def test_cell_tour_is_permutation(L):
     assert np.array_equal(np.sort(t), np.arange(1, L[0] * L[1] * L[2] + 1))
 
 
-def test_config_sizes_match_survey_counts():
+def test_config_sizes_match_reference_counts():
     # main_pattern.f95:66-73 with Rc=2, Rn=4, rho=10 (the counts quoted for the five configurations)
     for L, NT in (((30, 30, 30), 266984), ((100, 100, 100), 9996984), ((50, 50, 50), 1246984)):
         p = S.collective_params(Lx=float(L[0]), Ly=float(L[1]), Lz=float(L[2]), rhoa=10.0)
