@@ -308,8 +308,9 @@ def ensemble_engines(R, seed, split):
 
 def split_batched(R=256, steps=20, warmup=4, seed=4357787, fp64_value=None):
     """The headline ensemble mode: the replicas in split-precision storage,
-    advanced by one batch (HIP-graph replay, autotuned tile and cells per
-    workgroup).  Timed over `steps` intervals (two re-sorts in 20); the kernel
+    advanced by one batch (HIP-graph replay, autotuned tile -- which sets the
+    cells per workgroup of the cell pass -- and binning table slots).  Timed
+    over `steps` intervals (two re-sorts in 20); the kernel
     means come from a further run without the graph (HIP events)."""
     from main_amd.batch import CollectiveBatch
     from main_amd._lib import device_synchronize
@@ -318,7 +319,7 @@ def split_batched(R=256, steps=20, warmup=4, seed=4357787, fp64_value=None):
     print(f"# split batched: {R} replicas built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     batch = CollectiveBatch(engines, tile=4096)
     info = batch.info()
-    times = batch.autotune(tiles=(2048, 4096, 8192), cell_tiles=(256, 1024), slots=(1024, 2048), ncoll=10)
+    times = batch.autotune(tiles=(2048, 4096, 8192), slots=(1024, 2048), ncoll=10)
     batch.run(warmup)
     device_synchronize()
     t0 = time.perf_counter()
@@ -340,7 +341,7 @@ def split_batched(R=256, steps=20, warmup=4, seed=4357787, fp64_value=None):
           flush=True)
     out = {"replicas": R, "particles": n, "collision_cells": info["cells"], "ms_per_interval": ms,
            "particle_updates_per_s": value, "replicas_per_hour_2000_collisions": R * 3600.0 / (2000 * ms * 1e-3),
-           "tile": batch.tile, "cells_per_workgroup": batch.cell_tile, "bulk_slots": batch.bulk_slots,
+           "tile": batch.tile, "bulk_slots": batch.bulk_slots,
            "autotune_ms_per_interval": {"x".join(map(str, k)): v for k, v in times.items()},
            "roofline_step": step_roofline(SPLIT_STEP_BYTES, n, ms), "kernel_ms": kms,
            "kernels": kernels, "rng": "device Philox4x32-10", "graph": True,

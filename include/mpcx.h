@@ -310,9 +310,6 @@ int mpcx_cob_create(mpcx_cob** out, int32_t R, mpcx_co* const* engines, int32_t 
 int mpcx_cob_destroy(mpcx_cob* b);
 int mpcx_cob_run(mpcx_cob* b, int64_t ncoll, int32_t use_graph);
 int mpcx_cob_set_tile(mpcx_cob* b, int32_t tile);
-/* Collision cells per workgroup of the split-precision cell kernel (a
- * multiple of 256; default 256).  Results do not depend on it. */
-int mpcx_cob_set_cell_tile(mpcx_cob* b, int32_t cells);
 /* Collision-cell slots of the per-workgroup hash table of the split-precision
  * binning: 1024, 2048, or 0 (default: 2048 for tiles of 4096 particles and
  * more, else 1024).  Results do not depend on it. */

@@ -2,9 +2,10 @@
 mpcx_cob_* in include/mpcx.h): independent replicas with their own boxes,
 parameters and Philox keys, one launch per kernel of an interval for all of
 them, the normal intervals replayed from a HIP graph; FP64 or split-precision
-replicas.  The tile size of the particle kernels (and, for split replicas,
-the cells per workgroup of the cell kernel and the slots of the binning's
-hash table) is chosen by `autotune`, which times short runs.
+replicas.  The tile size of the particle kernels (for split replicas it also
+sets the cells per workgroup of the cell pass: each particle tile's owned
+cells, about a tenth as many) and the slots of the split binning's hash
+table are chosen by `autotune`, which times short runs.
 """
 from __future__ import annotations
 
@@ -23,7 +24,6 @@ class CollectiveBatch:
               "mpcx_cob_create")
         track(self, first=True)
         self.tile = tile
-        self.cell_tile = 256
         self.bulk_slots = 0
 
     def close(self):
@@ -75,11 +75,6 @@ class CollectiveBatch:
               "mpcx_cob_body_average")
         return counts.astype(np.float64) / max(n, 1), vsum / max(n, 1)
 
-    def set_cell_tile(self, cells: int):
-        """Collision cells per workgroup of the split-precision cell kernel."""
-        check(lib().mpcx_cob_set_cell_tile(self.h, C.c_int32(cells)), "mpcx_cob_set_cell_tile")
-        self.cell_tile = cells
-
     def set_bulk_slots(self, slots: int):
         """Collision-cell slots of the split binning's hash table (1024, 2048,
         or 0 for the default by tile size)."""
@@ -87,35 +82,25 @@ class CollectiveBatch:
         self.bulk_slots = slots
 
     def autotune(self, tiles=(512, 1024, 2048, 4096, 8192), ncoll: int = 4, use_graph: bool = True,
-                 cell_tiles=None, slots=None) -> dict:
-        """Time `ncoll` intervals at each tile size -- and, given cell_tiles
-        and slots, at each number of cells per workgroup of the cell kernel
-        and each table size of the binning -- advancing the systems, and keep
-        the fastest; returns {tile: ms per interval}, or {(tile, cells[,
-        slots]): ms per interval}.  With re-sorts every k intervals, ncoll = k
-        gives every setting one re-sort and the same spread of particle
-        disorder."""
+                 slots=None) -> dict:
+        """Time `ncoll` intervals at each tile size -- and, given slots, at
+        each table size of the split binning -- advancing the systems, and
+        keep the fastest; returns {tile: ms per interval}, or {(tile, slots):
+        ms per interval}.  With re-sorts every k intervals, ncoll = k gives
+        every setting one re-sort and the same spread of particle disorder."""
         times = {}
         for t in tiles:
             self.set_tile(t)
-            for c in (cell_tiles or (None,)):
-                if c is not None:
-                    self.set_cell_tile(c)
-                for g in (slots or (None,)):
-                    if g is not None:
-                        self.set_bulk_slots(g)
-                    self.run(1, use_graph)          # capture / warm
-                    self.run(ncoll, use_graph)
-                    key = (t,) + ((c,) if c is not None else ()) + ((g,) if g is not None else ())
-                    times[key if len(key) > 1 else t] = self.info()["run_ms"] / ncoll
+            for g in (slots or (None,)):
+                if g is not None:
+                    self.set_bulk_slots(g)
+                self.run(1, use_graph)          # capture / warm
+                self.run(ncoll, use_graph)
+                times[t if g is None else (t, g)] = self.info()["run_ms"] / ncoll
         best = min(times, key=times.get)
         if isinstance(best, tuple):
             self.set_tile(best[0])
-            rest = list(best[1:])
-            if cell_tiles:
-                self.set_cell_tile(rest.pop(0))
-            if slots:
-                self.set_bulk_slots(rest.pop(0))
+            self.set_bulk_slots(best[1])
         else:
             self.set_tile(best)
         return times

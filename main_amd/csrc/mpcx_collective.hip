@@ -166,8 +166,8 @@ struct CoD {
   unsigned long long* sown;
   const int* soff;        // [vol + 1] first particle of each tour position at the last re-sort
   float4* srec;
-  int* xtc;               // split storage B census, one copy per XCD: [8][vol]
-  unsigned* xfb;          // split storage first-B keys, one copy per XCD: [8][vol]
+  int* xtc;               // split storage B census, one copy per XCD: [8][vol], by tour position
+  unsigned* xfb;          // split storage first-B keys, one copy per XCD: [8][vol], by tour position
 };
 
 // ------------------------------------------------------------- streaming
@@ -4542,6 +4542,7 @@ __device__ __forceinline__ long long sb_fix(float v) { return __double2ll_rn((do
 // at most about half full.
 constexpr int SB_CT = 1024;    // census slots
 constexpr int SB_PROBE = 64;
+constexpr int SB_DEST = 64;    // owner tiles a tile sends foreign sums to (beyond: the XCD copies)
 template <int GT>
 struct SbTables {
   static_assert((GT & (GT - 1)) == 0 && GT >= SB_CT, "slots: a power of two, at least the census slots");
@@ -4550,12 +4551,14 @@ struct SbTables {
   unsigned long long rs[3][GT];
   int ck[SB_CT], cc[SB_CT];
   unsigned cm[SB_CT];
+  int dk[SB_DEST], dn[SB_DEST], db[SB_DEST];   // owner tile, entries for it, first inbox slot claimed
 };
 template <int GT>
 __device__ __forceinline__ void sb_tables_init(SbTables<GT>& G) {
   for (int j = threadIdx.x; j < GT; j += blockDim.x) {
     G.rk[j] = -1; G.rs[0][j] = 0ull; G.rs[1][j] = 0ull; G.rs[2][j] = 0ull;
     if (j < SB_CT) { G.ck[j] = -1; G.cc[j] = 0; G.cm[j] = 0xffffffffu; }
+    if (j < SB_DEST) { G.dk[j] = -1; G.dn[j] = 0; }
   }
   __syncthreads();
 }
@@ -4614,6 +4617,23 @@ __device__ __forceinline__ int sb_find(const int* keys, int key) {
   }
   return -1;
 }
+// The sums a tile binned into tour cells that another tile owns go to that
+// owner's inbox: one slot claimed per entry with a counter atomic, the entry
+// written with plain stores; the owner's cell pass merges them (integer sums:
+// the order of arrival does not matter).  Every atomic on device memory runs
+// in the memory fabric here, not in L2, so three per cell were the binning's
+// largest cost.
+struct FxEntry {
+  int p, pad;
+  unsigned long long w[3];
+};
+struct SbInbox {
+  const int* tb;   // [R] index of each system's first particle tile among all tiles
+  int* cnt;        // [tiles] entries appended this interval
+  FxEntry* box;    // [tiles][cap]
+  int2* rng;       // [tiles] owned tour positions [x, y) (set after every re-sort)
+  int T, cap;      // particles per tile, entries per inbox
+};
 // Cell ownership of a particle tile [b, e) of a system of NT particles: the
 // tour positions [own_bound(b), own_bound(e)) whose first particle at the last
 // re-sort (soff, non-decreasing) lies in the tile, the first tile's range from
@@ -4630,13 +4650,25 @@ __device__ __forceinline__ int own_bound(const int* soff, long long vol, long lo
   }
   return (int)lo;
 }
+// the owned tour positions of every particle tile of the list (after a
+// re-sort, and at the start of a run)
+__global__ __launch_bounds__(256) void sb_own_ranges(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds,
+                                                     const Tile* __restrict__ tiles, int n, SbInbox X) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const Tile tl = tiles[i];
+  if (tl.begin >= tl.end) return;
+  const long long vol = Ps[tl.r].vol, NT = Ps[tl.r].NT;
+  const int* soff = Ds[tl.r].soff;
+  X.rng[X.tb[tl.r] + (int)(tl.begin / X.T)] = make_int2(own_bound(soff, vol, tl.begin, NT), own_bound(soff, vol, tl.end, NT));
+}
 // every thread of the block together, after the tile: the sums of the cells
 // the tile owns (tour positions [plo, phi), one contiguous run of sown)
 // stored whole -- zero for an owned cell it binned nothing into -- and the
 // other table entries added to the workgroup's XCD copy
 template <int GT>
 __device__ __forceinline__ void sb_flush_owned(const CoP& P, const CoD& D, const SbV& A, SbTables<GT>& G, int plo,
-                                               int phi) {
+                                               int phi, const SbInbox& X, int r) {
   __syncthreads();
   const gp<unsigned long long> own = g1(D.sown);
   for (int p = plo + (int)threadIdx.x; p < phi; p += blockDim.x) {
@@ -4645,20 +4677,54 @@ __device__ __forceinline__ void sb_flush_owned(const CoP& P, const CoD& D, const
     if (j >= 0) { w0 = G.rs[0][j]; w1 = G.rs[1][j]; w2 = G.rs[2][j]; }
     own[3ll * p] = w0; own[3ll * p + 1] = w1; own[3ll * p + 2] = w2;
   }
-  for (int j = threadIdx.x; j < GT; j += blockDim.x) {
+  // the foreign entries grouped by owner tile: ranks within each group in
+  // LDS, one inbox claim per group, then the entries written at their slots
+  const int ntiles = (int)((P.NT + X.T - 1) / X.T);
+  constexpr int K = GT / 256;   // table slots per thread (at least 256 threads)
+  int ds[K], rk[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int j = threadIdx.x + k * blockDim.x;
+    ds[k] = -1; rk[k] = 0;
+    if (j < GT) {
+      const int t = G.rk[j];
+      if (t >= 0 && (t < plo || t >= phi) && t < P.vol) {   // owner: the tile holding the cell's first particle
+        const int d = sb_slot<SB_DEST>(G.dk, X.tb[r] + min(D.soff[t] / X.T, ntiles - 1));
+        if (d >= 0) { ds[k] = d; rk[k] = atomicAdd(&G.dn[d], 1); }
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < SB_DEST && G.dk[threadIdx.x] >= 0)
+    G.db[threadIdx.x] = atomicAdd(&X.cnt[G.dk[threadIdx.x]], G.dn[threadIdx.x]);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int j = threadIdx.x + k * blockDim.x;
+    if (j >= GT) continue;
     const int t = G.rk[j];
-    if (t >= 0) {
-      if (t < plo || t >= phi) {
+    if (t >= 0 && (t < plo || t >= phi)) {
+      bool sent = false;
+      if (ds[k] >= 0) {
+        const int idx = G.db[ds[k]] + rk[k];
+        if (idx < X.cap) {
+          FxEntry* e = X.box + (long long)G.dk[ds[k]] * X.cap + idx;
+          e->p = t;
+          e->w[0] = G.rs[0][j]; e->w[1] = G.rs[1][j]; e->w[2] = G.rs[2][j];
+          sent = true;
+        }
+      }
+      if (!sent) {   // a full inbox, too many owners or an unowned cell: the XCD copy
         const gp<unsigned long long> a = A.acc + 3ll * t;
         l2_add<unsigned long long>(a, G.rs[0][j]);
         l2_add<unsigned long long>(a + 1, G.rs[1][j]);
         l2_add<unsigned long long>(a + 2, G.rs[2][j]);
       }
     }
-    const int k = j < SB_CT ? G.ck[j] : -1;
-    if (k >= 0) {
-      l2_add<int>(A.tc + k, G.cc[j]);   // checked against CENSUS_MAX by sb_cells
-      l2_min(A.fb + k, G.cm[j]);
+    const int c = j < SB_CT ? G.ck[j] : -1;
+    if (c >= 0) {
+      l2_add<int>(A.tc + c, G.cc[j]);   // checked against CENSUS_MAX by sb_cells_own
+      l2_min(A.fb + c, G.cm[j]);
     }
   }
 }
@@ -4724,7 +4790,7 @@ __device__ __forceinline__ void sb_bin(const CoP& P, const CoD& D, const SbV& A,
     if (inside) key = (unsigned)P.NT + (unsigned)i;
     const long long u = (long long)ix + (long long)P.Lx * iy + (long long)P.Lx * P.Ly * iz;
     if (u < 0 || u >= P.vol) raise_error(&D.sc->err, ERR_CELL_RANGE);
-    else { cen = true; ck = (int)u; }
+    else { cen = true; ck = g1(D.mpc_pos)[u]; }   // by tour position, as the cell sums
   }
   sb_insert(P, A, G, ok, t, vx, vy, vz, cen, ck, key);
   if (valid) {
@@ -4805,7 +4871,8 @@ __device__ __forceinline__ void sb_fetch(const SbV& V, long long i, long long en
 // binned, so their latency overlaps the binning's LDS work and barriers.
 template <bool ALL, int T, int GT>
 __global__ __launch_bounds__(T) void sb_bulk(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds,
-                                             const SpD* __restrict__ Ss, const Tile* __restrict__ tiles, int do_bin) {
+                                             const SpD* __restrict__ Ss, const Tile* __restrict__ tiles, int do_bin,
+                                             SbInbox X) {
   __shared__ SbTables<GT> G;
   __shared__ SbSpheres sph;
   const Tile tl = tiles[blockIdx.x];
@@ -4816,13 +4883,8 @@ __global__ __launch_bounds__(T) void sb_bulk(const CoP* __restrict__ Ps, const C
   const int xcc = hw_xcc();
   const SbV V = sb_view(P, D, S, xcc);
   if (do_bin && threadIdx.x == 0) atomicOr(&D.sc->xmask, 1u << xcc);
-  __shared__ int own_lo, own_hi;
   if (ALL) sb_load_spheres(P, D.sc->x0, D.sc->y0, D.sc->z0, sph);
   else sb_load_spheres(P, D.sc->xs0, D.sc->ys0, D.sc->zs0, sph);
-  if (do_bin && threadIdx.x == 0) {
-    own_lo = own_bound(D.soff, P.vol, tl.begin, P.NT);
-    own_hi = own_bound(D.soff, P.vol, tl.end, P.NT);
-  }
   if (do_bin) sb_tables_init(G);
   else __syncthreads();
   double xs = 0, ys = 0, zs = 0;
@@ -4864,7 +4926,10 @@ __global__ __launch_bounds__(T) void sb_bulk(const CoP* __restrict__ Ps, const C
     if (do_bin)
       sb_bin(P, D, V, G, i, mine, inside, excl, ix, iy, iz, ox, oy, oz, vx, vy, vz, f, fxs, fys, fzs, zs, sph);
   }
-  if (do_bin) sb_flush_owned(P, D, V, G, own_lo, own_hi);
+  if (do_bin) {
+    const int2 own = X.rng[X.tb[tl.r] + (int)(tl.begin / X.T)];
+    sb_flush_owned(P, D, V, G, own.x, own.y, X, tl.r);
+  }
 }
 
 // Per system (blockIdx.y): the interval's start -- the collision counter and
@@ -4950,31 +5015,63 @@ __global__ __launch_bounds__(256) void sb_near(const CoP* __restrict__ Ps, const
   if (check && tl.begin == 0 && threadIdx.x == 0) check_flight(P, D);
 }
 
-// Tile of cells of one system, once every particle is binned: the record of
-// the collision cell at tour position t (mean velocity from the exact sums of
-// its owner tile and of the XCD copies, random axis of
-// mpc_rotation_mod.f95:184-194) with the copies cleared; the bulk B->A
-// reaction of unshifted cell t (linear index; reaction_mod.f95:28-66:
-// p from the host table of 1 - exp(-k1 nB tau), one draw per cell, the first
-// B member in census order turned A) with its B census cleared.  Split
-// storage keeps the census of B only: nothing else reads the A and S counts.
-__device__ __forceinline__ void sb_cell(const CoP& P, const CoD& D, const SpD& S, long long t);
-__global__ __launch_bounds__(256) void sb_cells(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds,
-                                                const SpD* __restrict__ Ss, const Tile* __restrict__ tiles) {
+// The cells of one system's particle tile, once every particle is binned:
+// those at the tour positions the tile owns (and, for its last tile, the
+// unowned extra z layer), OWNCAP at a time.  Per cell: the record of the
+// collision cell (mean velocity from the exact sums of its owner, of the
+// owner's inbox and of the XCD copies; random axis of
+// mpc_rotation_mod.f95:184-194) with the copies cleared; and, for an
+// unshifted cell, the bulk B->A reaction (reaction_mod.f95:28-66: p from the
+// host table of 1 - exp(-k1 nB tau), one draw per cell, the first B member in
+// census order turned A) with its B census cleared.  Split storage keeps the
+// census of B only: nothing else reads the A and S counts.
+constexpr int OWNCAP = 1024;
+__device__ __forceinline__ void sb_cell(const CoP& P, const CoD& D, const SpD& S, long long p, unsigned long long x0,
+                                        unsigned long long x1, unsigned long long x2);
+__global__ __launch_bounds__(256) void sb_cells_own(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds,
+                                                    const SpD* __restrict__ Ss, const Tile* __restrict__ tiles,
+                                                    SbInbox X) {
+  __shared__ unsigned long long acc[3][OWNCAP];
   const Tile tl = tiles[blockIdx.x];
+  if (tl.begin >= tl.end) return;
   const CoP P = Ps[tl.r];
   const CoD D = Ds[tl.r];
   const SpD S = Ss[tl.r];
-  for (long long t = tl.begin + threadIdx.x; t < tl.end; t += 256) sb_cell(P, D, S, t);
+  const int g = X.tb[tl.r] + (int)(tl.begin / X.T);
+  const int2 own = X.rng[g];
+  const long long plo = own.x, phi = own.y;
+  const long long hi = tl.end >= P.NT ? P.mpcnumcell : phi;
+  const int n = min(X.cnt[g], X.cap);
+  const FxEntry* box = X.box + (long long)g * X.cap;
+  if (plo == 0 && threadIdx.x == 0) D.sc->far_maxv2 = 0ull;   // sb_rotate publishes the next interval's far maximum
+  for (long long c0 = plo; c0 < hi; c0 += OWNCAP) {
+    const long long c1 = min(c0 + (long long)OWNCAP, hi);
+    for (int j = threadIdx.x; j < OWNCAP; j += 256) { acc[0][j] = 0ull; acc[1][j] = 0ull; acc[2][j] = 0ull; }
+    __syncthreads();
+    for (int e = threadIdx.x; e < n; e += 256) {
+      const long long q = box[e].p;
+      if (c0 == plo && (q < plo || q >= phi)) raise_error(&D.sc->err, ERR_CELL_RANGE);   // not this tile's cell
+      if (q >= c0 && q < c1) {
+        atomicAdd(&acc[0][q - c0], box[e].w[0]);
+        atomicAdd(&acc[1][q - c0], box[e].w[1]);
+        atomicAdd(&acc[2][q - c0], box[e].w[2]);
+      }
+    }
+    __syncthreads();
+    for (long long q = c0 + threadIdx.x; q < c1; q += 256)
+      sb_cell(P, D, S, q, acc[0][q - c0], acc[1][q - c0], acc[2][q - c0]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) X.cnt[g] = 0;   // the inbox is read: empty for the next interval
 }
-__device__ __forceinline__ void sb_cell(const CoP& P, const CoD& D, const SpD& S, long long t) {
+__device__ __forceinline__ void sb_cell(const CoP& P, const CoD& D, const SpD& S, long long p, unsigned long long x0,
+                                        unsigned long long x1, unsigned long long x2) {
   const unsigned xm = D.sc->xmask;   // copies filled this interval
-  if (t == 0) D.sc->far_maxv2 = 0ull;   // sb_rotate publishes the next interval's far maximum
-  if (t < P.mpcnumcell) {
-    const gp<const unsigned long long> own = g1((const unsigned long long*)D.sown) + 3 * t;
-    long long sx = (long long)own[0], sy = (long long)own[1], sz = (long long)own[2];
+  if (p < P.mpcnumcell) {
+    const gp<const unsigned long long> own = g1((const unsigned long long*)D.sown) + 3 * p;
+    long long sx = (long long)(own[0] + x0), sy = (long long)(own[1] + x1), sz = (long long)(own[2] + x2);
     for (unsigned m = xm; m; m &= m - 1) {
-      const gp<unsigned long long> a = g1(D.sacc + (long long)(__ffs(m) - 1) * 3 * P.mpcnumcell + 3 * t);
+      const gp<unsigned long long> a = g1(D.sacc + (long long)(__ffs(m) - 1) * 3 * P.mpcnumcell + 3 * p);
       sx += (long long)a[0]; sy += (long long)a[1]; sz += (long long)a[2];
       a[0] = 0ull; a[1] = 0ull; a[2] = 0ull;
     }
@@ -4987,25 +5084,26 @@ __device__ __forceinline__ void sb_cell(const CoP& P, const CoD& D, const SpD& S
       cz = (float)(((double)((sz - n) >> SB_NBITS) / SB_SCALE) / dn);
     }
     double u1, u2;   // the draw is keyed by the collision cell itself
-    philox_uniform2(P.seed, 1, D.sc->coll, (unsigned long long)g1(D.mpc_cell)[t], u1, u2);
+    philox_uniform2(P.seed, 1, D.sc->coll, (unsigned long long)g1(D.mpc_cell)[p], u1, u2);
     const double phi = P.twopi * u1, th = 2.0 * u2 - 1.0, st = sqrt(1.0 - th * th);
     double sp, cp;
     sincos(phi, &sp, &cp);
     const gp<f4v> rec = g1((f4v*)D.srec);
-    rec[2 * t] = f4v{cx, cy, cz, (float)n};
-    rec[2 * t + 1] = f4v{(float)(st * cp), (float)(st * sp), (float)th, 0.0f};
+    rec[2 * p] = f4v{cx, cy, cz, (float)n};
+    rec[2 * p + 1] = f4v{(float)(st * cp), (float)(st * sp), (float)th, 0.0f};
   }
-  if (t < P.vol) {
+  if (p < P.vol) {   // the unshifted cell at the same tour position (census by tour position)
+    const long long t = g1(D.mpc_cell)[p];
     int nb = 0;
     unsigned first = 0xffffffffu;
     for (unsigned m = xm; m; m &= m - 1) {
       const int x = __ffs(m) - 1;
       const gp<int> tc = g1(D.xtc + (long long)x * P.vol);
       const gp<unsigned> fb = g1(D.xfb + (long long)x * P.vol);
-      nb += tc[t];
-      first = min(first, fb[t]);
-      tc[t] = 0;
-      fb[t] = 0xffffffffu;
+      nb += tc[p];
+      first = min(first, fb[p]);
+      tc[p] = 0;
+      fb[p] = 0xffffffffu;
     }
     if (nb > CENSUS_MAX) raise_error(&D.sc->err, ERR_MEMBERS);   // the reference's MaxB
     if (t < P.rxnnumcell) {
@@ -5168,9 +5266,8 @@ struct mpcx_cob {
   CoP* d_P = nullptr;
   CoD* d_D = nullptr;
   std::vector<CoD> h_D;
-  Tile *d_pt = nullptr, *d_ct = nullptr, *d_vt = nullptr, *d_gt = nullptr, *d_st = nullptr;
-  int n_pt = 0, n_ct = 0, n_vt = 0, n_st = 0, tile = 2048;
-  int cell_tile = 256;   // cells per workgroup of sb_cells (a multiple of 256)
+  Tile *d_pt = nullptr, *d_ct = nullptr, *d_vt = nullptr, *d_gt = nullptr;
+  int n_pt = 0, n_ct = 0, n_vt = 0, tile = 2048;
   int bulk_slots = 0;   // cell-table slots of sb_bulk: 1024, 2048, or 0 (2048 for tiles of 4096 and more)
   // split replicas: particle tiles of sb_bulk and near-set tiles of sb_near
   // placed on the XCDs (workgroup b runs on XCD b % 8): each replica on one
@@ -5179,6 +5276,13 @@ struct mpcx_cob {
   Tile *d_xt = nullptr, *d_nt = nullptr;
   int n_xt = 0, n_nt = 0;
   std::vector<int> xcd_of;   // XCD of each replica (-1: spread)
+  // split replicas: each particle tile's inbox of sums binned by other tiles
+  // into the cells it owns (tile g of replica r: tb[r] + first particle / tile)
+  int* d_tb = nullptr;
+  int* d_icnt = nullptr;
+  FxEntry* d_ibox = nullptr;
+  int2* d_orng = nullptr;
+  int n_tiles = 0, icap = 0;
   CoSortD* d_X = nullptr;   // sort buffers of each replica
   std::vector<CoSortD> h_X;
   // split storage: the replicas' split arrays and their second (sort) buffers
@@ -5230,16 +5334,13 @@ __global__ void cob_errs_k(const CoD* __restrict__ D, int R, int* __restrict__ i
 }
 
 static int cob_tiles(mpcx_cob* b) {
-  std::vector<Tile> pt, ct, vt, sc;
+  std::vector<Tile> pt, ct, vt;
   for (int r = 0; r < b->R; r++) {
     const CoP& P = b->eng[r]->P;
     for (long long i = 0; i < P.NT; i += b->tile) pt.push_back(Tile{r, 0, i, std::min<long long>(i + b->tile, P.NT)});
     for (long long t = 0; t < P.mpcnumcell; t += CT)
       ct.push_back(Tile{r, 0, t, std::min<long long>(t + CT, P.mpcnumcell)});
     for (long long t = 0; t < P.vol; t += 256) vt.push_back(Tile{r, 0, t, std::min<long long>(t + 256, P.vol)});
-    const long long nc = std::max(P.mpcnumcell, P.vol);
-    for (long long t = 0; t < nc; t += b->cell_tile)
-      sc.push_back(Tile{r, 0, t, std::min<long long>(t + b->cell_tile, nc)});
   }
   // workgroup b runs on XCD b % 8: lay the tiles out so that each XCD's
   // workgroups take one contiguous run of tiles
@@ -5294,7 +5395,7 @@ static int cob_tiles(mpcx_cob* b) {
         for (int k = 0; k < K; k++) nt.push_back(Tile{r, 0, k, K});
     }
   }
-  for (Tile** d : {&b->d_pt, &b->d_ct, &b->d_vt, &b->d_gt, &b->d_st, &b->d_xt, &b->d_nt})
+  for (Tile** d : {&b->d_pt, &b->d_ct, &b->d_vt, &b->d_gt, &b->d_xt, &b->d_nt})
     if (*d) { (void)hipFree(*d); *d = nullptr; }
   HIPCHK(hipMalloc(&b->d_xt, xt.size() * sizeof(Tile)));
   HIPCHK(hipMalloc(&b->d_nt, nt.size() * sizeof(Tile)));
@@ -5306,17 +5407,38 @@ static int cob_tiles(mpcx_cob* b) {
   HIPCHK(hipMalloc(&b->d_gt, gt.size() * sizeof(Tile)));
   HIPCHK(hipMalloc(&b->d_ct, ct.size() * sizeof(Tile)));
   HIPCHK(hipMalloc(&b->d_vt, vt.size() * sizeof(Tile)));
-  HIPCHK(hipMalloc(&b->d_st, sc.size() * sizeof(Tile)));
   HIPCHK(hipMemcpy(b->d_gt, gt.data(), gt.size() * sizeof(Tile), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(b->d_pt, pt.data(), pt.size() * sizeof(Tile), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(b->d_ct, ct.data(), ct.size() * sizeof(Tile), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(b->d_vt, vt.data(), vt.size() * sizeof(Tile), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(b->d_st, sc.data(), sc.size() * sizeof(Tile), hipMemcpyHostToDevice));
   b->n_pt = (int)pt.size();
   b->n_ct = (int)ct.size();
   b->n_vt = (int)vt.size();
-  b->n_st = (int)sc.size();
+  if (b->split) {   // the inboxes: max(tile / 2, 256) entries per particle tile
+    for (void** d : {(void**)&b->d_tb, (void**)&b->d_icnt, (void**)&b->d_ibox, (void**)&b->d_orng})
+      if (*d) { (void)hipFree(*d); *d = nullptr; }
+    std::vector<int> tb(b->R);
+    long long n = 0;
+    for (int r = 0; r < b->R; r++) {
+      tb[r] = (int)n;
+      n += (b->eng[r]->P.NT + b->tile - 1) / b->tile;
+    }
+    b->icap = std::max(b->tile / 2, 256);
+    if (n * (long long)b->icap > (1ll << 31)) return set_error(MPCX_E_ARG, "too many particle tiles for the inboxes");
+    b->n_tiles = (int)n;
+    HIPCHK(hipMalloc(&b->d_tb, b->R * sizeof(int)));
+    HIPCHK(hipMalloc(&b->d_icnt, n * sizeof(int)));
+    HIPCHK(hipMalloc(&b->d_ibox, n * (size_t)b->icap * sizeof(FxEntry)));
+    HIPCHK(hipMalloc(&b->d_orng, n * sizeof(int2)));
+    HIPCHK(hipMemcpy(b->d_tb, tb.data(), b->R * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(b->d_icnt, 0, n * sizeof(int)));
+  }
   return 0;
+}
+static SbInbox cob_inbox(const mpcx_cob* b) {
+  SbInbox X;
+  X.tb = b->d_tb; X.cnt = b->d_icnt; X.box = b->d_ibox; X.rng = b->d_orng; X.T = b->tile; X.cap = b->icap;
+  return X;
 }
 
 static int cob_upload_D(mpcx_cob* b) {
@@ -5380,7 +5502,11 @@ static int cob_sort_all(mpcx_cob* b) {
     }
     std::swap(D.status, e->alt_status); std::swap(D.near_bits, e->alt_bits);
   }
-  return cob_upload_D(b);
+  const int rc = cob_upload_D(b);
+  if (rc == 0 && b->split)   // the cells each particle tile owns in the new order
+    hipLaunchKernelGGL(sb_own_ranges, dim3((b->n_xt + 255) / 256), dim3(256), 0, b->st, b->d_P, b->d_D, b->d_xt,
+                       b->n_xt, cob_inbox(b));
+  return rc;
 }
 
 static hipError_t cob_launch_near(mpcx_cob* b, int nsteps, int mode) {
@@ -5437,8 +5563,9 @@ static int cob_sort_interval(mpcx_cob* b) {
 template <int GT>
 static void sb_launch_bulk_t(mpcx_cob* b, bool all, hipStream_t s, int bin) {
   const dim3 g(b->n_xt);
-  if (all) hipLaunchKernelGGL((sb_bulk<true, 512, GT>), g, dim3(512), 0, s, b->d_P, b->d_D, b->d_S, b->d_xt, bin);
-  else hipLaunchKernelGGL((sb_bulk<false, 512, GT>), g, dim3(512), 0, s, b->d_P, b->d_D, b->d_S, b->d_xt, bin);
+  const SbInbox X = cob_inbox(b);
+  if (all) hipLaunchKernelGGL((sb_bulk<true, 512, GT>), g, dim3(512), 0, s, b->d_P, b->d_D, b->d_S, b->d_xt, bin, X);
+  else hipLaunchKernelGGL((sb_bulk<false, 512, GT>), g, dim3(512), 0, s, b->d_P, b->d_D, b->d_S, b->d_xt, bin, X);
 }
 static void sb_launch_bulk(mpcx_cob* b, bool all, hipStream_t s, int bin) {
   const int slots = b->bulk_slots ? b->bulk_slots : (b->tile >= 4096 ? 2048 : 1024);
@@ -5500,7 +5627,7 @@ static int sb_interval(mpcx_cob* b, bool sort_due, long long slot) {
     sb_launch_bulk(b, true, st, 1);
   }
   HIPCHK(tmark(2, 0, st));
-  hipLaunchKernelGGL(sb_cells, dim3(b->n_st), dim3(256), 0, st, b->d_P, b->d_D, b->d_S, b->d_st);
+  hipLaunchKernelGGL(sb_cells_own, dim3(b->n_xt), dim3(256), 0, st, b->d_P, b->d_D, b->d_S, b->d_xt, cob_inbox(b));
   HIPCHK(tmark(2, 1, st));
   HIPCHK(tmark(3, 0, st));
   hipLaunchKernelGGL(sb_rotate, dim3(b->n_pt), dim3(256), 0, st, b->d_P, b->d_D, b->d_S, b->d_pt, sort_due ? 1 : 2);
@@ -5521,8 +5648,8 @@ extern "C" int mpcx_cob_destroy(mpcx_cob* b) {
   if (b->st2) (void)hipStreamSynchronize(b->st2);
   cob_free_graph(b);
   for (void* p : {(void*)b->d_P, (void*)b->d_D, (void*)b->d_pt, (void*)b->d_ct, (void*)b->d_vt, (void*)b->d_gt,
-                  (void*)b->d_st, (void*)b->d_xt, (void*)b->d_nt, (void*)b->d_X, (void*)b->d_S, (void*)b->d_N,
-                  (void*)b->d_io})
+                  (void*)b->d_xt, (void*)b->d_nt, (void*)b->d_X, (void*)b->d_S, (void*)b->d_N, (void*)b->d_io,
+                  (void*)b->d_tb, (void*)b->d_icnt, (void*)b->d_ibox, (void*)b->d_orng})
     if (p) (void)hipFree(p);
   for (hipEvent_t ev : {b->e0, b->e1, b->fork, b->join})
     if (ev) (void)hipEventDestroy(ev);
@@ -5627,6 +5754,10 @@ static int cob_begin(mpcx_cob* b, int* cstep_out) {
     for (int r = 0; r < b->R; r++)
       big = std::max(big, NXCD * 3 * std::max(b->eng[r]->P.mpcnumcell, b->eng[r]->P.vol));
     hipLaunchKernelGGL(sb_clear, dim3(grid_for(big, 256, 1024), b->R), dim3(256), 0, b->st, b->d_P, b->d_D);
+    hipLaunchKernelGGL(fill_kernel<int>, dim3(grid_for(b->n_tiles, 256, 1024)), dim3(256), 0, b->st, b->d_icnt,
+                       (long long)b->n_tiles, 0);   // inboxes of an interval that ended in an error
+    hipLaunchKernelGGL(sb_own_ranges, dim3((b->n_xt + 255) / 256), dim3(256), 0, b->st, b->d_P, b->d_D, b->d_xt,
+                       b->n_xt, cob_inbox(b));
   }
   return 0;
 }
@@ -5726,16 +5857,6 @@ static int cob_run(mpcx_cob* b, int64_t ncoll, int32_t use_graph) {
 extern "C" int mpcx_cob_run(mpcx_cob* b, int64_t ncoll, int32_t use_graph) {
   if (!b || ncoll < 0) return set_error(MPCX_E_ARG, "bad arguments");
   return cob_run(b, ncoll, use_graph);
-}
-
-// cells per workgroup of the split cell kernel (the autotuner's second knob); drops a captured graph
-extern "C" int mpcx_cob_set_cell_tile(mpcx_cob* b, int32_t cells) {
-  if (!b || cells < 256 || cells % 256 != 0 || cells > 65536)
-    return set_error(MPCX_E_ARG, "cells per workgroup must be a multiple of 256, at most 65536");
-  HIPCHK(hipStreamSynchronize(b->st));
-  cob_free_graph(b);
-  b->cell_tile = cells;
-  return cob_tiles(b);
 }
 
 // mean kernel durations (ms) of the last split run without a graph
