@@ -309,9 +309,10 @@ def ensemble_engines(R, seed, split):
 def split_batched(R=256, steps=20, warmup=4, seed=4357787, fp64_value=None):
     """The headline ensemble mode: the replicas in split-precision storage,
     advanced by one batch (HIP-graph replay, autotuned tile -- which sets the
-    cells per workgroup of the cell pass -- and binning table slots).  Timed
-    over `steps` intervals (two re-sorts in 20); the kernel
-    means come from a further run without the graph (HIP events)."""
+    cells per workgroup of the cell pass -- binning table slots and re-sort
+    period).  Timed over at least `steps` intervals, a whole number of re-sort
+    periods (one re-sort in each); the kernel means come from a further run
+    without the graph (HIP events)."""
     from main_amd.batch import CollectiveBatch
     from main_amd._lib import device_synchronize
     t0 = time.time()
@@ -320,6 +321,8 @@ def split_batched(R=256, steps=20, warmup=4, seed=4357787, fp64_value=None):
     batch = CollectiveBatch(engines, tile=4096)
     info = batch.info()
     times = batch.autotune(tiles=(2048, 4096, 8192), slots=(1024, 2048), ncoll=10)
+    periods = batch.tune_sort_period((10, 20, 40))
+    steps = -(-steps // batch.sort_period) * batch.sort_period
     batch.run(warmup)
     device_synchronize()
     t0 = time.perf_counter()
@@ -339,9 +342,11 @@ def split_batched(R=256, steps=20, warmup=4, seed=4357787, fp64_value=None):
         v["traffic"] = read_traffic(k, TRAFFIC_SPLIT_FILE)
     print(f"# split batched: {ms:.3f} ms/interval, {value:.3e} particle-updates/s, kernels {kms}", file=sys.stderr,
           flush=True)
-    out = {"replicas": R, "particles": n, "collision_cells": info["cells"], "ms_per_interval": ms,
+    out = {"replicas": R, "particles": n, "collision_cells": info["cells"], "timed_intervals": steps,
+           "ms_per_interval": ms,
            "particle_updates_per_s": value, "replicas_per_hour_2000_collisions": R * 3600.0 / (2000 * ms * 1e-3),
-           "tile": batch.tile, "bulk_slots": batch.bulk_slots,
+           "tile": batch.tile, "bulk_slots": batch.bulk_slots, "sort_every_collisions": batch.sort_period,
+           "sort_period_ms_per_interval": periods,
            "autotune_ms_per_interval": {"x".join(map(str, k)): v for k, v in times.items()},
            "roofline_step": step_roofline(SPLIT_STEP_BYTES, n, ms), "kernel_ms": kms,
            "kernels": kernels, "rng": "device Philox4x32-10", "graph": True,

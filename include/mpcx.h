@@ -314,6 +314,12 @@ int mpcx_cob_set_tile(mpcx_cob* b, int32_t tile);
  * binning: 1024, 2048, or 0 (default: 2048 for tiles of 4096 particles and
  * more, else 1024).  Results do not depend on it. */
 int mpcx_cob_set_bulk_slots(mpcx_cob* b, int32_t slots);
+/* Re-sort period of a split-precision batch in MD steps (a positive multiple
+ * of MPCtstep; default: the engines' sort_step).  Split storage sums cells in
+ * exact integers and keys every draw by cell, so the period changes only the
+ * memory locality and the order of the near engines' force sums; a replica
+ * matches its engine run alone at the same period.  FP64 batches refuse it. */
+int mpcx_cob_set_sort_step(mpcx_cob* b, int32_t steps);
 int mpcx_cob_info(mpcx_cob* b, int64_t* particles, int64_t* cells, double* run_ms);
 /* Split batches run without the graph: mean kernel durations (ms) of the last
  * run from HIP events on the stream each runs on (every 8th interval, at most

@@ -81,6 +81,27 @@ class CollectiveBatch:
         check(lib().mpcx_cob_set_bulk_slots(self.h, C.c_int32(slots)), "mpcx_cob_set_bulk_slots")
         self.bulk_slots = slots
 
+    def set_sort_step(self, steps: int):
+        """Re-sort period of a split batch in MD steps (a multiple of MPCtstep)."""
+        check(lib().mpcx_cob_set_sort_step(self.h, C.c_int32(steps)), "mpcx_cob_set_sort_step")
+        self.sort_step = steps
+
+    def tune_sort_period(self, periods=(10, 20, 40), use_graph: bool = True) -> dict:
+        """Split batches: time one whole re-sort period (p intervals, one
+        re-sort) at each period p in collisions and keep the fastest; returns
+        {p: ms per interval}."""
+        S = int(self.engines[0].p["MPCtstep"])
+        times = {}
+        for p in periods:
+            self.set_sort_step(p * S)
+            self.run(1, use_graph)   # capture / warm
+            self.run(p, use_graph)
+            times[p] = self.info()["run_ms"] / p
+        best = min(times, key=times.get)
+        self.set_sort_step(best * S)
+        self.sort_period = best
+        return times
+
     def autotune(self, tiles=(512, 1024, 2048, 4096, 8192), ncoll: int = 4, use_graph: bool = True,
                  slots=None) -> dict:
         """Time `ncoll` intervals at each tile size -- and, given slots, at

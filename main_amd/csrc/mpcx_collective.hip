@@ -5886,6 +5886,18 @@ extern "C" int mpcx_cob_set_bulk_slots(mpcx_cob* b, int32_t slots) {
   return 0;
 }
 
+// re-sort period of a split batch in MD steps (a positive multiple of
+// MPCtstep): split storage sums cells in exact integers and keys every draw
+// by cell, so the particle order only sets the memory locality (and the order
+// of the near engines' force sums); FP64 batches keep the engines' period,
+// the reference's summation order follows it
+extern "C" int mpcx_cob_set_sort_step(mpcx_cob* b, int32_t steps) {
+  if (!b || steps <= 0 || steps % b->S != 0) return set_error(MPCX_E_ARG, "the re-sort period must be a multiple of MPCtstep");
+  if (!b->split) return set_error(MPCX_E_UNSUPPORTED, "FP64 batches re-sort at their engines' sort_step");
+  b->sort_step = steps;
+  return 0;
+}
+
 // re-tile the particle kernels (the autotuner's knob); drops a captured graph
 extern "C" int mpcx_cob_set_tile(mpcx_cob* b, int32_t tile) {
   if (!b || tile < 256 || tile % 256 != 0) return set_error(MPCX_E_ARG, "tile must be a positive multiple of 256");

@@ -84,6 +84,40 @@ def test_split_batch_replicas_equal_single_runs(use_graph, tile, slots):
             assert np.array_equal(g[k], a[k]), f"replica {i}: {k}"
 
 
+def test_split_batch_sort_period_knob_equals_engines_at_that_period():
+    """A split batch's re-sort period set through mpcx_cob_set_sort_step gives
+    the states of engines configured with that period run alone; FP64
+    batches refuse the knob."""
+    ncoll = 5   # re-sort at the 3rd collision (150 MD steps)
+    alone = []
+    for i, (L, EbN, RPab) in enumerate(REPLICAS[:3]):
+        e = _engine(_params(L, EbN, RPab, sort_step=150), 90 + i)
+        e.run(ncoll)
+        alone.append(e.get_state())
+        e.close()
+    engines = [_engine(_params(L, EbN, RPab, sort_step=100), 90 + i) for i, (L, EbN, RPab) in enumerate(REPLICAS[:3])]
+    batch = CollectiveBatch(engines, tile=1024)
+    try:
+        batch.set_sort_step(150)
+        batch.run(ncoll)
+        got = [e.get_state() for e in engines]
+    finally:
+        batch.close()
+        for e in engines:
+            e.close()
+    for i, (g, a) in enumerate(zip(got, alone)):
+        for k in KEYS:
+            assert np.array_equal(g[k], a[k]), f"replica {i}: {k}"
+    fp = [_engine(_params(20, sort_step=100), 95, split=False)]
+    b64 = CollectiveBatch(fp, tile=1024)
+    try:
+        with pytest.raises(Exception, match="sort_step"):
+            b64.set_sort_step(150)
+    finally:
+        b64.close()
+        fp[0].close()
+
+
 def _ensemble(R, seed):
     rng = np.random.default_rng(seed)
     return [(int(rng.integers(20, 41)), float(rng.uniform(0.1, 1.0)), float(rng.uniform(0.2, 1.0)))
