@@ -224,11 +224,14 @@ def selkov(steps=20, warmup=5, seed=8154543):
 #            offsets (16), collision cell (4), distance byte (1)                               = 51
 #  sb_rotate reads collision cell (4), velocities (12), distance byte (1), the cell record (32 B per
 #            cell, 3.2 per particle); writes velocities (12)                                  = 32.2
-#  sb_cells  per collision cell: one accumulator copy read and cleared (48), record written (32); per
-#            unshifted cell: B census and first key read and cleared (16); per particle       = 9.6
+#  sb_cells_own  per collision cell: the owner's sums read (24), one accumulator copy read and cleared
+#            (48), record written (32); per unshifted cell: B census and first key read and cleared (16);
+#            per particle (10 per cell)                                                    = 12.0
+# (the owner tiles' stores of their cells' sums, 2.4 B per particle, and the inbox entries between
+# tiles are the binning's own traffic, not counted as algorithmic)
 # The interval's figure is 57 B per particle-update: the split state read once (cell 4, offsets 12,
 # velocities 12, species 1) and written once (the same without the species, written only on a change).
-ALGO_BYTES_SPLIT = {"sb_bulk": 51.0, "sb_rotate": 32.2, "sb_cells": 9.6}
+ALGO_BYTES_SPLIT = {"sb_bulk": 51.0, "sb_rotate": 32.2, "sb_cells_own": 12.0}
 SPLIT_STEP_BYTES = 57.0
 FP64_STEP_BYTES = 100.0
 
@@ -338,8 +341,9 @@ def split_batched(R=256, steps=20, warmup=4, seed=4357787, fp64_value=None):
     ms = dt * 1000.0 / steps
     value = n * steps / dt
     kernels = kernel_rooflines(kms, ALGO_BYTES_SPLIT, n)
-    for k, v in kernels.items():
-        v["traffic"] = read_traffic(k, TRAFFIC_SPLIT_FILE)
+    slots = batch.bulk_slots or (2048 if batch.tile >= 4096 else 1024)
+    for k, v in kernels.items():   # the binning pass of the tuned table size
+        v["traffic"] = read_traffic(f"sb_bulk<false,512,{slots}>" if k == "sb_bulk" else k, TRAFFIC_SPLIT_FILE)
     print(f"# split batched: {ms:.3f} ms/interval, {value:.3e} particle-updates/s, kernels {kms}", file=sys.stderr,
           flush=True)
     out = {"replicas": R, "particles": n, "collision_cells": info["cells"], "timed_intervals": steps,
@@ -374,8 +378,8 @@ def step_traffic(sort_every=10):
     with open(TRAFFIC_FILE) as f:
         for line in f:
             m = re.match(r"traffic_bytes_per_launch\s+(\S+)\s+(\S+)", line)
-            if m:
-                per[m.group(1)] = float(m.group(2))
+            if m:   # by base name (co_near_engine<1> -> co_near_engine)
+                per.setdefault(m.group(1).split("<")[0], float(m.group(2)))
     if not all(k in per for k in ("co_near_engine", "co_far", "co_collide_far", "co_rotate_far")):
         return None
     return sum(per.get(k, 0.0) for k in STEP_KERNELS) + sum(per.get(k, 0.0) for k in SORT_KERNELS) / sort_every
@@ -383,15 +387,22 @@ def step_traffic(sort_every=10):
 
 def read_traffic(kernel, path=TRAFFIC_FILE):
     """Bytes per launch of `kernel` between L2 and memory from a committed
-    counter summary (an upper bound on its HBM bytes)."""
+    counter summary (an upper bound on its HBM bytes): the line of that exact
+    name (template arguments without spaces, e.g. sb_bulk<false,512,1024>),
+    else the first line of that base name."""
     if not os.path.exists(path):
         return None
+    base = None
     with open(path) as f:
         for line in f:
             m = re.match(r"traffic_bytes_per_launch\s+(\S+)\s+(\S+)", line)
-            if m and m.group(1) == kernel:
+            if not m:
+                continue
+            if m.group(1) == kernel:
                 return float(m.group(2))
-    return None
+            if base is None and m.group(1).split("<")[0] == kernel:
+                base = float(m.group(2))
+    return base
 
 
 # Counter passes of the traffic figures (one rocprofv3 --pmc run each; at most
@@ -420,7 +431,8 @@ def summarize_pmc(read_dir, write_dir):
                     c = row.get("Counter_Name")
                     if c not in counters:
                         continue
-                    name = re.sub(r"^void\s+", "", row["Kernel_Name"].split("(")[0].strip())
+                    name = re.sub(r"^void\s+", "", row["Kernel_Name"].replace("(anonymous namespace)::", ""))
+                    name = name.split("(")[0].strip()
                     per.setdefault(name, {}).setdefault(c, []).append(float(row["Counter_Value"]))
         return {k: {c: float(np.mean(v)) for c, v in d.items()} for k, d in per.items()}, \
                {k: max(len(v) for v in d.values()) for k, d in per.items()}
@@ -447,7 +459,7 @@ def summarize_pmc(read_dir, write_dir):
         short = re.sub(r"^(ne_solo|ne_batch)::", "", short)
         rb, wb = rbytes(k), wbytes(k)
         print(f"# {short[:44]:44s} {nr[k]:10d} {rb:12.4g} {wb:12.4g} {rb + wb:13.4g}")
-        print(f"traffic_bytes_per_launch {short.split('<')[0]} {rb + wb:.6g}")
+        print(f"traffic_bytes_per_launch {short.replace(' ', '')} {rb + wb:.6g}")
 
 
 def main():

@@ -324,7 +324,7 @@ int mpcx_cob_info(mpcx_cob* b, int64_t* particles, int64_t* cells, double* run_m
 /* Split batches run without the graph: mean kernel durations (ms) of the last
  * run from HIP events on the stream each runs on (every 8th interval, at most
  * 256 of them): [0] near engines, [1] sb_bulk (far flight and binning), [2]
- * sb_cells (cell records, reactions), [3] sb_rotate, [4] sb_near (near set
+ * sb_cells_own (cell records, reactions), [3] sb_rotate, [4] sb_near (near set
  * binning); *count = timed intervals. */
 int mpcx_cob_kernel_ms(mpcx_cob* b, double* ms5, int64_t* count);
 /* Advance n intervals and, after each, add every replica's body-frame B

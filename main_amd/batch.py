@@ -49,7 +49,7 @@ class CollectiveBatch:
         check(lib().mpcx_cob_info(self.h, C.byref(p), C.byref(c), C.byref(ms)), "mpcx_cob_info")
         return dict(particles=p.value, cells=c.value, run_ms=ms.value)
 
-    KERNELS = ("near_engine", "sb_bulk", "sb_cells", "sb_rotate", "sb_near")
+    KERNELS = ("near_engine", "sb_bulk", "sb_cells_own", "sb_rotate", "sb_near")
 
     def kernel_ms(self) -> dict:
         """Mean kernel durations (ms) of the last split run without the graph."""

@@ -286,7 +286,7 @@ class CollectiveEngine:
         n = C.c_int64(0)
         check(lib().mpcx_co_kernel_ms(self.h, ptr(a), C.byref(n)), "mpcx_co_kernel_ms")
         if self.split:   # the split interval's kernels (mpcx_cob_kernel_ms)
-            names = ["near_engine", "sb_bulk", "sb_cells", "sb_rotate", "sb_near"]
+            names = ["near_engine", "sb_bulk", "sb_cells_own", "sb_rotate", "sb_near"]
         else:
             names = [("co_far" if self.overlap and k == "co_bulk" else k) for k in self.KERNELS]
         return dict(zip(names, a.tolist()), intervals=int(n.value))

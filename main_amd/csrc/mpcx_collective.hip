@@ -4465,24 +4465,30 @@ __global__ __launch_bounds__(256) void sort_gather_b(const CoD* Ds, const CoSort
 // are done; every collision cell is then collided and every particle rotated.
 // The per-cell velocity sums are exact: each FP32 velocity is scaled by 2^40
 // (an integer for |v| >= 2^-17; below that the rounding is under 2^-41) and
-// added in 64-bit integer arithmetic, grouped per workgroup in LDS and then
-// added with one global atomic per word and distinct cell of the workgroup.
-// Integer sums do not depend on the order the members arrive in, so the far
-// particles need not wait for the near set, no cell is collided twice, and a
-// system's results do not depend on which workgroups ran it.  The census
-// (unshifted cells, distribute_mod.f95:95-183) is grouped the same way, and
-// the bulk B->A reaction (reaction_mod.f95:28-66) runs per unshifted cell from
-// it.  Philox streams: 0 the grid shift, 1 the axes (counter: collision cell),
-// 2 the reactions (counter: unshifted cell), keyed by the system's seed.
+// added in 64-bit integer arithmetic, grouped per particle tile in an LDS hash
+// table.  Cells are kept by tour position; after each re-sort every particle
+// tile owns the tour cells whose first particle it holds: it stores their sums
+// whole, and sends the sums of other tiles' cells to their owners' inboxes
+// (sb_flush_owned); the cell pass of each owner adds them.  Integer sums do
+// not depend on the order the members arrive in, so the far particles need
+// not wait for the near set, no cell is collided twice, and a system's results
+// do not depend on which workgroups ran it or on the particle order.  The
+// census (unshifted cells, distribute_mod.f95:95-183) is grouped per tile the
+// same way, and the bulk B->A reaction (reaction_mod.f95:28-66) runs per
+// unshifted cell from it.  Philox streams: 0 the grid shift, 1 the axes
+// (counter: collision cell), 2 the reactions (counter: unshifted cell), keyed
+// by the system's seed.
 // =====================================================================================
 constexpr double SB_SCALE = 0x1p40;   // fixed point of the cell sums
-// The sums and the census are added with atomics that execute in the L2 of
-// the XCD the workgroup runs on (workgroup scope: no bypass to memory, where
-// device-scope atomics serialise at about 3e10/s); each of the eight XCDs
-// adds into its own copy of the accumulators, named by the hardware XCC_ID
-// register, so no two L2s ever hold the same accumulator line; sb_cells adds
-// the copies (integer sums: the result does not depend on how the work was
-// spread) after the kernels that fill them have ended.
+// What is left to atomics -- the near set's sums (sb_near), cells no tile
+// owns or whose owner's inbox is full, and the census -- is added into one
+// copy of the accumulators per XCD, named by the hardware XCC_ID register, so
+// no two L2s hold the same accumulator line; sb_cells_own adds the copies
+// (integer sums: the result does not depend on how the work was spread) after
+// the kernels that fill them have ended.  The atomics are workgroup-scoped,
+// but the L2 does not execute them: the counters show every atomic request
+// going to the memory fabric (TCC_EA0_ATOMIC equals TCC_ATOMIC), at a rate
+// far below the L2's, hence the plain stores above for most sums.
 __device__ __forceinline__ int hw_xcc() {
   unsigned v;
   asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
@@ -4742,7 +4748,7 @@ __device__ __forceinline__ void sb_flush(const CoP& P, const SbV& A, SbTables<GT
     }
     const int k = j < SB_CT ? G.ck[j] : -1;
     if (k >= 0) {
-      l2_add<int>(A.tc + k, G.cc[j]);   // checked against CENSUS_MAX by sb_cells
+      l2_add<int>(A.tc + k, G.cc[j]);   // checked against CENSUS_MAX by sb_cells_own
       l2_min(A.fb + k, G.cm[j]);
     }
   }
@@ -5134,7 +5140,7 @@ __global__ __launch_bounds__(256) void sb_rotate(const CoP* __restrict__ Ps, con
   const double tau = P.tau * (1.0 + 1e-9);
   const double reach0 = P.guard + 1e-6 + (classify == 2 ? start_shift(P, D) : 0.0);
   const int lane = threadIdx.x & 63;
-  if (tl.begin == 0 && threadIdx.x == 0) D.sc->xmask = 0u;   // sb_cells has merged the copies
+  if (tl.begin == 0 && threadIdx.x == 0) D.sc->xmask = 0u;   // sb_cells_own has merged the copies
   const gp<const int> cell_of = g1((const int*)D.cell_of);
   const gp<const f4v> rec = g1((const f4v*)D.srec);
   const gp<const unsigned char> dq = g1((const unsigned char*)D.dq);
@@ -5167,7 +5173,7 @@ __global__ __launch_bounds__(256) void sb_rotate(const CoP* __restrict__ Ps, con
 }
 
 // Per system (blockIdx.y): cleared cell sums, census and first-B keys of
-// every XCD copy (every split run starts from them; each interval's sb_cells
+// every XCD copy (every split run starts from them; each interval's sb_cells_own
 // leaves the copies it merged cleared)
 __global__ __launch_bounds__(256) void sb_clear(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds) {
   const int r = blockIdx.y;
@@ -5304,7 +5310,7 @@ struct mpcx_cob {
   long long particles = 0, cells = 0;
   int fill_blocks = 64;
   // kernel timing of split runs without a graph: every TSTRIDE-th interval,
-  // [0] near engines, [1] sb_bulk (far flight and binning), [2] sb_cells,
+  // [0] near engines, [1] sb_bulk (far flight and binning), [2] sb_cells_own,
   // [3] sb_rotate, [4] sb_near (near set binning)
   static constexpr int NKT = 5, TMAX = 256, TSTRIDE = 8;
   std::vector<hipEvent_t> tev;
