@@ -5022,8 +5022,9 @@ __global__ __launch_bounds__(256) void sb_near(const CoP* __restrict__ Ps, const
 }
 
 // The cells of one system's particle tile, once every particle is binned:
-// those at the tour positions the tile owns (and, for its last tile, the
-// unowned extra z layer), OWNCAP at a time.  Per cell: the record of the
+// those at the tour positions the tile owns, OWNCAP at a time, then its share
+// of the extra z layer, which no tile owns (its sums are all in the XCD
+// copies), spread evenly over the system's tiles.  Per cell: the record of the
 // collision cell (mean velocity from the exact sums of its owner, of the
 // owner's inbox and of the XCD copies; random axis of
 // mpc_rotation_mod.f95:184-194) with the copies cleared; and, for an
@@ -5046,13 +5047,16 @@ __global__ __launch_bounds__(256) void sb_cells_own(const CoP* __restrict__ Ps, 
   const int g = X.tb[tl.r] + (int)(tl.begin / X.T);
   const int2 own = X.rng[g];
   const long long plo = own.x, phi = own.y;
-  const long long hi = tl.end >= P.NT ? P.mpcnumcell : phi;
+  const long long ntiles = (P.NT + X.T - 1) / X.T, k = tl.begin / X.T;
+  const long long ex = (P.mpcnumcell - P.vol + ntiles - 1) / ntiles;
+  const long long e0 = min(P.vol + k * ex, P.mpcnumcell), e1 = min(e0 + ex, P.mpcnumcell);
   const int n = min(X.cnt[g], X.cap);
   const FxEntry* box = X.box + (long long)g * X.cap;
   if (plo == 0 && threadIdx.x == 0) D.sc->far_maxv2 = 0ull;   // sb_rotate publishes the next interval's far maximum
-  for (long long c0 = plo; c0 < hi; c0 += OWNCAP) {
-    const long long c1 = min(c0 + (long long)OWNCAP, hi);
-    for (int j = threadIdx.x; j < OWNCAP; j += 256) { acc[0][j] = 0ull; acc[1][j] = 0ull; acc[2][j] = 0ull; }
+  for (long long q = e0 + threadIdx.x; q < e1; q += 256) sb_cell(P, D, S, q, 0ull, 0ull, 0ull);
+  for (long long c0 = plo; c0 < phi; c0 += OWNCAP) {
+    const long long c1 = min(c0 + (long long)OWNCAP, phi);
+    for (int j = threadIdx.x; j < c1 - c0; j += 256) { acc[0][j] = 0ull; acc[1][j] = 0ull; acc[2][j] = 0ull; }
     __syncthreads();
     for (int e = threadIdx.x; e < n; e += 256) {
       const long long q = box[e].p;
@@ -5068,7 +5072,10 @@ __global__ __launch_bounds__(256) void sb_cells_own(const CoP* __restrict__ Ps, 
       sb_cell(P, D, S, q, acc[0][q - c0], acc[1][q - c0], acc[2][q - c0]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) X.cnt[g] = 0;   // the inbox is read: empty for the next interval
+  if (threadIdx.x == 0) {
+    if (n > 0 && plo >= phi) raise_error(&D.sc->err, ERR_CELL_RANGE);   // entries for a tile that owns nothing
+    X.cnt[g] = 0;   // the inbox is read: empty for the next interval
+  }
 }
 __device__ __forceinline__ void sb_cell(const CoP& P, const CoD& D, const SpD& S, long long p, unsigned long long x0,
                                         unsigned long long x1, unsigned long long x2) {

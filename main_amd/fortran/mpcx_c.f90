@@ -219,6 +219,27 @@ module mpcx_c
       integer(c_int) :: rc
     end function
 
+    function mpcx_cob_set_tile(b, tile) bind(C, name="mpcx_cob_set_tile") result(rc)
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: b
+      integer(c_int32_t), value :: tile
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_cob_set_bulk_slots(b, slots) bind(C, name="mpcx_cob_set_bulk_slots") result(rc)
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: b
+      integer(c_int32_t), value :: slots
+      integer(c_int) :: rc
+    end function
+
+    function mpcx_cob_set_sort_step(b, steps) bind(C, name="mpcx_cob_set_sort_step") result(rc)
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: b
+      integer(c_int32_t), value :: steps
+      integer(c_int) :: rc
+    end function
+
     function mpcx_shutdown() bind(C, name="mpcx_shutdown") result(rc)
       import :: c_int
       integer(c_int) :: rc
