@@ -70,7 +70,8 @@ struct CoScalars {
   int n_wall, pad_w;            // far particles co_far left to co_bin_rest (they reach a wall)
   int n_late, pad_l;            // late collision cells listed (zone around the spheres, deferred particles)
   unsigned long long next_fmax2;   // far max |v|^2 of the next interval while far_maxv2 is still in use
-  unsigned xmask, pad_xm;          // split storage: XCDs whose accumulator copies hold this interval's sums
+  unsigned xmask;                  // split storage: XCDs whose accumulator copies hold this interval's census
+  unsigned sums_copied;            // split storage: some cell sums of this interval went to the copies
 };
 
 // Dimers whose near regions can overlap within an interval form a cluster;
@@ -102,6 +103,7 @@ struct CoP {
   double hs2, a0, cosrot, sinrot, omc, twopi, tau, guard, skinrad;
 };
 
+struct SpD;
 struct CoD {
   double *x, *y, *z, *vx, *vy, *vz;
   int* flag;
@@ -164,6 +166,7 @@ struct CoD {
   // it (the tile holding the cell's first particle at the last re-sort):
   // written whole, by plain stores, by that tile in every interval
   unsigned long long* sown;
+  const SpD* spd;         // a split batch's copy of the descriptor: the split arrays the near engine gathers from
   const int* soff;        // [vol + 1] first particle of each tour position at the last re-sort
   float4* srec;
   int* xtc;               // split storage B census, one copy per XCD: [8][vol], by tour position
@@ -541,6 +544,26 @@ __device__ __attribute__((noinline)) double clu_exchange_max(CluInfo* ci, int ci
   return m;
 }
 
+
+// Split-precision storage of the solvent (the kernels further below): per
+// particle the packed unshifted cell, FP32 offsets in it, FP32 velocities and
+// the species.  The batch near engines gather their near sets from it.
+struct SpD {
+  int* cell;
+  float *ox, *oy, *oz, *vx, *vy, *vz;
+  signed char* flag;
+};
+
+// The cell of a split-storage particle is held as its packed coordinates
+// ix | iy << 11 | iz << 22 (boxes up to 2048 x 2048 x 1024 cells), so no
+// kernel divides by the box edges to find them.
+__device__ __forceinline__ int sp_pack(int ix, int iy, int iz) { return ix | (iy << 11) | (iz << 22); }
+__device__ __forceinline__ void sp_cell_xyz(const CoP& P, int cell, int& ix, int& iy, int& iz) {
+  (void)P;
+  ix = cell & 2047;
+  iy = (cell >> 11) & 2047;
+  iz = (int)((unsigned)cell >> 22);
+}
 
 // The near engine (its LDS state and kernels): up to LCAP inside entries are
 // staged in LDS (150 KB with the rest of its state: one engine per CU); the
@@ -2684,25 +2707,14 @@ __global__ __launch_bounds__(256) void sort_gather_bits(CoP P, CoD D, const int*
 
 namespace mpcx {
 // ------------------------------------------------------------- split precision
-// Solvent stored as the 0-based unshifted cell (x + Lx y + Lx Ly z) with FP32
-// offsets in [0,1) inside it, FP32 velocities and an 8-bit species: 29 bytes
-// read and written per particle and interval instead of 52.  Far particles
-// stream, bounce, bin, collide and rotate in FP32 cell coordinates (no large
-// coordinates ever formed, walls handled relative to the wall's cell); the
-// per-cell velocity sums are Kahan-compensated FP32 in ascending particle
-// order.  Near particles and the dimers keep the FP64 near engines: their
-// state is expanded into the FP64 staging arrays before each near-engine
-// launch and stored back after it.
-struct SpD {
-  int* cell;
-  float *ox, *oy, *oz, *vx, *vy, *vz;
-  signed char* flag;
-};
-
-// The cell of a split-storage particle is held as its packed coordinates
-// ix | iy << 11 | iz << 22 (boxes up to 2048 x 2048 x 1024 cells), so no
-// kernel divides by the box edges to find them.
-__device__ __forceinline__ int sp_pack(int ix, int iy, int iz) { return ix | (iy << 11) | (iz << 22); }
+// Solvent stored as the unshifted cell with FP32 offsets in [0,1) inside it,
+// FP32 velocities and an 8-bit species: 29 bytes read and written per
+// particle and interval instead of 52.  Far particles stream, bounce, bin and
+// rotate in FP32 cell coordinates (no large coordinates ever formed, walls
+// handled relative to the wall's cell); the per-cell velocity sums are exact
+// 64-bit fixed point (the split-precision intervals below).  Near particles
+// and the dimers keep the FP64 near engines: their state is expanded into FP64
+// before each near-engine launch and stored back after it.
 __device__ __forceinline__ void sp_split(double x, double y, double z, const CoP& P, int& cell, float& ox, float& oy,
                                          float& oz) {
   int ix = (int)floor(x), iy = (int)floor(y), iz = (int)floor(z);
@@ -2715,12 +2727,6 @@ __device__ __forceinline__ void sp_split(double x, double y, double z, const CoP
   if (iy >= P.Ly) iy -= P.Ly; else if (iy < 0) iy += P.Ly;
   cell = sp_pack(ix, iy, iz);
   ox = fx; oy = fy; oz = fz;
-}
-__device__ __forceinline__ void sp_cell_xyz(const CoP& P, int cell, int& ix, int& iy, int& iz) {
-  (void)P;
-  ix = cell & 2047;
-  iy = (cell >> 11) & 2047;
-  iz = (int)((unsigned)cell >> 22);
 }
 
 // a split particle whose cell lies in the box and whose offsets and
@@ -4514,6 +4520,7 @@ struct SbV {
   gp<unsigned long long> acc;   // this workgroup's XCD copy: [3 * mpcnumcell], by collision cell
   gp<int> tc;                   // B census: [vol]
   gp<unsigned> fb;              // first-B keys: [vol]
+  gp<unsigned> copied;          // set when a cell sum goes to the copy (the cell pass then reads them)
 };
 __device__ __forceinline__ SbV sb_view(const CoP& P, const CoD& D, const SpD& S, int x) {
   SbV v;
@@ -4523,6 +4530,7 @@ __device__ __forceinline__ SbV sb_view(const CoP& P, const CoD& D, const SpD& S,
   v.acc = g1(D.sacc + (long long)x * 3 * P.mpcnumcell);
   v.tc = g1(D.xtc + (long long)x * P.vol);
   v.fb = g1(D.xfb + (long long)x * P.vol);
+  v.copied = g1(&D.sc->sums_copied);
   return v;
 }
 template <typename T>
@@ -4597,6 +4605,7 @@ __device__ __forceinline__ void sb_insert(const CoP& P, const SbV& A, SbTables<G
       l2_add<unsigned long long>(a, w0);
       l2_add<unsigned long long>(a + 1, w1);
       l2_add<unsigned long long>(a + 2, w2);
+      *A.copied = 1u;
     }
   }
   if (cen) {
@@ -4668,24 +4677,40 @@ __global__ __launch_bounds__(256) void sb_own_ranges(const CoP* __restrict__ Ps,
   const int* soff = Ds[tl.r].soff;
   X.rng[X.tb[tl.r] + (int)(tl.begin / X.T)] = make_int2(own_bound(soff, vol, tl.begin, NT), own_bound(soff, vol, tl.end, NT));
 }
+// The extra z layer of collision cells (tour positions [vol, mpcnumcell))
+// has no first particle to own it by: it is dealt to a system's particle
+// tiles in equal runs of ext_run(P, T) positions, tile k the k-th run.
+__device__ __forceinline__ long long ext_run(const CoP& P, int T) {
+  const long long nt = (P.NT + T - 1) / T;
+  return max((P.mpcnumcell - P.vol + nt - 1) / nt, 1ll);
+}
+// the owner tile (index among all tiles) of tour position t of system r
+__device__ __forceinline__ int owner_tile(const CoP& P, const CoD& D, const SbInbox& X, int r, int t) {
+  const int nt = (int)((P.NT + X.T - 1) / X.T);
+  const long long o = t < P.vol ? D.soff[t] / X.T : (t - P.vol) / ext_run(P, X.T);
+  return X.tb[r] + (int)min(o, (long long)nt - 1);
+}
 // every thread of the block together, after the tile: the sums of the cells
-// the tile owns (tour positions [plo, phi), one contiguous run of sown)
-// stored whole -- zero for an owned cell it binned nothing into -- and the
-// other table entries added to the workgroup's XCD copy
+// the tile owns (tour positions [plo, phi), one contiguous run of sown, and
+// its run [e0, e1) of the extra layer) stored whole -- zero for an owned cell
+// it binned nothing into -- and the other table entries sent to their
+// owners' inboxes (the near-set tiles of sb_near own nothing)
 template <int GT>
 __device__ __forceinline__ void sb_flush_owned(const CoP& P, const CoD& D, const SbV& A, SbTables<GT>& G, int plo,
-                                               int phi, const SbInbox& X, int r) {
+                                               int phi, int e0, int e1, const SbInbox& X, int r) {
   __syncthreads();
   const gp<unsigned long long> own = g1(D.sown);
-  for (int p = plo + (int)threadIdx.x; p < phi; p += blockDim.x) {
+  const int n0 = phi - plo, nown = n0 + (e1 - e0);
+  for (int i = (int)threadIdx.x; i < nown; i += blockDim.x) {
+    const int p = i < n0 ? plo + i : e0 + (i - n0);
     const int j = sb_find<GT>(G.rk, p);
     unsigned long long w0 = 0ull, w1 = 0ull, w2 = 0ull;
     if (j >= 0) { w0 = G.rs[0][j]; w1 = G.rs[1][j]; w2 = G.rs[2][j]; }
     own[3ll * p] = w0; own[3ll * p + 1] = w1; own[3ll * p + 2] = w2;
   }
+  auto owned = [&](int t) { return (t >= plo && t < phi) || (t >= e0 && t < e1); };
   // the foreign entries grouped by owner tile: ranks within each group in
   // LDS, one inbox claim per group, then the entries written at their slots
-  const int ntiles = (int)((P.NT + X.T - 1) / X.T);
   constexpr int K = GT / 256;   // table slots per thread (at least 256 threads)
   int ds[K], rk[K];
 #pragma unroll
@@ -4694,8 +4719,8 @@ __device__ __forceinline__ void sb_flush_owned(const CoP& P, const CoD& D, const
     ds[k] = -1; rk[k] = 0;
     if (j < GT) {
       const int t = G.rk[j];
-      if (t >= 0 && (t < plo || t >= phi) && t < P.vol) {   // owner: the tile holding the cell's first particle
-        const int d = sb_slot<SB_DEST>(G.dk, X.tb[r] + min(D.soff[t] / X.T, ntiles - 1));
+      if (t >= 0 && !owned(t)) {
+        const int d = sb_slot<SB_DEST>(G.dk, owner_tile(P, D, X, r, t));
         if (d >= 0) { ds[k] = d; rk[k] = atomicAdd(&G.dn[d], 1); }
       }
     }
@@ -4709,7 +4734,7 @@ __device__ __forceinline__ void sb_flush_owned(const CoP& P, const CoD& D, const
     const int j = threadIdx.x + k * blockDim.x;
     if (j >= GT) continue;
     const int t = G.rk[j];
-    if (t >= 0 && (t < plo || t >= phi)) {
+    if (t >= 0 && !owned(t)) {
       bool sent = false;
       if (ds[k] >= 0) {
         const int idx = G.db[ds[k]] + rk[k];
@@ -4720,36 +4745,18 @@ __device__ __forceinline__ void sb_flush_owned(const CoP& P, const CoD& D, const
           sent = true;
         }
       }
-      if (!sent) {   // a full inbox, too many owners or an unowned cell: the XCD copy
+      if (!sent) {   // a full inbox or too many owners: the XCD copy
         const gp<unsigned long long> a = A.acc + 3ll * t;
         l2_add<unsigned long long>(a, G.rs[0][j]);
         l2_add<unsigned long long>(a + 1, G.rs[1][j]);
         l2_add<unsigned long long>(a + 2, G.rs[2][j]);
+        *A.copied = 1u;
       }
     }
     const int c = j < SB_CT ? G.ck[j] : -1;
     if (c >= 0) {
       l2_add<int>(A.tc + c, G.cc[j]);   // checked against CENSUS_MAX by sb_cells_own
       l2_min(A.fb + c, G.cm[j]);
-    }
-  }
-}
-// every thread of the block together, after the tile: the tables into the global sums
-template <int GT>
-__device__ __forceinline__ void sb_flush(const CoP& P, const SbV& A, SbTables<GT>& G) {
-  __syncthreads();
-  for (int j = threadIdx.x; j < GT; j += blockDim.x) {
-    const int t = G.rk[j];
-    if (t >= 0) {
-      const gp<unsigned long long> a = A.acc + 3ll * t;
-      l2_add<unsigned long long>(a, G.rs[0][j]);
-      l2_add<unsigned long long>(a + 1, G.rs[1][j]);
-      l2_add<unsigned long long>(a + 2, G.rs[2][j]);
-    }
-    const int k = j < SB_CT ? G.ck[j] : -1;
-    if (k >= 0) {
-      l2_add<int>(A.tc + k, G.cc[j]);   // checked against CENSUS_MAX by sb_cells_own
-      l2_min(A.fb + k, G.cm[j]);
     }
   }
 }
@@ -4934,15 +4941,18 @@ __global__ __launch_bounds__(T) void sb_bulk(const CoP* __restrict__ Ps, const C
   }
   if (do_bin) {
     const int2 own = X.rng[X.tb[tl.r] + (int)(tl.begin / X.T)];
-    sb_flush_owned(P, D, V, G, own.x, own.y, X, tl.r);
+    const long long ex = ext_run(P, X.T), k = tl.begin / X.T;
+    const int e0 = (int)min(P.vol + k * ex, P.mpcnumcell), e1 = (int)min(P.vol + (k + 1) * ex, P.mpcnumcell);
+    sb_flush_owned(P, D, V, G, own.x, own.y, e0, e1, X, tl.r);
   }
 }
 
 // Per system (blockIdx.y): the interval's start -- the collision counter and
-// the spheres' start positions (start = 1) -- and the near set from split
-// storage into the FP64 arrays the near engine works on.
+// the spheres' start positions (start = 1) -- and, with load, the near set
+// from split storage into the FP64 arrays (for a solo engine's near engine;
+// a batch's near engines gather from split storage themselves).
 __global__ __launch_bounds__(256) void sb_load_near(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds,
-                                                    const SpD* __restrict__ Ss, int start) {
+                                                    const SpD* __restrict__ Ss, int start, int load) {
   const int r = blockIdx.y;
   const CoP P = Ps[r];
   const CoD D = Ds[r];
@@ -4953,7 +4963,7 @@ __global__ __launch_bounds__(256) void sb_load_near(const CoP* __restrict__ Ps, 
       D.sc->xs0[j] = D.sc->x0[j]; D.sc->ys0[j] = D.sc->y0[j]; D.sc->zs0[j] = D.sc->z0[j];
     }
   }
-  const long long nn = min(D.sc->near_num, D.near_cap);
+  const long long nn = load ? min(D.sc->near_num, D.near_cap) : 0;
   for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < nn; q += (long long)gridDim.x * 256) {
     const int i = D.near_list[q];
     int ix, iy, iz;
@@ -4966,12 +4976,13 @@ __global__ __launch_bounds__(256) void sb_load_near(const CoP* __restrict__ Ps, 
 
 // After the near engines: near-set tile (replica r, chunk begin of end
 // chunks; begin == end pads the placement on XCDs) of the near list back into
-// split storage and, with do_bin, binned (distances to the spheres' start
+// split storage and, with do_bin, binned (its sums to the cells' owner tiles;
+// distances to the spheres' start
 // positions, as sb_bulk<false>); with check, the flight check (the near
 // engine's rebuild events must add up to the interval sb_bulk flew).
 __global__ __launch_bounds__(256) void sb_near(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds,
                                                const SpD* __restrict__ Ss, const Tile* __restrict__ tiles, int do_bin,
-                                               int check) {
+                                               int check, SbInbox X) {
   __shared__ SbTables<SB_CT> G;
   __shared__ SbSpheres sph;
   const Tile tl = tiles[blockIdx.x];
@@ -5017,24 +5028,23 @@ __global__ __launch_bounds__(256) void sb_near(const CoP* __restrict__ Ps, const
     if (do_bin)
       sb_bin(P, D, V, G, i, valid, inside, excl, ix, iy, iz, ox, oy, oz, vx, vy, vz, f, fxs, fys, fzs, zs, sph);
   }
-  if (do_bin) sb_flush(P, V, G);
+  if (do_bin) sb_flush_owned(P, D, V, G, 0, 0, 0, 0, X, tl.r);   // owns nothing: every sum to its owner
   if (check && tl.begin == 0 && threadIdx.x == 0) check_flight(P, D);
 }
 
 // The cells of one system's particle tile, once every particle is binned:
-// those at the tour positions the tile owns, OWNCAP at a time, then its share
-// of the extra z layer, which no tile owns (its sums are all in the XCD
-// copies), spread evenly over the system's tiles.  Per cell: the record of the
-// collision cell (mean velocity from the exact sums of its owner, of the
-// owner's inbox and of the XCD copies; random axis of
-// mpc_rotation_mod.f95:184-194) with the copies cleared; and, for an
+// those it owns -- tour positions [plo, phi) and its run of the extra z
+// layer -- OWNCAP at a time, each with its inbox added in LDS.  Per cell:
+// the record of the collision cell (mean velocity from the exact sums of its
+// owner, of the owner's inbox and, when some sums fell back to them, of the
+// XCD copies; random axis of mpc_rotation_mod.f95:184-194); and, for an
 // unshifted cell, the bulk B->A reaction (reaction_mod.f95:28-66: p from the
 // host table of 1 - exp(-k1 nB tau), one draw per cell, the first B member in
 // census order turned A) with its B census cleared.  Split storage keeps the
 // census of B only: nothing else reads the A and S counts.
 constexpr int OWNCAP = 1024;
 __device__ __forceinline__ void sb_cell(const CoP& P, const CoD& D, const SpD& S, long long p, unsigned long long x0,
-                                        unsigned long long x1, unsigned long long x2);
+                                        unsigned long long x1, unsigned long long x2, bool copies);
 __global__ __launch_bounds__(256) void sb_cells_own(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds,
                                                     const SpD* __restrict__ Ss, const Tile* __restrict__ tiles,
                                                     SbInbox X) {
@@ -5046,44 +5056,44 @@ __global__ __launch_bounds__(256) void sb_cells_own(const CoP* __restrict__ Ps, 
   const SpD S = Ss[tl.r];
   const int g = X.tb[tl.r] + (int)(tl.begin / X.T);
   const int2 own = X.rng[g];
-  const long long plo = own.x, phi = own.y;
-  const long long ntiles = (P.NT + X.T - 1) / X.T, k = tl.begin / X.T;
-  const long long ex = (P.mpcnumcell - P.vol + ntiles - 1) / ntiles;
-  const long long e0 = min(P.vol + k * ex, P.mpcnumcell), e1 = min(e0 + ex, P.mpcnumcell);
+  const long long ex = ext_run(P, X.T), k = tl.begin / X.T;
+  const long long e0 = min(P.vol + k * ex, P.mpcnumcell), e1 = min(P.vol + (k + 1) * ex, P.mpcnumcell);
+  const long long lo[2] = {own.x, e0}, hi[2] = {own.y, e1};
   const int n = min(X.cnt[g], X.cap);
   const FxEntry* box = X.box + (long long)g * X.cap;
-  if (plo == 0 && threadIdx.x == 0) D.sc->far_maxv2 = 0ull;   // sb_rotate publishes the next interval's far maximum
-  for (long long q = e0 + threadIdx.x; q < e1; q += 256) sb_cell(P, D, S, q, 0ull, 0ull, 0ull);
-  for (long long c0 = plo; c0 < phi; c0 += OWNCAP) {
-    const long long c1 = min(c0 + (long long)OWNCAP, phi);
-    for (int j = threadIdx.x; j < c1 - c0; j += 256) { acc[0][j] = 0ull; acc[1][j] = 0ull; acc[2][j] = 0ull; }
-    __syncthreads();
-    for (int e = threadIdx.x; e < n; e += 256) {
-      const long long q = box[e].p;
-      if (c0 == plo && (q < plo || q >= phi)) raise_error(&D.sc->err, ERR_CELL_RANGE);   // not this tile's cell
-      if (q >= c0 && q < c1) {
-        atomicAdd(&acc[0][q - c0], box[e].w[0]);
-        atomicAdd(&acc[1][q - c0], box[e].w[1]);
-        atomicAdd(&acc[2][q - c0], box[e].w[2]);
+  const bool copies = D.sc->sums_copied != 0u;
+  if (own.x == 0 && threadIdx.x == 0) D.sc->far_maxv2 = 0ull;   // sb_rotate publishes the next interval's far maximum
+  for (int e = threadIdx.x; e < n; e += 256) {   // every entry is for a cell the tile owns
+    const long long q = box[e].p;
+    if (!((q >= lo[0] && q < hi[0]) || (q >= lo[1] && q < hi[1]))) raise_error(&D.sc->err, ERR_CELL_RANGE);
+  }
+  for (int part = 0; part < 2; part++)
+    for (long long c0 = lo[part]; c0 < hi[part]; c0 += OWNCAP) {
+      const long long c1 = min(c0 + (long long)OWNCAP, hi[part]);
+      for (int j = threadIdx.x; j < c1 - c0; j += 256) { acc[0][j] = 0ull; acc[1][j] = 0ull; acc[2][j] = 0ull; }
+      __syncthreads();
+      for (int e = threadIdx.x; e < n; e += 256) {
+        const long long q = box[e].p;
+        if (q >= c0 && q < c1) {
+          atomicAdd(&acc[0][q - c0], box[e].w[0]);
+          atomicAdd(&acc[1][q - c0], box[e].w[1]);
+          atomicAdd(&acc[2][q - c0], box[e].w[2]);
+        }
       }
+      __syncthreads();
+      for (long long q = c0 + threadIdx.x; q < c1; q += 256)
+        sb_cell(P, D, S, q, acc[0][q - c0], acc[1][q - c0], acc[2][q - c0], copies);
+      __syncthreads();
     }
-    __syncthreads();
-    for (long long q = c0 + threadIdx.x; q < c1; q += 256)
-      sb_cell(P, D, S, q, acc[0][q - c0], acc[1][q - c0], acc[2][q - c0]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    if (n > 0 && plo >= phi) raise_error(&D.sc->err, ERR_CELL_RANGE);   // entries for a tile that owns nothing
-    X.cnt[g] = 0;   // the inbox is read: empty for the next interval
-  }
+  if (threadIdx.x == 0) X.cnt[g] = 0;   // the inbox is read: empty for the next interval
 }
 __device__ __forceinline__ void sb_cell(const CoP& P, const CoD& D, const SpD& S, long long p, unsigned long long x0,
-                                        unsigned long long x1, unsigned long long x2) {
+                                        unsigned long long x1, unsigned long long x2, bool copies) {
   const unsigned xm = D.sc->xmask;   // copies filled this interval
   if (p < P.mpcnumcell) {
     const gp<const unsigned long long> own = g1((const unsigned long long*)D.sown) + 3 * p;
     long long sx = (long long)(own[0] + x0), sy = (long long)(own[1] + x1), sz = (long long)(own[2] + x2);
-    for (unsigned m = xm; m; m &= m - 1) {
+    for (unsigned m = copies ? xm : 0u; m; m &= m - 1) {
       const gp<unsigned long long> a = g1(D.sacc + (long long)(__ffs(m) - 1) * 3 * P.mpcnumcell + 3 * p);
       sx += (long long)a[0]; sy += (long long)a[1]; sz += (long long)a[2];
       a[0] = 0ull; a[1] = 0ull; a[2] = 0ull;
@@ -5147,7 +5157,7 @@ __global__ __launch_bounds__(256) void sb_rotate(const CoP* __restrict__ Ps, con
   const double tau = P.tau * (1.0 + 1e-9);
   const double reach0 = P.guard + 1e-6 + (classify == 2 ? start_shift(P, D) : 0.0);
   const int lane = threadIdx.x & 63;
-  if (tl.begin == 0 && threadIdx.x == 0) D.sc->xmask = 0u;   // sb_cells_own has merged the copies
+  if (tl.begin == 0 && threadIdx.x == 0) { D.sc->xmask = 0u; D.sc->sums_copied = 0u; }   // sb_cells_own merged the copies
   const gp<const int> cell_of = g1((const int*)D.cell_of);
   const gp<const f4v> rec = g1((const f4v*)D.srec);
   const gp<const unsigned char> dq = g1((const unsigned char*)D.dq);
@@ -5190,7 +5200,7 @@ __global__ __launch_bounds__(256) void sb_clear(const CoP* __restrict__ Ps, cons
   for (long long i = i0; i < NXCD * 3 * P.mpcnumcell; i += st) D.sacc[i] = 0ull;
   for (long long i = i0; i < NXCD * P.vol; i += st) D.xtc[i] = 0;
   for (long long i = i0; i < NXCD * P.vol; i += st) D.xfb[i] = 0xffffffffu;
-  if (i0 == 0) D.sc->xmask = 0u;
+  if (i0 == 0) { D.sc->xmask = 0u; D.sc->sums_copied = 0u; }
 }
 
 // Hilbert re-sort of split storage (sort_hilbert_mod.f95:50-92), per tile:
@@ -5458,6 +5468,7 @@ static int cob_upload_D(mpcx_cob* b) {
   for (int r = 0; r < b->R; r++) {
     mpcx_co* e = b->eng[r];
     b->h_D[r] = e->D;
+    b->h_D[r].spd = (b->split && !b->solo) ? b->d_S + r : nullptr;   // batch near engines gather from split storage
     CoSortD& X = b->h_X[r];
     X.tourpos = e->d_tourpos; X.scnt = e->d_scnt; X.soff = e->d_soff; X.sb = e->d_sb; X.key = e->d_skey;
     X.rk = e->d_srk;
@@ -5604,7 +5615,9 @@ static int sb_interval(mpcx_cob* b, bool sort_due, long long slot) {
   };
   if (slot >= 0)
     for (int k = 0; k < mpcx_cob::NKT; k++) b->tev_on[(size_t)slot * mpcx_cob::NKT + k] = 0;
-  hipLaunchKernelGGL(sb_load_near, ng, dim3(256), 0, st, b->d_P, b->d_D, b->d_S, 1);
+  // the interval's start; a solo engine's near engine also needs its near set in the FP64 arrays
+  if (b->solo) hipLaunchKernelGGL(sb_load_near, ng, dim3(256), 0, st, b->d_P, b->d_D, b->d_S, 1, 1);
+  else hipLaunchKernelGGL(sb_load_near, dim3(1, b->R), dim3(256), 0, st, b->d_P, b->d_D, b->d_S, 1, 0);
   const int bin = sort_due ? 0 : 1;
   if (ov) {
     HIPCHK(hipEventRecord(b->fork, st));
@@ -5627,16 +5640,16 @@ static int sb_interval(mpcx_cob* b, bool sort_due, long long slot) {
     HIPCHK(tmark(1, 1, st));
   }
   HIPCHK(tmark(4, 0, st));
-  hipLaunchKernelGGL(sb_near, dim3(b->n_nt), dim3(256), 0, st, b->d_P, b->d_D, b->d_S, b->d_nt, bin, 1);
+  hipLaunchKernelGGL(sb_near, dim3(b->n_nt), dim3(256), 0, st, b->d_P, b->d_D, b->d_S, b->d_nt, bin, 1, cob_inbox(b));
   HIPCHK(tmark(4, 1, st));
   if (ov) HIPCHK(hipStreamWaitEvent(st, b->join, 0));
   if (sort_due) {
     int rc = cob_sort_all(b);
     if (rc) return rc;
     cob_near_list(b);
-    hipLaunchKernelGGL(sb_load_near, ng, dim3(256), 0, st, b->d_P, b->d_D, b->d_S, 0);
+    if (b->solo) hipLaunchKernelGGL(sb_load_near, ng, dim3(256), 0, st, b->d_P, b->d_D, b->d_S, 0, 1);
     HIPCHK(cob_launch_near(b, 1, M_RESUME));
-    hipLaunchKernelGGL(sb_near, dim3(b->n_nt), dim3(256), 0, st, b->d_P, b->d_D, b->d_S, b->d_nt, 0, 0);
+    hipLaunchKernelGGL(sb_near, dim3(b->n_nt), dim3(256), 0, st, b->d_P, b->d_D, b->d_S, b->d_nt, 0, 0, cob_inbox(b));
     sb_launch_bulk(b, true, st, 1);
   }
   HIPCHK(tmark(2, 0, st));
