@@ -219,16 +219,21 @@ class CollectiveEngine:
         return dict(in_list=inside, out_list=outside, nb=nb, nbnum=nbnum, fpx=fpx, fpy=fpy, fpz=fpz, pot=potn)
 
     def get_cells(self) -> dict:
+        """The last collision's cell records (by collision cell), each
+        particle's collision cell (1-based) and, in FP64 storage, the census
+        (split storage clears its census after each collision: tA, tB, tS are
+        None there)."""
         mc, NT, vol = self.p["mpcnumcell"], self.p["NT"], self.p["vol"]
         rec = np.empty((mc, 8))
         cell = np.empty(NT, np.int32)
-        census = np.empty((3, vol), np.int32)
+        census = None if self.split else np.empty((3, vol), np.int32)
         tp = np.zeros(1)
-        check(lib().mpcx_co_get_cells(self.h, ptr(rec), ptr(cell), ptr(census), ptr(tp)), "mpcx_co_get_cells")
+        check(lib().mpcx_co_get_cells(self.h, ptr(rec), ptr(cell), None if census is None else ptr(census), ptr(tp)),
+              "mpcx_co_get_cells")
+        cen = (None, None, None) if census is None else tuple(census[i].astype(np.int64) for i in range(3))
         return dict(vxcm=rec[:, 0].copy(), vycm=rec[:, 1].copy(), vzcm=rec[:, 2].copy(), uvx=rec[:, 3].copy(),
                     uvy=rec[:, 4].copy(), uvz=rec[:, 5].copy(), tcs=rec[:, 6].copy(),
-                    ncn=cell.astype(np.int64) + 1, tA=census[0].astype(np.int64), tB=census[1].astype(np.int64),
-                    tS=census[2].astype(np.int64), tpotential=float(tp[0]))
+                    ncn=cell.astype(np.int64) + 1, tA=cen[0], tB=cen[1], tS=cen[2], tpotential=float(tp[0]))
 
     def tpotential(self) -> float:
         """TPotential of the last force evaluation (set-up or MD step)."""

@@ -139,6 +139,71 @@ def test_split_many_dimer_slab_runs():
     assert abs(ke / (1.5 * p["T"]) - 1.0) < 0.05
 
 
+def _slab(split, seed=3):
+    p = S.collective_params(Lx=200.0, Ly=200.0, Lz=10.0, numdims=100, rhoa=10.0, sort_step=500)
+    e = CollectiveEngine(p, rng_mode=1, seed=seed, split=split)
+    e.set_dimers(S.collective_dimers(p, seed=seed), tour=cell_tour(200, 200, 10))
+    e.init_solvent(seed)
+    e.setup(do_sort=True)
+    return p, e
+
+
+def _body_b(e, nd, nr=12, nz=24):
+    """B counts in the body frames of all dimers, summed: (nz, nr)."""
+    return sum(e.body_density(nr=nr, nz=nz, rmax=12.0, zmin=-16.0, zmax=12.0, dimer=k)[1].astype(np.float64)
+               for k in range(nd))
+
+
+def test_split_slab_errors_against_fp64():
+    """The 100-dimer slab (200 x 200 x 10 cells, 4.0e6 solvent) in split and
+    FP64 storage from the same initial state and Philox key, with the errors
+    of split storage stated per quantity: the cell momenta after one interval
+    (cells whose membership differs, by cell-boundary ties of FP32 offsets,
+    counted apart), then over 60 collisions the temperature, the B count, the
+    motor velocity (mean over the 100 dimers, within 2 sigma of block
+    averages) and the body-frame B-density field summed over the dimers
+    (against its counting noise)."""
+    cells, series, fields = {}, {}, {}
+    for split in (False, True):
+        p, e = _slab(split)
+        try:
+            e.run(1)
+            cells[split] = e.get_cells()
+            v, t, b, f = [], [], [], np.zeros((24, 12))
+            for c in range(60):
+                e.run(1)
+                o = e.observables()
+                v.append(o["v_prop"]); t.append(o["T_sol"]); b.append(o["nB"])
+                if c >= 30 and c % 3 == 0:
+                    f += _body_b(e, 100)
+            series[split] = (np.array(v), np.array(t), np.array(b))
+            fields[split] = f
+        finally:
+            e.close()
+    a, c = cells[False], cells[True]
+    same = a["tcs"] == c["tcs"]
+    occ = same & (a["tcs"] > 0)
+    dv = np.abs(np.stack([c[k][occ] - a[k][occ] for k in ("vxcm", "vycm", "vzcm")]))
+    print(f"cell momenta after one interval: {np.mean(~same):.2e} of the cells differ in membership; over the "
+          f"others |d vcm| median {np.median(dv):.2e}, max {dv.max():.2e}")
+    assert np.mean(~same) < 1e-3
+    assert np.median(dv) < 1e-6 and dv.max() < 1e-4
+    (v0, t0, b0), (v1, t1, b1) = series[False], series[True]
+    se = np.hypot(_block_se(v0, 10), _block_se(v1, 10))
+    print(f"slab motor velocity: fp64 {v0.mean():.5f}, split {v1.mean():.5f}, 2 sigma {2 * se:.5f}; T_sol rel diff "
+          f"{abs(t1.mean() / t0.mean() - 1):.2e}; nB rel diff {abs(b1.mean() / b0.mean() - 1):.2e}")
+    assert abs(v1.mean() - v0.mean()) < 2.0 * se
+    assert abs(t1.mean() / t0.mean() - 1.0) < 2e-3
+    assert abs(b1.mean() / b0.mean() - 1.0) < 0.03
+    f0, f1 = fields[False], fields[True]
+    m = (f0 + f1) > 20
+    chi = np.mean((f1[m] - f0[m]) ** 2 / (f0[m] + f1[m]))
+    rel = np.linalg.norm(f1 - f0) / np.linalg.norm(f0)
+    print(f"body-frame B field: relative L2 difference {rel:.3f}, (diff^2 / counting variance) {chi:.2f} over "
+          f"{int(m.sum())} bins")
+    assert chi < 3.0
+
+
 def test_split_keeps_particles_below_the_top_wall():
     """FP32 offsets cannot hold a z within half an ulp below Lz: a far particle
     that ends its flight there (or converts from FP64 there) must stay in the
