@@ -2917,6 +2917,7 @@ struct mpcx_co {
   // split precision: solvent in cell + FP32 offsets / FP32 velocities / 8-bit species
   bool split = false;
   SpD S, Salt;
+  bool split_dirty = false;   // the accumulator copies may hold partial sums of a run that failed
   mpcx_cob* solo = nullptr;   // split runs: this engine as a batch of one (made on the first run)
   double* d_rsum = nullptr;   // [4][MAXSG] per-sphere totals of the restart evaluation
   int g_bulk = 0, g_spbulk = 0;   // co_bulk / sp_bulk workgroups (0: not chosen yet)
@@ -5776,10 +5777,18 @@ static int cob_begin(mpcx_cob* b, int* cstep_out) {
   }
   *cstep_out = cstep;
   if (b->split) {
+    // a run that ends without an error leaves every accumulator copy cleared
+    // (the cell pass clears what it reads); after a failed one, clear them all
     long long big = 0;
-    for (int r = 0; r < b->R; r++)
+    bool dirty = false;
+    for (int r = 0; r < b->R; r++) {
       big = std::max(big, NXCD * 3 * std::max(b->eng[r]->P.mpcnumcell, b->eng[r]->P.vol));
-    hipLaunchKernelGGL(sb_clear, dim3(grid_for(big, 256, 1024), b->R), dim3(256), 0, b->st, b->d_P, b->d_D);
+      dirty = dirty || b->eng[r]->split_dirty;
+    }
+    if (dirty) {
+      hipLaunchKernelGGL(sb_clear, dim3(grid_for(big, 256, 1024), b->R), dim3(256), 0, b->st, b->d_P, b->d_D);
+      for (int r = 0; r < b->R; r++) b->eng[r]->split_dirty = false;
+    }
     hipLaunchKernelGGL(fill_kernel<int>, dim3(grid_for(b->n_tiles, 256, 1024)), dim3(256), 0, b->st, b->d_icnt,
                        (long long)b->n_tiles, 0);   // inboxes of an interval that ended in an error
     hipLaunchKernelGGL(sb_own_ranges, dim3((b->n_xt + 255) / 256), dim3(256), 0, b->st, b->d_P, b->d_D, b->d_xt,
@@ -5839,7 +5848,16 @@ static int cob_end(mpcx_cob* b) {
   return 0;
 }
 
+static int cob_run_body(mpcx_cob* b, int64_t ncoll, int32_t use_graph);
+static void cob_mark_dirty(mpcx_cob* b) {
+  for (int r = 0; r < b->R; r++) b->eng[r]->split_dirty = true;
+}
 static int cob_run(mpcx_cob* b, int64_t ncoll, int32_t use_graph) {
+  const int rc0 = cob_run_body(b, ncoll, use_graph);
+  if (rc0) cob_mark_dirty(b);
+  return rc0;
+}
+static int cob_run_body(mpcx_cob* b, int64_t ncoll, int32_t use_graph) {
   int cstep = 0;
   int rc = cob_begin(b, &cstep);
   if (rc) return rc;
@@ -6033,6 +6051,7 @@ extern "C" int mpcx_cob_body_average(mpcx_cob* b, int64_t n, int32_t nr, int32_t
                          (const SpD*)(b->split ? b->d_S : nullptr), G, acc, vacc);
   }
   if (!rc) rc = cob_end(b);
+  if (rc) cob_mark_dirty(b);
   if (!rc) {
     if (hipMemcpy(counts, acc, bins * 4, hipMemcpyDeviceToHost) != hipSuccess ||
         hipMemcpy(vsum, vacc, b->R * 8, hipMemcpyDeviceToHost) != hipSuccess)

@@ -244,3 +244,35 @@ def test_split_reports_a_particle_thrown_out_of_the_box():
             e.run(1)
     finally:
         e.close()
+
+
+def test_split_run_after_a_failed_run_equals_a_fresh_engine():
+    """A run stopped by an error leaves partial sums in the accumulator
+    copies; the next run clears them first (runs that end cleanly leave them
+    cleared), so a new state run in the same engine matches a fresh engine."""
+    from main_amd._lib import MpcxError
+    p = _params(L=20)
+    good = S.collective_initial_state(p, seed=9)
+    bad = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in good.items()}
+    pos = np.stack([bad["x"], bad["y"], bad["z"]], axis=1)
+    sph = np.concatenate([np.stack([bad["xd"][:, i], bad["yd"][:, i], bad["zd"][:, i]], axis=1) for i in (0, 1)])
+    pick = int(np.argmax(np.min(np.linalg.norm(pos[:, None, :] - sph[None, :, :], axis=2), axis=1)))
+    bad["vx"][pick] = -1e25
+    e = _engine(p, bad, True, seed=31)
+    try:
+        with pytest.raises(MpcxError):
+            e.run(1)
+        e.set_state(good, tour=cell_tour(20, 20, 20))
+        e.setup(do_sort=True)
+        e.run(3)
+        a = e.get_state()
+    finally:
+        e.close()
+    f = _engine(p, good, True, seed=31)
+    try:
+        f.run(3)
+        b = f.get_state()
+    finally:
+        f.close()
+    for k in ("x", "y", "z", "vx", "vy", "vz", "flag", "xd", "vxd"):
+        assert np.array_equal(a[k], b[k]), k
