@@ -458,6 +458,112 @@ __device__ __forceinline__ void dimer_forces_and_rattle_v(const CoP& P, Dimer<ND
   }
 }
 
+// Wave forms of the one-dimer step: every lane of the wave holds the dimer
+// and computes the same scalars; in each RATTLE iteration lane c (0..5)
+// updates component c of the update -- sphere c / 3, axis c % 3 -- with the
+// expression of the one-thread form above, and the six results are read back
+// from lanes 0..5 into every lane.  The results are bitwise those of the
+// one-thread form, and an iteration's chain holds two or three divisions
+// instead of twelve (a wave issues every instruction whatever lanes are
+// active).  Lane 0 alone reports errors.
+__device__ __forceinline__ double pick6(int c, double a0, double a1, double a2, double b0, double b1, double b2) {
+  return c < 3 ? (c == 0 ? a0 : c == 1 ? a1 : a2) : (c == 3 ? b0 : c == 4 ? b1 : b2);
+}
+__device__ __forceinline__ void bcast6(double v, double& a0, double& a1, double& a2, double& b0, double& b1,
+                                       double& b2) {
+  a0 = lane_read_d(v, 0); a1 = lane_read_d(v, 1); a2 = lane_read_d(v, 2);
+  b0 = lane_read_d(v, 3); b1 = lane_read_d(v, 4); b2 = lane_read_d(v, 5);
+}
+__device__ __forceinline__ void rattle_positions_wave(const CoP& P, Dimer<1>& d, int* err, int lane) {
+  const double MDt = P.MDt, L0 = P.box[0], L1 = P.box[1];
+  const double prX[2] = {d.v[DX][0], d.v[DX][1]}, prY[2] = {d.v[DY][0], d.v[DY][1]},
+               prZ[2] = {d.v[DZ][0], d.v[DZ][1]};
+  const double dP0 = pbc_half(d.v[DX][0] - d.v[DX][1], L0);
+  const double dP1 = pbc_half(d.v[DY][0] - d.v[DY][1], L1);
+  const double dP2 = d.v[DZ][0] - d.v[DZ][1];
+  d.v[DQX][0] = d.v[DVX][0] + P.hc * d.v[DFX][0];
+  d.v[DQY][0] = d.v[DVY][0] + P.hc * d.v[DFY][0];
+  d.v[DQZ][0] = d.v[DVZ][0] + P.hc * d.v[DFZ][0];
+  d.v[DQX][1] = d.v[DVX][1] + P.hn * d.v[DFX][1];
+  d.v[DQY][1] = d.v[DVY][1] + P.hn * d.v[DFY][1];
+  d.v[DQZ][1] = d.v[DVZ][1] + P.hn * d.v[DFZ][1];
+#pragma unroll
+  for (int s = 0; s < 2; s++) {
+    double x = d.v[DX][s] + MDt * d.v[DQX][s];
+    double y = d.v[DY][s] + MDt * d.v[DQY][s];
+    d.v[DZ][s] = d.v[DZ][s] + MDt * d.v[DQZ][s];
+    d.v[DX][s] = wrap_periodic_any(x, L0);
+    d.v[DY][s] = wrap_periodic_any(y, L1);
+  }
+  const int c = lane < 6 ? lane : 0, ax = c % 3;
+  const bool first = c < 3;   // sphere 0 (C, mass Mc) or 1 (N, mass Mn)
+  const double pr = pick6(c, prX[0], prY[0], prZ[0], prX[1], prY[1], prZ[1]);
+  const double dPa = ax == 0 ? dP0 : (ax == 1 ? dP1 : dP2);
+  const double M = first ? P.Mc : P.Mn;
+  double q = pick6(c, d.v[DQX][0], d.v[DQY][0], d.v[DQZ][0], d.v[DQX][1], d.v[DQY][1], d.v[DQZ][1]);
+  for (long long acc = 0;; acc++) {
+    if (acc == RATTLE_CAP) { if (lane == 0) raise_error(err, ERR_RATTLE); break; }
+    const double d0 = pbc_half(d.v[DX][0] - d.v[DX][1], L0);
+    const double d1 = pbc_half(d.v[DY][0] - d.v[DY][1], L1);
+    const double d2 = d.v[DZ][0] - d.v[DZ][1];
+    const double c1 = (d0 * d0 + d1 * d1 + d2 * d2) - P.Dcn2;
+    if (fabs(c1) < 1e-6) break;
+    const double G = -(((0.25 * c1) * P.IIMcIMn) / (d0 * dP0 + d1 * dP1 + d2 * dP2));
+    const double g2 = 2.0 * G;
+    const double base = pr + q * MDt, dl = (g2 * dPa) / M;
+    double x = first ? base + dl : base - dl;
+    if (ax == 0) x = wrap_periodic_any(x, L0);
+    else if (ax == 1) x = wrap_periodic_any(x, L1);
+    const double dq = (g2 * dPa) / (M * MDt);
+    q = first ? q + dq : q - dq;
+    bcast6(x, d.v[DX][0], d.v[DY][0], d.v[DZ][0], d.v[DX][1], d.v[DY][1], d.v[DZ][1]);
+  }
+  bcast6(q, d.v[DQX][0], d.v[DQY][0], d.v[DQZ][0], d.v[DQX][1], d.v[DQY][1], d.v[DQZ][1]);
+}
+__device__ __forceinline__ void dimer_forces_and_rattle_v_wave(const CoP& P, Dimer<1>& d, const double* sumF,
+                                                               double& tpot, int* err, int lane) {
+  double tpp[2];
+#pragma unroll
+  for (int i = 0; i < 2; i++) {   // one dimer: no sphere-sphere terms (evolve_mod.f95:580)
+    double fw, wp;
+    lj_mot_wall(P, d.v[DZ][i], i, fw, wp);
+    double fx = 0.0 - sumF[4 * i + 0], fy = 0.0 - sumF[4 * i + 1], fz = 0.0 - sumF[4 * i + 2];
+    double tp = 0.0 + sumF[4 * i + 3];
+    fz = fz + fw;
+    tp = tp + wp;
+    d.v[DFX][i] = fx; d.v[DFY][i] = fy; d.v[DFZ][i] = fz;
+    tpp[i] = tp;
+  }
+  tpot = 0.0;
+  tpot = tpot + tpp[0];
+  tpot = tpot + tpp[1];
+  d.v[DVX][0] = d.v[DQX][0] + P.hc * d.v[DFX][0];
+  d.v[DVY][0] = d.v[DQY][0] + P.hc * d.v[DFY][0];
+  d.v[DVZ][0] = d.v[DQZ][0] + P.hc * d.v[DFZ][0];
+  d.v[DVX][1] = d.v[DQX][1] + P.hn * d.v[DFX][1];
+  d.v[DVY][1] = d.v[DQY][1] + P.hn * d.v[DFY][1];
+  d.v[DVZ][1] = d.v[DQZ][1] + P.hn * d.v[DFZ][1];
+  const int c = lane < 6 ? lane : 0, ax = c % 3;
+  const bool first = c < 3;
+  const double M = first ? P.Mc : P.Mn;
+  double v = pick6(c, d.v[DVX][0], d.v[DVY][0], d.v[DVZ][0], d.v[DVX][1], d.v[DVY][1], d.v[DVZ][1]);
+  for (long long acc = 0;; acc++) {
+    if (acc == RATTLE_CAP) { if (lane == 0) raise_error(err, ERR_RATTLE); break; }
+    const double dvx = d.v[DVX][0] - d.v[DVX][1], dvy = d.v[DVY][0] - d.v[DVY][1], dvz = d.v[DVZ][0] - d.v[DVZ][1];
+    const double d0 = pbc_half(d.v[DX][0] - d.v[DX][1], P.box[0]);
+    const double d1 = pbc_half(d.v[DY][0] - d.v[DY][1], P.box[1]);
+    const double d2 = d.v[DZ][0] - d.v[DZ][1];
+    const double c2 = 2.0 * (dvx * d0 + dvy * d1 + dvz * d2);
+    if (fabs(c2) < 1e-8) break;
+    const double O = -((0.25 * (P.IIMcIMn * c2)) / P.Dcn2);
+    const double o2 = 2.0 * O;
+    const double da = ax == 0 ? d0 : (ax == 1 ? d1 : d2);
+    const double dl = (o2 * da) / M;
+    v = first ? v + dl : v - dl;
+    bcast6(v, d.v[DVX][0], d.v[DVY][0], d.v[DVZ][0], d.v[DVX][1], d.v[DVY][1], d.v[DVZ][1]);
+  }
+}
+
 // ------------------------------------------------------------- near engine
 // One workgroup advances the near set of one system through a whole collision
 // interval.  The near particles are gathered once per interval into
