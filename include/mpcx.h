@@ -320,6 +320,11 @@ int mpcx_cob_set_bulk_slots(mpcx_cob* b, int32_t slots);
  * memory locality and the order of the near engines' force sums; a replica
  * matches its engine run alone at the same period.  FP64 batches refuse it. */
 int mpcx_cob_set_sort_step(mpcx_cob* b, int32_t steps);
+/* Entries per inbox of a split-precision batch (0: max(tile / 2, 256)).  An
+ * inbox receives the cell sums other particle tiles binned into the cells
+ * its tile owns; what does not fit goes through atomics into the
+ * accumulator copies instead, so results do not depend on it. */
+int mpcx_cob_set_inbox_cap(mpcx_cob* b, int32_t cap);
 int mpcx_cob_info(mpcx_cob* b, int64_t* particles, int64_t* cells, double* run_ms);
 /* Split batches run without the graph: mean kernel durations (ms) of the last
  * run from HIP events on the stream each runs on (every 8th interval, at most

@@ -81,6 +81,10 @@ class CollectiveBatch:
         check(lib().mpcx_cob_set_bulk_slots(self.h, C.c_int32(slots)), "mpcx_cob_set_bulk_slots")
         self.bulk_slots = slots
 
+    def set_inbox_cap(self, cap: int):
+        """Entries per inbox of a split batch (0: the default by tile size)."""
+        check(lib().mpcx_cob_set_inbox_cap(self.h, C.c_int32(cap)), "mpcx_cob_set_inbox_cap")
+
     def set_sort_step(self, steps: int):
         """Re-sort period of a split batch in MD steps (a multiple of MPCtstep)."""
         check(lib().mpcx_cob_set_sort_step(self.h, C.c_int32(steps)), "mpcx_cob_set_sort_step")

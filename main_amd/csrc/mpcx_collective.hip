@@ -5413,6 +5413,7 @@ struct mpcx_cob {
   FxEntry* d_ibox = nullptr;
   int2* d_orng = nullptr;
   int n_tiles = 0, icap = 0;
+  int icap_set = 0;   // entries per inbox when set (else max(tile / 2, 256))
   CoSortD* d_X = nullptr;   // sort buffers of each replica
   std::vector<CoSortD> h_X;
   // split storage: the replicas' split arrays and their second (sort) buffers
@@ -5553,7 +5554,7 @@ static int cob_tiles(mpcx_cob* b) {
       tb[r] = (int)n;
       n += (b->eng[r]->P.NT + b->tile - 1) / b->tile;
     }
-    b->icap = std::max(b->tile / 2, 256);
+    b->icap = b->icap_set > 0 ? b->icap_set : std::max(b->tile / 2, 256);
     if (n * (long long)b->icap > (1ll << 31)) return set_error(MPCX_E_ARG, "too many particle tiles for the inboxes");
     b->n_tiles = (int)n;
     HIPCHK(hipMalloc(&b->d_tb, b->R * sizeof(int)));
@@ -6034,6 +6035,18 @@ extern "C" int mpcx_cob_set_bulk_slots(mpcx_cob* b, int32_t slots) {
   cob_free_graph(b);
   b->bulk_slots = slots;
   return 0;
+}
+
+// entries per inbox of a split batch (0: max(tile / 2, 256)); the sums that
+// do not fit go through the atomics into the XCD copies, so the results do
+// not depend on it; drops a captured graph
+extern "C" int mpcx_cob_set_inbox_cap(mpcx_cob* b, int32_t cap) {
+  if (!b || cap < 0 || cap > (1 << 20)) return set_error(MPCX_E_ARG, "inbox capacity must be 0..1048576");
+  if (!b->split) return set_error(MPCX_E_UNSUPPORTED, "FP64 batches have no inboxes");
+  HIPCHK(hipStreamSynchronize(b->st));
+  cob_free_graph(b);
+  b->icap_set = cap;
+  return cob_tiles(b);
 }
 
 // re-sort period of a split batch in MD steps (a positive multiple of

@@ -240,6 +240,13 @@ module mpcx_c
       integer(c_int) :: rc
     end function
 
+    function mpcx_cob_set_inbox_cap(b, cap) bind(C, name="mpcx_cob_set_inbox_cap") result(rc)
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: b
+      integer(c_int32_t), value :: cap
+      integer(c_int) :: rc
+    end function
+
     function mpcx_shutdown() bind(C, name="mpcx_shutdown") result(rc)
       import :: c_int
       integer(c_int) :: rc

@@ -118,6 +118,34 @@ def test_split_batch_sort_period_knob_equals_engines_at_that_period():
         fp[0].close()
 
 
+def test_split_batch_overflowing_inboxes_equal_single_runs():
+    """Inboxes of two entries: nearly every sum another tile binned into a
+    cell goes through the atomics into the accumulator copies instead (the
+    overflow path), and every replica still ends bitwise equal to its engine
+    run alone."""
+    ncoll = 4   # a re-sort at the 2nd collision
+    ps = [_params(L, EbN, RPab, sort_step=100) for L, EbN, RPab in REPLICAS[:3]]
+    alone = []
+    for i, p in enumerate(ps):
+        e = _engine(p, 110 + i)
+        e.run(ncoll)
+        alone.append(e.get_state())
+        e.close()
+    engines = [_engine(p, 110 + i) for i, p in enumerate(ps)]
+    batch = CollectiveBatch(engines, tile=1024)
+    try:
+        batch.set_inbox_cap(2)
+        batch.run(ncoll, use_graph=True)
+        got = [e.get_state() for e in engines]
+    finally:
+        batch.close()
+        for e in engines:
+            e.close()
+    for i, (g, a) in enumerate(zip(got, alone)):
+        for k in KEYS:
+            assert np.array_equal(g[k], a[k]), f"replica {i}: {k}"
+
+
 def _ensemble(R, seed):
     rng = np.random.default_rng(seed)
     return [(int(rng.integers(20, 41)), float(rng.uniform(0.1, 1.0)), float(rng.uniform(0.2, 1.0)))
