@@ -56,7 +56,7 @@ enum : int { DX = 0, DY, DZ, DVX, DVY, DVZ, DFX, DFY, DFZ, DQX, DQY, DQZ };
 
 struct CoScalars {
   int cstep, time_count, time_penetrate, in_num;
-  int near_num, n_ev, err, pad;
+  int near_num, n_ev, err, err_site;   // err_site: the SITE_* of a split kernel's err, or 0
   int events[MAXEV];
   unsigned long long far_maxv2;    // bits of a non-negative double
   double near_vmax2;
@@ -68,11 +68,21 @@ struct CoScalars {
   double xs0[MAXSG], ys0[MAXSG], zs0[MAXSG];   // copy of x0..z0 made at the interval start (the near
                                                // engine overwrites x0..z0 while co_far still reads them)
   int n_wall, pad_w;            // far particles co_far left to co_bin_rest (they reach a wall)
-  int n_late, pad_l;            // late collision cells listed (zone around the spheres, deferred particles)
+  int n_late, err_idx;          // late collision cells listed (zone around the spheres, deferred particles);
+                                // the particle of a split kernel's err (-1: none named)
   unsigned long long next_fmax2;   // far max |v|^2 of the next interval while far_maxv2 is still in use
   unsigned xmask;                  // split storage: XCDs whose accumulator copies hold this interval's census
   unsigned sums_copied;            // split storage: some cell sums of this interval went to the copies
 };
+// where a split-storage kernel raised the run's first error (named in its report)
+enum : int { SITE_BIN = 1, SITE_BIN_CENSUS, SITE_STREAM, SITE_BULK_CELL, SITE_NEAR_STORE, SITE_INBOX, SITE_ROTATE,
+             SITE_SORT_BIN, SITE_PATCH, SITE_COUNT };
+__device__ __forceinline__ void raise_error_at(CoScalars* sc, int code, int site, int idx = -1) {
+  if (atomicCAS(&sc->err, 0, code) == 0) {
+    __hip_atomic_store(&sc->err_site, site, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&sc->err_idx, idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
 
 // Dimers whose near regions can overlap within an interval form a cluster;
 // each cluster is advanced by its own near-engine workgroup, and the
@@ -2844,9 +2854,14 @@ __device__ __forceinline__ bool sp_sane(const CoP& P, int ix, int iy, int iz, fl
          oy >= 0.0f && oy < 1.0f && oz >= 0.0f && oz < 1.0f && __builtin_isfinite(vx) && __builtin_isfinite(vy) &&
          __builtin_isfinite(vz);
 }
-// an FP64 position inside the box (z walls: 0 <= z <= Lz) and a finite velocity
+// an FP64 position inside the box and a finite velocity.  x = Lx (y = Ly)
+// is the periodic image of 0, which sp_split stores as cell 0, offset 0: the
+// reference's wrap "if (x < 0) x = x + L" yields it when a flight ends a
+// rounding error below 0, and a near particle whose FP32 start and velocity
+// put its flight's end on the boundary exactly (x + v tau = 0) gets there
+// with either sign of its FP64 steps' rounding.  z walls: 0 <= z <= Lz.
 __device__ __forceinline__ bool fp64_sane(const CoP& P, double x, double y, double z, double vx, double vy, double vz) {
-  return x >= 0.0 && x < P.box[0] && y >= 0.0 && y < P.box[1] && z >= 0.0 && z <= P.box[2] &&
+  return x >= 0.0 && x <= P.box[0] && y >= 0.0 && y <= P.box[1] && z >= 0.0 && z <= P.box[2] &&
          __builtin_isfinite(vx) && __builtin_isfinite(vy) && __builtin_isfinite(vz);
 }
 
@@ -2897,7 +2912,7 @@ __device__ __forceinline__ void sp_below_top(int& iz, float& oz, int Lz) {
 // of free flight, bounce-back at the z walls, A<->B on a reactive patch of z = 0
 __device__ __forceinline__ bool sp_stream(int& ix, int& iy, int& iz, float& ox, float& oy, float& oz, float& vx,
                                           float& vy, float& vz, int& flag, float dtc, const CoP& P,
-                                          const int* react_box, int* err) {
+                                          const int* react_box, CoScalars* sc) {
   ox = ox + vx * dtc;
   oy = oy + vy * dtc;
   oz = oz + vz * dtc;
@@ -2913,7 +2928,7 @@ __device__ __forceinline__ bool sp_stream(int& ix, int& iy, int& iz, float& ox, 
     sp_carry(hx, ax, P.Lx);
     sp_carry(hy, ay, P.Ly);
     const int hit = hx + P.Lx * hy + 1;
-    if (hit < 1 || hit > P.nrb) raise_error(err, ERR_CELL_RANGE);
+    if (hit < 1 || hit > P.nrb) raise_error_at(sc, ERR_CELL_RANGE, SITE_PATCH);
     else if (react_box[hit - 1] == 1) {
       if (P.react_dir == 0 && flag == 0) flag = 1;
       else if (P.react_dir == 1 && flag == 1) flag = 0;
@@ -3038,10 +3053,12 @@ static T* dalloc(mpcx_co* e, size_t n) {
   return (T*)p;
 }
 
-static int check_device_error(mpcx_co* e) {
-  int err = 0;
-  if (hipMemcpy(&err, &e->D.sc->err, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+static int check_device_error(mpcx_co* e, int replica = -1) {
+  int ev[2] = {0, 0}, idx = -1;   // err, err_site; err_idx
+  if (hipMemcpy(ev, &e->D.sc->err, sizeof(ev), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(&idx, &e->D.sc->err_idx, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
     return set_error(MPCX_E_HIP, "hipMemcpy of the error flag failed");
+  const int err = ev[0], site = ev[1];
   if (err) {
     static const char* msg[] = {"none",
                                 "a particle maps outside the cell arrays",
@@ -3064,9 +3081,25 @@ static int check_device_error(mpcx_co* e) {
                                 "more far particles reach a wall in one interval than the deferred list holds",
                                 "a near or deferred particle ended in a cell collided before the near engine finished",
                                 "more late collision cells than their list holds"};
+    static const char* where[SITE_COUNT] = {"",
+                                            "sb_bin (shifted cell)",
+                                            "sb_bin (census cell)",
+                                            "sb_bulk (free flight)",
+                                            "sb_bulk (state after a re-sort)",
+                                            "sb_near (near set back into split storage)",
+                                            "sb_cells_own (inbox entry of a cell the tile does not own)",
+                                            "sb_rotate (collision cell)",
+                                            "sb_sort_bin (unshifted cell)",
+                                            "sp_stream (reactive patch)"};
     std::string m = std::string("device check failed: ") + (err > 0 && err < 21 ? msg[err] : "unknown");
-    int zero = 0;
-    (void)hipMemcpy(&e->D.sc->err, &zero, sizeof(int), hipMemcpyHostToDevice);
+    if (site > 0 && site < SITE_COUNT) {
+      m += std::string(" [") + where[site];
+      if (idx >= 0) m += ", particle " + std::to_string(idx) + " (0-based)";
+      m += "]";
+    }
+    if (replica >= 0) m += " [replica " + std::to_string(replica) + "]";
+    const int zero[2] = {0, 0};
+    (void)hipMemcpy(&e->D.sc->err, zero, sizeof(zero), hipMemcpyHostToDevice);
     return set_error(MPCX_E_DEVICE, m.c_str());
   }
   return 0;
@@ -4897,7 +4930,7 @@ __device__ __forceinline__ void sb_bin(const CoP& P, const CoD& D, const SbV& A,
     const int cz = iz + (int)floorf(oz + fzs) + (zs < 0.0 ? 1 : 0);
     const long long c = (long long)cx + (long long)P.Lx * cy + (long long)P.Lx * P.Ly * cz;
     if (cx < 0 || cx >= P.Lx || cy < 0 || cy >= P.Ly || cz < 0 || c >= P.mpcnumcell) {
-      raise_error(&D.sc->err, ERR_CELL_RANGE);
+      raise_error_at(D.sc, ERR_CELL_RANGE, SITE_BIN);
       ok = false;
     } else {
       t = g1(D.mpc_pos)[c];   // cells are kept by tour position: a tile's cells lie in a few runs
@@ -4909,7 +4942,7 @@ __device__ __forceinline__ void sb_bin(const CoP& P, const CoD& D, const SbV& A,
   if (ok && f == 1 && !(inside && excl)) {   // the B census (all the bulk reaction reads)
     if (inside) key = (unsigned)P.NT + (unsigned)i;
     const long long u = (long long)ix + (long long)P.Lx * iy + (long long)P.Lx * P.Ly * iz;
-    if (u < 0 || u >= P.vol) raise_error(&D.sc->err, ERR_CELL_RANGE);
+    if (u < 0 || u >= P.vol) raise_error_at(D.sc, ERR_CELL_RANGE, SITE_BIN_CENSUS);
     else { cen = true; ck = g1(D.mpc_pos)[u]; }   // by tour position, as the cell sums
   }
   sb_insert(P, A, G, ok, t, vx, vy, vz, cen, ck, key);
@@ -5028,18 +5061,18 @@ __global__ __launch_bounds__(T) void sb_bulk(const CoP* __restrict__ Ps, const C
       sp_cell_xyz(P, cur.cell, ix, iy, iz);
       if (!ALL) {
         const int f0 = f;
-        const bool bounced = sp_stream(ix, iy, iz, ox, oy, oz, vx, vy, vz, f, dtc, P, D.react_box, &D.sc->err);
+        const bool bounced = sp_stream(ix, iy, iz, ox, oy, oz, vx, vy, vz, f, dtc, P, D.react_box, D.sc);
         if (sp_sane(P, ix, iy, iz, ox, oy, oz, vx, vy, vz)) {
           V.cell[i] = sp_pack(ix, iy, iz);
           V.ox[i] = ox; V.oy[i] = oy; V.oz[i] = oz;
           if (bounced) { V.vx[i] = vx; V.vy[i] = vy; V.vz[i] = vz; }
           if (f != f0) V.flag[i] = (signed char)f;
         } else {
-          raise_error(&D.sc->err, ERR_CELL_RANGE);
+          raise_error_at(D.sc, ERR_CELL_RANGE, SITE_STREAM);
           mine = false;
         }
       } else if (!sp_sane(P, ix, iy, iz, ox, oy, oz, vx, vy, vz)) {
-        raise_error(&D.sc->err, ERR_CELL_RANGE);
+        raise_error_at(D.sc, ERR_CELL_RANGE, SITE_BULK_CELL);
         mine = false;
       }
     }
@@ -5117,7 +5150,7 @@ __global__ __launch_bounds__(256) void sb_near(const CoP* __restrict__ Ps, const
       i = D.near_list[q];
       const double x = D.x[i], y = D.y[i], z = D.z[i], dvx = D.vx[i], dvy = D.vy[i], dvz = D.vz[i];
       if (!fp64_sane(P, x, y, z, dvx, dvy, dvz)) {   // reported; the particle keeps its split state
-        raise_error(&D.sc->err, ERR_CELL_RANGE);
+        raise_error_at(D.sc, ERR_CELL_RANGE, SITE_NEAR_STORE, i);
         valid = false;
       }
     }
@@ -5172,7 +5205,7 @@ __global__ __launch_bounds__(256) void sb_cells_own(const CoP* __restrict__ Ps, 
   if (own.x == 0 && threadIdx.x == 0) D.sc->far_maxv2 = 0ull;   // sb_rotate publishes the next interval's far maximum
   for (int e = threadIdx.x; e < n; e += 256) {   // every entry is for a cell the tile owns
     const long long q = box[e].p;
-    if (!((q >= lo[0] && q < hi[0]) || (q >= lo[1] && q < hi[1]))) raise_error(&D.sc->err, ERR_CELL_RANGE);
+    if (!((q >= lo[0] && q < hi[0]) || (q >= lo[1] && q < hi[1]))) raise_error_at(D.sc, ERR_CELL_RANGE, SITE_INBOX);
   }
   for (int part = 0; part < 2; part++)
     for (long long c0 = lo[part]; c0 < hi[part]; c0 += OWNCAP) {
@@ -5282,7 +5315,7 @@ __global__ __launch_bounds__(256) void sb_rotate(const CoP* __restrict__ Ps, con
         sp_rotate_velocity(P, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, vx, vy, vz);
         svx[i] = vx; svy[i] = vy; svz[i] = vz;
       } else {
-        raise_error(&D.sc->err, ERR_CELL_RANGE);
+        raise_error_at(D.sc, ERR_CELL_RANGE, SITE_ROTATE);
       }
       const float v2 = (vx * vx + vy * vy) + vz * vz;
       nb = (double)dq[i] * (1.0 / 32.0) <= sqrt((double)v2) * tau + reach0;
@@ -5331,7 +5364,7 @@ __global__ __launch_bounds__(256) void sb_sort_bin(const CoP* __restrict__ Ps, c
       sp_cell_xyz(P, S.cell[i], ix, iy, iz);
       const long long u = (long long)ix + (long long)P.Lx * (iy + (long long)P.Ly * iz);
       if (ix >= P.Lx || iy >= P.Ly || iz >= P.Lz || X.tourpos[u] < 0) {
-        raise_error(&D.sc->err, ERR_CELL_RANGE);
+        raise_error_at(D.sc, ERR_CELL_RANGE, SITE_SORT_BIN);
         X.key[i] = 0; X.rk[i] = 0;
       } else {
         t = X.tourpos[u];
@@ -5949,7 +5982,7 @@ static int cob_end(mpcx_cob* b) {
   HIPCHK(hipStreamSynchronize(b->st));
   for (int r = 0; r < b->R; r++) {
     if (!b->h_io[r]) continue;
-    int rc = check_device_error(b->eng[r]);   // reports and clears it
+    int rc = check_device_error(b->eng[r], r);   // reports and clears it
     if (rc) return rc;
   }
   return 0;

@@ -225,6 +225,40 @@ def test_split_keeps_particles_below_the_top_wall():
     assert np.all(g["z"] < Lz) and np.all(g["z"] >= 0.0)
 
 
+def test_split_stores_x_equal_to_the_box_edge_as_its_periodic_image():
+    """x = Lx (y = Ly) is the periodic image of 0: the reference's wrap
+    "if (x < 0) x = x + L" yields it when a flight ends a rounding error below
+    0, and a near particle whose FP32 start and velocity put its flight's end
+    on the boundary exactly (x + v tau = 0) reaches it with either sign of the
+    FP64 steps' rounding.  Split storage keeps it as cell 0, offset 0, where
+    the conversion from FP64 used to reject it and stop the run with a range
+    error (a regression test: a replica of the 256-system ensemble stopped so
+    in its 94th collision).  The set-up without a re-sort stores every particle
+    through that same conversion."""
+    p = _params(L=20)
+    s = S.collective_initial_state(p, seed=8)
+    L = float(p["box"][0])
+    sph = np.concatenate([np.stack([s["xd"][:, i], s["yd"][:, i], s["zd"][:, i]], axis=1) for i in (0, 1)])
+    cases = [(L, 12.5, 4.5), (12.5, L, 3.5), (L, L, 16.5)]   # FP32-exact coordinates
+    for x, y, z in cases:
+        for c in sph:
+            e = np.array([x, y, z]) - c
+            e[:2] -= L * np.round(e[:2] / L)
+            assert np.linalg.norm(e) > 6.0, "the planted particles stay clear of the spheres"
+    picks = [0, 1, 2]
+    for n, (x, y, z) in zip(picks, cases):
+        s["x"][n], s["y"][n], s["z"][n] = x, y, z
+    e = CollectiveEngine(p, rng_mode=1, seed=77, split=True)
+    try:
+        e.set_state(s, tour=cell_tour(20, 20, 20))
+        e.setup(do_sort=False)
+        g = e.get_state()
+    finally:
+        e.close()
+    for n, (x, y, z) in zip(picks, cases):
+        assert (g["x"][n], g["y"][n], g["z"][n]) == (x % L, y % L, z), n
+
+
 def test_split_reports_a_particle_thrown_out_of_the_box():
     """A near particle that the FP64 near engine carries far outside the box
     (planted here with vx = -1e25) is reported when the near set goes back to
@@ -240,7 +274,7 @@ def test_split_reports_a_particle_thrown_out_of_the_box():
     s["vx"][pick] = -1e25
     e = _engine(p, s, True)
     try:
-        with pytest.raises(MpcxError, match="outside the cell arrays"):
+        with pytest.raises(MpcxError, match=r"outside the cell arrays \[sb_near .*particle \d+"):
             e.run(1)
     finally:
         e.close()
