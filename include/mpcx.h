@@ -298,8 +298,10 @@ int mpcx_co_kernel_ms(mpcx_co* e, double* ms5, int64_t* count);
  * sort_step and storage precision; boxes, parameters and keys may differ)
  * with one launch per kernel of an interval for all of them: the near engine
  * with one workgroup per replica, the particle and cell kernels over tiles
- * of `tile` particles / 256 cells.  Split-precision replicas stream and bin
- * their far particles on a second stream beside the near engines.  Each
+ * of `tile` particles / 256 cells.  A split batch runs its interval in
+ * order (the near engines hold a CU each; a far flight beside them would
+ * only wait for CUs); a one-engine split run streams its far particles on a
+ * second stream beside its near engine.  Each
  * replica ends bit-identical to running it alone (a split engine's
  * mpcx_co_run is a batch of one); its state is read with mpcx_co_get_state.
  * With use_graph, the launches of an interval without a re-sort are

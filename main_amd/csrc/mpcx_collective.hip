@@ -5739,9 +5739,14 @@ static void sb_launch_bulk(mpcx_cob* b, bool all, hipStream_t s, int bin) {
 }
 
 // One split-precision interval of every replica (the kernels above).  With
-// overlap the far particles' flight and binning (st2) run beside the near
-// engines (st); the near set is binned on st when they are done; every cell
-// is collided and every particle rotated once both are.  A re-sort interval
+// overlap (a solo engine, whose one near-engine workgroup leaves the other
+// CUs free) the far particles' flight and binning (st2) run beside the near
+// engine (st); the near set is binned on st when it is done; every cell is
+// collided and every particle rotated once both are.  A batch runs them in
+// order: its near engines hold a CU each (LDS and VGPRs), so a far flight
+// launched beside them only waits for CUs, in dispatch order, and started
+// beside them or beside the near sets' binning it measured 3.78 and 3.81 ms
+// per interval of the 256-replica ensemble against 3.77 in order.  A re-sort interval
 // stops the near engines before the last step's list rebuild (the far flight
 // is not binned), re-sorts, lets them resume, then bins every particle.
 // slot >= 0: the interval's kernels are timed (events into that slot).
@@ -5854,7 +5859,7 @@ static int cob_make(mpcx_cob** out, int32_t R, mpcx_co* const* engines, int32_t 
   b->tile = tile;
   b->split = e0->split;
   b->solo = solo;
-  b->overlap = !solo || (e0->near_mode == 0 && e0->overlap);
+  b->overlap = solo && e0->near_mode == 0 && e0->overlap;   // a batch runs its interval in order
   b->eng.assign(engines, engines + R);
   b->h_D.resize(R);
   b->h_X.resize(R);
