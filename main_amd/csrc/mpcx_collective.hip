@@ -4693,25 +4693,28 @@ __device__ __forceinline__ long long sb_fix(float v) { return __double2ll_rn((do
 // cells of a tile grow as the particles drift from their sorted order (a
 // 4096-particle tile of 10 per cell: about 580 just after a re-sort, about
 // 1020 nine collisions later), so the tile size picks GT to keep the table
-// at most about half full.
-constexpr int SB_CT = 1024;    // census slots
+// at most about half full.  The census holds only the unshifted cells of B
+// particles, about a tenth of a tile's particles' worth of keys: the particle
+// tiles take CT = GT / 2 slots (with the 4-sphere table of a batch,
+// sb_bulk<.., 1024> fits four workgroups in a CU's LDS, eight waves per SIMD).
+constexpr int SB_CT = 1024;    // census slots of the near-set binning
 constexpr int SB_PROBE = 64;
 constexpr int SB_DEST = 64;    // owner tiles a tile sends foreign sums to (beyond: the XCD copies)
-template <int GT>
+template <int GT, int CT>
 struct SbTables {
-  static_assert((GT & (GT - 1)) == 0 && GT >= SB_CT, "slots: a power of two, at least the census slots");
+  static_assert((GT & (GT - 1)) == 0 && (CT & (CT - 1)) == 0 && GT >= CT, "slots: powers of two, GT >= CT");
   static constexpr int SHIFT = 32 - __builtin_ctz(GT);
   int rk[GT];
   unsigned long long rs[3][GT];
-  int ck[SB_CT], cc[SB_CT];
-  unsigned cm[SB_CT];
+  int ck[CT], cc[CT];
+  unsigned cm[CT];
   int dk[SB_DEST], dn[SB_DEST], db[SB_DEST];   // owner tile, entries for it, first inbox slot claimed
 };
-template <int GT>
-__device__ __forceinline__ void sb_tables_init(SbTables<GT>& G) {
+template <int GT, int CT>
+__device__ __forceinline__ void sb_tables_init(SbTables<GT, CT>& G) {
   for (int j = threadIdx.x; j < GT; j += blockDim.x) {
     G.rk[j] = -1; G.rs[0][j] = 0ull; G.rs[1][j] = 0ull; G.rs[2][j] = 0ull;
-    if (j < SB_CT) { G.ck[j] = -1; G.cc[j] = 0; G.cm[j] = 0xffffffffu; }
+    if (j < CT) { G.ck[j] = -1; G.cc[j] = 0; G.cm[j] = 0xffffffffu; }
     if (j < SB_DEST) { G.dk[j] = -1; G.dn[j] = 0; }
   }
   __syncthreads();
@@ -4729,8 +4732,8 @@ __device__ __forceinline__ int sb_slot(int* keys, int key) {
   return -1;
 }
 // one lane's velocity into cell t (tour position; ok) and census entry ck with key (cen); no barrier
-template <int GT>
-__device__ __forceinline__ void sb_insert(const CoP& P, const SbV& A, SbTables<GT>& G, bool ok, int t, float vx, float vy,
+template <int GT, int CT>
+__device__ __forceinline__ void sb_insert(const CoP& P, const SbV& A, SbTables<GT, CT>& G, bool ok, int t, float vx, float vy,
                                           float vz, bool cen, int ck, unsigned key) {
   if (ok) {
     const unsigned long long w0 = (unsigned long long)sb_fix(vx), w1 = (unsigned long long)sb_fix(vy);
@@ -4749,7 +4752,7 @@ __device__ __forceinline__ void sb_insert(const CoP& P, const SbV& A, SbTables<G
     }
   }
   if (cen) {
-    const int j = sb_slot<SB_CT>(G.ck, ck);
+    const int j = sb_slot<CT>(G.ck, ck);
     if (j >= 0) {
       atomicAdd(&G.cc[j], 1);
       atomicMin(&G.cm[j], key);
@@ -4835,8 +4838,8 @@ __device__ __forceinline__ int owner_tile(const CoP& P, const CoD& D, const SbIn
 // its run [e0, e1) of the extra layer) stored whole -- zero for an owned cell
 // it binned nothing into -- and the other table entries sent to their
 // owners' inboxes (the near-set tiles of sb_near own nothing)
-template <int GT>
-__device__ __forceinline__ void sb_flush_owned(const CoP& P, const CoD& D, const SbV& A, SbTables<GT>& G, int plo,
+template <int GT, int CT>
+__device__ __forceinline__ void sb_flush_owned(const CoP& P, const CoD& D, const SbV& A, SbTables<GT, CT>& G, int plo,
                                                int phi, int e0, int e1, const SbInbox& X, int r) {
   __syncthreads();
   const gp<unsigned long long> own = g1(D.sown);
@@ -4893,7 +4896,7 @@ __device__ __forceinline__ void sb_flush_owned(const CoP& P, const CoD& D, const
         *A.copied = 1u;
       }
     }
-    const int c = j < SB_CT ? G.ck[j] : -1;
+    const int c = j < CT ? G.ck[j] : -1;
     if (c >= 0) {
       l2_add<int>(A.tc + c, G.cc[j]);   // checked against CENSUS_MAX by sb_cells_own
       l2_min(A.fb + c, G.cm[j]);
@@ -4903,9 +4906,10 @@ __device__ __forceinline__ void sb_flush_owned(const CoP& P, const CoD& D, const
 
 // The spheres of the distance bytes: each coordinate as its cell (integer)
 // and the offset in it (FP32)
-struct SbSpheres {
-  int ic[3][MAXSG];
-  float fr[3][MAXSG];
+template <int NS>
+struct SbSpheres {   // NS: sphere capacity (4 in a batch of one- or two-dimer replicas)
+  int ic[3][NS];
+  float fr[3][NS];
 };
 __device__ __forceinline__ float sp_pbc(float d, float L) {
   return d > 0.5f * L ? d - L : (d < -0.5f * L ? d + L : d);
@@ -4917,11 +4921,11 @@ __device__ __forceinline__ float sp_pbc(float d, float L) {
 // entry (outside-list particles keyed by index, inside ones after them, inside
 // ones within an LJ cutoff left out), and the distance byte to the spheres sph
 // (the selection of the next near set).
-template <int GT>
-__device__ __forceinline__ void sb_bin(const CoP& P, const CoD& D, const SbV& A, SbTables<GT>& G, int i, bool valid, bool inside,
+template <int GT, int CT, int NS>
+__device__ __forceinline__ void sb_bin(const CoP& P, const CoD& D, const SbV& A, SbTables<GT, CT>& G, int i, bool valid, bool inside,
                                        bool excl, int ix, int iy, int iz, float ox, float oy, float oz, float vx,
                                        float vy, float vz, int f, float fxs, float fys, float fzs, double zs,
-                                       const SbSpheres& sph) {
+                                       const SbSpheres<NS>& sph) {
   bool ok = valid;
   int t = 0;
   if (valid) {
@@ -4974,8 +4978,9 @@ __device__ __forceinline__ void sb_bin(const CoP& P, const CoD& D, const SbV& A,
   }
 }
 
+template <int NS>
 __device__ __forceinline__ void sb_load_spheres(const CoP& P, const double* x, const double* y, const double* z,
-                                                SbSpheres& sph) {
+                                                SbSpheres<NS>& sph) {
   for (int j = threadIdx.x; j < 2 * P.nd; j += blockDim.x) {
     const double c[3] = {x[j], y[j], z[j]};
     for (int a = 0; a < 3; a++) {
@@ -5020,14 +5025,17 @@ __device__ __forceinline__ void sb_fetch(const SbV& V, long long i, long long en
 // and, with do_bin, are binned with distances to the spheres' start
 // positions.  ALL = true (a re-sort interval, after the sort and the near
 // engine): every particle binned where it is, distances to the spheres' end
-// positions.  Each round's loads are issued before the previous round is
-// binned, so their latency overlaps the binning's LDS work and barriers.
-template <bool ALL, int T, int GT>
-__global__ __launch_bounds__(T) void sb_bulk(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds,
+// positions.  The next round's loads are issued before a round is binned,
+// but gfx9 counts loads and stores in one counter and the round's stores make
+// the compiler wait for all of them there: the latency is hidden by
+// occupancy (NS = 4 spheres and GT / 2 census slots: four 512-thread
+// workgroups per CU at GT = 1024), not by the prefetch.
+template <bool ALL, int T, int GT, int NS>
+__global__ __launch_bounds__(T, (NS == 4 && GT == 1024) ? 8 : 1) void sb_bulk(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds,
                                              const SpD* __restrict__ Ss, const Tile* __restrict__ tiles, int do_bin,
                                              SbInbox X) {
-  __shared__ SbTables<GT> G;
-  __shared__ SbSpheres sph;
+  __shared__ SbTables<GT, GT / 2> G;
+  __shared__ SbSpheres<NS> sph;
   const Tile tl = tiles[blockIdx.x];
   if (tl.begin >= tl.end) return;
   const CoP P = Ps[tl.r];
@@ -5123,8 +5131,8 @@ __global__ __launch_bounds__(256) void sb_load_near(const CoP* __restrict__ Ps, 
 __global__ __launch_bounds__(256) void sb_near(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds,
                                                const SpD* __restrict__ Ss, const Tile* __restrict__ tiles, int do_bin,
                                                int check, SbInbox X) {
-  __shared__ SbTables<SB_CT> G;
-  __shared__ SbSpheres sph;
+  __shared__ SbTables<SB_CT, SB_CT> G;
+  __shared__ SbSpheres<MAXSG> sph;
   const Tile tl = tiles[blockIdx.x];
   if (tl.begin >= tl.end) return;
   const CoP P = Ps[tl.r];
@@ -5724,13 +5732,20 @@ static int cob_sort_interval(mpcx_cob* b) {
   return 0;
 }
 
-// sb_bulk over the XCD-placed particle tiles, 512 threads per workgroup
-template <int GT>
-static void sb_launch_bulk_t(mpcx_cob* b, bool all, hipStream_t s, int bin) {
+// sb_bulk over the XCD-placed particle tiles, 512 threads per workgroup; the
+// replicas of a batch hold at most two dimers (checked at its creation), so
+// their sphere table has 4 entries
+template <int GT, int NS>
+static void sb_launch_bulk_ns(mpcx_cob* b, bool all, hipStream_t s, int bin) {
   const dim3 g(b->n_xt);
   const SbInbox X = cob_inbox(b);
-  if (all) hipLaunchKernelGGL((sb_bulk<true, 512, GT>), g, dim3(512), 0, s, b->d_P, b->d_D, b->d_S, b->d_xt, bin, X);
-  else hipLaunchKernelGGL((sb_bulk<false, 512, GT>), g, dim3(512), 0, s, b->d_P, b->d_D, b->d_S, b->d_xt, bin, X);
+  if (all) hipLaunchKernelGGL((sb_bulk<true, 512, GT, NS>), g, dim3(512), 0, s, b->d_P, b->d_D, b->d_S, b->d_xt, bin, X);
+  else hipLaunchKernelGGL((sb_bulk<false, 512, GT, NS>), g, dim3(512), 0, s, b->d_P, b->d_D, b->d_S, b->d_xt, bin, X);
+}
+template <int GT>
+static void sb_launch_bulk_t(mpcx_cob* b, bool all, hipStream_t s, int bin) {
+  if (!b->solo && 2 * b->nd <= 4) sb_launch_bulk_ns<GT, 4>(b, all, s, bin);
+  else sb_launch_bulk_ns<GT, MAXSG>(b, all, s, bin);
 }
 static void sb_launch_bulk(mpcx_cob* b, bool all, hipStream_t s, int bin) {
   const int slots = b->bulk_slots ? b->bulk_slots : (b->tile >= 4096 ? 2048 : 1024);
