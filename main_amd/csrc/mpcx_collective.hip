@@ -2204,7 +2204,10 @@ __global__ __launch_bounds__(256) void co_bulk(CoP P, CoD D, int do_stream, int 
   bulk_body<false>(P, D, do_stream, do_bin, (long long)blockIdx.x * blockDim.x, P.NT,
                    (long long)gridDim.x * blockDim.x);
 }
-__global__ __launch_bounds__(256) void co_far(CoP P, CoD D, int do_bin, int slab) {
+// six waves per SIMD (at most 80 VGPRs, four of them spilled) where the
+// register count alone gave five: 0.376 -> 0.358 ms at 100^3 (seven waves
+// spilled sixteen and gained less)
+__global__ __launch_bounds__(256, 6) void co_far(CoP P, CoD D, int do_bin, int slab) {
   bulk_body<true>(P, D, 1, do_bin, (long long)blockIdx.x * blockDim.x, P.NT, (long long)gridDim.x * blockDim.x,
                   slab);
 }
