@@ -5131,11 +5131,15 @@ __global__ __launch_bounds__(256) void sb_load_near(const CoP* __restrict__ Ps, 
 // distances to the spheres' start
 // positions, as sb_bulk<false>); with check, the flight check (the near
 // engine's rebuild events must add up to the interval sb_bulk flew).
+// NS = 4 (a batch of one- or two-dimer replicas) takes CT = SB_CT / 2 census
+// slots: 35.7 KB of LDS, four workgroups per CU, so a 256-replica batch's
+// 1024 near-set tiles run at once.
+template <int NS>
 __global__ __launch_bounds__(256) void sb_near(const CoP* __restrict__ Ps, const CoD* __restrict__ Ds,
                                                const SpD* __restrict__ Ss, const Tile* __restrict__ tiles, int do_bin,
                                                int check, SbInbox X) {
-  __shared__ SbTables<SB_CT, SB_CT> G;
-  __shared__ SbSpheres<MAXSG> sph;
+  __shared__ SbTables<SB_CT, NS == 4 ? SB_CT / 2 : SB_CT> G;
+  __shared__ SbSpheres<NS> sph;
   const Tile tl = tiles[blockIdx.x];
   if (tl.begin >= tl.end) return;
   const CoP P = Ps[tl.r];
@@ -5745,6 +5749,14 @@ static void sb_launch_bulk_ns(mpcx_cob* b, bool all, hipStream_t s, int bin) {
   if (all) hipLaunchKernelGGL((sb_bulk<true, 512, GT, NS>), g, dim3(512), 0, s, b->d_P, b->d_D, b->d_S, b->d_xt, bin, X);
   else hipLaunchKernelGGL((sb_bulk<false, 512, GT, NS>), g, dim3(512), 0, s, b->d_P, b->d_D, b->d_S, b->d_xt, bin, X);
 }
+// sb_near over the near-set tiles (the 4-sphere instance for a batch, as sb_bulk)
+static void sb_launch_near(mpcx_cob* b, hipStream_t s, int bin, int check) {
+  if (!b->solo && 2 * b->nd <= 4)
+    hipLaunchKernelGGL(sb_near<4>, dim3(b->n_nt), dim3(256), 0, s, b->d_P, b->d_D, b->d_S, b->d_nt, bin, check, cob_inbox(b));
+  else
+    hipLaunchKernelGGL(sb_near<MAXSG>, dim3(b->n_nt), dim3(256), 0, s, b->d_P, b->d_D, b->d_S, b->d_nt, bin, check,
+                       cob_inbox(b));
+}
 template <int GT>
 static void sb_launch_bulk_t(mpcx_cob* b, bool all, hipStream_t s, int bin) {
   if (!b->solo && 2 * b->nd <= 4) sb_launch_bulk_ns<GT, 4>(b, all, s, bin);
@@ -5804,7 +5816,7 @@ static int sb_interval(mpcx_cob* b, bool sort_due, long long slot) {
     HIPCHK(tmark(1, 1, st));
   }
   HIPCHK(tmark(4, 0, st));
-  hipLaunchKernelGGL(sb_near, dim3(b->n_nt), dim3(256), 0, st, b->d_P, b->d_D, b->d_S, b->d_nt, bin, 1, cob_inbox(b));
+  sb_launch_near(b, st, bin, 1);
   HIPCHK(tmark(4, 1, st));
   if (ov) HIPCHK(hipStreamWaitEvent(st, b->join, 0));
   if (sort_due) {
@@ -5813,7 +5825,7 @@ static int sb_interval(mpcx_cob* b, bool sort_due, long long slot) {
     cob_near_list(b);
     if (b->solo) hipLaunchKernelGGL(sb_load_near, ng, dim3(256), 0, st, b->d_P, b->d_D, b->d_S, 0, 1);
     HIPCHK(cob_launch_near(b, 1, M_RESUME));
-    hipLaunchKernelGGL(sb_near, dim3(b->n_nt), dim3(256), 0, st, b->d_P, b->d_D, b->d_S, b->d_nt, 0, 0, cob_inbox(b));
+    sb_launch_near(b, st, 0, 0);
     sb_launch_bulk(b, true, st, 1);
   }
   HIPCHK(tmark(2, 0, st));
