@@ -343,7 +343,7 @@ def split_batched(R=256, steps=20, warmup=4, seed=4357787, fp64_value=None):
     kernels = kernel_rooflines(kms, ALGO_BYTES_SPLIT, n)
     slots = batch.bulk_slots or (2048 if batch.tile >= 4096 else 1024)
     for k, v in kernels.items():   # the binning pass of the tuned table size
-        v["traffic"] = read_traffic(f"sb_bulk<false,512,{slots}>" if k == "sb_bulk" else k, TRAFFIC_SPLIT_FILE)
+        v["traffic"] = read_traffic(f"sb_bulk<false,512,{slots},4>" if k == "sb_bulk" else k, TRAFFIC_SPLIT_FILE)
     print(f"# split batched: {ms:.3f} ms/interval, {value:.3e} particle-updates/s, kernels {kms}", file=sys.stderr,
           flush=True)
     out = {"replicas": R, "particles": n, "collision_cells": info["cells"], "timed_intervals": steps,
@@ -388,7 +388,7 @@ def step_traffic(sort_every=10):
 def read_traffic(kernel, path=TRAFFIC_FILE):
     """Bytes per launch of `kernel` between L2 and memory from a committed
     counter summary (an upper bound on its HBM bytes): the line of that exact
-    name (template arguments without spaces, e.g. sb_bulk<false,512,1024>),
+    name (template arguments without spaces, e.g. sb_bulk<false,512,1024,4>),
     else the first line of that base name."""
     if not os.path.exists(path):
         return None
