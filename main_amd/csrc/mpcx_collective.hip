@@ -930,8 +930,9 @@ constexpr int SG_CELLS = 1024;  // coarse cells of the sphere grid
 constexpr int SG_CAP = 8;       // spheres listed per coarse cell (more: the cell scans all spheres)
 constexpr int MM_CAP = 8;       // sphere-sphere candidates per sphere (more: all spheres)
 struct GeSync {
-  unsigned count, gen;
+  unsigned count, gen;   // gen: the last barrier epoch released (set to 0 before every launch)
   int abort, pad;
+  unsigned arrive[GE_GMAX];   // the last barrier epoch each workgroup reached
   int cnt[GE_GMAX];
   unsigned long long vmax2[GE_GMAX];
   // decision block written by workgroup 0
@@ -958,6 +959,7 @@ struct GeShared {
   int wsum[GE_W + 1];
   int cstep, tc, tp, in_num, nev, rebuild;
   int qb, qe, ib, ie;
+  unsigned ep;   // grid barriers passed in this launch
   double vmax2, red[GE_W];
   unsigned long long tm[8];   // real-time ticks by phase (workgroup 0 reports them)
   unsigned touched[2][(MAXSG + 31) / 32];   // spheres this workgroup published, by step parity
@@ -981,35 +983,69 @@ __device__ __forceinline__ int wave_min_i(int v) {
   return v;
 }
 
-// grid barrier of the G workgroups (thread 0); bounded like clu_exchange_max
-__device__ __attribute__((noinline)) void ge_barrier_t0(GeSync* sy, int G, int* err) {
+// Grid barrier of the G workgroups, run by wave 0 of each: every workgroup
+// stores the barrier's epoch in its own arrival word, wave 0 of workgroup 0
+// polls all of them (two words a lane) and then releases the epoch through
+// gen, which the others poll.  The data the workgroups exchange across it
+// across the barriers of the MD steps (sphere sums, touch bits) are
+// agent-scope atomics (written through L2, read past it), and between list
+// rebuilds every entry and slot keeps its owner workgroup; so a step's
+// barrier is light: every wave waits for its stores to complete (s_waitcnt 0,
+// the wait an agent-scope release makes after its L2 write-back; a
+// workgroup-scope release fence emits none on gfx950) before the arrival, and
+// no L2 write-back or invalidate follows.  The barriers of a rebuild and of
+// the interval start, where entries change owners, keep the agent-scope
+// fences: without the write-back, a line the old owner left dirty in its
+// XCD's L2 could be evicted over the new owner's data.  With
+// agent-scope fences (L2 write-back and invalidate on every XCD) a barrier
+// of 128 workgroups measured 13 us, with one counter or with these words;
+// with workgroup-scope fences, 4 us.  Bounded like clu_exchange_max.
+__device__ __attribute__((noinline)) void ge_barrier_wave(GeSync* sy, int G, unsigned ep, int* err) {
+  const int lane = threadIdx.x & 63;
   if (G <= 1) return;
-  const unsigned gen = __hip_atomic_load(&sy->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-  __threadfence();
-  if (atomicAdd(&sy->count, 1u) == (unsigned)G - 1u) {
-    atomicExch(&sy->count, 0u);
-    __threadfence();
-    atomicAdd(&sy->gen, 1u);
-  } else {
+  if (lane == 0) __hip_atomic_store(&sy->arrive[blockIdx.x], ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (blockIdx.x == 0) {
     long long spins = 0;
-    while (__hip_atomic_load(&sy->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+    while (true) {
+      bool ok = true;
+      for (int g = lane; g < G; g += 64)
+        ok = ok && (int)(__hip_atomic_load(&sy->arrive[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - ep) >= 0;
+      if (__all(ok)) break;
+      if (__hip_atomic_load(&sy->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || ++spins > SPIN_CAP) {
+        if (lane == 0) { raise_error(err, ERR_BARRIER); atomicExch(&sy->abort, 1); }
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    if (lane == 0) __hip_atomic_store(&sy->gen, ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else if (lane == 0) {
+    long long spins = 0;
+    while ((int)(__hip_atomic_load(&sy->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - ep) < 0) {
       if (__hip_atomic_load(&sy->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || ++spins > SPIN_CAP) {
         raise_error(err, ERR_BARRIER);
         atomicExch(&sy->abort, 1);
         break;
       }
-      __builtin_amdgcn_s_sleep(2);
+      __builtin_amdgcn_s_sleep(1);
     }
   }
-  __threadfence();
 }
-__device__ __forceinline__ void ge_sync(const GeD& E, int* err) {
-  __threadfence();
+__device__ __forceinline__ void ge_sync(const GeD& E, int* err, bool light = false) {
+  if (light) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");   // keeps the compiler's order
+    __builtin_amdgcn_s_waitcnt(0);                           // this wave's stores have completed
+  } else {
+    __threadfence();
+  }
   __syncthreads();
+  const unsigned ep = g_ge.ep + 1u;
   GE_T0();
-  if (threadIdx.x == 0) ge_barrier_t0(E.sy, (int)gridDim.x, err);
+  if (threadIdx.x < 64) ge_barrier_wave(E.sy, (int)gridDim.x, ep, err);
   GE_T1(5);
   __syncthreads();
+  if (light) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  else __threadfence();
+  if (threadIdx.x == 0) g_ge.ep = ep;   // read again only after the next barrier's first __syncthreads
 }
 
 __device__ __forceinline__ int sg_cell(double x, double y, double z) {
@@ -1567,8 +1603,10 @@ __device__ __forceinline__ void ge_post(const CoP& P, const GeD& E, double mx, i
       a[c] = t;
       any = any || t != 0.0;
     }
-    if (any) {
-      for (int c = 0; c < 4; c++) part[4 * sph + c] = a[c];
+    if (any) {   // read by other workgroups after the barrier: agent-scope stores
+      for (int c = 0; c < 4; c++)
+        __hip_atomic_store((unsigned long long*)&part[4 * sph + c], (unsigned long long)__double_as_longlong(a[c]),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       atomicOr(&tb[sph * (GE_GMAX / 32) + (blockIdx.x >> 5)], 1u << (blockIdx.x & 31));
       atomicOr(&g_ge.touched[par][sph >> 5], 1u << (sph & 31));
     }
@@ -1576,7 +1614,8 @@ __device__ __forceinline__ void ge_post(const CoP& P, const GeD& E, double mx, i
   if (threadIdx.x == 0) {
     double m = 0.0;
     for (int w = 0; w < GE_W; w++) m = fmax(m, g_ge.red[w]);
-    part[4 * MAXSG] = m;
+    __hip_atomic_store((unsigned long long*)&part[4 * MAXSG], (unsigned long long)__double_as_longlong(m),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   (void)G;
 }
@@ -1639,6 +1678,7 @@ __global__ __launch_bounds__(GE_T) void co_ge_engine(CoP P, CoD D, GeD E, int ns
   const int nn = min(sc->near_num, D.near_cap);
   const int chunk = (nn + G - 1) / G;
   if (tid == 0) {
+    g_ge.ep = 0u;   // ge_reset cleared the barrier words before this launch
     g_ge.cstep = sc->cstep; g_ge.tc = sc->time_count; g_ge.tp = sc->time_penetrate; g_ge.in_num = sc->in_num;
     g_ge.nev = (mode & M_RESUME) ? sc->n_ev : 0;
     g_ge.vmax2 = sc->near_vmax2;
@@ -1720,7 +1760,7 @@ __global__ __launch_bounds__(GE_T) void co_ge_engine(CoP P, CoD D, GeD E, int ns
       ge_slot_store(D, k, x, y, z, vx, vy, vz, fl, ms);
     }
     ge_post(P, E, mx, par);
-    ge_sync(E, err);
+    ge_sync(E, err, true);
     const double m = ge_totals(P, E, par);
     par ^= 1;
     ge_dimers(P, D, E, false, true, m);
@@ -1756,7 +1796,7 @@ __global__ __launch_bounds__(GE_T) void co_ge_engine(CoP P, CoD D, GeD E, int ns
       ge_post(P, E, mx, par);
       GE_T1(2);
     }
-    ge_sync(E, err);
+    ge_sync(E, err, true);
     double m;
     {
       GE_T0();
@@ -3548,7 +3588,8 @@ static void co_near_list(mpcx_co* e) {
 // before every launch of the global engine: barrier state and the
 // workgroup bits of the sphere sums (the last step's bits stay set)
 __global__ void ge_reset(GeSync* sy, unsigned* touch) {
-  if (threadIdx.x == 0) { sy->count = 0u; sy->abort = 0; }
+  if (threadIdx.x == 0) { sy->count = 0u; sy->abort = 0; sy->gen = 0u; }
+  for (int g = threadIdx.x; g < GE_GMAX; g += blockDim.x) sy->arrive[g] = 0u;
   for (int j = threadIdx.x; j < 2 * MAXSG * (GE_GMAX / 32); j += blockDim.x) touch[j] = 0u;
 }
 

@@ -203,6 +203,33 @@ def test_reruns_are_bit_identical():
         assert np.array_equal(outs[0][k], outs[1][k]), k
 
 
+def test_global_engine_reruns_are_bit_identical():
+    """The cooperative near engine over many workgroups (the reference input's
+    20 dimers in 64x64x32, 1.25e6 particles): two runs from the same seed end
+    bitwise equal.  Its MD-step barriers skip the L2 write-back (the sphere
+    sums crossing them are agent-scope atomics and no entry changes owner
+    between list rebuilds), so a missed exchange through plain memory would
+    show here as a difference between reruns."""
+    from bench import MANY_DIMER
+    p = S.collective_params(**MANY_DIMER["reference_input_20_dimers_64x64x32"], rhoa=10.0, Rc=2.0, Rn=4.0, Dcn=6.5,
+                            EbN=0.1, T=1.0 / 6.0, skinrad=0.3, MDt=0.01, MPCt=0.5, sort_step=500, k1=1e-3)
+    out = []
+    for _ in range(2):
+        eng = CollectiveEngine(p, rng_mode=1, seed=77)
+        try:
+            eng.set_dimers(S.collective_dimers(p, seed=77), tour=cell_tour(64, 64, 32))
+            eng.init_solvent(77)
+            eng.setup(do_sort=True)
+            eng.run(12)
+            out.append(eng.get_state())
+        finally:
+            eng.close()
+    a, b = out
+    for k in ("x", "y", "z", "vx", "vy", "vz", "flag", "xd", "yd", "zd", "vxd", "vyd", "vzd", "fxd"):
+        assert np.array_equal(a[k], b[k]), k
+    assert a["cstep"] == b["cstep"] == 12 * p["MPCtstep"]
+
+
 def test_overlap_matches_the_event_replay():
     """The far particles' single flight beside the near engine against the
     bitwise replay of the rebuild events after it, 40^3 headline physics,
