@@ -1629,17 +1629,37 @@ __device__ __forceinline__ double ge_totals(const CoP& P, const GeD& E, int par)
   const int nd = P.nd, nv = 4 * 2 * nd, G = (int)gridDim.x;
   const double* part = E.part + (size_t)par * GE_GMAX * (4 * MAXSG + 1);
   const unsigned* tb = E.touch + (size_t)par * MAXSG * (GE_GMAX / 32);
+  // every word of the sphere's bits, then up to 8 of its workgroups' sums at
+  // a time, loaded before any is added (each load crosses L2 to memory), and
+  // added in workgroup order as before
+  constexpr int NW = GE_GMAX / 32, NB = 8;
   for (int v = threadIdx.x; v < nv; v += GE_T) {
     const int sph = v >> 2;
+    unsigned bw[NW];
+#pragma unroll
+    for (int wd = 0; wd < NW; wd++)
+      bw[wd] = __hip_atomic_load(&tb[sph * NW + wd], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     double a = 0.0;
-    for (int wd = 0; wd < GE_GMAX / 32; wd++) {
-      unsigned bits = __hip_atomic_load(&tb[sph * (GE_GMAX / 32) + wd], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+    for (int wd = 0; wd < NW; wd++) {
+      unsigned bits = bw[wd];
       while (bits) {
-        const int w = wd * 32 + __ffs(bits) - 1;
-        bits &= bits - 1u;
-        a = a + __longlong_as_double((long long)__hip_atomic_load(
-                    (unsigned long long*)&part[(size_t)w * (4 * MAXSG + 1) + v], __ATOMIC_RELAXED,
-                    __HIP_MEMORY_SCOPE_AGENT));
+        double val[NB];
+        int cnt = 0;
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+          val[j] = 0.0;
+          if (bits) {
+            const int w = wd * 32 + __ffs(bits) - 1;
+            bits &= bits - 1u;
+            val[j] = __longlong_as_double((long long)__hip_atomic_load(
+                (unsigned long long*)&part[(size_t)w * (4 * MAXSG + 1) + v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            cnt = j + 1;
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < NB; j++)
+          if (j < cnt) a = a + val[j];
       }
     }
     g_ge.fin[v] = a;
