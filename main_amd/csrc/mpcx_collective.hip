@@ -964,6 +964,8 @@ struct GeShared {
   int cstep, tc, tp, in_num, nev, rebuild;
   int qb, qe, ib, ie;
   unsigned ep;   // grid barriers passed in this launch
+  int rc[GE_GMAX];         // every workgroup's published count and max (ge_read_counts)
+  double rv[GE_GMAX];
   double vmax2, red[GE_W];
   unsigned long long tm[8];   // real-time ticks by phase (workgroup 0 reports them)
   unsigned touched[2][(MAXSG + 31) / 32];   // spheres this workgroup published, by step parity
@@ -1187,15 +1189,24 @@ __device__ __forceinline__ void ge_publish_count(const GeD& E, int c, double vm)
                        __HIP_MEMORY_SCOPE_AGENT);
   }
 }
+// (every thread together: one load per workgroup's words, one thread each,
+// where every thread loading all of them in turn waited on each)
 __device__ __forceinline__ void ge_read_counts(const GeD& E, int& before, int& total, double& vmax) {
+  const int G = (int)gridDim.x;
+  if ((int)threadIdx.x < G) {
+    g_ge.rc[threadIdx.x] = __hip_atomic_load(&E.sy->cnt[threadIdx.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    g_ge.rv[threadIdx.x] = __longlong_as_double((long long)__hip_atomic_load(&E.sy->vmax2[threadIdx.x],
+                                                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  }
+  __syncthreads();
   before = 0; total = 0; vmax = 0.0;
-  for (int w = 0; w < (int)gridDim.x; w++) {
-    const int c = __hip_atomic_load(&E.sy->cnt[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int w = 0; w < G; w++) {
+    const int c = g_ge.rc[w];
     if (w < (int)blockIdx.x) before += c;
     total += c;
-    vmax = fmax(vmax, __longlong_as_double((long long)__hip_atomic_load(&E.sy->vmax2[w], __ATOMIC_RELAXED,
-                                                                      __HIP_MEMORY_SCOPE_AGENT)));
+    vmax = fmax(vmax, g_ge.rv[w]);
   }
+  __syncthreads();   // the words are read again by the next call
 }
 
 // ordered compaction of this workgroup's near entries with pred(q) into
